@@ -1,0 +1,2125 @@
+// GFN2-xTB FP64 CPU oracle — TEST INFRASTRUCTURE ONLY.
+//
+// A restatement of axtb's CPU path for GFN2-xTB energy + analytic gradient in axtb's default
+// conventions (crates/axtb/src/conventions.rs:244-268): setup, integrals, SCC loop with Broyden
+// mixing, Fermi smearing, energy partition and the full analytic gradient. It exists to check the
+// HIP engine (tests, smoke(), bench.py's cpu_baseline leg) and is never linked into the product.
+// Every function names the reference file:line it follows. Parameter tables come from
+// gfn2_tables.h (generated data, see gen_tables.py).
+//
+// Pinned by: axtb's committed energies and per-term energies (tests/reference_matrix/results.json
+// struct-water/-caffeine/-taxol, bench/results/final-9985wx/gfn2-eg-t1.json), the D4 fixtures
+// (crates/d4/tests/fixtures/*.txt: CN, EEQ charges, C6, polarisabilities, 2-/3-body energies),
+// central finite differences of the energy (gradient) and the invariants (sum g = 0,
+// sum q = charge, Tr(PS) = N_el, self-overlap = 1). The eigensolver (Householder + implicit QL)
+// is pinned by residual/orthonormality checks; faer's bits are not reproduced (parity unpinned at
+// the last-bit level, which the 1e-8 Eh bar does not need).
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../axtb_amd/csrc/gfn2_tables.h"
+
+namespace {
+
+using std::vector;
+typedef vector<double> vec;
+
+// crates/base/src/units.rs:18-24 (CODATA 2022)
+const double AUTOAA = 5.29177210544e-11 * 1e10;
+const double AATOAU = 1.0 / AUTOAA;
+const double KB_HARTREE = 3.1668115634564e-6;
+const double PI = 3.14159265358979323846;
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+// ------------------------------------------------------------------ basis (basis.rs:20-81)
+struct Cgto {
+  int l = 0, nprim = 0;
+  double alpha[6] = {0}, coeff[6] = {0};
+  double min_alpha() const {
+    double a = INFINITY;
+    for (int i = 0; i < nprim; i++) a = std::min(a, alpha[i]);
+    return a;
+  }
+};
+
+inline int nsph(int l) { return 2 * l + 1; }
+inline int ncart(int l) { return (l + 1) * (l + 2) / 2; }
+
+// crates/base/src/math.rs:108-111 (tblite association of (2/pi) alpha)
+double primitive_norm_tblite(int l, double alpha) {
+  static const double DFACT[5] = {1.0, 1.0, 3.0, 15.0, 105.0};
+  double top = 2.0 / PI;
+  return std::pow(top * alpha, 0.75) * std::pow(std::sqrt(4.0 * alpha), l) / std::sqrt(DFACT[l]);
+}
+
+// basis.rs:20-81 (STO-NG of Stewart, GFN2 tables)
+Cgto slater_to_gauss(int ng, int n, int l, double zeta) {
+  if (n > 5 || n <= l) throw Error(1, "STO-NG: unsupported n,l");
+  int ityp;
+  switch (l) {
+    case 0: ityp = n; break;
+    case 1: ityp = 4 + n; break;
+    case 2: ityp = 7 + n; break;
+    default: throw Error(1, "STO-NG: l > 2 not supported here");
+  }
+  ityp -= 1;
+  Cgto c;
+  c.l = l;
+  c.nprim = ng;
+  double z2 = zeta * zeta;
+  for (int i = 0; i < ng; i++) {
+    double a, k;
+    switch (ng) {
+      case 1: a = STO_P_ALPHA1[ityp]; k = 1.0; break;
+      case 2: a = STO_P_ALPHA2[ityp][i]; k = STO_P_COEFF2[ityp][i]; break;
+      case 3: a = STO_P_ALPHA3[ityp][i]; k = STO_P_COEFF3[ityp][i]; break;
+      case 4: a = STO_P_ALPHA4[ityp][i]; k = STO_P_COEFF4[ityp][i]; break;
+      case 5: a = STO_P_ALPHA5[ityp][i]; k = STO_P_COEFF5[ityp][i]; break;
+      case 6: a = STO_P_ALPHA6[ityp][i]; k = STO_P_COEFF6[ityp][i]; break;
+      default: throw Error(1, "STO-NG: bad primitive count");
+    }
+    c.alpha[i] = a * z2;
+    c.coeff[i] = k;
+  }
+  for (int i = 0; i < ng; i++) c.coeff[i] *= primitive_norm_tblite(l, c.alpha[i]);
+  return c;
+}
+
+// element tables of the axtb conventions (param/mod.rs:39-47, conventions.rs:244-268)
+double pauling_en(int z) { return EL_PAULING_EN[z - 1]; }
+double covalent_rad_d3_codata2022(int z) { return 4.0 / 3.0 * (EL_COVALENT_RAD_2009_AA[z - 1] * AATOAU); }
+double atomic_rad_codata2022(int z) { return EL_ATOMIC_RAD_AA[z - 1] * AATOAU; }
+// element_data.rs:94-102: mctc bohr, 4/3 of the 2009 radii (the EEQ guess's CN radii)
+double covalent_rad_d3_mctc(int z) { return 4.0 / 3.0 * (AXTB_MCTC_AATOAU * EL_COVALENT_RAD_2009_AA[z - 1]); }
+
+inline double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+// ------------------------------------------------------------------ molecule + model
+struct Model {
+  int nat = 0, nsh = 0, nao = 0;
+  vector<int> z;                 // atomic numbers
+  vec xyz;                       // bohr, [nat][3]
+  double charge = 0;
+  // basis maps (basis.rs:165-229)
+  vector<vector<Cgto>> shells;   // per atom
+  vector<int> ish_at, nsh_at, iao_at, nao_at, sh2at, iao_sh, nao_sh, ao2at, ao2sh;
+  vec min_alpha_sh, min_alpha_at;
+  double intcut = 0;
+  // H0 (gfn2.rs:61-99)
+  vec se0, kcn, shpoly, refocc;  // per shell
+  vec rad;                       // per atom
+  vector<int> ang;               // per shell
+  vec zeta_sh;                   // Slater exponent per shell
+};
+
+const double WEXP = 0.5, ENSCALE = 2.0e-2;
+const double KDIAG[5] = {1.85, 2.23, 2.23, 2.23, 2.23};
+const double SHELL_HUBBARD_DERIVS[5] = {1.0, 0.5, 0.25, 0.25, 0.25};
+
+// gfn2.rs:24-30
+double kshell(int k, int l) {
+  if ((k == 2 && (l == 0 || l == 1)) || (l == 2 && (k == 0 || k == 1))) return 2.0;
+  return (KDIAG[l] + KDIAG[k]) / 2.0;
+}
+
+// hscale of the shell pair (shell ish of atom i, jsh of atom j), gfn2.rs:76-98, used as
+// hscale[jsp][isp][jsh][ish] by hamiltonian.rs:88
+double hscale(const Model& m, int zi, int ish, int zj, int jsh) {
+  double den = std::pow(pauling_en(zi) - pauling_en(zj), 2);
+  int il = GFN2_ANG_SHELL[zi - 1][ish], jl = GFN2_ANG_SHELL[zj - 1][jsh];
+  double za = GFN2_SLATER_EXPONENT[zi - 1][ish], zb = GFN2_SLATER_EXPONENT[zj - 1][jsh];
+  double zij = std::pow(2.0 * std::sqrt(za * zb) / (za + zb), WEXP);
+  double enp = 1.0 + ENSCALE * den;
+  double km = kshell(jl, il) * enp;
+  return zij * km;
+}
+
+Model build_model(int nat, const int* z, const double* xyz, double charge, double acc) {
+  Model m;
+  m.nat = nat;
+  m.z.assign(z, z + nat);
+  m.xyz.assign(xyz, xyz + 3 * nat);
+  m.charge = charge;
+  for (int i = 0; i < nat; i++)
+    if (z[i] < 1 || z[i] > AXTB_GFN2_MAX_ELEM) throw Error(4, "unsupported element");
+  // molecule validation (base/src/molecule.rs:12,35-64): atoms closer than 1e-3 bohr... keep the
+  // distance check the reference does
+  for (int i = 0; i < nat; i++)
+    for (int j = 0; j < i; j++) {
+      double d2 = 0;
+      for (int k = 0; k < 3; k++) d2 += std::pow(xyz[3 * i + k] - xyz[3 * j + k], 2);
+      if (!(d2 >= 1e-6)) throw Error(1, "atoms too close");
+    }
+  int ish = 0, iao = 0;
+  for (int i = 0; i < nat; i++) {
+    int iz = z[i] - 1;
+    vector<Cgto> sh;
+    for (int s = 0; s < GFN2_NSHELL[iz]; s++) {
+      int l = GFN2_ANG_SHELL[iz][s];
+      if (l > 2) throw Error(4, "f shells unsupported");
+      sh.push_back(slater_to_gauss(GFN2_NUMBER_OF_PRIMITIVES[iz][s], GFN2_PRINCIPAL_QUANTUM_NUMBER[iz][s], l,
+                                   GFN2_SLATER_EXPONENT[iz][s]));
+    }
+    m.ish_at.push_back(ish);
+    m.nsh_at.push_back((int)sh.size());
+    m.iao_at.push_back(iao);
+    double amin = INFINITY;
+    for (size_t s = 0; s < sh.size(); s++) {
+      m.sh2at.push_back(i);
+      m.iao_sh.push_back(iao);
+      m.nao_sh.push_back(nsph(sh[s].l));
+      double a = sh[s].min_alpha();
+      m.min_alpha_sh.push_back(a);
+      amin = std::min(amin, a);
+      for (int k = 0; k < nsph(sh[s].l); k++) {
+        m.ao2at.push_back(i);
+        m.ao2sh.push_back(ish);
+      }
+      iao += nsph(sh[s].l);
+      ish++;
+      int l = sh[s].l;
+      m.ang.push_back(l);
+      m.se0.push_back(GFN2_P_SELFENERGY[iz][s]);
+      m.kcn.push_back(GFN2_P_KCN[iz][l]);
+      m.shpoly.push_back(GFN2_P_SHPOLY[iz][l]);
+      m.refocc.push_back(GFN2_REFERENCE_OCC[iz][l]);
+      m.zeta_sh.push_back(GFN2_SLATER_EXPONENT[iz][s]);
+    }
+    m.nao_at.push_back(iao - m.iao_at[i]);
+    m.min_alpha_at.push_back(amin);
+    m.shells.push_back(sh);
+    m.rad.push_back(atomic_rad_codata2022(z[i]));
+  }
+  m.nsh = ish;
+  m.nao = iao;
+  // basis.rs:135-137 (xtb's cutoff, axtb conventions: primitive_cutoff = integral_cutoff_xtb)
+  m.intcut = std::max(25.0 - 10.0 * std::log10(acc), 20.0);
+  return m;
+}
+
+// ------------------------------------------------------------------ coordination numbers
+// base/src/ncoord.rs:68-95: double exponential counting function of GFN2
+inline double exp_count(double k, double r, double r0) { return 1.0 / (1.0 + std::exp(-k * (r0 / r - 1.0))); }
+inline double exp_dcount(double k, double r, double r0) {
+  double e = std::exp(-k * (r0 / r - 1.0));
+  return (-k * r0 * e) / (r * r * (e + 1.0) * (e + 1.0));
+}
+inline double dexp_count(double r, double rc) { return exp_count(10.0, r, rc) * exp_count(20.0, r, rc + 2.0); }
+inline double dexp_dcount(double r, double rc) {
+  return exp_dcount(10.0, r, rc) * exp_count(20.0, r, rc + 2.0) + exp_count(10.0, r, rc) * exp_dcount(20.0, r, rc + 2.0);
+}
+// base/src/ncoord.rs:140-152 (dftd4 erf_count, k = 7.5)
+const double SQRTPI = 1.7724538509055159;
+inline double erf_count(double k, double r, double rc) { return 0.5 * (1.0 + std::erf(-((k * (r - rc)) / rc))); }
+inline double erf_dcount(double k, double r, double rc) {
+  double e = k * (r - rc) / rc;
+  return -(k * std::exp(-(e * e))) / (SQRTPI * rc);
+}
+
+// GFN2 CN: axtb/src/ncoord.rs:29-41 over base/src/ncoord.rs:471-484 (cutoff 40, r2 < 1e-12 skipped)
+vec gfn2_cn(const Model& m) {
+  vec cn(m.nat, 0.0);
+  double c2 = 40.0 * 40.0;
+  for (int i = 0; i < m.nat; i++)
+    for (int j = 0; j < i; j++) {
+      const double* a = &m.xyz[3 * i];
+      const double* b = &m.xyz[3 * j];
+      double v[3] = {a[0] - b[0], a[1] - b[1], a[2] - b[2]};
+      double r2 = dot3(v, v);
+      if (r2 > c2 || r2 < 1e-12) continue;
+      double rc = covalent_rad_d3_codata2022(m.z[i]) + covalent_rad_d3_codata2022(m.z[j]);
+      double c = dexp_count(std::sqrt(r2), rc);
+      cn[i] += c;
+      cn[j] += c;
+    }
+  return cn;
+}
+
+// base/src/ncoord.rs:486-519 (GradientOrder::Scaled)
+void gfn2_cn_gradient(const Model& m, const vec& dedcn, vec& g) {
+  double c2 = 40.0 * 40.0;
+  for (int i = 0; i < m.nat; i++)
+    for (int j = 0; j < i; j++) {
+      double s = dedcn[i] + dedcn[j];
+      const double* a = &m.xyz[3 * i];
+      const double* b = &m.xyz[3 * j];
+      double v[3] = {a[0] - b[0], a[1] - b[1], a[2] - b[2]};
+      double r2 = dot3(v, v);
+      if (r2 > c2 || r2 < 1e-12) continue;
+      double r = std::sqrt(r2);
+      double rc = covalent_rad_d3_codata2022(m.z[i]) + covalent_rad_d3_codata2022(m.z[j]);
+      double t = dexp_dcount(r, rc) * s / r;
+      for (int k = 0; k < 3; k++) {
+        g[3 * i + k] += t * v[k];
+        g[3 * j + k] -= t * v[k];
+      }
+    }
+}
+
+// ------------------------------------------------------------------ dense helpers
+// column-major n x n
+struct Mat {
+  int n = 0, m = 0;
+  vec a;
+  Mat() {}
+  Mat(int n_, int m_) : n(n_), m(m_), a((size_t)n_ * m_, 0.0) {}
+  double& operator()(int i, int j) { return a[(size_t)j * n + i]; }
+  double operator()(int i, int j) const { return a[(size_t)j * n + i]; }
+};
+
+// Gaussian elimination with partial pivoting on the last maximum (base/src/linalg/lu.rs:81-107)
+bool gauss_last_max(vector<vec>& a, vec& b) {
+  int n = (int)b.size();
+  for (int col = 0; col < n; col++) {
+    int piv = col;
+    for (int i = col; i < n; i++)
+      if (std::fabs(a[i][col]) >= std::fabs(a[piv][col])) piv = i;
+    if (a[piv][col] == 0.0) return false;
+    std::swap(a[col], a[piv]);
+    std::swap(b[col], b[piv]);
+    for (int row = col + 1; row < n; row++) {
+      double f = a[row][col] / a[col][col];
+      if (f != 0.0) {
+        for (int k = col; k < n; k++) a[row][k] -= f * a[col][k];
+        b[row] -= f * b[col];
+      }
+    }
+  }
+  for (int col = n - 1; col >= 0; col--) {
+    double s = b[col];
+    for (int k = col + 1; k < n; k++) s -= a[col][k] * b[k];
+    b[col] = s / a[col][col];
+  }
+  return true;
+}
+
+// dense LU with partial pivoting (first maximum) for the EEQ system (faer's Lu in base/eeq.rs)
+vec lu_solve(Mat A, vec b) {
+  int n = A.n;
+  vector<int> perm(n);
+  for (int k = 0; k < n; k++) {
+    int p = k;
+    for (int i = k + 1; i < n; i++)
+      if (std::fabs(A(i, k)) > std::fabs(A(p, k))) p = i;
+    if (A(p, k) == 0.0) throw Error(2, "singular EEQ system");
+    if (p != k) {
+      for (int j = 0; j < n; j++) std::swap(A(k, j), A(p, j));
+      std::swap(b[k], b[p]);
+    }
+    double piv = A(k, k);
+    for (int i = k + 1; i < n; i++) A(i, k) /= piv;
+    for (int j = k + 1; j < n; j++) {
+      double akj = A(k, j);
+      if (akj == 0.0) continue;
+      double* col = &A.a[(size_t)j * n];
+      const double* lk = &A.a[(size_t)k * n];
+      for (int i = k + 1; i < n; i++) col[i] -= lk[i] * akj;
+    }
+  }
+  for (int i = 0; i < n; i++) {
+    double s = b[i];
+    for (int k = 0; k < i; k++) s -= A(i, k) * b[k];
+    b[i] = s;
+  }
+  for (int i = n - 1; i >= 0; i--) {
+    double s = b[i];
+    for (int k = i + 1; k < n; k++) s -= A(i, k) * b[k];
+    b[i] = s / A(i, i);
+  }
+  return b;
+}
+
+// ------------------------------------------------------------------ EEQ guess (axtb/src/eeq.rs:43-77)
+inline double log_cap(double cn, double cnmax) {  // base/src/ncoord.rs:258-262
+  double a = std::log(1.0 + std::exp(cnmax));
+  return a - std::log(std::exp(cnmax - cn) + 1.0);
+}
+
+vec eeq_charges(int nat, const int* z, const double* xyz, double charge) {
+  for (int i = 0; i < nat; i++)
+    if (z[i] > 103) throw Error(4, "EEQ: unsupported element");
+  vec cn(nat, 0.0);
+  double c2 = 25.0 * 25.0;
+  for (int i = 0; i < nat; i++)
+    for (int j = 0; j < i; j++) {
+      double v[3] = {xyz[3 * i] - xyz[3 * j], xyz[3 * i + 1] - xyz[3 * j + 1], xyz[3 * i + 2] - xyz[3 * j + 2]};
+      double r2 = dot3(v, v);
+      if (r2 > c2 || r2 < 1e-12) continue;
+      double rc = covalent_rad_d3_mctc(z[i]) + covalent_rad_d3_mctc(z[j]);
+      double c = erf_count(7.5, std::sqrt(r2), rc);
+      cn[i] += c;
+      cn[j] += c;
+    }
+  for (int i = 0; i < nat; i++) cn[i] = log_cap(cn[i], 8.0);
+  double sqrt2pi = std::sqrt(2.0 / PI);
+  Mat A(nat + 1, nat + 1);
+  for (int i = 0; i < nat; i++) {
+    int zi = z[i] - 1;
+    A(i, i) = EEQ_ETA[zi] + sqrt2pi / EEQ_RAD[zi];
+    for (int j = 0; j < i; j++) {
+      int zj = z[j] - 1;
+      double v[3] = {xyz[3 * i] - xyz[3 * j], xyz[3 * i + 1] - xyz[3 * j + 1], xyz[3 * i + 2] - xyz[3 * j + 2]};
+      double r2 = dot3(v, v);
+      double gam = 1.0 / (EEQ_RAD[zi] * EEQ_RAD[zi] + EEQ_RAD[zj] * EEQ_RAD[zj]);
+      double val = std::erf(std::sqrt(r2 * gam)) / std::sqrt(r2);
+      A(i, j) = val;
+      A(j, i) = val;
+    }
+    A(i, nat) = 1.0;
+    A(nat, i) = 1.0;
+  }
+  vec x(nat + 1);
+  for (int i = 0; i < nat; i++) {
+    int zi = z[i] - 1;
+    x[i] = -EEQ_CHI[zi] + EEQ_KCNCHI[zi] / std::sqrt(cn[i] + 1e-14) * cn[i];
+  }
+  x[nat] = charge;
+  vec q = lu_solve(A, x);
+  q.resize(nat);
+  for (double v : q)
+    if (!std::isfinite(v)) throw Error(2, "EEQ charges are not finite");
+  return q;
+}
+
+// ------------------------------------------------------------------ one-electron integrals
+// integrals.rs:29-45: Cartesian exponents in CCA order
+void cart_exps(int l, int out[][3]) {
+  int c = 0;
+  for (int lx = l; lx >= 0; lx--)
+    for (int ly = l - lx; ly >= 0; ly--) {
+      out[c][0] = lx;
+      out[c][1] = ly;
+      out[c][2] = l - lx - ly;
+      c++;
+    }
+}
+
+// integrals.rs:49-103 (s, p, d rows): spherical rows from Cartesian columns
+void sph_trafo(int l, double T[5][6]) {
+  for (int i = 0; i < 5; i++)
+    for (int j = 0; j < 6; j++) T[i][j] = 0.0;
+  if (l == 0) {
+    T[0][0] = 1.0;
+  } else if (l == 1) {  // p: y, z, x from x, y, z
+    T[0][1] = 1.0;
+    T[1][2] = 1.0;
+    T[2][0] = 1.0;
+  } else {
+    double s3 = std::sqrt(3.0), s3_4 = s3 * 0.5;
+    T[0][1] = s3;
+    T[1][4] = s3;
+    T[2][0] = -0.5; T[2][3] = -0.5; T[2][5] = 1.0;
+    T[3][2] = s3;
+    T[4][0] = s3_4; T[4][3] = -s3_4;
+  }
+}
+
+const int MT = 6;  // 1D table size (l <= 2, +2 for moments, +1 for derivatives)
+// 1D moment table t[i][j] = <(x-A)^i (x-B)^j> / (Gaussian prefactor), Obara-Saika recursions
+// (integrals.rs:103-127): rpa = P - A, rpb = P - B, h = 1/(2(a+b))
+void table_1d(int la, int lb, double rpa, double rpb, double oab, double t[MT][MT]) {
+  double h = 0.5 * oab;
+  for (int i = 0; i < MT; i++)
+    for (int j = 0; j < MT; j++) t[i][j] = 0.0;
+  t[0][0] = 1.0;
+  for (int i = 0; i < la; i++) t[i + 1][0] = rpa * t[i][0] + (i > 0 ? i * h * t[i - 1][0] : 0.0);
+  for (int j = 0; j < lb; j++)
+    for (int i = 0; i <= la; i++) {
+      double v = rpb * t[i][j];
+      if (i > 0) v += i * h * t[i - 1][j];
+      if (j > 0) v += j * h * t[i][j - 1];
+      t[i][j + 1] = v;
+    }
+}
+
+inline bool pair_screened(double a, double b, double r2, double intcut) {  // integrals.rs:216-218
+  return a * b * r2 / (a + b) > intcut * (1.0 + 1e-10);
+}
+
+// traceless form Q = 3/2 M - 1/2 tr(M), packed xx xy yy xz yz zz (integrals.rs:233-241)
+inline void traceless(double* q) {
+  double tr = 0.5 * (q[0] + q[2] + q[5]);
+  q[0] = 1.5 * q[0] - tr;
+  q[1] *= 1.5;
+  q[2] = 1.5 * q[2] - tr;
+  q[3] *= 1.5;
+  q[4] *= 1.5;
+  q[5] = 1.5 * q[5] - tr;
+}
+
+// moment exponents of s, x, y, z, xx, xy, yy, xz, yz, zz
+const int MOM[10][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {2, 0, 0},
+                        {1, 1, 0}, {0, 2, 0}, {1, 0, 1}, {0, 1, 1}, {0, 0, 2}};
+
+// Cartesian -> spherical of NV interleaved components: out[(jm + nsj*im)*NV + x]
+void to_sph(int lj, int li, int NV, const double* cart, double* out) {
+  double Tj[5][6], Ti[5][6];
+  sph_trafo(lj, Tj);
+  sph_trafo(li, Ti);
+  int ncj = ncart(lj), nci = ncart(li), nsj = nsph(lj), nsi = nsph(li);
+  for (int im = 0; im < nsi; im++)
+    for (int jm = 0; jm < nsj; jm++)
+      for (int x = 0; x < NV; x++) {
+        double acc = 0.0;
+        for (int ic = 0; ic < nci; ic++) {
+          if (Ti[im][ic] == 0.0) continue;
+          for (int jc = 0; jc < ncj; jc++) {
+            if (Tj[jm][jc] == 0.0) continue;
+            acc += Ti[im][ic] * Tj[jm][jc] * cart[(jc + ncj * ic) * NV + x];
+          }
+        }
+        out[(jm + nsj * im) * NV + x] = acc;
+      }
+}
+
+// Overlap, dipole and traceless quadrupole of a shell pair, moments around the ket centre R_i
+// (integrals.rs:277-367). Bra cj on R_j, ket ci on R_i, vec = R_i - R_j. Blocks [jm + nsj*im].
+void multipole(const Cgto& cj, const Cgto& ci, const double* vec, double intcut, double* s, double (*d)[3],
+               double (*q)[6]) {
+  double r2 = dot3(vec, vec);
+  int li = ci.l, lj = cj.l;
+  int ei[6][3], ej[6][3];
+  cart_exps(li, ei);
+  cart_exps(lj, ej);
+  int ncj = ncart(lj), nci = ncart(li);
+  double cart[36 * 10];
+  std::fill(cart, cart + 360, 0.0);
+  double t[3][MT][MT];
+  for (int ip = 0; ip < ci.nprim; ip++)
+    for (int jp = 0; jp < cj.nprim; jp++) {
+      double ai = ci.alpha[ip], aj = cj.alpha[jp];
+      double eab = ai + aj, oab = 1.0 / eab, est = ai * aj * r2 * oab;
+      if (est > intcut) continue;
+      double pre = std::exp(-est) * std::pow(std::sqrt(PI), 3) * std::pow(std::sqrt(oab), 3);
+      for (int k = 0; k < 3; k++) table_1d(li + 2, lj, -vec[k] * aj * oab, vec[k] * ai * oab, oab, t[k]);
+      double cc = ci.coeff[ip] * cj.coeff[jp] * pre;
+      for (int ic = 0; ic < nci; ic++)
+        for (int jc = 0; jc < ncj; jc++) {
+          double* c = &cart[(jc + ncj * ic) * 10];
+          for (int n = 0; n < 10; n++)
+            c[n] += cc * t[0][ei[ic][0] + MOM[n][0]][ej[jc][0]] * t[1][ei[ic][1] + MOM[n][1]][ej[jc][1]] *
+                    t[2][ei[ic][2] + MOM[n][2]][ej[jc][2]];
+        }
+    }
+  double sph[25 * 10];
+  to_sph(lj, li, 10, cart, sph);
+  for (int k = 0; k < nsph(lj) * nsph(li); k++) {
+    const double* c = &sph[k * 10];
+    s[k] = c[0];
+    for (int x = 0; x < 3; x++) d[k][x] = c[1 + x];
+    for (int x = 0; x < 6; x++) q[k][x] = c[4 + x];
+    traceless(q[k]);
+  }
+}
+
+// Derivatives with respect to the ket centre R_i (integrals.rs:400-521)
+struct MpGrad {
+  double s[25], ds[25][3], ddi[25][3][3], ddj[25][3][3], dqi[25][3][6], dqj[25][3][6];
+};
+
+void multipole_grad(const Cgto& cj, const Cgto& ci, const double* vec, double intcut, MpGrad& out) {
+  double r2 = dot3(vec, vec);
+  int li = ci.l, lj = cj.l;
+  int ei[6][3], ej[6][3];
+  cart_exps(li, ei);
+  cart_exps(lj, ej);
+  int ncj = ncart(lj), nci = ncart(li);
+  // per Cartesian pair: values [10] (moments around R_j) and derivatives [10][3]
+  double val[36 * 10], der[36 * 30];
+  std::fill(val, val + 360, 0.0);
+  std::fill(der, der + 1080, 0.0);
+  double t[3][MT][MT];
+  for (int ip = 0; ip < ci.nprim; ip++)
+    for (int jp = 0; jp < cj.nprim; jp++) {
+      double ai = ci.alpha[ip], aj = cj.alpha[jp];
+      double eab = ai + aj, oab = 1.0 / eab, est = ai * aj * r2 * oab;
+      if (est > intcut) continue;
+      double pre = std::exp(-est) * std::pow(std::sqrt(PI), 3) * std::pow(std::sqrt(oab), 3);
+      for (int k = 0; k < 3; k++) table_1d(li + 1, lj + 2, -vec[k] * aj * oab, vec[k] * ai * oab, oab, t[k]);
+      double cc = ci.coeff[ip] * cj.coeff[jp] * pre;
+      double ai2 = 2.0 * ai;
+      for (int ic = 0; ic < nci; ic++)
+        for (int jc = 0; jc < ncj; jc++) {
+          double v[3][3], g[3][3];
+          for (int k = 0; k < 3; k++) {
+            int a = ei[ic][k], b = ej[jc][k];
+            for (int mm = 0; mm < 3; mm++) {
+              v[k][mm] = t[k][a][b + mm];
+              g[k][mm] = ai2 * t[k][a + 1][b + mm] - (a > 0 ? a * t[k][a - 1][b + mm] : 0.0);
+            }
+          }
+          double* vc = &val[(jc + ncj * ic) * 10];
+          double* dc = &der[(jc + ncj * ic) * 30];
+          for (int n = 0; n < 10; n++) {
+            double x = v[0][MOM[n][0]], y = v[1][MOM[n][1]], zz = v[2][MOM[n][2]];
+            vc[n] += cc * x * y * zz;
+            dc[3 * n] += cc * g[0][MOM[n][0]] * y * zz;
+            dc[3 * n + 1] += cc * x * g[1][MOM[n][1]] * zz;
+            dc[3 * n + 2] += cc * x * y * g[2][MOM[n][2]];
+          }
+        }
+    }
+  double sv[25 * 10], sd[25 * 30];
+  to_sph(lj, li, 10, val, sv);
+  to_sph(lj, li, 30, der, sd);
+  static const int AB[6][2] = {{0, 0}, {0, 1}, {1, 1}, {0, 2}, {1, 2}, {2, 2}};
+  for (int blk = 0; blk < nsph(lj) * nsph(li); blk++) {
+    const double* v = &sv[blk * 10];
+    const double* dv = &sd[blk * 30];
+    double s = v[0];
+    double dj[3] = {v[1], v[2], v[3]};
+    double ds[3] = {dv[0], dv[1], dv[2]};
+    out.s[blk] = s;
+    for (int k = 0; k < 3; k++) out.ds[blk][k] = ds[k];
+    double ddj[3][3], dmj[3][6], ddi[3][3], dmi[3][6];
+    for (int k = 0; k < 3; k++) {
+      for (int c = 0; c < 3; c++) ddj[k][c] = dv[3 * (1 + c) + k];
+      for (int c = 0; c < 6; c++) dmj[k][c] = dv[3 * (4 + c) + k];
+    }
+    // shift to R_i: D_i = D_j - vec S; M_i = M_j - v_a D_j,b - v_b D_j,a + v_a v_b S
+    for (int k = 0; k < 3; k++) {
+      for (int c = 0; c < 3; c++) ddi[k][c] = ddj[k][c] - vec[c] * ds[k];
+      ddi[k][k] -= s;
+      for (int n = 0; n < 6; n++) {
+        int a = AB[n][0], b = AB[n][1];
+        double x = dmj[k][n] - vec[a] * ddj[k][b] - vec[b] * ddj[k][a] + vec[a] * vec[b] * ds[k];
+        if (a == k) x += -dj[b] + vec[b] * s;
+        if (b == k) x += -dj[a] + vec[a] * s;
+        dmi[k][n] = x;
+      }
+    }
+    for (int k = 0; k < 3; k++) {
+      traceless(dmi[k]);
+      traceless(dmj[k]);
+    }
+    std::memcpy(out.ddi[blk], ddi, sizeof ddi);
+    std::memcpy(out.ddj[blk], ddj, sizeof ddj);
+    std::memcpy(out.dqi[blk], dmi, sizeof dmi);
+    std::memcpy(out.dqj[blk], dmj, sizeof dmj);
+  }
+}
+
+// integrals.rs:991-1006: moments around the ket centre -> around the bra centre
+void shift_to_bra(const double* vec, double s, const double* di, const double* qi, double* dj, double* qj) {
+  for (int k = 0; k < 3; k++) dj[k] = di[k] + vec[k] * s;
+  qj[0] = 2.0 * vec[0] * di[0] + vec[0] * vec[0] * s;
+  qj[1] = vec[0] * di[1] + vec[1] * di[0] + vec[0] * vec[1] * s;
+  qj[2] = 2.0 * vec[1] * di[1] + vec[1] * vec[1] * s;
+  qj[3] = vec[0] * di[2] + vec[2] * di[0] + vec[0] * vec[2] * s;
+  qj[4] = vec[1] * di[2] + vec[2] * di[1] + vec[1] * vec[2] * s;
+  qj[5] = 2.0 * vec[2] * di[2] + vec[2] * vec[2] * s;
+  traceless(qj);
+  for (int k = 0; k < 6; k++) qj[k] += qi[k];
+}
+
+// ------------------------------------------------------------------ H0, S, D, Q (ao.rs:195-291, hamiltonian.rs:77-94)
+struct Ints {
+  Mat S, H0, D[3], Q[6];
+};
+
+Ints build_integrals(const Model& m, const vec& se) {
+  int n = m.nao;
+  Ints I;
+  I.S = Mat(n, n);
+  I.H0 = Mat(n, n);
+  for (int k = 0; k < 3; k++) I.D[k] = Mat(n, n);
+  for (int k = 0; k < 6; k++) I.Q[k] = Mat(n, n);
+  double s[25], d[25][3], q[25][6];
+  for (int iat = 0; iat < m.nat; iat++) {
+    for (int jat = 0; jat <= iat; jat++) {
+      double vec[3];
+      for (int k = 0; k < 3; k++) vec[k] = m.xyz[3 * iat + k] - m.xyz[3 * jat + k];
+      double r2 = dot3(vec, vec);
+      bool onsite = iat == jat;
+      if (!onsite && pair_screened(m.min_alpha_at[iat], m.min_alpha_at[jat], r2, m.intcut)) continue;
+      double rr = onsite ? 0.0 : std::sqrt(std::sqrt(r2) / (m.rad[jat] + m.rad[iat]));
+      for (int ish = 0; ish < m.nsh_at[iat]; ish++) {
+        int si = m.ish_at[iat] + ish;
+        const Cgto& ci = m.shells[iat][ish];
+        for (int jsh = 0; jsh < m.nsh_at[jat]; jsh++) {
+          int sj = m.ish_at[jat] + jsh;
+          const Cgto& cj = m.shells[jat][jsh];
+          if (pair_screened(m.min_alpha_sh[si], m.min_alpha_sh[sj], r2, m.intcut)) continue;
+          multipole(cj, ci, vec, m.intcut, s, d, q);
+          double hij;
+          if (onsite) {
+            hij = 0.5 * (se[si] + se[sj]);
+          } else {
+            double sp = (1.0 + m.shpoly[si] * rr) * (1.0 + m.shpoly[sj] * rr);
+            hij = 0.5 * (se[si] + se[sj]) * hscale(m, m.z[jat], jsh, m.z[iat], ish) * sp;
+          }
+          int nj = nsph(cj.l);
+          for (int iao = 0; iao < nsph(ci.l); iao++)
+            for (int jao = 0; jao < nj; jao++) {
+              int ij = jao + nj * iao;
+              int mu = m.iao_sh[sj] + jao, nu = m.iao_sh[si] + iao;
+              I.S(mu, nu) = s[ij];
+              I.H0(mu, nu) = s[ij] * hij;
+              for (int k = 0; k < 3; k++) I.D[k](mu, nu) = d[ij][k];
+              for (int k = 0; k < 6; k++) I.Q[k](mu, nu) = q[ij][k];
+              if (!onsite) {
+                double dj[3], qj[6];
+                shift_to_bra(vec, s[ij], d[ij], q[ij], dj, qj);
+                I.S(nu, mu) = s[ij];
+                I.H0(nu, mu) = s[ij] * hij;
+                for (int k = 0; k < 3; k++) I.D[k](nu, mu) = dj[k];
+                for (int k = 0; k < 6; k++) I.Q[k](nu, mu) = qj[k];
+              }
+            }
+        }
+      }
+    }
+  }
+  return I;
+}
+
+// ------------------------------------------------------------------ electrostatics (coulomb.rs)
+struct Coulomb {
+  vec hardness;     // per shell
+  vec gamma3;       // per shell (ES3)
+  vec gam;          // nsh x nsh row-major
+  // AES (coulomb.rs:352-397)
+  vec dkernel, qkernel, mrad, dmrdcn;  // per atom
+};
+
+const double QSCALE[6] = {1.0, 2.0, 1.0, 2.0, 2.0, 1.0};
+
+Coulomb build_coulomb(const Model& m, const vec& cn) {
+  Coulomb c;
+  int nsh = m.nsh;
+  c.hardness.resize(nsh);
+  c.gamma3.resize(nsh);
+  for (int s = 0; s < nsh; s++) {
+    int iz = m.z[m.sh2at[s]] - 1;
+    c.hardness[s] = GFN2_HUBBARD_PARAMETER[iz] * GFN2_SHELL_HUBBARD[iz][m.ang[s]];
+    c.gamma3[s] = GFN2_P_HUBBARD_DERIVS[iz] * SHELL_HUBBARD_DERIVS[m.ang[s]];
+  }
+  // coulomb.rs:61-119 (gexp = 2)
+  c.gam.assign((size_t)nsh * nsh, 0.0);
+  for (int iat = 0; iat < m.nat; iat++)
+    for (int jat = 0; jat <= iat; jat++) {
+      double v[3];
+      for (int k = 0; k < 3; k++) v[k] = m.xyz[3 * jat + k] - m.xyz[3 * iat + k];
+      double r1g = std::pow(std::sqrt(dot3(v, v)), 2.0);
+      for (int ish = 0; ish < m.nsh_at[iat]; ish++) {
+        int jmax = iat == jat ? ish + 1 : m.nsh_at[jat];
+        for (int jsh = 0; jsh < jmax; jsh++) {
+          int a = m.ish_at[iat] + ish, b = m.ish_at[jat] + jsh;
+          double eta = 0.5 * (c.hardness[a] + c.hardness[b]);
+          double val = iat == jat ? eta : 1.0 / std::pow(r1g + std::pow(eta, -2.0), 0.5);
+          c.gam[(size_t)a * nsh + b] = val;
+          c.gam[(size_t)b * nsh + a] = val;
+        }
+      }
+    }
+  int nat = m.nat;
+  c.dkernel.resize(nat);
+  c.qkernel.resize(nat);
+  c.mrad.resize(nat);
+  c.dmrdcn.resize(nat);
+  for (int i = 0; i < nat; i++) {
+    int iz = m.z[i] - 1;
+    c.dkernel[i] = GFN2_P_DKERNEL[iz];
+    c.qkernel[i] = GFN2_P_QKERNEL[iz];
+    double arg = cn[i] - GFN2_P_VCN[iz] - 1.2;
+    double t1 = std::exp(-4.0 * arg);
+    double t2 = (5.0 - GFN2_P_RAD[iz]) / (1.0 + t1);
+    c.mrad[i] = GFN2_P_RAD[iz] + t2;
+    c.dmrdcn[i] = -t2 * 4.0 * t1 / (1.0 + t1);
+  }
+  return c;
+}
+
+// g3, g5 of the AES pair (coulomb.rs:383-394)
+inline void aes_g(const Model& m, const Coulomb& c, int i, int j, double& g3, double& g5) {
+  double v[3];
+  for (int k = 0; k < 3; k++) v[k] = m.xyz[3 * i + k] - m.xyz[3 * j + k];
+  double r1 = std::sqrt(dot3(v, v));
+  double g1 = 1.0 / r1;
+  double g3v = g1 * g1 * g1, g5v = g3v * g1 * g1;
+  double rr = 0.5 * (c.mrad[j] + c.mrad[i]) * g1;
+  double f3 = 1.0 / (1.0 + 6.0 * std::pow(rr, 3.0));
+  double f5 = 1.0 / (1.0 + 6.0 * std::pow(rr, 4.0));
+  g3 = g3v * f3;
+  g5 = g5v * f5;
+}
+
+struct Moments {
+  vec qsh, qat, dp, qp;  // dp [nat][3], qp [nat][6]
+};
+struct Pots {
+  vec vsh, vat, vdp, vqp;
+};
+
+// coulomb.rs:407-502 AES potentials (added to vat, vdp, vqp)
+void aes_potentials(const Model& m, const Coulomb& c, const Moments& s, Pots& p) {
+  int nat = m.nat;
+  for (int a = 0; a < nat; a++) {
+    for (int b = 0; b < nat; b++) {
+      if (a == b) continue;
+      double g3, g5;
+      aes_g(m, c, std::max(a, b), std::min(a, b), g3, g5);
+      double w[3];
+      for (int k = 0; k < 3; k++) w[k] = m.xyz[3 * a + k] - m.xyz[3 * b + k];
+      double r2 = dot3(w, w);
+      double tc[6] = {w[0] * w[0], w[0] * w[1], w[1] * w[1], w[0] * w[2], w[1] * w[2], w[2] * w[2]};
+      double t[6];
+      for (int k = 0; k < 6; k++) t[k] = QSCALE[k] * tc[k] * g5;
+      const double* mb = &s.dp[3 * b];
+      double mbw = dot3(mb, w);
+      p.vat[a] += mbw * g3;
+      for (int k = 0; k < 3; k++) p.vdp[3 * a + k] -= w[k] * g3 * s.qat[b];
+      for (int k = 0; k < 3; k++) p.vdp[3 * a + k] += g5 * (r2 * mb[k] - 3.0 * w[k] * mbw);
+      double thb = 0.0;
+      for (int k = 0; k < 6; k++) {
+        p.vqp[6 * a + k] += t[k] * s.qat[b];
+        thb += t[k] * s.qp[6 * b + k];
+      }
+      p.vat[a] += thb;
+    }
+  }
+  for (int i = 0; i < nat; i++) {
+    for (int k = 0; k < 3; k++) p.vdp[3 * i + k] += 2.0 * c.dkernel[i] * s.dp[3 * i + k];
+    for (int k = 0; k < 6; k++) p.vqp[6 * i + k] += 2.0 * c.qkernel[i] * s.qp[6 * i + k] * QSCALE[k];
+  }
+}
+
+// coulomb.rs:668-731 AES atom energies
+void aes_energies(const Model& m, const Coulomb& c, const Moments& s, vec& e) {
+  int nat = m.nat;
+  for (int i = 1; i < nat; i++)
+    for (int j = 0; j < i; j++) {
+      double g3, g5;
+      aes_g(m, c, i, j, g3, g5);
+      double v[3];
+      for (int k = 0; k < 3; k++) v[k] = m.xyz[3 * i + k] - m.xyz[3 * j + k];
+      double r2 = dot3(v, v);
+      const double *di = &s.dp[3 * i], *dj = &s.dp[3 * j];
+      double miv = dot3(di, v), mjv = dot3(dj, v);
+      double en = (s.qat[i] * mjv - s.qat[j] * miv) * g3;
+      en += g5 * (r2 * dot3(di, dj) - 3.0 * miv * mjv);
+      double tc[6] = {v[0] * v[0], v[0] * v[1], v[1] * v[1], v[0] * v[2], v[1] * v[2], v[2] * v[2]};
+      for (int k = 0; k < 6; k++) en += QSCALE[k] * tc[k] * g5 * (s.qat[j] * s.qp[6 * i + k] + s.qat[i] * s.qp[6 * j + k]);
+      e[i] += 0.5 * en;
+      e[j] += 0.5 * en;
+    }
+  for (int i = 0; i < nat; i++) {
+    e[i] += c.dkernel[i] * dot3(&s.dp[3 * i], &s.dp[3 * i]);
+    double qq = 0.0;
+    for (int k = 0; k < 6; k++) qq += QSCALE[k] * s.qp[6 * i + k] * s.qp[6 * i + k];
+    e[i] += c.qkernel[i] * qq;
+  }
+}
+
+// coulomb.rs:896-1001 AES gradient; adds dE/dCN to dedcn
+void aes_gradient(const Model& m, const Coulomb& c, const Moments& s, vec& g, vec& dedcn) {
+  int nat = m.nat;
+  vec dedr(nat, 0.0);
+  const double kdmp3 = 3.0, kdmp5 = 4.0;
+  for (int iat = 1; iat < nat; iat++)
+    for (int jat = 0; jat < iat; jat++) {
+      double rr = 0.5 * (c.mrad[iat] + c.mrad[jat]);
+      double vec[3];
+      for (int k = 0; k < 3; k++) vec[k] = m.xyz[3 * jat + k] - m.xyz[3 * iat + k];
+      double r2 = dot3(vec, vec), r1 = std::sqrt(r2);
+      double g1 = 1.0 / r1, g3 = g1 * g1 * g1, g5 = g3 * g1 * g1, g7 = g5 * g1 * g1;
+      double fdmp3 = 1.0 / (1.0 + 6.0 * std::pow(rr * g1, kdmp3));
+      double ddmp3 = -3.0 * g5 * fdmp3 - kdmp3 * fdmp3 * (fdmp3 - 1.0) * g5;
+      double fdmp5 = 1.0 / (1.0 + 6.0 * std::pow(rr * g1, kdmp5));
+      double ddmp5 = -5.0 * fdmp5 - kdmp5 * (fdmp5 * fdmp5 - fdmp5);
+      double qi = s.qat[iat], qj = s.qat[jat];
+      const double *di = &s.dp[3 * iat], *dj = &s.dp[3 * jat];
+      double dpiqj = dot3(vec, di) * qj, qidpj = dot3(vec, dj) * qi;
+      double fddr = 3.0 * (dpiqj - qidpj) * kdmp3 * fdmp3 * g3 * (fdmp3 / rr) * std::pow(rr * g1, kdmp3);
+      double dg[3];
+      for (int k = 0; k < 3; k++) dg[k] = -ddmp3 * vec[k] * (dpiqj - qidpj) + fdmp3 * g3 * (qi * dj[k] - qj * di[k]);
+      double dpidpj = dot3(dj, di), dpiv = dot3(di, vec), dpjv = dot3(dj, vec);
+      double edd = dpidpj * r2 - 3.0 * dpjv * dpiv;
+      fddr += 3.0 * edd * kdmp5 * fdmp5 * g5 * (fdmp5 / rr) * std::pow(rr * g1, kdmp5);
+      for (int k = 0; k < 3; k++)
+        dg[k] += -2.0 * fdmp5 * g5 * dpidpj * vec[k] + 3.0 * fdmp5 * g5 * (dpiv * dj[k] + dpjv * di[k]) -
+                 edd * ddmp5 * g7 * vec[k];
+      const double *ti = &s.qp[6 * iat], *tj = &s.qp[6 * jat];
+      double eq = 2.0 * (qj * ti[1] + tj[1] * qi) * vec[0] * vec[1] + 2.0 * (qj * ti[3] + tj[3] * qi) * vec[0] * vec[2] +
+                  2.0 * (qj * ti[4] + tj[4] * qi) * vec[1] * vec[2] + (qj * ti[0] + tj[0] * qi) * vec[0] * vec[0] +
+                  (qj * ti[2] + tj[2] * qi) * vec[1] * vec[1] + (qj * ti[5] + tj[5] * qi) * vec[2] * vec[2];
+      fddr += eq * 3.0 * kdmp5 * fdmp5 * g5 * fdmp5 / rr * std::pow(rr * g1, kdmp5);
+      double tvj[3] = {vec[0] * tj[0] + vec[1] * tj[1] + vec[2] * tj[3], vec[0] * tj[1] + vec[1] * tj[2] + vec[2] * tj[4],
+                       vec[0] * tj[3] + vec[1] * tj[4] + vec[2] * tj[5]};
+      double tvi[3] = {vec[0] * ti[0] + vec[1] * ti[1] + vec[2] * ti[3], vec[0] * ti[1] + vec[1] * ti[2] + vec[2] * ti[4],
+                       vec[0] * ti[3] + vec[1] * ti[4] + vec[2] * ti[5]};
+      for (int k = 0; k < 3; k++)
+        dg[k] += -eq * ddmp5 * g7 * vec[k] - 2.0 * fdmp5 * g5 * qi * tvj[k] - 2.0 * fdmp5 * g5 * qj * tvi[k];
+      dedr[iat] += fddr;
+      dedr[jat] += fddr;
+      for (int k = 0; k < 3; k++) {
+        g[3 * iat + k] += dg[k];
+        g[3 * jat + k] -= dg[k];
+      }
+    }
+  for (int i = 0; i < nat; i++) dedcn[i] += dedr[i] * c.dmrdcn[i];
+}
+
+// coulomb.rs:123-169 ES2 gradient
+void es2_gradient(const Model& m, const Coulomb& c, const vec& qsh, vec& g) {
+  for (int iat = 1; iat < m.nat; iat++)
+    for (int jat = 0; jat < iat; jat++) {
+      double v[3];
+      for (int k = 0; k < 3; k++) v[k] = m.xyz[3 * iat + k] - m.xyz[3 * jat + k];
+      double r1 = std::sqrt(dot3(v, v));
+      double dg = 0.0;
+      for (int ish = 0; ish < m.nsh_at[iat]; ish++)
+        for (int jsh = 0; jsh < m.nsh_at[jat]; jsh++) {
+          int a = m.ish_at[iat] + ish, b = m.ish_at[jat] + jsh;
+          double eta = 0.5 * (c.hardness[a] + c.hardness[b]);
+          double dtmp = 1.0 / (std::pow(r1, 2.0) + std::pow(eta, -2.0));
+          dtmp = -std::pow(r1, 0.0) * dtmp * std::pow(dtmp, 0.5);
+          dg += dtmp * qsh[a] * qsh[b];
+        }
+      for (int k = 0; k < 3; k++) {
+        g[3 * iat + k] += dg * v[k];
+        g[3 * jat + k] -= dg * v[k];
+      }
+    }
+}
+
+// ------------------------------------------------------------------ repulsion (repulsion.rs:44-73)
+double repulsion(const Model& m, vec& e, vec* g) {
+  double c2 = 40.0 * 40.0;
+  for (int i = 0; i < m.nat; i++)
+    for (int j = 0; j < i; j++) {
+      double v[3];
+      for (int k = 0; k < 3; k++) v[k] = m.xyz[3 * i + k] - m.xyz[3 * j + k];
+      double r2 = dot3(v, v);
+      if (r2 > c2 || r2 < 1e-12) continue;
+      int zi = m.z[i] - 1, zj = m.z[j] - 1;
+      double r1 = std::sqrt(r2);
+      double kexp = (m.z[i] > 2 || m.z[j] > 2) ? 1.5 : 1.0;
+      double alpha = std::sqrt(GFN2_REP_ALPHA[zj] * GFN2_REP_ALPHA[zi]);
+      double zeff = GFN2_REP_ZEFF[zj] * GFN2_REP_ZEFF[zi];
+      double r1k = std::pow(r1, kexp);
+      double exa = std::exp(-alpha * r1k);
+      double de = zeff * exa / std::pow(r1, 1.0);
+      e[i] += 0.5 * de;
+      e[j] += 0.5 * de;
+      if (g) {
+        double f = -(alpha * r1k * kexp + 1.0) * de / r2;
+        for (int k = 0; k < 3; k++) {
+          (*g)[3 * i + k] += f * v[k];
+          (*g)[3 * j + k] -= f * v[k];
+        }
+      }
+    }
+  double t = 0;
+  for (double x : e) t += x;
+  return t;
+}
+
+// ------------------------------------------------------------------ D4 (crates/d4/src)
+const int MAXREF = AXTB_D4_MAX_REF, NFREQ = AXTB_D4_NFREQ;
+const double FREQ[NFREQ] = {0.000001, 0.050000, 0.100000, 0.200000, 0.300000, 0.400000, 0.500000, 0.600000,
+                            0.700000, 0.800000, 0.900000, 1.000000, 1.200000, 1.400000, 1.600000, 1.800000,
+                            2.000000, 2.500000, 3.000000, 4.000000, 5.000000, 7.500000, 10.00000};
+
+struct D4Options {
+  int charge_model = 1;   // 0 EEQ reference charges, 1 GFN2
+  int rcov_mode = 1;      // 0 mctc 4/3 radii + D4 EN table, 1 axtb codata-2022 radii + Pauling EN
+  double cn_cut = 40.0, disp2 = 72.0, disp3 = 40.0, width2 = 1.0, width3 = 1.0;
+};
+
+// D4 rational damping of GFN2 (d4/src/disp.rs:20-22)
+const double S6 = 1.0, S8 = 2.7, S9 = 5.0, A1 = 0.52, A2 = 5.0, ALP = 16.0;
+const double GA = 3.0, GC = 2.0, WF = 6.0;
+
+struct D4 {
+  int nat = 0;
+  vector<int> z;
+  vec xyz;
+  D4Options o;
+  vector<int> nref;
+  vector<vector<int>> ngw;
+  vector<vec> refcn, refq;           // per atom [MAXREF]
+  vector<vector<vec>> aiw;           // per atom [ref][freq]
+  vec rcov, en, zeff, eta, r4r2;     // per atom
+  vec cn;
+};
+
+// d4/src/model.rs:83-90 zeta function
+inline double zeta_f(double a, double c, double qref, double qmod) {
+  if (qmod < 0.0) return std::exp(a);
+  return std::exp(a * (1.0 - std::exp(c * (1.0 - qref / qmod))));
+}
+inline void zeta_df(double a, double c, double qref, double qmod, double& z, double& dz) {
+  if (qmod < 0.0) {
+    z = std::exp(a);
+    dz = 0.0;
+    return;
+  }
+  double e = std::exp(c * (1.0 - qref / qmod));
+  z = std::exp(a * (1.0 - e));
+  dz = -((a * c * e * z * qref) / (qmod * qmod));
+}
+
+double trapzd(const double* p) {  // d4/src/model.rs:120-141
+  double w[NFREQ];
+  w[0] = 0.5 * (FREQ[1] - FREQ[0]);
+  for (int k = 1; k < NFREQ - 1; k++) w[k] = 0.5 * ((FREQ[k] - FREQ[k - 1]) + (FREQ[k + 1] - FREQ[k]));
+  w[NFREQ - 1] = 0.5 * (FREQ[NFREQ - 1] - FREQ[NFREQ - 2]);
+  double s = 0.0;
+  for (int k = 0; k < NFREQ; k++) s += p[k] * w[k];
+  return s;
+}
+
+inline double tabz(const double* t, int z) { return (z == 0 || z > 118) ? 0.0 : t[z - 1]; }
+
+D4 build_d4(int nat, const int* z, const double* xyz, const D4Options& o) {
+  D4 d;
+  d.nat = nat;
+  d.z.assign(z, z + nat);
+  d.xyz.assign(xyz, xyz + 3 * nat);
+  d.o = o;
+  for (int i = 0; i < nat; i++) {
+    int Z = z[i];
+    // d4/src/model.rs:60-71: covalent radius (mctc bohr) and r4r2
+    double rc = o.rcov_mode ? covalent_rad_d3_codata2022(Z) : 4.0 / 3.0 * (AXTB_MCTC_AATOAU * EL_COVALENT_RAD_2009_AA[Z - 1]);
+    d.rcov.push_back(rc);
+    d.en.push_back(EL_PAULING_EN[Z - 1]);
+    d.zeff.push_back(tabz(D4_EFFECTIVE_NUCLEAR_CHARGE, Z));
+    d.eta.push_back(tabz(D4_CHEMICAL_HARDNESS, Z));
+    d.r4r2.push_back(std::sqrt(0.5 * (D4_R4_OVER_R2[Z - 1] * std::sqrt((double)Z))));
+    int nr = D4_REFN[Z - 1];
+    d.nref.push_back(nr);
+    // reference_ngw (model.rs:143-155)
+    int cnc[20] = {0};
+    cnc[0] = 1;
+    auto icn = [](double x) {
+      long v = std::lround(x);
+      if (v < 0) v = 0;
+      if (v > 19) v = 19;
+      return (int)v;
+    };
+    for (int r = 0; r < nr; r++) cnc[icn(D4_REFCN[Z - 1][r])]++;
+    vector<int> ngw(MAXREF, 1);
+    for (int r = 0; r < nr; r++) {
+      int c = cnc[icn(D4_REFCN[Z - 1][r])];
+      ngw[r] = c * (c + 1) / 2;
+    }
+    d.ngw.push_back(ngw);
+    vec rcn(MAXREF, 0.0), rq(MAXREF, 0.0);
+    for (int r = 0; r < nr; r++) {
+      rcn[r] = D4_REFCOVCN[Z - 1][r];
+      rq[r] = o.charge_model == 0 ? D4_REFQ_EEQ[Z - 1][r] : D4_REFQ_GFN2[Z - 1][r];
+    }
+    d.refcn.push_back(rcn);
+    d.refq.push_back(rq);
+    // reference_alpha (model.rs:197-228)
+    vector<vec> al(MAXREF, vec(NFREQ, 0.0));
+    for (int r = 0; r < nr; r++) {
+      int is = D4_REFSYS[Z - 1][r];
+      if (is == 0) continue;
+      double iz = tabz(D4_EFFECTIVE_NUCLEAR_CHARGE, is);
+      double gam = tabz(D4_CHEMICAL_HARDNESS, is) * GC;
+      double refh = o.charge_model == 0 ? D4_REFH_EEQ[Z - 1][r] : D4_REFH_GFN2[Z - 1][r];
+      double zv = zeta_f(GA, gam, iz, refh + iz);
+      double ascale = D4_ASCALE[Z - 1][r], hcount = D4_HCOUNT[Z - 1][r];
+      for (int k = 0; k < NFREQ; k++) {
+        double aiw = D4_SSCALE[is - 1] * D4_SECAIW[is - 1][k] * zv;
+        double v = ascale * (D4_ALPHAIW[Z - 1][r][k] - hcount * aiw);
+        al[r][k] = v > 0.0 ? v : 0.0;
+      }
+    }
+    d.aiw.push_back(al);
+  }
+  if (o.rcov_mode == 0) {
+    // dftd4 default: en table of dftd4 = Pauling (same values in mctc-lib)
+  }
+  // D4 CN (d4/src/ncoord.rs): erf counting with the EN factor, cutoff, Jacobian gradient order
+  d.cn.assign(nat, 0.0);
+  double c2 = o.cn_cut * o.cn_cut;
+  for (int i = 0; i < nat; i++)
+    for (int j = 0; j < i; j++) {
+      double v[3];
+      for (int k = 0; k < 3; k++) v[k] = xyz[3 * i + k] - xyz[3 * j + k];
+      double r2 = dot3(v, v);
+      if (r2 > c2 || r2 < 1e-12) continue;
+      double x = std::fabs(d.en[i] - d.en[j]) + 19.08857;
+      double w = 4.10451 * std::exp(-((x * x) / (2.0 * (11.28174 * 11.28174))));
+      double c = w * erf_count(7.5, std::sqrt(r2), d.rcov[i] + d.rcov[j]);
+      d.cn[i] += c;
+      d.cn[j] += c;
+    }
+  return d;
+}
+
+inline double d4_refc6(const D4& d, int i, int j, int ri, int rj) {
+  double prod[NFREQ];
+  for (int k = 0; k < NFREQ; k++) prod[k] = d.aiw[i][ri][k] * d.aiw[j][rj][k];
+  return 3.0 / PI * trapzd(prod);
+}
+
+// d4/src/model.rs:468-545 reference weights (and derivatives)
+void d4_weights(const D4& d, const vec& q, vector<vec>& gw, vector<vec>* dgwdcn, vector<vec>* dgwdq) {
+  int nat = d.nat;
+  gw.assign(nat, vec(MAXREF, 0.0));
+  if (dgwdcn) dgwdcn->assign(nat, vec(MAXREF, 0.0));
+  if (dgwdq) dgwdq->assign(nat, vec(MAXREF, 0.0));
+  double eps_norm = std::sqrt(2.2250738585072014e-308);
+  for (int i = 0; i < nat; i++) {
+    int nref = d.nref[i];
+    double zi = d.zeff[i], gi = d.eta[i] * GC, cni = d.cn[i];
+    double norm = 0.0, dnorm = 0.0;
+    for (int r = 0; r < nref; r++)
+      for (int g = 1; g <= d.ngw[i][r]; g++) {
+        double wf = g * WF;
+        double dd = cni - d.refcn[i][r];
+        double gg = std::exp(-(wf * (dd * dd)));
+        norm += gg;
+        dnorm += 2.0 * wf * (d.refcn[i][r] - cni) * gg;
+      }
+    norm = std::fabs(norm) > eps_norm ? 1.0 / norm : 0.0;
+    double maxcn = -INFINITY;
+    for (int r = 0; r < nref; r++) maxcn = std::max(maxcn, d.refcn[i][r]);
+    for (int r = 0; r < nref; r++) {
+      double expw = 0.0, expd = 0.0;
+      for (int g = 1; g <= d.ngw[i][r]; g++) {
+        double wf = g * WF;
+        double dd = cni - d.refcn[i][r];
+        double gg = std::exp(-(wf * (dd * dd)));
+        expw += gg;
+        expd += 2.0 * wf * (d.refcn[i][r] - cni) * gg;
+      }
+      double gwk = expw * norm;
+      if (!std::isfinite(gwk) || norm == 0.0) gwk = std::fabs(maxcn - d.refcn[i][r]) < 1e-12 ? 1.0 : 0.0;
+      double qref = d.refq[i][r] + zi, qmod = q[i] + zi, zz, dz;
+      zeta_df(GA, gi, qref, qmod, zz, dz);
+      gw[i][r] = gwk * zz;
+      if (dgwdq) (*dgwdq)[i][r] = gwk * dz;
+      if (dgwdcn) {
+        double dgwk = norm * (expd - expw * dnorm * norm);
+        if (!std::isfinite(dgwk) || norm == 0.0) dgwk = 0.0;
+        (*dgwdcn)[i][r] = dgwk * zz;
+      }
+    }
+  }
+}
+
+// c6 and its derivatives from two weight sets (model.rs:607-627)
+void d4_pair_c6(const D4& d, int i, int j, const vec& wi, const vec& di, const vec& wj, const vec& dj, double& c6,
+                double& dci, double& dcj) {
+  c6 = dci = dcj = 0.0;
+  for (int ri = 0; ri < d.nref[i]; ri++)
+    for (int rj = 0; rj < d.nref[j]; rj++) {
+      double r = d4_refc6(d, i, j, ri, rj);
+      c6 += wi[ri] * wj[rj] * r;
+      dci += di[ri] * wj[rj] * r;
+      dcj += wi[ri] * dj[rj] * r;
+    }
+}
+
+// d4/src/disp.rs:43-63 smooth cutoff
+inline void smooth_cutoff(double r, double cutoff, double width, double& sw, double& dsw) {
+  if (width <= 0.0 || cutoff <= 0.0) {
+    sw = 1.0;
+    dsw = 0.0;
+    return;
+  }
+  double w = std::min(width, cutoff), inner = cutoff - w;
+  if (r <= inner) {
+    sw = 1.0;
+    dsw = 0.0;
+  } else if (r >= cutoff) {
+    sw = 0.0;
+    dsw = 0.0;
+  } else {
+    double x = (cutoff - r) / w, x2 = x * x;
+    sw = (x2 * x) * (10.0 + x * (-15.0 + 6.0 * x));
+    double omx = 1.0 - x;
+    dsw = -((30.0 * x2 * (omx * omx)) / w);
+  }
+}
+
+inline double pw3(double x) { return (x * x) * x; }
+inline double pw4(double x) { double x2 = x * x; return x2 * x2; }
+inline double pw6(double x) { double x3 = pw3(x); return x3 * x3; }
+inline double pw8(double x) { double x4 = pw4(x); return x4 * x4; }
+
+// pair_e, pair_g of one pair (disp.rs:240-272)
+inline bool d4_pair(const D4& d, int i, int j, double& e, double& gdr) {
+  double v[3];
+  for (int k = 0; k < 3; k++) v[k] = d.xyz[3 * i + k] - d.xyz[3 * j + k];
+  double r2 = dot3(v, v);
+  double cut = d.o.disp2;
+  if (r2 > cut * cut || r2 < 2.220446049250313e-16) return false;
+  double r = std::sqrt(r2), sw, dsw;
+  smooth_cutoff(r, cut, d.o.width2, sw, dsw);
+  if (sw <= 0.0) return false;
+  double rr = 3.0 * d.r4r2[i] * d.r4r2[j];
+  double r0 = A1 * std::sqrt(rr) + A2;
+  double t6 = 1.0 / (pw3(r2) + pw6(r0)), t8 = 1.0 / (pw4(r2) + pw8(r0));
+  double d6 = -(6.0 * (r2 * r2) * (t6 * t6)), d8 = -(8.0 * pw3(r2) * (t8 * t8));
+  double e0 = S6 * t6 + S8 * rr * t8, g0 = S6 * d6 + S8 * rr * d8;
+  e = sw * e0;
+  gdr = sw * g0 + dsw * e0 / r;
+  return true;
+}
+
+// two-body energy, potential dE/dq (disp.rs:380-455)
+void d4_energy_potential(const D4& d, const vec& q, vec* e, vec* vat) {
+  vector<vec> gw, dgwdq;
+  d4_weights(d, q, gw, nullptr, &dgwdq);
+  for (int i = 0; i < d.nat; i++)
+    for (int j = 0; j < i; j++) {
+      double pe, pg;
+      if (!d4_pair(d, i, j, pe, pg)) continue;
+      double c6, dqi, dqj;
+      d4_pair_c6(d, i, j, gw[i], dgwdq[i], gw[j], dgwdq[j], c6, dqi, dqj);
+      if (vat) {
+        (*vat)[i] -= dqi * pe;
+        (*vat)[j] -= dqj * pe;
+      }
+      if (e) {
+        double de = -(c6 * pe * 0.5);
+        (*e)[i] += de;
+        (*e)[j] += de;
+      }
+    }
+}
+
+void d4_cn_gradient(const D4& d, const vec& dedcn, vec& g) {  // d4/src/ncoord.rs (Jacobian order)
+  double c2 = d.o.cn_cut * d.o.cn_cut;
+  for (int i = 0; i < d.nat; i++)
+    for (int j = 0; j < i; j++) {
+      double s = dedcn[i] + dedcn[j];
+      if (s == 0.0) continue;
+      double v[3];
+      for (int k = 0; k < 3; k++) v[k] = d.xyz[3 * i + k] - d.xyz[3 * j + k];
+      double r2 = dot3(v, v);
+      if (r2 > c2 || r2 < 1e-12) continue;
+      double r = std::sqrt(r2);
+      double x = std::fabs(d.en[i] - d.en[j]) + 19.08857;
+      double w = 4.10451 * std::exp(-((x * x) / (2.0 * (11.28174 * 11.28174))));
+      double dc = w * erf_dcount(7.5, r, d.rcov[i] + d.rcov[j]);
+      for (int k = 0; k < 3; k++) {
+        double t = dc * v[k] / r * s;
+        g[3 * i + k] += t;
+        g[3 * j + k] -= t;
+      }
+    }
+}
+
+// disp.rs:460-484 two-body gradient at fixed charges
+void d4_gradient(const D4& d, const vec& q, vec& g) {
+  vector<vec> gw, dgwdcn;
+  d4_weights(d, q, gw, &dgwdcn, nullptr);
+  vec dedcn(d.nat, 0.0);
+  for (int i = 0; i < d.nat; i++)
+    for (int j = 0; j < i; j++) {
+      double pe, pg;
+      if (!d4_pair(d, i, j, pe, pg)) continue;
+      double c6, dci, dcj;
+      d4_pair_c6(d, i, j, gw[i], dgwdcn[i], gw[j], dgwdcn[j], c6, dci, dcj);
+      dedcn[i] -= dci * pe;
+      dedcn[j] -= dcj * pe;
+      for (int k = 0; k < 3; k++) {
+        double dg = -c6 * pg * (d.xyz[3 * i + k] - d.xyz[3 * j + k]);
+        g[3 * i + k] += dg;
+        g[3 * j + k] -= dg;
+      }
+    }
+  d4_cn_gradient(d, dedcn, g);
+}
+
+// ATM three-body at q = 0, per triple (disp.rs:665-870; the pair-sum form gives the same
+// values up to rounding). Adds atom energies and, with g, the gradient incl. the D4 CN chain.
+void d4_three_body(const D4& d, vec& e, vec* g) {
+  int nat = d.nat;
+  vec q0(nat, 0.0);
+  vector<vec> gw, dgwdcn;
+  d4_weights(d, q0, gw, &dgwdcn, nullptr);
+  double cut = d.o.disp3, alp3 = ALP / 3.0;
+  // pair table
+  Mat C6(nat, nat), QA(nat, nat), SW(nat, nat), DSWR(nat, nat), T(nat, nat), R2(nat, nat);
+  vector<char> ok((size_t)nat * nat, 0);
+  for (int a = 0; a < nat; a++)
+    for (int b = 0; b < a; b++) {
+      double v[3];
+      for (int k = 0; k < 3; k++) v[k] = d.xyz[3 * b + k] - d.xyz[3 * a + k];
+      double r2 = dot3(v, v);
+      if (r2 > cut * cut || r2 < 2.220446049250313e-16) continue;
+      double r = std::sqrt(r2), sw, dsw;
+      smooth_cutoff(r, cut, d.o.width3, sw, dsw);
+      double rr = 3.0 * d.r4r2[a] * d.r4r2[b];
+      double r0 = A1 * std::sqrt(rr) + A2;
+      double c6, dca, dcb;
+      d4_pair_c6(d, a, b, gw[a], dgwdcn[a], gw[b], dgwdcn[b], c6, dca, dcb);
+      double t = std::pow(r0 / r, alp3);
+      double qa = c6 != 0.0 ? dca / c6 : 0.0, qb = c6 != 0.0 ? dcb / c6 : 0.0;
+      for (int pass = 0; pass < 2; pass++) {
+        int x = pass ? b : a, y = pass ? a : b;
+        ok[(size_t)y * nat + x] = sw > 0.0;
+        C6(x, y) = c6;
+        SW(x, y) = sw;
+        DSWR(x, y) = dsw / r;
+        T(x, y) = t;
+        R2(x, y) = r2;
+        QA(x, y) = pass ? qb : qa;  // dlnC6/dCN of the first index
+      }
+    }
+  vec dedcn(nat, 0.0);
+  for (int i = 0; i < nat; i++)
+    for (int j = 0; j < i; j++) {
+      if (!ok[(size_t)j * nat + i]) continue;
+      for (int k = 0; k < j; k++) {
+        if (!ok[(size_t)k * nat + i] || !ok[(size_t)k * nat + j]) continue;
+        double swij = SW(i, j), swik = SW(i, k), swjk = SW(j, k);
+        double sw = swij * swik * swjk;
+        if (sw <= 0.0) continue;
+        double r2ij = R2(i, j), r2ik = R2(i, k), r2jk = R2(j, k);
+        double c9 = -(S9 * std::sqrt(std::fabs(C6(i, j) * C6(i, k) * C6(j, k))));
+        double ir2 = 1.0 / r2ij * (1.0 / r2ik) * (1.0 / r2jk);
+        double ir1 = std::sqrt(ir2);
+        double ir3 = ir2 * ir1, ir5 = ir3 * ir2;
+        double rpow = T(i, j) * T(i, k) * T(j, k);
+        double fdmp = 1.0 / (1.0 + 6.0 * rpow);
+        double ang = 0.375 * (r2ij + r2jk - r2ik) * (r2ij - r2jk + r2ik) * (-r2ij + r2jk + r2ik) * ir5 + ir3;
+        double rr = ang * fdmp;
+        double de0 = rr * c9;
+        double de = de0 * sw;
+        e[i] -= de / 3.0;
+        e[j] -= de / 3.0;
+        e[k] -= de / 3.0;
+        if (!g) continue;
+        double dfdmp = -(2.0 * ALP * rpow * (fdmp * fdmp));
+        double swc9 = sw * c9;
+        auto dangf = [&](double a, double b, double c) {  // derivative of ang wrt a = r2 of the pair
+          return -((0.375 * (pw3(a) + (a * a) * (b + c) + a * (3.0 * (b * b) + 2.0 * b * c + 3.0 * (c * c)) -
+                             5.0 * ((b - c) * (b - c)) * (b + c))) * ir5);
+        };
+        double fij = swc9 * (-(dangf(r2ij, r2jk, r2ik) * fdmp) + ang * dfdmp) / r2ij - de0 * DSWR(i, j) * swik * swjk;
+        double fik = swc9 * (-(dangf(r2ik, r2jk, r2ij) * fdmp) + ang * dfdmp) / r2ik - de0 * DSWR(i, k) * swij * swjk;
+        double fjk = swc9 * (-(dangf(r2jk, r2ik, r2ij) * fdmp) + ang * dfdmp) / r2jk - de0 * DSWR(j, k) * swij * swik;
+        for (int x = 0; x < 3; x++) {
+          double vij = d.xyz[3 * j + x] - d.xyz[3 * i + x];
+          double vik = d.xyz[3 * k + x] - d.xyz[3 * i + x];
+          double vjk = d.xyz[3 * k + x] - d.xyz[3 * j + x];
+          double dgij = fij * vij, dgik = fik * vik, dgjk = fjk * vjk;
+          (*g)[3 * i + x] -= dgij + dgik;
+          (*g)[3 * j + x] += dgij - dgjk;
+          (*g)[3 * k + x] += dgik + dgjk;
+        }
+        double h = de * 0.5;
+        dedcn[i] -= h * (QA(i, j) + QA(i, k));
+        dedcn[j] -= h * (QA(j, i) + QA(j, k));
+        dedcn[k] -= h * (QA(k, i) + QA(k, j));
+      }
+    }
+  if (g) d4_cn_gradient(d, dedcn, *g);
+}
+
+// ------------------------------------------------------------------ eigensolver
+// Householder reduction to tridiagonal form with accumulated transformation, then implicit QL
+// with Wilkinson-type shifts (the textbook EISPACK tred2/tql2 scheme, own implementation).
+// In: a (n x n, column-major, symmetric; overwritten by the eigenvectors). Out: w ascending.
+void eigh(int n, vec& a, vec& w) {
+  auto A = [&](int i, int j) -> double& { return a[(size_t)j * n + i]; };
+  vec d(n), e(n);
+  // tridiagonalise from the last row up (work in row-major view: use A(i,j) symmetric)
+  for (int i = n - 1; i > 0; i--) {
+    int l = i - 1;
+    double h = 0.0, scale = 0.0;
+    if (l > 0) {
+      for (int k = 0; k <= l; k++) scale += std::fabs(A(i, k));
+      if (scale == 0.0) {
+        e[i] = A(i, l);
+      } else {
+        for (int k = 0; k <= l; k++) {
+          A(i, k) /= scale;
+          h += A(i, k) * A(i, k);
+        }
+        double f = A(i, l);
+        double g = f >= 0.0 ? -std::sqrt(h) : std::sqrt(h);
+        e[i] = scale * g;
+        h -= f * g;
+        A(i, l) = f - g;
+        f = 0.0;
+        for (int j = 0; j <= l; j++) {
+          A(j, i) = A(i, j) / h;
+          g = 0.0;
+          for (int k = 0; k <= j; k++) g += A(j, k) * A(i, k);
+          for (int k = j + 1; k <= l; k++) g += A(k, j) * A(i, k);
+          e[j] = g / h;
+          f += e[j] * A(i, j);
+        }
+        double hh = f / (h + h);
+        for (int j = 0; j <= l; j++) {
+          f = A(i, j);
+          e[j] = g = e[j] - hh * f;
+          for (int k = 0; k <= j; k++) A(j, k) -= (f * e[k] + g * A(i, k));
+        }
+      }
+    } else {
+      e[i] = A(i, l);
+    }
+    d[i] = h;
+  }
+  d[0] = 0.0;
+  e[0] = 0.0;
+  for (int i = 0; i < n; i++) {
+    int l = i - 1;
+    if (d[i] != 0.0) {
+      for (int j = 0; j <= l; j++) {
+        double g = 0.0;
+        for (int k = 0; k <= l; k++) g += A(i, k) * A(k, j);
+        for (int k = 0; k <= l; k++) A(k, j) -= g * A(k, i);
+      }
+    }
+    d[i] = A(i, i);
+    A(i, i) = 1.0;
+    for (int j = 0; j <= l; j++) A(j, i) = A(i, j) = 0.0;
+  }
+  // implicit QL on (d, e) with vectors in A (rows = components)
+  for (int i = 1; i < n; i++) e[i - 1] = e[i];
+  if (n > 0) e[n - 1] = 0.0;
+  for (int l = 0; l < n; l++) {
+    int iter = 0, mm;
+    do {
+      for (mm = l; mm < n - 1; mm++) {
+        double dd = std::fabs(d[mm]) + std::fabs(d[mm + 1]);
+        if (std::fabs(e[mm]) <= 2.220446049250313e-16 * dd) break;
+      }
+      if (mm != l) {
+        if (iter++ == 60) throw Error(2, "QL did not converge");
+        double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+        double r = std::hypot(g, 1.0);
+        g = d[mm] - d[l] + e[l] / (g + (g >= 0.0 ? std::fabs(r) : -std::fabs(r)));
+        double s = 1.0, c = 1.0, p = 0.0;
+        int i;
+        for (i = mm - 1; i >= l; i--) {
+          double f = s * e[i], b = c * e[i];
+          e[i + 1] = (r = std::hypot(f, g));
+          if (r == 0.0) {
+            d[i + 1] -= p;
+            e[mm] = 0.0;
+            break;
+          }
+          s = f / r;
+          c = g / r;
+          g = d[i + 1] - p;
+          r = (d[i] - g) * s + 2.0 * c * b;
+          d[i + 1] = g + (p = s * r);
+          g = c * r - b;
+          for (int k = 0; k < n; k++) {
+            f = A(k, i + 1);
+            A(k, i + 1) = s * A(k, i) + c * f;
+            A(k, i) = c * A(k, i) - s * f;
+          }
+        }
+        if (r == 0.0 && i >= l) continue;
+        d[l] -= p;
+        e[l] = g;
+        e[mm] = 0.0;
+      }
+    } while (mm != l);
+  }
+  // sort ascending
+  vector<int> idx(n);
+  for (int i = 0; i < n; i++) idx[i] = i;
+  std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return d[x] < d[y]; });
+  vec out((size_t)n * n);
+  w.resize(n);
+  for (int j = 0; j < n; j++) {
+    w[j] = d[idx[j]];
+    std::memcpy(&out[(size_t)j * n], &a[(size_t)idx[j] * n], sizeof(double) * n);
+  }
+  a.swap(out);
+}
+
+// ------------------------------------------------------------------ occupations (base/src/occupations.rs)
+int aufbau(double nel, vec& occ) {
+  int n = (int)occ.size();
+  int full = (int)std::floor(nel);
+  for (int i = 0; i < n; i++) occ[i] = i < full ? 1.0 : 0.0;
+  double rem = std::fmod(nel, 1.0);
+  if (full < n) occ[full] = rem;
+  return full + (rem > 0.5 ? 1 : 0);
+}
+
+inline void fermi(double arg, double kt, double& f, double& df) {
+  if (arg > 50.0) {
+    f = 0.0;
+    df = 0.0;
+  } else if (arg < -50.0) {
+    f = 1.0;
+    df = 0.0;
+  } else {
+    f = 1.0 / (std::exp(arg) + 1.0);
+    df = -f * (1.0 - f) / kt;
+  }
+}
+
+// occupations.rs:94-125 (Newton::XTB680: tol 1e5 eps, 200 steps)
+double fill_tblite(double nel, double kt, const vec& emo, vec& occ) {
+  int homo = aufbau(nel, occ);
+  if (!(nel > 0.0 && kt > 0.0)) return NAN;
+  int nao = (int)emo.size();
+  double sqrttiny = std::sqrt(2.2250738585072014e-308);
+  int i0 = std::max(homo, 1) - 1, i1 = std::min(homo + 1, nao) - 1;
+  double ef = 0.5 * (emo[i0] + emo[i1]), ef_used = ef;
+  for (int it = 0; it < 200; it++) {
+    double total = 0.0, dtotal = 0.0;
+    ef_used = ef;
+    for (int i = 0; i < nao; i++) {
+      double f, df;
+      fermi((emo[i] - ef) / kt, kt, f, df);
+      occ[i] = f;
+      total += f;
+      dtotal -= df;
+    }
+    if (std::fabs(nel - total) <= 1e5 * 2.220446049250313e-16) break;
+    if (dtotal > sqrttiny)
+      ef += (nel - total) / dtotal;
+    else
+      break;
+  }
+  return ef_used;
+}
+
+double entropy_tblite(const vec& occ, double kt) {  // occupations.rs:179-181
+  double s = 0.0;
+  for (double o : occ) s += std::log(std::pow(o, o) * std::pow(1.0 - o, 1.0 - o));
+  return s * kt;
+}
+
+// ------------------------------------------------------------------ Broyden (scf/mixing.rs:192-338)
+struct Broyden {
+  int ndim, memory, iter = 0;
+  double damp;
+  vector<vec> df, u;
+  vector<vec> a;
+  vec omega, q_in, dq, qlast, dqlast;
+  Broyden(int n, int mem, double d) : ndim(n), memory(std::max(mem, 1)), damp(d), q_in(n), dq(n), qlast(n), dqlast(n) {}
+  void set_residual(const vec& out) {
+    for (int k = 0; k < ndim; k++) dq[k] = out[k] - q_in[k];
+  }
+  static double dotv(const vec& x, const vec& y) {
+    double s = 0;
+    for (size_t i = 0; i < x.size(); i++) s += x[i] * y[i];
+    return s;
+  }
+  bool next() {
+    iter++;
+    int n = ndim;
+    double alpha = damp;
+    if (iter == 1) {
+      dqlast = dq;
+      qlast = q_in;
+      for (int i = 0; i < n; i++) q_in[i] += alpha * dq[i];
+      return true;
+    }
+    int itn = iter - 1;
+    int it1 = (itn - 1) % memory;
+    int nhist = std::min(memory, itn);
+    int first = itn + 1 > memory ? itn + 1 - memory : 1;
+    vector<int> hist(nhist);
+    for (int ih = 0; ih < nhist; ih++) hist[ih] = (first + ih - 1) % memory;
+    const double OMEGA0 = 0.01, MINW = 1.0, MAXW = 100000.0, WFAC = 0.01;
+    if (it1 == (int)df.size()) {
+      df.push_back(vec(n));
+      u.push_back(vec(n));
+      omega.push_back(0.0);
+      for (auto& row : a) row.push_back(0.0);
+      a.push_back(vec(df.size(), 0.0));
+    }
+    double norm = std::sqrt(dotv(dq, dq));
+    omega[it1] = norm > WFAC / MAXW ? WFAC / norm : MAXW;
+    if (omega[it1] < MINW) omega[it1] = MINW;
+    for (int i = 0; i < n; i++) df[it1][i] = dq[i] - dqlast[i];
+    double inv = 1.0 / std::max(std::sqrt(dotv(df[it1], df[it1])), 2.220446049250313e-16);
+    for (double& x : df[it1]) x *= inv;
+    vec c(nhist);
+    vector<vec> beta(nhist, vec(nhist));
+    for (int ih = 0; ih < nhist; ih++) {
+      int i = hist[ih];
+      c[ih] = omega[i] * dotv(df[i], dq);
+      for (int jh = 0; jh < nhist; jh++) {
+        int j = hist[jh];
+        a[i][j] = dotv(df[i], df[j]);
+      }
+    }
+    for (int ih = 0; ih < nhist; ih++) {
+      for (int jh = 0; jh < nhist; jh++) beta[ih][jh] = omega[hist[ih]] * omega[hist[jh]] * a[hist[ih]][hist[jh]];
+      beta[ih][ih] += OMEGA0 * OMEGA0;
+    }
+    if (!gauss_last_max(beta, c)) return false;
+    for (int k = 0; k < n; k++) u[it1][k] = alpha * df[it1][k] + inv * (q_in[k] - qlast[k]);
+    dqlast = dq;
+    qlast = q_in;
+    for (int k = 0; k < n; k++) q_in[k] += alpha * dq[k];
+    for (int ih = 0; ih < nhist; ih++) {
+      int i = hist[ih];
+      double f = omega[i] * c[ih];
+      for (int k = 0; k < n; k++) q_in[k] -= f * u[i][k];
+    }
+    return true;
+  }
+};
+
+// ------------------------------------------------------------------ the calculation (xtb.rs:437-861)
+struct Options {
+  double acc = 1.0, etemp = 300.0, damping = 0.4;
+  int max_iter = 250, guess = 0 /*0 EEQ, 1 SAD*/, want_grad = 1, dispersion = 1;
+};
+
+struct Result {
+  double energy = 0;
+  double terms[8] = {0};  // repulsion dispersion electronic isotropic_es third_order anisotropic entropy band
+  vec grad, qat, qsh, dp, qp, emo, focc;
+  int iterations = 0, nao = 0, nsh = 0, converged = 0;
+  double efermi = 0, residual = 0, energy_change = 0;
+};
+
+const double SQRT2 = 1.4142135623730951;
+const bool QOFF[6] = {false, true, false, true, true, false};
+
+Result singlepoint(int nat, const int* z, const double* xyz, double charge, const Options& opt) {
+  if (!(opt.acc > 0.0 && std::isfinite(opt.acc))) throw Error(1, "accuracy must be positive");
+  if (!(opt.etemp >= 0.0 && std::isfinite(opt.etemp))) throw Error(1, "etemp must be >= 0");
+  if (!(opt.damping > 0.0 && opt.damping <= 1.0)) throw Error(1, "damping must be in (0, 1]");
+  if (opt.max_iter <= 0) throw Error(1, "max_iter must be >= 1");
+  Model m = build_model(nat, z, xyz, charge, opt.acc);
+  int nsh = m.nsh, n = m.nao;
+  // electrons (xtb.rs:1178-1212), closed shell / lowest spin
+  double nref = 0.0;
+  for (int s = 0; s < nsh; s++) nref += m.refocc[s];
+  if (std::fmod(charge, 1.0) != 0.0) throw Error(1, "the charge must be an integer");
+  double nocc = nref - charge;
+  long nel_int = std::lround(nocc);
+  double nuhf = (double)(((nel_int % 2) + 2) % 2);
+  if (nocc < 0.0) throw Error(1, "negative electron count");
+  if (nuhf > nocc) throw Error(1, "too many unpaired electrons");
+  double nalpha = (nocc + nuhf) / 2.0, nbeta = (nocc - nuhf) / 2.0;
+  if (nalpha > n + 1e-9) throw Error(1, "too many electrons for the basis");
+  double kt = opt.etemp * KB_HARTREE;
+
+  vec g(3 * nat, 0.0), dedcn(nat, 0.0);
+  vec energies(nat, 0.0), erep(nat, 0.0);
+  // geometry (xtb.rs:275-292)
+  vec cn = gfn2_cn(m);
+  vec se(nsh), dsedcn(nsh);
+  for (int s = 0; s < nsh; s++) {
+    se[s] = m.se0[s] - m.kcn[s] * cn[m.sh2at[s]];
+    dsedcn[s] = -m.kcn[s];
+  }
+  Coulomb cou = build_coulomb(m, cn);
+  D4Options d4o;
+  D4 d4 = build_d4(nat, z, xyz, d4o);
+  Result res;
+  res.terms[0] = repulsion(m, erep, opt.want_grad ? &g : nullptr);
+  vec e3b(nat, 0.0);
+  if (opt.dispersion) d4_three_body(d4, e3b, opt.want_grad ? &g : nullptr);
+  double e3sum = 0;
+  for (int i = 0; i < nat; i++) {
+    energies[i] += e3b[i];
+    e3sum += e3b[i];
+  }
+  for (int i = 0; i < nat; i++) energies[i] += erep[i];
+
+  Ints I = build_integrals(m, se);
+  // Cholesky S = L L^T (eigen.rs:72-113)
+  Mat L = I.S;
+  for (int j = 0; j < n; j++) {
+    double s = L(j, j);
+    for (int k = 0; k < j; k++) s -= L(j, k) * L(j, k);
+    if (!(s > 0.0)) throw Error(2, "overlap matrix is not positive definite");
+    double ljj = std::sqrt(s);
+    L(j, j) = ljj;
+    for (int i = j + 1; i < n; i++) {
+      double t = L(i, j);
+      for (int k = 0; k < j; k++) t -= L(i, k) * L(j, k);
+      L(i, j) = t / ljj;
+    }
+    for (int i = 0; i < j; i++) L(i, j) = 0.0;
+  }
+  // Linv (lower)
+  Mat Li(n, n);
+  for (int j = 0; j < n; j++) {
+    Li(j, j) = 1.0 / L(j, j);
+    for (int i = j + 1; i < n; i++) {
+      double s = 0.0;
+      for (int k = j; k < i; k++) s -= L(i, k) * Li(k, j);
+      Li(i, j) = s / L(i, i);
+    }
+  }
+
+  // SCC vector layout (scf/variables.rs): [qsh | dp (3N) | qp (6N, off-diagonals x sqrt2)]
+  int ndim = nsh + 9 * nat;
+  auto pack = [&](const Moments& s, vec& out) {
+    for (int i = 0; i < nsh; i++) out[i] = s.qsh[i];
+    for (int a = 0; a < nat; a++) {
+      for (int c = 0; c < 3; c++) out[nsh + 3 * a + c] = s.dp[3 * a + c];
+      for (int c = 0; c < 6; c++) out[nsh + 3 * nat + 6 * a + c] = s.qp[6 * a + c] * (QOFF[c] ? SQRT2 : 1.0);
+    }
+  };
+  auto qat_of = [&](Moments& s) {
+    s.qat.assign(nat, 0.0);
+    for (int i = 0; i < nsh; i++) s.qat[m.sh2at[i]] += s.qsh[i];
+  };
+  auto unpack = [&](const vec& v, Moments& s) {
+    for (int i = 0; i < nsh; i++) s.qsh[i] = v[i];
+    for (int a = 0; a < nat; a++) {
+      for (int c = 0; c < 3; c++) s.dp[3 * a + c] = v[nsh + 3 * a + c];
+      for (int c = 0; c < 6; c++) s.qp[6 * a + c] = v[nsh + 3 * nat + 6 * a + c] / (QOFF[c] ? SQRT2 : 1.0);
+    }
+    qat_of(s);
+  };
+  Moments start;
+  start.qsh.assign(nsh, 0.0);
+  start.dp.assign(3 * nat, 0.0);
+  start.qp.assign(6 * nat, 0.0);
+  if (opt.guess == 0) {
+    try {
+      vec q = eeq_charges(nat, z, xyz, charge);
+      for (int iat = 0; iat < nat; iat++) {
+        int is = m.ish_at[iat], ns = m.nsh_at[iat];
+        double tot = 0.0;
+        for (int s = 0; s < ns; s++) tot += m.refocc[is + s];
+        for (int s = 0; s < ns; s++) start.qsh[is + s] = tot > 0.0 ? q[iat] * m.refocc[is + s] / tot : 0.0;
+      }
+    } catch (Error&) {
+    }
+  }
+  qat_of(start);
+
+  Moments st = start, out = start;
+  Pots pot;
+  Mat F(n, n), C(n, n), P(n, n), W(n, n), Tmp(n, n);
+  vec emo(n), occa(n), occb(n), focc(n), vao(n);
+  double ts = 0.0, elast = 0.0, efermi = 0.0;
+  vec eelec(nat);
+  Broyden mix(ndim, opt.max_iter, opt.damping);
+  vec packed(ndim);
+  bool converged = false;
+  int iter = 0;
+  double res_norm = 0, de_last = 0;
+  for (iter = 1; iter <= opt.max_iter; iter++) {
+    if (iter > 1) {
+      if (!mix.next()) throw Error(2, "Broyden mixing failed");
+      unpack(mix.q_in, st);
+    }
+    // potentials (xtb.rs:316-340)
+    pot.vsh.assign(nsh, 0.0);
+    pot.vat.assign(nat, 0.0);
+    pot.vdp.assign(3 * nat, 0.0);
+    pot.vqp.assign(6 * nat, 0.0);
+    for (int a = 0; a < nsh; a++) {
+      double s = 0.0;
+      for (int b = 0; b < nsh; b++) s += cou.gam[(size_t)a * nsh + b] * st.qsh[b];
+      pot.vsh[a] += s;
+    }
+    for (int a = 0; a < nsh; a++) pot.vsh[a] += st.qsh[a] * st.qsh[a] * cou.gamma3[a];
+    aes_potentials(m, cou, st, pot);
+    if (opt.dispersion) d4_energy_potential(d4, st.qat, nullptr, &pot.vat);
+    for (int mu = 0; mu < n; mu++) vao[mu] = pot.vsh[m.ao2sh[mu]] + pot.vat[m.ao2at[mu]];
+    // Fock (scf/fock.rs:22-104)
+    for (int nu = 0; nu < n; nu++) {
+      int at = m.ao2at[nu];
+      for (int mu = 0; mu < n; mu++) {
+        double b = I.S(mu, nu) * vao[nu];
+        for (int k = 0; k < 3; k++) b += I.D[k](mu, nu) * pot.vdp[3 * at + k];
+        for (int k = 0; k < 6; k++) b += I.Q[k](mu, nu) * pot.vqp[6 * at + k];
+        Tmp(mu, nu) = b;
+      }
+    }
+    for (int nu = 0; nu < n; nu++)
+      for (int mu = 0; mu < n; mu++) {
+        F(mu, nu) = I.H0(mu, nu) - 0.5 * (Tmp(mu, nu) + Tmp(nu, mu));
+        if (!std::isfinite(F(mu, nu))) throw Error(2, "non-finite Fock matrix");
+      }
+    pack(st, mix.q_in);
+    // A = Linv F Linv^T, eig, C = Linv^T U
+    Mat X(n, n);
+    for (int j = 0; j < n; j++)
+      for (int i = 0; i < n; i++) {
+        double s = 0;
+        for (int k = 0; k <= i; k++) s += Li(i, k) * F(k, j);
+        X(i, j) = s;
+      }
+    vec A((size_t)n * n);
+    for (int j = 0; j < n; j++)
+      for (int i = 0; i < n; i++) {
+        double s = 0;
+        for (int k = 0; k <= j; k++) s += X(i, k) * Li(j, k);
+        A[(size_t)j * n + i] = s;
+      }
+    for (int j = 0; j < n; j++)
+      for (int i = 0; i < j; i++) {
+        double v = 0.5 * (A[(size_t)j * n + i] + A[(size_t)i * n + j]);
+        A[(size_t)j * n + i] = A[(size_t)i * n + j] = v;
+      }
+    eigh(n, A, emo);
+    for (int j = 0; j < n; j++)
+      for (int i = 0; i < n; i++) {
+        double s = 0;
+        for (int k = i; k < n; k++) s += Li(k, i) * A[(size_t)j * n + k];
+        C(i, j) = s;
+      }
+    // Fermi filling (xtb.rs:997-1031)
+    if (kt > 0.0) {
+      efermi = fill_tblite(nalpha, kt, emo, occa);
+      fill_tblite(nbeta, kt, emo, occb);
+    } else {
+      aufbau(nalpha, occa);
+      aufbau(nbeta, occb);
+    }
+    double pa = 0, pb = 0;
+    for (int i = 0; i < n; i++) {
+      pa += occa[i];
+      pb += occb[i];
+    }
+    if (std::fabs(pa - nalpha) > 1e-6 || std::fabs(pb - nbeta) > 1e-6) throw Error(2, "Fermi level not found");
+    ts = entropy_tblite(occa, kt) + entropy_tblite(occb, kt);
+    for (int i = 0; i < n; i++) focc[i] = occa[i] + occb[i];
+    // density P = C f C^T (xtb.rs:1033-1038)
+    for (int j = 0; j < n; j++)
+      for (int i = j; i < n; i++) {
+        double s = 0;
+        for (int k = 0; k < n; k++)
+          if (focc[k] > 1e-16) s += C(i, k) * focc[k] * C(j, k);
+        P(i, j) = s;
+        P(j, i) = s;
+      }
+    // Mulliken (scf/mulliken.rs:19-59) and the band energy per atom
+    out.qsh = m.refocc;
+    out.dp.assign(3 * nat, 0.0);
+    out.qp.assign(6 * nat, 0.0);
+    vec eband(nat, 0.0);
+    double band = 0.0;
+    for (int nu = 0; nu < n; nu++) {
+      double r[11] = {0};
+      for (int mu = 0; mu < n; mu++) {
+        double p = P(mu, nu);
+        r[0] += p * I.S(mu, nu);
+        for (int k = 0; k < 3; k++) r[1 + k] += p * I.D[k](mu, nu);
+        for (int k = 0; k < 6; k++) r[4 + k] += p * I.Q[k](mu, nu);
+        r[10] += p * I.H0(mu, nu);
+      }
+      int iat = m.ao2at[nu];
+      out.qsh[m.ao2sh[nu]] -= r[0];
+      for (int k = 0; k < 3; k++) out.dp[3 * iat + k] -= r[1 + k];
+      for (int k = 0; k < 6; k++) out.qp[6 * iat + k] -= r[4 + k];
+      eband[iat] += r[10];
+      band += r[10];
+    }
+    qat_of(out);
+    pack(out, packed);
+    mix.set_residual(packed);
+    // energy of the output moments (xtb.rs:1058-1146)
+    std::fill(eelec.begin(), eelec.end(), 0.0);
+    double es2 = 0.0;
+    for (int a = 0; a < nsh; a++) {
+      double v = 0.0;
+      for (int b = 0; b < nsh; b++) v += cou.gam[(size_t)a * nsh + b] * out.qsh[b];
+      es2 += out.qsh[a] * v;
+      eelec[m.sh2at[a]] += 0.5 * out.qsh[a] * v;
+    }
+    es2 *= 0.5;
+    vec e3(nat, 0.0), eaes(nat, 0.0), ed(nat, 0.0);
+    for (int a = 0; a < nsh; a++) e3[m.sh2at[a]] += std::pow(out.qsh[a], 3) * cou.gamma3[a] / 3.0;
+    aes_energies(m, cou, out, eaes);
+    if (opt.dispersion) d4_energy_potential(d4, out.qat, &ed, nullptr);
+    double se3 = 0, saes = 0, sd = 0;
+    for (int i = 0; i < nat; i++) {
+      se3 += e3[i];
+      saes += eaes[i];
+      sd += ed[i];
+      eelec[i] += e3[i] + eaes[i] + ed[i] + ts / nat + eband[i];
+    }
+    double energy = 0.0;
+    for (int i = 0; i < nat; i++) energy += eelec[i];
+    if (!std::isfinite(energy)) throw Error(2, "non-finite energy");
+    res_norm = 0.0;
+    for (double x : mix.dq) res_norm += x * x;
+    res_norm = std::sqrt(res_norm / ndim);
+    de_last = energy - elast;
+    elast = energy;
+    res.terms[2] = 0;
+    for (int i = 0; i < nat; i++) res.terms[2] += eelec[i];
+    res.terms[1] = e3sum + sd;
+    res.terms[3] = es2;
+    res.terms[4] = se3;
+    res.terms[5] = saes;
+    res.terms[6] = ts;
+    res.terms[7] = band;
+    // convergence (scf/convergence.rs:99-121)
+    if (std::fabs(de_last) < 1e-6 * opt.acc && res_norm < 2e-5 * opt.acc) {
+      converged = true;
+      break;
+    }
+  }
+  if (!converged) throw Error(3, "SCF not converged");
+  res.iterations = iter;
+  res.converged = 1;
+  res.residual = res_norm;
+  res.energy_change = de_last;
+  double etot = 0.0;
+  for (int i = 0; i < nat; i++) etot += energies[i] + eelec[i];
+  res.energy = etot;
+  res.nao = n;
+  res.nsh = nsh;
+  res.qsh = out.qsh;
+  res.qat = out.qat;
+  res.dp = out.dp;
+  res.qp = out.qp;
+  res.emo = emo;
+  res.focc = focc;
+  res.efermi = efermi;
+  if (opt.want_grad) {
+    // W = C diag(f eps) C^T (xtb.rs:688-696)
+    for (int j = 0; j < n; j++)
+      for (int i = j; i < n; i++) {
+        double s = 0;
+        for (int k = 0; k < n; k++)
+          if (focc[k] > 1e-16) s += C(i, k) * focc[k] * emo[k] * C(j, k);
+        W(i, j) = s;
+        W(j, i) = s;
+      }
+    // add_scc_gradient (xtb.rs:346-387), output moments, last input potentials
+    es2_gradient(m, cou, out.qsh, g);
+    aes_gradient(m, cou, out, g, dedcn);
+    if (opt.dispersion) d4_gradient(d4, out.qat, g);
+    // H0/S/multipole gradient (hamiltonian.rs:137-223)
+    MpGrad mg;
+    for (int iat = 0; iat < nat; iat++) {
+      for (int jat = 0; jat < iat; jat++) {
+        double vec3[3];
+        for (int k = 0; k < 3; k++) vec3[k] = m.xyz[3 * iat + k] - m.xyz[3 * jat + k];
+        double r2 = dot3(vec3, vec3);
+        if (pair_screened(m.min_alpha_at[iat], m.min_alpha_at[jat], r2, m.intcut)) continue;
+        if (r2 < 2.220446049250313e-16) continue;
+        double rr = std::sqrt(std::sqrt(r2) / (m.rad[jat] + m.rad[iat]));
+        double dg[3] = {0, 0, 0}, dcni = 0, dcnj = 0;
+        for (int ish = 0; ish < m.nsh_at[iat]; ish++) {
+          int si = m.ish_at[iat] + ish;
+          const Cgto& ci = m.shells[iat][ish];
+          for (int jsh = 0; jsh < m.nsh_at[jat]; jsh++) {
+            int sj = m.ish_at[jat] + jsh;
+            const Cgto& cj = m.shells[jat][jsh];
+            if (pair_screened(m.min_alpha_sh[si], m.min_alpha_sh[sj], r2, m.intcut)) continue;
+            multipole_grad(cj, ci, vec3, m.intcut, mg);
+            double shpi = 1.0 + m.shpoly[si] * rr, shpj = 1.0 + m.shpoly[sj] * rr;
+            double shp = shpi * shpj;
+            double dshp = (shpi * m.shpoly[sj] + shpj * m.shpoly[si]) * 0.5 * rr / r2;
+            double dsv[3];
+            for (int k = 0; k < 3; k++) dsv[k] = dshp / shp * vec3[k];
+            double hs = hscale(m, m.z[jat], jsh, m.z[iat], ish) * shp;
+            double hij = 0.5 * (se[si] + se[sj]) * hs;
+            double dhdcni = dsedcn[si] * hs, dhdcnj = dsedcn[sj] * hs;
+            int nj = nsph(cj.l);
+            for (int iao = 0; iao < nsph(ci.l); iao++)
+              for (int jao = 0; jao < nj; jao++) {
+                int ij = jao + nj * iao;
+                int mu = m.iao_sh[sj] + jao, nu = m.iao_sh[si] + iao;
+                double pij = P(mu, nu);
+                double hpij = pij * hij;
+                double sval = -pij * (vao[mu] + vao[nu]) + 2.0 * hpij - 2.0 * W(mu, nu);
+                for (int k = 0; k < 3; k++) {
+                  double mp = 0.0;
+                  for (int c = 0; c < 3; c++)
+                    mp += mg.ddi[ij][k][c] * pot.vdp[3 * iat + c] + mg.ddj[ij][k][c] * pot.vdp[3 * jat + c];
+                  for (int c = 0; c < 6; c++)
+                    mp += mg.dqi[ij][k][c] * pot.vqp[6 * iat + c] + mg.dqj[ij][k][c] * pot.vqp[6 * jat + c];
+                  dg[k] += sval * mg.ds[ij][k] + 2.0 * hpij * mg.s[ij] * dsv[k] - pij * mp;
+                }
+                dcni += dhdcni * pij * mg.s[ij];
+                dcnj += dhdcnj * pij * mg.s[ij];
+              }
+          }
+        }
+        dedcn[iat] += dcni;
+        dedcn[jat] += dcnj;
+        for (int k = 0; k < 3; k++) {
+          g[3 * iat + k] += dg[k];
+          g[3 * jat + k] -= dg[k];
+        }
+      }
+      for (int ish = 0; ish < m.nsh_at[iat]; ish++) {
+        int s = m.ish_at[iat] + ish;
+        int ii = m.iao_sh[s];
+        double d = 0.0;
+        for (int iao = 0; iao < m.nao_sh[s]; iao++) d += dsedcn[s] * P(ii + iao, ii + iao);
+        dedcn[iat] += d;
+      }
+    }
+    // CN chain (terms/chain.rs:288, base/src/ncoord.rs:486-519)
+    gfn2_cn_gradient(m, dedcn, g);
+    res.grad = g;
+  }
+  return res;
+}
+
+thread_local std::string last_error;
+
+}  // namespace
+
+extern "C" {
+
+// Sizes of the basis of a molecule: nsh, nao. Returns 0 or an error code.
+int oracle_sizes(int nat, const int* z, int* nsh, int* nao) {
+  try {
+    double xyz0[3] = {0, 0, 0};
+    (void)xyz0;
+    int s = 0, a = 0;
+    for (int i = 0; i < nat; i++) {
+      if (z[i] < 1 || z[i] > AXTB_GFN2_MAX_ELEM) return 4;
+      for (int k = 0; k < GFN2_NSHELL[z[i] - 1]; k++) {
+        s++;
+        a += 2 * GFN2_ANG_SHELL[z[i] - 1][k] + 1;
+      }
+    }
+    *nsh = s;
+    *nao = a;
+    return 0;
+  } catch (...) {
+    return 1;
+  }
+}
+
+const char* oracle_last_error() { return last_error.c_str(); }
+
+// GFN2-xTB single point. Outputs (caller-owned, may be null): energy, terms[8], grad[3N], qat[N],
+// qsh[nsh], dp[3N], qp[6N], emo[nao], iterations. Returns 0 ok, 1 input, 2 numerical,
+// 3 SCF not converged, 4 unsupported element.
+int oracle_engrad(int nat, const int* z, const double* xyz, double charge, double acc, double etemp, int max_iter,
+                  int guess, int want_grad, int dispersion, double* energy, double* terms, double* grad, double* qat,
+                  double* qsh, double* dp, double* qp, double* emo, int* iterations) {
+  try {
+    Options o;
+    o.acc = acc;
+    o.etemp = etemp;
+    o.max_iter = max_iter;
+    o.guess = guess;
+    o.want_grad = want_grad;
+    o.dispersion = dispersion;
+    Result r = singlepoint(nat, z, xyz, charge, o);
+    if (energy) *energy = r.energy;
+    if (terms) std::memcpy(terms, r.terms, sizeof r.terms);
+    if (grad && want_grad) std::memcpy(grad, r.grad.data(), sizeof(double) * 3 * nat);
+    if (qat) std::memcpy(qat, r.qat.data(), sizeof(double) * nat);
+    if (qsh) std::memcpy(qsh, r.qsh.data(), sizeof(double) * r.nsh);
+    if (dp) std::memcpy(dp, r.dp.data(), sizeof(double) * 3 * nat);
+    if (qp) std::memcpy(qp, r.qp.data(), sizeof(double) * 6 * nat);
+    if (emo) std::memcpy(emo, r.emo.data(), sizeof(double) * r.nao);
+    if (iterations) *iterations = r.iterations;
+    return 0;
+  } catch (Error& e) {
+    last_error = e.what();
+    return e.code;
+  } catch (std::exception& e) {
+    last_error = e.what();
+    return 2;
+  }
+}
+
+// EEQ-2019 charges of the SCF start (axtb/src/eeq.rs:43-77)
+int oracle_eeq(int nat, const int* z, const double* xyz, double charge, double* q) {
+  try {
+    vec r = eeq_charges(nat, z, xyz, charge);
+    std::memcpy(q, r.data(), sizeof(double) * nat);
+    return 0;
+  } catch (Error& e) {
+    last_error = e.what();
+    return e.code;
+  }
+}
+
+// D4 pieces for the fixtures: cn[N], c6[N*N], alpha[N], e2, e3 (sums), grad[3N] (2-body at fixed q
+// + 3-body, incl. the D4 CN chain). cutoffs = {cn, disp2, disp3, width2, width3}.
+int oracle_d4(int nat, const int* z, const double* xyz, const double* q, int charge_model, int rcov_mode,
+              const double* cutoffs, double* cn, double* c6, double* alpha, double* e2, double* e3, double* grad) {
+  try {
+    D4Options o;
+    o.charge_model = charge_model;
+    o.rcov_mode = rcov_mode;
+    o.cn_cut = cutoffs[0];
+    o.disp2 = cutoffs[1];
+    o.disp3 = cutoffs[2];
+    o.width2 = cutoffs[3];
+    o.width3 = cutoffs[4];
+    D4 d = build_d4(nat, z, xyz, o);
+    vec qq(q, q + nat);
+    if (cn) std::memcpy(cn, d.cn.data(), sizeof(double) * nat);
+    vector<vec> gw, dgwdcn, dgwdq;
+    d4_weights(d, qq, gw, &dgwdcn, &dgwdq);
+    if (c6)
+      for (int i = 0; i < nat; i++)
+        for (int j = 0; j < nat; j++) {
+          double c, a, b;
+          d4_pair_c6(d, i, j, gw[i], dgwdq[i], gw[j], dgwdq[j], c, a, b);
+          c6[i * nat + j] = c;
+        }
+    if (alpha)
+      for (int i = 0; i < nat; i++) {
+        alpha[i] = 0.0;
+        for (int r = 0; r < d.nref[i]; r++) alpha[i] += gw[i][r] * d.aiw[i][r][0];
+      }
+    vec ea(nat, 0.0), eb(nat, 0.0);
+    d4_energy_potential(d, qq, &ea, nullptr);
+    vec g(3 * nat, 0.0);
+    d4_three_body(d, eb, grad ? &g : nullptr);
+    if (grad) d4_gradient(d, qq, g);
+    double s2 = 0, s3 = 0;
+    for (int i = 0; i < nat; i++) {
+      s2 += ea[i];
+      s3 += eb[i];
+    }
+    if (e2) *e2 = s2;
+    if (e3) *e3 = s3;
+    if (grad) std::memcpy(grad, g.data(), sizeof(double) * 3 * nat);
+    return 0;
+  } catch (Error& e) {
+    last_error = e.what();
+    return e.code;
+  }
+}
+
+// Overlap matrix (column-major nao x nao) and the GFN2 CN of a molecule, for integral checks.
+int oracle_overlap(int nat, const int* z, const double* xyz, double acc, double* S, double* cn_out) {
+  try {
+    Model m = build_model(nat, z, xyz, 0.0, acc);
+    vec cn = gfn2_cn(m);
+    vec se(m.nsh, 0.0);
+    Ints I = build_integrals(m, se);
+    if (S) std::memcpy(S, I.S.a.data(), sizeof(double) * m.nao * m.nao);
+    if (cn_out) std::memcpy(cn_out, cn.data(), sizeof(double) * nat);
+    return 0;
+  } catch (Error& e) {
+    last_error = e.what();
+    return e.code;
+  }
+}
+
+// Dense symmetric eigensolver of the oracle (for its own residual checks): a overwritten by
+// the eigenvectors (column-major), w ascending.
+int oracle_eigh(int n, double* a, double* w) {
+  try {
+    vec A(a, a + (size_t)n * n), W;
+    eigh(n, A, W);
+    std::memcpy(a, A.data(), sizeof(double) * n * n);
+    std::memcpy(w, W.data(), sizeof(double) * n);
+    return 0;
+  } catch (Error& e) {
+    last_error = e.what();
+    return e.code;
+  }
+}
+
+}  // extern "C"
