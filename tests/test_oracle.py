@@ -1,0 +1,176 @@
+"""The CPU oracle against axtb's committed reference data, closed forms and finite differences.
+
+Golden data (tests/golden): axtb's energies, per-term energies, charges and iteration counts
+(reference_matrix/results.json at accuracy 1e-6; the axtb CLI runs of bench/results/final-9985wx
+at accuracy 1), D4 fixtures of dftd4 4.3.1 / tblite 0.8.0 (crates/d4/tests/fixtures).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from axtb_amd import structures
+from conftest import GOLDEN, d4_fixture, structure
+
+GOLD = json.load(open(os.path.join(GOLDEN, "golden.json")))
+
+
+@pytest.mark.parametrize("name", ["water", "caffeine", "taxol"])
+def test_energy_terms_charges_acc1e6(name):
+    g = GOLD["acc1e-6"][name]
+    z, x = structure(name)
+    r = oracle.engrad(z, x, acc=1e-6, gradient=False)
+    assert abs(r["energy"] - g["energy"]) < 1e-9
+    assert r["iterations"] == g["iterations"]
+    for k in ("repulsion", "dispersion", "electronic", "isotropic_es", "third_order", "anisotropic", "entropy"):
+        assert abs(r["terms"][k] - g["terms"][k]) < 1e-9, k
+    if g["charges"] is not None:
+        assert np.abs(r["qat"] - np.array(g["charges"])).max() < 1e-8
+    dip = (r["qat"][:, None] * x).sum(0) + r["dipoles"].sum(0)
+    assert np.abs(dip - np.array(g["dipole"])).max() < 1e-7
+
+
+@pytest.mark.parametrize("name", ["water", "caffeine", "C20H42", "taxol"])
+def test_energy_acc1(name):
+    g = GOLD["acc1"][name]
+    z, x = structures.alkane_bench(20) if name == "C20H42" else structure(name)
+    r = oracle.engrad(z, x, acc=1.0, gradient=False)
+    assert r["iterations"] == g["iterations"]
+    assert abs(r["energy"] - g["energy"]) < 1e-9
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("name", ["C50H102"])
+def test_energy_acc1_large(name):
+    g = GOLD["acc1"][name]
+    z, x = structure("c50h102")
+    r = oracle.engrad(z, x, acc=1.0, gradient=False)
+    assert r["iterations"] == g["iterations"]
+    assert abs(r["energy"] - g["energy"]) < 1e-8
+
+
+@pytest.mark.parametrize("name", ["water", "benzene", "alkane_c40"])
+def test_d4_fixtures(name):
+    z, rec = d4_fixture(name)
+    x = rec["positions"]
+    q = rec["d4.q"]
+    # EEQ-2019 charges of dftd4 (multicharge) = the SCF start's EEQ model
+    assert np.abs(oracle.eeq(z, x) - q).max() < 1e-9
+    for cset in ("dftd4", "tbcut", "wide"):
+        c = rec[cset + ".cutoff"]  # disp2 disp3 cn width2 width3
+        r = oracle.d4(z, x, q, charge_model="eeq", radii="mctc", cutoffs=(c[2], c[0], c[1], c[3], c[4]))
+        assert abs(r["e2"] - rec[cset + ".e2"][0]) <= 1e-12 * abs(rec[cset + ".e2"][0])
+        assert abs(r["e3"] - rec[cset + ".e3"][0]) <= 1e-11 * abs(rec[cset + ".e3"][0])
+        if cset == "dftd4":
+            n = len(z)
+            r30 = oracle.d4(z, x, q, charge_model="eeq", radii="mctc", cutoffs=(30.0, c[0], c[1], c[3], c[4]))
+            assert np.abs(r30["cn"] - rec["d4.cn"]).max() <= 1e-12 * np.abs(rec["d4.cn"]).max()
+            assert np.abs(r30["c6"].ravel() - rec["d4.c6"]).max() <= 1e-12 * np.abs(rec["d4.c6"]).max()
+            assert np.abs(r30["alpha"] - rec["d4.alpha"]).max() <= 1e-12 * np.abs(rec["d4.alpha"]).max()
+    if "tblite_gfn2.e3" in rec:
+        # tblite's GFN2 D4 (GFN2 reference charges, tblite's cutoffs): three-body energy
+        r = oracle.d4(z, x, np.zeros(len(z)), charge_model="gfn2", radii="mctc",
+                      cutoffs=(30.0, 50.0, 25.0, 0.05, 0.05))
+        ref = rec["tblite_gfn2.e3"][0]
+        assert abs(r["e3"] - ref) <= 1e-12 * abs(ref)
+        r = oracle.d4(z, x, rec["tblite_gfn2.q"], charge_model="gfn2", radii="mctc",
+                      cutoffs=(30.0, 50.0, 25.0, 0.05, 0.05))
+        ref = rec["tblite_gfn2.e2_hf"][0]
+        assert abs(r["e2"] - ref) <= 1e-8 * abs(ref)
+
+
+def test_d4_gradient_fd():
+    """Two- and three-body D4 gradient at fixed charges against central differences."""
+    z, rec = d4_fixture("water")
+    x = rec["positions"]
+    q = rec["d4.q"]
+    cut = (40.0, 72.0, 40.0, 1.0, 1.0)
+    g = oracle.d4(z, x, q, charge_model="gfn2", radii="codata", cutoffs=cut, gradient=True)["gradient"]
+    h = 1e-5
+    for i in range(len(z)):
+        for k in range(3):
+            xp, xm = x.copy(), x.copy()
+            xp[i, k] += h
+            xm[i, k] -= h
+            ep = oracle.d4(z, xp, q, "gfn2", "codata", cut)
+            em = oracle.d4(z, xm, q, "gfn2", "codata", cut)
+            fd = ((ep["e2"] + ep["e3"]) - (em["e2"] + em["e3"])) / (2 * h)
+            assert abs(fd - g[i, k]) < 1e-9
+
+
+def test_tblite_scc_charges():
+    """GFN2 Mulliken charges of tblite 0.8.0 (d4 fixture) at ~1e-6: axtb's conventions differ
+    from tblite's only in constants (CODATA 2022 radii, kB) and cutoffs."""
+    z, rec = d4_fixture("water")
+    r = oracle.engrad(z, rec["positions"], acc=1e-6, gradient=False)
+    assert np.abs(r["qat"] - rec["tblite_gfn2.q"]).max() < 1e-5
+
+
+def test_gradient_fd_water():
+    """Analytic gradient against central differences, h = 1e-4 bohr (tests/consistency.rs:26-58)."""
+    z, x = structure("water")
+    r = oracle.engrad(z, x, acc=1e-8)
+    g = r["gradient"]
+    assert np.abs(g.sum(0)).max() < 1e-12
+    h = 1e-4
+    fd = np.zeros_like(g)
+    for i in range(len(z)):
+        for k in range(3):
+            xp, xm = x.copy(), x.copy()
+            xp[i, k] += h
+            xm[i, k] -= h
+            fd[i, k] = (oracle.engrad(z, xp, acc=1e-8, gradient=False)["energy"]
+                        - oracle.engrad(z, xm, acc=1e-8, gradient=False)["energy"]) / (2 * h)
+    assert np.abs(fd - g).max() < 2e-7
+
+
+def test_gradient_fd_alkane_noise():
+    z, x = structures.alkane(3, rng=np.random.default_rng(3), noise=0.1)
+    r = oracle.engrad(z, x, acc=1e-8)
+    g = r["gradient"]
+    h = 1e-4
+    for (i, k) in [(0, 0), (1, 2), (4, 1), (7, 0), (10, 2)]:
+        xp, xm = x.copy(), x.copy()
+        xp[i, k] += h
+        xm[i, k] -= h
+        fd = (oracle.engrad(z, xp, acc=1e-8, gradient=False)["energy"]
+              - oracle.engrad(z, xm, acc=1e-8, gradient=False)["energy"]) / (2 * h)
+        assert abs(fd - g[i, k]) < 2e-7
+
+
+def test_invariants_charged():
+    """Sum of charges = molecular charge; rotating the molecule leaves E unchanged and rotates g."""
+    z, x = structure("caffeine")
+    r = oracle.engrad(z, x, charge=1.0, acc=1e-6)
+    assert abs(r["qat"].sum() - 1.0) < 1e-10
+    th = 0.3
+    R = np.array([[np.cos(th), -np.sin(th), 0], [np.sin(th), np.cos(th), 0], [0, 0, 1.0]])
+    r2 = oracle.engrad(z, x @ R.T, charge=1.0, acc=1e-6)
+    assert abs(r2["energy"] - r["energy"]) < 1e-9
+    assert np.abs(r2["gradient"] - r["gradient"] @ R.T).max() < 1e-7
+
+
+def test_self_overlap_and_eigh():
+    z, x = structure("caffeine")
+    S, _ = oracle.overlap(z, x)
+    assert np.abs(np.diag(S) - 1.0).max() < 1e-9  # STO-nG contraction norm (integrals.rs:1021-1034)
+    assert np.abs(S - S.T).max() < 1e-15
+    w, V = oracle.eigh(S)
+    assert np.abs(S @ V - V * w).max() < 1e-12
+    assert np.abs(V.T @ V - np.eye(len(w))).max() < 1e-12
+    assert np.allclose(w, np.linalg.eigvalsh(S), atol=1e-12)
+
+
+def test_errors():
+    z, x = structure("water")
+    with pytest.raises(oracle.OracleError) as e:
+        oracle.engrad(z, x, charge=9.0)
+    assert e.value.code == 1
+    with pytest.raises(oracle.OracleError) as e:
+        oracle.engrad(z, x, max_iter=2)
+    assert e.value.code == 3
+    with pytest.raises(oracle.OracleError) as e:
+        oracle.engrad(z, x, charge=0.5)
+    assert e.value.code == 1
