@@ -1,0 +1,150 @@
+"""ctypes binding of libaxtb_hip.so (the C ABI declared in include/axtb_hip.h).
+
+The library is built in-tree (``axtb_amd/libaxtb_hip.so``, see ``axtb_amd.build``). Loading
+fails loudly when it is missing: there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libaxtb_hip.so")
+
+OK, ERR_INPUT, ERR_NUMERICAL, ERR_SCF, ERR_ELEMENT, ERR_DEVICE = range(6)
+ERROR_KINDS = {1: "input", 2: "numerical", 3: "scf_not_converged", 4: "unsupported_element", 5: "device"}
+TERMS = ("repulsion", "dispersion", "electronic", "isotropic_es", "third_order", "anisotropic", "entropy", "band")
+EXPORTS = ("axtbhip_version", "axtbhip_context_create", "axtbhip_context_destroy", "axtbhip_last_error",
+           "axtbhip_basis_size", "axtbhip_gfn2_engrad", "axtbhip_gfn2_engrad_batch")
+
+
+class AxtbError(RuntimeError):
+    """An error of the engine; ``kind`` is axtb's error kind (crates/base/src/error.rs:8-24)."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"{ERROR_KINDS.get(code, code)}: {msg}")
+        self.code = code
+        self.kind = ERROR_KINDS.get(code, "unknown")
+
+
+class Mol(ctypes.Structure):
+    _fields_ = [("nat", ctypes.c_int), ("numbers", ctypes.c_void_p), ("positions", ctypes.c_void_p),
+                ("charge", ctypes.c_double), ("uhf", ctypes.c_int)]
+
+
+class Opts(ctypes.Structure):
+    _fields_ = [("accuracy", ctypes.c_double), ("etemp", ctypes.c_double), ("max_iter", ctypes.c_int),
+                ("damping", ctypes.c_double), ("guess", ctypes.c_int), ("want_gradient", ctypes.c_int),
+                ("precision", ctypes.c_int), ("dispersion", ctypes.c_int)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("energy", ctypes.c_double), ("gradient", ctypes.c_void_p), ("charges", ctypes.c_void_p),
+                ("shell_charges", ctypes.c_void_p), ("dipoles", ctypes.c_void_p), ("quadrupoles", ctypes.c_void_p),
+                ("orbital_energies", ctypes.c_void_p), ("terms", ctypes.c_double * 8), ("efermi", ctypes.c_double),
+                ("residual", ctypes.c_double), ("energy_change", ctypes.c_double), ("iterations", ctypes.c_int),
+                ("status", ctypes.c_int), ("nsh", ctypes.c_int), ("nao", ctypes.c_int)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise RuntimeError(f"{LIBPATH} is not built: run `python -c 'import __graft_entry__ as g; g.build()'`")
+        L = ctypes.CDLL(LIBPATH)
+        L.axtbhip_version.restype = ctypes.c_char_p
+        L.axtbhip_context_create.restype = ctypes.c_void_p
+        L.axtbhip_context_create.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]
+        L.axtbhip_context_destroy.argtypes = [ctypes.c_void_p]
+        L.axtbhip_last_error.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]
+        L.axtbhip_basis_size.argtypes = [ctypes.POINTER(Mol), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+        L.axtbhip_gfn2_engrad.argtypes = [ctypes.c_void_p, ctypes.POINTER(Mol), ctypes.POINTER(Opts),
+                                          ctypes.POINTER(Result)]
+        L.axtbhip_gfn2_engrad_batch.argtypes = [ctypes.c_void_p, ctypes.POINTER(Mol), ctypes.c_int,
+                                                ctypes.POINTER(Opts), ctypes.POINTER(Result)]
+        _lib = L
+    return _lib
+
+
+def make_opts(accuracy=1.0, etemp=300.0, max_iter=250, damping=0.4, guess="eeq", gradient=True, precision="fp64",
+              dispersion=True):
+    return Opts(float(accuracy), float(etemp), int(max_iter), float(damping), 0 if guess == "eeq" else 1,
+                int(bool(gradient)), 0 if precision == "fp64" else 1, int(bool(dispersion)))
+
+
+class Context:
+    """A device context (one HIP stream and its buffers) of the engine."""
+
+    def __init__(self, device=0):
+        err = ctypes.create_string_buffer(512)
+        self._h = lib().axtbhip_context_create(int(device), err, 512)
+        if not self._h:
+            raise AxtbError(ERR_DEVICE, err.value.decode())
+
+    def close(self):
+        if self._h:
+            lib().axtbhip_context_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def last_error(self):
+        buf = ctypes.create_string_buffer(1024)
+        lib().axtbhip_last_error(self._h, buf, 1024)
+        return buf.value.decode()
+
+    def engrad_batch(self, molecules, opts, raise_on_error=False):
+        """molecules: list of (numbers, positions_bohr[, charge[, uhf]]). Returns a list of dicts."""
+        n = len(molecules)
+        mols = (Mol * n)()
+        res = (Result * n)()
+        keep, outs = [], []
+        for i, m in enumerate(molecules):
+            z = np.ascontiguousarray(m[0], dtype=np.int32)
+            x = np.ascontiguousarray(m[1], dtype=np.float64).reshape(-1, 3)
+            charge = float(m[2]) if len(m) > 2 else 0.0
+            uhf = int(m[3]) if len(m) > 3 and m[3] is not None else -1
+            nat = len(z)
+            mols[i] = Mol(nat, z.ctypes.data, x.ctypes.data, charge, uhf)
+            nsh, nao = ctypes.c_int(), ctypes.c_int()
+            lib().axtbhip_basis_size(ctypes.byref(mols[i]), ctypes.byref(nsh), ctypes.byref(nao))
+            o = {"gradient": np.zeros((nat, 3)), "charges": np.zeros(nat), "shell_charges": np.zeros(max(nsh.value, 1)),
+                 "dipoles": np.zeros((nat, 3)), "quadrupoles": np.zeros((nat, 6)),
+                 "orbital_energies": np.zeros(max(nao.value, 1))}
+            r = res[i]
+            for k, v in o.items():
+                setattr(r, k, v.ctypes.data)
+            keep += [z, x]
+            outs.append(o)
+        rc = lib().axtbhip_gfn2_engrad_batch(self._h, mols, n, ctypes.byref(opts), res)
+        if rc == ERR_DEVICE or (raise_on_error and rc):
+            raise AxtbError(rc, self.last_error())
+        results = []
+        for i in range(n):
+            r = res[i]
+            o = outs[i]
+            d = {"status": r.status, "energy": r.energy, "iterations": r.iterations, "efermi": r.efermi,
+                 "residual": r.residual, "energy_change": r.energy_change,
+                 "terms": dict(zip(TERMS, list(r.terms))), "nsh": r.nsh, "nao": r.nao}
+            if r.status == 0:
+                d.update(o)
+                d["shell_charges"] = o["shell_charges"][:r.nsh]
+                d["orbital_energies"] = o["orbital_energies"][:r.nao]
+                if not opts.want_gradient:
+                    d["gradient"] = None
+            results.append(d)
+        return results
+
+    def engrad(self, numbers, positions, charge=0.0, uhf=None, **kw):
+        opts = make_opts(**kw)
+        r = self.engrad_batch([(numbers, positions, charge, uhf)], opts)[0]
+        if r["status"]:
+            raise AxtbError(r["status"], self.last_error())
+        return r

@@ -1,0 +1,1038 @@
+// GFN2-xTB energy + analytic gradient on one GPU: host side of the engine and its C ABI.
+//
+// A calculation takes a batch of molecules (one molecule is a batch of one) through
+//   setup    coordination numbers, EEQ start charges, gamma, repulsion, D4 (three-body at q = 0),
+//            the S/H0/D/Q integral build and the Loewdin orthogonaliser X = S^-1/2;
+//   SCC      potentials -> Fock update -> A = X F X -> eigenpairs -> C = X U -> Fermi filling ->
+//            P = C f C^T -> Mulliken moments -> energy -> convergence test -> Broyden mixing,
+//            every kernel over the molecules still iterating (per-molecule masks);
+//   gradient W = C f eps C^T and the pair/integral-derivative kernels with both CN chain rules.
+// The stages follow XtbModel::singlepoint_state (crates/axtb/src/xtb.rs:437-861) in axtb's
+// default conventions (crates/axtb/src/conventions.rs:244-268).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/axtb_hip.h"
+#include "gfn2_tables.h"
+#include "kernels.hip"
+#include "linalg.hip"
+
+namespace axtb {
+
+namespace {
+
+const double AATOAU_2022 = 1.0 / (5.29177210544e-11 * 1e10);  // base/src/units.rs:18-20
+const double KB_HARTREE = 3.1668115634564e-6;                   // base/src/units.rs:24
+const double PI = 3.14159265358979323846;
+
+struct Fail {
+  int code;
+  std::string msg;
+};
+
+// STO-nG contraction with tblite's primitive normalisation (basis.rs:20-81, math.rs:108-111)
+void sto_ng(int ng, int n, int l, double zeta, double* alpha, double* coeff) {
+  int ityp = (l == 0 ? n : 4 + n) - 1;
+  double z2 = zeta * zeta;
+  for (int i = 0; i < ng; i++) {
+    double a, c;
+    switch (ng) {
+      case 1: a = STO_P_ALPHA1[ityp]; c = 1.0; break;
+      case 2: a = STO_P_ALPHA2[ityp][i]; c = STO_P_COEFF2[ityp][i]; break;
+      case 3: a = STO_P_ALPHA3[ityp][i]; c = STO_P_COEFF3[ityp][i]; break;
+      case 4: a = STO_P_ALPHA4[ityp][i]; c = STO_P_COEFF4[ityp][i]; break;
+      case 5: a = STO_P_ALPHA5[ityp][i]; c = STO_P_COEFF5[ityp][i]; break;
+      default: a = STO_P_ALPHA6[ityp][i]; c = STO_P_COEFF6[ityp][i]; break;
+    }
+    alpha[i] = a * z2;
+    double top = 2.0 / PI;
+    coeff[i] = c * (std::pow(top * alpha[i], 0.75) * std::pow(std::sqrt(4.0 * alpha[i]), l));
+  }
+}
+
+// Whether the GFN2 basis of element z has only s and p shells (what the integral kernels take).
+bool sp_element(int z) {
+  if (z < 1 || z > AXTB_GFN2_MAX_ELEM) return false;
+  for (int s = 0; s < GFN2_NSHELL[z - 1]; s++)
+    if (GFN2_ANG_SHELL[z - 1][s] > 1) return false;
+  return true;
+}
+
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  void ensure(size_t m) {
+    if (m <= n) return;
+    if (p) HIP_OK(hipFree(p));
+    p = nullptr;
+    HIP_OK(hipMalloc(&p, std::max<size_t>(m, 1) * sizeof(T)));
+    n = std::max<size_t>(m, 1);
+  }
+  void upload(const std::vector<T>& v, hipStream_t s) {
+    ensure(v.size());
+    if (!v.empty()) HIP_OK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+  ~DBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+inline unsigned blocks(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+}  // namespace
+
+struct MolIn {
+  int nat;
+  const int* z;
+  const double* xyz;
+  double charge;
+  int uhf;
+};
+
+struct MolOut {
+  int status = 0;
+  std::string msg;
+  double energy = 0, efermi = 0, residual = 0, energy_change = 0;
+  double terms[8] = {0};
+  int iterations = 0, nsh = 0, nao = 0;
+  std::vector<double> grad, qat, qsh, dp, qp, emo;
+};
+
+struct Engine {
+  int device = 0;
+  hipStream_t st = nullptr;
+  std::string last_error;
+  // host mirrors of the batch tables
+  std::vector<Mol> mols;
+  // device tables
+  DBuf<Mol> d_mol;
+  DBuf<int> i_z, i_sp, i_mol, i_ish, i_nsh, i_d4nref, i_d4ngw, s_at, s_l, s_np, s_iao, ao_sh, ao_at;
+  DBuf<double> a_xyz, a_rcov, a_en, a_rad, a_ralpha, a_rzeff, a_dk, a_qk, a_arad, a_avcn, a_chi, a_eta, a_kcn,
+      a_erad, a_reeq, a_d4rcov, a_d4zeff, a_d4gi, a_d4r4r2, a_d4refcn, a_d4refq;
+  DBuf<double> s_alpha, s_coeff, s_se0, s_kcn, s_shpoly, s_refocc, s_hard, s_gam3, s_minal, s_zeta, c6ref;
+  // work
+  DBuf<double> cn, cneeq, cnd4, se, dsedcn, mrad, dmrdcn, gam, pool, X, F, T, Am, Vw, Z, C, P, Wm, w, focc, fwork,
+      fst, eeqA, eeqV, eeqZ, eeqw, eeqr, eeqy, qstart, gw, dgw, atmP, e3, g3, dcn3, erep, grep;
+  DBuf<double> qin, qout, dq, qlast, dqlast, hdf, hu, hs, qsh, qat, dp, qp, qsh_o, qat_o, dp_o, qp_o, vsh, vat, vdp,
+      vqp, gq, cols, eband, eel, eterm, mst, grad, dedcn, dedcn4;
+  DBuf<int> active, status, kocc, itc, jstat, eeqfail;
+  DBuf<long long> eq0;
+  DBuf<GemmJob> gj;
+  DBuf<EigJob> ej;
+
+  explicit Engine(int dev) : device(dev) {
+    HIP_OK(hipSetDevice(dev));
+    HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  }
+  ~Engine() {
+    if (st) (void)hipStreamDestroy(st);
+  }
+
+  void gemm(std::vector<GemmJob>& jobs) {
+    if (jobs.empty()) return;
+    int mm = 0, nn = 0;
+    for (auto& j : jobs) {
+      mm = std::max(mm, j.m);
+      nn = std::max(nn, j.n);
+    }
+    gj.upload(jobs, st);
+    dim3 grid(blocks(nn, GT), blocks(mm, GT), (unsigned)jobs.size());
+    hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, st, gj.p);
+    HIP_OK(hipGetLastError());
+  }
+
+  // eigenpairs of every job's symmetric matrix (Jacobi, one workgroup per matrix)
+  void eigh(std::vector<EigJob>& jobs, int max_sweeps = 60) {
+    if (jobs.empty()) return;
+    int nmax = 0;
+    for (auto& j : jobs) nmax = std::max(nmax, j.n);
+    ej.upload(jobs, st);
+    jstat.ensure(jobs.size());
+    bool lds = (size_t)nmax * nmax * 8 <= 96 * 1024;
+    size_t shm = jacobi_lds(nmax, lds);
+    if (lds) {
+      HIP_OK(hipFuncSetAttribute((const void*)k_jacobi<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+      hipLaunchKernelGGL(k_jacobi<true>, dim3(jobs.size()), dim3(256), shm, st, ej.p, max_sweeps, jstat.p);
+    } else {
+      HIP_OK(hipFuncSetAttribute((const void*)k_jacobi<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+      hipLaunchKernelGGL(k_jacobi<false>, dim3(jobs.size()), dim3(256), shm, st, ej.p, max_sweeps, jstat.p);
+    }
+    HIP_OK(hipGetLastError());
+  }
+
+  void fill(double* p, long long n, double v = 0.0) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill, dim3(std::min<long long>(blocks(n, 256), 65535)), dim3(256), 0, st, p, n, v);
+  }
+
+  // ------------------------------------------------------------------------------ one batch
+  void run(const std::vector<MolIn>& in, const axtb_opts& o, std::vector<MolOut>& out) {
+    HIP_OK(hipSetDevice(device));
+    out.assign(in.size(), MolOut());
+    // validation (base/src/molecule.rs:35-64, xtb.rs:447,1159-1212)
+    if (!(o.accuracy > 0.0 && std::isfinite(o.accuracy))) throw Fail{AXTB_ERR_INPUT, "accuracy must be positive"};
+    if (!(o.etemp >= 0.0 && std::isfinite(o.etemp))) throw Fail{AXTB_ERR_INPUT, "electronic temperature must be >= 0 K"};
+    if (!(o.damping > 0.0 && o.damping <= 1.0)) throw Fail{AXTB_ERR_INPUT, "damping must be in (0, 1]"};
+    if (o.max_iter < 1) throw Fail{AXTB_ERR_INPUT, "max_iter must be >= 1"};
+    std::vector<int> use;  // input index of each batch molecule
+    for (size_t i = 0; i < in.size(); i++) {
+      const MolIn& m = in[i];
+      MolOut& r = out[i];
+      if (m.nat < 1) {
+        r.status = AXTB_ERR_INPUT;
+        r.msg = "no atoms";
+        continue;
+      }
+      bool ok = true;
+      for (int a = 0; a < m.nat && ok; a++) {
+        if (!sp_element(m.z[a])) {
+          r.status = AXTB_ERR_UNSUPPORTED_ELEMENT;
+          r.msg = "element " + std::to_string(m.z[a]) + " is not supported by the GPU engine";
+          ok = false;
+        }
+        for (int k = 0; k < 3 && ok; k++)
+          if (!std::isfinite(m.xyz[3 * a + k])) {
+            r.status = AXTB_ERR_INPUT;
+            r.msg = "non-finite coordinate";
+            ok = false;
+          }
+      }
+      for (int a = 0; a < m.nat && ok; a++)
+        for (int b = 0; b < a && ok; b++) {
+          double d2 = 0;
+          for (int k = 0; k < 3; k++) d2 += std::pow(m.xyz[3 * a + k] - m.xyz[3 * b + k], 2);
+          if (!(d2 >= 1e-6)) {
+            r.status = AXTB_ERR_INPUT;
+            r.msg = "atoms " + std::to_string(b) + " and " + std::to_string(a) + " coincide";
+            ok = false;
+          }
+        }
+      if (!ok) continue;
+      if (std::fmod(m.charge, 1.0) != 0.0 || !std::isfinite(m.charge)) {
+        r.status = AXTB_ERR_INPUT;
+        r.msg = "the charge must be an integer";
+        continue;
+      }
+      use.push_back((int)i);
+    }
+    if (use.empty()) return;
+    build(in, use, o, out);
+    if (mols.empty()) return;
+    setup(o);
+    scf(o);
+    if (o.want_gradient) gradient();
+    collect(in, use, o, out);
+  }
+
+  // ------------------------------------------------------------------------------ tables
+  std::vector<int> hz, hsp, hmol, hish, hnsh, hnref, hngw, hsat, hsl, hsnp, hsiao, haosh, haoat;
+  std::vector<double> hxyz, hrcov, hen, hrad, hra, hrz, hdk, hqk, har, hav, hchi, heta, hkcn, herad, hreeq, hd4rcov,
+      hd4zeff, hd4gi, hd4r4r2, hd4refcn, hd4refq, hsal, hsco, hse0, hskcn, hsshp, hsref, hshard, hsgam3, hsmin, hszeta;
+  std::vector<int> map_use;  // batch molecule -> input index
+  int cap = 64;
+  long long tot_mat = 0, tot_pair = 0, tot_shp = 0, tot_eq = 0, tot_hist = 0;
+  int tot_vec = 0;
+
+  void build(const std::vector<MolIn>& in, const std::vector<int>& use, const axtb_opts& o,
+             std::vector<MolOut>& out) {
+    for (auto* v : {&hz, &hsp, &hmol, &hish, &hnsh, &hnref, &hngw, &hsat, &hsl, &hsnp, &hsiao, &haosh, &haoat}) v->clear();
+    for (auto* v : {&hxyz, &hrcov, &hen, &hrad, &hra, &hrz, &hdk, &hqk, &har, &hav, &hchi, &heta, &hkcn, &herad, &hreeq,
+                    &hd4rcov, &hd4zeff, &hd4gi, &hd4r4r2, &hd4refcn, &hd4refq, &hsal, &hsco, &hse0, &hskcn, &hsshp,
+                    &hsref, &hshard, &hsgam3, &hsmin, &hszeta})
+      v->clear();
+    mols.clear();
+    map_use.clear();
+    cap = std::max(1, std::min(o.max_iter, 64));
+    std::map<int, int> species;
+    tot_mat = tot_pair = tot_shp = tot_eq = tot_hist = 0;
+    tot_vec = 0;
+    int at0 = 0, sh0 = 0, ao0 = 0;
+    for (int ii : use) {
+      const MolIn& m = in[ii];
+      Mol M{};
+      M.nat = m.nat;
+      M.at0 = at0;
+      M.sh0 = sh0;
+      M.ao0 = ao0;
+      M.charge = m.charge;
+      int nsh = 0, nao = 0;
+      double nref = 0.0;
+      for (int a = 0; a < m.nat; a++) {
+        int z = m.z[a], iz = z - 1;
+        if (!species.count(z)) {
+          int s = (int)species.size();
+          species[z] = s;
+        }
+        hz.push_back(z);
+        hsp.push_back(species[z]);
+        hmol.push_back((int)mols.size());
+        hish.push_back(sh0 + nsh);
+        hnsh.push_back(GFN2_NSHELL[iz]);
+        for (int k = 0; k < 3; k++) hxyz.push_back(m.xyz[3 * a + k]);
+        hrcov.push_back(4.0 / 3.0 * (EL_COVALENT_RAD_2009_AA[iz] * AATOAU_2022));
+        hen.push_back(EL_PAULING_EN[iz]);
+        hrad.push_back(EL_ATOMIC_RAD_AA[iz] * AATOAU_2022);
+        hra.push_back(GFN2_REP_ALPHA[iz]);
+        hrz.push_back(GFN2_REP_ZEFF[iz]);
+        hdk.push_back(GFN2_P_DKERNEL[iz]);
+        hqk.push_back(GFN2_P_QKERNEL[iz]);
+        har.push_back(GFN2_P_RAD[iz]);
+        hav.push_back(GFN2_P_VCN[iz]);
+        hchi.push_back(EEQ_CHI[iz]);
+        heta.push_back(EEQ_ETA[iz]);
+        hkcn.push_back(EEQ_KCNCHI[iz]);
+        herad.push_back(EEQ_RAD[iz]);
+        hreeq.push_back(4.0 / 3.0 * (AXTB_MCTC_AATOAU * EL_COVALENT_RAD_2009_AA[iz]));
+        // D4 (d4/src/model.rs:60-71, 260-340), GFN2 reference charges, axtb CN radii
+        hd4rcov.push_back(4.0 / 3.0 * (EL_COVALENT_RAD_2009_AA[iz] * AATOAU_2022));
+        hd4zeff.push_back(D4_EFFECTIVE_NUCLEAR_CHARGE[iz]);
+        hd4gi.push_back(D4_CHEMICAL_HARDNESS[iz] * 2.0);
+        hd4r4r2.push_back(std::sqrt(0.5 * (D4_R4_OVER_R2[iz] * std::sqrt((double)z))));
+        int nr = D4_REFN[iz];
+        hnref.push_back(nr);
+        int cnc[20] = {0};
+        cnc[0] = 1;
+        auto icn = [](double x) { return (int)std::min<long>(19, std::max<long>(0, std::lround(x))); };
+        for (int r = 0; r < nr; r++) cnc[icn(D4_REFCN[iz][r])]++;
+        for (int r = 0; r < MAXREF; r++) {
+          int c = r < nr ? cnc[icn(D4_REFCN[iz][r])] : 0;
+          hngw.push_back(r < nr ? c * (c + 1) / 2 : 1);
+          hd4refcn.push_back(r < nr ? D4_REFCOVCN[iz][r] : 0.0);
+          hd4refq.push_back(r < nr ? D4_REFQ_GFN2[iz][r] : 0.0);
+        }
+        for (int s = 0; s < GFN2_NSHELL[iz]; s++) {
+          int l = GFN2_ANG_SHELL[iz][s];
+          int np = GFN2_NUMBER_OF_PRIMITIVES[iz][s];
+          double al[MAXPRIM] = {0}, co[MAXPRIM] = {0};
+          sto_ng(np, GFN2_PRINCIPAL_QUANTUM_NUMBER[iz][s], l, GFN2_SLATER_EXPONENT[iz][s], al, co);
+          double amin = 1e300;
+          for (int k = 0; k < np; k++) amin = std::min(amin, al[k]);
+          hsat.push_back(at0 + a);
+          hsl.push_back(l);
+          hsnp.push_back(np);
+          hsiao.push_back(nao);
+          for (int k = 0; k < MAXPRIM; k++) {
+            hsal.push_back(al[k]);
+            hsco.push_back(co[k]);
+          }
+          hse0.push_back(GFN2_P_SELFENERGY[iz][s]);
+          hskcn.push_back(GFN2_P_KCN[iz][l]);
+          hsshp.push_back(GFN2_P_SHPOLY[iz][l]);
+          hsref.push_back(GFN2_REFERENCE_OCC[iz][l]);
+          nref += GFN2_REFERENCE_OCC[iz][l];
+          hshard.push_back(GFN2_HUBBARD_PARAMETER[iz] * GFN2_SHELL_HUBBARD[iz][l]);
+          const double SHD[3] = {1.0, 0.5, 0.25};
+          hsgam3.push_back(GFN2_P_HUBBARD_DERIVS[iz] * SHD[l]);
+          hsmin.push_back(amin);
+          hszeta.push_back(GFN2_SLATER_EXPONENT[iz][s]);
+          for (int k = 0; k < 2 * l + 1; k++) {
+            haosh.push_back(sh0 + nsh);
+            haoat.push_back(at0 + a);
+          }
+          nao += 2 * l + 1;
+          nsh++;
+        }
+      }
+      // electrons (xtb.rs:1178-1212)
+      double nocc = nref - m.charge;
+      long nel_int = std::lround(nocc);
+      double nuhf = m.uhf >= 0 ? (double)m.uhf : (double)(((nel_int % 2) + 2) % 2);
+      MolOut& r = out[ii];
+      if (m.uhf >= 0 && (m.uhf % 2) != (int)(((nel_int % 2) + 2) % 2)) {
+        r.status = AXTB_ERR_INPUT;
+        r.msg = "electron count and unpaired electrons are not compatible";
+      } else if (nocc < 0.0) {
+        r.status = AXTB_ERR_INPUT;
+        r.msg = "the charge leaves a negative electron count";
+      } else if (nuhf > nocc) {
+        r.status = AXTB_ERR_INPUT;
+        r.msg = "more unpaired electrons than electrons";
+      } else if ((nocc + nuhf) / 2.0 > nao + 1e-9) {
+        r.status = AXTB_ERR_INPUT;
+        r.msg = "more electrons than the basis holds";
+      }
+      if (r.status) {
+        // drop the molecule again
+        size_t na = hz.size() - m.nat, ns = hsat.size() - nsh, no = haosh.size() - nao;
+        hz.resize(na); hsp.resize(na); hmol.resize(na); hish.resize(na); hnsh.resize(na); hnref.resize(na);
+        hngw.resize(na * MAXREF); hd4refcn.resize(na * MAXREF); hd4refq.resize(na * MAXREF);
+        for (auto* v : {&hrcov, &hen, &hrad, &hra, &hrz, &hdk, &hqk, &har, &hav, &hchi, &heta, &hkcn, &herad, &hreeq,
+                        &hd4rcov, &hd4zeff, &hd4gi, &hd4r4r2})
+          v->resize(na);
+        hxyz.resize(3 * na);
+        hsat.resize(ns); hsl.resize(ns); hsnp.resize(ns); hsiao.resize(ns);
+        hsal.resize(ns * MAXPRIM); hsco.resize(ns * MAXPRIM);
+        for (auto* v : {&hse0, &hskcn, &hsshp, &hsref, &hshard, &hsgam3, &hsmin, &hszeta}) v->resize(ns);
+        haosh.resize(no);
+        haoat.resize(no);
+        continue;
+      }
+      M.nsh = nsh;
+      M.nao = nao;
+      M.ndim = nsh + 9 * m.nat;
+      M.nalpha = (nocc + nuhf) / 2.0;
+      M.nbeta = (nocc - nuhf) / 2.0;
+      M.kt = o.etemp > 0.0 ? o.etemp * KB_HARTREE : 0.0;
+      M.vec0 = tot_vec;
+      M.mat0 = tot_mat;
+      M.pair0 = tot_pair;
+      M.shp0 = tot_shp;
+      M.hist0 = tot_hist;
+      tot_vec += M.ndim;
+      tot_mat += (long long)nao * nao;
+      tot_pair += (long long)m.nat * m.nat;
+      tot_shp += (long long)nsh * nsh;
+      tot_eq += (long long)(m.nat + 1) * (m.nat + 1);
+      tot_hist += (long long)cap * M.ndim;
+      mols.push_back(M);
+      map_use.push_back(ii);
+      at0 += m.nat;
+      sh0 += nsh;
+      ao0 += nao;
+      out[ii].nsh = nsh;
+      out[ii].nao = nao;
+    }
+    // reference C6 of every species pair (d4/src/model.rs:197-258)
+    int nsp = (int)species.size();
+    std::vector<double> hc6((size_t)nsp * nsp * 49, 0.0);
+    std::vector<std::vector<std::vector<double>>> aiw(nsp);
+    for (auto& kv : species) {
+      int z = kv.first, iz = z - 1, s = kv.second;
+      aiw[s].assign(MAXREF, std::vector<double>(NFREQ, 0.0));
+      for (int r = 0; r < D4_REFN[iz]; r++) {
+        int is = D4_REFSYS[iz][r];
+        if (is == 0) continue;
+        double izs = D4_EFFECTIVE_NUCLEAR_CHARGE[is - 1];
+        double gam = D4_CHEMICAL_HARDNESS[is - 1] * 2.0;
+        double qmod = D4_REFH_GFN2[iz][r] + izs;
+        double zv = qmod < 0.0 ? std::exp(3.0) : std::exp(3.0 * (1.0 - std::exp(gam * (1.0 - izs / qmod))));
+        for (int k = 0; k < NFREQ; k++) {
+          double a = D4_SSCALE[is - 1] * D4_SECAIW[is - 1][k] * zv;
+          double v = D4_ASCALE[iz][r] * (D4_ALPHAIW[iz][r][k] - D4_HCOUNT[iz][r] * a);
+          aiw[s][r][k] = v > 0.0 ? v : 0.0;
+        }
+      }
+    }
+    const double FREQ[NFREQ] = {0.000001, 0.050000, 0.100000, 0.200000, 0.300000, 0.400000, 0.500000, 0.600000,
+                                0.700000, 0.800000, 0.900000, 1.000000, 1.200000, 1.400000, 1.600000, 1.800000,
+                                2.000000, 2.500000, 3.000000, 4.000000, 5.000000, 7.500000, 10.00000};
+    double tw[NFREQ];
+    tw[0] = 0.5 * (FREQ[1] - FREQ[0]);
+    for (int k = 1; k < NFREQ - 1; k++) tw[k] = 0.5 * ((FREQ[k] - FREQ[k - 1]) + (FREQ[k + 1] - FREQ[k]));
+    tw[NFREQ - 1] = 0.5 * (FREQ[NFREQ - 1] - FREQ[NFREQ - 2]);
+    for (auto& ki : species)
+      for (auto& kj : species) {
+        int si = ki.second, sj = kj.second;
+        for (int ri = 0; ri < D4_REFN[ki.first - 1]; ri++)
+          for (int rj = 0; rj < D4_REFN[kj.first - 1]; rj++) {
+            double s = 0.0;
+            for (int k = 0; k < NFREQ; k++) s += aiw[si][ri][k] * aiw[sj][rj][k] * tw[k];
+            hc6[((size_t)si * nsp + sj) * 49 + ri * 7 + rj] = 3.0 / PI * s;
+          }
+      }
+    nsp_ = nsp;
+    c6ref.upload(hc6, st);
+    if (mols.empty()) return;
+    d_mol.upload(mols, st);
+    i_z.upload(hz, st); i_sp.upload(hsp, st); i_mol.upload(hmol, st); i_ish.upload(hish, st); i_nsh.upload(hnsh, st);
+    i_d4nref.upload(hnref, st); i_d4ngw.upload(hngw, st);
+    s_at.upload(hsat, st); s_l.upload(hsl, st); s_np.upload(hsnp, st); s_iao.upload(hsiao, st);
+    ao_sh.upload(haosh, st); ao_at.upload(haoat, st);
+    a_xyz.upload(hxyz, st); a_rcov.upload(hrcov, st); a_en.upload(hen, st); a_rad.upload(hrad, st);
+    a_ralpha.upload(hra, st); a_rzeff.upload(hrz, st); a_dk.upload(hdk, st); a_qk.upload(hqk, st);
+    a_arad.upload(har, st); a_avcn.upload(hav, st); a_chi.upload(hchi, st); a_eta.upload(heta, st);
+    a_kcn.upload(hkcn, st); a_erad.upload(herad, st); a_reeq.upload(hreeq, st); a_d4rcov.upload(hd4rcov, st);
+    a_d4zeff.upload(hd4zeff, st); a_d4gi.upload(hd4gi, st); a_d4r4r2.upload(hd4r4r2, st);
+    a_d4refcn.upload(hd4refcn, st); a_d4refq.upload(hd4refq, st);
+    s_alpha.upload(hsal, st); s_coeff.upload(hsco, st); s_se0.upload(hse0, st); s_kcn.upload(hskcn, st);
+    s_shpoly.upload(hsshp, st); s_refocc.upload(hsref, st); s_hard.upload(hshard, st); s_gam3.upload(hsgam3, st);
+    s_minal.upload(hsmin, st); s_zeta.upload(hszeta, st);
+    intcut_ = std::max(25.0 - 10.0 * std::log10(o.accuracy), 20.0);  // basis.rs:135-137
+  }
+  int nsp_ = 0;
+  double intcut_ = 25.0;
+
+  Batch batch() const {
+    Batch b{};
+    b.nmol = (int)mols.size();
+    b.nat = (int)hz.size();
+    b.nsh = (int)hsat.size();
+    b.nao = (int)haosh.size();
+    b.mol = d_mol.p;
+    b.a = AtomP{i_z.p, i_sp.p, i_mol.p, i_ish.p, i_nsh.p, a_xyz.p, a_rcov.p, a_en.p, a_rad.p, a_ralpha.p,
+                a_rzeff.p, a_dk.p, a_qk.p, a_arad.p, a_avcn.p, a_chi.p, a_eta.p, a_kcn.p, a_erad.p, a_reeq.p,
+                a_d4rcov.p, a_d4zeff.p, a_d4gi.p, a_d4r4r2.p, a_d4refcn.p, a_d4refq.p, i_d4nref.p, i_d4ngw.p};
+    b.s = ShellP{s_at.p, s_l.p, s_np.p, s_iao.p, s_alpha.p, s_coeff.p, s_se0.p, s_kcn.p, s_shpoly.p,
+                 s_refocc.p, s_hard.p, s_gam3.p, s_minal.p, s_zeta.p};
+    b.ao_sh = ao_sh.p;
+    b.ao_at = ao_at.p;
+    b.c6ref = c6ref.p;
+    b.nsp = nsp_;
+    b.intcut = intcut_;
+    return b;
+  }
+
+  int nmol() const { return (int)mols.size(); }
+  int nat() const { return (int)hz.size(); }
+  int nshell() const { return (int)hsat.size(); }
+  int naos() const { return (int)haosh.size(); }
+
+  // ------------------------------------------------------------------------------ setup
+  void setup(const axtb_opts& o) {
+    Batch b = batch();
+    const int NA = nat(), NS = nshell(), NO = naos(), NM = nmol();
+    int nshmax = 0, naomax = 0, natmax = 0;
+    for (auto& M : mols) {
+      nshmax = std::max(nshmax, M.nsh);
+      naomax = std::max(naomax, M.nao);
+      natmax = std::max(natmax, M.nat);
+    }
+    for (auto* d : {&cn, &cneeq, &cnd4, &mrad, &dmrdcn, &qat, &qat_o, &vat, &eband, &eel, &e3, &dcn3, &erep, &dedcn,
+                    &dedcn4})
+      d->ensure(NA);
+    for (auto* d : {&dp, &dp_o, &vdp, &g3, &grep, &grad}) d->ensure(3 * (size_t)NA);
+    for (auto* d : {&qp, &qp_o, &vqp}) d->ensure(6 * (size_t)NA);
+    eterm.ensure(8 * (size_t)NA);
+    gw.ensure(MAXREF * (size_t)NA);
+    dgw.ensure(MAXREF * (size_t)NA);
+    for (auto* d : {&se, &dsedcn, &qstart, &qsh, &qsh_o, &vsh, &gq}) d->ensure(NS);
+    for (auto* d : {&w, &focc}) d->ensure(NO);
+    fwork.ensure(2 * (size_t)NO);
+    cols.ensure(11 * (size_t)NO);
+    gam.ensure(tot_shp);
+    pool.ensure(11 * (size_t)tot_mat);
+    for (auto* d : {&X, &F, &T, &Am, &Vw, &Z, &C, &P, &Wm}) d->ensure(tot_mat);
+    atmP.ensure(8 * (size_t)tot_pair);
+    for (auto* d : {&eeqA, &eeqV, &eeqZ}) d->ensure(tot_eq);
+    for (auto* d : {&eeqw, &eeqr, &eeqy}) d->ensure(NA + NM);
+    for (auto* d : {&qin, &qout, &dq, &qlast, &dqlast}) d->ensure(tot_vec);
+    hdf.ensure(tot_hist);
+    hu.ensure(tot_hist);
+    hs.ensure((size_t)NM * (cap + cap * cap));
+    fst.ensure(2 * (size_t)NM);
+    mst.ensure(8 * (size_t)NM);
+    for (auto* d : {&active, &status, &kocc, &itc, &eeqfail}) d->ensure(NM);
+    eq0.ensure(NM);
+    std::vector<long long> heq0(NM);
+    long long off = 0;
+    for (int m = 0; m < NM; m++) {
+      heq0[m] = off;
+      off += (long long)(mols[m].nat + 1) * (mols[m].nat + 1);
+    }
+    eq0.upload(heq0, st);
+    std::vector<int> ones(NM, 1), zeros(NM, 0);
+    active.upload(ones, st);
+    status.upload(zeros, st);
+    itc.upload(zeros, st);
+    eeqfail.upload(zeros, st);
+    fill(mst.p, 8LL * NM);
+
+    // coordination numbers, levels, AES radii
+    hipLaunchKernelGGL(k_atom_setup, dim3(blocks(NA, 4)), dim3(256), 0, st, b, (const int*)nullptr, cn.p, cneeq.p,
+                       cnd4.p, se.p, dsedcn.p, mrad.p, dmrdcn.p);
+    hipLaunchKernelGGL(k_gamma, dim3(blocks((long long)nshmax * nshmax, 256), NM), dim3(256), 0, st, b, gam.p);
+    // integrals
+    fill(pool.p, 11 * tot_mat);
+    hipLaunchKernelGGL(k_integrals, dim3(blocks((long long)nshmax * nshmax, 128), NM), dim3(128), 0, st, b, se.p,
+                       pool.p, (long long)tot_mat);
+    // repulsion
+    hipLaunchKernelGGL(k_repulsion, dim3(blocks(NA, 4)), dim3(256), 0, st, b, (const int*)nullptr, erep.p, grep.p);
+    // D4 three-body at q = 0 (weights with dgw/dCN)
+    fill(fwork.p, 0);
+    DBuf<double>& zq = eeqy;  // zero charges
+    fill(zq.p, NA);
+    hipLaunchKernelGGL(k_d4_weights, dim3(blocks(NA, 128)), dim3(128), 0, st, b, (const int*)nullptr, cnd4.p, zq.p,
+                       gw.p, dgw.p, 1);
+    hipLaunchKernelGGL(k_atm_pairs, dim3(blocks((long long)natmax * natmax, 256), NM), dim3(256), 0, st, b, gw.p,
+                       dgw.p, atmP.p);
+    if (o.dispersion) {
+      hipLaunchKernelGGL(k_atm, dim3(blocks(NA, 4)), dim3(256), 0, st, b, atmP.p, e3.p, g3.p, dcn3.p);
+    } else {
+      fill(e3.p, NA);
+      fill(g3.p, 3LL * NA);
+      fill(dcn3.p, NA);
+    }
+    // start: EEQ charges (or neutral atoms)
+    fill(qstart.p, NS);
+    if (o.guess == 0) {
+      hipLaunchKernelGGL(k_eeq_system, dim3(blocks((long long)(natmax + 1) * (natmax + 1), 256), NM), dim3(256), 0, st,
+                         b, eq0.p, cneeq.p, eeqA.p, eeqr.p);
+      std::vector<EigJob> jobs(NM);
+      for (int m = 0; m < NM; m++) {
+        int n1 = mols[m].nat + 1;
+        jobs[m] = EigJob{n1, eeqA.p + heq0[m], eeqV.p + heq0[m], eeqw.p + mols[m].at0 + m, eeqZ.p + heq0[m]};
+      }
+      eigh(jobs);
+      hipLaunchKernelGGL(k_eeq_solve, dim3(NM), dim3(256), 0, st, b, eq0.p, eeqZ.p, eeqw.p, eeqr.p, eeqy.p, qstart.p,
+                         eeqfail.p);
+    }
+    // Loewdin orthogonaliser X = U s^-1/2 U^T from the eigenpairs of S
+    {
+      std::vector<EigJob> jobs(NM);
+      for (int m = 0; m < NM; m++) {
+        long long o0 = mols[m].mat0;
+        HIP_OK(hipMemcpyAsync(Am.p + o0, pool.p + o0, sizeof(double) * mols[m].nao * mols[m].nao,
+                              hipMemcpyDeviceToDevice, st));
+        jobs[m] = EigJob{mols[m].nao, Am.p + o0, Vw.p + o0, w.p + mols[m].ao0, Z.p + o0};
+      }
+      eigh(jobs);
+      std::vector<double> hw(NO);
+      HIP_OK(hipMemcpyAsync(hw.data(), w.p, sizeof(double) * NO, hipMemcpyDeviceToHost, st));
+      HIP_OK(hipStreamSynchronize(st));
+      std::vector<double> hs12(NO);
+      for (int m = 0; m < NM; m++)
+        for (int i = 0; i < mols[m].nao; i++) {
+          double s = hw[mols[m].ao0 + i];
+          if (!(s > 0.0)) throw Fail{AXTB_ERR_NUMERICAL, "overlap matrix is not positive definite"};
+          hs12[mols[m].ao0 + i] = 1.0 / std::sqrt(s);
+        }
+      s12_.upload(hs12, st);
+      std::vector<GemmJob> gjobs(NM);
+      for (int m = 0; m < NM; m++) {
+        long long o0 = mols[m].mat0;
+        int n = mols[m].nao;
+        gjobs[m] = GemmJob{n, n, n, 0, 1, Z.p + o0, Z.p + o0, X.p + o0, s12_.p + mols[m].ao0, n, n, n, 1.0, 0.0};
+      }
+      gemm(gjobs);
+    }
+  }
+  DBuf<double> s12_;
+
+  // ------------------------------------------------------------------------------ SCC loop
+  std::vector<int> hactive;
+  int iters_ = 0;
+
+  void scf(const axtb_opts& o) {
+    Batch b = batch();
+    const int NA = nat(), NS = nshell(), NO = naos(), NM = nmol();
+    int naomax = 0;
+    for (auto& M : mols) naomax = std::max(naomax, M.nao);
+    const long long stride = tot_mat;
+    // start moments -> q_in (dipoles and quadrupoles zero)
+    fill(dp.p, 3LL * NA);
+    fill(qp.p, 6LL * NA);
+    hipLaunchKernelGGL(k_pack, dim3(blocks(NA, 128)), dim3(128), 0, st, b, (const int*)nullptr, qstart.p, dp.p, qp.p,
+                       qin.p, (const double*)nullptr, (double*)nullptr);
+    double econv = 1e-6 * o.accuracy, pconv = 2e-5 * o.accuracy;
+    size_t bshm = (16 + cap + (size_t)cap * cap) * sizeof(double) + cap * sizeof(int);
+    hactive.assign(NM, 1);
+    std::vector<GemmJob> g1(NM), g2(NM), g3v(NM), gP(NM);
+    std::vector<EigJob> ejobs(NM);
+    for (int m = 0; m < NM; m++) {
+      const Mol& M = mols[m];
+      long long o0 = M.mat0;
+      int n = M.nao;
+      g1[m] = GemmJob{n, n, n, 0, 0, F.p + o0, X.p + o0, T.p + o0, nullptr, n, n, n, 1.0, 0.0};   // T = F X
+      g2[m] = GemmJob{n, n, n, 0, 0, X.p + o0, T.p + o0, Am.p + o0, nullptr, n, n, n, 1.0, 0.0};  // A = X T
+      g3v[m] = GemmJob{n, n, n, 0, 0, X.p + o0, Z.p + o0, C.p + o0, nullptr, n, n, n, 1.0, 0.0};  // C = X U
+      gP[m] = GemmJob{n, n, n, 0, 1, C.p + o0, C.p + o0, P.p + o0, focc.p + M.ao0, n, n, n, 1.0, 0.0};
+      ejobs[m] = EigJob{n, Am.p + o0, Vw.p + o0, w.p + M.ao0, Z.p + o0};
+    }
+    int it = 0;
+    for (it = 1; it <= o.max_iter; it++) {
+      std::vector<GemmJob> a1, a2, a3, a4;
+      std::vector<EigJob> ea;
+      for (int m = 0; m < NM; m++)
+        if (hactive[m]) {
+          a1.push_back(g1[m]);
+          a2.push_back(g2[m]);
+          a3.push_back(g3v[m]);
+          a4.push_back(gP[m]);
+          ea.push_back(ejobs[m]);
+        }
+      if (a1.empty()) break;
+      if (it > 1)
+        hipLaunchKernelGGL(k_broyden, dim3(NM), dim3(256), bshm, st, b, active.p, cap, o.damping, qin.p, dq.p,
+                           qlast.p, dqlast.p, hdf.p, hu.p, hs.p, itc.p, status.p);
+      hipLaunchKernelGGL(k_unpack, dim3(blocks(NA, 128)), dim3(128), 0, st, b, active.p, qin.p, qsh.p, qat.p, dp.p,
+                         qp.p);
+      // potentials of the input moments (xtb.rs:316-340)
+      hipLaunchKernelGGL(k_shell_pot, dim3(blocks(NS, 4)), dim3(256), 0, st, b, active.p, gam.p, qsh.p, vsh.p, 1);
+      hipLaunchKernelGGL(k_d4_weights, dim3(blocks(NA, 128)), dim3(128), 0, st, b, active.p, cnd4.p, qat.p, gw.p,
+                         dgw.p, 0);
+      if (!o.dispersion) fill(gw.p, MAXREF * (long long)NA);
+      hipLaunchKernelGGL(k_atom_pot, dim3(blocks(NA, 4)), dim3(256), 0, st, b, active.p, mrad.p, qat.p, dp.p, qp.p,
+                         gw.p, dgw.p, vat.p, vdp.p, vqp.p);
+      hipLaunchKernelGGL(k_fock, dim3(blocks((long long)naomax * naomax, 256), NM), dim3(256), 0, st, b, active.p,
+                         pool.p, stride, vsh.p, vat.p, vdp.p, vqp.p, F.p);
+      // generalized eigenproblem in the Loewdin basis
+      gemm(a1);
+      gemm(a2);
+      eigh(ea);
+      gemm(a3);
+      hipLaunchKernelGGL(k_fermi, dim3(NM), dim3(256), 0, st, b, active.p, w.p, focc.p, fwork.p, fst.p, kocc.p,
+                         status.p);
+      gemm(a4);  // P = C diag(f) C^T
+      hipLaunchKernelGGL(k_mulliken_cols, dim3(blocks(NO, 4)), dim3(256), 0, st, b, active.p, pool.p, stride, P.p,
+                         cols.p);
+      hipLaunchKernelGGL(k_mulliken_atoms, dim3(blocks(NA, 128)), dim3(128), 0, st, b, active.p, cols.p, qsh_o.p,
+                         qat_o.p, dp_o.p, qp_o.p, eband.p);
+      hipLaunchKernelGGL(k_pack, dim3(blocks(NA, 128)), dim3(128), 0, st, b, active.p, qsh_o.p, dp_o.p, qp_o.p,
+                         qout.p, (const double*)qin.p, dq.p);
+      // energy of the output moments
+      hipLaunchKernelGGL(k_shell_pot, dim3(blocks(NS, 4)), dim3(256), 0, st, b, active.p, gam.p, qsh_o.p, gq.p, 0);
+      hipLaunchKernelGGL(k_d4_weights, dim3(blocks(NA, 128)), dim3(128), 0, st, b, active.p, cnd4.p, qat_o.p, gw.p,
+                         dgw.p, 0);
+      if (!o.dispersion) fill(gw.p, MAXREF * (long long)NA);
+      hipLaunchKernelGGL(k_energy_atoms, dim3(blocks(NA, 4)), dim3(256), 0, st, b, active.p, mrad.p, gq.p, qsh_o.p,
+                         qat_o.p, dp_o.p, qp_o.p, gw.p, dgw.p, eband.p, fst.p, eel.p, eterm.p);
+      hipLaunchKernelGGL(k_converge, dim3(NM), dim3(256), 0, st, b, active.p, eel.p, dq.p, fst.p, econv, pconv, mst.p,
+                         status.p);
+      HIP_OK(hipGetLastError());
+      HIP_OK(hipMemcpyAsync(hactive.data(), active.p, sizeof(int) * NM, hipMemcpyDeviceToHost, st));
+      HIP_OK(hipStreamSynchronize(st));
+      bool any = false;
+      for (int m = 0; m < NM; m++) any |= hactive[m] != 0;
+      if (!any) break;
+    }
+    iters_ = it;
+  }
+
+  // ------------------------------------------------------------------------------ gradient
+  std::vector<int> hok;
+
+  void gradient() {
+    Batch b = batch();
+    const int NA = nat(), NM = nmol();
+    const long long stride = tot_mat;
+    // molecules that converged without errors
+    std::vector<double> hm(8 * (size_t)NM);
+    std::vector<int> hst(NM);
+    HIP_OK(hipMemcpyAsync(hm.data(), mst.p, sizeof(double) * 8 * NM, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(hst.data(), status.p, sizeof(int) * NM, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    hok.assign(NM, 0);
+    std::vector<GemmJob> gW;
+    for (int m = 0; m < NM; m++) {
+      hok[m] = (hm[8 * m + 6] == 1.0 && hst[m] == 0) ? 1 : 0;
+      if (!hok[m]) continue;
+      const Mol& M = mols[m];
+      int n = M.nao;
+      gW.push_back(GemmJob{n, n, n, 0, 1, C.p + M.mat0, C.p + M.mat0, Wm.p + M.mat0, fwork.p + M.ao0, n, n, n, 1.0, 0.0});
+    }
+    DBuf<int> okd;
+    okd.upload(hok, st);
+    // focc * eps into fwork (k_fermi's work is no longer needed)
+    std::vector<double> hf(naos()), hw(naos());
+    HIP_OK(hipMemcpyAsync(hf.data(), focc.p, sizeof(double) * naos(), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(hw.data(), w.p, sizeof(double) * naos(), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    for (int i = 0; i < naos(); i++) hf[i] *= hw[i];
+    HIP_OK(hipMemcpyAsync(fwork.p, hf.data(), sizeof(double) * naos(), hipMemcpyHostToDevice, st));
+    gemm(gW);
+    // gradient = repulsion + three-body (setup) + pair terms + CN chains
+    HIP_OK(hipMemcpyAsync(grad.p, grep.p, sizeof(double) * 3 * NA, hipMemcpyDeviceToDevice, st));
+    std::vector<double> hg3(3 * NA), hgr(3 * NA);
+    HIP_OK(hipMemcpyAsync(hg3.data(), g3.p, sizeof(double) * 3 * NA, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(hgr.data(), grep.p, sizeof(double) * 3 * NA, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    for (int i = 0; i < 3 * NA; i++) hgr[i] += hg3[i];
+    HIP_OK(hipMemcpyAsync(grad.p, hgr.data(), sizeof(double) * 3 * NA, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_d4_weights, dim3(blocks(NA, 128)), dim3(128), 0, st, b, (const int*)okd.p, cnd4.p, qat_o.p,
+                       gw.p, dgw.p, 1);
+    hipLaunchKernelGGL(k_pair_grad, dim3(blocks(NA, 4)), dim3(256), 0, st, b, (const int*)okd.p, pool.p, stride, P.p,
+                       Wm.p, se.p, dsedcn.p, vsh.p, vat.p, vdp.p, vqp.p, qsh_o.p, qat_o.p, dp_o.p, qp_o.p, mrad.p,
+                       dmrdcn.p, gw.p, dgw.p, dcn3.p, grad.p, dedcn.p, dedcn4.p);
+    hipLaunchKernelGGL(k_cn_chain, dim3(blocks(NA, 4)), dim3(256), 0, st, b, (const int*)okd.p, dedcn.p, dedcn4.p,
+                       grad.p);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(st));
+  }
+
+  // ------------------------------------------------------------------------------ results
+  void collect(const std::vector<MolIn>& in, const std::vector<int>& use, const axtb_opts& o,
+               std::vector<MolOut>& out) {
+    const int NA = nat(), NS = nshell(), NO = naos(), NM = nmol();
+    auto get = [&](DBuf<double>& d, size_t n) {
+      std::vector<double> v(n);
+      if (n) HIP_OK(hipMemcpyAsync(v.data(), d.p, n * sizeof(double), hipMemcpyDeviceToHost, st));
+      return v;
+    };
+    auto m8 = get(mst, 8 * (size_t)NM);
+    auto er = get(erep, NA), e3h = get(e3, NA), eelh = get(eel, NA), et = get(eterm, 8 * (size_t)NA);
+    auto qa = get(qat_o, NA), qs = get(qsh_o, NS), d = get(dp_o, 3 * (size_t)NA), q = get(qp_o, 6 * (size_t)NA);
+    auto e = get(w, NO);
+    auto g = o.want_gradient ? get(grad, 3 * (size_t)NA) : std::vector<double>();
+    std::vector<int> hst(NM), hef(NM);
+    HIP_OK(hipMemcpyAsync(hst.data(), status.p, sizeof(int) * NM, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    for (int m = 0; m < NM; m++) {
+      const Mol& M = mols[m];
+      MolOut& r = out[map_use[m]];
+      double* s = &m8[8 * m];
+      r.iterations = (int)s[5];
+      r.residual = s[2];
+      r.energy_change = s[1];
+      r.efermi = s[3];
+      if (hst[m] & 2) {
+        r.status = AXTB_ERR_NUMERICAL;
+        r.msg = "Fermi level not found";
+      } else if (hst[m] & 4) {
+        r.status = AXTB_ERR_NUMERICAL;
+        r.msg = "non-finite energy";
+      } else if (hst[m] & 8) {
+        r.status = AXTB_ERR_NUMERICAL;
+        r.msg = "Broyden mixing failed";
+      } else if (s[6] != 1.0) {
+        r.status = AXTB_ERR_SCF_NOT_CONVERGED;
+        r.msg = "SCF not converged in " + std::to_string(r.iterations) + " iterations";
+      }
+      double etot = 0.0, erp = 0.0, ed3 = 0.0, eelt = 0.0, t[5] = {0, 0, 0, 0, 0};
+      for (int a = M.at0; a < M.at0 + M.nat; a++) {
+        etot += er[a] + e3h[a] + eelh[a];
+        erp += er[a];
+        ed3 += e3h[a];
+        eelt += eelh[a];
+        for (int k = 0; k < 5; k++) t[k] += et[8 * a + k];
+      }
+      r.energy = etot;
+      // terms: repulsion dispersion electronic isotropic_es third_order anisotropic entropy band
+      r.terms[0] = erp;
+      r.terms[1] = ed3 + t[3];
+      r.terms[2] = eelt;
+      r.terms[3] = t[0];
+      r.terms[4] = t[1];
+      r.terms[5] = t[2];
+      r.terms[6] = s[4];
+      r.terms[7] = t[4];
+      r.qat.assign(qa.begin() + M.at0, qa.begin() + M.at0 + M.nat);
+      r.qsh.assign(qs.begin() + M.sh0, qs.begin() + M.sh0 + M.nsh);
+      r.dp.assign(d.begin() + 3 * M.at0, d.begin() + 3 * (M.at0 + M.nat));
+      r.qp.assign(q.begin() + 6 * M.at0, q.begin() + 6 * (M.at0 + M.nat));
+      r.emo.assign(e.begin() + M.ao0, e.begin() + M.ao0 + M.nao);
+      if (o.want_gradient && r.status == 0) r.grad.assign(g.begin() + 3 * M.at0, g.begin() + 3 * (M.at0 + M.nat));
+    }
+  }
+};
+
+}  // namespace axtb
+
+// ================================================================================ C ABI
+using axtb::Engine;
+
+struct axtbhip_context {
+  std::unique_ptr<Engine> eng;
+  std::string err;
+};
+
+static void fill_result(const axtb::MolOut& r, const axtb_mol& m, axtb_result* out) {
+  out->status = r.status;
+  out->energy = r.energy;
+  out->iterations = r.iterations;
+  out->nsh = r.nsh;
+  out->nao = r.nao;
+  out->efermi = r.efermi;
+  out->residual = r.residual;
+  out->energy_change = r.energy_change;
+  for (int k = 0; k < 8; k++) out->terms[k] = r.terms[k];
+  auto put = [](double* dst, const std::vector<double>& v) {
+    if (dst && !v.empty()) std::memcpy(dst, v.data(), v.size() * sizeof(double));
+  };
+  put(out->gradient, r.grad);
+  put(out->charges, r.qat);
+  put(out->shell_charges, r.qsh);
+  put(out->dipoles, r.dp);
+  put(out->quadrupoles, r.qp);
+  put(out->orbital_energies, r.emo);
+  (void)m;
+}
+
+extern "C" {
+
+const char* axtbhip_version(void) { return "axtb-hip 0.1.0 (GFN2-xTB, gfx950)"; }
+
+axtbhip_context* axtbhip_context_create(int device, char* err, size_t errlen) {
+  try {
+    auto* c = new axtbhip_context;
+    c->eng.reset(new Engine(device));
+    return c;
+  } catch (std::exception& e) {
+    if (err && errlen) snprintf(err, errlen, "%s", e.what());
+    return nullptr;
+  }
+}
+
+void axtbhip_context_destroy(axtbhip_context* ctx) { delete ctx; }
+
+int axtbhip_last_error(axtbhip_context* ctx, char* buf, size_t len) {
+  if (!ctx || !buf || !len) return 0;
+  snprintf(buf, len, "%s", ctx->err.c_str());
+  return (int)ctx->err.size();
+}
+
+int axtbhip_basis_size(const axtb_mol* mol, int* nsh, int* nao) {
+  int s = 0, a = 0;
+  for (int i = 0; i < mol->nat; i++) {
+    int z = mol->numbers[i];
+    if (z < 1 || z > AXTB_GFN2_MAX_ELEM) return AXTB_ERR_UNSUPPORTED_ELEMENT;
+    for (int k = 0; k < GFN2_NSHELL[z - 1]; k++) {
+      s++;
+      a += 2 * GFN2_ANG_SHELL[z - 1][k] + 1;
+    }
+  }
+  if (nsh) *nsh = s;
+  if (nao) *nao = a;
+  return 0;
+}
+
+int axtbhip_gfn2_engrad_batch(axtbhip_context* ctx, const axtb_mol* mols, int nmol, const axtb_opts* opts,
+                              axtb_result* outs) {
+  if (!ctx || !mols || !opts || !outs || nmol < 0) return AXTB_ERR_INPUT;
+  std::vector<axtb::MolIn> in(nmol);
+  for (int i = 0; i < nmol; i++) in[i] = axtb::MolIn{mols[i].nat, mols[i].numbers, mols[i].positions, mols[i].charge, mols[i].uhf};
+  std::vector<axtb::MolOut> res;
+  try {
+    ctx->eng->run(in, *opts, res);
+  } catch (axtb::Fail& f) {
+    ctx->err = f.msg;
+    for (int i = 0; i < nmol; i++) outs[i].status = f.code;
+    return f.code;
+  } catch (std::exception& e) {
+    ctx->err = e.what();
+    for (int i = 0; i < nmol; i++) outs[i].status = AXTB_ERR_DEVICE;
+    return AXTB_ERR_DEVICE;
+  }
+  int worst = 0;
+  ctx->err.clear();
+  for (int i = 0; i < nmol; i++) {
+    fill_result(res[i], mols[i], &outs[i]);
+    if (res[i].status && !worst) {
+      worst = res[i].status;
+      ctx->err = res[i].msg;
+    }
+  }
+  return worst;
+}
+
+int axtbhip_gfn2_engrad(axtbhip_context* ctx, const axtb_mol* mol, const axtb_opts* opts, axtb_result* out) {
+  return axtbhip_gfn2_engrad_batch(ctx, mol, 1, opts, out);
+}
+
+}  // extern "C"
+
+// ================================================================================ test hooks
+// Kernel-level entry points for the parity tests (declared in include/axtb_hip_test.h): each
+// runs one piece of the engine on host arrays.
+extern "C" {
+
+int axtbhip_test_gemm(int m, int n, int k, int ta, int tb, const double* A, int lda, const double* B, int ldb,
+                      const double* dk, double* C, int ldc) {
+  try {
+    using namespace axtb;
+    Engine e(0);
+    DBuf<double> a, b, c, d;
+    size_t na = (size_t)lda * (ta ? m : k), nb = (size_t)ldb * (tb ? k : n), nc = (size_t)ldc * n;
+    a.upload(std::vector<double>(A, A + na), e.st);
+    b.upload(std::vector<double>(B, B + nb), e.st);
+    c.ensure(nc);
+    if (dk) d.upload(std::vector<double>(dk, dk + k), e.st);
+    std::vector<GemmJob> j{GemmJob{m, n, k, ta, tb, a.p, b.p, c.p, dk ? d.p : nullptr, lda, ldb, ldc, 1.0, 0.0}};
+    e.gemm(j);
+    HIP_OK(hipMemcpyAsync(C, c.p, nc * sizeof(double), hipMemcpyDeviceToHost, e.st));
+    HIP_OK(hipStreamSynchronize(e.st));
+    return 0;
+  } catch (std::exception&) {
+    return AXTB_ERR_DEVICE;
+  }
+}
+
+// eigenpairs of `nb` symmetric n x n matrices (column-major, consecutive): w [nb][n], Z [nb][n][n]
+int axtbhip_test_eigh(int n, int nb, const double* A, double* w, double* Z, int* sweeps) {
+  try {
+    using namespace axtb;
+    Engine e(0);
+    DBuf<double> a, v, ww, z;
+    size_t nn = (size_t)n * n;
+    a.upload(std::vector<double>(A, A + nn * nb), e.st);
+    v.ensure(nn * nb);
+    ww.ensure((size_t)n * nb);
+    z.ensure(nn * nb);
+    std::vector<EigJob> jobs(nb);
+    for (int i = 0; i < nb; i++) jobs[i] = EigJob{n, a.p + nn * i, v.p + nn * i, ww.p + (size_t)n * i, z.p + nn * i};
+    e.eigh(jobs);
+    HIP_OK(hipMemcpyAsync(w, ww.p, (size_t)n * nb * sizeof(double), hipMemcpyDeviceToHost, e.st));
+    HIP_OK(hipMemcpyAsync(Z, z.p, nn * nb * sizeof(double), hipMemcpyDeviceToHost, e.st));
+    if (sweeps) HIP_OK(hipMemcpyAsync(sweeps, e.jstat.p, nb * sizeof(int), hipMemcpyDeviceToHost, e.st));
+    HIP_OK(hipStreamSynchronize(e.st));
+    return 0;
+  } catch (std::exception&) {
+    return AXTB_ERR_DEVICE;
+  }
+}
+
+// Setup of one molecule: S, H0 (nao x nao column-major), D (3 nao^2), Q (6 nao^2), the GFN2 CN,
+// the start shell charges, X = S^-1/2 and the D4 three-body energy per atom.
+int axtbhip_test_setup(const axtb_mol* mol, const axtb_opts* opts, double* S, double* H0, double* D, double* Q,
+                       double* cn, double* qstart, double* X, double* e3) {
+  try {
+    using namespace axtb;
+    Engine e(0);
+    std::vector<MolIn> in{MolIn{mol->nat, mol->numbers, mol->positions, mol->charge, mol->uhf}};
+    std::vector<MolOut> out(1);
+    e.build(in, {0}, *opts, out);
+    if (e.mols.empty()) return out[0].status ? out[0].status : AXTB_ERR_INPUT;
+    e.setup(*opts);
+    size_t nn = (size_t)e.mols[0].nao * e.mols[0].nao;
+    auto cp = [&](double* dst, const double* src, size_t n) {
+      if (dst) HIP_OK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, e.st));
+    };
+    cp(S, e.pool.p, nn);
+    cp(H0, e.pool.p + e.tot_mat, nn);
+    for (int k = 0; k < 3; k++) cp(D ? D + k * nn : nullptr, e.pool.p + (2 + k) * e.tot_mat, nn);
+    for (int k = 0; k < 6; k++) cp(Q ? Q + k * nn : nullptr, e.pool.p + (5 + k) * e.tot_mat, nn);
+    cp(cn, e.cn.p, mol->nat);
+    cp(qstart, e.qstart.p, e.mols[0].nsh);
+    cp(X, e.X.p, nn);
+    cp(e3, e.e3.p, mol->nat);
+    HIP_OK(hipStreamSynchronize(e.st));
+    return 0;
+  } catch (axtb::Fail& f) {
+    return f.code;
+  } catch (std::exception&) {
+    return AXTB_ERR_DEVICE;
+  }
+}
+
+// The first SCC iteration of one molecule from its start: F, eigenvalues, P and the output
+// shell charges, plus the potentials it used (vsh [nsh], vat [nat], vdp [3 nat], vqp [6 nat]).
+int axtbhip_test_iter1(const axtb_mol* mol, const axtb_opts* opts, double* F, double* w, double* P, double* qsh_out,
+                       double* vsh, double* vat, double* vdp, double* vqp) {
+  try {
+    using namespace axtb;
+    Engine e(0);
+    std::vector<MolIn> in{MolIn{mol->nat, mol->numbers, mol->positions, mol->charge, mol->uhf}};
+    std::vector<MolOut> out(1);
+    e.build(in, {0}, *opts, out);
+    if (e.mols.empty()) return out[0].status ? out[0].status : AXTB_ERR_INPUT;
+    e.setup(*opts);
+    axtb_opts o1 = *opts;
+    o1.max_iter = 1;
+    e.scf(o1);
+    size_t nn = (size_t)e.mols[0].nao * e.mols[0].nao;
+    auto cp = [&](double* dst, const double* src, size_t n) {
+      if (dst) HIP_OK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, e.st));
+    };
+    cp(F, e.F.p, nn);
+    cp(w, e.w.p, e.mols[0].nao);
+    cp(P, e.P.p, nn);
+    cp(qsh_out, e.qsh_o.p, e.mols[0].nsh);
+    cp(vsh, e.vsh.p, e.mols[0].nsh);
+    cp(vat, e.vat.p, mol->nat);
+    cp(vdp, e.vdp.p, 3 * mol->nat);
+    cp(vqp, e.vqp.p, 6 * mol->nat);
+    HIP_OK(hipStreamSynchronize(e.st));
+    return 0;
+  } catch (axtb::Fail& f) {
+    return f.code;
+  } catch (std::exception&) {
+    return AXTB_ERR_DEVICE;
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,1252 @@
+// GFN2-xTB kernels of the HIP engine: geometry setup (coordination numbers, EEQ system, gamma,
+// repulsion, D4), the integral build, the SCC iteration pieces (potentials, Fock update, Fermi
+// filling, Mulliken moments, energy, Broyden mixing) and the analytic gradient.
+//
+// Every kernel works on a batch of molecules (Batch/Mol descriptors); a single molecule is a
+// batch of one. Per-molecule work is skipped where `active[m] == 0` (converged or failed).
+// Reductions run in a fixed order, so results do not depend on scheduling.
+#include "common.hpp"
+#include "shellpair.hpp"
+
+namespace axtb {
+
+__global__ void k_fill(double* p, long long n, double v) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long st = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += st) p[i] = v;
+}
+
+// ------------------------------------------------------------------ counting functions
+__device__ __forceinline__ double exp_count(double k, double r, double r0) { return 1.0 / (1.0 + exp(-k * (r0 / r - 1.0))); }
+__device__ __forceinline__ double exp_dcount(double k, double r, double r0) {
+  double e = exp(-k * (r0 / r - 1.0));
+  return (-k * r0 * e) / (r * r * (e + 1.0) * (e + 1.0));
+}
+// GFN2 double-exponential CN (base/src/ncoord.rs:68-115)
+__device__ __forceinline__ double dexp_count(double r, double rc) { return exp_count(10.0, r, rc) * exp_count(20.0, r, rc + 2.0); }
+__device__ __forceinline__ double dexp_dcount(double r, double rc) {
+  return exp_dcount(10.0, r, rc) * exp_count(20.0, r, rc + 2.0) + exp_count(10.0, r, rc) * exp_dcount(20.0, r, rc + 2.0);
+}
+// dftd4 erf counting (base/src/ncoord.rs:140-152), k = 7.5
+__device__ __forceinline__ double erf_count(double r, double rc) { return 0.5 * (1.0 + erf(-((7.5 * (r - rc)) / rc))); }
+__device__ __forceinline__ double erf_dcount(double r, double rc) {
+  double e = 7.5 * (r - rc) / rc;
+  return -(7.5 * exp(-(e * e))) / (1.7724538509055159 * rc);
+}
+__device__ __forceinline__ double d4_en_factor(double ei, double ej) {  // d4/src/ncoord.rs:19-27
+  double x = fabs(ei - ej) + 19.08857;
+  return 4.10451 * exp(-((x * x) / (2.0 * (11.28174 * 11.28174))));
+}
+
+// ------------------------------------------------------------------ setup: per-atom quantities
+// One wavefront per atom: the GFN2 CN (40 bohr), the EEQ CN (multicharge, 25 bohr, capped at 8)
+// and the D4 CN (40 bohr, EN-weighted); then shell levels and AES radii of the atom.
+__global__ void k_atom_setup(Batch b, const int* active, double* cn, double* cn_eeq, double* cn_d4, double* se,
+                             double* dsedcn, double* mrad, double* dmrdcn) {
+  int a = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  int lane = threadIdx.x & 63;
+  if (a >= b.nat) return;
+  int m = b.a.mol[a];
+  if (active && !active[m]) return;
+  const Mol M = b.mol[m];
+  const double* xa = &b.a.xyz[3 * a];
+  double c1 = 0.0, c2 = 0.0, c3 = 0.0;
+  for (int j = M.at0 + lane; j < M.at0 + M.nat; j += 64) {
+    if (j == a) continue;
+    const double* xj = &b.a.xyz[3 * j];
+    double v[3] = {xa[0] - xj[0], xa[1] - xj[1], xa[2] - xj[2]};
+    double r2 = dot3(v, v);
+    if (r2 < 1e-12) continue;
+    double r = sqrt(r2);
+    if (r2 <= 1600.0) {
+      c1 += dexp_count(r, b.a.rcov_cn[a] + b.a.rcov_cn[j]);
+      c3 += d4_en_factor(b.a.en[a], b.a.en[j]) * erf_count(r, b.a.d4_rcov[a] + b.a.d4_rcov[j]);
+    }
+    if (r2 <= 625.0) c2 += erf_count(r, b.a.rcov_eeq[a] + b.a.rcov_eeq[j]);
+  }
+  c1 = wave_sum(c1);
+  c2 = wave_sum(c2);
+  c3 = wave_sum(c3);
+  if (lane == 0) {
+    cn[a] = c1;
+    // log cap at 8 (base/src/ncoord.rs:258-262)
+    cn_eeq[a] = log(1.0 + exp(8.0)) - log(exp(8.0 - c2) + 1.0);
+    cn_d4[a] = c3;
+    // AES radii (coulomb.rs:368-381)
+    double arg = c1 - b.a.aes_vcn[a] - 1.2;
+    double t1 = exp(-4.0 * arg);
+    double t2 = (5.0 - b.a.aes_rad[a]) / (1.0 + t1);
+    mrad[a] = b.a.aes_rad[a] + t2;
+    dmrdcn[a] = -t2 * 4.0 * t1 / (1.0 + t1);
+    // shell levels (hamiltonian.rs:42-54)
+    for (int s = b.a.ish[a]; s < b.a.ish[a] + b.a.nsh[a]; s++) {
+      se[s] = b.s.se0[s] - b.s.kcn[s] * c1;
+      dsedcn[s] = -b.s.kcn[s];
+    }
+  }
+}
+
+// gamma of the shell pair (coulomb.rs:61-119, exponent 2), stored nsh x nsh per molecule
+__global__ void k_gamma(Batch b, double* gam) {
+  int m = blockIdx.y;
+  const Mol M = b.mol[m];
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)M.nsh * M.nsh) return;
+  int i = (int)(idx / M.nsh), j = (int)(idx % M.nsh);
+  int si = M.sh0 + i, sj = M.sh0 + j;
+  int ai = b.s.at[si], aj = b.s.at[sj];
+  double eta = 0.5 * (b.s.hard[si] + b.s.hard[sj]);
+  double val;
+  if (ai == aj) {
+    val = eta;
+  } else {
+    const double* x = &b.a.xyz[3 * ai];
+    const double* y = &b.a.xyz[3 * aj];
+    double v[3] = {y[0] - x[0], y[1] - x[1], y[2] - x[2]};
+    val = 1.0 / sqrt(dot3(v, v) + 1.0 / (eta * eta));
+  }
+  gam[M.shp0 + idx] = val;
+}
+
+// EEQ-2019 bordered system (axtb/src/eeq.rs:43-77, base/src/eeq.rs:69-104) of each molecule:
+// (nat+1)^2 column-major at eq0 = the molecule's pair0 + its index (pair pool sized for it).
+__global__ void k_eeq_system(Batch b, const long long* eq0, const double* cn_eeq, double* A, double* rhs) {
+  int m = blockIdx.y;
+  const Mol M = b.mol[m];
+  int n1 = M.nat + 1;
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)n1 * n1) return;
+  int i = (int)(idx % n1), j = (int)(idx / n1);
+  double v;
+  if (i == M.nat && j == M.nat) {
+    v = 0.0;
+  } else if (i == M.nat || j == M.nat) {
+    v = 1.0;
+  } else {
+    int ai = M.at0 + i, aj = M.at0 + j;
+    if (i == j) {
+      v = b.a.eeq_eta[ai] + 0.7978845608028654 / b.a.eeq_rad[ai];  // sqrt(2/pi)
+    } else {
+      const double* x = &b.a.xyz[3 * ai];
+      const double* y = &b.a.xyz[3 * aj];
+      double d[3] = {x[0] - y[0], x[1] - y[1], x[2] - y[2]};
+      double r2 = dot3(d, d);
+      double ri = b.a.eeq_rad[ai], rj = b.a.eeq_rad[aj];
+      double gam = 1.0 / (ri * ri + rj * rj);
+      v = erf(sqrt(r2 * gam)) / sqrt(r2);
+    }
+  }
+  A[eq0[m] + idx] = v;
+  if (j == 0) {
+    double r;
+    if (i == M.nat) {
+      r = M.charge;
+    } else {
+      int ai = M.at0 + i;
+      double c = cn_eeq[ai];
+      r = -b.a.eeq_chi[ai] + b.a.eeq_kcnchi[ai] / sqrt(c + 1e-14) * c;
+    }
+    rhs[(long long)M.at0 + m + i] = r;
+  }
+}
+
+// q = V diag(1/w) V^T rhs of the EEQ system from its eigendecomposition, then the start shell
+// charges split by reference occupation (xtb.rs:530-547). One workgroup per molecule; a
+// singular system or non-finite charges leave neutral atoms (the reference's "sad" start).
+__global__ void k_eeq_solve(Batch b, const long long* eq0, const double* V, const double* w, const double* rhs,
+                            double* y, double* qstart, int* eeq_failed) {
+  __shared__ int bad;
+  int m = blockIdx.x;
+  const Mol M = b.mol[m];
+  int n1 = M.nat + 1;
+  const double* Vm = V + eq0[m];
+  const double* r = rhs + (long long)M.at0 + m;
+  double* ym = y + (long long)M.at0 + m;
+  const double* wm = w + (long long)M.at0 + m;
+  if (threadIdx.x == 0) bad = 0;
+  __syncthreads();
+  for (int k = threadIdx.x; k < n1; k += blockDim.x) {
+    double s = 0.0;
+    for (int i = 0; i < n1; i++) s += Vm[(long long)k * n1 + i] * r[i];
+    if (!(fabs(wm[k]) > 1e-300)) bad = 1;
+    ym[k] = s / wm[k];
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < M.nat; i += blockDim.x) {
+    double q = 0.0;
+    for (int k = 0; k < n1; k++) q += Vm[(long long)k * n1 + i] * ym[k];
+    if (!isfinite(q)) bad = 1;
+    int a = M.at0 + i;
+    double tot = 0.0;
+    for (int s = b.a.ish[a]; s < b.a.ish[a] + b.a.nsh[a]; s++) tot += b.s.refocc[s];
+    for (int s = b.a.ish[a]; s < b.a.ish[a] + b.a.nsh[a]; s++) qstart[s] = tot > 0.0 ? q * b.s.refocc[s] / tot : 0.0;
+  }
+  __syncthreads();
+  if (bad) {
+    for (int s = M.sh0 + threadIdx.x; s < M.sh0 + M.nsh; s += blockDim.x) qstart[s] = 0.0;
+    if (threadIdx.x == 0) eeq_failed[m] = 1;
+  }
+}
+
+// ------------------------------------------------------------------ integrals
+// hscale of shells (sa on atom A, sb on atom B) (gfn2.rs:24-30,76-98)
+__device__ __forceinline__ double hscale(const Batch& b, int sa, int sb) {
+  const double KD[3] = {1.85, 2.23, 2.23};
+  int la = b.s.l[sa], lb = b.s.l[sb];
+  double km = (KD[la] + KD[lb]) / 2.0;  // s and p shells only: kshell = average of KDIAG
+  double dEN = b.a.en[b.s.at[sa]] - b.a.en[b.s.at[sb]];
+  double za = b.s.zeta[sa], zb = b.s.zeta[sb];
+  double zij = sqrt(2.0 * sqrt(za * zb) / (za + zb));
+  return zij * (km * (1.0 + 0.02 * (dEN * dEN)));
+}
+
+// S, H0, D (3), Q (6) of every shell pair (ish of atom A >= jsh of atom B): each AO element
+// written once (ao.rs:195-291, hamiltonian.rs:77-94). Pools are `stride` elements apart:
+// slot 0 S, 1 H0, 2-4 D, 5-10 Q; element (mu, nu) of molecule m at mat0 + mu + nu nao.
+__global__ void k_integrals(Batch b, const double* se, double* pool, long long stride) {
+  int m = blockIdx.y;
+  const Mol M = b.mol[m];
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)M.nsh * M.nsh) return;
+  int i = (int)(idx / M.nsh), j = (int)(idx % M.nsh);  // ket shell i (column), bra shell j (row)
+  int si = M.sh0 + i, sj = M.sh0 + j;
+  int ai = b.s.at[si], aj = b.s.at[sj];
+  if (ai < aj) return;
+  const double* xi = &b.a.xyz[3 * ai];
+  const double* xj = &b.a.xyz[3 * aj];
+  double vec[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+  double r2 = dot3(vec, vec);
+  bool onsite = ai == aj;
+  if (!onsite) {
+    // atom-pair and shell-pair exponent screening (integrals.rs:216-218)
+    double a1 = b.s.minalpha[si], a2 = b.s.minalpha[sj];
+    if (a1 * a2 * r2 / (a1 + a2) > b.intcut * (1.0 + 1e-10)) return;
+  }
+  double blk[9][10];
+  shellpair_multipole(b.s.l[si], b.s.nprim[si], &b.s.alpha[MAXPRIM * si], &b.s.coeff[MAXPRIM * si], b.s.l[sj],
+                      b.s.nprim[sj], &b.s.alpha[MAXPRIM * sj], &b.s.coeff[MAXPRIM * sj], vec, b.intcut, blk);
+  double hij;
+  if (onsite) {
+    hij = 0.5 * (se[si] + se[sj]);
+  } else {
+    double rr = sqrt(sqrt(r2) / (b.a.rad[aj] + b.a.rad[ai]));
+    double sp = (1.0 + b.s.shpoly[si] * rr) * (1.0 + b.s.shpoly[sj] * rr);
+    hij = 0.5 * (se[si] + se[sj]) * hscale(b, sj, si) * sp;
+  }
+  int nsi = 2 * b.s.l[si] + 1, nsj = 2 * b.s.l[sj] + 1;
+  int ii = b.s.iao[si], jj = b.s.iao[sj];
+  long long n = M.nao;
+  for (int im = 0; im < nsi; im++)
+    for (int jm = 0; jm < nsj; jm++) {
+      const double* o = blk[jm + nsj * im];
+      long long mu = jj + jm, nu = ii + im;
+      long long e1 = M.mat0 + mu + nu * n;
+      pool[e1] = o[0];
+      pool[stride + e1] = o[0] * hij;
+      for (int k = 0; k < 3; k++) pool[(2 + k) * stride + e1] = o[1 + k];
+      for (int k = 0; k < 6; k++) pool[(5 + k) * stride + e1] = o[4 + k];
+      if (!onsite) {
+        double dj[3], qj[6];
+        shift_to_bra(vec, o[0], &o[1], &o[4], dj, qj);
+        long long e2 = M.mat0 + nu + mu * n;
+        pool[e2] = o[0];
+        pool[stride + e2] = o[0] * hij;
+        for (int k = 0; k < 3; k++) pool[(2 + k) * stride + e2] = dj[k];
+        for (int k = 0; k < 6; k++) pool[(5 + k) * stride + e2] = qj[k];
+      }
+    }
+}
+
+// ------------------------------------------------------------------ repulsion (repulsion.rs:44-73)
+// One wavefront per atom: E_A (half of each pair) and dE/dR_A, cutoff 40 bohr.
+__global__ void k_repulsion(Batch b, const int* active, double* e, double* g) {
+  int a = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  int lane = threadIdx.x & 63;
+  if (a >= b.nat) return;
+  int m = b.a.mol[a];
+  if (active && !active[m]) return;
+  const Mol M = b.mol[m];
+  const double* xa = &b.a.xyz[3 * a];
+  double es = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
+  for (int j = M.at0 + lane; j < M.at0 + M.nat; j += 64) {
+    if (j == a) continue;
+    const double* xj = &b.a.xyz[3 * j];
+    double v[3] = {xa[0] - xj[0], xa[1] - xj[1], xa[2] - xj[2]};
+    double r2 = dot3(v, v);
+    if (r2 > 1600.0 || r2 < 1e-12) continue;
+    double r1 = sqrt(r2);
+    double kexp = (b.a.z[a] > 2 || b.a.z[j] > 2) ? 1.5 : 1.0;
+    double alpha = sqrt(b.a.rep_alpha[j] * b.a.rep_alpha[a]);
+    double zeff = b.a.rep_zeff[j] * b.a.rep_zeff[a];
+    double r1k = kexp == 1.5 ? r1 * sqrt(r1) : r1;
+    double de = zeff * exp(-alpha * r1k) / r1;
+    es += 0.5 * de;
+    double f = -(alpha * r1k * kexp + 1.0) * de / r2;
+    gx += f * v[0];
+    gy += f * v[1];
+    gz += f * v[2];
+  }
+  es = wave_sum(es);
+  gx = wave_sum(gx);
+  gy = wave_sum(gy);
+  gz = wave_sum(gz);
+  if (lane == 0) {
+    e[a] = es;
+    g[3 * a] = gx;
+    g[3 * a + 1] = gy;
+    g[3 * a + 2] = gz;
+  }
+}
+
+// ------------------------------------------------------------------ D4 (crates/d4/src)
+__device__ __forceinline__ void zeta_dz(double a, double c, double qref, double qmod, double& z, double& dz) {
+  if (qmod < 0.0) {
+    z = exp(a);
+    dz = 0.0;
+    return;
+  }
+  double e = exp(c * (1.0 - qref / qmod));
+  z = exp(a * (1.0 - e));
+  dz = -((a * c * e * z * qref) / (qmod * qmod));
+}
+
+// Reference weights of every atom (d4/src/model.rs:468-545) at charges q: gw, and either
+// dgw/dq (mode 0) or dgw/dCN (mode 1) into dgw. One thread per atom.
+__global__ void k_d4_weights(Batch b, const int* active, const double* cn_d4, const double* q, double* gw,
+                             double* dgw, int mode) {
+  int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= b.nat) return;
+  if (active && !active[b.a.mol[a]]) return;
+  const int nref = b.a.d4_nref[a];
+  const double* refcn = &b.a.d4_refcn[MAXREF * a];
+  const int* ngw = &b.a.d4_ngw[MAXREF * a];
+  double cni = cn_d4[a];
+  double norm = 0.0, dnorm = 0.0;
+  for (int r = 0; r < nref; r++)
+    for (int g = 1; g <= ngw[r]; g++) {
+      double wf = g * 6.0, d = cni - refcn[r];
+      double gg = exp(-(wf * (d * d)));
+      norm += gg;
+      dnorm += 2.0 * wf * (refcn[r] - cni) * gg;
+    }
+  norm = fabs(norm) > 1.4916681462400413e-154 ? 1.0 / norm : 0.0;
+  double maxcn = -INFINITY;
+  for (int r = 0; r < nref; r++) maxcn = fmax(maxcn, refcn[r]);
+  for (int r = 0; r < MAXREF; r++) {
+    double w = 0.0, dw = 0.0;
+    if (r < nref) {
+      double expw = 0.0, expd = 0.0;
+      for (int g = 1; g <= ngw[r]; g++) {
+        double wf = g * 6.0, d = cni - refcn[r];
+        double gg = exp(-(wf * (d * d)));
+        expw += gg;
+        expd += 2.0 * wf * (refcn[r] - cni) * gg;
+      }
+      double gwk = expw * norm;
+      if (!isfinite(gwk) || norm == 0.0) gwk = fabs(maxcn - refcn[r]) < 1e-12 ? 1.0 : 0.0;
+      double z, dz;
+      zeta_dz(3.0, b.a.d4_gi[a], b.a.d4_refq[MAXREF * a + r] + b.a.d4_zeff[a], q[a] + b.a.d4_zeff[a], z, dz);
+      w = gwk * z;
+      if (mode == 0) {
+        dw = gwk * dz;
+      } else {
+        double dgwk = norm * (expd - expw * dnorm * norm);
+        if (!isfinite(dgwk) || norm == 0.0) dgwk = 0.0;
+        dw = dgwk * z;
+      }
+    }
+    gw[MAXREF * a + r] = w;
+    dgw[MAXREF * a + r] = dw;
+  }
+}
+
+// C6 of a pair and its derivative contractions (model.rs:607-627)
+__device__ __forceinline__ void pair_c6(const Batch& b, int i, int j, const double* gw, const double* dgw, double& c6,
+                                        double& di, double& dj) {
+  const double* tab = &b.c6ref[((long long)b.a.sp[i] * b.nsp + b.a.sp[j]) * 49];
+  int ni = b.a.d4_nref[i], nj = b.a.d4_nref[j];
+  c6 = di = dj = 0.0;
+  for (int ri = 0; ri < ni; ri++) {
+    double wi = gw[MAXREF * i + ri], dwi = dgw[MAXREF * i + ri];
+    for (int rj = 0; rj < nj; rj++) {
+      double r = tab[ri * 7 + rj];
+      double wj = gw[MAXREF * j + rj];
+      c6 += wi * wj * r;
+      di += dwi * wj * r;
+      dj += wi * dgw[MAXREF * j + rj] * r;
+    }
+  }
+}
+
+__device__ __forceinline__ double pw3(double x) { return (x * x) * x; }
+__device__ __forceinline__ double pw4(double x) { double x2 = x * x; return x2 * x2; }
+
+// d4/src/disp.rs:43-63: 5th-order smoothstep over the last `width` bohr
+__device__ __forceinline__ void smooth_cut(double r, double cut, double width, double& sw, double& dsw) {
+  double w = fmin(width, cut), inner = cut - w;
+  if (r <= inner) {
+    sw = 1.0;
+    dsw = 0.0;
+  } else if (r >= cut) {
+    sw = 0.0;
+    dsw = 0.0;
+  } else {
+    double x = (cut - r) / w, x2 = x * x;
+    sw = (x2 * x) * (10.0 + x * (-15.0 + 6.0 * x));
+    double omx = 1.0 - x;
+    dsw = -((30.0 * x2 * (omx * omx)) / w);
+  }
+}
+
+// rational-damped pair energy factor with the smooth 72/1 bohr cutoff (disp.rs:240-272)
+__device__ __forceinline__ bool d4_pair_e(const Batch& b, int i, int j, const double* v, double r2, double& e,
+                                          double& gdr) {
+  if (r2 > 72.0 * 72.0 || r2 < 2.220446049250313e-16) return false;
+  double r = sqrt(r2), sw, dsw;
+  smooth_cut(r, 72.0, 1.0, sw, dsw);
+  if (sw <= 0.0) return false;
+  double rr = 3.0 * b.a.d4_r4r2[i] * b.a.d4_r4r2[j];
+  double r0 = 0.52 * sqrt(rr) + 5.0;
+  double r06 = pw3(r0 * r0);
+  double t6 = 1.0 / (pw3(r2) + r06), t8 = 1.0 / (pw4(r2) + r06 * r0 * r0);
+  double d6 = -(6.0 * (r2 * r2) * (t6 * t6)), d8 = -(8.0 * pw3(r2) * (t8 * t8));
+  double e0 = t6 + 2.7 * rr * t8, g0 = d6 + 2.7 * rr * d8;
+  e = sw * e0;
+  gdr = sw * g0 + dsw * e0 / r;
+  return true;
+}
+
+// ATM pair table (disp.rs:548-605): per ordered pair (a, k) of a molecule, nat x nat row-major at
+// pair0: r2, 1/r, t = (r0/r)^(alpha/3), w = sqrt|C6| sw; plus C6, dlnC6/dCN_a, sw, dsw/r.
+__global__ void k_atm_pairs(Batch b, const double* gw, const double* dgw, double* P) {
+  int m = blockIdx.y;
+  const Mol M = b.mol[m];
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long nn = (long long)M.nat * M.nat;
+  if (idx >= nn) return;
+  int a = M.at0 + (int)(idx / M.nat), k = M.at0 + (int)(idx % M.nat);
+  long long o = M.pair0 + idx;
+  long long tot = b.mol[b.nmol - 1].pair0 + (long long)b.mol[b.nmol - 1].nat * b.mol[b.nmol - 1].nat;
+  double r2v = 0, irv = 0, tv = 0, wv = 0, c6v = 0, qav = 0, swv = 0, dswv = 0;
+  if (a != k) {
+    const double* xa = &b.a.xyz[3 * a];
+    const double* xk = &b.a.xyz[3 * k];
+    double v[3] = {xk[0] - xa[0], xk[1] - xa[1], xk[2] - xa[2]};
+    double r2 = dot3(v, v);
+    if (r2 <= 1600.0 && r2 >= 2.220446049250313e-16) {
+      double r = sqrt(r2), sw, dsw;
+      smooth_cut(r, 40.0, 1.0, sw, dsw);
+      double rr = 3.0 * b.a.d4_r4r2[a] * b.a.d4_r4r2[k];
+      double r0 = 0.52 * sqrt(rr) + 5.0;
+      double c6, da, dk;
+      pair_c6(b, a, k, gw, dgw, c6, da, dk);
+      r2v = r2;
+      irv = 1.0 / r;
+      tv = pow(r0 / r, 16.0 / 3.0);
+      wv = sqrt(fabs(c6)) * sw;
+      c6v = c6;
+      qav = c6 != 0.0 ? da / c6 : 0.0;
+      swv = sw;
+      dswv = dsw / r;
+    }
+  }
+  P[o] = r2v;
+  P[tot + o] = irv;
+  P[2 * tot + o] = tv;
+  P[3 * tot + o] = wv;
+  P[4 * tot + o] = c6v;
+  P[5 * tot + o] = qav;
+  P[6 * tot + o] = swv;
+  P[7 * tot + o] = dswv;
+}
+
+// ATM three-body energy and gradient by pair sums (disp.rs:607-720, axtb's d4_atm_pair_sums):
+// one wavefront per atom a sums, over its partners b with sw > 0, the pair's (g, d) over third
+// atoms k, and keeps its own share: E_a -= sw d / 6 per pair, the pair force along R_b - R_a and
+// dE/dCN_a. Writes e3[a], g3[3a], dcn3[a].
+__global__ void k_atm(Batch b, const double* P, double* e3, double* g3, double* dcn3) {
+  int a = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  int lane = threadIdx.x & 63;
+  if (a >= b.nat) return;
+  int m = b.a.mol[a];
+  const Mol M = b.mol[m];
+  long long tot = b.mol[b.nmol - 1].pair0 + (long long)b.mol[b.nmol - 1].nat * b.mol[b.nmol - 1].nat;
+  const double *R2 = P, *IR = P + tot, *T = P + 2 * tot, *W = P + 3 * tot, *C6 = P + 4 * tot, *QA = P + 5 * tot,
+               *SW = P + 6 * tot, *DSW = P + 7 * tot;
+  int la = a - M.at0;
+  long long rowa = M.pair0 + (long long)la * M.nat;
+  double es = 0.0, gx = 0.0, gy = 0.0, gz = 0.0, dc = 0.0;
+  const double alp3 = 16.0 / 3.0;
+  for (int lb = 0; lb < M.nat; lb++) {
+    if (lb == la) continue;
+    double swab = SW[rowa + lb];
+    if (!(swab > 0.0)) continue;
+    long long rowb = M.pair0 + (long long)lb * M.nat;
+    double r2 = R2[rowa + lb], ir = IR[rowa + lb], t6 = 6.0 * T[rowa + lb];
+    double r2_2 = r2 * r2;
+    double gs = 0.0, ds = 0.0;
+    for (int lk = lane; lk < M.nat; lk += 64) {
+      double c = W[rowa + lk] * W[rowb + lk];
+      double cralp = t6 * T[rowa + lk] * T[rowb + lk];
+      double fdmp = 1.0 / (1.0 + cralp);
+      double ir1 = ir * IR[rowa + lk] * IR[rowb + lk];
+      double rrr2 = ir1 * ir1, rrr3 = rrr2 * ir1;
+      double angf = 0.375 * rrr2 * rrr3;
+      double r2ak = R2[rowa + lk], r2bk = R2[rowb + lk];
+      double s1 = r2ak + r2bk, d1 = (r2bk - r2ak) * (r2bk - r2ak);
+      double ang = angf * ((r2_2 - d1) * (s1 - r2)) + rrr3;
+      double cf = c * fdmp;
+      ds += ang * cf;
+      double p = r2 * (r2 * (r2 + s1) + 2.0 * (s1 * s1) + d1) - 5.0 * (d1 * s1);
+      gs += cf * (angf * p - alp3 * (cralp * fdmp) * ang);
+    }
+    gs = wave_sum(gs);
+    ds = wave_sum(ds);
+    double cab = -(5.0 * sqrt(fabs(C6[rowa + lb])));
+    double g = gs * cab, d = ds * cab;
+    double de = swab * d;
+    es -= de / 6.0;
+    // force on a from the pair (a, b): -(f) (R_b - R_a) with f = sw g / r2 - dsw/r d
+    double f = swab * g / r2 - DSW[rowa + lb] * d;
+    const double* xa = &b.a.xyz[3 * a];
+    const double* xb = &b.a.xyz[3 * (M.at0 + lb)];
+    gx -= f * (xb[0] - xa[0]);
+    gy -= f * (xb[1] - xa[1]);
+    gz -= f * (xb[2] - xa[2]);
+    dc -= 0.5 * de * QA[rowa + lb];
+  }
+  if (lane == 0) {
+    e3[a] = es;
+    g3[3 * a] = gx;
+    g3[3 * a + 1] = gy;
+    g3[3 * a + 2] = gz;
+    dcn3[a] = dc;
+  }
+}
+
+// ------------------------------------------------------------------ SCC iteration
+// Unpack the packed SCC vector [q_sh | mu (3N) | theta (6N), off-diagonals x sqrt2] of each active
+// molecule into shell charges, atomic charges, dipoles and quadrupoles (scf/variables.rs).
+__global__ void k_unpack(Batch b, const int* active, const double* v, double* qsh, double* qat, double* dp, double* qp) {
+  int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= b.nat) return;
+  int m = b.a.mol[a];
+  if (active && !active[m]) return;
+  const Mol M = b.mol[m];
+  const double* vm = v + M.vec0;
+  int la = a - M.at0;
+  double q = 0.0;
+  for (int s = b.a.ish[a]; s < b.a.ish[a] + b.a.nsh[a]; s++) {
+    double x = vm[s - M.sh0];
+    qsh[s] = x;
+    q += x;
+  }
+  qat[a] = q;
+  const double inv = 1.0 / 1.4142135623730951;
+  for (int c = 0; c < 3; c++) dp[3 * a + c] = vm[M.nsh + 3 * la + c];
+  for (int c = 0; c < 6; c++) {
+    double x = vm[M.nsh + 3 * M.nat + 6 * la + c];
+    qp[6 * a + c] = (c == 1 || c == 3 || c == 4) ? x * inv : x;
+  }
+}
+
+// Pack moments into the SCC vector layout; with `resid`, also dq = packed - q_in.
+__global__ void k_pack(Batch b, const int* active, const double* qsh, const double* dp, const double* qp, double* v,
+                       const double* qin, double* dq) {
+  int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= b.nat) return;
+  int m = b.a.mol[a];
+  if (active && !active[m]) return;
+  const Mol M = b.mol[m];
+  int la = a - M.at0;
+  const double sq2 = 1.4142135623730951;
+  auto put = [&](int k, double x) {
+    long long o = M.vec0 + k;
+    v[o] = x;
+    if (dq) dq[o] = x - qin[o];
+  };
+  for (int s = b.a.ish[a]; s < b.a.ish[a] + b.a.nsh[a]; s++) put(s - M.sh0, qsh[s]);
+  for (int c = 0; c < 3; c++) put(M.nsh + 3 * la + c, dp[3 * a + c]);
+  for (int c = 0; c < 6; c++) put(M.nsh + 3 * M.nat + 6 * la + c, qp[6 * a + c] * ((c == 1 || c == 3 || c == 4) ? sq2 : 1.0));
+}
+
+// v_sh = gamma q (+ Gamma q^2 with `third`): one wavefront per shell row (coulomb.rs:1006-1021,
+// 1141-1143).
+__global__ void k_shell_pot(Batch b, const int* active, const double* gam, const double* qsh, double* out, int third) {
+  int s = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  int lane = threadIdx.x & 63;
+  if (s >= b.nsh) return;
+  int m = b.a.mol[b.s.at[s]];
+  if (active && !active[m]) return;
+  const Mol M = b.mol[m];
+  const double* row = gam + M.shp0 + (long long)(s - M.sh0) * M.nsh;
+  double acc = 0.0;
+  for (int j = lane; j < M.nsh; j += 64) acc += row[j] * qsh[M.sh0 + j];
+  acc = wave_sum(acc);
+  if (lane == 0) out[s] = third ? acc + qsh[s] * qsh[s] * b.s.gam3[s] : acc;
+}
+
+__constant__ double QSCALE[6] = {1.0, 2.0, 1.0, 2.0, 2.0, 1.0};
+
+// g3, g5 of the damped multipole kernels of a pair (coulomb.rs:383-394)
+__device__ __forceinline__ void aes_g(double r1, double mri, double mrj, double& g3, double& g5) {
+  double g1 = 1.0 / r1;
+  double g3v = g1 * g1 * g1, g5v = g3v * g1 * g1;
+  double rr = 0.5 * (mrj + mri) * g1;
+  double rr3 = rr * rr * rr;
+  g3 = g3v / (1.0 + 6.0 * rr3);
+  g5 = g5v / (1.0 + 6.0 * rr3 * rr);
+}
+
+// Atom-resolved potentials of one wavefront per atom: AES (coulomb.rs:407-502) into vat, vdp,
+// vqp and the D4 dE/dq (disp.rs:420-455) into vat.
+__global__ void k_atom_pot(Batch b, const int* active, const double* mrad, const double* qat, const double* dp,
+                           const double* qp, const double* gw, const double* dgwdq, double* vat, double* vdp,
+                           double* vqp) {
+  int a = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  int lane = threadIdx.x & 63;
+  if (a >= b.nat) return;
+  int m = b.a.mol[a];
+  if (active && !active[m]) return;
+  const Mol M = b.mol[m];
+  const double* xa = &b.a.xyz[3 * a];
+  double va = 0.0, vd[3] = {0, 0, 0}, vq[6] = {0, 0, 0, 0, 0, 0};
+  for (int j = M.at0 + lane; j < M.at0 + M.nat; j += 64) {
+    if (j == a) continue;
+    const double* xj = &b.a.xyz[3 * j];
+    double w[3] = {xa[0] - xj[0], xa[1] - xj[1], xa[2] - xj[2]};
+    double r2 = dot3(w, w);
+    double r1 = sqrt(r2);
+    double g3, g5;
+    aes_g(r1, mrad[a], mrad[j], g3, g5);
+    double tc[6] = {w[0] * w[0], w[0] * w[1], w[1] * w[1], w[0] * w[2], w[1] * w[2], w[2] * w[2]};
+    const double* mb = &dp[3 * j];
+    double mbw = dot3(mb, w);
+    va += mbw * g3;
+    for (int k = 0; k < 3; k++) vd[k] += -w[k] * g3 * qat[j] + g5 * (r2 * mb[k] - 3.0 * w[k] * mbw);
+    double thb = 0.0;
+    for (int k = 0; k < 6; k++) {
+      double t = QSCALE[k] * tc[k] * g5;
+      vq[k] += t * qat[j];
+      thb += t * qp[6 * j + k];
+    }
+    va += thb;
+    double pe, pg;
+    if (d4_pair_e(b, a, j, w, r2, pe, pg)) {
+      double c6, da, dj;
+      pair_c6(b, a, j, gw, dgwdq, c6, da, dj);
+      va -= da * pe;
+    }
+  }
+  va = wave_sum(va);
+  for (int k = 0; k < 3; k++) vd[k] = wave_sum(vd[k]);
+  for (int k = 0; k < 6; k++) vq[k] = wave_sum(vq[k]);
+  if (lane == 0) {
+    vat[a] = va;
+    for (int k = 0; k < 3; k++) vdp[3 * a + k] = vd[k] + 2.0 * b.a.dkernel[a] * dp[3 * a + k];
+    for (int k = 0; k < 6; k++) vqp[6 * a + k] = vq[k] + 2.0 * b.a.qkernel[a] * qp[6 * a + k] * QSCALE[k];
+  }
+}
+
+// Fock matrix F = H0 - 1/2 (B + B^T), B(mu,nu) = S v(nu) + D.vd(A(nu)) + Q.vq(A(nu))
+// (scf/fock.rs:22-104). B(nu,mu) is rebuilt from the (mu,nu) integrals by shifting their moments
+// to the atom of mu, so every read is coalesced. `out` has the layout of the integral pools.
+__global__ void k_fock(Batch b, const int* active, const double* pool, long long stride, const double* vsh,
+                       const double* vat, const double* vdp, const double* vqp, double* F) {
+  int m = blockIdx.y;
+  if (active && !active[m]) return;
+  const Mol M = b.mol[m];
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long nn = (long long)M.nao * M.nao;
+  if (idx >= nn) return;
+  int mu = (int)(idx % M.nao), nu = (int)(idx / M.nao);
+  int gmu = M.ao0 + mu, gnu = M.ao0 + nu;
+  int A = b.ao_at[gmu], B = b.ao_at[gnu];
+  long long e = M.mat0 + idx;
+  double s = pool[e], h = pool[stride + e];
+  double d[3], q[6];
+  for (int k = 0; k < 3; k++) d[k] = pool[(2 + k) * stride + e];
+  for (int k = 0; k < 6; k++) q[k] = pool[(5 + k) * stride + e];
+  double vmu = vsh[b.ao_sh[gmu]] + vat[A], vnu = vsh[b.ao_sh[gnu]] + vat[B];
+  double bsum = s * (vmu + vnu);
+  for (int k = 0; k < 3; k++) bsum += d[k] * vdp[3 * B + k];
+  for (int k = 0; k < 6; k++) bsum += q[k] * vqp[6 * B + k];
+  // moments of (nu, mu): around the atom of mu
+  const double* xa = &b.a.xyz[3 * A];
+  const double* xb = &b.a.xyz[3 * B];
+  double vec[3] = {xb[0] - xa[0], xb[1] - xa[1], xb[2] - xa[2]};
+  double dt[3], qt[6];
+  shift_to_bra(vec, s, d, q, dt, qt);
+  for (int k = 0; k < 3; k++) bsum += dt[k] * vdp[3 * A + k];
+  for (int k = 0; k < 6; k++) bsum += qt[k] * vqp[6 * A + k];
+  F[M.mat0 + idx] = h - 0.5 * bsum;
+}
+
+// Fermi filling of each molecule (base/src/occupations.rs:94-125, Newton::XTB680), entropy term
+// -TS (:179-181) and the occupations focc = f_alpha + f_beta. One workgroup per molecule.
+// st: per molecule {efermi, ts} into dst[2*m]; kocc (last index with focc > 1e-16, +1) into kocc.
+__device__ double fermi_fill(const double* w, int n, double nel, double kt, double* occ, double* red, double& ef_out,
+                             bool& ok) {
+  // aufbau start (occupations.rs:55-67)
+  int full = (int)floor(nel);
+  double rem = fmod(nel, 1.0);
+  int homo = full + (rem > 0.5 ? 1 : 0);
+  int i0 = max(homo, 1) - 1, i1 = min(homo + 1, n) - 1;
+  double ef = 0.5 * (w[i0] + w[i1]), ef_used = ef;
+  for (int it = 0; it < 200; it++) {
+    ef_used = ef;
+    double tot = 0.0, dtot = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      double arg = (w[i] - ef) / kt, f, df;
+      if (arg > 50.0) {
+        f = 0.0;
+        df = 0.0;
+      } else if (arg < -50.0) {
+        f = 1.0;
+        df = 0.0;
+      } else {
+        f = 1.0 / (exp(arg) + 1.0);
+        df = -f * (1.0 - f) / kt;
+      }
+      occ[i] = f;
+      tot += f;
+      dtot -= df;
+    }
+    tot = block_sum(tot, red);
+    dtot = block_sum(dtot, red);
+    if (fabs(nel - tot) <= 1e5 * 2.220446049250313e-16) break;
+    if (dtot > 1.4916681462400413e-154)
+      ef += (nel - tot) / dtot;
+    else
+      break;
+  }
+  double placed = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) placed += occ[i];
+  placed = block_sum(placed, red);
+  ok = fabs(placed - nel) <= 1e-6;
+  ef_out = ef_used;
+  double ts = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    double o = occ[i];
+    ts += log(pow(o, o) * pow(1.0 - o, 1.0 - o));
+  }
+  return block_sum(ts, red) * kt;
+}
+
+__global__ void k_fermi(Batch b, const int* active, const double* w, double* focc, double* work, double* st, int* kocc,
+                        int* status) {
+  __shared__ double red[16];
+  int m = blockIdx.x;
+  if (active && !active[m]) return;
+  const Mol M = b.mol[m];
+  int n = M.nao;
+  const double* wm = w + M.ao0;
+  double* fo = focc + M.ao0;
+  double* oa = work + 2 * (long long)M.ao0;
+  double* ob = oa + n;
+  double efa = 0.0, ts = 0.0;
+  bool ok = true;
+  if (M.kt > 0.0) {
+    bool oka, okb;
+    double efb;
+    ts = fermi_fill(wm, n, M.nalpha, M.kt, oa, red, efa, oka);
+    ts += fermi_fill(wm, n, M.nbeta, M.kt, ob, red, efb, okb);
+    ok = oka && okb;
+  } else {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      double fa = i < (int)floor(M.nalpha) ? 1.0 : (i == (int)floor(M.nalpha) ? fmod(M.nalpha, 1.0) : 0.0);
+      double fb = i < (int)floor(M.nbeta) ? 1.0 : (i == (int)floor(M.nbeta) ? fmod(M.nbeta, 1.0) : 0.0);
+      oa[i] = fa;
+      ob[i] = fb;
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += blockDim.x) fo[i] = oa[i] + ob[i];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int k = 0;
+    for (int i = n - 1; i >= 0; i--)
+      if (fo[i] > 1e-16) {
+        k = i + 1;
+        break;
+      }
+    kocc[m] = k;
+    st[2 * m] = efa;
+    st[2 * m + 1] = ts;
+    if (!ok) atomicOr(&status[m], 2);
+  }
+}
+
+// Mulliken column sums (scf/mulliken.rs:19-59) and the band energy Tr(H0 P) per column: one
+// wavefront per AO column nu writes cols[11*nu + x] = sum_mu P(mu,nu) M_x(mu,nu) for
+// M = S, D (3), Q (6), H0.
+__global__ void k_mulliken_cols(Batch b, const int* active, const double* pool, long long stride, const double* P,
+                                double* cols) {
+  int g = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  int lane = threadIdx.x & 63;
+  if (g >= b.nao) return;
+  int m = b.a.mol[b.ao_at[g]];
+  if (active && !active[m]) return;
+  const Mol M = b.mol[m];
+  int nu = g - M.ao0;
+  long long c0 = M.mat0 + (long long)nu * M.nao;
+  double r[11];
+  for (int x = 0; x < 11; x++) r[x] = 0.0;
+  for (int mu = lane; mu < M.nao; mu += 64) {
+    long long e = c0 + mu;
+    double p = P[e];
+    r[0] += p * pool[e];
+    for (int k = 0; k < 3; k++) r[1 + k] += p * pool[(2 + k) * stride + e];
+    for (int k = 0; k < 6; k++) r[4 + k] += p * pool[(5 + k) * stride + e];
+    r[10] += p * pool[stride + e];
+  }
+  for (int x = 0; x < 11; x++) r[x] = wave_sum(r[x]);
+  if (lane == 0)
+    for (int x = 0; x < 11; x++) cols[11LL * g + x] = r[x];
+}
+
+// Shell charges, atomic dipoles/quadrupoles and band energy per atom from the column sums,
+// summed in AO order (one thread per atom; AOs of an atom are contiguous).
+__global__ void k_mulliken_atoms(Batch b, const int* active, const double* cols, double* qsh, double* qat, double* dp,
+                                 double* qp, double* eband) {
+  int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= b.nat) return;
+  int m = b.a.mol[a];
+  if (active && !active[m]) return;
+  double d[3] = {0, 0, 0}, q[6] = {0, 0, 0, 0, 0, 0}, eb = 0.0, qa = 0.0;
+  const Mol M = b.mol[m];
+  for (int s = b.a.ish[a]; s < b.a.ish[a] + b.a.nsh[a]; s++) {
+    double pop = 0.0;
+    int nao = 2 * b.s.l[s] + 1;
+    for (int k = 0; k < nao; k++) {
+      long long g = M.ao0 + b.s.iao[s] + k;
+      const double* c = &cols[11 * g];
+      pop += c[0];
+      for (int x = 0; x < 3; x++) d[x] -= c[1 + x];
+      for (int x = 0; x < 6; x++) q[x] -= c[4 + x];
+      eb += c[10];
+    }
+    qsh[s] = b.s.refocc[s] - pop;
+    qa += qsh[s];
+  }
+  qat[a] = qa;
+  for (int x = 0; x < 3; x++) dp[3 * a + x] = d[x];
+  for (int x = 0; x < 6; x++) qp[6 * a + x] = q[x];
+  eband[a] = eb;
+}
+
+// Electronic energy per atom of the output moments (xtb.rs:1058-1146): ES2 from gq = gamma q_out,
+// ES3, AES (coulomb.rs:668-731), D4 two-body (disp.rs:380-399), -TS / nat and the band energy.
+// One wavefront per atom.
+__global__ void k_energy_atoms(Batch b, const int* active, const double* mrad, const double* gq, const double* qsh,
+                               const double* qat, const double* dp, const double* qp, const double* gw,
+                               const double* dgw, const double* eband, const double* st, double* eel, double* eterm) {
+  int a = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  int lane = threadIdx.x & 63;
+  if (a >= b.nat) return;
+  int m = b.a.mol[a];
+  if (active && !active[m]) return;
+  const Mol M = b.mol[m];
+  const double* xa = &b.a.xyz[3 * a];
+  double eaes = 0.0, ed4 = 0.0;
+  for (int j = M.at0 + lane; j < M.at0 + M.nat; j += 64) {
+    if (j == a) continue;
+    const double* xj = &b.a.xyz[3 * j];
+    double v[3] = {xa[0] - xj[0], xa[1] - xj[1], xa[2] - xj[2]};
+    double r2 = dot3(v, v);
+    double g3, g5;
+    aes_g(sqrt(r2), mrad[a], mrad[j], g3, g5);
+    const double *di = &dp[3 * a], *dj = &dp[3 * j];
+    double miv = dot3(di, v), mjv = dot3(dj, v);
+    double en = (qat[a] * mjv - qat[j] * miv) * g3 + g5 * (r2 * dot3(di, dj) - 3.0 * miv * mjv);
+    double tc[6] = {v[0] * v[0], v[0] * v[1], v[1] * v[1], v[0] * v[2], v[1] * v[2], v[2] * v[2]};
+    for (int k = 0; k < 6; k++) en += QSCALE[k] * tc[k] * g5 * (qat[j] * qp[6 * a + k] + qat[a] * qp[6 * j + k]);
+    eaes += 0.5 * en;
+    double pe, pg;
+    if (d4_pair_e(b, a, j, v, r2, pe, pg)) {
+      double c6, x, y;
+      pair_c6(b, a, j, gw, dgw, c6, x, y);
+      ed4 += -(c6 * pe * 0.5);
+    }
+  }
+  eaes = wave_sum(eaes);
+  ed4 = wave_sum(ed4);
+  if (lane == 0) {
+    double es2 = 0.0, es3 = 0.0;
+    for (int s = b.a.ish[a]; s < b.a.ish[a] + b.a.nsh[a]; s++) {
+      es2 += 0.5 * qsh[s] * gq[s];
+      es3 += qsh[s] * qsh[s] * qsh[s] * b.s.gam3[s] / 3.0;
+    }
+    const double* d = &dp[3 * a];
+    eaes += b.a.dkernel[a] * dot3(d, d);
+    double qq = 0.0;
+    for (int k = 0; k < 6; k++) qq += QSCALE[k] * qp[6 * a + k] * qp[6 * a + k];
+    eaes += b.a.qkernel[a] * qq;
+    double ts = st[2 * m + 1];
+    eel[a] = es2 + es3 + eaes + ed4 + ts / M.nat + eband[a];
+    double* t = &eterm[8 * (long long)a];
+    t[0] = es2;
+    t[1] = es3;
+    t[2] = eaes;
+    t[3] = ed4;
+    t[4] = eband[a];
+  }
+}
+
+// Per molecule: the energy (sum of atoms in order), the residual norm of dq, the convergence
+// test (scf/convergence.rs:99-121) and the iteration record. One workgroup per molecule.
+// mst[m*8]: {energy, energy_change, residual, efermi, ts, iter, converged, -}
+__global__ void k_converge(Batch b, int* active, const double* eel, const double* dq, const double* st, double econv,
+                           double pconv, double* mst, int* status) {
+  __shared__ double red[16];
+  int m = blockIdx.x;
+  if (!active[m]) return;
+  const Mol M = b.mol[m];
+  double e = 0.0;
+  if (threadIdx.x == 0)
+    for (int a = M.at0; a < M.at0 + M.nat; a++) e += eel[a];
+  double r = 0.0;
+  for (int k = threadIdx.x; k < M.ndim; k += blockDim.x) {
+    double x = dq[M.vec0 + k];
+    r += x * x;
+  }
+  r = block_sum(r, red);
+  if (threadIdx.x == 0) {
+    double* s = &mst[8 * m];
+    double res = sqrt(r / M.ndim);
+    double de = e - s[0];
+    s[0] = e;
+    s[1] = de;
+    s[2] = res;
+    s[3] = st[2 * m];
+    s[4] = st[2 * m + 1];
+    s[5] += 1.0;
+    if (!isfinite(e)) {
+      atomicOr(&status[m], 4);
+      active[m] = 0;
+    } else if (fabs(de) < econv && res < pconv) {
+      s[6] = 1.0;
+      active[m] = 0;
+    } else if (status[m]) {
+      active[m] = 0;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ Broyden mixing
+// Modified Broyden mixing of one molecule per workgroup (scf/mixing.rs:251-338, omega0 0.01,
+// minw 1, maxw 1e5, wfac 0.01), history of `cap` slots (the reference's `memory`), the small
+// system solved by Gaussian elimination with the last-maximum pivot (linalg/lu.rs:81-107).
+// State per molecule: q_in, dq, qlast, dqlast (vectors at vec0), df/u history (cap*ndim at hist0),
+// omega[cap], a[cap*cap] at hs0 = m*(cap+cap*cap), iteration count it[m].
+__global__ void k_broyden(Batch b, const int* active, int cap, double damp, double* qin, const double* dq,
+                          double* qlast, double* dqlast, double* df, double* u, double* hs, int* itc, int* status) {
+  extern __shared__ double sm[];
+  int m = blockIdx.x;
+  if (!active[m]) return;
+  const Mol M = b.mol[m];
+  const int n = M.ndim;
+  double* q = qin + M.vec0;
+  const double* d = dq + M.vec0;
+  double* ql = qlast + M.vec0;
+  double* dl = dqlast + M.vec0;
+  double* omega = hs + (long long)m * (cap + cap * cap);
+  double* amat = omega + cap;
+  double* red = sm;                 // 16
+  double* cvec = sm + 16;           // cap
+  double* beta = cvec + cap;        // cap*cap
+  int* hist = (int*)(beta + cap * cap);
+  __shared__ int it_sh, ok_sh;
+  if (threadIdx.x == 0) it_sh = ++itc[m];
+  __syncthreads();
+  const int iter = it_sh;
+  if (iter == 1) {
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+      dl[k] = d[k];
+      ql[k] = q[k];
+      q[k] += damp * d[k];
+    }
+    return;
+  }
+  const int itn = iter - 1;
+  const int it1 = (itn - 1) % cap;
+  const int nhist = min(cap, itn);
+  const int first = itn + 1 > cap ? itn + 1 - cap : 1;
+  if (threadIdx.x < nhist) hist[threadIdx.x] = (first + threadIdx.x - 1) % cap;
+  double* dfi = df + M.hist0 + (long long)it1 * n;
+  double* ui = u + M.hist0 + (long long)it1 * n;
+  double nn = 0.0;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) nn += d[k] * d[k];
+  nn = block_sum(nn, red);
+  double norm = sqrt(nn);
+  double om = norm > 0.01 / 100000.0 ? 0.01 / norm : 100000.0;
+  if (om < 1.0) om = 1.0;
+  double ff = 0.0;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    double x = d[k] - dl[k];
+    dfi[k] = x;
+    ff += x * x;
+  }
+  ff = block_sum(ff, red);
+  double inv = 1.0 / fmax(sqrt(ff), 2.220446049250313e-16);
+  for (int k = threadIdx.x; k < n; k += blockDim.x) dfi[k] *= inv;
+  if (threadIdx.x == 0) omega[it1] = om;
+  __syncthreads();
+  // a(i, it1) for the history slots and c
+  for (int ih = 0; ih < nhist; ih++) {
+    int i = hist[ih];
+    const double* dfh = df + M.hist0 + (long long)i * n;
+    double s1 = 0.0, s2 = 0.0;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+      s1 += dfh[k] * d[k];
+      s2 += dfh[k] * dfi[k];
+    }
+    s1 = block_sum(s1, red);
+    s2 = block_sum(s2, red);
+    if (threadIdx.x == 0) {
+      cvec[ih] = omega[i] * s1;
+      amat[i * cap + it1] = s2;
+      amat[it1 * cap + i] = s2;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    for (int ih = 0; ih < nhist; ih++) {
+      for (int jh = 0; jh < nhist; jh++)
+        beta[ih * cap + jh] = omega[hist[ih]] * omega[hist[jh]] * amat[hist[ih] * cap + hist[jh]];
+      beta[ih * cap + ih] += 0.01 * 0.01;
+    }
+    // Gaussian elimination, pivot on the last maximum of |beta(i, col)|
+    int okv = 1;
+    for (int col = 0; col < nhist && okv; col++) {
+      int piv = col;
+      for (int i = col; i < nhist; i++)
+        if (fabs(beta[i * cap + col]) >= fabs(beta[piv * cap + col])) piv = i;
+      if (beta[piv * cap + col] == 0.0) {
+        okv = 0;
+        break;
+      }
+      if (piv != col) {
+        for (int k = 0; k < nhist; k++) {
+          double t = beta[col * cap + k];
+          beta[col * cap + k] = beta[piv * cap + k];
+          beta[piv * cap + k] = t;
+        }
+        double t = cvec[col];
+        cvec[col] = cvec[piv];
+        cvec[piv] = t;
+      }
+      for (int row = col + 1; row < nhist; row++) {
+        double f = beta[row * cap + col] / beta[col * cap + col];
+        if (f != 0.0) {
+          for (int k = col; k < nhist; k++) beta[row * cap + k] -= f * beta[col * cap + k];
+          cvec[row] -= f * cvec[col];
+        }
+      }
+    }
+    if (okv)
+      for (int col = nhist - 1; col >= 0; col--) {
+        double s = cvec[col];
+        for (int k = col + 1; k < nhist; k++) s -= beta[col * cap + k] * cvec[k];
+        cvec[col] = s / beta[col * cap + col];
+      }
+    ok_sh = okv;
+    if (!okv) atomicOr(&status[m], 8);
+  }
+  __syncthreads();
+  if (!ok_sh) return;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    double uk = damp * dfi[k] + inv * (q[k] - ql[k]);
+    ui[k] = uk;
+    dl[k] = d[k];
+    ql[k] = q[k];
+    double x = q[k] + damp * d[k];
+    for (int ih = 0; ih < nhist; ih++) {
+      int i = hist[ih];
+      double uv = (i == it1) ? uk : u[M.hist0 + (long long)i * n + k];
+      x -= omega[i] * cvec[ih] * uv;
+    }
+    q[k] = x;
+  }
+}
+
+// ------------------------------------------------------------------ gradient
+// All pair gradients of atom a (one wavefront per atom; every pair is evaluated from both atoms,
+// each keeping its own share): ES2 (coulomb.rs:123-169), AES with dE/dCN through the damping
+// radii (coulomb.rs:896-1001), D4 two-body at fixed charges with dE/dCN_D4 (disp.rs:460-484), and
+// the H0/S/multipole integral derivatives contracted with P, W and the potentials, with
+// dE/dCN of the levels (hamiltonian.rs:137-223).
+__global__ void k_pair_grad(Batch b, const int* okm, const double* pool, long long stride, const double* P,
+                            const double* W, const double* se, const double* dsedcn, const double* vsh,
+                            const double* vat, const double* vdp, const double* vqp, const double* qsh,
+                            const double* qat, const double* dp, const double* qp, const double* mrad,
+                            const double* dmrdcn, const double* gw, const double* dgwdcn, const double* dcn3, double* g,
+                            double* dedcn, double* dedcn_d4) {
+  int a = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  int lane = threadIdx.x & 63;
+  if (a >= b.nat) return;
+  int m = b.a.mol[a];
+  if (okm && !okm[m]) return;
+  const Mol M = b.mol[m];
+  double ga[3] = {0, 0, 0}, dcn = 0.0, dcn4 = 0.0, dedr = 0.0;
+  const double kdmp3 = 3.0, kdmp5 = 4.0;
+  for (int j = M.at0 + lane; j < M.at0 + M.nat; j += 64) {
+    if (j == a) continue;
+    const int iat = a > j ? a : j, jat = a > j ? j : a;
+    const double sgn = a == iat ? 1.0 : -1.0;
+    const double* xi = &b.a.xyz[3 * iat];
+    const double* xj = &b.a.xyz[3 * jat];
+    double v[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};  // R_iat - R_jat
+    double r2 = dot3(v, v), r1 = sqrt(r2);
+    // ES2
+    {
+      double dgs = 0.0;
+      for (int s = b.a.ish[iat]; s < b.a.ish[iat] + b.a.nsh[iat]; s++)
+        for (int t = b.a.ish[jat]; t < b.a.ish[jat] + b.a.nsh[jat]; t++) {
+          double eta = 0.5 * (b.s.hard[s] + b.s.hard[t]);
+          double dt = 1.0 / (r2 + 1.0 / (eta * eta));
+          dgs += -dt * sqrt(dt) * qsh[s] * qsh[t];
+        }
+      for (int k = 0; k < 3; k++) ga[k] += sgn * dgs * v[k];
+    }
+    // AES (vec = R_jat - R_iat)
+    {
+      double vec[3] = {-v[0], -v[1], -v[2]};
+      double rr = 0.5 * (mrad[iat] + mrad[jat]);
+      double g1 = 1.0 / r1, g3 = g1 * g1 * g1, g5 = g3 * g1 * g1, g7 = g5 * g1 * g1;
+      double x = rr * g1, x3 = x * x * x, x4 = x3 * x;
+      double fdmp3 = 1.0 / (1.0 + 6.0 * x3);
+      double ddmp3 = -3.0 * g5 * fdmp3 - kdmp3 * fdmp3 * (fdmp3 - 1.0) * g5;
+      double fdmp5 = 1.0 / (1.0 + 6.0 * x4);
+      double ddmp5 = -5.0 * fdmp5 - kdmp5 * (fdmp5 * fdmp5 - fdmp5);
+      double qi = qat[iat], qj = qat[jat];
+      const double *di = &dp[3 * iat], *dj = &dp[3 * jat];
+      double dpiqj = dot3(vec, di) * qj, qidpj = dot3(vec, dj) * qi;
+      double fddr = 3.0 * (dpiqj - qidpj) * kdmp3 * fdmp3 * g3 * (fdmp3 / rr) * x3;
+      double dg[3];
+      for (int k = 0; k < 3; k++) dg[k] = -ddmp3 * vec[k] * (dpiqj - qidpj) + fdmp3 * g3 * (qi * dj[k] - qj * di[k]);
+      double dpidpj = dot3(dj, di), dpiv = dot3(di, vec), dpjv = dot3(dj, vec);
+      double edd = dpidpj * r2 - 3.0 * dpjv * dpiv;
+      fddr += 3.0 * edd * kdmp5 * fdmp5 * g5 * (fdmp5 / rr) * x4;
+      for (int k = 0; k < 3; k++)
+        dg[k] += -2.0 * fdmp5 * g5 * dpidpj * vec[k] + 3.0 * fdmp5 * g5 * (dpiv * dj[k] + dpjv * di[k]) -
+                 edd * ddmp5 * g7 * vec[k];
+      const double *ti = &qp[6 * iat], *tj = &qp[6 * jat];
+      double eq = 2.0 * (qj * ti[1] + tj[1] * qi) * vec[0] * vec[1] + 2.0 * (qj * ti[3] + tj[3] * qi) * vec[0] * vec[2] +
+                  2.0 * (qj * ti[4] + tj[4] * qi) * vec[1] * vec[2] + (qj * ti[0] + tj[0] * qi) * vec[0] * vec[0] +
+                  (qj * ti[2] + tj[2] * qi) * vec[1] * vec[1] + (qj * ti[5] + tj[5] * qi) * vec[2] * vec[2];
+      fddr += eq * 3.0 * kdmp5 * fdmp5 * g5 * fdmp5 / rr * x4;
+      double tvj[3] = {vec[0] * tj[0] + vec[1] * tj[1] + vec[2] * tj[3], vec[0] * tj[1] + vec[1] * tj[2] + vec[2] * tj[4],
+                       vec[0] * tj[3] + vec[1] * tj[4] + vec[2] * tj[5]};
+      double tvi[3] = {vec[0] * ti[0] + vec[1] * ti[1] + vec[2] * ti[3], vec[0] * ti[1] + vec[1] * ti[2] + vec[2] * ti[4],
+                       vec[0] * ti[3] + vec[1] * ti[4] + vec[2] * ti[5]};
+      for (int k = 0; k < 3; k++)
+        dg[k] += -eq * ddmp5 * g7 * vec[k] - 2.0 * fdmp5 * g5 * qi * tvj[k] - 2.0 * fdmp5 * g5 * qj * tvi[k];
+      for (int k = 0; k < 3; k++) ga[k] += sgn * dg[k];
+      dedr += fddr;
+    }
+    // D4 two-body at fixed charges
+    {
+      double pe, pg;
+      if (d4_pair_e(b, iat, jat, v, r2, pe, pg)) {
+        double c6, dci, dcj;
+        pair_c6(b, iat, jat, gw, dgwdcn, c6, dci, dcj);
+        dcn4 -= (a == iat ? dci : dcj) * pe;
+        for (int k = 0; k < 3; k++) ga[k] += sgn * (-c6 * pg * v[k]);
+      }
+    }
+    // H0 / S / multipole integral derivatives over the screened shell pairs
+    {
+      double amin_i = 1e300, amin_j = 1e300;
+      for (int s = b.a.ish[iat]; s < b.a.ish[iat] + b.a.nsh[iat]; s++) amin_i = fmin(amin_i, b.s.minalpha[s]);
+      for (int s = b.a.ish[jat]; s < b.a.ish[jat] + b.a.nsh[jat]; s++) amin_j = fmin(amin_j, b.s.minalpha[s]);
+      if (!(amin_i * amin_j * r2 / (amin_i + amin_j) > b.intcut * (1.0 + 1e-10)) && r2 >= 2.220446049250313e-16) {
+        double rr = sqrt(r1 / (b.a.rad[jat] + b.a.rad[iat]));
+        double dg[3] = {0, 0, 0}, dcni = 0.0, dcnj = 0.0;
+        for (int si = b.a.ish[iat]; si < b.a.ish[iat] + b.a.nsh[iat]; si++)
+          for (int sj = b.a.ish[jat]; sj < b.a.ish[jat] + b.a.nsh[jat]; sj++) {
+            double a1 = b.s.minalpha[si], a2 = b.s.minalpha[sj];
+            if (a1 * a2 * r2 / (a1 + a2) > b.intcut * (1.0 + 1e-10)) continue;
+            double shpi = 1.0 + b.s.shpoly[si] * rr, shpj = 1.0 + b.s.shpoly[sj] * rr;
+            double shp = shpi * shpj;
+            double dshp = (shpi * b.s.shpoly[sj] + shpj * b.s.shpoly[si]) * 0.5 * rr / r2;
+            double dsv[3] = {dshp / shp * v[0], dshp / shp * v[1], dshp / shp * v[2]};
+            double hs = hscale(b, sj, si) * shp;
+            double hij = 0.5 * (se[si] + se[sj]) * hs;
+            int li = b.s.l[si], lj = b.s.l[sj];
+            int nsi = 2 * li + 1, nsj = 2 * lj + 1;
+            GradCoef cf;
+            for (int im = 0; im < nsi; im++)
+              for (int jm = 0; jm < nsj; jm++) {
+                int blk = jm + nsj * im;
+                long long mu = b.s.iao[sj] + jm, nu = b.s.iao[si] + im;
+                long long e = M.mat0 + mu + nu * M.nao;
+                double pij = P[e];
+                double vmu = vsh[b.ao_sh[M.ao0 + mu]] + vat[b.ao_at[M.ao0 + mu]];
+                double vnu = vsh[b.ao_sh[M.ao0 + nu]] + vat[b.ao_at[M.ao0 + nu]];
+                cf.p[blk] = pij;
+                cf.hp2[blk] = 2.0 * pij * hij;
+                cf.sval[blk] = -pij * (vmu + vnu) + 2.0 * pij * hij - 2.0 * W[e];
+              }
+            double ps = shellpair_grad(li, b.s.nprim[si], &b.s.alpha[MAXPRIM * si], &b.s.coeff[MAXPRIM * si], lj,
+                                       b.s.nprim[sj], &b.s.alpha[MAXPRIM * sj], &b.s.coeff[MAXPRIM * sj], v, b.intcut,
+                                       cf, dsv, &vdp[3 * iat], &vdp[3 * jat], &vqp[6 * iat], &vqp[6 * jat], dg);
+            dcni += dsedcn[si] * hs * ps;
+            dcnj += dsedcn[sj] * hs * ps;
+          }
+        for (int k = 0; k < 3; k++) ga[k] += sgn * dg[k];
+        dcn += a == iat ? dcni : dcnj;
+      }
+    }
+  }
+  for (int k = 0; k < 3; k++) ga[k] = wave_sum(ga[k]);
+  dcn = wave_sum(dcn);
+  dcn4 = wave_sum(dcn4);
+  dedr = wave_sum(dedr);
+  if (lane == 0) {
+    // on-site: the CN dependence of the levels (hamiltonian.rs:212-221)
+    for (int s = b.a.ish[a]; s < b.a.ish[a] + b.a.nsh[a]; s++) {
+      int n = 2 * b.s.l[s] + 1;
+      for (int k = 0; k < n; k++) {
+        long long mu = b.s.iao[s] + k;
+        dcn += dsedcn[s] * P[M.mat0 + mu + mu * M.nao];
+      }
+    }
+    dcn += dedr * dmrdcn[a];
+    for (int k = 0; k < 3; k++) g[3 * a + k] += ga[k];
+    dedcn[a] = dcn;
+    dedcn_d4[a] = dcn4 + dcn3[a];
+  }
+}
+
+// Chain rules of the two coordination numbers (base/src/ncoord.rs:486-519): the GFN2 CN with
+// dE/dCN from the levels and the AES radii, and the D4 CN (Jacobian order, zero weights skipped)
+// with dE/dCN of the two- and three-body dispersion. One wavefront per atom.
+__global__ void k_cn_chain(Batch b, const int* okm, const double* dedcn, const double* dedcn_d4, double* g) {
+  int a = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  int lane = threadIdx.x & 63;
+  if (a >= b.nat) return;
+  int m = b.a.mol[a];
+  if (okm && !okm[m]) return;
+  const Mol M = b.mol[m];
+  const double* xa = &b.a.xyz[3 * a];
+  double ga[3] = {0, 0, 0};
+  for (int j = M.at0 + lane; j < M.at0 + M.nat; j += 64) {
+    if (j == a) continue;
+    const double* xj = &b.a.xyz[3 * j];
+    double v[3] = {xa[0] - xj[0], xa[1] - xj[1], xa[2] - xj[2]};
+    double r2 = dot3(v, v);
+    if (r2 > 1600.0 || r2 < 1e-12) continue;
+    double r = sqrt(r2);
+    double t = dexp_dcount(r, b.a.rcov_cn[a] + b.a.rcov_cn[j]) * (dedcn[a] + dedcn[j]) / r;
+    double s4 = dedcn_d4[a] + dedcn_d4[j];
+    double t4 = s4 != 0.0 ? d4_en_factor(b.a.en[a], b.a.en[j]) * erf_dcount(r, b.a.d4_rcov[a] + b.a.d4_rcov[j]) * s4 / r
+                          : 0.0;
+    for (int k = 0; k < 3; k++) ga[k] += (t + t4) * v[k];
+  }
+  for (int k = 0; k < 3; k++) ga[k] = wave_sum(ga[k]);
+  if (lane == 0)
+    for (int k = 0; k < 3; k++) g[3 * a + k] += ga[k];
+}
+
+}  // namespace axtb

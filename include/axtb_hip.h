@@ -1,0 +1,89 @@
+/* axtb-hip: GFN2-xTB energy + analytic gradient on AMD MI355X (gfx950), C ABI.
+ *
+ * The drop-in boundary of axtb's GFN2 calculator. Each entry point replaces one call of the
+ * reference's Rust API (paths relative to the axtb checkout):
+ *
+ *   axtbhip_gfn2_engrad        Method::singlepoint(positions, Want::GRADIENT)
+ *                              (crates/base/src/method/mod.rs:62, implemented for GFN2 by
+ *                              axtb::Calculator, crates/axtb/src/calculator.rs:378-478, which runs
+ *                              XtbModel::singlepoint_state, crates/axtb/src/xtb.rs:437-861)
+ *   axtbhip_gfn2_engrad_batch  many independent Method::singlepoint calls, as
+ *                              Session.compute_many does (crates/axtb-py/src/engine.rs:844)
+ *   axtbhip_basis_size         the AO/shell counts of axtb::basis::Basis (crates/axtb/src/basis.rs:165-229)
+ *
+ * Conventions: positions in bohr, row-major [nat][3]; energies in Hartree, gradients in
+ * Hartree/bohr; axtb's default numerical conventions (crates/axtb/src/conventions.rs:244-268).
+ * All arrays are caller-owned; result pointers may be NULL to skip an output.
+ * Status codes mirror axtb_base::Error (crates/base/src/error.rs:8-24).
+ */
+#ifndef AXTB_HIP_H
+#define AXTB_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AXTB_OK 0
+#define AXTB_ERR_INPUT 1               /* Error::Input */
+#define AXTB_ERR_NUMERICAL 2           /* Error::Numerical */
+#define AXTB_ERR_SCF_NOT_CONVERGED 3   /* Error::ScfNotConverged */
+#define AXTB_ERR_UNSUPPORTED_ELEMENT 4 /* Error::UnsupportedElement */
+#define AXTB_ERR_DEVICE 5              /* a HIP runtime failure (no counterpart in the reference) */
+
+/* A molecule: Molecule::new(numbers, positions, charge, uhf) (crates/base/src/molecule.rs:35-64). */
+typedef struct axtb_mol {
+  int nat;
+  const int* numbers;      /* atomic numbers [nat] */
+  const double* positions; /* bohr [nat][3] */
+  double charge;           /* molecular charge (integer valued) */
+  int uhf;                 /* unpaired electrons; -1: the lowest spin of the electron count */
+} axtb_mol;
+
+/* Options of one calculation: axtb::Options / ScfOptions (crates/axtb/src/lib.rs:92-153,
+ * crates/axtb/src/xtb.rs:65-99). */
+typedef struct axtb_opts {
+  double accuracy;   /* SCF accuracy: energy threshold 1e-6*acc, residual 2e-5*acc (default 1) */
+  double etemp;      /* electronic temperature in K (default 300) */
+  int max_iter;      /* SCF iterations (default 250) */
+  double damping;    /* Broyden damping (default 0.4) */
+  int guess;         /* 0: EEQ charges (default), 1: neutral atoms */
+  int want_gradient; /* 1: analytic gradient */
+  int precision;     /* 0: FP64 throughout */
+  int dispersion;    /* 1: D4 (default) */
+} axtb_opts;
+
+/* Results: the SinglePoint fields of the path (crates/base/src/method/results.rs:39-230). */
+typedef struct axtb_result {
+  double energy;
+  double* gradient;         /* [nat][3] or NULL */
+  double* charges;          /* [nat] or NULL */
+  double* shell_charges;    /* [nsh] or NULL */
+  double* dipoles;          /* atomic dipoles [nat][3] or NULL */
+  double* quadrupoles;      /* atomic traceless quadrupoles [nat][6] (xx xy yy xz yz zz) or NULL */
+  double* orbital_energies; /* [nao] or NULL */
+  /* repulsion, dispersion, electronic, isotropic_es, third_order, anisotropic, entropy, band */
+  double terms[8];
+  double efermi, residual, energy_change;
+  int iterations, status, nsh, nao;
+} axtb_result;
+
+typedef struct axtbhip_context axtbhip_context;
+
+const char* axtbhip_version(void);
+/* One context per host thread and device (it owns a HIP stream and its device buffers). */
+axtbhip_context* axtbhip_context_create(int device, char* err, size_t errlen);
+void axtbhip_context_destroy(axtbhip_context* ctx);
+int axtbhip_last_error(axtbhip_context* ctx, char* buf, size_t len);
+int axtbhip_basis_size(const axtb_mol* mol, int* nsh, int* nao);
+int axtbhip_gfn2_engrad(axtbhip_context* ctx, const axtb_mol* mol, const axtb_opts* opts, axtb_result* out);
+/* Returns 0 if every molecule succeeded, else the status of the first that failed; each
+ * out[i].status holds the molecule's own status. */
+int axtbhip_gfn2_engrad_batch(axtbhip_context* ctx, const axtb_mol* mols, int nmol, const axtb_opts* opts,
+                              axtb_result* outs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -116,3 +116,23 @@ def eigh(a):
     w = np.zeros(n)
     _check(lib().oracle_eigh(n, _p(buf), _p(w)))
     return w, buf.T.copy()
+
+
+def debug(numbers, positions, charge=0.0, acc=1.0):
+    """Setup and first-iteration intermediates of the oracle (S, H0, D, Q, cn, qstart, potentials,
+    F, eigenvalues, P, output shell charges)."""
+    z = np.ascontiguousarray(numbers, dtype=np.int32)
+    x = np.ascontiguousarray(positions, dtype=np.float64)
+    nat = len(z)
+    nsh, nao = sizes(z)
+    nn = nao * nao
+    o = {k: np.zeros(v) for k, v in dict(S=nn, H0=nn, D=3 * nn, Q=6 * nn, cn=nat, qstart=nsh, vsh=nsh, vat=nat,
+                                          vdp=3 * nat, vqp=6 * nat, F=nn, w=nao, P=nn, qsh=nsh).items()}
+    _check(lib().oracle_debug(nat, _p(z), _p(x), ctypes.c_double(charge), ctypes.c_double(acc),
+                              *[_p(o[k]) for k in ("S", "H0", "D", "Q", "cn", "qstart", "vsh", "vat", "vdp", "vqp",
+                                                   "F", "w", "P", "qsh")]))
+    for k in ("S", "H0", "F", "P"):
+        o[k] = o[k].reshape(nao, nao).T.copy()
+    o["D"] = o["D"].reshape(3, nao, nao).transpose(0, 2, 1).copy()
+    o["Q"] = o["Q"].reshape(6, nao, nao).transpose(0, 2, 1).copy()
+    return o

@@ -1587,6 +1587,15 @@ struct Result {
   double efermi = 0, residual = 0, energy_change = 0;
 };
 
+struct DebugOut {
+  double *S, *H0, *D, *Q, *cn, *qstart, *vsh, *vat, *vdp, *vqp, *F, *w, *P, *qsh;
+};
+DebugOut debug_out;
+bool debug_on = false;
+inline void dbg_copy(double* dst, const double* src, size_t n) {
+  if (debug_on && dst) std::memcpy(dst, src, n * sizeof(double));
+}
+
 const double SQRT2 = 1.4142135623730951;
 const bool QOFF[6] = {false, true, false, true, true, false};
 
@@ -1634,6 +1643,14 @@ Result singlepoint(int nat, const int* z, const double* xyz, double charge, cons
   for (int i = 0; i < nat; i++) energies[i] += erep[i];
 
   Ints I = build_integrals(m, se);
+  {
+    size_t nn = (size_t)m.nao * m.nao;
+    dbg_copy(debug_out.S, I.S.a.data(), nn);
+    dbg_copy(debug_out.H0, I.H0.a.data(), nn);
+    for (int k = 0; k < 3; k++) dbg_copy(debug_out.D ? debug_out.D + k * nn : nullptr, I.D[k].a.data(), nn);
+    for (int k = 0; k < 6; k++) dbg_copy(debug_out.Q ? debug_out.Q + k * nn : nullptr, I.Q[k].a.data(), nn);
+    dbg_copy(debug_out.cn, cn.data(), nat);
+  }
   // Cholesky S = L L^T (eigen.rs:72-113)
   Mat L = I.S;
   for (int j = 0; j < n; j++) {
@@ -1698,6 +1715,7 @@ Result singlepoint(int nat, const int* z, const double* xyz, double charge, cons
     }
   }
   qat_of(start);
+  dbg_copy(debug_out.qstart, start.qsh.data(), nsh);
 
   Moments st = start, out = start;
   Pots pot;
@@ -1745,6 +1763,13 @@ Result singlepoint(int nat, const int* z, const double* xyz, double charge, cons
         if (!std::isfinite(F(mu, nu))) throw Error(2, "non-finite Fock matrix");
       }
     pack(st, mix.q_in);
+    if (iter == 1) {
+      dbg_copy(debug_out.vsh, pot.vsh.data(), nsh);
+      dbg_copy(debug_out.vat, pot.vat.data(), nat);
+      dbg_copy(debug_out.vdp, pot.vdp.data(), 3 * nat);
+      dbg_copy(debug_out.vqp, pot.vqp.data(), 6 * nat);
+      dbg_copy(debug_out.F, F.a.data(), (size_t)n * n);
+    }
     // A = Linv F Linv^T, eig, C = Linv^T U
     Mat X(n, n);
     for (int j = 0; j < n; j++)
@@ -1820,6 +1845,11 @@ Result singlepoint(int nat, const int* z, const double* xyz, double charge, cons
       band += r[10];
     }
     qat_of(out);
+    if (iter == 1) {
+      dbg_copy(debug_out.w, emo.data(), n);
+      dbg_copy(debug_out.P, P.a.data(), (size_t)n * n);
+      dbg_copy(debug_out.qsh, out.qsh.data(), nsh);
+    }
     pack(out, packed);
     mix.set_residual(packed);
     // energy of the output moments (xtb.rs:1058-1146)
@@ -2098,6 +2128,14 @@ int oracle_overlap(int nat, const int* z, const double* xyz, double acc, double*
     vec cn = gfn2_cn(m);
     vec se(m.nsh, 0.0);
     Ints I = build_integrals(m, se);
+  {
+    size_t nn = (size_t)m.nao * m.nao;
+    dbg_copy(debug_out.S, I.S.a.data(), nn);
+    dbg_copy(debug_out.H0, I.H0.a.data(), nn);
+    for (int k = 0; k < 3; k++) dbg_copy(debug_out.D ? debug_out.D + k * nn : nullptr, I.D[k].a.data(), nn);
+    for (int k = 0; k < 6; k++) dbg_copy(debug_out.Q ? debug_out.Q + k * nn : nullptr, I.Q[k].a.data(), nn);
+    dbg_copy(debug_out.cn, cn.data(), nat);
+  }
     if (S) std::memcpy(S, I.S.a.data(), sizeof(double) * m.nao * m.nao);
     if (cn_out) std::memcpy(cn_out, cn.data(), sizeof(double) * nat);
     return 0;
@@ -2117,6 +2155,33 @@ int oracle_eigh(int n, double* a, double* w) {
     std::memcpy(w, W.data(), sizeof(double) * n);
     return 0;
   } catch (Error& e) {
+    last_error = e.what();
+    return e.code;
+  }
+}
+
+
+// Setup and first SCC iteration of the oracle for kernel-level parity checks: S, H0 (nao^2
+// column-major), D (3 nao^2), Q (6 nao^2), GFN2 CN, start shell charges, then the first iteration's
+// potentials (vsh, vat, vdp, vqp), Fock matrix F, eigenvalues w, density P and output shell charges.
+int oracle_debug(int nat, const int* z, const double* xyz, double charge, double acc, double* S, double* H0,
+                 double* D, double* Q, double* cn_out, double* qstart, double* vsh, double* vat, double* vdp,
+                 double* vqp, double* F, double* w, double* P, double* qsh_out) {
+  try {
+    Options opt;
+    opt.acc = acc;
+    opt.max_iter = 1;
+    debug_out = DebugOut{S, H0, D, Q, cn_out, qstart, vsh, vat, vdp, vqp, F, w, P, qsh_out};
+    debug_on = true;
+    try {
+      singlepoint(nat, z, xyz, charge, opt);
+    } catch (Error& e) {
+      if (e.code != 3) throw;
+    }
+    debug_on = false;
+    return 0;
+  } catch (Error& e) {
+    debug_on = false;
     last_error = e.what();
     return e.code;
   }
