@@ -15,7 +15,9 @@ OK, ERR_INPUT, ERR_NUMERICAL, ERR_SCF, ERR_ELEMENT, ERR_DEVICE = range(6)
 ERROR_KINDS = {1: "input", 2: "numerical", 3: "scf_not_converged", 4: "unsupported_element", 5: "device"}
 TERMS = ("repulsion", "dispersion", "electronic", "isotropic_es", "third_order", "anisotropic", "entropy", "band")
 EXPORTS = ("axtbhip_version", "axtbhip_context_create", "axtbhip_context_destroy", "axtbhip_last_error",
-           "axtbhip_basis_size", "axtbhip_gfn2_engrad", "axtbhip_gfn2_engrad_batch")
+           "axtbhip_basis_size", "axtbhip_gfn2_engrad", "axtbhip_gfn2_engrad_batch", "axtbhip_batch_prepare",
+           "axtbhip_batch_run", "axtbhip_batch_results", "axtbhip_profile_enable", "axtbhip_profile_report",
+           "axtbhip_synchronize")
 
 
 class AxtbError(RuntimeError):
@@ -65,6 +67,13 @@ def lib():
                                           ctypes.POINTER(Result)]
         L.axtbhip_gfn2_engrad_batch.argtypes = [ctypes.c_void_p, ctypes.POINTER(Mol), ctypes.c_int,
                                                 ctypes.POINTER(Opts), ctypes.POINTER(Result)]
+        L.axtbhip_batch_prepare.argtypes = [ctypes.c_void_p, ctypes.POINTER(Mol), ctypes.c_int, ctypes.POINTER(Opts)]
+        L.axtbhip_batch_run.argtypes = [ctypes.c_void_p]
+        L.axtbhip_batch_results.argtypes = [ctypes.c_void_p, ctypes.POINTER(Mol), ctypes.c_int,
+                                            ctypes.POINTER(Result)]
+        L.axtbhip_profile_enable.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.axtbhip_profile_report.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]
+        L.axtbhip_synchronize.argtypes = [ctypes.c_void_p]
         _lib = L
     return _lib
 
@@ -102,6 +111,44 @@ class Context:
 
     def engrad_batch(self, molecules, opts, raise_on_error=False):
         """molecules: list of (numbers, positions_bohr[, charge[, uhf]]). Returns a list of dicts."""
+        mols, res, keep, outs = self._marshal(molecules)
+        rc = lib().axtbhip_gfn2_engrad_batch(self._h, mols, len(molecules), ctypes.byref(opts), res)
+        return self._unmarshal(rc, res, outs, opts, raise_on_error)
+
+    def prepare(self, molecules, opts):
+        """Upload a batch once; run() then computes on the resident inputs, results() downloads."""
+        self._prep = self._marshal(molecules) + (opts,)
+        rc = lib().axtbhip_batch_prepare(self._h, self._prep[0], len(molecules), ctypes.byref(opts))
+        if rc:
+            raise AxtbError(rc, self.last_error())
+
+    def run(self):
+        rc = lib().axtbhip_batch_run(self._h)
+        if rc:
+            raise AxtbError(rc, self.last_error())
+
+    def results(self, raise_on_error=False):
+        mols, res, keep, outs, opts = self._prep
+        rc = lib().axtbhip_batch_results(self._h, mols, len(outs), res)
+        return self._unmarshal(rc, res, outs, opts, raise_on_error)
+
+    def profile(self, on=True):
+        lib().axtbhip_profile_enable(self._h, int(on))
+
+    def profile_report(self):
+        """{kernel: (launches, total_ms, work)} of the launches since the last report."""
+        buf = ctypes.create_string_buffer(1 << 16)
+        lib().axtbhip_profile_report(self._h, buf, 1 << 16)
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, cnt, ms, work = line.rsplit(" ", 3)
+            out[name] = (int(cnt), float(ms), float(work))
+        return out
+
+    def synchronize(self):
+        lib().axtbhip_synchronize(self._h)
+
+    def _marshal(self, molecules):
         n = len(molecules)
         mols = (Mol * n)()
         res = (Result * n)()
@@ -123,7 +170,10 @@ class Context:
                 setattr(r, k, v.ctypes.data)
             keep += [z, x]
             outs.append(o)
-        rc = lib().axtbhip_gfn2_engrad_batch(self._h, mols, n, ctypes.byref(opts), res)
+        return mols, res, keep, outs
+
+    def _unmarshal(self, rc, res, outs, opts, raise_on_error):
+        n = len(outs)
         if rc == ERR_DEVICE or (raise_on_error and rc):
             raise AxtbError(rc, self.last_error())
         results = []

@@ -87,6 +87,78 @@ inline unsigned blocks(long long n, int bs) { return (unsigned)((n + bs - 1) / b
 
 }  // namespace
 
+// Kernel timing with HIP events on the engine's stream (enabled per context): every launch
+// through LAUNCH records a start/stop pair under the kernel's name.
+struct Prof {
+  bool on = false;
+  struct Rec {
+    std::string name;
+    hipEvent_t a, b;
+    double work;  // algorithmic flops (GEMM, eigensolver) or HBM bytes (streaming kernels) of the launch
+  };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> pool;
+  hipEvent_t ev() {
+    if (!pool.empty()) {
+      hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    HIP_OK(hipEventCreate(&e));
+    return e;
+  }
+  struct Sum {
+    long n = 0;
+    double ms = 0, work = 0;
+  };
+  // name -> launches, total ms, total work; the recorded events are returned to the pool
+  std::map<std::string, Sum> drain() {
+    std::map<std::string, Sum> m;
+    for (auto& r : recs) {
+      HIP_OK(hipEventSynchronize(r.b));
+      float ms = 0.f;
+      HIP_OK(hipEventElapsedTime(&ms, r.a, r.b));
+      auto& x = m[r.name];
+      x.n++;
+      x.ms += ms;
+      x.work += r.work;
+      pool.push_back(r.a);
+      pool.push_back(r.b);
+    }
+    recs.clear();
+    return m;
+  }
+};
+
+struct ProfScope {
+  Prof* p;
+  hipStream_t s;
+  Prof::Rec r;
+  ProfScope(Prof* p_, hipStream_t s_, const char* name, double work = 0.0) : p(p_), s(s_) {
+    if (p->on) {
+      r.name = name;
+      r.work = work;
+      r.a = p->ev();
+      r.b = p->ev();
+      HIP_OK(hipEventRecord(r.a, s));
+    }
+  }
+  ~ProfScope() {
+    if (p->on) {
+      (void)hipEventRecord(r.b, s);
+      p->recs.push_back(r);
+    }
+  }
+};
+
+#define LAUNCH(kern, grid, block, shm, ...) LAUNCHW(0.0, kern, grid, block, shm, __VA_ARGS__)
+#define LAUNCHW(work, kern, grid, block, shm, ...)                          \
+  do {                                                                      \
+    ProfScope ps_(&prof, st, #kern, work);                                  \
+    hipLaunchKernelGGL(kern, grid, block, shm, st, __VA_ARGS__);            \
+  } while (0)
+
 struct MolIn {
   int nat;
   const int* z;
@@ -107,6 +179,7 @@ struct MolOut {
 struct Engine {
   int device = 0;
   hipStream_t st = nullptr;
+  Prof prof;
   std::string last_error;
   // host mirrors of the batch tables
   std::vector<Mol> mols;
@@ -143,36 +216,45 @@ struct Engine {
     }
     gj.upload(jobs, st);
     dim3 grid(blocks(nn, GT), blocks(mm, GT), (unsigned)jobs.size());
-    hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, st, gj.p);
+    double flops = 0.0;
+    for (auto& j : jobs) flops += 2.0 * j.m * j.n * j.k;
+    LAUNCHW(flops, k_gemm, grid, dim3(256), 0, gj.p);
     HIP_OK(hipGetLastError());
   }
 
   // eigenpairs of every job's symmetric matrix (Jacobi, one workgroup per matrix)
+  // work: the algorithmic flops of a dense eigendecomposition, (4/3) n^3 for the reduction to
+  // tridiagonal form plus 2 n^2 k for the k needed eigenvectors (k = n here)
   void eigh(std::vector<EigJob>& jobs, int max_sweeps = 60) {
     if (jobs.empty()) return;
     int nmax = 0;
-    for (auto& j : jobs) nmax = std::max(nmax, j.n);
+    double flops = 0.0;
+    for (auto& j : jobs) {
+      nmax = std::max(nmax, j.n);
+      flops += 4.0 / 3.0 * j.n * (double)j.n * j.n + 2.0 * j.n * (double)j.n * j.n;
+    }
     ej.upload(jobs, st);
     jstat.ensure(jobs.size());
     bool lds = (size_t)nmax * nmax * 8 <= 96 * 1024;
     size_t shm = jacobi_lds(nmax, lds);
     if (lds) {
       HIP_OK(hipFuncSetAttribute((const void*)k_jacobi<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-      hipLaunchKernelGGL(k_jacobi<true>, dim3(jobs.size()), dim3(256), shm, st, ej.p, max_sweeps, jstat.p);
+      LAUNCHW(flops, k_jacobi<true>, dim3(jobs.size()), dim3(256), shm, ej.p, max_sweeps, jstat.p);
     } else {
       HIP_OK(hipFuncSetAttribute((const void*)k_jacobi<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-      hipLaunchKernelGGL(k_jacobi<false>, dim3(jobs.size()), dim3(256), shm, st, ej.p, max_sweeps, jstat.p);
+      LAUNCHW(flops, k_jacobi<false>, dim3(jobs.size()), dim3(256), shm, ej.p, max_sweeps, jstat.p);
     }
     HIP_OK(hipGetLastError());
   }
 
   void fill(double* p, long long n, double v = 0.0) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_fill, dim3(std::min<long long>(blocks(n, 256), 65535)), dim3(256), 0, st, p, n, v);
+    LAUNCH(k_fill, dim3(std::min<long long>(blocks(n, 256), 65535)), dim3(256), 0, p, n, v);
   }
 
   // ------------------------------------------------------------------------------ one batch
-  void run(const std::vector<MolIn>& in, const axtb_opts& o, std::vector<MolOut>& out) {
+  // validate and upload a batch (per-molecule input errors are reported in `out`)
+  void prepare(const std::vector<MolIn>& in, const axtb_opts& o, std::vector<MolOut>& out) {
     HIP_OK(hipSetDevice(device));
     out.assign(in.size(), MolOut());
     // validation (base/src/molecule.rs:35-64, xtb.rs:447,1159-1212)
@@ -221,14 +303,29 @@ struct Engine {
       }
       use.push_back((int)i);
     }
-    if (use.empty()) return;
+    if (use.empty()) {
+      mols.clear();
+      return;
+    }
     build(in, use, o, out);
-    if (mols.empty()) return;
-    setup(o);
-    scf(o);
-    if (o.want_gradient) gradient();
-    collect(in, use, o, out);
+    opts_ = o;
   }
+
+  // the prepared batch on the device: setup, SCC and gradient (inputs resident in HBM)
+  void compute() {
+    if (mols.empty()) return;
+    setup(opts_);
+    scf(opts_);
+    if (opts_.want_gradient) gradient();
+    HIP_OK(hipStreamSynchronize(st));
+  }
+
+  void run(const std::vector<MolIn>& in, const axtb_opts& o, std::vector<MolOut>& out) {
+    prepare(in, o, out);
+    compute();
+    if (!mols.empty()) collect(o, out);
+  }
+  axtb_opts opts_{};
 
   // ------------------------------------------------------------------------------ tables
   std::vector<int> hz, hsp, hmol, hish, hnsh, hnref, hngw, hsat, hsl, hsnp, hsiao, haosh, haoat;
@@ -534,25 +631,27 @@ struct Engine {
     fill(mst.p, 8LL * NM);
 
     // coordination numbers, levels, AES radii
-    hipLaunchKernelGGL(k_atom_setup, dim3(blocks(NA, 4)), dim3(256), 0, st, b, (const int*)nullptr, cn.p, cneeq.p,
+    LAUNCH(k_atom_setup, dim3(blocks(NA, 4)), dim3(256), 0, b, (const int*)nullptr, cn.p, cneeq.p,
                        cnd4.p, se.p, dsedcn.p, mrad.p, dmrdcn.p);
-    hipLaunchKernelGGL(k_gamma, dim3(blocks((long long)nshmax * nshmax, 256), NM), dim3(256), 0, st, b, gam.p);
+    LAUNCH(k_gamma, dim3(blocks((long long)nshmax * nshmax, 256), NM), dim3(256), 0, b, gam.p);
     // integrals
     fill(pool.p, 11 * tot_mat);
-    hipLaunchKernelGGL(k_integrals, dim3(blocks((long long)nshmax * nshmax, 128), NM), dim3(128), 0, st, b, se.p,
+    double ibytes = 0.0;  // the 11 matrices written once
+    for (auto& M : mols) ibytes += 11.0 * 8.0 * M.nao * (double)M.nao;
+    LAUNCHW(ibytes, k_integrals, dim3(blocks((long long)nshmax * nshmax, 128), NM), dim3(128), 0, b, se.p,
                        pool.p, (long long)tot_mat);
     // repulsion
-    hipLaunchKernelGGL(k_repulsion, dim3(blocks(NA, 4)), dim3(256), 0, st, b, (const int*)nullptr, erep.p, grep.p);
+    LAUNCH(k_repulsion, dim3(blocks(NA, 4)), dim3(256), 0, b, (const int*)nullptr, erep.p, grep.p);
     // D4 three-body at q = 0 (weights with dgw/dCN)
     fill(fwork.p, 0);
     DBuf<double>& zq = eeqy;  // zero charges
     fill(zq.p, NA);
-    hipLaunchKernelGGL(k_d4_weights, dim3(blocks(NA, 128)), dim3(128), 0, st, b, (const int*)nullptr, cnd4.p, zq.p,
+    LAUNCH(k_d4_weights, dim3(blocks(NA, 128)), dim3(128), 0, b, (const int*)nullptr, cnd4.p, zq.p,
                        gw.p, dgw.p, 1);
-    hipLaunchKernelGGL(k_atm_pairs, dim3(blocks((long long)natmax * natmax, 256), NM), dim3(256), 0, st, b, gw.p,
+    LAUNCH(k_atm_pairs, dim3(blocks((long long)natmax * natmax, 256), NM), dim3(256), 0, b, gw.p,
                        dgw.p, atmP.p);
     if (o.dispersion) {
-      hipLaunchKernelGGL(k_atm, dim3(blocks(NA, 4)), dim3(256), 0, st, b, atmP.p, e3.p, g3.p, dcn3.p);
+      LAUNCH(k_atm, dim3(blocks(NA, 4)), dim3(256), 0, b, atmP.p, e3.p, g3.p, dcn3.p);
     } else {
       fill(e3.p, NA);
       fill(g3.p, 3LL * NA);
@@ -561,7 +660,7 @@ struct Engine {
     // start: EEQ charges (or neutral atoms)
     fill(qstart.p, NS);
     if (o.guess == 0) {
-      hipLaunchKernelGGL(k_eeq_system, dim3(blocks((long long)(natmax + 1) * (natmax + 1), 256), NM), dim3(256), 0, st,
+      LAUNCH(k_eeq_system, dim3(blocks((long long)(natmax + 1) * (natmax + 1), 256), NM), dim3(256), 0,
                          b, eq0.p, cneeq.p, eeqA.p, eeqr.p);
       std::vector<EigJob> jobs(NM);
       for (int m = 0; m < NM; m++) {
@@ -569,7 +668,7 @@ struct Engine {
         jobs[m] = EigJob{n1, eeqA.p + heq0[m], eeqV.p + heq0[m], eeqw.p + mols[m].at0 + m, eeqZ.p + heq0[m]};
       }
       eigh(jobs);
-      hipLaunchKernelGGL(k_eeq_solve, dim3(NM), dim3(256), 0, st, b, eq0.p, eeqZ.p, eeqw.p, eeqr.p, eeqy.p, qstart.p,
+      LAUNCH(k_eeq_solve, dim3(NM), dim3(256), 0, b, eq0.p, eeqZ.p, eeqw.p, eeqr.p, eeqy.p, qstart.p,
                          eeqfail.p);
     }
     // Loewdin orthogonaliser X = U s^-1/2 U^T from the eigenpairs of S
@@ -617,7 +716,7 @@ struct Engine {
     // start moments -> q_in (dipoles and quadrupoles zero)
     fill(dp.p, 3LL * NA);
     fill(qp.p, 6LL * NA);
-    hipLaunchKernelGGL(k_pack, dim3(blocks(NA, 128)), dim3(128), 0, st, b, (const int*)nullptr, qstart.p, dp.p, qp.p,
+    LAUNCH(k_pack, dim3(blocks(NA, 128)), dim3(128), 0, b, (const int*)nullptr, qstart.p, dp.p, qp.p,
                        qin.p, (const double*)nullptr, (double*)nullptr);
     double econv = 1e-6 * o.accuracy, pconv = 2e-5 * o.accuracy;
     size_t bshm = (16 + cap + (size_t)cap * cap) * sizeof(double) + cap * sizeof(int);
@@ -648,41 +747,46 @@ struct Engine {
         }
       if (a1.empty()) break;
       if (it > 1)
-        hipLaunchKernelGGL(k_broyden, dim3(NM), dim3(256), bshm, st, b, active.p, cap, o.damping, qin.p, dq.p,
+        LAUNCH(k_broyden, dim3(NM), dim3(256), bshm, b, active.p, cap, o.damping, qin.p, dq.p,
                            qlast.p, dqlast.p, hdf.p, hu.p, hs.p, itc.p, status.p);
-      hipLaunchKernelGGL(k_unpack, dim3(blocks(NA, 128)), dim3(128), 0, st, b, active.p, qin.p, qsh.p, qat.p, dp.p,
+      LAUNCH(k_unpack, dim3(blocks(NA, 128)), dim3(128), 0, b, active.p, qin.p, qsh.p, qat.p, dp.p,
                          qp.p);
       // potentials of the input moments (xtb.rs:316-340)
-      hipLaunchKernelGGL(k_shell_pot, dim3(blocks(NS, 4)), dim3(256), 0, st, b, active.p, gam.p, qsh.p, vsh.p, 1);
-      hipLaunchKernelGGL(k_d4_weights, dim3(blocks(NA, 128)), dim3(128), 0, st, b, active.p, cnd4.p, qat.p, gw.p,
+      LAUNCH(k_shell_pot, dim3(blocks(NS, 4)), dim3(256), 0, b, active.p, gam.p, qsh.p, vsh.p, 1);
+      LAUNCH(k_d4_weights, dim3(blocks(NA, 128)), dim3(128), 0, b, active.p, cnd4.p, qat.p, gw.p,
                          dgw.p, 0);
       if (!o.dispersion) fill(gw.p, MAXREF * (long long)NA);
-      hipLaunchKernelGGL(k_atom_pot, dim3(blocks(NA, 4)), dim3(256), 0, st, b, active.p, mrad.p, qat.p, dp.p, qp.p,
+      LAUNCH(k_atom_pot, dim3(blocks(NA, 4)), dim3(256), 0, b, active.p, mrad.p, qat.p, dp.p, qp.p,
                          gw.p, dgw.p, vat.p, vdp.p, vqp.p);
-      hipLaunchKernelGGL(k_fock, dim3(blocks((long long)naomax * naomax, 256), NM), dim3(256), 0, st, b, active.p,
+      // HBM bytes of the streaming kernels: Fock reads S, H0, D, Q and writes F (12 n^2 doubles);
+      // Mulliken reads P, S, D, Q, H0 (12 n^2 doubles)
+      double sbytes = 0.0;
+      for (int m = 0; m < NM; m++)
+        if (hactive[m]) sbytes += 12.0 * 8.0 * mols[m].nao * (double)mols[m].nao;
+      LAUNCHW(sbytes, k_fock, dim3(blocks((long long)naomax * naomax, 256), NM), dim3(256), 0, b, active.p,
                          pool.p, stride, vsh.p, vat.p, vdp.p, vqp.p, F.p);
       // generalized eigenproblem in the Loewdin basis
       gemm(a1);
       gemm(a2);
       eigh(ea);
       gemm(a3);
-      hipLaunchKernelGGL(k_fermi, dim3(NM), dim3(256), 0, st, b, active.p, w.p, focc.p, fwork.p, fst.p, kocc.p,
+      LAUNCH(k_fermi, dim3(NM), dim3(256), 0, b, active.p, w.p, focc.p, fwork.p, fst.p, kocc.p,
                          status.p);
       gemm(a4);  // P = C diag(f) C^T
-      hipLaunchKernelGGL(k_mulliken_cols, dim3(blocks(NO, 4)), dim3(256), 0, st, b, active.p, pool.p, stride, P.p,
+      LAUNCHW(sbytes, k_mulliken_cols, dim3(blocks(NO, 4)), dim3(256), 0, b, active.p, pool.p, stride, P.p,
                          cols.p);
-      hipLaunchKernelGGL(k_mulliken_atoms, dim3(blocks(NA, 128)), dim3(128), 0, st, b, active.p, cols.p, qsh_o.p,
+      LAUNCH(k_mulliken_atoms, dim3(blocks(NA, 128)), dim3(128), 0, b, active.p, cols.p, qsh_o.p,
                          qat_o.p, dp_o.p, qp_o.p, eband.p);
-      hipLaunchKernelGGL(k_pack, dim3(blocks(NA, 128)), dim3(128), 0, st, b, active.p, qsh_o.p, dp_o.p, qp_o.p,
+      LAUNCH(k_pack, dim3(blocks(NA, 128)), dim3(128), 0, b, active.p, qsh_o.p, dp_o.p, qp_o.p,
                          qout.p, (const double*)qin.p, dq.p);
       // energy of the output moments
-      hipLaunchKernelGGL(k_shell_pot, dim3(blocks(NS, 4)), dim3(256), 0, st, b, active.p, gam.p, qsh_o.p, gq.p, 0);
-      hipLaunchKernelGGL(k_d4_weights, dim3(blocks(NA, 128)), dim3(128), 0, st, b, active.p, cnd4.p, qat_o.p, gw.p,
+      LAUNCH(k_shell_pot, dim3(blocks(NS, 4)), dim3(256), 0, b, active.p, gam.p, qsh_o.p, gq.p, 0);
+      LAUNCH(k_d4_weights, dim3(blocks(NA, 128)), dim3(128), 0, b, active.p, cnd4.p, qat_o.p, gw.p,
                          dgw.p, 0);
       if (!o.dispersion) fill(gw.p, MAXREF * (long long)NA);
-      hipLaunchKernelGGL(k_energy_atoms, dim3(blocks(NA, 4)), dim3(256), 0, st, b, active.p, mrad.p, gq.p, qsh_o.p,
+      LAUNCH(k_energy_atoms, dim3(blocks(NA, 4)), dim3(256), 0, b, active.p, mrad.p, gq.p, qsh_o.p,
                          qat_o.p, dp_o.p, qp_o.p, gw.p, dgw.p, eband.p, fst.p, eel.p, eterm.p);
-      hipLaunchKernelGGL(k_converge, dim3(NM), dim3(256), 0, st, b, active.p, eel.p, dq.p, fst.p, econv, pconv, mst.p,
+      LAUNCH(k_converge, dim3(NM), dim3(256), 0, b, active.p, eel.p, dq.p, fst.p, econv, pconv, mst.p,
                          status.p);
       HIP_OK(hipGetLastError());
       HIP_OK(hipMemcpyAsync(hactive.data(), active.p, sizeof(int) * NM, hipMemcpyDeviceToHost, st));
@@ -734,20 +838,19 @@ struct Engine {
     HIP_OK(hipStreamSynchronize(st));
     for (int i = 0; i < 3 * NA; i++) hgr[i] += hg3[i];
     HIP_OK(hipMemcpyAsync(grad.p, hgr.data(), sizeof(double) * 3 * NA, hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_d4_weights, dim3(blocks(NA, 128)), dim3(128), 0, st, b, (const int*)okd.p, cnd4.p, qat_o.p,
+    LAUNCH(k_d4_weights, dim3(blocks(NA, 128)), dim3(128), 0, b, (const int*)okd.p, cnd4.p, qat_o.p,
                        gw.p, dgw.p, 1);
-    hipLaunchKernelGGL(k_pair_grad, dim3(blocks(NA, 4)), dim3(256), 0, st, b, (const int*)okd.p, pool.p, stride, P.p,
+    LAUNCH(k_pair_grad, dim3(blocks(NA, 4)), dim3(256), 0, b, (const int*)okd.p, pool.p, stride, P.p,
                        Wm.p, se.p, dsedcn.p, vsh.p, vat.p, vdp.p, vqp.p, qsh_o.p, qat_o.p, dp_o.p, qp_o.p, mrad.p,
                        dmrdcn.p, gw.p, dgw.p, dcn3.p, grad.p, dedcn.p, dedcn4.p);
-    hipLaunchKernelGGL(k_cn_chain, dim3(blocks(NA, 4)), dim3(256), 0, st, b, (const int*)okd.p, dedcn.p, dedcn4.p,
+    LAUNCH(k_cn_chain, dim3(blocks(NA, 4)), dim3(256), 0, b, (const int*)okd.p, dedcn.p, dedcn4.p,
                        grad.p);
     HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(st));
   }
 
   // ------------------------------------------------------------------------------ results
-  void collect(const std::vector<MolIn>& in, const std::vector<int>& use, const axtb_opts& o,
-               std::vector<MolOut>& out) {
+  void collect(const axtb_opts& o, std::vector<MolOut>& out) {
     const int NA = nat(), NS = nshell(), NO = naos(), NM = nmol();
     auto get = [&](DBuf<double>& d, size_t n) {
       std::vector<double> v(n);
@@ -819,6 +922,7 @@ using axtb::Engine;
 struct axtbhip_context {
   std::unique_ptr<Engine> eng;
   std::string err;
+  std::vector<axtb::MolOut> prepared;  // per-molecule input errors of the prepared batch
 };
 
 static void fill_result(const axtb::MolOut& r, const axtb_mol& m, axtb_result* out) {
@@ -912,6 +1016,85 @@ int axtbhip_gfn2_engrad_batch(axtbhip_context* ctx, const axtb_mol* mols, int nm
 
 int axtbhip_gfn2_engrad(axtbhip_context* ctx, const axtb_mol* mol, const axtb_opts* opts, axtb_result* out) {
   return axtbhip_gfn2_engrad_batch(ctx, mol, 1, opts, out);
+}
+
+int axtbhip_batch_prepare(axtbhip_context* ctx, const axtb_mol* mols, int nmol, const axtb_opts* opts) {
+  if (!ctx || !mols || !opts || nmol < 0) return AXTB_ERR_INPUT;
+  std::vector<axtb::MolIn> in(nmol);
+  for (int i = 0; i < nmol; i++) in[i] = axtb::MolIn{mols[i].nat, mols[i].numbers, mols[i].positions, mols[i].charge, mols[i].uhf};
+  try {
+    ctx->eng->prepare(in, *opts, ctx->prepared);
+    ctx->err.clear();
+    return 0;
+  } catch (axtb::Fail& f) {
+    ctx->err = f.msg;
+    return f.code;
+  } catch (std::exception& e) {
+    ctx->err = e.what();
+    return AXTB_ERR_DEVICE;
+  }
+}
+
+int axtbhip_batch_run(axtbhip_context* ctx) {
+  if (!ctx) return AXTB_ERR_INPUT;
+  try {
+    ctx->eng->compute();
+    return 0;
+  } catch (axtb::Fail& f) {
+    ctx->err = f.msg;
+    return f.code;
+  } catch (std::exception& e) {
+    ctx->err = e.what();
+    return AXTB_ERR_DEVICE;
+  }
+}
+
+int axtbhip_batch_results(axtbhip_context* ctx, const axtb_mol* mols, int nmol, axtb_result* outs) {
+  if (!ctx || nmol != (int)ctx->prepared.size()) return AXTB_ERR_INPUT;
+  try {
+    std::vector<axtb::MolOut> res = ctx->prepared;
+    if (!ctx->eng->mols.empty()) ctx->eng->collect(ctx->eng->opts_, res);
+    int worst = 0;
+    for (int i = 0; i < nmol; i++) {
+      fill_result(res[i], mols[i], &outs[i]);
+      if (res[i].status && !worst) {
+        worst = res[i].status;
+        ctx->err = res[i].msg;
+      }
+    }
+    return worst;
+  } catch (std::exception& e) {
+    ctx->err = e.what();
+    return AXTB_ERR_DEVICE;
+  }
+}
+
+int axtbhip_synchronize(axtbhip_context* ctx) {
+  if (!ctx) return AXTB_ERR_INPUT;
+  return hipDeviceSynchronize() == hipSuccess ? 0 : AXTB_ERR_DEVICE;
+}
+
+void axtbhip_profile_enable(axtbhip_context* ctx, int on) {
+  if (ctx) {
+    ctx->eng->prof.on = on != 0;
+    if (!on) ctx->eng->prof.drain();
+  }
+}
+
+int axtbhip_profile_report(axtbhip_context* ctx, char* buf, size_t len) {
+  if (!ctx || !buf || !len) return 0;
+  try {
+    std::string out;
+    for (auto& kv : ctx->eng->prof.drain()) {
+      char line[256];
+      snprintf(line, sizeof line, "%s %ld %.6f %.6e\n", kv.first.c_str(), kv.second.n, kv.second.ms, kv.second.work);
+      out += line;
+    }
+    snprintf(buf, len, "%s", out.c_str());
+    return (int)out.size();
+  } catch (std::exception&) {
+    return 0;
+  }
 }
 
 }  // extern "C"

@@ -1,0 +1,151 @@
+#!/usr/bin/env python3
+"""Throughput of GFN2-xTB energy + analytic gradient on one MI355X.
+
+Workload (default): the uneven 4096-molecule batch of the benchmark configuration: per molecule,
+at random from the seed, a water cluster of 3-20 waters on a cubic lattice (2.9 A, random
+orientations) or an all-trans n-alkane C3..C20, with 0.05 A Gaussian coordinate noise; axtb's
+default conventions, SCC energy convergence 1e-8 Eh (accuracy 1e-2), 300 K, D4, EEQ start.
+A step is one energy + gradient pass over the whole batch (setup, SCC with per-molecule
+convergence masks, gradient), inputs resident on the device. Other workloads: --workload
+water1000 / water200 / c100h202 / water20 / c30h62 (one molecule per step).
+
+Prints one JSON line: E+grad/s (value), the roofline of the dominant kernel measured with HIP
+events on the engine's stream during the timed steps, and the CPU oracle's E+grad/s on a
+bounded sample of the same workload as the reported baseline.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_MFMA = 78.6e12   # MI355X dense FP64 matrix (flop/s)
+PEAK_HBM = 8.0e12          # MI355X HBM3E (byte/s)
+STREAMING = {"k_fock", "k_mulliken_cols", "k_integrals"}
+
+
+def workload(name, seed=0):
+    from axtb_amd import structures as S
+    if name == "batch4096":
+        return S.batch(4096, seed=seed)
+    if name.startswith("batch"):
+        return S.batch(int(name[5:]), seed=seed)
+    return [S.config(name, seed=seed)]
+
+
+def cpu_baseline(mols, budget_s, acc):
+    """The CPU oracle (C++ restatement of axtb's FP64 path, one thread) on the first molecules of
+    the workload until `budget_s` seconds of CPU work are spent."""
+    import oracle
+    t0 = time.perf_counter()
+    n = 0
+    for z, x in mols:
+        oracle.engrad(z, x, acc=acc)
+        n += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return n / dt, n, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="batch4096")
+    ap.add_argument("--precision", default="fp64", choices=["fp64"])
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--check", type=int, default=8, help="molecules checked against the oracle after timing")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the batch shards by molecule with no exchange between GPUs; "
+              "multi-GPU runs are not part of this build (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    from axtb_amd import _lib
+    acc = 1e-2
+    mols = workload(args.workload)
+    opts = _lib.make_opts(accuracy=acc, etemp=300.0, precision=args.precision, gradient=True)
+    ctx = _lib.Context(0)
+    ctx.prepare(mols, opts)
+    print(f"[bench] workload {args.workload}: {len(mols)} molecules, {sum(len(m[0]) for m in mols)} atoms", file=sys.stderr,
+          flush=True)
+    for i in range(args.warmup):
+        ctx.run()
+        print(f"[bench] warmup {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+    ctx.synchronize()
+    ctx.profile(True)
+    ctx.profile_report()  # drop the warmup records
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        ctx.run()
+        print(f"[bench] step {i + 1}/{args.steps} done", file=sys.stderr, flush=True)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    prof = ctx.profile_report()
+    ctx.profile(False)
+    res = ctx.results()
+    bad = [i for i, r in enumerate(res) if r["status"] != 0]
+    iters = [r["iterations"] for r in res if r["status"] == 0]
+
+    # parity of a few molecules against the CPU oracle (not timed)
+    import oracle
+    dmax_e = dmax_g = 0.0
+    for i in range(min(args.check, len(mols))):
+        if res[i]["status"]:
+            continue
+        o = oracle.engrad(mols[i][0], mols[i][1], acc=acc)
+        dmax_e = max(dmax_e, abs(res[i]["energy"] - o["energy"]))
+        dmax_g = max(dmax_g, float(np.abs(res[i]["gradient"] - o["gradient"]).max()))
+
+    # dominant kernel and its roofline from the live HIP-event timings
+    total_gpu_ms = sum(v[1] for v in prof.values())
+    name, (nl, ms, work) = max(prof.items(), key=lambda kv: kv[1][1])
+    avg_s = ms / nl / 1e3
+    per_launch = work / nl
+    if any(name.startswith(s) for s in STREAMING):
+        bound, peak, unit, achieved = "hbm", PEAK_HBM / 1e9, "GB/s", per_launch / avg_s / 1e9
+    else:
+        bound, peak, unit, achieved = "mfma", PEAK_FP64_MFMA / 1e12, "TFLOP/s", per_launch / avg_s / 1e12
+
+    value = len(mols) * args.steps / dt
+    line = {
+        "metric": "E+grad/s (GFN2-xTB energy + analytic gradient)",
+        "value": value,
+        "unit": "E+grad/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded molecules of the benchmark configuration)",
+        "config": {"workload": args.workload, "molecules": len(mols), "precision": args.precision,
+                   "accuracy": acc, "etemp_K": 300.0, "mean_scf_iterations": float(np.mean(iters)) if iters else None,
+                   "failed": len(bad), "parity_checked": min(args.check, len(mols)),
+                   "max_abs_dE_vs_oracle": dmax_e, "max_abs_dg_vs_oracle": dmax_g},
+        "roofline": {"bound": bound, "kernel": name, "achieved": achieved, "peak": peak, "unit": unit,
+                     "frac": achieved / peak, "traffic": None, "avg_launch_ms": avg_s * 1e3, "launches": nl,
+                     "kernel_share_of_gpu_time": ms / total_gpu_ms if total_gpu_ms else None},
+        "kernels_ms": {k: round(v[1], 3) for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])[:12]},
+    }
+    if not args.no_cpu:
+        rate, n, cdt = cpu_baseline(mols, args.cpu_budget, acc)
+        line["cpu_baseline"] = {"value": rate, "unit": "E+grad/s", "cores": 1, "kind": "port",
+                                "sample": f"first {n} of {len(mols)} molecules of the workload, {cdt:.1f} s, "
+                                          f"C++ FP64 oracle on 1 thread ({platform.processor() or platform.machine()})"}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

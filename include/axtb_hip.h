@@ -83,6 +83,22 @@ int axtbhip_gfn2_engrad(axtbhip_context* ctx, const axtb_mol* mol, const axtb_op
 int axtbhip_gfn2_engrad_batch(axtbhip_context* ctx, const axtb_mol* mols, int nmol, const axtb_opts* opts,
                               axtb_result* outs);
 
+/* A batch kept on the device between calls, for repeated or timed runs: prepare validates and
+ * uploads the molecules (host arrays may be released afterwards), run computes on the resident
+ * inputs, results downloads them (mols gives the array sizes). No counterpart in the reference:
+ * its calculators own their inputs the same way (crates/axtb/src/calculator.rs:31-52). */
+int axtbhip_batch_prepare(axtbhip_context* ctx, const axtb_mol* mols, int nmol, const axtb_opts* opts);
+int axtbhip_batch_run(axtbhip_context* ctx);
+int axtbhip_batch_results(axtbhip_context* ctx, const axtb_mol* mols, int nmol, axtb_result* outs);
+
+/* Per-kernel timing with HIP events on the context's stream (the counterpart of the reference's
+ * phase timers, crates/base/src/timing.rs:11-54): report lines "kernel launches total_ms work",
+ * work = the launches' algorithmic flops (GEMM, eigensolver) or HBM bytes (streaming kernels). */
+void axtbhip_profile_enable(axtbhip_context* ctx, int on);
+/* hipDeviceSynchronize on the context's device. */
+int axtbhip_synchronize(axtbhip_context* ctx);
+int axtbhip_profile_report(axtbhip_context* ctx, char* buf, size_t len);
+
 #ifdef __cplusplus
 }
 #endif

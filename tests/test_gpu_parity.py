@@ -1,0 +1,163 @@
+"""Parity of the HIP engine with the CPU oracle (through the C ABI), on the GPU.
+
+Bars: FP64 path |dE| <= 1e-8 Eh and max |dg| <= 1e-7 Eh/bohr against the oracle on the same
+inputs, same SCF iteration counts; kernel-level pieces (integrals, potentials, Fock matrix, GEMM,
+eigensolver) to round-off.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, structure
+
+pytestmark = pytest.mark.gpu
+
+GOLD = json.load(open(os.path.join(GOLDEN, "golden.json")))
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from axtb_amd import _lib
+    return _lib.Context(0)
+
+
+def check(ctx, z, x, acc, charge=0.0, ediff=1e-8, gdiff=1e-7):
+    import oracle
+    r = ctx.engrad(z, x, charge=charge, accuracy=acc)
+    o = oracle.engrad(z, x, charge=charge, acc=acc)
+    print(f"dE {r['energy'] - o['energy']:.2e} dg {np.abs(r['gradient'] - o['gradient']).max():.2e} "
+          f"it {r['iterations']}/{o['iterations']}", flush=True)
+    assert abs(r["energy"] - o["energy"]) <= ediff
+    assert np.abs(r["gradient"] - o["gradient"]).max() <= gdiff
+    assert r["iterations"] == o["iterations"]
+    assert np.abs(r["charges"] - o["qat"]).max() <= 1e-7
+    for k in ("repulsion", "dispersion", "isotropic_es", "third_order", "anisotropic"):
+        assert abs(r["terms"][k] - o["terms"][k]) <= 1e-8, k
+    return r, o
+
+
+@pytest.mark.parametrize("name", ["water", "caffeine"])
+def test_golden_structures(ctx, name):
+    z, x = structure(name)
+    r, o = check(ctx, z, x, 1e-6)
+    g = GOLD["acc1e-6"][name]
+    assert abs(r["energy"] - g["energy"]) <= 1e-8
+    assert r["iterations"] == g["iterations"]
+
+
+def test_golden_acc1_c20h42(ctx):
+    from axtb_amd import structures as S
+    z, x = S.alkane_bench(20)
+    r = ctx.engrad(z, x, accuracy=1.0)
+    g = GOLD["acc1"]["C20H42"]
+    assert abs(r["energy"] - g["energy"]) <= 1e-8
+    assert r["iterations"] == g["iterations"]
+
+
+def test_water20_config0(ctx):
+    from axtb_amd import structures as S
+    z, x = S.config("water20")
+    check(ctx, z, x, 1e-2)
+
+
+def test_charged_and_invariants(ctx):
+    z, x = structure("caffeine")
+    r, o = check(ctx, z, x, 1e-6, charge=1.0)
+    assert abs(r["charges"].sum() - 1.0) < 1e-10
+    assert np.abs(r["gradient"].sum(0)).max() < 1e-10
+
+
+def test_batch_uneven(ctx):
+    import oracle
+    from axtb_amd import _lib, structures as S
+    mols = S.batch(24, seed=5)
+    rs = ctx.engrad_batch(mols, _lib.make_opts(accuracy=1e-2))
+    for (z, x), r in zip(mols, rs):
+        assert r["status"] == 0
+        o = oracle.engrad(z, x, acc=1e-2)
+        assert abs(r["energy"] - o["energy"]) <= 1e-8
+        assert np.abs(r["gradient"] - o["gradient"]).max() <= 1e-7
+        assert r["iterations"] == o["iterations"]
+
+
+def test_repeat_bitwise(ctx):
+    from axtb_amd import _lib, structures as S
+    mols = S.batch(8, seed=2)
+    a = ctx.engrad_batch(mols, _lib.make_opts(accuracy=1e-2))
+    b = ctx.engrad_batch(mols, _lib.make_opts(accuracy=1e-2))
+    for r1, r2 in zip(a, b):
+        assert r1["energy"] == r2["energy"]
+        assert np.array_equal(r1["gradient"], r2["gradient"])
+
+
+def test_errors(ctx):
+    from axtb_amd import _lib, structures as S
+    z, x = structure("water")
+    rs = ctx.engrad_batch([(z, x, 9.0), (z, x, 0.5), (np.array([8, 1, 17], dtype=np.int32), x), (z, x)],
+                          _lib.make_opts(accuracy=1e-2))
+    assert [r["status"] for r in rs] == [1, 1, 4, 0]
+    r = ctx.engrad_batch([(z, x)], _lib.make_opts(max_iter=2))[0]
+    assert r["status"] == 3
+    with pytest.raises(_lib.AxtbError):
+        ctx.engrad(z, x, accuracy=-1.0)
+
+
+def test_prepared_batch(ctx):
+    from axtb_amd import _lib, structures as S
+    mols = S.batch(6, seed=9)
+    opts = _lib.make_opts(accuracy=1e-2)
+    ref = ctx.engrad_batch(mols, opts)
+    ctx.prepare(mols, opts)
+    ctx.run()
+    ctx.run()
+    got = ctx.results()
+    for r1, r2 in zip(ref, got):
+        assert r1["energy"] == r2["energy"]
+
+
+# ---------------------------------------------------------------- kernel-level parity
+def test_gemm():
+    from axtb_amd import _testhooks as H
+    rng = np.random.default_rng(1)
+    for (m, n, k, ta, tb) in [(5, 7, 3, 0, 0), (70, 65, 130, 0, 1), (64, 64, 64, 1, 0), (100, 33, 17, 1, 1)]:
+        A = rng.standard_normal((k, m) if ta else (m, k))
+        B = rng.standard_normal((n, k) if tb else (k, n))
+        d = rng.random(k)
+        C = H.gemm(A, B, ta, tb, d)
+        R = (A.T if ta else A) @ np.diag(d) @ (B.T if tb else B)
+        assert np.abs(C - R).max() < 1e-12 * k
+
+
+def test_eigh():
+    from axtb_amd import _testhooks as H
+    rng = np.random.default_rng(2)
+    for n in (1, 2, 7, 33, 64, 130):
+        A = rng.standard_normal((n, n))
+        A = A + A.T
+        w, Z, sw = H.eigh(A)
+        assert sw > 0 or n == 1
+        assert np.abs(w - np.linalg.eigvalsh(A)).max() < 1e-12 * n
+        assert np.abs(A @ Z - Z * w).max() < 1e-12 * n
+        assert np.abs(Z.T @ Z - np.eye(n)).max() < 1e-12 * n
+    # degenerate spectrum
+    Q, _ = np.linalg.qr(rng.standard_normal((20, 20)))
+    A = Q @ np.diag([1.0] * 10 + [2.0] * 10) @ Q.T
+    w, Z, _ = H.eigh(A)
+    assert np.abs(A @ Z - Z * w).max() < 1e-12
+
+
+def test_setup_and_first_iteration():
+    import oracle
+    from axtb_amd import _testhooks as H
+    z, x = structure("caffeine")
+    g = H.setup(z, x)
+    o = oracle.debug(z, x)
+    for k in ("S", "H0", "D", "Q", "cn", "qstart"):
+        assert np.abs(g[k] - o[k]).max() < 1e-12, k
+    n = len(o["S"])
+    assert np.abs(g["X"] @ o["S"] @ g["X"] - np.eye(n)).max() < 1e-11
+    gi = H.iter1(z, x)
+    for k in ("vsh", "vat", "vdp", "vqp", "F", "w", "P", "qsh"):
+        assert np.abs(gi[k] - o[k]).max() < 1e-10, k
