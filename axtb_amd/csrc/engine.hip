@@ -190,7 +190,7 @@ struct Engine {
       a_erad, a_reeq, a_d4rcov, a_d4zeff, a_d4gi, a_d4r4r2, a_d4refcn, a_d4refq;
   DBuf<double> s_alpha, s_coeff, s_se0, s_kcn, s_shpoly, s_refocc, s_hard, s_gam3, s_minal, s_zeta, c6ref;
   // work
-  DBuf<double> cn, cneeq, cnd4, se, dsedcn, mrad, dmrdcn, gam, pool, X, F, T, Am, Vw, Z, C, P, Wm, w, focc, fwork,
+  DBuf<double> cn, cneeq, cnd4, se, dsedcn, mrad, dmrdcn, gam, pool, X, Bm, F, T, Am, Vw, Z, C, P, Wm, w, focc, fwork,
       fst, eeqA, eeqV, eeqZ, eeqw, eeqr, eeqy, qstart, gw, dgw, atmP, e3, g3, dcn3, erep, grep;
   DBuf<double> qin, qout, dq, qlast, dqlast, hdf, hu, hs, qsh, qat, dp, qp, qsh_o, qat_o, dp_o, qp_o, vsh, vat, vdp,
       vqp, gq, cols, eband, eel, eterm, mst, grad, dedcn, dedcn4;
@@ -224,28 +224,50 @@ struct Engine {
 
   // eigenpairs of every job's symmetric matrix (Jacobi, one workgroup per matrix)
   // work: the algorithmic flops of a dense eigendecomposition, (4/3) n^3 for the reduction to
-  // tridiagonal form plus 2 n^2 k for the k needed eigenvectors (k = n here)
+  // tridiagonal form plus 2 n^2 k for the k needed eigenvectors (k = n here). Matrices that fit
+  // the LDS of a workgroup run from LDS, the others from global memory (separate launches).
+  int max_lds = 0;
   void eigh(std::vector<EigJob>& jobs, int max_sweeps = 60) {
     if (jobs.empty()) return;
-    int nmax = 0;
-    double flops = 0.0;
+    if (!max_lds) {
+      HIP_OK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+      if (max_lds <= 0) max_lds = 64 * 1024;
+    }
+    std::vector<EigJob> groups[3];
     for (auto& j : jobs) {
-      nmax = std::max(nmax, j.n);
-      flops += 4.0 / 3.0 * j.n * (double)j.n * j.n + 2.0 * j.n * (double)j.n * j.n;
+      int mode = jacobi_lds(j.n, 2) <= (size_t)max_lds ? 2 : (jacobi_lds(j.n, 1) <= (size_t)max_lds ? 1 : 0);
+      groups[mode].push_back(j);
     }
-    ej.upload(jobs, st);
     jstat.ensure(jobs.size());
-    bool lds = (size_t)nmax * nmax * 8 <= 96 * 1024;
-    size_t shm = jacobi_lds(nmax, lds);
-    if (lds) {
-      HIP_OK(hipFuncSetAttribute((const void*)k_jacobi<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-      LAUNCHW(flops, k_jacobi<true>, dim3(jobs.size()), dim3(256), shm, ej.p, max_sweeps, jstat.p);
-    } else {
-      HIP_OK(hipFuncSetAttribute((const void*)k_jacobi<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-      LAUNCHW(flops, k_jacobi<false>, dim3(jobs.size()), dim3(256), shm, ej.p, max_sweeps, jstat.p);
+    size_t off = 0;
+    for (int mode = 2; mode >= 0; mode--) {
+      auto& js = groups[mode];
+      if (js.empty()) continue;
+      int nmax = 0;
+      double flops = 0.0;
+      for (auto& j : js) {
+        nmax = std::max(nmax, j.n);
+        flops += 4.0 / 3.0 * j.n * (double)j.n * j.n + 2.0 * j.n * (double)j.n * j.n;
+      }
+      DBuf<EigJob>& buf = ejb[mode];
+      buf.upload(js, st);
+      size_t shm = jacobi_lds(nmax, mode);
+      int* stat = jstat.p + off;
+      off += js.size();
+      if (mode == 2) {
+        HIP_OK(hipFuncSetAttribute((const void*)k_jacobi<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+        LAUNCHW(flops, k_jacobi<2>, dim3(js.size()), dim3(512), shm, buf.p, max_sweeps, stat);
+      } else if (mode == 1) {
+        HIP_OK(hipFuncSetAttribute((const void*)k_jacobi<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+        LAUNCHW(flops, k_jacobi<1>, dim3(js.size()), dim3(512), shm, buf.p, max_sweeps, stat);
+      } else {
+        HIP_OK(hipFuncSetAttribute((const void*)k_jacobi<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+        LAUNCHW(flops, k_jacobi<0>, dim3(js.size()), dim3(512), shm, buf.p, max_sweeps, stat);
+      }
+      HIP_OK(hipGetLastError());
     }
-    HIP_OK(hipGetLastError());
   }
+  DBuf<EigJob> ejb[3];
 
   void fill(double* p, long long n, double v = 0.0) {
     if (n <= 0) return;
@@ -604,7 +626,7 @@ struct Engine {
     cols.ensure(11 * (size_t)NO);
     gam.ensure(tot_shp);
     pool.ensure(11 * (size_t)tot_mat);
-    for (auto* d : {&X, &F, &T, &Am, &Vw, &Z, &C, &P, &Wm}) d->ensure(tot_mat);
+    for (auto* d : {&X, &Bm, &F, &T, &Am, &Vw, &Z, &C, &P, &Wm}) d->ensure(tot_mat);
     atmP.ensure(8 * (size_t)tot_pair);
     for (auto* d : {&eeqA, &eeqV, &eeqZ}) d->ensure(tot_eq);
     for (auto* d : {&eeqw, &eeqr, &eeqy}) d->ensure(NA + NM);
@@ -727,12 +749,15 @@ struct Engine {
       const Mol& M = mols[m];
       long long o0 = M.mat0;
       int n = M.nao;
-      g1[m] = GemmJob{n, n, n, 0, 0, F.p + o0, X.p + o0, T.p + o0, nullptr, n, n, n, 1.0, 0.0};   // T = F X
-      g2[m] = GemmJob{n, n, n, 0, 0, X.p + o0, T.p + o0, Am.p + o0, nullptr, n, n, n, 1.0, 0.0};  // A = X T
-      g3v[m] = GemmJob{n, n, n, 0, 0, X.p + o0, Z.p + o0, C.p + o0, nullptr, n, n, n, 1.0, 0.0};  // C = X U
+      // the eigenproblem in the basis B: the Loewdin basis X at the start, then the previous
+      // iteration's orbitals (A = B^T F B is then nearly diagonal and Jacobi needs few sweeps)
+      g1[m] = GemmJob{n, n, n, 0, 0, F.p + o0, Bm.p + o0, T.p + o0, nullptr, n, n, n, 1.0, 0.0};   // T = F B
+      g2[m] = GemmJob{n, n, n, 1, 0, Bm.p + o0, T.p + o0, Am.p + o0, nullptr, n, n, n, 1.0, 0.0};  // A = B^T T
+      g3v[m] = GemmJob{n, n, n, 0, 0, Bm.p + o0, Z.p + o0, C.p + o0, nullptr, n, n, n, 1.0, 0.0};  // C = B U
       gP[m] = GemmJob{n, n, n, 0, 1, C.p + o0, C.p + o0, P.p + o0, focc.p + M.ao0, n, n, n, 1.0, 0.0};
       ejobs[m] = EigJob{n, Am.p + o0, Vw.p + o0, w.p + M.ao0, Z.p + o0};
     }
+    HIP_OK(hipMemcpyAsync(Bm.p, X.p, sizeof(double) * tot_mat, hipMemcpyDeviceToDevice, st));
     int it = 0;
     for (it = 1; it <= o.max_iter; it++) {
       std::vector<GemmJob> a1, a2, a3, a4;
@@ -770,6 +795,7 @@ struct Engine {
       gemm(a2);
       eigh(ea);
       gemm(a3);
+      HIP_OK(hipMemcpyAsync(Bm.p, C.p, sizeof(double) * tot_mat, hipMemcpyDeviceToDevice, st));
       LAUNCH(k_fermi, dim3(NM), dim3(256), 0, b, active.p, w.p, focc.p, fwork.p, fst.p, kocc.p,
                          status.p);
       gemm(a4);  // P = C diag(f) C^T
@@ -840,9 +866,13 @@ struct Engine {
     HIP_OK(hipMemcpyAsync(grad.p, hgr.data(), sizeof(double) * 3 * NA, hipMemcpyHostToDevice, st));
     LAUNCH(k_d4_weights, dim3(blocks(NA, 128)), dim3(128), 0, b, (const int*)okd.p, cnd4.p, qat_o.p,
                        gw.p, dgw.p, 1);
-    LAUNCH(k_pair_grad, dim3(blocks(NA, 4)), dim3(256), 0, b, (const int*)okd.p, pool.p, stride, P.p,
-                       Wm.p, se.p, dsedcn.p, vsh.p, vat.p, vdp.p, vqp.p, qsh_o.p, qat_o.p, dp_o.p, qp_o.p, mrad.p,
-                       dmrdcn.p, gw.p, dgw.p, dcn3.p, grad.p, dedcn.p, dedcn4.p);
+    int natmax = 0;
+    for (auto& M : mols) natmax = std::max(natmax, M.nat);
+    LAUNCH(k_h0_grad, dim3(blocks((long long)natmax * natmax, 64), NM), dim3(64), 0, b, (const int*)okd.p, P.p, Wm.p,
+           se.p, dsedcn.p, vsh.p, vat.p, vdp.p, vqp.p, atmP.p, (long long)tot_pair);
+    LAUNCH(k_pair_grad, dim3(blocks(NA, 4)), dim3(256), 0, b, (const int*)okd.p, pool.p, stride, P.p, Wm.p, se.p,
+           dsedcn.p, vsh.p, vat.p, vdp.p, vqp.p, qsh_o.p, qat_o.p, dp_o.p, qp_o.p, mrad.p, dmrdcn.p, gw.p, dgw.p,
+           dcn3.p, (const double*)atmP.p, (long long)tot_pair, grad.p, dedcn.p, dedcn4.p);
     LAUNCH(k_cn_chain, dim3(blocks(NA, 4)), dim3(256), 0, b, (const int*)okd.p, dedcn.p, dedcn4.p,
                        grad.p);
     HIP_OK(hipGetLastError());

@@ -200,10 +200,48 @@ __device__ __forceinline__ double hscale(const Batch& b, int sa, int sb) {
   return zij * (km * (1.0 + 0.02 * (dEN * dEN)));
 }
 
+// One shell-pair block of the integral pools: element (mu, nu) and, off-site, its transpose with
+// the moments shifted to the bra atom.
+template <int LI, int LJ>
+__device__ __forceinline__ void integral_block(const Batch& b, const Mol& M, int si, int sj, const double* vec,
+                                               bool onsite, double hij, double* pool, long long stride) {
+  constexpr int NSI = 2 * LI + 1, NSJ = 2 * LJ + 1;
+  double blk[NSI * NSJ][10];
+  shellpair_multipole_t<LI, LJ>(b.s.nprim[si], &b.s.alpha[MAXPRIM * si], &b.s.coeff[MAXPRIM * si], b.s.nprim[sj],
+                                &b.s.alpha[MAXPRIM * sj], &b.s.coeff[MAXPRIM * sj], vec, b.intcut, blk);
+  const int ii = b.s.iao[si], jj = b.s.iao[sj];
+  const long long n = M.nao;
+#pragma unroll
+  for (int im = 0; im < NSI; im++)
+#pragma unroll
+    for (int jm = 0; jm < NSJ; jm++) {
+      const double* o = blk[jm + NSJ * im];
+      long long mu = jj + jm, nu = ii + im;
+      long long e1 = M.mat0 + mu + nu * n;
+      pool[e1] = o[0];
+      pool[stride + e1] = o[0] * hij;
+#pragma unroll
+      for (int k = 0; k < 3; k++) pool[(2 + k) * stride + e1] = o[1 + k];
+#pragma unroll
+      for (int k = 0; k < 6; k++) pool[(5 + k) * stride + e1] = o[4 + k];
+      if (!onsite) {
+        double dj[3], qj[6];
+        shift_to_bra(vec, o[0], &o[1], &o[4], dj, qj);
+        long long e2 = M.mat0 + nu + mu * n;
+        pool[e2] = o[0];
+        pool[stride + e2] = o[0] * hij;
+#pragma unroll
+        for (int k = 0; k < 3; k++) pool[(2 + k) * stride + e2] = dj[k];
+#pragma unroll
+        for (int k = 0; k < 6; k++) pool[(5 + k) * stride + e2] = qj[k];
+      }
+    }
+}
+
 // S, H0, D (3), Q (6) of every shell pair (ish of atom A >= jsh of atom B): each AO element
 // written once (ao.rs:195-291, hamiltonian.rs:77-94). Pools are `stride` elements apart:
 // slot 0 S, 1 H0, 2-4 D, 5-10 Q; element (mu, nu) of molecule m at mat0 + mu + nu nao.
-__global__ void k_integrals(Batch b, const double* se, double* pool, long long stride) {
+__global__ __launch_bounds__(128) void k_integrals(Batch b, const double* se, double* pool, long long stride) {
   int m = blockIdx.y;
   const Mol M = b.mol[m];
   long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -222,9 +260,6 @@ __global__ void k_integrals(Batch b, const double* se, double* pool, long long s
     double a1 = b.s.minalpha[si], a2 = b.s.minalpha[sj];
     if (a1 * a2 * r2 / (a1 + a2) > b.intcut * (1.0 + 1e-10)) return;
   }
-  double blk[9][10];
-  shellpair_multipole(b.s.l[si], b.s.nprim[si], &b.s.alpha[MAXPRIM * si], &b.s.coeff[MAXPRIM * si], b.s.l[sj],
-                      b.s.nprim[sj], &b.s.alpha[MAXPRIM * sj], &b.s.coeff[MAXPRIM * sj], vec, b.intcut, blk);
   double hij;
   if (onsite) {
     hij = 0.5 * (se[si] + se[sj]);
@@ -233,28 +268,15 @@ __global__ void k_integrals(Batch b, const double* se, double* pool, long long s
     double sp = (1.0 + b.s.shpoly[si] * rr) * (1.0 + b.s.shpoly[sj] * rr);
     hij = 0.5 * (se[si] + se[sj]) * hscale(b, sj, si) * sp;
   }
-  int nsi = 2 * b.s.l[si] + 1, nsj = 2 * b.s.l[sj] + 1;
-  int ii = b.s.iao[si], jj = b.s.iao[sj];
-  long long n = M.nao;
-  for (int im = 0; im < nsi; im++)
-    for (int jm = 0; jm < nsj; jm++) {
-      const double* o = blk[jm + nsj * im];
-      long long mu = jj + jm, nu = ii + im;
-      long long e1 = M.mat0 + mu + nu * n;
-      pool[e1] = o[0];
-      pool[stride + e1] = o[0] * hij;
-      for (int k = 0; k < 3; k++) pool[(2 + k) * stride + e1] = o[1 + k];
-      for (int k = 0; k < 6; k++) pool[(5 + k) * stride + e1] = o[4 + k];
-      if (!onsite) {
-        double dj[3], qj[6];
-        shift_to_bra(vec, o[0], &o[1], &o[4], dj, qj);
-        long long e2 = M.mat0 + nu + mu * n;
-        pool[e2] = o[0];
-        pool[stride + e2] = o[0] * hij;
-        for (int k = 0; k < 3; k++) pool[(2 + k) * stride + e2] = dj[k];
-        for (int k = 0; k < 6; k++) pool[(5 + k) * stride + e2] = qj[k];
-      }
-    }
+  const int key = 2 * b.s.l[si] + b.s.l[sj];
+  if (key == 0)
+    integral_block<0, 0>(b, M, si, sj, vec, onsite, hij, pool, stride);
+  else if (key == 1)
+    integral_block<0, 1>(b, M, si, sj, vec, onsite, hij, pool, stride);
+  else if (key == 2)
+    integral_block<1, 0>(b, M, si, sj, vec, onsite, hij, pool, stride);
+  else
+    integral_block<1, 1>(b, M, si, sj, vec, onsite, hij, pool, stride);
 }
 
 // ------------------------------------------------------------------ repulsion (repulsion.rs:44-73)
@@ -1071,17 +1093,17 @@ __global__ void k_broyden(Batch b, const int* active, int cap, double damp, doub
 }
 
 // ------------------------------------------------------------------ gradient
-// All pair gradients of atom a (one wavefront per atom; every pair is evaluated from both atoms,
-// each keeping its own share): ES2 (coulomb.rs:123-169), AES with dE/dCN through the damping
-// radii (coulomb.rs:896-1001), D4 two-body at fixed charges with dE/dCN_D4 (disp.rs:460-484), and
-// the H0/S/multipole integral derivatives contracted with P, W and the potentials, with
-// dE/dCN of the levels (hamiltonian.rs:137-223).
-__global__ void k_pair_grad(Batch b, const int* okm, const double* pool, long long stride, const double* P,
+// The classical pair gradients of atom a (one wavefront per atom; every pair is evaluated from
+// both atoms, each keeping its own share): ES2 (coulomb.rs:123-169), AES with dE/dCN through the
+// damping radii (coulomb.rs:896-1001) and D4 two-body at fixed charges with dE/dCN_D4
+// (disp.rs:460-484); plus the atom's share of the integral-derivative pair terms of k_h0_grad and
+// the on-site dE/dCN of its levels (hamiltonian.rs:212-221).
+__global__ __launch_bounds__(256) void k_pair_grad(Batch b, const int* okm, const double* pool, long long stride, const double* P,
                             const double* W, const double* se, const double* dsedcn, const double* vsh,
                             const double* vat, const double* vdp, const double* vqp, const double* qsh,
                             const double* qat, const double* dp, const double* qp, const double* mrad,
-                            const double* dmrdcn, const double* gw, const double* dgwdcn, const double* dcn3, double* g,
-                            double* dedcn, double* dedcn_d4) {
+                            const double* dmrdcn, const double* gw, const double* dgwdcn, const double* dcn3,
+                            const double* hg, long long hstride, double* g, double* dedcn, double* dedcn_d4) {
   int a = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
   int lane = threadIdx.x & 63;
   if (a >= b.nat) return;
@@ -1155,48 +1177,16 @@ __global__ void k_pair_grad(Batch b, const int* okm, const double* pool, long lo
         for (int k = 0; k < 3; k++) ga[k] += sgn * (-c6 * pg * v[k]);
       }
     }
-    // H0 / S / multipole integral derivatives over the screened shell pairs
-    {
-      double amin_i = 1e300, amin_j = 1e300;
-      for (int s = b.a.ish[iat]; s < b.a.ish[iat] + b.a.nsh[iat]; s++) amin_i = fmin(amin_i, b.s.minalpha[s]);
-      for (int s = b.a.ish[jat]; s < b.a.ish[jat] + b.a.nsh[jat]; s++) amin_j = fmin(amin_j, b.s.minalpha[s]);
-      if (!(amin_i * amin_j * r2 / (amin_i + amin_j) > b.intcut * (1.0 + 1e-10)) && r2 >= 2.220446049250313e-16) {
-        double rr = sqrt(r1 / (b.a.rad[jat] + b.a.rad[iat]));
-        double dg[3] = {0, 0, 0}, dcni = 0.0, dcnj = 0.0;
-        for (int si = b.a.ish[iat]; si < b.a.ish[iat] + b.a.nsh[iat]; si++)
-          for (int sj = b.a.ish[jat]; sj < b.a.ish[jat] + b.a.nsh[jat]; sj++) {
-            double a1 = b.s.minalpha[si], a2 = b.s.minalpha[sj];
-            if (a1 * a2 * r2 / (a1 + a2) > b.intcut * (1.0 + 1e-10)) continue;
-            double shpi = 1.0 + b.s.shpoly[si] * rr, shpj = 1.0 + b.s.shpoly[sj] * rr;
-            double shp = shpi * shpj;
-            double dshp = (shpi * b.s.shpoly[sj] + shpj * b.s.shpoly[si]) * 0.5 * rr / r2;
-            double dsv[3] = {dshp / shp * v[0], dshp / shp * v[1], dshp / shp * v[2]};
-            double hs = hscale(b, sj, si) * shp;
-            double hij = 0.5 * (se[si] + se[sj]) * hs;
-            int li = b.s.l[si], lj = b.s.l[sj];
-            int nsi = 2 * li + 1, nsj = 2 * lj + 1;
-            GradCoef cf;
-            for (int im = 0; im < nsi; im++)
-              for (int jm = 0; jm < nsj; jm++) {
-                int blk = jm + nsj * im;
-                long long mu = b.s.iao[sj] + jm, nu = b.s.iao[si] + im;
-                long long e = M.mat0 + mu + nu * M.nao;
-                double pij = P[e];
-                double vmu = vsh[b.ao_sh[M.ao0 + mu]] + vat[b.ao_at[M.ao0 + mu]];
-                double vnu = vsh[b.ao_sh[M.ao0 + nu]] + vat[b.ao_at[M.ao0 + nu]];
-                cf.p[blk] = pij;
-                cf.hp2[blk] = 2.0 * pij * hij;
-                cf.sval[blk] = -pij * (vmu + vnu) + 2.0 * pij * hij - 2.0 * W[e];
-              }
-            double ps = shellpair_grad(li, b.s.nprim[si], &b.s.alpha[MAXPRIM * si], &b.s.coeff[MAXPRIM * si], lj,
-                                       b.s.nprim[sj], &b.s.alpha[MAXPRIM * sj], &b.s.coeff[MAXPRIM * sj], v, b.intcut,
-                                       cf, dsv, &vdp[3 * iat], &vdp[3 * jat], &vqp[6 * iat], &vqp[6 * jat], dg);
-            dcni += dsedcn[si] * hs * ps;
-            dcnj += dsedcn[sj] * hs * ps;
-          }
-        for (int k = 0; k < 3; k++) ga[k] += sgn * dg[k];
-        dcn += a == iat ? dcni : dcnj;
-      }
+  }
+  // the integral-derivative terms of the pairs (a, j < a) and (i > a, a) from k_h0_grad
+  {
+    const int la = a - M.at0;
+    for (int lj = lane; lj < M.nat; lj += 64) {
+      if (lj == la) continue;
+      long long e = la > lj ? M.pair0 + (long long)la * M.nat + lj : M.pair0 + (long long)lj * M.nat + la;
+      double sg = la > lj ? 1.0 : -1.0;
+      for (int k = 0; k < 3; k++) ga[k] += sg * hg[k * hstride + e];
+      dcn += la > lj ? hg[3 * hstride + e] : hg[4 * hstride + e];
     }
   }
   for (int k = 0; k < 3; k++) ga[k] = wave_sum(ga[k]);
@@ -1217,6 +1207,87 @@ __global__ void k_pair_grad(Batch b, const int* okm, const double* pool, long lo
     dedcn[a] = dcn;
     dedcn_d4[a] = dcn4 + dcn3[a];
   }
+}
+
+// H0 / S / multipole integral derivatives of one atom pair (iat > jat) per thread, over its
+// screened shell pairs, contracted with P, W, the AO potentials and the multipole potentials
+// (hamiltonian.rs:137-223). Writes hg[x * hstride + pair0 + iat*nat + jat] for x = dg (3),
+// dE/dCN of iat, dE/dCN of jat.
+template <int LI, int LJ>
+__device__ __forceinline__ void h0_grad_shell(const Batch& b, const Mol& M, int si, int sj, int iat, int jat,
+                                              const double* v, double r2, double rr, const double* P,
+                                              const double* W, const double* se, const double* dsedcn,
+                                              const double* vsh, const double* vat, const double* vdp,
+                                              const double* vqp, double* dg, double& dcni, double& dcnj) {
+  constexpr int NSI = 2 * LI + 1, NSJ = 2 * LJ + 1;
+  double shpi = 1.0 + b.s.shpoly[si] * rr, shpj = 1.0 + b.s.shpoly[sj] * rr;
+  double shp = shpi * shpj;
+  double dshp = (shpi * b.s.shpoly[sj] + shpj * b.s.shpoly[si]) * 0.5 * rr / r2;
+  double dsv[3] = {dshp / shp * v[0], dshp / shp * v[1], dshp / shp * v[2]};
+  double hs = hscale(b, sj, si) * shp;
+  double hij = 0.5 * (se[si] + se[sj]) * hs;
+  double sval[NSI * NSJ], hp2[NSI * NSJ], pp[NSI * NSJ];
+  const int ii = b.s.iao[si], jj = b.s.iao[sj];
+  const double vi = vsh[si] + vat[iat], vj = vsh[sj] + vat[jat];
+#pragma unroll
+  for (int im = 0; im < NSI; im++)
+#pragma unroll
+    for (int jm = 0; jm < NSJ; jm++) {
+      const int blk = jm + NSJ * im;
+      long long e = M.mat0 + (jj + jm) + (long long)(ii + im) * M.nao;
+      double pij = P[e];
+      pp[blk] = pij;
+      hp2[blk] = 2.0 * pij * hij;
+      sval[blk] = -pij * (vj + vi) + 2.0 * pij * hij - 2.0 * W[e];
+    }
+  double ps = shellpair_grad_t<LI, LJ>(b.s.nprim[si], &b.s.alpha[MAXPRIM * si], &b.s.coeff[MAXPRIM * si],
+                                       b.s.nprim[sj], &b.s.alpha[MAXPRIM * sj], &b.s.coeff[MAXPRIM * sj], v, b.intcut,
+                                       sval, hp2, pp, dsv, &vdp[3 * iat], &vdp[3 * jat], &vqp[6 * iat], &vqp[6 * jat],
+                                       dg);
+  dcni += dsedcn[si] * hs * ps;
+  dcnj += dsedcn[sj] * hs * ps;
+}
+
+__global__ __launch_bounds__(64) void k_h0_grad(Batch b, const int* okm, const double* P, const double* W, const double* se,
+                          const double* dsedcn, const double* vsh, const double* vat, const double* vdp,
+                          const double* vqp, double* hg, long long hstride) {
+  int m = blockIdx.y;
+  if (okm && !okm[m]) return;
+  const Mol M = b.mol[m];
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)M.nat * M.nat) return;
+  int li = (int)(idx / M.nat), lj = (int)(idx % M.nat);
+  if (li <= lj) return;
+  const int iat = M.at0 + li, jat = M.at0 + lj;
+  const double* xi = &b.a.xyz[3 * iat];
+  const double* xj = &b.a.xyz[3 * jat];
+  double v[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+  double r2 = dot3(v, v);
+  double dg[3] = {0, 0, 0}, dcni = 0.0, dcnj = 0.0;
+  double amin_i = 1e300, amin_j = 1e300;
+  for (int s = b.a.ish[iat]; s < b.a.ish[iat] + b.a.nsh[iat]; s++) amin_i = fmin(amin_i, b.s.minalpha[s]);
+  for (int s = b.a.ish[jat]; s < b.a.ish[jat] + b.a.nsh[jat]; s++) amin_j = fmin(amin_j, b.s.minalpha[s]);
+  if (!(amin_i * amin_j * r2 / (amin_i + amin_j) > b.intcut * (1.0 + 1e-10)) && r2 >= 2.220446049250313e-16) {
+    double rr = sqrt(sqrt(r2) / (b.a.rad[jat] + b.a.rad[iat]));
+    for (int si = b.a.ish[iat]; si < b.a.ish[iat] + b.a.nsh[iat]; si++)
+      for (int sj = b.a.ish[jat]; sj < b.a.ish[jat] + b.a.nsh[jat]; sj++) {
+        double a1 = b.s.minalpha[si], a2 = b.s.minalpha[sj];
+        if (a1 * a2 * r2 / (a1 + a2) > b.intcut * (1.0 + 1e-10)) continue;
+        const int key = 2 * b.s.l[si] + b.s.l[sj];
+        if (key == 0)
+          h0_grad_shell<0, 0>(b, M, si, sj, iat, jat, v, r2, rr, P, W, se, dsedcn, vsh, vat, vdp, vqp, dg, dcni, dcnj);
+        else if (key == 1)
+          h0_grad_shell<0, 1>(b, M, si, sj, iat, jat, v, r2, rr, P, W, se, dsedcn, vsh, vat, vdp, vqp, dg, dcni, dcnj);
+        else if (key == 2)
+          h0_grad_shell<1, 0>(b, M, si, sj, iat, jat, v, r2, rr, P, W, se, dsedcn, vsh, vat, vdp, vqp, dg, dcni, dcnj);
+        else
+          h0_grad_shell<1, 1>(b, M, si, sj, iat, jat, v, r2, rr, P, W, se, dsedcn, vsh, vat, vdp, vqp, dg, dcni, dcnj);
+      }
+  }
+  long long e = M.pair0 + idx;
+  for (int k = 0; k < 3; k++) hg[k * hstride + e] = dg[k];
+  hg[3 * hstride + e] = dcni;
+  hg[4 * hstride + e] = dcnj;
 }
 
 // Chain rules of the two coordination numbers (base/src/ncoord.rs:486-519): the GFN2 CN with

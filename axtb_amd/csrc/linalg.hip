@@ -117,111 +117,152 @@ __device__ __forceinline__ void tour_pair(int r, int k, int nn, int& p, int& q) 
   }
 }
 
-// One workgroup per matrix. USE_LDS: the matrix is copied to LDS (n*n doubles must fit).
+// One workgroup per matrix. A round of the tournament rotates n/2 disjoint pairs (p_k, q_k) at
+// once; A <- J^T A J then transforms every 2 x 2 block (pair k1 rows, pair k2 columns) on its own,
+// B' = R_k1^T B R_k2 with R = [[c, s], [-s, c]], so one pass over the lower blocks (mirrored)
+// updates the whole matrix. V <- V J alongside. MODE 0: A and V in global memory; 1: A in LDS;
+// 2: A and V in LDS. A sweep starts with a test of every off-diagonal element against the
+// rotation threshold, so a converged matrix costs one pass, not a sweep.
 // status[job] = number of sweeps (negative if not converged within max_sweeps).
-template <bool USE_LDS>
-__global__ __launch_bounds__(256) void k_jacobi(const EigJob* jobs, int max_sweeps, int* status) {
+template <int MODE>
+__global__ __launch_bounds__(512) void k_jacobi(const EigJob* jobs, int max_sweeps, int* status) {
   extern __shared__ double lds[];
   __shared__ int rotated;
   __shared__ double anorm;
   const EigJob J = jobs[blockIdx.x];
   const int n = J.n, tid = threadIdx.x, nt = blockDim.x;
   if (n <= 0) return;
-  // dynamic LDS: [A (n*n, USE_LDS only)] [c, s of two rounds: 4*npmax] [p, q of two rounds]
-  const int npmax = (n + 1) / 2;
-  double* A = USE_LDS ? lds : J.A;
-  double* csb = lds + (USE_LDS ? n * n : 0);
-  int* pqb = (int*)(csb + 4 * npmax);
-#define CS(b, x, k) csb[((b)*2 + (x)) * npmax + (k)]
-#define PQ(b, x, k) pqb[((b)*2 + (x)) * npmax + (k)]
-  double* V = J.V;
-  if (USE_LDS)
+  const int nn = n + (n & 1), np = nn / 2;
+  // dynamic LDS: [A (n*n, MODE >= 1)] [V (n*n, MODE 2)] [c, s per pair] [p, q per pair]
+  double* A = MODE >= 1 ? lds : J.A;
+  double* V = MODE == 2 ? lds + n * n : J.V;
+  double* csb = lds + (MODE >= 1 ? n * n : 0) + (MODE == 2 ? n * n : 0);
+  int* pqb = (int*)(csb + 2 * np);
+  if (MODE >= 1)
     for (int e = tid; e < n * n; e += nt) A[e] = J.A[e];
   for (int e = tid; e < n * n; e += nt) V[e] = (e % n == e / n) ? 1.0 : 0.0;
+  if (tid == 0) rotated = 0;
   __syncthreads();
-  if (tid == 0) {
+  if (tid < 64) {
     double s = 0.0;
-    for (int i = 0; i < n; i++) s = fmax(s, fabs(A[i + i * n]));
-    for (int e = 0; e < n * n; e++) s = fmax(s, fabs(A[e]));
-    anorm = s;
+    for (int e = tid; e < n * n; e += 64) s = fmax(s, fabs(A[e]));
+    for (int o = 32; o > 0; o >>= 1) s = fmax(s, __shfl_xor(s, o, 64));
+    if (tid == 0) anorm = s;
   }
-  const int nn = n + (n & 1), np = nn / 2;
+  __syncthreads();
+  const double an = anorm;
+  const int nblk = np * (np + 1) / 2;
   int sweep = 0;
   bool conv = n == 1;
   for (sweep = 0; sweep < max_sweeps && !conv; sweep++) {
-    if (tid == 0) rotated = 0;
+    // convergence test over all off-diagonal elements
+    for (int e = tid; e < n * n; e += nt) {
+      int i = e % n, j = e / n;
+      if (i < j) {
+        double aij = A[e];
+        if (fabs(aij) > 1e-15 * sqrt(fabs(A[i + i * n]) * fabs(A[j + j * n])) && fabs(aij) > 1e-17 * an) rotated = 1;
+      }
+    }
     __syncthreads();
+    conv = rotated == 0;
+    __syncthreads();
+    if (conv) break;
+    if (tid == 0) rotated = 0;
     for (int r = 0; r < nn - 1; r++) {
-      const int bsel = r & 1;
       for (int k = tid; k < np; k += nt) {
         int p, q;
         tour_pair(r, k, nn, p, q);
         double c = 1.0, s = 0.0;
-        int ok = 0;
         if (q < n) {
           double app = A[p + p * n], aqq = A[q + q * n], apq = A[p + q * n];
-          double t1 = 1e-15 * sqrt(fabs(app) * fabs(aqq)), t2 = 1e-17 * anorm;
-          if (fabs(apq) > t1 && fabs(apq) > t2) {
+          if (fabs(apq) > 1e-15 * sqrt(fabs(app) * fabs(aqq)) && fabs(apq) > 1e-17 * an) {
             double tau = (aqq - app) / (2.0 * apq);
             double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
             c = 1.0 / sqrt(1.0 + t * t);
             s = t * c;
-            ok = 1;
           }
         }
-        CS(bsel, 0, k) = c;
-        CS(bsel, 1, k) = s;
-        PQ(bsel, 0, k) = p;
-        PQ(bsel, 1, k) = ok ? q : -1;
-        if (ok) rotated = 1;
+        csb[2 * k] = c;
+        csb[2 * k + 1] = s;
+        pqb[2 * k] = p;
+        pqb[2 * k + 1] = q;
       }
       __syncthreads();
-      // A <- A J (columns p, q of every rotated pair)
-      for (int e = tid; e < np * n; e += nt) {
-        int k = e / n, i = e % n;
-        int q = PQ(bsel, 1, k);
-        if (q < 0) continue;
-        int p = PQ(bsel, 0, k);
-        double c = CS(bsel, 0, k), s = CS(bsel, 1, k);
-        double aip = A[i + p * n], aiq = A[i + q * n];
-        A[i + p * n] = c * aip - s * aiq;
-        A[i + q * n] = s * aip + c * aiq;
+      // lower 2 x 2 blocks (k1 >= k2) of A, mirrored; block b = k1 (k1 + 1) / 2 + k2
+      for (int b = tid; b < nblk; b += nt) {
+        int k1 = (int)((sqrtf(8.0f * b + 1.0f) - 1.0f) * 0.5f);
+        while ((k1 + 1) * (k1 + 2) / 2 <= b) k1++;
+        while (k1 * (k1 + 1) / 2 > b) k1--;
+        const int k2 = b - k1 * (k1 + 1) / 2;
+        const double s1 = csb[2 * k1 + 1], s2 = csb[2 * k2 + 1];
+        if (s1 == 0.0 && s2 == 0.0) continue;
+        const int p1 = pqb[2 * k1], q1 = pqb[2 * k1 + 1], p2 = pqb[2 * k2], q2 = pqb[2 * k2 + 1];
+        const double c1 = csb[2 * k1], c2 = csb[2 * k2];
+        const bool vq1 = q1 < n, vq2 = q2 < n;
+        double b00 = A[p1 + p2 * n];
+        double b01 = vq2 ? A[p1 + q2 * n] : 0.0;
+        double b10 = vq1 ? A[q1 + p2 * n] : 0.0;
+        double b11 = (vq1 && vq2) ? A[q1 + q2 * n] : 0.0;
+        double t00 = c2 * b00 - s2 * b01, t01 = s2 * b00 + c2 * b01;
+        double t10 = c2 * b10 - s2 * b11, t11 = s2 * b10 + c2 * b11;
+        double n00 = c1 * t00 - s1 * t10, n01 = c1 * t01 - s1 * t11;
+        double n10 = s1 * t00 + c1 * t10, n11 = s1 * t01 + c1 * t11;
+        if (k1 == k2) {
+          A[p1 + p1 * n] = n00;
+          if (vq1) {
+            A[q1 + q1 * n] = n11;
+            A[p1 + q1 * n] = 0.0;
+            A[q1 + p1 * n] = 0.0;
+          }
+          continue;
+        }
+        A[p1 + p2 * n] = n00;
+        A[p2 + p1 * n] = n00;
+        if (vq2) {
+          A[p1 + q2 * n] = n01;
+          A[q2 + p1 * n] = n01;
+        }
+        if (vq1) {
+          A[q1 + p2 * n] = n10;
+          A[p2 + q1 * n] = n10;
+        }
+        if (vq1 && vq2) {
+          A[q1 + q2 * n] = n11;
+          A[q2 + q1 * n] = n11;
+        }
       }
-      __syncthreads();
-      // A <- J^T A (rows p, q)
-      for (int e = tid; e < np * n; e += nt) {
-        int k = e / n, j = e % n;
-        int q = PQ(bsel, 1, k);
-        if (q < 0) continue;
-        int p = PQ(bsel, 0, k);
-        double c = CS(bsel, 0, k), s = CS(bsel, 1, k);
-        double apj = A[p + j * n], aqj = A[q + j * n];
-        A[p + j * n] = c * apj - s * aqj;
-        A[q + j * n] = s * apj + c * aqj;
-      }
-      __syncthreads();
-      for (int k = tid; k < np; k += nt) {
-        int q = PQ(bsel, 1, k);
-        if (q < 0) continue;
-        int p = PQ(bsel, 0, k);
-        A[p + q * n] = 0.0;
-        A[q + p * n] = 0.0;
-      }
-      // V <- V J (reads this round's rotations; the next round writes the other buffer)
-      for (int e = tid; e < np * n; e += nt) {
-        int k = e / n, i = e % n;
-        int q = PQ(bsel, 1, k);
-        if (q < 0) continue;
-        int p = PQ(bsel, 0, k);
-        double c = CS(bsel, 0, k), s = CS(bsel, 1, k);
-        double vip = V[i + p * n], viq = V[i + q * n];
-        V[i + p * n] = c * vip - s * viq;
-        V[i + q * n] = s * vip + c * viq;
+      // V <- V J: loads of a thread's elements first (memory-level parallelism in MODE < 2)
+      constexpr int CH = 8;
+      for (int e0 = tid; e0 < np * n; e0 += nt * CH) {
+        double vp[CH], vq[CH];
+#pragma unroll
+        for (int u = 0; u < CH; u++) {
+          int e = e0 + u * nt;
+          vp[u] = vq[u] = 0.0;
+          if (e < np * n) {
+            int k = e / n, i = e % n;
+            if (csb[2 * k + 1] != 0.0) {
+              vp[u] = V[i + pqb[2 * k] * n];
+              vq[u] = V[i + pqb[2 * k + 1] * n];
+            }
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < CH; u++) {
+          int e = e0 + u * nt;
+          if (e < np * n) {
+            int k = e / n, i = e % n;
+            const double s = csb[2 * k + 1];
+            if (s != 0.0) {
+              const double c = csb[2 * k];
+              V[i + pqb[2 * k] * n] = c * vp[u] - s * vq[u];
+              V[i + pqb[2 * k + 1] * n] = s * vp[u] + c * vq[u];
+            }
+          }
+        }
       }
       __syncthreads();
     }
-    conv = rotated == 0;
-    __syncthreads();
   }
   // eigenvalues ascending (ties by index), vectors permuted alike
   for (int i = tid; i < n; i += nt) {
@@ -235,17 +276,16 @@ __global__ __launch_bounds__(256) void k_jacobi(const EigJob* jobs, int max_swee
     for (int r = 0; r < n; r++) J.Z[r + (long long)rank * n] = V[r + (long long)i * n];
   }
   if (tid == 0) status[blockIdx.x] = conv ? sweep : -sweep;
-#undef CS
-#undef PQ
 }
+
+template __global__ void k_jacobi<0>(const EigJob*, int, int*);
+template __global__ void k_jacobi<1>(const EigJob*, int, int*);
+template __global__ void k_jacobi<2>(const EigJob*, int, int*);
 
 // dynamic LDS bytes of k_jacobi for an n x n matrix
-inline size_t jacobi_lds(int n, bool use_lds) {
+inline size_t jacobi_lds(int n, int mode) {
   size_t np = (n + 1) / 2;
-  return (use_lds ? (size_t)n * n * 8 : 0) + np * 4 * 8 + np * 4 * 4;
+  return (size_t)(mode >= 1 ? 1 : 0) * n * n * 8 + (size_t)(mode == 2 ? 1 : 0) * n * n * 8 + np * 2 * 8 + np * 2 * 4;
 }
-
-template __global__ void k_jacobi<true>(const EigJob*, int, int*);
-template __global__ void k_jacobi<false>(const EigJob*, int, int*);
 
 }  // namespace axtb
