@@ -53,6 +53,18 @@ def eigh(A):
     return (w[0], Z[0], sw[0]) if one else (w, Z, sw)
 
 
+def dense_eigh(A, k=None):
+    """All eigenvalues and the first k eigenvectors by the dense device solver."""
+    A = np.asarray(A, dtype=np.float64)
+    n = A.shape[0]
+    k = n if k is None else k
+    buf = np.ascontiguousarray(A.T)
+    w = np.zeros(n)
+    Z = np.zeros((k, n))
+    _rc(lib().axtbhip_test_dense_eigh(n, _p(buf), k, _p(w), _p(Z)))
+    return w, Z.T.copy()
+
+
 def setup(z, x, charge=0.0, **kw):
     mol, keep = _mol(z, x, charge)
     nsh, nao = ctypes.c_int(), ctypes.c_int()

@@ -21,6 +21,7 @@
 #include "gfn2_tables.h"
 #include "kernels.hip"
 #include "linalg.hip"
+#include "dense_eigh.hip"
 
 namespace axtb {
 
@@ -268,6 +269,198 @@ struct Engine {
     }
   }
   DBuf<EigJob> ejb[3];
+
+  // ---------------------------------------------------------------- dense eigensolver (large n)
+  // Eigenvalues of the symmetric n x n matrix A (destroyed: it keeps the Householder vectors)
+  // into w; the tridiagonal eigenvectors stay in dq_Q for dense_vectors().
+  DBuf<double> dtau, dd, de, dpart, dwv, dq_Q, dq_Q2, dD, dD2, dds, dzs, drot_cs, dkd, dkz, dtaus, dzh, dval, dU, dG,
+      dQG, dleaf, dleafV, dleafZ, dleafw, dbtV, dbtY, dbtY2, dbtT, dbtVtV;
+  DBuf<int> dperm, dkind, drot_ij, dkcol, dorg, dsrc, dnk, dnr, dls, dlm;
+  DBuf<double> drho;
+  DBuf<long long> duoff, dgoff;
+  DBuf<DcNode> dnodes;
+  double* dense_A = nullptr;
+  int dense_n = 0;
+
+  void dense_values(double* A, int n, double* w) {
+    dense_A = A;
+    dense_n = n;
+    for (auto* b : {&dtau, &dd, &de, &dD, &dD2, &dds, &dzs, &dkd, &dkz, &dtaus, &dzh, &dval}) b->ensure(n);
+    drot_cs.ensure(2 * (size_t)n);
+    for (auto* b : {&dperm, &dkind, &dkcol, &dorg, &dsrc}) b->ensure(n);
+    drot_ij.ensure(2 * (size_t)n);
+    const int nchunk = (n + 63) / 64;
+    dpart.ensure((size_t)nchunk * n);
+    dwv.ensure(n);
+    // 1. Householder tridiagonalisation
+    for (int i = 0; i + 1 < n; i++) {
+      const int m = n - i - 1;
+      LAUNCH(k_hh_reflector, dim3(1), dim3(256), 0, A, n, i, dtau.p, dd.p, de.p);
+      if (m > 1) {
+        const int nc = (m + 63) / 64;
+        LAUNCHW(8.0 * m * (double)m, k_symv_part, dim3(blocks(m, 256), nc), dim3(256), 0, (const double*)A, n, i,
+                dpart.p);
+        LAUNCH(k_hh_w, dim3(1), dim3(1024), 0, (const double*)A, n, i, (const double*)dtau.p, (const double*)dpart.p,
+               nc, dwv.p);
+        LAUNCHW(16.0 * m * (double)m, k_syr2, dim3(blocks(m, 256), m), dim3(256), 0, A, n, i, (const double*)dwv.p);
+      }
+    }
+    LAUNCH(k_hh_last, dim3(1), dim3(1), 0, (const double*)A, n, dd.p, de.p, dtau.p);
+    HIP_OK(hipGetLastError());
+    // 2. divide and conquer on (d, e): the merge tree by recursive halving (leaves <= 32)
+    std::vector<double> hd(n), he(n);
+    HIP_OK(hipMemcpyAsync(hd.data(), dd.p, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(he.data(), de.p, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    std::vector<std::vector<DcNode>> levels;  // levels[0] = root
+    std::vector<DcNode> cur{DcNode{0, n, n / 2}};
+    std::vector<int> ls, lm;
+    while (!cur.empty()) {
+      std::vector<DcNode> next, merges;
+      for (auto& N : cur) {
+        if (N.m <= 32) {
+          ls.push_back(N.s);
+          lm.push_back(N.m);
+        } else {
+          DcNode M{N.s, N.m, N.m / 2};
+          merges.push_back(M);
+          next.push_back(DcNode{N.s, M.m1, 0});
+          next.push_back(DcNode{N.s + M.m1, N.m - M.m1, 0});
+          // tearing: T = diag(T1', T2') + |beta| u u^T
+          double b = std::fabs(he[N.s + M.m1 - 1]);
+          hd[N.s + M.m1 - 1] -= b;
+          hd[N.s + M.m1] -= b;
+        }
+      }
+      if (!merges.empty()) levels.push_back(merges);
+      cur = next;
+    }
+    DBuf<double>& dt = dkd;  // torn diagonal (dkd is free until the merges)
+    dt.upload(hd, st);
+    const int nleaf = (int)ls.size();
+    dls.upload(ls, st);
+    dlm.upload(lm, st);
+    dleaf.ensure((size_t)nleaf * 32 * 32);
+    dleafV.ensure((size_t)nleaf * 32 * 32);
+    dleafZ.ensure((size_t)nleaf * 32 * 32);
+    dleafw.ensure((size_t)nleaf * 32);
+    LAUNCH(k_dc_leaves, dim3(nleaf), dim3(256), 0, (const int*)dls.p, (const int*)dlm.p, (const double*)dt.p,
+           (const double*)de.p, dleaf.p, 32);
+    std::vector<EigJob> jobs(nleaf);
+    for (int l = 0; l < nleaf; l++)
+      jobs[l] = EigJob{lm[l], dleaf.p + (size_t)l * 1024, dleafV.p + (size_t)l * 1024, dleafw.p + (size_t)l * 32,
+                       dleafZ.p + (size_t)l * 1024};
+    eigh(jobs);
+    dq_Q.ensure((size_t)n * n);
+    dq_Q2.ensure((size_t)n * n);
+    fill(dq_Q.p, (long long)n * n);
+    fill(dq_Q2.p, (long long)n * n);
+    LAUNCH(k_dc_scatter, dim3(nleaf), dim3(256), 0, (const int*)dls.p, (const int*)dlm.p, (const double*)dleafZ.p,
+           (const double*)dleafw.p, 32, dq_Q.p, n, dD.p);
+    // merges, deepest level first
+    for (int lv = (int)levels.size() - 1; lv >= 0; lv--) {
+      auto& nodes = levels[lv];
+      const int nn = (int)nodes.size();
+      dnodes.upload(nodes, st);
+      dnk.ensure(nn);
+      dnr.ensure(nn);
+      drho.ensure(nn);
+      int mmax = 0;
+      for (auto& N : nodes) mmax = std::max(mmax, N.m);
+      LAUNCH(k_dc_prep, dim3(nn), dim3(256), 0, (const DcNode*)dnodes.p, (const double*)dq_Q.p, n,
+             (const double*)dD.p, (const double*)de.p, dds.p, dzs.p, dperm.p, dkind.p, drot_ij.p, drot_cs.p, dnk.p,
+             dnr.p, drho.p);
+      std::vector<int> hk(nn), hr(nn);
+      HIP_OK(hipMemcpyAsync(hk.data(), dnk.p, nn * sizeof(int), hipMemcpyDeviceToHost, st));
+      HIP_OK(hipMemcpyAsync(hr.data(), dnr.p, nn * sizeof(int), hipMemcpyDeviceToHost, st));
+      HIP_OK(hipStreamSynchronize(st));
+      int kmax = 0, rmax = 0;
+      std::vector<long long> uo(nn), go(nn);
+      long long ut = 0, gt = 0;
+      for (int i = 0; i < nn; i++) {
+        kmax = std::max(kmax, hk[i]);
+        rmax = std::max(rmax, hr[i]);
+        uo[i] = ut;
+        go[i] = gt;
+        ut += (long long)hk[i] * hk[i];
+        gt += (long long)nodes[i].m * hk[i];
+      }
+      duoff.upload(uo, st);
+      dgoff.upload(go, st);
+      dU.ensure(std::max<long long>(ut, 1));
+      dG.ensure(std::max<long long>(gt, 1));
+      dQG.ensure(std::max<long long>(gt, 1));
+      if (rmax > 0)
+        LAUNCH(k_dc_rot, dim3(blocks(mmax, 256), nn), dim3(256), 0, (const DcNode*)dnodes.p, (const int*)dnr.p,
+               (const int*)drot_ij.p, (const double*)drot_cs.p, dq_Q.p, n);
+      LAUNCH(k_dc_compact, dim3(nn), dim3(64), 0, (const DcNode*)dnodes.p, (const int*)dkind.p, (const double*)dds.p,
+             (const double*)dzs.p, (const int*)dperm.p, dkd.p, dkz.p, dkcol.p);
+      if (kmax > 0) {
+        LAUNCH(k_dc_secular, dim3(blocks(kmax, 64), nn), dim3(64), 0, (const DcNode*)dnodes.p, (const int*)dnk.p,
+               (const double*)drho.p, (const double*)dkd.p, (const double*)dkz.p, dtaus.p, dorg.p);
+        LAUNCH(k_dc_zhat, dim3(blocks(kmax, 64), nn), dim3(64), 0, (const DcNode*)dnodes.p, (const int*)dnk.p,
+               (const double*)drho.p, (const double*)dkd.p, (const double*)dkz.p, (const double*)dtaus.p,
+               (const int*)dorg.p, dzh.p);
+        LAUNCH(k_dc_u, dim3(kmax, nn), dim3(256), 0, (const DcNode*)dnodes.p, (const int*)dnk.p,
+               (const long long*)duoff.p, (const double*)dkd.p, (const double*)dzh.p, (const double*)dtaus.p,
+               (const int*)dorg.p, dU.p);
+        LAUNCH(k_dc_gather, dim3(blocks((long long)mmax * kmax, 256), nn), dim3(256), 0, (const DcNode*)dnodes.p,
+               (const int*)dnk.p, (const long long*)dgoff.p, (const int*)dkcol.p, (const double*)dq_Q.p, n, dG.p);
+        std::vector<GemmJob> gjobs;
+        for (int i = 0; i < nn; i++)
+          if (hk[i] > 0) {
+            int m = nodes[i].m, K = hk[i];
+            gjobs.push_back(GemmJob{m, K, K, 0, 0, dG.p + go[i], dU.p + uo[i], dQG.p + go[i], nullptr, m, K, m, 1.0, 0.0});
+          }
+        gemm(gjobs);
+      }
+      LAUNCH(k_dc_vals, dim3(blocks(nn, 64)), dim3(64), 0, (const DcNode*)dnodes.p, nn, (const int*)dnk.p,
+             (const int*)dkind.p, (const double*)dds.p, (const int*)dperm.p, (const double*)dkd.p,
+             (const double*)dtaus.p, (const int*)dorg.p, dval.p, dsrc.p);
+      LAUNCH(k_dc_place, dim3(mmax, nn), dim3(256), 0, (const DcNode*)dnodes.p, (const long long*)dgoff.p,
+             (const double*)dval.p, (const int*)dsrc.p, (const double*)dQG.p, (const double*)dq_Q.p, n, dq_Q2.p,
+             dD2.p);
+      LAUNCH(k_dc_copyback, dim3(mmax, nn), dim3(256), 0, (const DcNode*)dnodes.p, (const double*)dq_Q2.p,
+             (const double*)dD2.p, n, dq_Q.p, dD.p);
+      HIP_OK(hipGetLastError());
+    }
+    HIP_OK(hipMemcpyAsync(w, dD.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+  }
+
+  // The first k eigenvectors of the last dense_values() matrix into Z (n x k, ld n): the
+  // tridiagonal eigenvectors transformed back with the Householder reflectors, nb per panel
+  // (Z <- (I - V T V^T) Z per panel, last panel first).
+  void dense_vectors(int k, double* Z) {
+    const int n = dense_n;
+    double* A = dense_A;
+    if (k <= 0) return;
+    HIP_OK(hipMemcpyAsync(Z, dq_Q.p, (size_t)n * k * sizeof(double), hipMemcpyDeviceToDevice, st));
+    const int nb = 64;
+    const int nref = n - 1;  // reflectors 0..n-2
+    dbtV.ensure((size_t)n * nb);
+    dbtY.ensure((size_t)nb * k);
+    dbtY2.ensure((size_t)nb * k);
+    dbtT.ensure((size_t)nb * nb);
+    dbtVtV.ensure((size_t)nb * nb);
+    int np = (nref + nb - 1) / nb;
+    for (int p = np - 1; p >= 0; p--) {
+      const int i0 = p * nb;
+      const int b = std::min(nb, nref - i0);
+      const int rows = n - i0 - 1;
+      LAUNCH(k_bt_panel, dim3(blocks((long long)rows * b, 256)), dim3(256), 0, (const double*)A, n, i0, b, dbtV.p);
+      std::vector<GemmJob> j1{GemmJob{b, b, rows, 1, 0, dbtV.p, dbtV.p, dbtVtV.p, nullptr, rows, rows, b, 1.0, 0.0}};
+      gemm(j1);
+      LAUNCH(k_bt_tfactor, dim3(1), dim3(128), 0, (const double*)dbtVtV.p, (const double*)dtau.p, i0, b, dbtT.p);
+      double* Zr = Z + i0 + 1;
+      std::vector<GemmJob> j2{GemmJob{b, k, rows, 1, 0, dbtV.p, Zr, dbtY.p, nullptr, rows, n, b, 1.0, 0.0}};
+      gemm(j2);
+      std::vector<GemmJob> j3{GemmJob{b, k, b, 0, 0, dbtT.p, dbtY.p, dbtY2.p, nullptr, b, b, b, 1.0, 0.0}};
+      gemm(j3);
+      std::vector<GemmJob> j4{GemmJob{rows, k, b, 0, 0, dbtV.p, dbtY2.p, Zr, nullptr, rows, b, n, -1.0, 1.0}};
+      gemm(j4);
+    }
+    HIP_OK(hipGetLastError());
+  }
 
   void fill(double* p, long long n, double v = 0.0) {
     if (n <= 0) return;
@@ -1172,6 +1365,27 @@ int axtbhip_test_eigh(int n, int nb, const double* A, double* w, double* Z, int*
     HIP_OK(hipMemcpyAsync(w, ww.p, (size_t)n * nb * sizeof(double), hipMemcpyDeviceToHost, e.st));
     HIP_OK(hipMemcpyAsync(Z, z.p, nn * nb * sizeof(double), hipMemcpyDeviceToHost, e.st));
     if (sweeps) HIP_OK(hipMemcpyAsync(sweeps, e.jstat.p, nb * sizeof(int), hipMemcpyDeviceToHost, e.st));
+    HIP_OK(hipStreamSynchronize(e.st));
+    return 0;
+  } catch (std::exception&) {
+    return AXTB_ERR_DEVICE;
+  }
+}
+
+// The dense eigensolver (tridiagonalisation, divide and conquer, back-transformation) on one
+// symmetric n x n matrix: all eigenvalues w, the first k eigenvectors Z (n x k).
+int axtbhip_test_dense_eigh(int n, const double* A, int k, double* w, double* Z) {
+  try {
+    using namespace axtb;
+    Engine e(0);
+    DBuf<double> a, ww, z;
+    a.upload(std::vector<double>(A, A + (size_t)n * n), e.st);
+    ww.ensure(n);
+    z.ensure((size_t)n * std::max(k, 1));
+    e.dense_values(a.p, n, ww.p);
+    e.dense_vectors(k, z.p);
+    HIP_OK(hipMemcpyAsync(w, ww.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, e.st));
+    if (k > 0) HIP_OK(hipMemcpyAsync(Z, z.p, (size_t)n * k * sizeof(double), hipMemcpyDeviceToHost, e.st));
     HIP_OK(hipStreamSynchronize(e.st));
     return 0;
   } catch (std::exception&) {

@@ -8,6 +8,7 @@ extern "C" {
 int axtbhip_test_gemm(int m, int n, int k, int ta, int tb, const double* A, int lda, const double* B, int ldb,
                       const double* dk, double* C, int ldc);
 int axtbhip_test_eigh(int n, int nb, const double* A, double* w, double* Z, int* sweeps);
+int axtbhip_test_dense_eigh(int n, const double* A, int k, double* w, double* Z);
 int axtbhip_test_setup(const axtb_mol* mol, const axtb_opts* opts, double* S, double* H0, double* D, double* Q,
                        double* cn, double* qstart, double* X, double* e3);
 int axtbhip_test_iter1(const axtb_mol* mol, const axtb_opts* opts, double* F, double* w, double* P, double* qsh_out,

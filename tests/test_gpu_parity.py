@@ -148,6 +148,31 @@ def test_eigh():
     assert np.abs(A @ Z - Z * w).max() < 1e-12
 
 
+def test_dense_eigh():
+    """Householder + divide and conquer + blocked back-transformation: unbalanced merge trees
+    (65, 257), several levels (600), one leaf (20) and a clustered spectrum with deflation."""
+    from axtb_amd import _testhooks as H
+    rng = np.random.default_rng(3)
+    for n in (1, 2, 20, 65, 257, 600):
+        A = rng.standard_normal((n, n))
+        A = A + A.T
+        k = max(n // 2, 1)
+        w, Z = H.dense_eigh(A, k)
+        scale = np.abs(w).max()
+        assert np.abs(w - np.linalg.eigvalsh(A)).max() < 1e-14 * n * scale, n
+        assert np.abs(A @ Z - Z * w[:k]).max() < 1e-14 * n * scale, n
+        assert np.abs(Z.T @ Z - np.eye(k)).max() < 1e-13, n
+    n = 300
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    ev = np.repeat(np.linspace(-1, 1, 30), 10) + 1e-9 * rng.standard_normal(n)
+    A = Q @ np.diag(ev) @ Q.T
+    A = 0.5 * (A + A.T)
+    w, Z = H.dense_eigh(A)
+    assert np.abs(w - np.sort(ev)).max() < 1e-13
+    assert np.abs(A @ Z - Z * w).max() < 1e-13
+    assert np.abs(Z.T @ Z - np.eye(n)).max() < 1e-13
+
+
 def test_setup_and_first_iteration():
     import oracle
     from axtb_amd import _testhooks as H
