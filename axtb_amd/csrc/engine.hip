@@ -11,6 +11,7 @@
 // default conventions (crates/axtb/src/conventions.rs:244-268).
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -200,15 +201,32 @@ struct Engine {
   DBuf<GemmJob> gj;
   DBuf<EigJob> ej;
 
+  // matrices of order above dense_min are diagonalised by the dense solver (Householder, divide
+  // and conquer), smaller ones by the batched Jacobi solver; AXTB_HIP_DENSE_MIN overrides
+  int dense_min = 200;
+
   explicit Engine(int dev) : device(dev) {
     HIP_OK(hipSetDevice(dev));
     HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    if (const char* v = std::getenv("AXTB_HIP_DENSE_MIN")) dense_min = std::max(1, std::atoi(v));
+  }
+  bool dense(int n) const { return n > dense_min; }
+
+  // All eigenpairs of one symmetric matrix by the dense solver: A (destroyed), w, Z (n x n);
+  // Q is an n x n workspace.
+  DBuf<double> dtau1;
+  void dense_eigh(double* A, int n, double* w, double* Q, double* Z) {
+    dtau1.ensure(n);
+    dense_values(A, n, w, dtau1.p, Q);
+    dense_vectors(A, n, dtau1.p, Q, n, Z);
   }
   ~Engine() {
     if (st) (void)hipStreamDestroy(st);
   }
 
   void gemm(std::vector<GemmJob>& jobs) {
+    jobs.erase(std::remove_if(jobs.begin(), jobs.end(), [](const GemmJob& j) { return j.m <= 0 || j.n <= 0; }),
+               jobs.end());
     if (jobs.empty()) return;
     int mm = 0, nn = 0;
     for (auto& j : jobs) {
@@ -271,21 +289,18 @@ struct Engine {
   DBuf<EigJob> ejb[3];
 
   // ---------------------------------------------------------------- dense eigensolver (large n)
-  // Eigenvalues of the symmetric n x n matrix A (destroyed: it keeps the Householder vectors)
-  // into w; the tridiagonal eigenvectors stay in dq_Q for dense_vectors().
-  DBuf<double> dtau, dd, de, dpart, dwv, dq_Q, dq_Q2, dD, dD2, dds, dzs, drot_cs, dkd, dkz, dtaus, dzh, dval, dU, dG,
-      dQG, dleaf, dleafV, dleafZ, dleafw, dbtV, dbtY, dbtY2, dbtT, dbtVtV;
+  // Eigenvalues of the symmetric n x n matrix A into w (ascending). A is overwritten with the
+  // Householder vectors, tau (n) receives their scalars and Q (n x n) the eigenvectors of the
+  // tridiagonal matrix: the state dense_vectors() transforms back.
+  DBuf<double> dd, de, dpart, dwv, dq_Q2, dD, dD2, dds, dzs, drot_cs, dkd, dkz, dtaus, dzh, dval, dU, dG, dQG, dleaf,
+      dleafV, dleafZ, dleafw, dbtV, dbtY, dbtY2, dbtT, dbtVtV;
   DBuf<int> dperm, dkind, drot_ij, dkcol, dorg, dsrc, dnk, dnr, dls, dlm;
   DBuf<double> drho;
   DBuf<long long> duoff, dgoff;
   DBuf<DcNode> dnodes;
-  double* dense_A = nullptr;
-  int dense_n = 0;
 
-  void dense_values(double* A, int n, double* w) {
-    dense_A = A;
-    dense_n = n;
-    for (auto* b : {&dtau, &dd, &de, &dD, &dD2, &dds, &dzs, &dkd, &dkz, &dtaus, &dzh, &dval}) b->ensure(n);
+  void dense_values(double* A, int n, double* w, double* tau, double* Q) {
+    for (auto* b : {&dd, &de, &dD, &dD2, &dds, &dzs, &dkd, &dkz, &dtaus, &dzh, &dval}) b->ensure(n);
     drot_cs.ensure(2 * (size_t)n);
     for (auto* b : {&dperm, &dkind, &dkcol, &dorg, &dsrc}) b->ensure(n);
     drot_ij.ensure(2 * (size_t)n);
@@ -295,17 +310,17 @@ struct Engine {
     // 1. Householder tridiagonalisation
     for (int i = 0; i + 1 < n; i++) {
       const int m = n - i - 1;
-      LAUNCH(k_hh_reflector, dim3(1), dim3(256), 0, A, n, i, dtau.p, dd.p, de.p);
+      LAUNCH(k_hh_reflector, dim3(1), dim3(256), 0, A, n, i, tau, dd.p, de.p);
       if (m > 1) {
         const int nc = (m + 63) / 64;
         LAUNCHW(8.0 * m * (double)m, k_symv_part, dim3(blocks(m, 256), nc), dim3(256), 0, (const double*)A, n, i,
                 dpart.p);
-        LAUNCH(k_hh_w, dim3(1), dim3(1024), 0, (const double*)A, n, i, (const double*)dtau.p, (const double*)dpart.p,
+        LAUNCH(k_hh_w, dim3(1), dim3(1024), 0, (const double*)A, n, i, (const double*)tau, (const double*)dpart.p,
                nc, dwv.p);
         LAUNCHW(16.0 * m * (double)m, k_syr2, dim3(blocks(m, 256), m), dim3(256), 0, A, n, i, (const double*)dwv.p);
       }
     }
-    LAUNCH(k_hh_last, dim3(1), dim3(1), 0, (const double*)A, n, dd.p, de.p, dtau.p);
+    LAUNCH(k_hh_last, dim3(1), dim3(1), 0, (const double*)A, n, dd.p, de.p, tau);
     HIP_OK(hipGetLastError());
     // 2. divide and conquer on (d, e): the merge tree by recursive halving (leaves <= 32)
     std::vector<double> hd(n), he(n);
@@ -351,12 +366,11 @@ struct Engine {
       jobs[l] = EigJob{lm[l], dleaf.p + (size_t)l * 1024, dleafV.p + (size_t)l * 1024, dleafw.p + (size_t)l * 32,
                        dleafZ.p + (size_t)l * 1024};
     eigh(jobs);
-    dq_Q.ensure((size_t)n * n);
     dq_Q2.ensure((size_t)n * n);
-    fill(dq_Q.p, (long long)n * n);
+    fill(Q, (long long)n * n);
     fill(dq_Q2.p, (long long)n * n);
     LAUNCH(k_dc_scatter, dim3(nleaf), dim3(256), 0, (const int*)dls.p, (const int*)dlm.p, (const double*)dleafZ.p,
-           (const double*)dleafw.p, 32, dq_Q.p, n, dD.p);
+           (const double*)dleafw.p, 32, Q, n, dD.p);
     // merges, deepest level first
     for (int lv = (int)levels.size() - 1; lv >= 0; lv--) {
       auto& nodes = levels[lv];
@@ -367,7 +381,7 @@ struct Engine {
       drho.ensure(nn);
       int mmax = 0;
       for (auto& N : nodes) mmax = std::max(mmax, N.m);
-      LAUNCH(k_dc_prep, dim3(nn), dim3(256), 0, (const DcNode*)dnodes.p, (const double*)dq_Q.p, n,
+      LAUNCH(k_dc_prep, dim3(nn), dim3(256), 0, (const DcNode*)dnodes.p, (const double*)Q, n,
              (const double*)dD.p, (const double*)de.p, dds.p, dzs.p, dperm.p, dkind.p, drot_ij.p, drot_cs.p, dnk.p,
              dnr.p, drho.p);
       std::vector<int> hk(nn), hr(nn);
@@ -392,7 +406,7 @@ struct Engine {
       dQG.ensure(std::max<long long>(gt, 1));
       if (rmax > 0)
         LAUNCH(k_dc_rot, dim3(blocks(mmax, 256), nn), dim3(256), 0, (const DcNode*)dnodes.p, (const int*)dnr.p,
-               (const int*)drot_ij.p, (const double*)drot_cs.p, dq_Q.p, n);
+               (const int*)drot_ij.p, (const double*)drot_cs.p, Q, n);
       LAUNCH(k_dc_compact, dim3(nn), dim3(64), 0, (const DcNode*)dnodes.p, (const int*)dkind.p, (const double*)dds.p,
              (const double*)dzs.p, (const int*)dperm.p, dkd.p, dkz.p, dkcol.p);
       if (kmax > 0) {
@@ -405,7 +419,7 @@ struct Engine {
                (const long long*)duoff.p, (const double*)dkd.p, (const double*)dzh.p, (const double*)dtaus.p,
                (const int*)dorg.p, dU.p);
         LAUNCH(k_dc_gather, dim3(blocks((long long)mmax * kmax, 256), nn), dim3(256), 0, (const DcNode*)dnodes.p,
-               (const int*)dnk.p, (const long long*)dgoff.p, (const int*)dkcol.p, (const double*)dq_Q.p, n, dG.p);
+               (const int*)dnk.p, (const long long*)dgoff.p, (const int*)dkcol.p, (const double*)Q, n, dG.p);
         std::vector<GemmJob> gjobs;
         for (int i = 0; i < nn; i++)
           if (hk[i] > 0) {
@@ -418,10 +432,10 @@ struct Engine {
              (const int*)dkind.p, (const double*)dds.p, (const int*)dperm.p, (const double*)dkd.p,
              (const double*)dtaus.p, (const int*)dorg.p, dval.p, dsrc.p);
       LAUNCH(k_dc_place, dim3(mmax, nn), dim3(256), 0, (const DcNode*)dnodes.p, (const long long*)dgoff.p,
-             (const double*)dval.p, (const int*)dsrc.p, (const double*)dQG.p, (const double*)dq_Q.p, n, dq_Q2.p,
+             (const double*)dval.p, (const int*)dsrc.p, (const double*)dQG.p, (const double*)Q, n, dq_Q2.p,
              dD2.p);
       LAUNCH(k_dc_copyback, dim3(mmax, nn), dim3(256), 0, (const DcNode*)dnodes.p, (const double*)dq_Q2.p,
-             (const double*)dD2.p, n, dq_Q.p, dD.p);
+             (const double*)dD2.p, n, Q, dD.p);
       HIP_OK(hipGetLastError());
     }
     HIP_OK(hipMemcpyAsync(w, dD.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
@@ -430,11 +444,9 @@ struct Engine {
   // The first k eigenvectors of the last dense_values() matrix into Z (n x k, ld n): the
   // tridiagonal eigenvectors transformed back with the Householder reflectors, nb per panel
   // (Z <- (I - V T V^T) Z per panel, last panel first).
-  void dense_vectors(int k, double* Z) {
-    const int n = dense_n;
-    double* A = dense_A;
+  void dense_vectors(const double* A, int n, const double* tau, const double* Q, int k, double* Z) {
     if (k <= 0) return;
-    HIP_OK(hipMemcpyAsync(Z, dq_Q.p, (size_t)n * k * sizeof(double), hipMemcpyDeviceToDevice, st));
+    HIP_OK(hipMemcpyAsync(Z, Q, (size_t)n * k * sizeof(double), hipMemcpyDeviceToDevice, st));
     const int nb = 64;
     const int nref = n - 1;  // reflectors 0..n-2
     dbtV.ensure((size_t)n * nb);
@@ -450,7 +462,7 @@ struct Engine {
       LAUNCH(k_bt_panel, dim3(blocks((long long)rows * b, 256)), dim3(256), 0, (const double*)A, n, i0, b, dbtV.p);
       std::vector<GemmJob> j1{GemmJob{b, b, rows, 1, 0, dbtV.p, dbtV.p, dbtVtV.p, nullptr, rows, rows, b, 1.0, 0.0}};
       gemm(j1);
-      LAUNCH(k_bt_tfactor, dim3(1), dim3(128), 0, (const double*)dbtVtV.p, (const double*)dtau.p, i0, b, dbtT.p);
+      LAUNCH(k_bt_tfactor, dim3(1), dim3(128), 0, (const double*)dbtVtV.p, tau, i0, b, dbtT.p);
       double* Zr = Z + i0 + 1;
       std::vector<GemmJob> j2{GemmJob{b, k, rows, 1, 0, dbtV.p, Zr, dbtY.p, nullptr, rows, n, b, 1.0, 0.0}};
       gemm(j2);
@@ -877,10 +889,14 @@ struct Engine {
     if (o.guess == 0) {
       LAUNCH(k_eeq_system, dim3(blocks((long long)(natmax + 1) * (natmax + 1), 256), NM), dim3(256), 0,
                          b, eq0.p, cneeq.p, eeqA.p, eeqr.p);
-      std::vector<EigJob> jobs(NM);
+      std::vector<EigJob> jobs;
       for (int m = 0; m < NM; m++) {
         int n1 = mols[m].nat + 1;
-        jobs[m] = EigJob{n1, eeqA.p + heq0[m], eeqV.p + heq0[m], eeqw.p + mols[m].at0 + m, eeqZ.p + heq0[m]};
+        EigJob j{n1, eeqA.p + heq0[m], eeqV.p + heq0[m], eeqw.p + mols[m].at0 + m, eeqZ.p + heq0[m]};
+        if (dense(n1))
+          dense_eigh(j.A, n1, j.w, j.V, j.Z);
+        else
+          jobs.push_back(j);
       }
       eigh(jobs);
       LAUNCH(k_eeq_solve, dim3(NM), dim3(256), 0, b, eq0.p, eeqZ.p, eeqw.p, eeqr.p, eeqy.p, qstart.p,
@@ -888,12 +904,16 @@ struct Engine {
     }
     // Loewdin orthogonaliser X = U s^-1/2 U^T from the eigenpairs of S
     {
-      std::vector<EigJob> jobs(NM);
+      std::vector<EigJob> jobs;
       for (int m = 0; m < NM; m++) {
         long long o0 = mols[m].mat0;
         HIP_OK(hipMemcpyAsync(Am.p + o0, pool.p + o0, sizeof(double) * mols[m].nao * mols[m].nao,
                               hipMemcpyDeviceToDevice, st));
-        jobs[m] = EigJob{mols[m].nao, Am.p + o0, Vw.p + o0, w.p + mols[m].ao0, Z.p + o0};
+        EigJob j{mols[m].nao, Am.p + o0, Vw.p + o0, w.p + mols[m].ao0, Z.p + o0};
+        if (dense(j.n))
+          dense_eigh(j.A, j.n, j.w, j.V, j.Z);
+        else
+          jobs.push_back(j);
       }
       eigh(jobs);
       std::vector<double> hw(NO);
@@ -916,7 +936,7 @@ struct Engine {
       gemm(gjobs);
     }
   }
-  DBuf<double> s12_;
+  DBuf<double> s12_, mtau;
 
   // ------------------------------------------------------------------------------ SCC loop
   std::vector<int> hactive;
@@ -938,30 +958,39 @@ struct Engine {
     hactive.assign(NM, 1);
     std::vector<GemmJob> g1(NM), g2(NM), g3v(NM), gP(NM);
     std::vector<EigJob> ejobs(NM);
+    std::vector<int> isdense(NM);
+    bool anydense = false;
+    mtau.ensure(NO);
     for (int m = 0; m < NM; m++) {
       const Mol& M = mols[m];
       long long o0 = M.mat0;
       int n = M.nao;
+      isdense[m] = dense(n);
+      anydense |= isdense[m] != 0;
       // the eigenproblem in the basis B: the Loewdin basis X at the start, then the previous
-      // iteration's orbitals (A = B^T F B is then nearly diagonal and Jacobi needs few sweeps)
-      g1[m] = GemmJob{n, n, n, 0, 0, F.p + o0, Bm.p + o0, T.p + o0, nullptr, n, n, n, 1.0, 0.0};   // T = F B
-      g2[m] = GemmJob{n, n, n, 1, 0, Bm.p + o0, T.p + o0, Am.p + o0, nullptr, n, n, n, 1.0, 0.0};  // A = B^T T
-      g3v[m] = GemmJob{n, n, n, 0, 0, Bm.p + o0, Z.p + o0, C.p + o0, nullptr, n, n, n, 1.0, 0.0};  // C = B U
+      // iteration's orbitals (A = B^T F B is then nearly diagonal and Jacobi needs few sweeps).
+      // The dense solver does not profit from a nearly diagonal matrix: B = X throughout.
+      double* B = isdense[m] ? X.p + o0 : Bm.p + o0;
+      g1[m] = GemmJob{n, n, n, 0, 0, F.p + o0, B, T.p + o0, nullptr, n, n, n, 1.0, 0.0};   // T = F B
+      g2[m] = GemmJob{n, n, n, 1, 0, B, T.p + o0, Am.p + o0, nullptr, n, n, n, 1.0, 0.0};  // A = B^T T
+      g3v[m] = GemmJob{n, n, n, 0, 0, B, Z.p + o0, C.p + o0, nullptr, n, n, n, 1.0, 0.0};  // C = B U
       gP[m] = GemmJob{n, n, n, 0, 1, C.p + o0, C.p + o0, P.p + o0, focc.p + M.ao0, n, n, n, 1.0, 0.0};
       ejobs[m] = EigJob{n, Am.p + o0, Vw.p + o0, w.p + M.ao0, Z.p + o0};
     }
+    std::vector<int> hkocc(NM);
     HIP_OK(hipMemcpyAsync(Bm.p, X.p, sizeof(double) * tot_mat, hipMemcpyDeviceToDevice, st));
     int it = 0;
     for (it = 1; it <= o.max_iter; it++) {
-      std::vector<GemmJob> a1, a2, a3, a4;
+      std::vector<GemmJob> a1, a2, a3;
       std::vector<EigJob> ea;
       for (int m = 0; m < NM; m++)
         if (hactive[m]) {
           a1.push_back(g1[m]);
           a2.push_back(g2[m]);
-          a3.push_back(g3v[m]);
-          a4.push_back(gP[m]);
-          ea.push_back(ejobs[m]);
+          if (!isdense[m]) {
+            a3.push_back(g3v[m]);
+            ea.push_back(ejobs[m]);
+          }
         }
       if (a1.empty()) break;
       if (it > 1)
@@ -987,10 +1016,33 @@ struct Engine {
       gemm(a1);
       gemm(a2);
       eigh(ea);
+      for (int m = 0; m < NM; m++)
+        if (hactive[m] && isdense[m])
+          dense_values(ejobs[m].A, ejobs[m].n, ejobs[m].w, mtau.p + mols[m].ao0, ejobs[m].V);
       gemm(a3);
       HIP_OK(hipMemcpyAsync(Bm.p, C.p, sizeof(double) * tot_mat, hipMemcpyDeviceToDevice, st));
       LAUNCH(k_fermi, dim3(NM), dim3(256), 0, b, active.p, w.p, focc.p, fwork.p, fst.p, kocc.p,
                          status.p);
+      // dense molecules: only the kocc orbitals with occupation > 1e-16 are formed, C = X U
+      if (anydense) {
+        HIP_OK(hipMemcpyAsync(hkocc.data(), kocc.p, sizeof(int) * NM, hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+      }
+      std::vector<GemmJob> a4, a5;
+      for (int m = 0; m < NM; m++) {
+        if (!hactive[m]) continue;
+        GemmJob p = gP[m];
+        if (isdense[m]) {
+          const int n = mols[m].nao, k = hkocc[m];
+          dense_vectors(ejobs[m].A, n, mtau.p + mols[m].ao0, ejobs[m].V, k, ejobs[m].Z);
+          GemmJob c = g3v[m];
+          c.n = k;
+          a5.push_back(c);
+          p.k = k;
+        }
+        a4.push_back(p);
+      }
+      gemm(a5);
       gemm(a4);  // P = C diag(f) C^T
       LAUNCHW(sbytes, k_mulliken_cols, dim3(blocks(NO, 4)), dim3(256), 0, b, active.p, pool.p, stride, P.p,
                          cols.p);
@@ -1032,12 +1084,17 @@ struct Engine {
     HIP_OK(hipStreamSynchronize(st));
     hok.assign(NM, 0);
     std::vector<GemmJob> gW;
+    std::vector<int> hk(NM);
+    HIP_OK(hipMemcpyAsync(hk.data(), kocc.p, sizeof(int) * NM, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
     for (int m = 0; m < NM; m++) {
       hok[m] = (hm[8 * m + 6] == 1.0 && hst[m] == 0) ? 1 : 0;
       if (!hok[m]) continue;
       const Mol& M = mols[m];
       int n = M.nao;
-      gW.push_back(GemmJob{n, n, n, 0, 1, C.p + M.mat0, C.p + M.mat0, Wm.p + M.mat0, fwork.p + M.ao0, n, n, n, 1.0, 0.0});
+      // dense molecules hold only the first kocc orbitals (the others have occupation <= 1e-16)
+      int k = dense(n) ? hk[m] : n;
+      gW.push_back(GemmJob{n, n, k, 0, 1, C.p + M.mat0, C.p + M.mat0, Wm.p + M.mat0, fwork.p + M.ao0, n, n, n, 1.0, 0.0});
     }
     DBuf<int> okd;
     okd.upload(hok, st);
@@ -1378,12 +1435,14 @@ int axtbhip_test_dense_eigh(int n, const double* A, int k, double* w, double* Z)
   try {
     using namespace axtb;
     Engine e(0);
-    DBuf<double> a, ww, z;
+    DBuf<double> a, ww, z, tau, q;
     a.upload(std::vector<double>(A, A + (size_t)n * n), e.st);
     ww.ensure(n);
+    tau.ensure(n);
+    q.ensure((size_t)n * n);
     z.ensure((size_t)n * std::max(k, 1));
-    e.dense_values(a.p, n, ww.p);
-    e.dense_vectors(k, z.p);
+    e.dense_values(a.p, n, ww.p, tau.p, q.p);
+    e.dense_vectors(a.p, n, tau.p, q.p, k, z.p);
     HIP_OK(hipMemcpyAsync(w, ww.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, e.st));
     if (k > 0) HIP_OK(hipMemcpyAsync(Z, z.p, (size_t)n * k * sizeof(double), hipMemcpyDeviceToHost, e.st));
     HIP_OK(hipStreamSynchronize(e.st));

@@ -62,6 +62,35 @@ def test_water20_config0(ctx):
     check(ctx, z, x, 1e-2)
 
 
+def test_c100h202_dense_solver(ctx):
+    """n = 602: EEQ, Loewdin and every SCC step on the dense eigensolver."""
+    from axtb_amd import structures as S
+    z, x = S.config("c100h202")
+    r, _ = check(ctx, z, x, 1e-2)
+    assert r["nao"] == 602
+
+
+def test_dense_solver_everywhere(monkeypatch):
+    """Caffeine with every matrix (EEQ 25, overlap and Fock 66) on the dense eigensolver, and a
+    batch mixing both solvers."""
+    from axtb_amd import _lib
+    from axtb_amd import structures as S
+    monkeypatch.setenv("AXTB_HIP_DENSE_MIN", "1")
+    c1 = _lib.Context(0)
+    z, x = structure("caffeine")
+    check(c1, z, x, 1e-6)
+    monkeypatch.setenv("AXTB_HIP_DENSE_MIN", "40")
+    c2 = _lib.Context(0)
+    import oracle
+    mols = [structure("water"), structure("caffeine"), S.config("c30h62")]
+    res = c2.engrad_batch(mols, _lib.make_opts(accuracy=1e-2))
+    for (z, x), r in zip(mols, res):
+        o = oracle.engrad(z, x, acc=1e-2)
+        assert abs(r["energy"] - o["energy"]) <= 1e-8
+        assert np.abs(r["gradient"] - o["gradient"]).max() <= 1e-7
+        assert r["iterations"] == o["iterations"]
+
+
 def test_charged_and_invariants(ctx):
     z, x = structure("caffeine")
     r, o = check(ctx, z, x, 1e-6, charge=1.0)
