@@ -7,92 +7,215 @@
 namespace axtb {
 
 // ------------------------------------------------------------------ Householder tridiagonalisation
-// Reflector of column i: x = A[i+1:n, i]; H = I - tau v v^T with v[0] = 1 maps x to (beta, 0, ...).
-// v is stored in A[i+1:n, i], d[i] = A[i][i], e[i] = beta. One workgroup.
-__global__ void k_hh_reflector(double* A, int n, int i, double* tau, double* d, double* e) {
-  __shared__ double red[16];
-  const int m = n - i - 1;
-  double* x = A + (long long)i * n + i + 1;
-  double s = 0.0;
-  for (int k = 1 + threadIdx.x; k < m; k += blockDim.x) s += x[k] * x[k];
-  s = block_sum(s, red);
-  const double alpha = x[0];
-  if (threadIdx.x == 0) d[i] = A[(long long)i * n + i];
-  if (s == 0.0) {
-    // nothing to annihilate: H = I
-    if (threadIdx.x == 0) {
-      tau[i] = 0.0;
-      e[i] = alpha;
-      x[0] = 1.0;
-    }
+// Blocked reduction to tridiagonal form (LAPACK dsytrd/dlatrd, lower): panels of TRD_NB columns;
+// within a panel the trailing matrix is left as it was and corrected through the panel's
+// reflectors V (kept in A below the diagonal, unit first element stored) and W; at the end of a
+// panel A22 -= V W^T + W V^T runs as two GEMMs on the matrix cores. Two launches per column:
+// k_trd_col (W column of the previous reflector, update of the column, new reflector) and
+// k_trd_symv (A22 v in row x column-chunk partials, W^T v, V^T v and v^T A22 v partials).
+// With y1 = W^T v, y2 = V^T v and p = A22 v of reflector v (scalar tau):
+//   w = tau (p - V y1 - W y2) + alpha v,  alpha = -tau^2 (v^T p - 2 y1.y2) / 2.
+// A is kept symmetric in full storage (both triangles of the trailing matrix are updated).
+constexpr int TRD_NB = 64;   // panel width
+constexpr int TRD_CB = 64;   // rows per workgroup of the column kernel (4 wavefronts share the terms)
+constexpr int TRD_SR = 64;   // rows per symv workgroup (4 wavefronts split the chunk's columns)
+constexpr int TRD_CH = 128;  // columns per symv chunk
+constexpr int TRD_DR = 1024; // rows per dot workgroup (one column per wavefront)
+
+// Reflector of x = (ax, x1...) with sum(x1^2) = tot: H x = (beta, 0, ...), v = (1, x1 * scal).
+__device__ __forceinline__ void trd_reflector(double ax, double tot, double& tau, double& beta, double& scal) {
+  if (tot == 0.0) {
+    tau = 0.0;
+    beta = ax;
+    scal = 1.0;
     return;
   }
-  const double nrm = sqrt(alpha * alpha + s);
-  const double beta = alpha >= 0.0 ? -nrm : nrm;
-  const double scal = 1.0 / (alpha - beta);
-  for (int k = 1 + threadIdx.x; k < m; k += blockDim.x) x[k] *= scal;
-  if (threadIdx.x == 0) {
-    tau[i] = (beta - alpha) / beta;
-    e[i] = beta;
-    x[0] = 1.0;
-  }
+  const double nrm = sqrt(ax * ax + tot);
+  beta = ax >= 0.0 ? -nrm : nrm;
+  tau = (beta - ax) / beta;
+  scal = 1.0 / (ax - beta);
 }
 
-// Partial products of p = A22 v (A22 = A[i+1:, i+1:]) over column chunks of 64: part[chunk][row].
-__global__ void k_symv_part(const double* A, int n, int i, double* part) {
-  const int m = n - i - 1;
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  const int c0 = blockIdx.y * 64;
-  __shared__ double vs[64];
-  const double* v = A + (long long)i * n + i + 1;
-  if (threadIdx.x < 64) vs[threadIdx.x] = c0 + (int)threadIdx.x < m ? v[c0 + threadIdx.x] : 0.0;
+// Sum of the column kernel's squares partials of column i (fixed order; the same in every lane
+// and every wavefront).
+__device__ __forceinline__ double trd_sumsq(const double* sq, int n, int i, int lane) {
+  const int nq = (n - i + TRD_CB - 1) / TRD_CB;
+  double t = 0.0;
+  for (int q = lane; q < nq; q += 64) t += sq[q];
+  return wave_sum(t);
+}
+
+// Column kernel for column i = i0 + j, rows [i, n), 64 rows per workgroup of 4 wavefronts (the
+// wavefronts share the column terms, c = w mod 4). For the previous column ip = i - 1 (j > 0):
+// its reflector (from the squares partials and the stored first element ax[ip]) is scaled in
+// place, tau/e/d written, and W(:, j-1) formed from the symv partials. Then (unless
+// finalize_only) column i is updated with the panel's reflectors so far, its first
+// sub-diagonal element saved in ax[i] and its squares partials written to sq_cur. Nothing here
+// reads what another workgroup of the same launch writes.
+__global__ __launch_bounds__(256) void k_trd_col(double* A, int n, int i0, int j, double* W, const double* part,
+                                                 const double* dpart, const double* vpart, const double* sq_prev,
+                                                 double* sq_cur, double* axv, double* tau, double* d, double* e,
+                                                 int finalize_only) {
+  __shared__ double y1[TRD_NB], y2[TRD_NB], wi[TRD_NB], vi[TRD_NB];
+  __shared__ double red[3][4][64];
+  __shared__ double rs[16];
+  const int i = i0 + j, jj = j - 1, ip = i - 1;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int r = i + blockIdx.x * TRD_CB + lane;  // global row
+  const long long ln = n;
+  double t = 0.0, alpha = 0.0, scal = 1.0;
+  int nch = 0;
+  if (j > 0) {
+    const int mp = n - i;  // rows of the previous reflector and of its symv
+    const int nrb = (mp + TRD_SR - 1) / TRD_SR, ndb = (mp + TRD_DR - 1) / TRD_DR;
+    nch = (mp + TRD_CH - 1) / TRD_CH;
+    double beta;
+    trd_reflector(axv[ip], trd_sumsq(sq_prev, n, ip, lane), t, beta, scal);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      tau[ip] = t;
+      e[ip] = beta;
+      d[ip] = A[ip * ln + ip];
+    }
+    for (int c = threadIdx.x; c < 2 * jj; c += 256) {
+      double s = 0.0;
+      for (int q = 0; q < ndb; q++) s += dpart[q * 2 * TRD_NB + c];
+      if (c < jj)
+        y1[c] = s;
+      else
+        y2[c - jj] = s;
+    }
+    double va = 0.0;
+    for (int k = threadIdx.x; k < nrb * nch; k += 256) va += vpart[k];
+    va = block_sum(va, rs);  // its barriers also publish y1, y2
+    double yy = 0.0;
+    for (int c = 0; c < jj; c++) yy += y1[c] * y2[c];
+    alpha = -0.5 * t * t * (va - 2.0 * yy);
+  }
+  if (!finalize_only) {
+    // coefficients of row i: V(i, c), W(i, c); V(i, jj) is the unit first element
+    const int c = threadIdx.x;
+    if (c < jj) {
+      vi[c] = A[(i0 + c) * ln + i];
+      wi[c] = W[c * ln + i];
+    }
+    if (j > 0 && wv == 0) {
+      double p = 0.0;
+      for (int ch = lane; ch < nch; ch += 64) p += part[ch * ln + i];
+      double sw = lane < jj ? A[(i0 + lane) * ln + i] * y1[lane] + W[lane * ln + i] * y2[lane] : 0.0;
+      p = wave_sum(p);
+      sw = wave_sum(sw);
+      if (lane == 0) {
+        vi[jj] = 1.0;
+        wi[jj] = t * (p - sw) + alpha;
+      }
+    }
+    __syncthreads();
+  }
+  // per wavefront: its share of p_r, of sw_r = V y1 + W y2 and of u_r = V wi + W vi
+  double ps = 0.0, sw = 0.0, u = 0.0;
+  if (r < n) {
+    for (int ch = wv; ch < nch; ch += 4) ps += part[ch * ln + r];
+#pragma unroll 4
+    for (int c = wv; c < jj; c += 4) {
+      const double v = A[(i0 + c) * ln + r], w = W[c * ln + r];
+      sw += v * y1[c] + w * y2[c];
+      u += v * wi[c] + w * vi[c];
+    }
+  }
+  red[0][wv][lane] = ps;
+  red[1][wv][lane] = sw;
+  red[2][wv][lane] = u;
   __syncthreads();
-  if (r >= m) return;
-  const double* a = A + (long long)(i + 1) * n + (i + 1);
-  double s = 0.0;
-  const int c1 = min(c0 + 64, m);
-  for (int c = c0; c < c1; c++) s += a[(long long)c * n + r] * vs[c - c0];
-  part[(long long)blockIdx.y * m + r] = s;
+  if (wv != 0) return;
+  double a = 0.0;
+  if (r < n) {
+    const double p = red[0][0][lane] + red[0][1][lane] + red[0][2][lane] + red[0][3][lane];
+    const double swt = red[1][0][lane] + red[1][1][lane] + red[1][2][lane] + red[1][3][lane];
+    double ut = red[2][0][lane] + red[2][1][lane] + red[2][2][lane] + red[2][3][lane];
+    if (j > 0) {
+      const double vp = r == i ? 1.0 : A[ip * ln + r] * scal;
+      A[ip * ln + r] = vp;
+      const double wr = t * (p - swt) + alpha * vp;
+      W[jj * ln + r] = wr;
+      if (!finalize_only) ut += vp * wi[jj] + wr * vi[jj];
+    }
+    if (!finalize_only) {
+      a = A[i * ln + r] - ut;
+      A[i * ln + r] = a;
+      if (r == i + 1) axv[i] = a;
+    }
+  }
+  if (finalize_only) return;
+  const double s = wave_sum((r < n && r >= i + 2) ? a * a : 0.0);
+  if (lane == 0) sq_cur[blockIdx.x] = s;
 }
 
-// p = tau * sum of the chunks, then w = p - (tau/2)(p.v) v (one workgroup).
-__global__ void k_hh_w(const double* A, int n, int i, const double* tau, const double* part, int nchunk, double* w) {
-  __shared__ double red[16];
-  const int m = n - i - 1;
-  const double t = tau[i];
-  const double* v = A + (long long)i * n + i + 1;
-  double dot = 0.0;
-  for (int r = threadIdx.x; r < m; r += blockDim.x) {
+// Partials of p = A22 v (A22 = A[i+1:, i+1:], v the reflector of column i, scaled on the fly)
+// over row blocks of 64 x column chunks of 128 (32 columns per wavefront), with v^T p per
+// workgroup. Workgroups with blockIdx.y >= nch (j > 0) form W^T v and V^T v partials, one
+// column per wavefront over 1024 rows.
+__global__ __launch_bounds__(256) void k_trd_symv(const double* A, int n, int i0, int j, const double* W,
+                                                  const double* sq, const double* axv, double* part, double* dpart,
+                                                  double* vpart) {
+  __shared__ double vs[TRD_CH];
+  __shared__ double red[4][TRD_SR];
+  const int i = i0 + j, m = n - i - 1;
+  const long long ln = n;
+  const double* x = A + i * ln + i + 1;
+  const int nrb = (m + TRD_SR - 1) / TRD_SR, nch = (m + TRD_CH - 1) / TRD_CH, ndb = (m + TRD_DR - 1) / TRD_DR;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  double t, beta, scal;
+  trd_reflector(axv[i], trd_sumsq(sq, n, i, lane), t, beta, scal);
+  auto vec = [&](int k) { return k == 0 ? 1.0 : x[k] * scal; };
+  if ((int)blockIdx.y < nch) {
+    const int rb = blockIdx.x;
+    const int c0 = blockIdx.y * TRD_CH;
+    const int c1 = min(TRD_CH, m - c0);
+    if (threadIdx.x < TRD_CH) vs[threadIdx.x] = (int)threadIdx.x < c1 ? vec(c0 + threadIdx.x) : 0.0;
+    __syncthreads();
+    const int rl = rb * TRD_SR + lane;
     double s = 0.0;
-    for (int c = 0; c < nchunk; c++) s += part[(long long)c * m + r];
-    s *= t;
-    w[r] = s;
-    dot += s * v[r];
+    if (rl < m) {
+      const double* a = A + (i + 1 + c0) * ln + (i + 1) + rl;
+      const int q0 = wv * (TRD_CH / 4), q1 = min(q0 + TRD_CH / 4, c1);
+#pragma unroll 8
+      for (int c = q0; c < q1; c++) s += a[c * ln] * vs[c];
+    }
+    red[wv][lane] = s;
+    __syncthreads();
+    if (wv == 0) {
+      const double tt = red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane];
+      if (rl < m) part[blockIdx.y * ln + (i + 1 + rl)] = tt;
+      const double pv = wave_sum(rl < m ? tt * vec(rl) : 0.0);
+      if (lane == 0) vpart[blockIdx.y * nrb + rb] = pv;
+    }
+  } else {
+    const int rb = blockIdx.x;
+    const int c = 4 * (blockIdx.y - nch) + wv;
+    if (rb < ndb && c < 2 * j) {
+      const double* y = c < j ? W + c * ln : A + (i0 + c - j) * ln;
+      double s = 0.0;
+#pragma unroll 4
+      for (int q = lane; q < TRD_DR; q += 64) {
+        const int rl = rb * TRD_DR + q;
+        if (rl < m) s += y[i + 1 + rl] * vec(rl);
+      }
+      s = wave_sum(s);
+      if (lane == 0) dpart[rb * 2 * TRD_NB + c] = s;
+    }
   }
-  dot = block_sum(dot, red);
-  const double f = 0.5 * t * dot;
-  for (int r = threadIdx.x; r < m; r += blockDim.x) w[r] -= f * v[r];
 }
 
-// A22 -= v w^T + w v^T
-__global__ void k_syr2(double* A, int n, int i, const double* w) {
-  const int m = n - i - 1;
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  const int c = blockIdx.y;
-  if (r >= m || c >= m) return;
-  const double* v = A + (long long)i * n + i + 1;
-  double* a = A + (long long)(i + 1 + c) * n + (i + 1);
-  a[r] -= v[r] * w[c] + w[r] * v[c];
-}
-
+// The last diagonal element (no reflector for the last column) and zero padding of e, tau.
 __global__ void k_hh_last(const double* A, int n, double* d, double* e, double* tau) {
-  // the last diagonal element; no reflector for the last column
   d[n - 1] = A[(long long)(n - 1) * n + (n - 1)];
-  if (n >= 2) {
-    // column n-2 has a 1-element x: its reflector was formed by k_hh_reflector
-  }
   tau[n - 1] = 0.0;
   e[n - 1] = 0.0;
+}
+
+__global__ void k_zero_u32(unsigned* p, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = 0u;
 }
 
 // ------------------------------------------------------------------ divide and conquer
@@ -264,22 +387,29 @@ __global__ void k_dc_compact(const DcNode* nodes, const int* kind, const double*
 
 // Secular equation 1 + rho sum_i z_i^2 / (d_i - lambda) = 0 of each kept root j: lambda_j in
 // (d_j, d_{j+1}) (the last in (d_K-1, d_K-1 + rho)), found by bisection in tau = lambda - d_org
-// relative to the nearer pole (org), which keeps lambda - d_i accurate.
-__global__ void k_dc_secular(const DcNode* nodes, const int* nk, const double* rho_, const double* kd,
-                             const double* kz, double* tau, int* org) {
+// relative to the nearer pole (org), which keeps lambda - d_i accurate. One wavefront per root
+// (4 per workgroup), the lanes sharing the sum; the butterfly sum is identical in every lane,
+// so the bisection stays uniform across the wavefront.
+__global__ __launch_bounds__(256) void k_dc_secular(const DcNode* nodes, const int* nk, const double* rho_,
+                                                    const double* kd, const double* kz, double* tau, int* org) {
   const DcNode N = nodes[blockIdx.y];
   const int K = nk[blockIdx.y];
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (j >= K) return;
   const double rho = rho_[blockIdx.y];
   const double* d = kd + N.s;
   const double* z = kz + N.s;
+  auto secular = [&](double dorg, double t) {
+    double g = 0.0;
+    for (int i = lane; i < K; i += 64) g += z[i] * z[i] / ((d[i] - dorg) - t);
+    return 1.0 / rho + wave_sum(g);
+  };
   const double lo_d = d[j];
   const double hi_d = j + 1 < K ? d[j + 1] : d[j] + rho;
-  // f at the midpoint decides the origin
+  // the sign at the midpoint decides the origin
   const double mid = 0.5 * (hi_d - lo_d);
-  double f = 1.0 / rho;
-  for (int i = 0; i < K; i++) f += z[i] * z[i] / ((d[i] - lo_d) - mid);
+  const double f = secular(lo_d, mid);
   int o;
   double lo, hi;
   if (f >= 0.0 || j + 1 >= K) {
@@ -293,15 +423,17 @@ __global__ void k_dc_secular(const DcNode* nodes, const int* nk, const double* r
   }
   const double dorg = d[o];
   for (int it = 0; it < 200; it++) {
-    double t = 0.5 * (lo + hi);
+    const double t = 0.5 * (lo + hi);
     if (t == lo || t == hi) break;
-    double g = 1.0 / rho;
-    for (int i = 0; i < K; i++) g += z[i] * z[i] / ((d[i] - dorg) - t);
-    if (g > 0.0) hi = t;
-    else lo = t;
+    if (secular(dorg, t) > 0.0)
+      hi = t;
+    else
+      lo = t;
   }
-  tau[N.s + j] = 0.5 * (lo + hi);
-  org[N.s + j] = o;
+  if (lane == 0) {
+    tau[N.s + j] = 0.5 * (lo + hi);
+    org[N.s + j] = o;
+  }
 }
 
 // Loewner z-hat (Gu-Eisenstat): zh_i^2 = (lambda_i - d_i) prod_{j != i} (lambda_j - d_i)/(d_j - d_i) / rho.
@@ -418,8 +550,8 @@ __global__ void k_dc_copyback(const DcNode* nodes, const double* Q2, const doubl
 
 // ------------------------------------------------------------------ back-transformation
 // Explicit panel of reflectors i0..i0+nb-1 (rows i0+1..n-1 of the result): V[r][j] with unit
-// diagonal, zero above (reflector j acts on rows i0+j+1..).
-__global__ void k_bt_panel(const double* A, int n, int i0, int nb, double* V) {
+// first element, zero above; a reflector with tau = 0 (the identity) gets a zero column.
+__global__ void k_bt_panel(const double* A, const double* tau, int n, int i0, int nb, double* V) {
   const int rows = n - i0 - 1;
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (long long)rows * nb) return;
@@ -428,29 +560,39 @@ __global__ void k_bt_panel(const double* A, int n, int i0, int nb, double* V) {
   // global row of r: i0 + 1 + r; the reflector of column col covers rows col+1..n-1
   const int g = i0 + 1 + r;
   double v = 0.0;
-  if (g >= col + 1) v = A[(long long)col * n + g];
+  if (g >= col + 1 && tau[col] != 0.0) v = A[(long long)col * n + g];
   V[idx] = v;
 }
 
-// T factor (upper triangular, nb x nb) of the panel from VtV = V^T V and tau (LAPACK dlarft,
-// forward columnwise): T[j][j] = tau_j, T[0:j, j] = -tau_j T[0:j, 0:j] VtV[0:j, j]. One workgroup.
-__global__ void k_bt_tfactor(const double* VtV, const double* tau, int i0, int nb, double* T) {
-  __shared__ double tmp[128];
-  for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) T[e] = 0.0;
-  __syncthreads();
-  for (int j = 0; j < nb; j++) {
-    const double tj = tau[i0 + j];
-    if (threadIdx.x < j) {
-      int r = threadIdx.x;
-      double s = 0.0;
-      for (int k = r; k < j; k++) s += T[r + k * nb] * VtV[k + j * nb];
-      tmp[r] = -tj * s;
+// The panel's product H_i0 ... H_i0+nb-1 = I - V T V^T has T^-1 = striu(V^T V) + diag(1/tau)
+// (the UT transform; an identity reflector has a zero column and the diagonal entry 1). T, upper
+// triangular, by inverting T^-1 column by column (one thread per column of T) from LDS; VtV is
+// overwritten by T.
+__global__ __launch_bounds__(64) void k_bt_tfactor(double* VtV, const double* tau, int i0, int nb) {
+  __shared__ double L[64][65];  // T^-1, then T
+  __shared__ double X[64][65];
+  const int c = threadIdx.x;
+  for (int r = 0; r < nb; r++) {
+    double v = 0.0;
+    if (c < nb) {
+      const double t = tau[i0 + r];
+      v = c > r ? VtV[r + c * nb] : (c == r ? (t != 0.0 ? 1.0 / t : 1.0) : 0.0);
     }
-    __syncthreads();
-    if (threadIdx.x < j) T[threadIdx.x + j * nb] = tmp[threadIdx.x];
-    if (threadIdx.x == 0) T[j + j * nb] = tj;
-    __syncthreads();
+    L[r][c] = v;
   }
+  __syncthreads();
+  // column c of T: solve T^-1 x = e_c (back substitution, rows c..0)
+  if (c < nb) {
+    for (int r = nb - 1; r > c; r--) X[r][c] = 0.0;
+    for (int r = c; r >= 0; r--) {
+      double acc = r == c ? 1.0 : 0.0;
+      for (int q = r + 1; q <= c; q++) acc -= L[r][q] * X[q][c];
+      X[r][c] = tau[i0 + r] != 0.0 ? acc * tau[i0 + r] : acc;
+    }
+  }
+  __syncthreads();
+  if (c < nb)
+    for (int r = 0; r < nb; r++) VtV[r + c * nb] = X[r][c];
 }
 
 }  // namespace axtb

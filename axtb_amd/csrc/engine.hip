@@ -229,14 +229,33 @@ struct Engine {
                jobs.end());
     if (jobs.empty()) return;
     int mm = 0, nn = 0;
+    bool same = true;
     for (auto& j : jobs) {
       mm = std::max(mm, j.m);
       nn = std::max(nn, j.n);
+      same &= j.ta == jobs[0].ta && j.tb == jobs[0].tb;
+    }
+    double flops = 0.0;
+    for (auto& j : jobs) flops += 2.0 * j.m * j.n * j.k;
+    // large products on 128 x 128 tiles (one transpose pair per launch), the batch of small
+    // per-molecule products on 64 x 64 tiles
+    if (same && (long long)mm * nn >= 256LL * 256 && jobs.size() <= 8) {
+      gj.upload(jobs, st);
+      dim3 grid(blocks(nn, BT), blocks(mm, BT), (unsigned)jobs.size());
+      const int t = jobs[0].ta * 2 + jobs[0].tb;
+      if (t == 0)
+        LAUNCHW(flops, k_gemm128_nn, grid, dim3(256), 0, gj.p);
+      else if (t == 1)
+        LAUNCHW(flops, k_gemm128_nt, grid, dim3(256), 0, gj.p);
+      else if (t == 2)
+        LAUNCHW(flops, k_gemm128_tn, grid, dim3(256), 0, gj.p);
+      else
+        LAUNCHW(flops, k_gemm128_tt, grid, dim3(256), 0, gj.p);
+      HIP_OK(hipGetLastError());
+      return;
     }
     gj.upload(jobs, st);
     dim3 grid(blocks(nn, GT), blocks(mm, GT), (unsigned)jobs.size());
-    double flops = 0.0;
-    for (auto& j : jobs) flops += 2.0 * j.m * j.n * j.k;
     LAUNCHW(flops, k_gemm, grid, dim3(256), 0, gj.p);
     HIP_OK(hipGetLastError());
   }
@@ -292,8 +311,9 @@ struct Engine {
   // Eigenvalues of the symmetric n x n matrix A into w (ascending). A is overwritten with the
   // Householder vectors, tau (n) receives their scalars and Q (n x n) the eigenvectors of the
   // tridiagonal matrix: the state dense_vectors() transforms back.
-  DBuf<double> dd, de, dpart, dwv, dq_Q2, dD, dD2, dds, dzs, drot_cs, dkd, dkz, dtaus, dzh, dval, dU, dG, dQG, dleaf,
-      dleafV, dleafZ, dleafw, dbtV, dbtY, dbtY2, dbtT, dbtVtV;
+  DBuf<double> dtrW, dtrDots, dtrVp, dtrSq, dtrAx;
+  DBuf<double> dd, de, dpart, dq_Q2, dD, dD2, dds, dzs, drot_cs, dkd, dkz, dtaus, dzh, dval, dU, dG, dQG, dleaf,
+      dleafV, dleafZ, dleafw, dbtV, dbtY, dbtVtV, dbtVT;
   DBuf<int> dperm, dkind, drot_ij, dkcol, dorg, dsrc, dnk, dnr, dls, dlm;
   DBuf<double> drho;
   DBuf<long long> duoff, dgoff;
@@ -304,21 +324,39 @@ struct Engine {
     drot_cs.ensure(2 * (size_t)n);
     for (auto* b : {&dperm, &dkind, &dkcol, &dorg, &dsrc}) b->ensure(n);
     drot_ij.ensure(2 * (size_t)n);
-    const int nchunk = (n + 63) / 64;
-    dpart.ensure((size_t)nchunk * n);
-    dwv.ensure(n);
-    // 1. Householder tridiagonalisation
-    for (int i = 0; i + 1 < n; i++) {
-      const int m = n - i - 1;
-      LAUNCH(k_hh_reflector, dim3(1), dim3(256), 0, A, n, i, tau, dd.p, de.p);
-      if (m > 1) {
-        const int nc = (m + 63) / 64;
-        LAUNCHW(8.0 * m * (double)m, k_symv_part, dim3(blocks(m, 256), nc), dim3(256), 0, (const double*)A, n, i,
-                dpart.p);
-        LAUNCH(k_hh_w, dim3(1), dim3(1024), 0, (const double*)A, n, i, (const double*)tau, (const double*)dpart.p,
-               nc, dwv.p);
-        LAUNCHW(16.0 * m * (double)m, k_syr2, dim3(blocks(m, 256), m), dim3(256), 0, A, n, i, (const double*)dwv.p);
+    // 1. blocked Householder tridiagonalisation
+    const int nrbmax = (n + TRD_SR - 1) / TRD_SR, nchmax = (n + TRD_CH - 1) / TRD_CH;
+    const int ndbmax = (n + TRD_DR - 1) / TRD_DR, nsq = (n + TRD_CB - 1) / TRD_CB;
+    dpart.ensure((size_t)nchmax * n);
+    dtrW.ensure((size_t)n * TRD_NB);
+    dtrDots.ensure((size_t)ndbmax * 2 * TRD_NB);
+    dtrVp.ensure((size_t)nrbmax * nchmax);
+    dtrSq.ensure(2 * (size_t)nsq);
+    dtrAx.ensure(n);
+    for (int i0 = 0; i0 + 1 < n; i0 += TRD_NB) {
+      const int b = std::min(TRD_NB, n - 1 - i0);
+      for (int j = 0; j <= b; j++) {
+        const int i = i0 + j;
+        double* sq_cur = dtrSq.p + (size_t)(i & 1) * nsq;
+        const double* sq_prev = dtrSq.p + (size_t)((i + 1) & 1) * nsq;
+        LAUNCH(k_trd_col, dim3(blocks(n - i, TRD_CB)), dim3(256), 0, A, n, i0, j, dtrW.p, (const double*)dpart.p,
+               (const double*)dtrDots.p, (const double*)dtrVp.p, sq_prev, sq_cur, dtrAx.p, tau, dd.p, de.p,
+               j == b ? 1 : 0);
+        if (j == b) break;
+        const int m = n - i - 1;
+        const double bytes = 8.0 * m * (double)m;
+        LAUNCHW(bytes, k_trd_symv, dim3(blocks(m, TRD_SR), blocks(m, TRD_CH) + blocks(2 * j, 4)), dim3(256), 0,
+                (const double*)A, n, i0, j, (const double*)dtrW.p, (const double*)sq_cur, (const double*)dtrAx.p,
+                dpart.p, dtrDots.p, dtrVp.p);
       }
+      const int r0 = i0 + b, mt = n - r0;
+      double* A22 = A + (long long)r0 * n + r0;
+      const double* V = A + (long long)i0 * n + r0;
+      const double* Wt = dtrW.p + r0;
+      std::vector<GemmJob> g1{GemmJob{mt, mt, b, 0, 1, V, Wt, A22, nullptr, n, n, n, -1.0, 1.0}};
+      gemm(g1);
+      std::vector<GemmJob> g2{GemmJob{mt, mt, b, 0, 1, Wt, V, A22, nullptr, n, n, n, -1.0, 1.0}};
+      gemm(g2);
     }
     LAUNCH(k_hh_last, dim3(1), dim3(1), 0, (const double*)A, n, dd.p, de.p, tau);
     HIP_OK(hipGetLastError());
@@ -410,7 +448,7 @@ struct Engine {
       LAUNCH(k_dc_compact, dim3(nn), dim3(64), 0, (const DcNode*)dnodes.p, (const int*)dkind.p, (const double*)dds.p,
              (const double*)dzs.p, (const int*)dperm.p, dkd.p, dkz.p, dkcol.p);
       if (kmax > 0) {
-        LAUNCH(k_dc_secular, dim3(blocks(kmax, 64), nn), dim3(64), 0, (const DcNode*)dnodes.p, (const int*)dnk.p,
+        LAUNCH(k_dc_secular, dim3(blocks(kmax, 4), nn), dim3(256), 0, (const DcNode*)dnodes.p, (const int*)dnk.p,
                (const double*)drho.p, (const double*)dkd.p, (const double*)dkz.p, dtaus.p, dorg.p);
         LAUNCH(k_dc_zhat, dim3(blocks(kmax, 64), nn), dim3(64), 0, (const DcNode*)dnodes.p, (const int*)dnk.p,
                (const double*)drho.p, (const double*)dkd.p, (const double*)dkz.p, (const double*)dtaus.p,
@@ -451,25 +489,24 @@ struct Engine {
     const int nref = n - 1;  // reflectors 0..n-2
     dbtV.ensure((size_t)n * nb);
     dbtY.ensure((size_t)nb * k);
-    dbtY2.ensure((size_t)nb * k);
-    dbtT.ensure((size_t)nb * nb);
     dbtVtV.ensure((size_t)nb * nb);
+    dbtVT.ensure((size_t)n * nb);
     int np = (nref + nb - 1) / nb;
     for (int p = np - 1; p >= 0; p--) {
       const int i0 = p * nb;
       const int b = std::min(nb, nref - i0);
       const int rows = n - i0 - 1;
-      LAUNCH(k_bt_panel, dim3(blocks((long long)rows * b, 256)), dim3(256), 0, (const double*)A, n, i0, b, dbtV.p);
-      std::vector<GemmJob> j1{GemmJob{b, b, rows, 1, 0, dbtV.p, dbtV.p, dbtVtV.p, nullptr, rows, rows, b, 1.0, 0.0}};
-      gemm(j1);
-      LAUNCH(k_bt_tfactor, dim3(1), dim3(128), 0, (const double*)dbtVtV.p, tau, i0, b, dbtT.p);
+      LAUNCH(k_bt_panel, dim3(blocks((long long)rows * b, 256)), dim3(256), 0, A, tau, n, i0, b, dbtV.p);
       double* Zr = Z + i0 + 1;
-      std::vector<GemmJob> j2{GemmJob{b, k, rows, 1, 0, dbtV.p, Zr, dbtY.p, nullptr, rows, n, b, 1.0, 0.0}};
+      // V^T V and Yt = Zr^T V; T from V^T V; VT = V T; Zr -= VT Yt^T
+      std::vector<GemmJob> j1{GemmJob{b, b, rows, 1, 0, dbtV.p, dbtV.p, dbtVtV.p, nullptr, rows, rows, b, 1.0, 0.0},
+                              GemmJob{k, b, rows, 1, 0, Zr, dbtV.p, dbtY.p, nullptr, n, rows, k, 1.0, 0.0}};
+      gemm(j1);
+      LAUNCH(k_bt_tfactor, dim3(1), dim3(64), 0, dbtVtV.p, tau, i0, b);
+      std::vector<GemmJob> j2{GemmJob{rows, b, b, 0, 0, dbtV.p, dbtVtV.p, dbtVT.p, nullptr, rows, b, rows, 1.0, 0.0}};
       gemm(j2);
-      std::vector<GemmJob> j3{GemmJob{b, k, b, 0, 0, dbtT.p, dbtY.p, dbtY2.p, nullptr, b, b, b, 1.0, 0.0}};
+      std::vector<GemmJob> j3{GemmJob{rows, k, b, 0, 1, dbtVT.p, dbtY.p, Zr, nullptr, rows, k, n, -1.0, 1.0}};
       gemm(j3);
-      std::vector<GemmJob> j4{GemmJob{rows, k, b, 0, 0, dbtV.p, dbtY2.p, Zr, nullptr, rows, b, n, -1.0, 1.0}};
-      gemm(j4);
     }
     HIP_OK(hipGetLastError());
   }

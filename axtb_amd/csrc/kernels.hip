@@ -994,7 +994,8 @@ __global__ void k_broyden(Batch b, const int* active, int cap, double damp, doub
   const int it1 = (itn - 1) % cap;
   const int nhist = min(cap, itn);
   const int first = itn + 1 > cap ? itn + 1 - cap : 1;
-  if (threadIdx.x < nhist) hist[threadIdx.x] = (first + threadIdx.x - 1) % cap;
+  const int th = threadIdx.x;
+  if (th < nhist) hist[th] = (first + th - 1) % cap;
   double* dfi = df + M.hist0 + (long long)it1 * n;
   double* ui = u + M.hist0 + (long long)it1 * n;
   double nn = 0.0;

@@ -95,6 +95,125 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmJob* jobs) {
       }
 }
 
+// Large-tile variant: 128 x 128 output tile per workgroup of 4 wavefronts (each 64 x 64 as 4 x 4
+// MFMA blocks), K in steps of 16 through double-buffered LDS with the next step's global loads
+// held in registers while the current one is multiplied; one barrier per step. TA / TB select
+// op(A) = A^T / op(B) = B^T at compile time. Every job of a launch has the same TA, TB.
+constexpr int BT = 128;
+constexpr int BK = 16;
+constexpr int BP = BT + 4;  // padded LDS row
+
+template <int TA, int TB>
+__global__ __launch_bounds__(256) void k_gemm128(const GemmJob* jobs) {
+  const GemmJob J = jobs[blockIdx.z];
+  const int row0 = blockIdx.y * BT, col0 = blockIdx.x * BT;
+  if (row0 >= J.m || col0 >= J.n) return;
+  __shared__ double As[2][BK][BP];
+  __shared__ double Bs[2][BK][BP];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wr = (w & 1) * 64, wc = (w >> 1) * 64;
+  double ra[8], rb[8];
+  // element q of this thread's share of a k-step: (tile row / col, k)
+  auto a_rk = [&](int q, int& r, int& k) {
+    if (TA == 0) {
+      r = tid & 127;
+      k = (tid >> 7) * 8 + q;
+    } else {
+      k = tid & 15;
+      r = (tid >> 4) + 16 * q;
+    }
+  };
+  auto b_ck = [&](int q, int& c, int& k) {
+    if (TB == 1) {
+      c = tid & 127;
+      k = (tid >> 7) * 8 + q;
+    } else {
+      k = tid & 15;
+      c = (tid >> 4) + 16 * q;
+    }
+  };
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      int r, k;
+      a_rk(q, r, k);
+      const int gr = row0 + r, gk = k0 + k;
+      double v = 0.0;
+      if (gr < J.m && gk < J.k) {
+        v = TA == 0 ? J.A[gr + (long long)gk * J.lda] : J.A[gk + (long long)gr * J.lda];
+        if (J.dk) v *= J.dk[gk];
+      }
+      ra[q] = v;
+      int c;
+      b_ck(q, c, k);
+      const int gc = col0 + c, gkb = k0 + k;
+      double u = 0.0;
+      if (gc < J.n && gkb < J.k) u = TB == 0 ? J.B[gkb + (long long)gc * J.ldb] : J.B[gc + (long long)gkb * J.ldb];
+      rb[q] = u;
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      int r, k;
+      a_rk(q, r, k);
+      As[buf][k][r] = ra[q];
+      int c;
+      b_ck(q, c, k);
+      Bs[buf][k][c] = rb[q];
+    }
+  };
+  d4v acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[i][j] = (d4v){0.0, 0.0, 0.0, 0.0};
+  const int nk = (J.k + BK - 1) / BK;
+  load(0);
+  store(0);
+  __syncthreads();
+  for (int kt = 0; kt < nk; kt++) {
+    const int buf = kt & 1;
+    if (kt + 1 < nk) load((kt + 1) * BK);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      const int kl = kk + (lane >> 4), li = lane & 15;
+      double a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        a[i] = As[buf][kl][wr + 16 * i + li];
+        b[i] = Bs[buf][kl][wc + 16 * i + li];
+      }
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+    if (kt + 1 < nk) store(buf ^ 1);
+    __syncthreads();
+  }
+  // lane l, element r of a 16 x 16 block holds row l / 16 + 4 r, column l % 16
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int gr = row0 + wr + 16 * i + (lane >> 4) + 4 * r;
+        const int gc = col0 + wc + 16 * j + (lane & 15);
+        if (gr < J.m && gc < J.n) {
+          double* c = &J.C[gr + (long long)gc * J.ldc];
+          const double v = J.alpha * acc[i][j][r];
+          *c = J.beta == 0.0 ? v : v + J.beta * *c;
+        }
+      }
+}
+
+static constexpr auto k_gemm128_nn = k_gemm128<0, 0>;
+static constexpr auto k_gemm128_nt = k_gemm128<0, 1>;
+static constexpr auto k_gemm128_tn = k_gemm128<1, 0>;
+static constexpr auto k_gemm128_tt = k_gemm128<1, 1>;
+
 // ------------------------------------------------------------------ Jacobi eigensolver
 struct EigJob {
   int n;
