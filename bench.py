@@ -39,6 +39,17 @@ def workload(name, seed=0):
     return [S.config(name, seed=seed)]
 
 
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed counter summary
+    (profiles/traffic.json: FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the copy-kernel
+    calibration), or None when it was not measured."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    rec = json.load(open(path)).get(kernel, {}).get(workload)
+    return None if rec is None else rec["bytes_per_launch"]
+
+
 def cpu_baseline(mols, budget_s, acc):
     """The CPU oracle (C++ restatement of axtb's FP64 path, one thread) on the first molecules of
     the workload until `budget_s` seconds of CPU work are spent."""
@@ -64,7 +75,19 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check", type=int, default=8, help="molecules checked against the oracle after timing")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device to device (a kernel of known traffic for the HBM counters)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        nbytes = args.calibrate_copy << 20
+        src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes_read": nbytes, "bytes_written": nbytes}), flush=True)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the batch shards by molecule with no exchange between GPUs; "
               "multi-GPU runs are not part of this build (run with --gpus 1)", file=sys.stderr)
@@ -116,6 +139,7 @@ def main():
     else:
         bound, peak, unit, achieved = "mfma", PEAK_FP64_MFMA / 1e12, "TFLOP/s", per_launch / avg_s / 1e12
 
+    traffic = measured_traffic(name, args.workload)
     value = len(mols) * args.steps / dt
     line = {
         "metric": "E+grad/s (GFN2-xTB energy + analytic gradient)",
@@ -135,7 +159,7 @@ def main():
                    "failed": len(bad), "parity_checked": min(args.check, len(mols)),
                    "max_abs_dE_vs_oracle": dmax_e, "max_abs_dg_vs_oracle": dmax_g},
         "roofline": {"bound": bound, "kernel": name, "achieved": achieved, "peak": peak, "unit": unit,
-                     "frac": achieved / peak, "traffic": None, "avg_launch_ms": avg_s * 1e3, "launches": nl,
+                     "frac": achieved / peak, "traffic": traffic, "avg_launch_ms": avg_s * 1e3, "launches": nl,
                      "kernel_share_of_gpu_time": ms / total_gpu_ms if total_gpu_ms else None},
         "kernels_ms": {k: round(v[1], 3) for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])[:12]},
     }
