@@ -114,9 +114,17 @@ struct Prof {
     long n = 0;
     double ms = 0, work = 0;
   };
+  // host-side counters reported like kernels (launches = items, work = total), 0 ms
+  std::map<std::string, Sum> counters;
+  void count(const std::string& name, long items, double total) {
+    auto& x = counters[name];
+    x.n += items;
+    x.work += total;
+  }
   // name -> launches, total ms, total work; the recorded events are returned to the pool
   std::map<std::string, Sum> drain() {
-    std::map<std::string, Sum> m;
+    std::map<std::string, Sum> m = counters;
+    counters.clear();
     for (auto& r : recs) {
       HIP_OK(hipEventSynchronize(r.b));
       float ms = 0.f;
@@ -155,6 +163,11 @@ struct ProfScope {
 };
 
 #define LAUNCH(kern, grid, block, shm, ...) LAUNCHW(0.0, kern, grid, block, shm, __VA_ARGS__)
+#define LAUNCHS(stream, work, kern, grid, block, shm, ...)                  \
+  do {                                                                      \
+    ProfScope ps_(&prof, stream, #kern, work);                              \
+    hipLaunchKernelGGL(kern, grid, block, shm, stream, __VA_ARGS__);        \
+  } while (0)
 #define LAUNCHW(work, kern, grid, block, shm, ...)                          \
   do {                                                                      \
     ProfScope ps_(&prof, st, #kern, work);                                  \
@@ -181,6 +194,8 @@ struct MolOut {
 struct Engine {
   int device = 0;
   hipStream_t st = nullptr;
+  hipStream_t st2 = nullptr;  // side stream (default priority) for overlapped eigensolver launches
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   Prof prof;
   std::string last_error;
   // host mirrors of the batch tables
@@ -208,7 +223,11 @@ struct Engine {
   explicit Engine(int dev) : device(dev) {
     HIP_OK(hipSetDevice(dev));
     HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    if (const char* v = std::getenv("AXTB_HIP_DENSE_MIN")) dense_min = std::max(1, std::atoi(v));
+    HIP_OK(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));
+    HIP_OK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    HIP_OK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+    // the batched Jacobi kernel handles n <= 200
+    if (const char* v = std::getenv("AXTB_HIP_DENSE_MIN")) dense_min = std::min(200, std::max(1, std::atoi(v)));
   }
   bool dense(int n) const { return n > dense_min; }
 
@@ -221,7 +240,10 @@ struct Engine {
     dense_vectors(A, n, dtau1.p, Q, n, Z);
   }
   ~Engine() {
+    if (st2) (void)hipStreamDestroy(st2);
     if (st) (void)hipStreamDestroy(st);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
   }
 
   void gemm(std::vector<GemmJob>& jobs) {
@@ -229,28 +251,59 @@ struct Engine {
                jobs.end());
     if (jobs.empty()) return;
     int mm = 0, nn = 0;
-    bool same = true;
+    bool same = true, two = false;
     for (auto& j : jobs) {
       mm = std::max(mm, j.m);
       nn = std::max(nn, j.n);
       same &= j.ta == jobs[0].ta && j.tb == jobs[0].tb;
+      two |= j.k2 > 0;
     }
     double flops = 0.0;
-    for (auto& j : jobs) flops += 2.0 * j.m * j.n * j.k;
-    // large products on 128 x 128 tiles (one transpose pair per launch), the batch of small
-    // per-molecule products on 64 x 64 tiles
-    if (same && (long long)mm * nn >= 256LL * 256 && jobs.size() <= 8) {
+    for (auto& j : jobs) flops += 2.0 * j.m * j.n * ((double)j.k + j.k2);
+    // one product with few output tiles and a long inner dimension: split K over workgroups,
+    // partials reduced in a fixed order
+    if (jobs.size() == 1 && !two) {
+      const GemmJob& J = jobs[0];
+      const long long tiles = (long long)blocks(J.m, BT) * blocks(J.n, BT);
+      const int splits = (int)std::min<long long>(std::min<long long>(256 / std::max(1LL, tiles), J.k / 256), 64);
+      if (tiles < 128 && splits >= 2) {
+        const int kc = (J.k + splits - 1) / splits;
+        std::vector<GemmJob> parts;
+        gpart.ensure((size_t)splits * J.m * J.n);
+        for (int q = 0; q < splits; q++) {
+          const int k0 = q * kc, k1 = std::min(J.k, k0 + kc);
+          if (k1 <= k0) break;
+          GemmJob P = J;
+          P.k = k1 - k0;
+          P.A = J.ta == 0 ? J.A + (long long)k0 * J.lda : J.A + k0;
+          P.B = J.tb == 0 ? J.B + k0 : J.B + (long long)k0 * J.ldb;
+          P.dk = J.dk ? J.dk + k0 : nullptr;
+          P.C = gpart.p + (size_t)parts.size() * J.m * J.n;
+          P.ldc = J.m;
+          P.beta = 0.0;
+          parts.push_back(P);
+        }
+        gj.upload(parts, st);
+        dim3 grid(blocks(J.n, BT), blocks(J.m, BT), (unsigned)parts.size());
+        launch128(J.ta * 2 + J.tb, grid, flops);
+        LAUNCH(k_gemm_reduce, dim3(blocks((long long)J.m * J.n, 256)), dim3(256), 0, (const double*)gpart.p,
+               (int)parts.size(), J.m, J.n, J.C, J.ldc, J.beta);
+        HIP_OK(hipGetLastError());
+        return;
+      }
+    }
+    // large products on 128 x 128 tiles (one transpose pair per launch; two-term products always),
+    // the batch of small per-molecule products on 64 x 64 tiles
+    if (two || (same && (long long)mm * nn >= 256LL * 256 && jobs.size() <= 8)) {
+      if (!same) throw DeviceError("gemm: mixed transposes in one launch");
       gj.upload(jobs, st);
       dim3 grid(blocks(nn, BT), blocks(mm, BT), (unsigned)jobs.size());
-      const int t = jobs[0].ta * 2 + jobs[0].tb;
-      if (t == 0)
-        LAUNCHW(flops, k_gemm128_nn, grid, dim3(256), 0, gj.p);
-      else if (t == 1)
-        LAUNCHW(flops, k_gemm128_nt, grid, dim3(256), 0, gj.p);
-      else if (t == 2)
-        LAUNCHW(flops, k_gemm128_tn, grid, dim3(256), 0, gj.p);
-      else
-        LAUNCHW(flops, k_gemm128_tt, grid, dim3(256), 0, gj.p);
+      if (two) {
+        if (jobs[0].ta != 0 || jobs[0].tb != 1) throw DeviceError("gemm: two-term products are A B^T + A2 B2^T");
+        LAUNCHW(flops, k_gemm128_nt2, grid, dim3(256), 0, gj.p);
+      } else {
+        launch128(jobs[0].ta * 2 + jobs[0].tb, grid, flops);
+      }
       HIP_OK(hipGetLastError());
       return;
     }
@@ -259,12 +312,26 @@ struct Engine {
     LAUNCHW(flops, k_gemm, grid, dim3(256), 0, gj.p);
     HIP_OK(hipGetLastError());
   }
+  DBuf<double> gpart;
+  void launch128(int t, dim3 grid, double flops) {
+    if (t == 0)
+      LAUNCHW(flops, k_gemm128_nn, grid, dim3(256), 0, gj.p);
+    else if (t == 1)
+      LAUNCHW(flops, k_gemm128_nt, grid, dim3(256), 0, gj.p);
+    else if (t == 2)
+      LAUNCHW(flops, k_gemm128_tn, grid, dim3(256), 0, gj.p);
+    else
+      LAUNCHW(flops, k_gemm128_tt, grid, dim3(256), 0, gj.p);
+  }
 
   // eigenpairs of every job's symmetric matrix (Jacobi, one workgroup per matrix)
   // work: the algorithmic flops of a dense eigendecomposition, (4/3) n^3 for the reduction to
   // tridiagonal form plus 2 n^2 k for the k needed eigenvectors (k = n here). Matrices that fit
   // the LDS of a workgroup run from LDS, the others from global memory (separate launches).
   int max_lds = 0;
+  // Launches: matrices that fit in LDS with their vectors (mode 2), with A only (mode 1), neither
+  // (mode 0). Within a launch the largest matrices go first (shorter tail); the mode 1/0 launches
+  // run on a second stream beside the mode 2 launch, both joined back into the engine stream.
   void eigh(std::vector<EigJob>& jobs, int max_sweeps = 60) {
     if (jobs.empty()) return;
     if (!max_lds) {
@@ -273,14 +340,22 @@ struct Engine {
     }
     std::vector<EigJob> groups[3];
     for (auto& j : jobs) {
+      if (j.n > 200) throw DeviceError("Jacobi eigensolver called for n > 200");
       int mode = jacobi_lds(j.n, 2) <= (size_t)max_lds ? 2 : (jacobi_lds(j.n, 1) <= (size_t)max_lds ? 1 : 0);
       groups[mode].push_back(j);
     }
+    for (auto& g : groups) std::stable_sort(g.begin(), g.end(), [](const EigJob& a, const EigJob& b) { return a.n > b.n; });
     jstat.ensure(jobs.size());
+    const bool side = !groups[2].empty() && (!groups[1].empty() || !groups[0].empty());
+    if (side) {
+      HIP_OK(hipEventRecord(ev_fork, st));
+      HIP_OK(hipStreamWaitEvent(st2, ev_fork, 0));
+    }
     size_t off = 0;
     for (int mode = 2; mode >= 0; mode--) {
       auto& js = groups[mode];
       if (js.empty()) continue;
+      hipStream_t s = (side && mode < 2) ? st2 : st;
       int nmax = 0;
       double flops = 0.0;
       for (auto& j : js) {
@@ -288,21 +363,39 @@ struct Engine {
         flops += 4.0 / 3.0 * j.n * (double)j.n * j.n + 2.0 * j.n * (double)j.n * j.n;
       }
       DBuf<EigJob>& buf = ejb[mode];
-      buf.upload(js, st);
+      buf.upload(js, s);
       size_t shm = jacobi_lds(nmax, mode);
       int* stat = jstat.p + off;
       off += js.size();
       if (mode == 2) {
         HIP_OK(hipFuncSetAttribute((const void*)k_jacobi<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-        LAUNCHW(flops, k_jacobi<2>, dim3(js.size()), dim3(512), shm, buf.p, max_sweeps, stat);
+        LAUNCHS(s, flops, k_jacobi<2>, dim3(js.size()), dim3(512), shm, buf.p, max_sweeps, stat);
       } else if (mode == 1) {
         HIP_OK(hipFuncSetAttribute((const void*)k_jacobi<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-        LAUNCHW(flops, k_jacobi<1>, dim3(js.size()), dim3(512), shm, buf.p, max_sweeps, stat);
+        LAUNCHS(s, flops, k_jacobi<1>, dim3(js.size()), dim3(512), shm, buf.p, max_sweeps, stat);
       } else {
         HIP_OK(hipFuncSetAttribute((const void*)k_jacobi<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-        LAUNCHW(flops, k_jacobi<0>, dim3(js.size()), dim3(512), shm, buf.p, max_sweeps, stat);
+        LAUNCHS(s, flops, k_jacobi<0>, dim3(js.size()), dim3(512), shm, buf.p, max_sweeps, stat);
       }
       HIP_OK(hipGetLastError());
+      if (prof.on) {
+        // sweep statistics of the launch (profiling only: synchronises)
+        std::vector<int> hs(js.size());
+        HIP_OK(hipMemcpyAsync(hs.data(), stat, sizeof(int) * js.size(), hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        double sum = 0.0;
+        long bad = 0;
+        for (int v : hs) {
+          sum += std::abs(v);
+          bad += v < 0;
+        }
+        prof.count("sweeps_jacobi<" + std::to_string(mode) + ">", (long)hs.size(), sum);
+        if (bad) prof.count("unconverged_jacobi<" + std::to_string(mode) + ">", bad, (double)bad);
+      }
+    }
+    if (side) {
+      HIP_OK(hipEventRecord(ev_join, st2));
+      HIP_OK(hipStreamWaitEvent(st, ev_join, 0));
     }
   }
   DBuf<EigJob> ejb[3];
@@ -353,10 +446,13 @@ struct Engine {
       double* A22 = A + (long long)r0 * n + r0;
       const double* V = A + (long long)i0 * n + r0;
       const double* Wt = dtrW.p + r0;
-      std::vector<GemmJob> g1{GemmJob{mt, mt, b, 0, 1, V, Wt, A22, nullptr, n, n, n, -1.0, 1.0}};
+      // A22 -= V W^T + W V^T in one pass over A22
+      GemmJob g{mt, mt, b, 0, 1, V, Wt, A22, nullptr, n, n, n, -1.0, 1.0};
+      g.A2 = Wt;
+      g.B2 = V;
+      g.k2 = b;
+      std::vector<GemmJob> g1{g};
       gemm(g1);
-      std::vector<GemmJob> g2{GemmJob{mt, mt, b, 0, 1, Wt, V, A22, nullptr, n, n, n, -1.0, 1.0}};
-      gemm(g2);
     }
     LAUNCH(k_hh_last, dim3(1), dim3(1), 0, (const double*)A, n, dd.p, de.p, tau);
     HIP_OK(hipGetLastError());

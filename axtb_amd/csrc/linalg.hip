@@ -14,6 +14,11 @@ struct GemmJob {
   const double* dk;  // optional scaling of the inner index
   long long lda, ldb, ldc;
   double alpha, beta;
+  // optional second term (128-tile kernel only): C = alpha (op(A) op(B) + op(A2) op(B2)) + beta C,
+  // A2, B2 with the leading dimensions and transposes of A, B and inner size k2 (no scaling)
+  const double* A2 = nullptr;
+  const double* B2 = nullptr;
+  int k2 = 0;
 };
 
 typedef double d4v __attribute__((ext_vector_type(4)));
@@ -103,7 +108,7 @@ constexpr int BT = 128;
 constexpr int BK = 16;
 constexpr int BP = BT + 4;  // padded LDS row
 
-template <int TA, int TB>
+template <int TA, int TB, bool TWO>
 __global__ __launch_bounds__(256) void k_gemm128(const GemmJob* jobs) {
   const GemmJob J = jobs[blockIdx.z];
   const int row0 = blockIdx.y * BT, col0 = blockIdx.x * BT;
@@ -132,23 +137,38 @@ __global__ __launch_bounds__(256) void k_gemm128(const GemmJob* jobs) {
       c = (tid >> 4) + 16 * q;
     }
   };
+  // inner index gk < k: first term; k <= gk < k + k2: second term at gk - k
   auto load = [&](int k0) {
 #pragma unroll
     for (int q = 0; q < 8; q++) {
       int r, k;
       a_rk(q, r, k);
-      const int gr = row0 + r, gk = k0 + k;
+      const int gr = row0 + r;
+      int gk = k0 + k;
       double v = 0.0;
-      if (gr < J.m && gk < J.k) {
-        v = TA == 0 ? J.A[gr + (long long)gk * J.lda] : J.A[gk + (long long)gr * J.lda];
-        if (J.dk) v *= J.dk[gk];
+      if (gr < J.m) {
+        if (gk < J.k) {
+          v = TA == 0 ? J.A[gr + (long long)gk * J.lda] : J.A[gk + (long long)gr * J.lda];
+          if (J.dk) v *= J.dk[gk];
+        } else if (TWO && gk - J.k < J.k2) {
+          gk -= J.k;
+          v = TA == 0 ? J.A2[gr + (long long)gk * J.lda] : J.A2[gk + (long long)gr * J.lda];
+        }
       }
       ra[q] = v;
       int c;
       b_ck(q, c, k);
-      const int gc = col0 + c, gkb = k0 + k;
+      const int gc = col0 + c;
+      int gkb = k0 + k;
       double u = 0.0;
-      if (gc < J.n && gkb < J.k) u = TB == 0 ? J.B[gkb + (long long)gc * J.ldb] : J.B[gc + (long long)gkb * J.ldb];
+      if (gc < J.n) {
+        if (gkb < J.k) {
+          u = TB == 0 ? J.B[gkb + (long long)gc * J.ldb] : J.B[gc + (long long)gkb * J.ldb];
+        } else if (TWO && gkb - J.k < J.k2) {
+          gkb -= J.k;
+          u = TB == 0 ? J.B2[gkb + (long long)gc * J.ldb] : J.B2[gc + (long long)gkb * J.ldb];
+        }
+      }
       rb[q] = u;
     }
   };
@@ -168,7 +188,7 @@ __global__ __launch_bounds__(256) void k_gemm128(const GemmJob* jobs) {
   for (int i = 0; i < 4; i++)
 #pragma unroll
     for (int j = 0; j < 4; j++) acc[i][j] = (d4v){0.0, 0.0, 0.0, 0.0};
-  const int nk = (J.k + BK - 1) / BK;
+  const int nk = (J.k + (TWO ? J.k2 : 0) + BK - 1) / BK;
   load(0);
   store(0);
   __syncthreads();
@@ -209,10 +229,23 @@ __global__ __launch_bounds__(256) void k_gemm128(const GemmJob* jobs) {
       }
 }
 
-static constexpr auto k_gemm128_nn = k_gemm128<0, 0>;
-static constexpr auto k_gemm128_nt = k_gemm128<0, 1>;
-static constexpr auto k_gemm128_tn = k_gemm128<1, 0>;
-static constexpr auto k_gemm128_tt = k_gemm128<1, 1>;
+// C = sum over s of the split-K partials P_s (m x n, ld m, back to back) + beta C, the partials
+// in a fixed order.
+__global__ void k_gemm_reduce(const double* P, int splits, int m, int n, double* C, long long ldc, double beta) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)m * n) return;
+  const int r = (int)(e % m), c = (int)(e / m);
+  double s = 0.0;
+  for (int q = 0; q < splits; q++) s += P[(long long)q * m * n + e];
+  double* out = &C[r + c * ldc];
+  *out = beta == 0.0 ? s : s + beta * *out;
+}
+
+static constexpr auto k_gemm128_nn = k_gemm128<0, 0, false>;
+static constexpr auto k_gemm128_nt = k_gemm128<0, 1, false>;
+static constexpr auto k_gemm128_tn = k_gemm128<1, 0, false>;
+static constexpr auto k_gemm128_tt = k_gemm128<1, 1, false>;
+static constexpr auto k_gemm128_nt2 = k_gemm128<0, 1, true>;  // C = alpha (A B^T + A2 B2^T) + beta C
 
 // ------------------------------------------------------------------ Jacobi eigensolver
 struct EigJob {
@@ -224,9 +257,14 @@ struct EigJob {
 };
 
 // Pair k of round r of the round-robin tournament over nn (even) indices: index 0 stays, the
-// others rotate.
+// others rotate (r < nn - 1, so one conditional subtraction replaces the modulo).
 __device__ __forceinline__ void tour_pair(int r, int k, int nn, int& p, int& q) {
-  auto pos = [&](int s) { return s == 0 ? 0 : 1 + (s - 1 + r) % (nn - 1); };
+  auto pos = [&](int s) {
+    if (s == 0) return 0;
+    int x = s - 1 + r;
+    if (x >= nn - 1) x -= nn - 1;
+    return 1 + x;
+  };
   p = pos(k);
   q = pos(nn - 1 - k);
   if (p > q) {
@@ -271,15 +309,48 @@ __global__ __launch_bounds__(512) void k_jacobi(const EigJob* jobs, int max_swee
   __syncthreads();
   const double an = anorm;
   const int nblk = np * (np + 1) / 2;
+  // this thread's lower 2 x 2 blocks (k1 >= k2; block b = k1 (k1 + 1) / 2 + k2, b = tid + j nt),
+  // the same in every round
+  constexpr int MAXB = 10;  // nblk <= 5050 for n <= 200
+  int bk1[MAXB], bk2[MAXB];
+  {
+    int k1 = 0, k2 = tid;
+    while (k2 > k1) {
+      k2 -= k1 + 1;
+      k1++;
+    }
+#pragma unroll
+    for (int j = 0; j < MAXB; j++) {
+      bk1[j] = k1;
+      bk2[j] = k2;
+      k2 += nt;
+      while (k2 > k1) {
+        k2 -= k1 + 1;
+        k1++;
+      }
+    }
+  }
+  // V elements e = tid + j nt as (pair k, row i): advanced by dk pairs and di rows per step
+  const int dk = nt / n, di = nt % n;
   int sweep = 0;
   bool conv = n == 1;
   for (sweep = 0; sweep < max_sweeps && !conv; sweep++) {
-    // convergence test over all off-diagonal elements
-    for (int e = tid; e < n * n; e += nt) {
-      int i = e % n, j = e / n;
-      if (i < j) {
-        double aij = A[e];
-        if (fabs(aij) > 1e-15 * sqrt(fabs(A[i + i * n]) * fabs(A[j + j * n])) && fabs(aij) > 1e-17 * an) rotated = 1;
+    // convergence test over all off-diagonal elements (i < j)
+    {
+      int j = tid / n, i = tid % n;
+      const int dj = nt / n, dii = nt % n;
+      for (int e = tid; e < n * n; e += nt) {
+        if (i < j) {
+          const double aij = A[e];
+          if (fabs(aij) > 1e-15 * sqrt(fabs(A[i + i * n]) * fabs(A[j + j * n])) && fabs(aij) > 1e-17 * an)
+            rotated = 1;
+        }
+        i += dii;
+        j += dj;
+        if (i >= n) {
+          i -= n;
+          j++;
+        }
       }
     }
     __syncthreads();
@@ -307,12 +378,11 @@ __global__ __launch_bounds__(512) void k_jacobi(const EigJob* jobs, int max_swee
         pqb[2 * k + 1] = q;
       }
       __syncthreads();
-      // lower 2 x 2 blocks (k1 >= k2) of A, mirrored; block b = k1 (k1 + 1) / 2 + k2
-      for (int b = tid; b < nblk; b += nt) {
-        int k1 = (int)((sqrtf(8.0f * b + 1.0f) - 1.0f) * 0.5f);
-        while ((k1 + 1) * (k1 + 2) / 2 <= b) k1++;
-        while (k1 * (k1 + 1) / 2 > b) k1--;
-        const int k2 = b - k1 * (k1 + 1) / 2;
+      // lower 2 x 2 blocks of A, mirrored
+#pragma unroll
+      for (int jb = 0; jb < MAXB; jb++) {
+        if (tid + jb * nt >= nblk) break;
+        const int k1 = bk1[jb], k2 = bk2[jb];
         const double s1 = csb[2 * k1 + 1], s2 = csb[2 * k2 + 1];
         if (s1 == 0.0 && s2 == 0.0) continue;
         const int p1 = pqb[2 * k1], q1 = pqb[2 * k1 + 1], p2 = pqb[2 * k2], q2 = pqb[2 * k2 + 1];
@@ -350,33 +420,44 @@ __global__ __launch_bounds__(512) void k_jacobi(const EigJob* jobs, int max_swee
           A[q2 + q1 * n] = n11;
         }
       }
-      // V <- V J: loads of a thread's elements first (memory-level parallelism in MODE < 2)
+      // V <- V J: loads of CH of this thread's elements first (memory-level parallelism in MODE < 2)
       constexpr int CH = 8;
+      int k0 = tid / n, i0 = tid % n;
       for (int e0 = tid; e0 < np * n; e0 += nt * CH) {
-        double vp[CH], vq[CH];
+        double vp[CH], vq[CH], cc[CH], ss[CH];
+        int ip[CH], iq[CH];
+        int k = k0, i = i0;
 #pragma unroll
         for (int u = 0; u < CH; u++) {
-          int e = e0 + u * nt;
           vp[u] = vq[u] = 0.0;
-          if (e < np * n) {
-            int k = e / n, i = e % n;
-            if (csb[2 * k + 1] != 0.0) {
-              vp[u] = V[i + pqb[2 * k] * n];
-              vq[u] = V[i + pqb[2 * k + 1] * n];
+          ss[u] = 0.0;
+          cc[u] = 1.0;
+          ip[u] = iq[u] = 0;
+          if (e0 + u * nt < np * n) {
+            const double sv = csb[2 * k + 1];
+            if (sv != 0.0) {
+              ss[u] = sv;
+              cc[u] = csb[2 * k];
+              ip[u] = i + pqb[2 * k] * n;
+              iq[u] = i + pqb[2 * k + 1] * n;
+              vp[u] = V[ip[u]];
+              vq[u] = V[iq[u]];
             }
           }
+          i += di;
+          k += dk;
+          if (i >= n) {
+            i -= n;
+            k++;
+          }
         }
+        k0 = k;
+        i0 = i;
 #pragma unroll
         for (int u = 0; u < CH; u++) {
-          int e = e0 + u * nt;
-          if (e < np * n) {
-            int k = e / n, i = e % n;
-            const double s = csb[2 * k + 1];
-            if (s != 0.0) {
-              const double c = csb[2 * k];
-              V[i + pqb[2 * k] * n] = c * vp[u] - s * vq[u];
-              V[i + pqb[2 * k + 1] * n] = s * vp[u] + c * vq[u];
-            }
+          if (ss[u] != 0.0) {
+            V[ip[u]] = cc[u] * vp[u] - ss[u] * vq[u];
+            V[iq[u]] = ss[u] * vp[u] + cc[u] * vq[u];
           }
         }
       }

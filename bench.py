@@ -129,6 +129,10 @@ def main():
         dmax_e = max(dmax_e, abs(res[i]["energy"] - o["energy"]))
         dmax_g = max(dmax_g, float(np.abs(res[i]["gradient"] - o["gradient"]).max()))
 
+    # host-side counters (Jacobi sweeps per solve) are reported separately from the kernels
+    counters = {k: v for k, v in prof.items() if not k.startswith("k_")}
+    prof = {k: v for k, v in prof.items() if k.startswith("k_")}
+    sweeps = {k: round(v[2] / v[0], 3) for k, v in counters.items() if k.startswith("sweeps_") and v[0]}
     # dominant kernel and its roofline from the live HIP-event timings
     total_gpu_ms = sum(v[1] for v in prof.values())
     name, (nl, ms, work) = max(prof.items(), key=lambda kv: kv[1][1])
@@ -157,7 +161,8 @@ def main():
         "config": {"workload": args.workload, "molecules": len(mols), "precision": args.precision,
                    "accuracy": acc, "etemp_K": 300.0, "mean_scf_iterations": float(np.mean(iters)) if iters else None,
                    "failed": len(bad), "parity_checked": min(args.check, len(mols)),
-                   "max_abs_dE_vs_oracle": dmax_e, "max_abs_dg_vs_oracle": dmax_g},
+                   "max_abs_dE_vs_oracle": dmax_e, "max_abs_dg_vs_oracle": dmax_g,
+                   "mean_jacobi_sweeps": sweeps},
         "roofline": {"bound": bound, "kernel": name, "achieved": achieved, "peak": peak, "unit": unit,
                      "frac": achieved / peak, "traffic": traffic, "avg_launch_ms": avg_s * 1e3, "launches": nl,
                      "kernel_share_of_gpu_time": ms / total_gpu_ms if total_gpu_ms else None},

@@ -150,7 +150,11 @@ def test_prepared_batch(ctx):
 def test_gemm():
     from axtb_amd import _testhooks as H
     rng = np.random.default_rng(1)
-    for (m, n, k, ta, tb) in [(5, 7, 3, 0, 0), (70, 65, 130, 0, 1), (64, 64, 64, 1, 0), (100, 33, 17, 1, 1)]:
+    # small tiles (batched kernel), 128-tile kernel, and split-K (few output tiles, long k)
+    for (m, n, k, ta, tb) in [(5, 7, 3, 0, 0), (70, 65, 130, 0, 1), (64, 64, 64, 1, 0), (100, 33, 17, 1, 1),
+                              (300, 270, 90, 0, 0), (257, 300, 45, 1, 1), (300, 64, 3000, 1, 0),
+                              (64, 64, 2500, 1, 0), (130, 70, 1111, 0, 1), (100, 120, 700, 1, 1),
+                              (260, 40, 900, 0, 0)]:
         A = rng.standard_normal((k, m) if ta else (m, k))
         B = rng.standard_normal((n, k) if tb else (k, n))
         d = rng.random(k)
