@@ -228,6 +228,7 @@ struct Engine {
     HIP_OK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
     // the batched Jacobi kernel handles n <= 200
     if (const char* v = std::getenv("AXTB_HIP_DENSE_MIN")) dense_min = std::min(200, std::max(1, std::atoi(v)));
+    if (const char* v = std::getenv("AXTB_HIP_OA_STEPS")) oa_steps = std::min(4, std::max(1, std::atoi(v)));
   }
   bool dense(int n) const { return n > dense_min; }
 
@@ -332,16 +333,19 @@ struct Engine {
   // Launches: matrices that fit in LDS with their vectors (mode 2), with A only (mode 1), neither
   // (mode 0). Within a launch the largest matrices go first (shorter tail); the mode 1/0 launches
   // run on a second stream beside the mode 2 launch, both joined back into the engine stream.
-  void eigh(std::vector<EigJob>& jobs, int max_sweeps = 60) {
+  // fp32: the rotations in FP32 (A and V in LDS; Z and w returned in FP64 for the refinement).
+  void eigh(std::vector<EigJob>& jobs, bool fp32 = false, int max_sweeps = 60) {
     if (jobs.empty()) return;
     if (!max_lds) {
       HIP_OK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
       if (max_lds <= 0) max_lds = 64 * 1024;
     }
+    const size_t es = fp32 ? 4 : 8;
     std::vector<EigJob> groups[3];
     for (auto& j : jobs) {
       if (j.n > 200) throw DeviceError("Jacobi eigensolver called for n > 200");
-      int mode = jacobi_lds(j.n, 2) <= (size_t)max_lds ? 2 : (jacobi_lds(j.n, 1) <= (size_t)max_lds ? 1 : 0);
+      int mode = jacobi_lds(j.n, 2, es) <= (size_t)max_lds ? 2 : (jacobi_lds(j.n, 1, es) <= (size_t)max_lds ? 1 : 0);
+      if (fp32 && mode == 0) throw DeviceError("FP32 Jacobi: matrix does not fit in LDS");
       groups[mode].push_back(j);
     }
     for (auto& g : groups) std::stable_sort(g.begin(), g.end(), [](const EigJob& a, const EigJob& b) { return a.n > b.n; });
@@ -364,18 +368,29 @@ struct Engine {
       }
       DBuf<EigJob>& buf = ejb[mode];
       buf.upload(js, s);
-      size_t shm = jacobi_lds(nmax, mode);
+      size_t shm = jacobi_lds(nmax, mode, es);
       int* stat = jstat.p + off;
       off += js.size();
-      if (mode == 2) {
-        HIP_OK(hipFuncSetAttribute((const void*)k_jacobi<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-        LAUNCHS(s, flops, k_jacobi<2>, dim3(js.size()), dim3(512), shm, buf.p, max_sweeps, stat);
-      } else if (mode == 1) {
-        HIP_OK(hipFuncSetAttribute((const void*)k_jacobi<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-        LAUNCHS(s, flops, k_jacobi<1>, dim3(js.size()), dim3(512), shm, buf.p, max_sweeps, stat);
-      } else {
-        HIP_OK(hipFuncSetAttribute((const void*)k_jacobi<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-        LAUNCHS(s, flops, k_jacobi<0>, dim3(js.size()), dim3(512), shm, buf.p, max_sweeps, stat);
+      auto go = [&](const void* fn, auto kern) {
+        HIP_OK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+        hipLaunchKernelGGL(kern, dim3(js.size()), dim3(512), shm, s, buf.p, max_sweeps, stat);
+      };
+      {
+        const char* nm = fp32 ? (mode == 2 ? "k_jacobi2_f32" : "k_jacobi1_f32")
+                              : (mode == 2 ? "k_jacobi2" : (mode == 1 ? "k_jacobi1" : "k_jacobi0"));
+        ProfScope ps_(&prof, s, nm, flops);
+        if (fp32) {
+          if (mode == 2)
+            go((const void*)k_jacobi2_f32, k_jacobi2_f32);
+          else
+            go((const void*)k_jacobi1_f32, k_jacobi1_f32);
+        } else if (mode == 2) {
+          go((const void*)k_jacobi2, k_jacobi2);
+        } else if (mode == 1) {
+          go((const void*)k_jacobi1, k_jacobi1);
+        } else {
+          go((const void*)k_jacobi0, k_jacobi0);
+        }
       }
       HIP_OK(hipGetLastError());
       if (prof.on) {
@@ -389,8 +404,9 @@ struct Engine {
           sum += std::abs(v);
           bad += v < 0;
         }
-        prof.count("sweeps_jacobi<" + std::to_string(mode) + ">", (long)hs.size(), sum);
-        if (bad) prof.count("unconverged_jacobi<" + std::to_string(mode) + ">", bad, (double)bad);
+        const std::string tag = std::to_string(mode) + (fp32 ? "_f32" : "");
+        prof.count("sweeps_jacobi" + tag, (long)hs.size(), sum);
+        if (bad) prof.count("unconverged_jacobi" + tag, bad, (double)bad);
       }
     }
     if (side) {
@@ -1069,7 +1085,38 @@ struct Engine {
       gemm(gjobs);
     }
   }
-  DBuf<double> s12_, mtau;
+  DBuf<double> s12_, mtau, oa_emax;
+  DBuf<EigJob> oa_jobs;
+  // One FP64 refinement step of the eigenvectors U (job.Z) of job.A; the refined vectors go to T
+  // and the eigenvalues to job.w. Works in the F, Vw, Wm and T pools at the job's offset.
+  void refine(const std::vector<EigJob>& ea) {
+    if (ea.empty()) return;
+    std::vector<GemmJob> gau, gsg, gu;
+    std::vector<EigJob> oj;
+    for (const EigJob& j : ea) {
+      const long long o0 = j.Z - Z.p;
+      const int n = j.n;
+      gau.push_back(GemmJob{n, n, n, 0, 0, j.A, j.Z, T.p + o0, nullptr, n, n, n, 1.0, 0.0});         // A U
+      gsg.push_back(GemmJob{n, n, n, 1, 0, j.Z, T.p + o0, F.p + o0, nullptr, n, n, n, 1.0, 0.0});    // S
+      gsg.push_back(GemmJob{n, n, n, 1, 0, j.Z, j.Z, Vw.p + o0, nullptr, n, n, n, 1.0, 0.0});        // G
+      oj.push_back(EigJob{n, F.p + o0, Vw.p + o0, j.w, Wm.p + o0});
+      gu.push_back(GemmJob{n, n, n, 0, 0, j.Z, Wm.p + o0, T.p + o0, nullptr, n, n, n, 1.0, 0.0});    // U'
+    }
+    oa_jobs.upload(oj, st);
+    oa_emax.ensure(oj.size());
+    for (int step = 0; step < oa_steps; step++) {
+      if (step > 0) {
+        // next step from the refined vectors: T -> Z
+        for (const EigJob& j : ea)
+          HIP_OK(hipMemcpyAsync(j.Z, T.p + (j.Z - Z.p), sizeof(double) * j.n * j.n, hipMemcpyDeviceToDevice, st));
+      }
+      gemm(gau);
+      gemm(gsg);
+      LAUNCH(k_oa_step, dim3(oj.size()), dim3(256), 0, (const EigJob*)oa_jobs.p, oa_emax.p);
+      gemm(gu);
+    }
+  }
+  int oa_steps = 1;  // refinement steps per SCC iteration (AXTB_HIP_OA_STEPS)
 
   // ------------------------------------------------------------------------------ SCC loop
   std::vector<int> hactive;
@@ -1106,7 +1153,9 @@ struct Engine {
       double* B = isdense[m] ? X.p + o0 : Bm.p + o0;
       g1[m] = GemmJob{n, n, n, 0, 0, F.p + o0, B, T.p + o0, nullptr, n, n, n, 1.0, 0.0};   // T = F B
       g2[m] = GemmJob{n, n, n, 1, 0, B, T.p + o0, Am.p + o0, nullptr, n, n, n, 1.0, 0.0};  // A = B^T T
-      g3v[m] = GemmJob{n, n, n, 0, 0, B, Z.p + o0, C.p + o0, nullptr, n, n, n, 1.0, 0.0};  // C = B U
+      // C = B U (mixed precision: U refined into T)
+      const bool refined = o.precision == 1 && !isdense[m];
+      g3v[m] = GemmJob{n, n, n, 0, 0, B, (refined ? T.p : Z.p) + o0, C.p + o0, nullptr, n, n, n, 1.0, 0.0};
       gP[m] = GemmJob{n, n, n, 0, 1, C.p + o0, C.p + o0, P.p + o0, focc.p + M.ao0, n, n, n, 1.0, 0.0};
       ejobs[m] = EigJob{n, Am.p + o0, Vw.p + o0, w.p + M.ao0, Z.p + o0};
     }
@@ -1148,7 +1197,14 @@ struct Engine {
       // generalized eigenproblem in the Loewdin basis
       gemm(a1);
       gemm(a2);
-      eigh(ea);
+      if (o.precision == 1) {
+        // mixed: Jacobi in FP32, then one FP64 Ogita-Aishima step U' = U (I + E) from
+        // S = U^T A U and G = U^T U (U in Z, A in Am; S -> F, G -> Vw, I + E -> Wm, U' -> T)
+        eigh(ea, true);
+        refine(ea);
+      } else {
+        eigh(ea);
+      }
       for (int m = 0; m < NM; m++)
         if (hactive[m] && isdense[m])
           dense_values(ejobs[m].A, ejobs[m].n, ejobs[m].w, mtau.p + mols[m].ao0, ejobs[m].V);

@@ -281,33 +281,48 @@ __device__ __forceinline__ void tour_pair(int r, int k, int nn, int& p, int& q) 
 // 2: A and V in LDS. A sweep starts with a test of every off-diagonal element against the
 // rotation threshold, so a converged matrix costs one pass, not a sweep.
 // status[job] = number of sweeps (negative if not converged within max_sweeps).
-template <int MODE>
+// Rotation thresholds per arithmetic: an element is rotated while |a_pq| > REL sqrt(|a_pp a_qq|)
+// and |a_pq| > ABS max|A|; in FP32 the stop is at the rounding level (the result is refined in
+// FP64 afterwards).
+template <typename T> struct JacobiTol;
+template <> struct JacobiTol<double> {
+  static constexpr double REL = 1e-15, ABS = 1e-17;
+};
+template <> struct JacobiTol<float> {
+  static constexpr float REL = 2.5e-7f, ABS = 2.5e-7f;
+};
+
+template <int MODE, typename T>
 __global__ __launch_bounds__(512) void k_jacobi(const EigJob* jobs, int max_sweeps, int* status) {
-  extern __shared__ double lds[];
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T* lds = (T*)lds_raw;
   __shared__ int rotated;
-  __shared__ double anorm;
+  __shared__ T anorm;
   const EigJob J = jobs[blockIdx.x];
   const int n = J.n, tid = threadIdx.x, nt = blockDim.x;
   if (n <= 0) return;
   const int nn = n + (n & 1), np = nn / 2;
-  // dynamic LDS: [A (n*n, MODE >= 1)] [V (n*n, MODE 2)] [c, s per pair] [p, q per pair]
-  double* A = MODE >= 1 ? lds : J.A;
-  double* V = MODE == 2 ? lds + n * n : J.V;
-  double* csb = lds + (MODE >= 1 ? n * n : 0) + (MODE == 2 ? n * n : 0);
-  int* pqb = (int*)(csb + 2 * np);
+  const int lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
+  // dynamic LDS: [A (n*n, MODE >= 1)] [V (n*n, MODE 2)] [c, s per pair: 2 buffers] [p, q per pair:
+  // 2 buffers]; MODE 0 works on J.A, J.V in place (FP64 only)
+  T* A = MODE >= 1 ? lds : (T*)J.A;
+  T* V = MODE == 2 ? lds + n * n : (T*)J.V;
+  T* csb0 = lds + (MODE >= 1 ? n * n : 0) + (MODE == 2 ? n * n : 0);
+  int* pqb0 = (int*)(csb0 + 4 * np);
   if (MODE >= 1)
-    for (int e = tid; e < n * n; e += nt) A[e] = J.A[e];
-  for (int e = tid; e < n * n; e += nt) V[e] = (e % n == e / n) ? 1.0 : 0.0;
+    for (int e = tid; e < n * n; e += nt) A[e] = (T)J.A[e];
+  for (int e = tid; e < n * n; e += nt) V[e] = (e % n == e / n) ? (T)1 : (T)0;
   if (tid == 0) rotated = 0;
   __syncthreads();
   if (tid < 64) {
-    double s = 0.0;
+    T s = 0;
     for (int e = tid; e < n * n; e += 64) s = fmax(s, fabs(A[e]));
     for (int o = 32; o > 0; o >>= 1) s = fmax(s, __shfl_xor(s, o, 64));
     if (tid == 0) anorm = s;
   }
   __syncthreads();
-  const double an = anorm;
+  const T an = anorm;
+  constexpr T REL = JacobiTol<T>::REL, ABS = JacobiTol<T>::ABS;
   const int nblk = np * (np + 1) / 2;
   // this thread's lower 2 x 2 blocks (k1 >= k2; block b = k1 (k1 + 1) / 2 + k2, b = tid + j nt),
   // the same in every round
@@ -330,8 +345,29 @@ __global__ __launch_bounds__(512) void k_jacobi(const EigJob* jobs, int max_swee
       }
     }
   }
-  // V elements e = tid + j nt as (pair k, row i): advanced by dk pairs and di rows per step
-  const int dk = nt / n, di = nt % n;
+  // rotation (c, s) and pair (p, q) of every pair k of round r into buffer `buf`
+  auto rotations = [&](int r, int buf) {
+    T* csb = csb0 + buf * 2 * np;
+    int* pqb = pqb0 + buf * 2 * np;
+    for (int k = tid; k < np; k += nt) {
+      int p, q;
+      tour_pair(r, k, nn, p, q);
+      T c = 1, s = 0;
+      if (q < n) {
+        const T app = A[p + p * n], aqq = A[q + q * n], apq = A[p + q * n];
+        if (fabs(apq) > REL * sqrt(fabs(app) * fabs(aqq)) && fabs(apq) > ABS * an) {
+          const T tau = (aqq - app) / (2 * apq);
+          const T t = (tau >= 0 ? (T)1 : (T)-1) / (fabs(tau) + sqrt(1 + tau * tau));
+          c = 1 / sqrt(1 + t * t);
+          s = t * c;
+        }
+      }
+      csb[2 * k] = c;
+      csb[2 * k + 1] = s;
+      pqb[2 * k] = p;
+      pqb[2 * k + 1] = q;
+    }
+  };
   int sweep = 0;
   bool conv = n == 1;
   for (sweep = 0; sweep < max_sweeps && !conv; sweep++) {
@@ -341,8 +377,8 @@ __global__ __launch_bounds__(512) void k_jacobi(const EigJob* jobs, int max_swee
       const int dj = nt / n, dii = nt % n;
       for (int e = tid; e < n * n; e += nt) {
         if (i < j) {
-          const double aij = A[e];
-          if (fabs(aij) > 1e-15 * sqrt(fabs(A[i + i * n]) * fabs(A[j + j * n])) && fabs(aij) > 1e-17 * an)
+          const T aij = A[e];
+          if (fabs(aij) > REL * sqrt(fabs(A[i + i * n]) * fabs(A[j + j * n])) && fabs(aij) > ABS * an)
             rotated = 1;
         }
         i += dii;
@@ -358,50 +394,36 @@ __global__ __launch_bounds__(512) void k_jacobi(const EigJob* jobs, int max_swee
     __syncthreads();
     if (conv) break;
     if (tid == 0) rotated = 0;
+    rotations(0, 0);
+    __syncthreads();
     for (int r = 0; r < nn - 1; r++) {
-      for (int k = tid; k < np; k += nt) {
-        int p, q;
-        tour_pair(r, k, nn, p, q);
-        double c = 1.0, s = 0.0;
-        if (q < n) {
-          double app = A[p + p * n], aqq = A[q + q * n], apq = A[p + q * n];
-          if (fabs(apq) > 1e-15 * sqrt(fabs(app) * fabs(aqq)) && fabs(apq) > 1e-17 * an) {
-            double tau = (aqq - app) / (2.0 * apq);
-            double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-            c = 1.0 / sqrt(1.0 + t * t);
-            s = t * c;
-          }
-        }
-        csb[2 * k] = c;
-        csb[2 * k + 1] = s;
-        pqb[2 * k] = p;
-        pqb[2 * k + 1] = q;
-      }
-      __syncthreads();
-      // lower 2 x 2 blocks of A, mirrored
+      const int buf = r & 1;
+      const T* csb = csb0 + buf * 2 * np;
+      const int* pqb = pqb0 + buf * 2 * np;
+      // A <- J^T A J on the lower 2 x 2 blocks, mirrored
 #pragma unroll
       for (int jb = 0; jb < MAXB; jb++) {
         if (tid + jb * nt >= nblk) break;
         const int k1 = bk1[jb], k2 = bk2[jb];
-        const double s1 = csb[2 * k1 + 1], s2 = csb[2 * k2 + 1];
-        if (s1 == 0.0 && s2 == 0.0) continue;
+        const T s1 = csb[2 * k1 + 1], s2 = csb[2 * k2 + 1];
+        if (s1 == 0 && s2 == 0) continue;
         const int p1 = pqb[2 * k1], q1 = pqb[2 * k1 + 1], p2 = pqb[2 * k2], q2 = pqb[2 * k2 + 1];
-        const double c1 = csb[2 * k1], c2 = csb[2 * k2];
+        const T c1 = csb[2 * k1], c2 = csb[2 * k2];
         const bool vq1 = q1 < n, vq2 = q2 < n;
-        double b00 = A[p1 + p2 * n];
-        double b01 = vq2 ? A[p1 + q2 * n] : 0.0;
-        double b10 = vq1 ? A[q1 + p2 * n] : 0.0;
-        double b11 = (vq1 && vq2) ? A[q1 + q2 * n] : 0.0;
-        double t00 = c2 * b00 - s2 * b01, t01 = s2 * b00 + c2 * b01;
-        double t10 = c2 * b10 - s2 * b11, t11 = s2 * b10 + c2 * b11;
-        double n00 = c1 * t00 - s1 * t10, n01 = c1 * t01 - s1 * t11;
-        double n10 = s1 * t00 + c1 * t10, n11 = s1 * t01 + c1 * t11;
+        const T b00 = A[p1 + p2 * n];
+        const T b01 = vq2 ? A[p1 + q2 * n] : (T)0;
+        const T b10 = vq1 ? A[q1 + p2 * n] : (T)0;
+        const T b11 = (vq1 && vq2) ? A[q1 + q2 * n] : (T)0;
+        const T t00 = c2 * b00 - s2 * b01, t01 = s2 * b00 + c2 * b01;
+        const T t10 = c2 * b10 - s2 * b11, t11 = s2 * b10 + c2 * b11;
+        const T n00 = c1 * t00 - s1 * t10, n01 = c1 * t01 - s1 * t11;
+        const T n10 = s1 * t00 + c1 * t10, n11 = s1 * t01 + c1 * t11;
         if (k1 == k2) {
           A[p1 + p1 * n] = n00;
           if (vq1) {
             A[q1 + q1 * n] = n11;
-            A[p1 + q1 * n] = 0.0;
-            A[q1 + p1 * n] = 0.0;
+            A[p1 + q1 * n] = 0;
+            A[q1 + p1 * n] = 0;
           }
           continue;
         }
@@ -420,45 +442,20 @@ __global__ __launch_bounds__(512) void k_jacobi(const EigJob* jobs, int max_swee
           A[q2 + q1 * n] = n11;
         }
       }
-      // V <- V J: loads of CH of this thread's elements first (memory-level parallelism in MODE < 2)
-      constexpr int CH = 8;
-      int k0 = tid / n, i0 = tid % n;
-      for (int e0 = tid; e0 < np * n; e0 += nt * CH) {
-        double vp[CH], vq[CH], cc[CH], ss[CH];
-        int ip[CH], iq[CH];
-        int k = k0, i = i0;
-#pragma unroll
-        for (int u = 0; u < CH; u++) {
-          vp[u] = vq[u] = 0.0;
-          ss[u] = 0.0;
-          cc[u] = 1.0;
-          ip[u] = iq[u] = 0;
-          if (e0 + u * nt < np * n) {
-            const double sv = csb[2 * k + 1];
-            if (sv != 0.0) {
-              ss[u] = sv;
-              cc[u] = csb[2 * k];
-              ip[u] = i + pqb[2 * k] * n;
-              iq[u] = i + pqb[2 * k + 1] * n;
-              vp[u] = V[ip[u]];
-              vq[u] = V[iq[u]];
-            }
-          }
-          i += di;
-          k += dk;
-          if (i >= n) {
-            i -= n;
-            k++;
-          }
-        }
-        k0 = k;
-        i0 = i;
-#pragma unroll
-        for (int u = 0; u < CH; u++) {
-          if (ss[u] != 0.0) {
-            V[ip[u]] = cc[u] * vp[u] - ss[u] * vq[u];
-            V[iq[u]] = ss[u] * vp[u] + cc[u] * vq[u];
-          }
+      __syncthreads();
+      // the next round's rotations (from the updated A) beside this round's V <- V J
+      if (r + 1 < nn - 1) rotations(r + 1, buf ^ 1);
+      // V: one pair per wavefront at a time, the lanes over consecutive rows
+      for (int k = wv; k < np; k += nw) {
+        const T s = csb[2 * k + 1];
+        if (s == 0) continue;
+        const T c = csb[2 * k];
+        T* vp = V + pqb[2 * k] * n;
+        T* vq = V + pqb[2 * k + 1] * n;
+        for (int i = lane; i < n; i += 64) {
+          const T a = vp[i], b = vq[i];
+          vp[i] = c * a - s * b;
+          vq[i] = s * a + c * b;
         }
       }
       __syncthreads();
@@ -466,26 +463,94 @@ __global__ __launch_bounds__(512) void k_jacobi(const EigJob* jobs, int max_swee
   }
   // eigenvalues ascending (ties by index), vectors permuted alike
   for (int i = tid; i < n; i += nt) {
-    double di = A[i + i * n];
+    const T di = A[i + i * n];
     int rank = 0;
     for (int j = 0; j < n; j++) {
-      double dj = A[j + j * n];
+      const T dj = A[j + j * n];
       if (dj < di || (dj == di && j < i)) rank++;
     }
-    J.w[rank] = di;
-    for (int r = 0; r < n; r++) J.Z[r + (long long)rank * n] = V[r + (long long)i * n];
+    J.w[rank] = (double)di;
+    for (int r = 0; r < n; r++) J.Z[r + (long long)rank * n] = (double)V[r + (long long)i * n];
   }
   if (tid == 0) status[blockIdx.x] = conv ? sweep : -sweep;
 }
 
-template __global__ void k_jacobi<0>(const EigJob*, int, int*);
-template __global__ void k_jacobi<1>(const EigJob*, int, int*);
-template __global__ void k_jacobi<2>(const EigJob*, int, int*);
+template __global__ void k_jacobi<0, double>(const EigJob*, int, int*);
+template __global__ void k_jacobi<1, double>(const EigJob*, int, int*);
+template __global__ void k_jacobi<2, double>(const EigJob*, int, int*);
+template __global__ void k_jacobi<1, float>(const EigJob*, int, int*);
+template __global__ void k_jacobi<2, float>(const EigJob*, int, int*);
 
-// dynamic LDS bytes of k_jacobi for an n x n matrix
-inline size_t jacobi_lds(int n, int mode) {
+static constexpr auto k_jacobi0 = k_jacobi<0, double>;
+static constexpr auto k_jacobi1 = k_jacobi<1, double>;
+static constexpr auto k_jacobi2 = k_jacobi<2, double>;
+static constexpr auto k_jacobi1_f32 = k_jacobi<1, float>;
+static constexpr auto k_jacobi2_f32 = k_jacobi<2, float>;
+
+// One Ogita-Aishima refinement step of approximate eigenvectors U of A (FP64), given
+// S = U^T A U and G = U^T U (R = I - G): lambda_i = s_ii / g_ii;
+//   e_ij = (s_ij - lambda_j g_ij) / (lambda_j - lambda_i)  if |lambda_i - lambda_j| > delta,
+//   e_ij = -g_ij / 2                                         otherwise (a cluster: orthonormalised only),
+//   e_ii = (1 - g_ii) / 2,  delta = 2 (||S - diag(lambda)||_F + max|lambda| ||R||_F),
+// and writes I + E (so that U' = U (I + E)) and lambda into w (T. Ogita, K. Aishima, Japan J. Indust.
+// Appl. Math. 35 (2018) 1007-1035, Algorithm 1). One workgroup per matrix; job.A = S, job.V = G,
+// job.Z = I + E, job.w = lambda.
+__global__ __launch_bounds__(256) void k_oa_step(const EigJob* jobs, double* emax) {
+  __shared__ double red[16];
+  const EigJob J = jobs[blockIdx.x];
+  const int n = J.n;
+  const double* S = J.A;
+  const double* G = J.V;
+  double* lam = J.w;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) lam[i] = S[i + (long long)i * n] / G[i + (long long)i * n];
+  __syncthreads();
+  double so = 0.0, ro = 0.0, am = 0.0;
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+    const int i = e % n, j = e / n;
+    const double sd = S[e] - (i == j ? lam[i] : 0.0);
+    const double rd = (i == j ? 1.0 : 0.0) - G[e];
+    so += sd * sd;
+    ro += rd * rd;
+  }
+  for (int i = threadIdx.x; i < n; i += blockDim.x) am = fmax(am, fabs(lam[i]));
+  so = block_sum(so, red);
+  ro = block_sum(ro, red);
+  // max |lambda| over the workgroup
+  for (int o = 32; o > 0; o >>= 1) am = fmax(am, __shfl_xor(am, o, 64));
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = am;
+  __syncthreads();
+  am = 0.0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); w++) am = fmax(am, red[w]);
+  const double delta = 2.0 * (sqrt(so) + am * sqrt(ro));
+  double emx = 0.0;
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+    const int i = e % n, j = e / n;
+    double v;
+    if (i == j) {
+      v = 0.5 * (1.0 - G[e]);
+    } else {
+      const double gap = lam[j] - lam[i];
+      v = fabs(gap) > delta ? (S[e] - lam[j] * G[e]) / gap : -0.5 * G[e];
+    }
+    emx = fmax(emx, fabs(v));
+    J.Z[e] = (i == j ? 1.0 : 0.0) + v;
+  }
+  for (int o = 32; o > 0; o >>= 1) emx = fmax(emx, __shfl_xor(emx, o, 64));
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = emx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) m = fmax(m, red[w]);
+    emax[blockIdx.x] = m;
+  }
+}
+
+// dynamic LDS bytes of k_jacobi for an n x n matrix in elements of `es` bytes
+inline size_t jacobi_lds(int n, int mode, size_t es = 8) {
   size_t np = (n + 1) / 2;
-  return (size_t)(mode >= 1 ? 1 : 0) * n * n * 8 + (size_t)(mode == 2 ? 1 : 0) * n * n * 8 + np * 2 * 8 + np * 2 * 4;
+  return (size_t)(mode >= 1 ? 1 : 0) * n * n * es + (size_t)(mode == 2 ? 1 : 0) * n * n * es + np * 4 * es + np * 4 * 4;
 }
 
 }  // namespace axtb

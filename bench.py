@@ -27,7 +27,7 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP64_MFMA = 78.6e12   # MI355X dense FP64 matrix (flop/s)
 PEAK_HBM = 8.0e12          # MI355X HBM3E (byte/s)
-STREAMING = {"k_fock", "k_mulliken_cols", "k_integrals"}
+STREAMING = {"k_fock", "k_mulliken_cols", "k_integrals", "k_trd_symv"}
 
 
 def workload(name, seed=0):
@@ -71,7 +71,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="batch4096")
-    ap.add_argument("--precision", default="fp64", choices=["fp64"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"],
+                    help="fp64, or mixed: FP32 Jacobi + one FP64 Ogita-Aishima refinement per SCC step")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check", type=int, default=8, help="molecules checked against the oracle after timing")
@@ -121,13 +122,15 @@ def main():
 
     # parity of a few molecules against the CPU oracle (not timed)
     import oracle
-    dmax_e = dmax_g = 0.0
+    dmax_e = dmax_g = dmax_q = dmax_eps = 0.0
     for i in range(min(args.check, len(mols))):
         if res[i]["status"]:
             continue
         o = oracle.engrad(mols[i][0], mols[i][1], acc=acc)
         dmax_e = max(dmax_e, abs(res[i]["energy"] - o["energy"]))
         dmax_g = max(dmax_g, float(np.abs(res[i]["gradient"] - o["gradient"]).max()))
+        dmax_q = max(dmax_q, float(np.abs(res[i]["charges"] - o["qat"]).max()))
+        dmax_eps = max(dmax_eps, float(np.abs(np.sort(res[i]["orbital_energies"]) - np.sort(o["emo"])).max()))
 
     # host-side counters (Jacobi sweeps per solve) are reported separately from the kernels
     counters = {k: v for k, v in prof.items() if not k.startswith("k_")}
@@ -156,12 +159,13 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
         "data": "synthetic (seeded molecules of the benchmark configuration)",
         "config": {"workload": args.workload, "molecules": len(mols), "precision": args.precision,
                    "accuracy": acc, "etemp_K": 300.0, "mean_scf_iterations": float(np.mean(iters)) if iters else None,
                    "failed": len(bad), "parity_checked": min(args.check, len(mols)),
                    "max_abs_dE_vs_oracle": dmax_e, "max_abs_dg_vs_oracle": dmax_g,
+                   "max_abs_dq_vs_oracle": dmax_q, "max_abs_deps_vs_oracle": dmax_eps,
                    "mean_jacobi_sweeps": sweeps},
         "roofline": {"bound": bound, "kernel": name, "achieved": achieved, "peak": peak, "unit": unit,
                      "frac": achieved / peak, "traffic": traffic, "avg_launch_ms": avg_s * 1e3, "launches": nl,

@@ -91,6 +91,25 @@ def test_dense_solver_everywhere(monkeypatch):
         assert r["iterations"] == o["iterations"]
 
 
+def test_mixed_precision_batch(ctx):
+    """FP32 Jacobi + one FP64 Ogita-Aishima refinement per SCC step: |dE| <= 1e-6 Eh,
+    max |dg| <= 1e-5 Eh/bohr against the FP64 oracle; charges and orbital energies reported."""
+    import oracle
+    from axtb_amd import _lib, structures as S
+    mols = S.batch(16, seed=7) + [S.config("water20"), S.config("c30h62")]
+    rs = ctx.engrad_batch(mols, _lib.make_opts(accuracy=1e-2, precision="mixed"))
+    worst = [0.0] * 4
+    for (z, x), r in zip(mols, rs):
+        assert r["status"] == 0
+        o = oracle.engrad(z, x, acc=1e-2)
+        d = [abs(r["energy"] - o["energy"]), np.abs(r["gradient"] - o["gradient"]).max(),
+             np.abs(r["charges"] - o["qat"]).max(),
+             np.abs(np.sort(r["orbital_energies"]) - np.sort(o["emo"])).max()]
+        worst = [max(a, b) for a, b in zip(worst, d)]
+        assert d[0] <= 1e-6 and d[1] <= 1e-5
+    print("mixed: max |dE| %.2e |dg| %.2e |dq| %.2e |deps| %.2e" % tuple(worst), flush=True)
+
+
 def test_charged_and_invariants(ctx):
     z, x = structure("caffeine")
     r, o = check(ctx, z, x, 1e-6, charge=1.0)
