@@ -1085,38 +1085,43 @@ struct Engine {
       gemm(gjobs);
     }
   }
-  DBuf<double> s12_, mtau, oa_emax;
+  DBuf<double> s12_, mtau, oa_emax, oa_delta;
   DBuf<EigJob> oa_jobs;
-  // One FP64 refinement step of the eigenvectors U (job.Z) of job.A; the refined vectors go to T
-  // and the eigenvalues to job.w. Works in the F, Vw, Wm and T pools at the job's offset.
+  // FP64 refinement steps of the eigenvectors U (job.Z) of job.A; the eigenvalues go to job.w and
+  // the refined vectors alternate between the Z and T pools (step s reads Z for even s, T for odd
+  // s), so they end in refined_pool(). Scratch: C (A U), F (S), Vw (G), Wm (I + E).
   void refine(const std::vector<EigJob>& ea) {
     if (ea.empty()) return;
-    std::vector<GemmJob> gau, gsg, gu;
     std::vector<EigJob> oj;
     for (const EigJob& j : ea) {
       const long long o0 = j.Z - Z.p;
-      const int n = j.n;
-      gau.push_back(GemmJob{n, n, n, 0, 0, j.A, j.Z, T.p + o0, nullptr, n, n, n, 1.0, 0.0});         // A U
-      gsg.push_back(GemmJob{n, n, n, 1, 0, j.Z, T.p + o0, F.p + o0, nullptr, n, n, n, 1.0, 0.0});    // S
-      gsg.push_back(GemmJob{n, n, n, 1, 0, j.Z, j.Z, Vw.p + o0, nullptr, n, n, n, 1.0, 0.0});        // G
-      oj.push_back(EigJob{n, F.p + o0, Vw.p + o0, j.w, Wm.p + o0});
-      gu.push_back(GemmJob{n, n, n, 0, 0, j.Z, Wm.p + o0, T.p + o0, nullptr, n, n, n, 1.0, 0.0});    // U'
+      oj.push_back(EigJob{j.n, F.p + o0, Vw.p + o0, j.w, Wm.p + o0});
     }
     oa_jobs.upload(oj, st);
     oa_emax.ensure(oj.size());
+    oa_delta.ensure(oj.size());
     for (int step = 0; step < oa_steps; step++) {
-      if (step > 0) {
-        // next step from the refined vectors: T -> Z
-        for (const EigJob& j : ea)
-          HIP_OK(hipMemcpyAsync(j.Z, T.p + (j.Z - Z.p), sizeof(double) * j.n * j.n, hipMemcpyDeviceToDevice, st));
+      double* in = step % 2 == 0 ? Z.p : T.p;
+      double* out = step % 2 == 0 ? T.p : Z.p;
+      std::vector<GemmJob> gau, gsg, gu;
+      for (const EigJob& j : ea) {
+        const long long o0 = j.Z - Z.p;
+        const int n = j.n;
+        double* U = in + o0;
+        gau.push_back(GemmJob{n, n, n, 0, 0, j.A, U, C.p + o0, nullptr, n, n, n, 1.0, 0.0});        // A U
+        gsg.push_back(GemmJob{n, n, n, 1, 0, U, C.p + o0, F.p + o0, nullptr, n, n, n, 1.0, 0.0});   // S
+        gsg.push_back(GemmJob{n, n, n, 1, 0, U, U, Vw.p + o0, nullptr, n, n, n, 1.0, 0.0});         // G
+        gu.push_back(GemmJob{n, n, n, 0, 0, U, Wm.p + o0, out + o0, nullptr, n, n, n, 1.0, 0.0});   // U'
       }
       gemm(gau);
       gemm(gsg);
-      LAUNCH(k_oa_step, dim3(oj.size()), dim3(256), 0, (const EigJob*)oa_jobs.p, oa_emax.p);
+      LAUNCH(k_oa_step, dim3(oj.size()), dim3(256), 0, (const EigJob*)oa_jobs.p, oa_emax.p, oa_delta.p,
+             step == 0 ? 1 : 0);
       gemm(gu);
     }
   }
-  int oa_steps = 1;  // refinement steps per SCC iteration (AXTB_HIP_OA_STEPS)
+  double* refined_pool() { return oa_steps % 2 == 0 ? Z.p : T.p; }
+  int oa_steps = 2;  // refinement steps per SCC iteration (AXTB_HIP_OA_STEPS)
 
   // ------------------------------------------------------------------------------ SCC loop
   std::vector<int> hactive;
@@ -1153,9 +1158,9 @@ struct Engine {
       double* B = isdense[m] ? X.p + o0 : Bm.p + o0;
       g1[m] = GemmJob{n, n, n, 0, 0, F.p + o0, B, T.p + o0, nullptr, n, n, n, 1.0, 0.0};   // T = F B
       g2[m] = GemmJob{n, n, n, 1, 0, B, T.p + o0, Am.p + o0, nullptr, n, n, n, 1.0, 0.0};  // A = B^T T
-      // C = B U (mixed precision: U refined into T)
+      // C = B U (mixed precision: U after the refinement steps)
       const bool refined = o.precision == 1 && !isdense[m];
-      g3v[m] = GemmJob{n, n, n, 0, 0, B, (refined ? T.p : Z.p) + o0, C.p + o0, nullptr, n, n, n, 1.0, 0.0};
+      g3v[m] = GemmJob{n, n, n, 0, 0, B, (refined ? refined_pool() : Z.p) + o0, C.p + o0, nullptr, n, n, n, 1.0, 0.0};
       gP[m] = GemmJob{n, n, n, 0, 1, C.p + o0, C.p + o0, P.p + o0, focc.p + M.ao0, n, n, n, 1.0, 0.0};
       ejobs[m] = EigJob{n, Am.p + o0, Vw.p + o0, w.p + M.ao0, Z.p + o0};
     }

@@ -495,7 +495,7 @@ static constexpr auto k_jacobi2_f32 = k_jacobi<2, float>;
 // and writes I + E (so that U' = U (I + E)) and lambda into w (T. Ogita, K. Aishima, Japan J. Indust.
 // Appl. Math. 35 (2018) 1007-1035, Algorithm 1). One workgroup per matrix; job.A = S, job.V = G,
 // job.Z = I + E, job.w = lambda.
-__global__ __launch_bounds__(256) void k_oa_step(const EigJob* jobs, double* emax) {
+__global__ __launch_bounds__(256) void k_oa_step(const EigJob* jobs, double* emax, double* delta_keep, int first) {
   __shared__ double red[16];
   const EigJob J = jobs[blockIdx.x];
   const int n = J.n;
@@ -522,7 +522,15 @@ __global__ __launch_bounds__(256) void k_oa_step(const EigJob* jobs, double* ema
   __syncthreads();
   am = 0.0;
   for (int w = 0; w < (int)(blockDim.x >> 6); w++) am = fmax(am, red[w]);
-  const double delta = 2.0 * (sqrt(so) + am * sqrt(ro));
+  // the first step's cluster threshold is kept for the later steps: a pair left unrotated as a
+  // cluster is not rotated later from a smaller delta; pairs whose first-order mixing would
+  // exceed 0.05 are treated as clusters too
+  double delta = 2.0 * (sqrt(so) + am * sqrt(ro));
+  if (first) {
+    if (threadIdx.x == 0) delta_keep[blockIdx.x] = delta;
+  } else {
+    delta = fmax(delta, delta_keep[blockIdx.x]);
+  }
   double emx = 0.0;
   for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
     const int i = e % n, j = e / n;
@@ -531,7 +539,8 @@ __global__ __launch_bounds__(256) void k_oa_step(const EigJob* jobs, double* ema
       v = 0.5 * (1.0 - G[e]);
     } else {
       const double gap = lam[j] - lam[i];
-      v = fabs(gap) > delta ? (S[e] - lam[j] * G[e]) / gap : -0.5 * G[e];
+      const double num = S[e] - lam[j] * G[e];
+      v = (fabs(gap) > delta && fabs(num) < 0.05 * fabs(gap)) ? num / gap : -0.5 * G[e];
     }
     emx = fmax(emx, fabs(v));
     J.Z[e] = (i == j ? 1.0 : 0.0) + v;
