@@ -1033,58 +1033,107 @@ struct Engine {
       fill(g3.p, 3LL * NA);
       fill(dcn3.p, NA);
     }
-    // start: EEQ charges (or neutral atoms)
+    // start: EEQ charges (or neutral atoms); systems that fit in LDS by LU, larger ones from the
+    // eigendecomposition
+    if (!max_lds) {
+      HIP_OK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+      if (max_lds <= 0) max_lds = 64 * 1024;
+    }
+    auto fits = [&](int n) { return ((size_t)n * n + n) * sizeof(double) <= (size_t)max_lds; };
     fill(qstart.p, NS);
     if (o.guess == 0) {
       LAUNCH(k_eeq_system, dim3(blocks((long long)(natmax + 1) * (natmax + 1), 256), NM), dim3(256), 0,
                          b, eq0.p, cneeq.p, eeqA.p, eeqr.p);
       std::vector<EigJob> jobs;
+      std::vector<int> lu, ev;
+      int nlu = 0;
       for (int m = 0; m < NM; m++) {
         int n1 = mols[m].nat + 1;
+        if (fits(n1)) {
+          lu.push_back(m);
+          nlu = std::max(nlu, n1);
+          continue;
+        }
+        ev.push_back(m);
         EigJob j{n1, eeqA.p + heq0[m], eeqV.p + heq0[m], eeqw.p + mols[m].at0 + m, eeqZ.p + heq0[m]};
         if (dense(n1))
           dense_eigh(j.A, n1, j.w, j.V, j.Z);
         else
           jobs.push_back(j);
       }
-      eigh(jobs);
-      LAUNCH(k_eeq_solve, dim3(NM), dim3(256), 0, b, eq0.p, eeqZ.p, eeqw.p, eeqr.p, eeqy.p, qstart.p,
-                         eeqfail.p);
+      if (!lu.empty()) {
+        mlu.upload(lu, st);
+        const size_t shm = ((size_t)nlu * nlu + nlu) * sizeof(double);
+        HIP_OK(hipFuncSetAttribute((const void*)k_eeq_lu, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+        LAUNCH(k_eeq_lu, dim3(lu.size()), dim3(512), shm, b, (const int*)mlu.p, eq0.p, (const double*)eeqA.p,
+               (const double*)eeqr.p, qstart.p, eeqfail.p);
+      }
+      if (!ev.empty()) {
+        eigh(jobs);
+        mev.upload(ev, st);
+        LAUNCH(k_eeq_solve, dim3(ev.size()), dim3(256), 0, b, (const int*)mev.p, eq0.p, eeqZ.p, eeqw.p, eeqr.p,
+               eeqy.p, qstart.p, eeqfail.p);
+      }
     }
-    // Loewdin orthogonaliser X = U s^-1/2 U^T from the eigenpairs of S
+    // S-orthonormal start basis X: L^-T from a Cholesky factor in LDS where S fits, else the
+    // Loewdin orthogonaliser U s^-1/2 U^T from the eigenpairs of S
     {
-      std::vector<EigJob> jobs;
+      std::vector<EigJob> jobs, cj;
+      std::vector<int> lw;
+      int nch = 0;
       for (int m = 0; m < NM; m++) {
         long long o0 = mols[m].mat0;
-        HIP_OK(hipMemcpyAsync(Am.p + o0, pool.p + o0, sizeof(double) * mols[m].nao * mols[m].nao,
-                              hipMemcpyDeviceToDevice, st));
-        EigJob j{mols[m].nao, Am.p + o0, Vw.p + o0, w.p + mols[m].ao0, Z.p + o0};
+        const int n = mols[m].nao;
+        if (!dense(n) && fits(n)) {
+          cj.push_back(EigJob{n, pool.p + o0, nullptr, nullptr, X.p + o0});
+          nch = std::max(nch, n);
+          continue;
+        }
+        lw.push_back(m);
+        HIP_OK(hipMemcpyAsync(Am.p + o0, pool.p + o0, sizeof(double) * n * n, hipMemcpyDeviceToDevice, st));
+        EigJob j{n, Am.p + o0, Vw.p + o0, w.p + mols[m].ao0, Z.p + o0};
         if (dense(j.n))
           dense_eigh(j.A, j.n, j.w, j.V, j.Z);
         else
           jobs.push_back(j);
       }
-      eigh(jobs);
-      std::vector<double> hw(NO);
-      HIP_OK(hipMemcpyAsync(hw.data(), w.p, sizeof(double) * NO, hipMemcpyDeviceToHost, st));
-      HIP_OK(hipStreamSynchronize(st));
-      std::vector<double> hs12(NO);
-      for (int m = 0; m < NM; m++)
-        for (int i = 0; i < mols[m].nao; i++) {
-          double s = hw[mols[m].ao0 + i];
-          if (!(s > 0.0)) throw Fail{AXTB_ERR_NUMERICAL, "overlap matrix is not positive definite"};
-          hs12[mols[m].ao0 + i] = 1.0 / std::sqrt(s);
-        }
-      s12_.upload(hs12, st);
-      std::vector<GemmJob> gjobs(NM);
-      for (int m = 0; m < NM; m++) {
-        long long o0 = mols[m].mat0;
-        int n = mols[m].nao;
-        gjobs[m] = GemmJob{n, n, n, 0, 1, Z.p + o0, Z.p + o0, X.p + o0, s12_.p + mols[m].ao0, n, n, n, 1.0, 0.0};
+      if (!cj.empty()) {
+        cjobs.upload(cj, st);
+        cfail.ensure(cj.size());
+        const size_t shm = ((size_t)nch * nch + nch) * sizeof(double);
+        HIP_OK(hipFuncSetAttribute((const void*)k_chol_basis, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+        LAUNCH(k_chol_basis, dim3(cj.size()), dim3(512), shm, (const EigJob*)cjobs.p, cfail.p);
+        std::vector<int> hf(cj.size());
+        HIP_OK(hipMemcpyAsync(hf.data(), cfail.p, sizeof(int) * cj.size(), hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+        for (int f : hf)
+          if (f) throw Fail{AXTB_ERR_NUMERICAL, "overlap matrix is not positive definite"};
       }
-      gemm(gjobs);
+      if (!lw.empty()) {
+        eigh(jobs);
+        std::vector<double> hw(NO);
+        HIP_OK(hipMemcpyAsync(hw.data(), w.p, sizeof(double) * NO, hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+        std::vector<double> hs12(NO, 0.0);
+        for (int m : lw)
+          for (int i = 0; i < mols[m].nao; i++) {
+            double sv = hw[mols[m].ao0 + i];
+            if (!(sv > 0.0)) throw Fail{AXTB_ERR_NUMERICAL, "overlap matrix is not positive definite"};
+            hs12[mols[m].ao0 + i] = 1.0 / std::sqrt(sv);
+          }
+        s12_.upload(hs12, st);
+        std::vector<GemmJob> gjobs;
+        for (int m : lw) {
+          long long o0 = mols[m].mat0;
+          int n = mols[m].nao;
+          gjobs.push_back(GemmJob{n, n, n, 0, 1, Z.p + o0, Z.p + o0, X.p + o0, s12_.p + mols[m].ao0, n, n, n, 1.0, 0.0});
+        }
+        gemm(gjobs);
+      }
     }
   }
+  DBuf<int> mlu, mev, cfail;
+  DBuf<EigJob> cjobs;
   DBuf<double> s12_, mtau, oa_emax, oa_delta;
   DBuf<EigJob> oa_jobs;
   // FP64 refinement steps of the eigenvectors U (job.Z) of job.A; the eigenvalues go to job.w and

@@ -153,10 +153,10 @@ __global__ void k_eeq_system(Batch b, const long long* eq0, const double* cn_eeq
 // q = V diag(1/w) V^T rhs of the EEQ system from its eigendecomposition, then the start shell
 // charges split by reference occupation (xtb.rs:530-547). One workgroup per molecule; a
 // singular system or non-finite charges leave neutral atoms (the reference's "sad" start).
-__global__ void k_eeq_solve(Batch b, const long long* eq0, const double* V, const double* w, const double* rhs,
-                            double* y, double* qstart, int* eeq_failed) {
+__global__ void k_eeq_solve(Batch b, const int* mlist, const long long* eq0, const double* V, const double* w,
+                            const double* rhs, double* y, double* qstart, int* eeq_failed) {
   __shared__ int bad;
-  int m = blockIdx.x;
+  int m = mlist[blockIdx.x];
   const Mol M = b.mol[m];
   int n1 = M.nat + 1;
   const double* Vm = V + eq0[m];

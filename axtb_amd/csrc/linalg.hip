@@ -556,6 +556,178 @@ __global__ __launch_bounds__(256) void k_oa_step(const EigJob* jobs, double* ema
   }
 }
 
+// S-orthonormal start basis B0 = L^-T from the Cholesky factor S = L L^T (the reduction of
+// GenEigSolver, base/src/linalg/eigen.rs:72-113), one workgroup per matrix in LDS (n^2 + n doubles):
+// L in the lower triangle, then X = L^-1 by right-looking forward substitution with its strictly
+// lower part stored transposed in the upper triangle (X(i, j) at [j + i n], which is where B0 = X^T
+// keeps it) and its diagonal in xd. job.A = S, job.Z = B0 (upper triangular); fail[job] = 1 if S
+// is not positive definite.
+__global__ __launch_bounds__(512) void k_chol_basis(const EigJob* jobs, int* fail) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  __shared__ int bad;
+  const EigJob J = jobs[blockIdx.x];
+  const int n = J.n, tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
+  double* M = (double*)lds_raw;
+  double* xd = M + n * n;
+  for (int e = tid; e < n * n; e += nt) M[e] = J.A[e];
+  if (tid == 0) bad = 0;
+  __syncthreads();
+  for (int k = 0; k < n; k++) {
+    if (tid == 0) {
+      double d = M[k + k * n];
+      if (!(d > 0.0)) {
+        bad = 1;
+        d = 1.0;
+      }
+      M[k + k * n] = sqrt(d);
+    }
+    __syncthreads();
+    const double lkk = M[k + k * n];
+    for (int i = k + 1 + tid; i < n; i += nt) M[i + k * n] /= lkk;
+    __syncthreads();
+    // trailing lower triangle: column j by one wavefront, rows i >= j over the lanes
+    for (int j = k + 1 + wv; j < n; j += nw) {
+      const double ljk = M[j + k * n];
+      for (int i = j + lane; i < n; i += 64) M[i + j * n] -= M[i + k * n] * ljk;
+    }
+    __syncthreads();
+  }
+  // strictly upper part <- 0 (X^T lives there), xd <- 1
+  for (int e = tid; e < n * n; e += nt)
+    if (e % n < e / n) M[e] = 0.0;
+  for (int i = tid; i < n; i += nt) xd[i] = 1.0;
+  __syncthreads();
+  for (int k = 0; k < n; k++) {
+    const double lkk = M[k + k * n];
+    // row k of X is final once divided by L(k, k)
+    for (int j = tid; j <= k; j += nt) {
+      if (j == k)
+        xd[k] /= lkk;
+      else
+        M[j + k * n] /= lkk;
+    }
+    __syncthreads();
+    // X(i, j) -= L(i, k) X(k, j) for i > k, j <= k: row i by one wavefront, columns over the lanes
+    for (int i = k + 1 + wv; i < n; i += nw) {
+      const double lik = M[i + k * n];
+      for (int j = lane; j <= k; j += 64) M[j + i * n] -= lik * (j == k ? xd[k] : M[j + k * n]);
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < n * n; e += nt) {
+    const int i = e % n, j = e / n;
+    J.Z[e] = i < j ? M[e] : (i == j ? xd[i] : 0.0);
+  }
+  if (tid == 0) fail[blockIdx.x] = bad;
+}
+
+// EEQ charges of the listed molecules by LU with partial pivoting (the first maximum, as the
+// reference's dense LU in base/src/eeq.rs), in LDS ((n+1)^2 + n+1 doubles), then the start shell
+// charges as in k_eeq_solve; a zero pivot or a non-finite charge gives the neutral start.
+__global__ __launch_bounds__(512) void k_eeq_lu(Batch b, const int* mlist, const long long* eq0, const double* Ag,
+                                                const double* rhs, double* qstart, int* eeq_failed) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  __shared__ int bad, piv;
+  __shared__ double pv[8];
+  __shared__ int pi[8];
+  const int m = mlist[blockIdx.x];
+  const Mol M = b.mol[m];
+  const int n = M.nat + 1, tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
+  double* A = (double*)lds_raw;
+  double* x = A + n * n;
+  for (int e = tid; e < n * n; e += nt) A[e] = Ag[eq0[m] + e];
+  for (int i = tid; i < n; i += nt) x[i] = rhs[(long long)M.at0 + m + i];
+  if (tid == 0) bad = 0;
+  __syncthreads();
+  for (int k = 0; k < n; k++) {
+    // pivot: the first row i >= k of maximum |A(i, k)|
+    double best = -1.0;
+    int bi = n;
+    for (int i = k + tid; i < n; i += nt) {
+      const double v = fabs(A[i + k * n]);
+      if (v > best) {
+        best = v;
+        bi = i;
+      }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      const double ov = __shfl_xor(best, o, 64);
+      const int oi = __shfl_xor(bi, o, 64);
+      if (ov > best || (ov == best && oi < bi)) {
+        best = ov;
+        bi = oi;
+      }
+    }
+    if (lane == 0) {
+      pv[wv] = best;
+      pi[wv] = bi;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double bv = pv[0];
+      int b0 = pi[0];
+      for (int w = 1; w < nw; w++)
+        if (pv[w] > bv || (pv[w] == bv && pi[w] < b0)) {
+          bv = pv[w];
+          b0 = pi[w];
+        }
+      piv = b0;
+      if (!(bv > 0.0)) bad = 1;
+    }
+    __syncthreads();
+    if (bad) break;
+    const int p = piv;
+    if (p != k) {
+      for (int j = tid; j < n; j += nt) {
+        const double t = A[k + j * n];
+        A[k + j * n] = A[p + j * n];
+        A[p + j * n] = t;
+      }
+      if (tid == 0) {
+        const double t = x[k];
+        x[k] = x[p];
+        x[p] = t;
+      }
+      __syncthreads();
+    }
+    const double akk = A[k + k * n];
+    for (int i = k + 1 + tid; i < n; i += nt) A[i + k * n] /= akk;
+    __syncthreads();
+    // trailing update: column j by one wavefront, rows over the lanes
+    for (int j = k + 1 + wv; j < n; j += nw) {
+      const double akj = A[k + j * n];
+      for (int i = k + 1 + lane; i < n; i += 64) A[i + j * n] -= A[i + k * n] * akj;
+    }
+    __syncthreads();
+  }
+  if (!bad) {
+    // L y = P b (unit lower), then U x = y
+    for (int k = 0; k < n; k++) {
+      const double xk = x[k];
+      for (int i = k + 1 + tid; i < n; i += nt) x[i] -= A[i + k * n] * xk;
+      __syncthreads();
+    }
+    for (int k = n - 1; k >= 0; k--) {
+      if (tid == 0) x[k] /= A[k + k * n];
+      __syncthreads();
+      const double xk = x[k];
+      for (int i = tid; i < k; i += nt) x[i] -= A[i + k * n] * xk;
+      __syncthreads();
+    }
+  }
+  for (int i = tid; i < M.nat; i += nt)
+    if (!isfinite(x[i])) bad = 1;
+  __syncthreads();
+  for (int i = tid; i < M.nat; i += nt) {
+    const int a = M.at0 + i;
+    double tot = 0.0;
+    for (int s = b.a.ish[a]; s < b.a.ish[a] + b.a.nsh[a]; s++) tot += b.s.refocc[s];
+    for (int s = b.a.ish[a]; s < b.a.ish[a] + b.a.nsh[a]; s++)
+      qstart[s] = (bad || !(tot > 0.0)) ? 0.0 : x[i] * b.s.refocc[s] / tot;
+  }
+  if (tid == 0 && bad) eeq_failed[m] = 1;
+}
+
 // dynamic LDS bytes of k_jacobi for an n x n matrix in elements of `es` bytes
 inline size_t jacobi_lds(int n, int mode, size_t es = 8) {
   size_t np = (n + 1) / 2;

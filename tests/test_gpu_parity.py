@@ -234,7 +234,8 @@ def test_setup_and_first_iteration():
     for k in ("S", "H0", "D", "Q", "cn", "qstart"):
         assert np.abs(g[k] - o[k]).max() < 1e-12, k
     n = len(o["S"])
-    assert np.abs(g["X"] @ o["S"] @ g["X"] - np.eye(n)).max() < 1e-11
+    # the start basis (L^-T of a Cholesky factor, or the Loewdin S^-1/2) is S-orthonormal
+    assert np.abs(g["X"].T @ o["S"] @ g["X"] - np.eye(n)).max() < 1e-11
     gi = H.iter1(z, x)
     for k in ("vsh", "vat", "vdp", "vqp", "F", "w", "P", "qsh"):
         assert np.abs(gi[k] - o[k]).max() < 1e-10, k
