@@ -229,6 +229,9 @@ struct Engine {
     // the batched Jacobi kernel handles n <= 200
     if (const char* v = std::getenv("AXTB_HIP_DENSE_MIN")) dense_min = std::min(200, std::max(1, std::atoi(v)));
     if (const char* v = std::getenv("AXTB_HIP_OA_STEPS")) oa_steps = std::min(4, std::max(1, std::atoi(v)));
+    if (const char* v = std::getenv("AXTB_HIP_DENSE_FULL")) dense_full = std::max(1, std::atoi(v));
+    if (const char* v = std::getenv("AXTB_HIP_DENSE_REFINE")) dense_refine = std::atoi(v) != 0;
+    if (const char* v = std::getenv("AXTB_HIP_GEMM128_BATCH")) gemm128_batch = std::atoi(v) != 0;
   }
   bool dense(int n) const { return n > dense_min; }
 
@@ -295,7 +298,8 @@ struct Engine {
     }
     // large products on 128 x 128 tiles (one transpose pair per launch; two-term products always),
     // the batch of small per-molecule products on 64 x 64 tiles
-    if (two || (same && (long long)mm * nn >= 256LL * 256 && jobs.size() <= 8)) {
+    if (two || (same && (long long)mm * nn >= 256LL * 256 && jobs.size() <= 8) ||
+        (same && (gemm128_batch || in_refine) && std::max(mm, nn) > 64)) {
       if (!same) throw DeviceError("gemm: mixed transposes in one launch");
       gj.upload(jobs, st);
       dim3 grid(blocks(nn, BT), blocks(mm, BT), (unsigned)jobs.size());
@@ -611,9 +615,11 @@ struct Engine {
       LAUNCH(k_bt_panel, dim3(blocks((long long)rows * b, 256)), dim3(256), 0, A, tau, n, i0, b, dbtV.p);
       double* Zr = Z + i0 + 1;
       // V^T V and Yt = Zr^T V; T from V^T V; VT = V T; Zr -= VT Yt^T
-      std::vector<GemmJob> j1{GemmJob{b, b, rows, 1, 0, dbtV.p, dbtV.p, dbtVtV.p, nullptr, rows, rows, b, 1.0, 0.0},
-                              GemmJob{k, b, rows, 1, 0, Zr, dbtV.p, dbtY.p, nullptr, n, rows, k, 1.0, 0.0}};
+      // two launches, so that each long-k product takes the split-K path
+      std::vector<GemmJob> j1{GemmJob{b, b, rows, 1, 0, dbtV.p, dbtV.p, dbtVtV.p, nullptr, rows, rows, b, 1.0, 0.0}};
       gemm(j1);
+      std::vector<GemmJob> j1b{GemmJob{k, b, rows, 1, 0, Zr, dbtV.p, dbtY.p, nullptr, n, rows, k, 1.0, 0.0}};
+      gemm(j1b);
       LAUNCH(k_bt_tfactor, dim3(1), dim3(64), 0, dbtVtV.p, tau, i0, b);
       std::vector<GemmJob> j2{GemmJob{rows, b, b, 0, 0, dbtV.p, dbtVtV.p, dbtVT.p, nullptr, rows, b, rows, 1.0, 0.0}};
       gemm(j2);
@@ -1135,41 +1141,111 @@ struct Engine {
   DBuf<int> mlu, mev, cfail;
   DBuf<EigJob> cjobs;
   DBuf<double> s12_, mtau, oa_emax, oa_delta;
+  DBuf<int> oa_slot;
   DBuf<EigJob> oa_jobs;
-  // FP64 refinement steps of the eigenvectors U (job.Z) of job.A; the eigenvalues go to job.w and
-  // the refined vectors alternate between the Z and T pools (step s reads Z for even s, T for odd
-  // s), so they end in refined_pool(). Scratch: C (A U), F (S), Vw (G), Wm (I + E).
-  void refine(const std::vector<EigJob>& ea) {
-    if (ea.empty()) return;
-    std::vector<EigJob> oj;
-    for (const EigJob& j : ea) {
-      const long long o0 = j.Z - Z.p;
-      oj.push_back(EigJob{j.n, F.p + o0, Vw.p + o0, j.w, Wm.p + o0});
-    }
-    oa_jobs.upload(oj, st);
-    oa_emax.ensure(oj.size());
-    oa_delta.ensure(oj.size());
-    for (int step = 0; step < oa_steps; step++) {
-      double* in = step % 2 == 0 ? Z.p : T.p;
-      double* out = step % 2 == 0 ? T.p : Z.p;
-      std::vector<GemmJob> gau, gsg, gu;
-      for (const EigJob& j : ea) {
-        const long long o0 = j.Z - Z.p;
-        const int n = j.n;
-        double* U = in + o0;
-        gau.push_back(GemmJob{n, n, n, 0, 0, j.A, U, C.p + o0, nullptr, n, n, n, 1.0, 0.0});        // A U
-        gsg.push_back(GemmJob{n, n, n, 1, 0, U, C.p + o0, F.p + o0, nullptr, n, n, n, 1.0, 0.0});   // S
-        gsg.push_back(GemmJob{n, n, n, 1, 0, U, U, Vw.p + o0, nullptr, n, n, n, 1.0, 0.0});         // G
-        gu.push_back(GemmJob{n, n, n, 0, 0, U, Wm.p + o0, out + o0, nullptr, n, n, n, 1.0, 0.0});   // U'
+  // FP64 refinement steps of the eigenvectors U (job.Z) of job.A; the eigenvalues go to job.w.
+  // Steps come in pairs (Z -> T -> Z, so the refined vectors end in Z): every matrix gets
+  // oa_steps (even) steps, a matrix whose last step still corrected by more than 1e-7 (max |e_ij|;
+  // the error left is about its square) two more; one still above that is solved again by the
+  // FP64 Jacobi (returned list). Scratch: C (A U), F (S), Vw (G), Wm (I + E).
+  std::vector<EigJob> refine(const std::vector<EigJob>& ea) {
+    std::vector<EigJob> cur = ea, redo;
+    if (ea.empty()) return redo;
+    std::vector<int> idx(ea.size());
+    for (size_t q = 0; q < ea.size(); q++) idx[q] = (int)q;
+    oa_delta.ensure(ea.size());
+    long steps_total = 0;
+    in_refine = true;
+    for (int round = 0; round < 2 && !cur.empty(); round++) {
+      const int nsteps = round == 0 ? std::max(2, oa_steps + (oa_steps & 1)) : 2;
+      for (int step = 0; step < nsteps; step++) {
+        std::vector<GemmJob> gau, gsg, gu;
+        std::vector<EigJob> oj;
+        for (const EigJob& j : cur) {
+          const long long o0 = j.Z - Z.p;
+          const int n = j.n;
+          double* U = (step % 2 == 0 ? Z.p : T.p) + o0;
+          double* out = (step % 2 == 0 ? T.p : Z.p) + o0;
+          gau.push_back(GemmJob{n, n, n, 0, 0, j.A, U, C.p + o0, nullptr, n, n, n, 1.0, 0.0});        // A U
+          gsg.push_back(GemmJob{n, n, n, 1, 0, U, C.p + o0, F.p + o0, nullptr, n, n, n, 1.0, 0.0});   // S
+          gsg.push_back(GemmJob{n, n, n, 1, 0, U, U, Vw.p + o0, nullptr, n, n, n, 1.0, 0.0});         // G
+          gu.push_back(GemmJob{n, n, n, 0, 0, U, Wm.p + o0, out, nullptr, n, n, n, 1.0, 0.0});        // U'
+          oj.push_back(EigJob{n, F.p + o0, Vw.p + o0, j.w, Wm.p + o0});
+        }
+        oa_jobs.upload(oj, st);
+        oa_emax.ensure(oj.size());
+        oa_slot.upload(idx, st);  // the kept cluster threshold of a job lives at its index in `ea`
+        gemm(gau);
+        gemm(gsg);
+        LAUNCH(k_oa_step, dim3(oj.size()), dim3(256), 0, (const EigJob*)oa_jobs.p, oa_emax.p, oa_delta.p,
+               (const int*)oa_slot.p, round == 0 && step == 0 ? 1 : 0);
+        gemm(gu);
+        steps_total += (long)cur.size();
       }
-      gemm(gau);
-      gemm(gsg);
-      LAUNCH(k_oa_step, dim3(oj.size()), dim3(256), 0, (const EigJob*)oa_jobs.p, oa_emax.p, oa_delta.p,
-             step == 0 ? 1 : 0);
-      gemm(gu);
+      std::vector<double> h(cur.size());
+      HIP_OK(hipMemcpyAsync(h.data(), oa_emax.p, sizeof(double) * cur.size(), hipMemcpyDeviceToHost, st));
+      HIP_OK(hipStreamSynchronize(st));
+      std::vector<EigJob> next;
+      std::vector<int> nidx;
+      for (size_t q = 0; q < cur.size(); q++)
+        if (!(h[q] <= 1e-7)) {
+          next.push_back(cur[q]);
+          nidx.push_back(idx[q]);
+        }
+      cur.swap(next);
+      idx.swap(nidx);
     }
+    in_refine = false;
+    prof.count("refine_steps", (long)ea.size(), (double)steps_total);
+    if (!cur.empty()) prof.count("refine_fallbacks", (long)cur.size(), (double)cur.size());
+    return cur;
   }
-  double* refined_pool() { return oa_steps % 2 == 0 ? Z.p : T.p; }
+
+  // The same refinement for one large matrix (the dense-solver molecules): the GEMMs on 128-tile
+  // kernels, the elementwise step over many workgroups. The refined vectors end in Z; returns the
+  // last max |e_ij|.
+  DBuf<double> oab_part, oab_emax, oab_delta;
+  double refine_big(const EigJob& j) {
+    const long long o0 = j.Z - Z.p;
+    const int n = j.n;
+    const int nb = 1024;
+    oab_part.ensure(3 * nb);
+    oab_emax.ensure(nb);
+    oab_delta.ensure(1);
+    const EigJob oj{n, F.p + o0, Vw.p + o0, j.w, Wm.p + o0};
+    double emax = 0.0;
+    int step = 0;
+    for (; step < oa_steps; step++) {
+      double* U = (step % 2 == 0 ? Z.p : T.p) + o0;
+      double* out = (step % 2 == 0 ? T.p : Z.p) + o0;
+      std::vector<GemmJob> gau{GemmJob{n, n, n, 0, 0, j.A, U, C.p + o0, nullptr, n, n, n, 1.0, 0.0}};
+      gemm(gau);
+      std::vector<GemmJob> gsg{GemmJob{n, n, n, 1, 0, U, C.p + o0, F.p + o0, nullptr, n, n, n, 1.0, 0.0},
+                               GemmJob{n, n, n, 1, 0, U, U, Vw.p + o0, nullptr, n, n, n, 1.0, 0.0}};
+      gemm(gsg);
+      LAUNCH(k_oa_big_norms, dim3(nb), dim3(256), 0, oj, oab_part.p);
+      LAUNCH(k_oa_big_e, dim3(nb), dim3(256), 0, oj, (const double*)oab_part.p, nb, oab_delta.p, step == 0 ? 1 : 0,
+             oab_emax.p);
+      std::vector<GemmJob> gu{GemmJob{n, n, n, 0, 0, U, Wm.p + o0, out, nullptr, n, n, n, 1.0, 0.0}};
+      gemm(gu);
+      std::vector<double> h(nb);
+      HIP_OK(hipMemcpyAsync(h.data(), oab_emax.p, sizeof(double) * nb, hipMemcpyDeviceToHost, st));
+      HIP_OK(hipStreamSynchronize(st));
+      emax = 0.0;
+      for (double v : h) emax = std::max(emax, v);
+    }
+    if (step % 2 == 1)  // odd number of steps: the result is in T
+      HIP_OK(hipMemcpyAsync(j.Z, T.p + o0, sizeof(double) * n * n, hipMemcpyDeviceToDevice, st));
+    prof.count("refine_steps_dense", 1, (double)step);
+    return emax;
+  }
+  int dense_full = 3;  // mixed: SCC iterations solved in full before refinement (AXTB_HIP_DENSE_FULL)
+  // mixed: refine the previous iteration's vectors of dense-solver molecules instead of a full
+  // solve (AXTB_HIP_DENSE_REFINE=1; off: on spectra with bands of near-degenerate levels the
+  // unresolved clusters leave the orbital energies and W inexact)
+  bool dense_refine = false;
+  bool gemm128_batch = false;  // batches of per-molecule products on the 128-tile kernel (AXTB_HIP_GEMM128_BATCH)
+  bool in_refine = false;      // the refinement's batches always take the 128-tile kernel (measured faster)
   int oa_steps = 2;  // refinement steps per SCC iteration (AXTB_HIP_OA_STEPS)
 
   // ------------------------------------------------------------------------------ SCC loop
@@ -1207,13 +1283,16 @@ struct Engine {
       double* B = isdense[m] ? X.p + o0 : Bm.p + o0;
       g1[m] = GemmJob{n, n, n, 0, 0, F.p + o0, B, T.p + o0, nullptr, n, n, n, 1.0, 0.0};   // T = F B
       g2[m] = GemmJob{n, n, n, 1, 0, B, T.p + o0, Am.p + o0, nullptr, n, n, n, 1.0, 0.0};  // A = B^T T
-      // C = B U (mixed precision: U after the refinement steps)
-      const bool refined = o.precision == 1 && !isdense[m];
-      g3v[m] = GemmJob{n, n, n, 0, 0, B, (refined ? refined_pool() : Z.p) + o0, C.p + o0, nullptr, n, n, n, 1.0, 0.0};
+      // C = B U (mixed precision: U after the refinement steps, also in Z)
+      g3v[m] = GemmJob{n, n, n, 0, 0, B, Z.p + o0, C.p + o0, nullptr, n, n, n, 1.0, 0.0};
       gP[m] = GemmJob{n, n, n, 0, 1, C.p + o0, C.p + o0, P.p + o0, focc.p + M.ao0, n, n, n, 1.0, 0.0};
       ejobs[m] = EigJob{n, Am.p + o0, Vw.p + o0, w.p + M.ao0, Z.p + o0};
     }
     std::vector<int> hkocc(NM);
+    // dense molecules in mixed mode: the next eigenproblem by a full solve (all n vectors kept for
+    // the refinement) or by refining the previous iteration's vectors
+    const bool mixed = o.precision == 1;
+    std::vector<int> dfull(NM, 1), drefined(NM, 0);
     HIP_OK(hipMemcpyAsync(Bm.p, X.p, sizeof(double) * tot_mat, hipMemcpyDeviceToDevice, st));
     int it = 0;
     for (it = 1; it <= o.max_iter; it++) {
@@ -1255,13 +1334,23 @@ struct Engine {
         // mixed: Jacobi in FP32, then one FP64 Ogita-Aishima step U' = U (I + E) from
         // S = U^T A U and G = U^T U (U in Z, A in Am; S -> F, G -> Vw, I + E -> Wm, U' -> T)
         eigh(ea, true);
-        refine(ea);
+        std::vector<EigJob> redo = refine(ea);
+        eigh(redo);  // not converged by the refinement: FP64 Jacobi
       } else {
         eigh(ea);
       }
-      for (int m = 0; m < NM; m++)
-        if (hactive[m] && isdense[m])
+      for (int m = 0; m < NM; m++) {
+        if (!hactive[m] || !isdense[m]) continue;
+        drefined[m] = mixed && dense_refine && !dfull[m];
+        if (drefined[m]) {
+          const double emax = refine_big(ejobs[m]);
+          // a refinement whose last correction is not small is followed by a full solve
+          dfull[m] = emax > 1e-6 ? 1 : 0;
+        } else {
           dense_values(ejobs[m].A, ejobs[m].n, ejobs[m].w, mtau.p + mols[m].ao0, ejobs[m].V);
+          dfull[m] = it < dense_full ? 1 : 0;
+        }
+      }
       gemm(a3);
       HIP_OK(hipMemcpyAsync(Bm.p, C.p, sizeof(double) * tot_mat, hipMemcpyDeviceToDevice, st));
       LAUNCH(k_fermi, dim3(NM), dim3(256), 0, b, active.p, w.p, focc.p, fwork.p, fst.p, kocc.p,
@@ -1277,8 +1366,13 @@ struct Engine {
         GemmJob p = gP[m];
         if (isdense[m]) {
           const int n = mols[m].nao, k = hkocc[m];
-          dense_vectors(ejobs[m].A, n, mtau.p + mols[m].ao0, ejobs[m].V, k, ejobs[m].Z);
           GemmJob c = g3v[m];
+          if (!drefined[m]) {
+            // mixed: all vectors, the start of the next iteration's refinement (refine_big reads and
+            // leaves the vectors in Z)
+            dense_vectors(ejobs[m].A, n, mtau.p + mols[m].ao0, ejobs[m].V, mixed && dense_refine ? n : k, ejobs[m].Z);
+          }
+          c.B = ejobs[m].Z;
           c.n = k;
           a5.push_back(c);
           p.k = k;

@@ -495,7 +495,8 @@ static constexpr auto k_jacobi2_f32 = k_jacobi<2, float>;
 // and writes I + E (so that U' = U (I + E)) and lambda into w (T. Ogita, K. Aishima, Japan J. Indust.
 // Appl. Math. 35 (2018) 1007-1035, Algorithm 1). One workgroup per matrix; job.A = S, job.V = G,
 // job.Z = I + E, job.w = lambda.
-__global__ __launch_bounds__(256) void k_oa_step(const EigJob* jobs, double* emax, double* delta_keep, int first) {
+__global__ __launch_bounds__(256) void k_oa_step(const EigJob* jobs, double* emax, double* delta_keep, const int* slot,
+                                                 int first) {
   __shared__ double red[16];
   const EigJob J = jobs[blockIdx.x];
   const int n = J.n;
@@ -527,9 +528,9 @@ __global__ __launch_bounds__(256) void k_oa_step(const EigJob* jobs, double* ema
   // exceed 0.05 are treated as clusters too
   double delta = 2.0 * (sqrt(so) + am * sqrt(ro));
   if (first) {
-    if (threadIdx.x == 0) delta_keep[blockIdx.x] = delta;
+    if (threadIdx.x == 0) delta_keep[slot[blockIdx.x]] = delta;
   } else {
-    delta = fmax(delta, delta_keep[blockIdx.x]);
+    delta = fmax(delta, delta_keep[slot[blockIdx.x]]);
   }
   double emx = 0.0;
   for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
@@ -553,6 +554,91 @@ __global__ __launch_bounds__(256) void k_oa_step(const EigJob* jobs, double* ema
     double m = 0.0;
     for (int w = 0; w < (int)(blockDim.x >> 6); w++) m = fmax(m, red[w]);
     emax[blockIdx.x] = m;
+  }
+}
+
+// The Ogita-Aishima step of k_oa_step for one large matrix, over many workgroups:
+// k_oa_big_norms: lambda_i = s_ii / g_ii (workgroup 0 writes them) and per-workgroup partial sums
+// of ||S - diag(lambda)||_F^2, ||I - G||_F^2 and max |lambda| (part[3 * block + 0..2]);
+// k_oa_big_e: delta from the partials (every workgroup the same, fixed order), then I + E
+// elementwise; emax_part[block] = max |e_ij| of the workgroup.
+__global__ __launch_bounds__(256) void k_oa_big_norms(const EigJob J, double* part) {
+  __shared__ double red[16];
+  const int n = J.n;
+  const long long nn = (long long)n * n;
+  double so = 0.0, ro = 0.0, am = 0.0;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < nn; e += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(e % n), j = (int)(e / n);
+    const double s = J.A[e], g = J.V[e];
+    double sd = s, rd = -g;
+    if (i == j) {
+      const double l = s / g;
+      sd = s - l;
+      rd = 1.0 - g;
+      am = fmax(am, fabs(l));
+      J.w[i] = l;  // every workgroup writes its own diagonal entries
+    }
+    so += sd * sd;
+    ro += rd * rd;
+  }
+  so = block_sum(so, red);
+  ro = block_sum(ro, red);
+  for (int o = 32; o > 0; o >>= 1) am = fmax(am, __shfl_xor(am, o, 64));
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = am;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) m = fmax(m, red[w]);
+    part[3 * blockIdx.x] = so;
+    part[3 * blockIdx.x + 1] = ro;
+    part[3 * blockIdx.x + 2] = m;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_oa_big_e(const EigJob J, const double* part, int nparts, double* delta_keep,
+                                                  int first, double* emax_part) {
+  __shared__ double red[16];
+  __shared__ double s_delta;
+  const int n = J.n;
+  const long long nn = (long long)n * n;
+  if (threadIdx.x == 0) {
+    double so = 0.0, ro = 0.0, am = 0.0;
+    for (int q = 0; q < nparts; q++) {
+      so += part[3 * q];
+      ro += part[3 * q + 1];
+      am = fmax(am, part[3 * q + 2]);
+    }
+    double d = 2.0 * (sqrt(so) + am * sqrt(ro));
+    if (!first) d = fmax(d, *delta_keep);
+    s_delta = d;
+  }
+  __syncthreads();
+  const double delta = s_delta;
+  if (first && blockIdx.x == 0 && threadIdx.x == 0) *delta_keep = delta;
+  double emx = 0.0;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < nn; e += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(e % n), j = (int)(e / n);
+    const double g = J.V[e];
+    double v;
+    if (i == j) {
+      v = 0.5 * (1.0 - g);
+    } else {
+      const double gap = J.w[j] - J.w[i];
+      const double num = J.A[e] - J.w[j] * g;
+      v = (fabs(gap) > delta && fabs(num) < 0.05 * fabs(gap)) ? num / gap : -0.5 * g;
+    }
+    emx = fmax(emx, fabs(v));
+    J.Z[e] = (i == j ? 1.0 : 0.0) + v;
+  }
+  for (int o = 32; o > 0; o >>= 1) emx = fmax(emx, __shfl_xor(emx, o, 64));
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = emx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) m = fmax(m, red[w]);
+    emax_part[blockIdx.x] = m;
   }
 }
 

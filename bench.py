@@ -9,6 +9,13 @@ A step is one energy + gradient pass over the whole batch (setup, SCC with per-m
 convergence masks, gradient), inputs resident on the device. Other workloads: --workload
 water1000 / water200 / c100h202 / water20 / c30h62 (one molecule per step).
 
+Precision (default both): the workload is run in plain FP64 and in mixed precision (FP32 Jacobi
+eigensolves and FP64 Ogita-Aishima refinement; for large molecules refinement of the previous
+SCC iteration's eigenvectors), each prepared once and timed separately. `value` is the mixed
+E+grad/s; `config` holds the FP64 E+grad/s, the speedup and the largest differences of the
+mixed results from the FP64 results over all molecules, and both against the CPU oracle on the
+first molecules.
+
 Prints one JSON line: E+grad/s (value), the roofline of the dominant kernel measured with HIP
 events on the engine's stream during the timed steps, and the CPU oracle's E+grad/s on a
 bounded sample of the same workload as the reported baseline.
@@ -65,14 +72,54 @@ def cpu_baseline(mols, budget_s, acc):
     return n / dt, n, dt
 
 
+def measure(mols, precision, steps, warmup, acc):
+    """Prepare the batch once, run `warmup` untimed and `steps` timed passes; returns the wall time of
+    the timed passes, the HIP-event kernel profile of those passes and the results."""
+    from axtb_amd import _lib
+    opts = _lib.make_opts(accuracy=acc, etemp=300.0, precision=precision, gradient=True)
+    ctx = _lib.Context(0)
+    ctx.prepare(mols, opts)
+    for i in range(warmup):
+        ctx.run()
+        print(f"[bench] {precision} warmup {i + 1}/{warmup} done", file=sys.stderr, flush=True)
+    ctx.synchronize()
+    ctx.profile(True)
+    ctx.profile_report()  # drop the warmup records
+    t0 = time.perf_counter()
+    for i in range(steps):
+        ctx.run()
+        print(f"[bench] {precision} step {i + 1}/{steps} done", file=sys.stderr, flush=True)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    prof = ctx.profile_report()
+    ctx.profile(False)
+    res = ctx.results()
+    ctx.close()
+    return dt, prof, res
+
+
+def max_diffs(a, b):
+    """max |dE|, |dg|, |dq|, |d eps| between two result sets (eps compared sorted)"""
+    d = [0.0] * 4
+    for r, o in zip(a, b):
+        if r["status"] or o["status"]:
+            continue
+        d[0] = max(d[0], abs(r["energy"] - o["energy"]))
+        d[1] = max(d[1], float(np.abs(r["gradient"] - o["gradient"]).max()))
+        d[2] = max(d[2], float(np.abs(r["charges"] - o["charges"]).max()))
+        d[3] = max(d[3], float(np.abs(np.sort(r["orbital_energies"]) - np.sort(o["orbital_energies"])).max()))
+    return d
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="batch4096")
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"],
-                    help="fp64, or mixed: FP32 Jacobi + one FP64 Ogita-Aishima refinement per SCC step")
+    ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed"],
+                    help="both (default): FP64 and mixed on the same inputs, value = mixed, with the FP64 "
+                         "rate and the mixed-vs-FP64 differences; or one of fp64 / mixed alone")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check", type=int, default=8, help="molecules checked against the oracle after timing")
@@ -94,48 +141,34 @@ def main():
               "multi-GPU runs are not part of this build (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
-    from axtb_amd import _lib
     acc = 1e-2
     mols = workload(args.workload)
-    opts = _lib.make_opts(accuracy=acc, etemp=300.0, precision=args.precision, gradient=True)
-    ctx = _lib.Context(0)
-    ctx.prepare(mols, opts)
-    print(f"[bench] workload {args.workload}: {len(mols)} molecules, {sum(len(m[0]) for m in mols)} atoms", file=sys.stderr,
-          flush=True)
-    for i in range(args.warmup):
-        ctx.run()
-        print(f"[bench] warmup {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
-    ctx.synchronize()
-    ctx.profile(True)
-    ctx.profile_report()  # drop the warmup records
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        ctx.run()
-        print(f"[bench] step {i + 1}/{args.steps} done", file=sys.stderr, flush=True)
-    ctx.synchronize()
-    dt = time.perf_counter() - t0
-    prof = ctx.profile_report()
-    ctx.profile(False)
-    res = ctx.results()
+    print(f"[bench] workload {args.workload}: {len(mols)} molecules, {sum(len(m[0]) for m in mols)} atoms",
+          file=sys.stderr, flush=True)
+    runs = {}
+    for prec in (["fp64", "mixed"] if args.precision == "both" else [args.precision]):
+        runs[prec] = measure(mols, prec, args.steps, args.warmup, acc)
+    main_prec = "mixed" if "mixed" in runs else "fp64"
+    dt, prof, res = runs[main_prec]
     bad = [i for i, r in enumerate(res) if r["status"] != 0]
     iters = [r["iterations"] for r in res if r["status"] == 0]
 
     # parity of a few molecules against the CPU oracle (not timed)
     import oracle
-    dmax_e = dmax_g = dmax_q = dmax_eps = 0.0
+    checked = {p: [0.0] * 4 for p in runs}
     for i in range(min(args.check, len(mols))):
-        if res[i]["status"]:
-            continue
         o = oracle.engrad(mols[i][0], mols[i][1], acc=acc)
-        dmax_e = max(dmax_e, abs(res[i]["energy"] - o["energy"]))
-        dmax_g = max(dmax_g, float(np.abs(res[i]["gradient"] - o["gradient"]).max()))
-        dmax_q = max(dmax_q, float(np.abs(res[i]["charges"] - o["qat"]).max()))
-        dmax_eps = max(dmax_eps, float(np.abs(np.sort(res[i]["orbital_energies"]) - np.sort(o["emo"])).max()))
+        ref = {"status": 0, "energy": o["energy"], "gradient": o["gradient"], "charges": o["qat"],
+               "orbital_energies": o["emo"]}
+        for p, (_, _, rs) in runs.items():
+            checked[p] = [max(a, b) for a, b in zip(checked[p], max_diffs([rs[i]], [ref]))]
 
     # host-side counters (Jacobi sweeps per solve) are reported separately from the kernels
     counters = {k: v for k, v in prof.items() if not k.startswith("k_")}
     prof = {k: v for k, v in prof.items() if k.startswith("k_")}
     sweeps = {k: round(v[2] / v[0], 3) for k, v in counters.items() if k.startswith("sweeps_") and v[0]}
+    refinement = {k: round(v[2] / v[0], 3) if k == "refine_steps" else v[0] for k, v in counters.items()
+                  if k.startswith("refine") and v[0]}
     # dominant kernel and its roofline from the live HIP-event timings
     total_gpu_ms = sum(v[1] for v in prof.values())
     name, (nl, ms, work) = max(prof.items(), key=lambda kv: kv[1][1])
@@ -148,6 +181,16 @@ def main():
 
     traffic = measured_traffic(name, args.workload)
     value = len(mols) * args.steps / dt
+    names = ("dE", "dg", "dq", "deps")
+    cfg_extra = {}
+    for p in runs:
+        cfg_extra[f"{p}_vs_oracle_max_abs"] = dict(zip(names, checked[p]))
+    if len(runs) == 2:
+        d = max_diffs(runs["mixed"][2], runs["fp64"][2])
+        fp64_rate = len(mols) * args.steps / runs["fp64"][0]
+        cfg_extra.update({"fp64_value": fp64_rate, "mixed_speedup_vs_fp64": value / fp64_rate,
+                          "mixed_vs_fp64_max_abs_all_molecules": dict(zip(names, d)),
+                          "mixed_within_1e-6_Eh": d[0] <= 1e-6})
     line = {
         "metric": "E+grad/s (GFN2-xTB energy + analytic gradient)",
         "value": value,
@@ -159,14 +202,12 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "dtype": "f64" if main_prec == "fp64" else "f32+f64",
         "data": "synthetic (seeded molecules of the benchmark configuration)",
-        "config": {"workload": args.workload, "molecules": len(mols), "precision": args.precision,
+        "config": {"workload": args.workload, "molecules": len(mols), "precision": main_prec,
                    "accuracy": acc, "etemp_K": 300.0, "mean_scf_iterations": float(np.mean(iters)) if iters else None,
-                   "failed": len(bad), "parity_checked": min(args.check, len(mols)),
-                   "max_abs_dE_vs_oracle": dmax_e, "max_abs_dg_vs_oracle": dmax_g,
-                   "max_abs_dq_vs_oracle": dmax_q, "max_abs_deps_vs_oracle": dmax_eps,
-                   "mean_jacobi_sweeps": sweeps},
+                   "failed": len(bad), "parity_checked": min(args.check, len(mols)), **cfg_extra,
+                   "mean_jacobi_sweeps": sweeps, "refinement": refinement},
         "roofline": {"bound": bound, "kernel": name, "achieved": achieved, "peak": peak, "unit": unit,
                      "frac": achieved / peak, "traffic": traffic, "avg_launch_ms": avg_s * 1e3, "launches": nl,
                      "kernel_share_of_gpu_time": ms / total_gpu_ms if total_gpu_ms else None},
