@@ -33,6 +33,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP64_MFMA = 78.6e12   # MI355X dense FP64 matrix (flop/s)
+PEAK_FP32_MFMA = 157.3e12  # MI355X dense FP32 matrix (flop/s); FP32 kernels are priced against it
 PEAK_HBM = 8.0e12          # MI355X HBM3E (byte/s)
 STREAMING = {"k_fock", "k_mulliken_cols", "k_integrals", "k_trd_symv"}
 
@@ -177,7 +178,8 @@ def main():
     if any(name.startswith(s) for s in STREAMING):
         bound, peak, unit, achieved = "hbm", PEAK_HBM / 1e9, "GB/s", per_launch / avg_s / 1e9
     else:
-        bound, peak, unit, achieved = "mfma", PEAK_FP64_MFMA / 1e12, "TFLOP/s", per_launch / avg_s / 1e12
+        pk = PEAK_FP32_MFMA if name.endswith("_f32") else PEAK_FP64_MFMA
+        bound, peak, unit, achieved = "mfma", pk / 1e12, "TFLOP/s", per_launch / avg_s / 1e12
 
     traffic = measured_traffic(name, args.workload)
     value = len(mols) * args.steps / dt
