@@ -69,6 +69,7 @@ def lib():
                                                 ctypes.POINTER(Opts), ctypes.POINTER(Result)]
         L.axtbhip_batch_prepare.argtypes = [ctypes.c_void_p, ctypes.POINTER(Mol), ctypes.c_int, ctypes.POINTER(Opts)]
         L.axtbhip_batch_run.argtypes = [ctypes.c_void_p]
+        L.axtbhip_batch_gradient.argtypes = [ctypes.c_void_p]
         L.axtbhip_batch_results.argtypes = [ctypes.c_void_p, ctypes.POINTER(Mol), ctypes.c_int,
                                             ctypes.POINTER(Result)]
         L.axtbhip_profile_enable.argtypes = [ctypes.c_void_p, ctypes.c_int]
@@ -123,14 +124,23 @@ class Context:
             raise AxtbError(rc, self.last_error())
 
     def run(self):
+        self._grad_ready = False
         rc = lib().axtbhip_batch_run(self._h)
         if rc:
             raise AxtbError(rc, self.last_error())
 
+    def gradient(self):
+        """The analytic gradient of the last run() (which may have run without gradient)."""
+        rc = lib().axtbhip_batch_gradient(self._h)
+        if rc:
+            raise AxtbError(rc, self.last_error())
+        self._grad_ready = True
+
     def results(self, raise_on_error=False):
         mols, res, keep, outs, opts = self._prep
         rc = lib().axtbhip_batch_results(self._h, mols, len(outs), res)
-        return self._unmarshal(rc, res, outs, opts, raise_on_error)
+        return self._unmarshal(rc, res, outs, opts, raise_on_error,
+                               with_gradient=opts.want_gradient or getattr(self, "_grad_ready", False))
 
     def profile(self, on=True):
         lib().axtbhip_profile_enable(self._h, int(on))
@@ -172,7 +182,7 @@ class Context:
             outs.append(o)
         return mols, res, keep, outs
 
-    def _unmarshal(self, rc, res, outs, opts, raise_on_error):
+    def _unmarshal(self, rc, res, outs, opts, raise_on_error, with_gradient=None):
         n = len(outs)
         if rc == ERR_DEVICE or (raise_on_error and rc):
             raise AxtbError(rc, self.last_error())
@@ -187,7 +197,7 @@ class Context:
                 d.update(o)
                 d["shell_charges"] = o["shell_charges"][:r.nsh]
                 d["orbital_energies"] = o["orbital_energies"][:r.nao]
-                if not opts.want_gradient:
+                if not (opts.want_gradient if with_gradient is None else with_gradient):
                     d["gradient"] = None
             results.append(d)
         return results

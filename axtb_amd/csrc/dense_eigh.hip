@@ -10,16 +10,17 @@ namespace axtb {
 // Blocked reduction to tridiagonal form (LAPACK dsytrd/dlatrd, lower): panels of TRD_NB columns;
 // within a panel the trailing matrix is left as it was and corrected through the panel's
 // reflectors V (kept in A below the diagonal, unit first element stored) and W; at the end of a
-// panel A22 -= V W^T + W V^T runs as two GEMMs on the matrix cores. Two launches per column:
+// panel A22 -= V W^T + W V^T runs as one two-term GEMM on the matrix cores. Two launches per column:
 // k_trd_col (W column of the previous reflector, update of the column, new reflector) and
-// k_trd_symv (A22 v in row x column-chunk partials, W^T v, V^T v and v^T A22 v partials).
+// k_trd_symv (A22 v in row-block x column-block partials from the lower triangle, W^T v, V^T v and
+// v^T A22 v partials).
 // With y1 = W^T v, y2 = V^T v and p = A22 v of reflector v (scalar tau):
 //   w = tau (p - V y1 - W y2) + alpha v,  alpha = -tau^2 (v^T p - 2 y1.y2) / 2.
-// A is kept symmetric in full storage (both triangles of the trailing matrix are updated).
+// Only the lower triangle of A is read and kept up to date (the trailing update skips the tiles
+// above the diagonal).
 constexpr int TRD_NB = 64;   // panel width
 constexpr int TRD_CB = 64;   // rows per workgroup of the column kernel (4 wavefronts share the terms)
-constexpr int TRD_SR = 64;   // rows per symv workgroup (4 wavefronts split the chunk's columns)
-constexpr int TRD_CH = 128;  // columns per symv chunk
+constexpr int TRD_CH = 64;   // symv tile (TRD_CH x TRD_CH of the lower triangle)
 constexpr int TRD_DR = 1024; // rows per dot workgroup (one column per wavefront)
 
 // Reflector of x = (ax, x1...) with sum(x1^2) = tot: H x = (beta, 0, ...), v = (1, x1 * scal).
@@ -67,8 +68,9 @@ __global__ __launch_bounds__(256) void k_trd_col(double* A, int n, int i0, int j
   int nch = 0;
   if (j > 0) {
     const int mp = n - i;  // rows of the previous reflector and of its symv
-    const int nrb = (mp + TRD_SR - 1) / TRD_SR, ndb = (mp + TRD_DR - 1) / TRD_DR;
+    const int ndb = (mp + TRD_DR - 1) / TRD_DR;
     nch = (mp + TRD_CH - 1) / TRD_CH;
+    const int ntile = nch * (nch + 1) / 2;
     double beta;
     trd_reflector(axv[ip], trd_sumsq(sq_prev, n, ip, lane), t, beta, scal);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -85,7 +87,7 @@ __global__ __launch_bounds__(256) void k_trd_col(double* A, int n, int i0, int j
         y2[c - jj] = s;
     }
     double va = 0.0;
-    for (int k = threadIdx.x; k < nrb * nch; k += 256) va += vpart[k];
+    for (int k = threadIdx.x; k < ntile; k += 256) va += vpart[k];
     va = block_sum(va, rs);  // its barriers also publish y1, y2
     double yy = 0.0;
     for (int c = 0; c < jj; c++) yy += y1[c] * y2[c];
@@ -151,48 +153,78 @@ __global__ __launch_bounds__(256) void k_trd_col(double* A, int n, int i0, int j
 }
 
 // Partials of p = A22 v (A22 = A[i+1:, i+1:], v the reflector of column i, scaled on the fly)
-// over row blocks of 64 x column chunks of 128 (32 columns per wavefront), with v^T p per
-// workgroup. Workgroups with blockIdx.y >= nch (j > 0) form W^T v and V^T v partials, one
-// column per wavefront over 1024 rows.
+// from the lower triangle only: the workgroups < nt = nb (nb + 1) / 2 take the 64 x 64 tiles
+// (rb, cb), rb >= cb, of A22 (nb = ceil(m / 64)); tile (rb, cb) writes the row sums of its rows as
+// partial cb (part[cb][rows of rb]) and, below the diagonal, its column sums A^T v as partial rb of
+// the rows of block cb (part[rb][rows of cb]), so that every row r gets exactly one partial q for
+// each q < nb; vpart[t] = v^T (both sums) of tile t. Diagonal tiles read the upper half from its
+// mirror in the lower half. The workgroups after nt form W^T v and V^T v partials, one column per
+// wavefront over 1024 rows.
 __global__ __launch_bounds__(256) void k_trd_symv(const double* A, int n, int i0, int j, const double* W,
                                                   const double* sq, const double* axv, double* part, double* dpart,
                                                   double* vpart) {
-  __shared__ double vs[TRD_CH];
-  __shared__ double red[4][TRD_SR];
+  __shared__ double tile[TRD_CH][TRD_CH + 1];  // tile[c][r] = A22(R0 + r, C0 + c)
+  __shared__ double vr[TRD_CH], vc[TRD_CH];
+  __shared__ double red[2][4][TRD_CH];
   const int i = i0 + j, m = n - i - 1;
   const long long ln = n;
   const double* x = A + i * ln + i + 1;
-  const int nrb = (m + TRD_SR - 1) / TRD_SR, nch = (m + TRD_CH - 1) / TRD_CH, ndb = (m + TRD_DR - 1) / TRD_DR;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int nb = (m + TRD_CH - 1) / TRD_CH, nt = nb * (nb + 1) / 2, ndb = (m + TRD_DR - 1) / TRD_DR;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   double t, beta, scal;
   trd_reflector(axv[i], trd_sumsq(sq, n, i, lane), t, beta, scal);
   auto vec = [&](int k) { return k == 0 ? 1.0 : x[k] * scal; };
-  if ((int)blockIdx.y < nch) {
-    const int rb = blockIdx.x;
-    const int c0 = blockIdx.y * TRD_CH;
-    const int c1 = min(TRD_CH, m - c0);
-    if (threadIdx.x < TRD_CH) vs[threadIdx.x] = (int)threadIdx.x < c1 ? vec(c0 + threadIdx.x) : 0.0;
-    __syncthreads();
-    const int rl = rb * TRD_SR + lane;
-    double s = 0.0;
-    if (rl < m) {
-      const double* a = A + (i + 1 + c0) * ln + (i + 1) + rl;
-      const int q0 = wv * (TRD_CH / 4), q1 = min(q0 + TRD_CH / 4, c1);
-#pragma unroll 8
-      for (int c = q0; c < q1; c++) s += a[c * ln] * vs[c];
+  if ((int)blockIdx.x < nt) {
+    const int q = blockIdx.x;
+    int rb = (int)((sqrt(8.0 * q + 1.0) - 1.0) * 0.5);
+    while ((rb + 1) * (rb + 2) / 2 <= q) rb++;
+    while (rb * (rb + 1) / 2 > q) rb--;
+    const int cb = q - rb * (rb + 1) / 2;
+    const int R0 = rb * TRD_CH, C0 = cb * TRD_CH;
+    const double* a = A + (long long)(i + 1 + C0) * ln + (i + 1 + R0);
+#pragma unroll 4
+    for (int e = tid; e < TRD_CH * TRD_CH; e += 256) {
+      const int c = e / TRD_CH, r = e % TRD_CH;
+      tile[c][r] = (R0 + r < m && C0 + c < m) ? a[c * ln + r] : 0.0;
     }
-    red[wv][lane] = s;
+    if (tid < TRD_CH) {
+      vr[tid] = R0 + tid < m ? vec(R0 + tid) : 0.0;
+    } else if (tid < 2 * TRD_CH) {
+      const int c = tid - TRD_CH;
+      vc[c] = C0 + c < m ? vec(C0 + c) : 0.0;
+    }
+    __syncthreads();
+    const bool diag = rb == cb;
+    // row sums: lane = row, wavefront = quarter of the columns
+    double s = 0.0, u = 0.0;
+    const int k0 = wv * (TRD_CH / 4);
+#pragma unroll 8
+    for (int c = k0; c < k0 + TRD_CH / 4; c++) s += (diag && c > lane ? tile[lane][c] : tile[c][lane]) * vc[c];
+    // column sums (below the diagonal): lane = column, wavefront = quarter of the rows
+    if (!diag) {
+#pragma unroll 8
+      for (int r = k0; r < k0 + TRD_CH / 4; r++) u += tile[lane][r] * vr[r];
+    }
+    red[0][wv][lane] = s;
+    red[1][wv][lane] = u;
     __syncthreads();
     if (wv == 0) {
-      const double tt = red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane];
-      if (rl < m) part[blockIdx.y * ln + (i + 1 + rl)] = tt;
-      const double pv = wave_sum(rl < m ? tt * vec(rl) : 0.0);
-      if (lane == 0) vpart[blockIdx.y * nrb + rb] = pv;
+      const double rs = red[0][0][lane] + red[0][1][lane] + red[0][2][lane] + red[0][3][lane];
+      if (R0 + lane < m) part[cb * ln + (i + 1 + R0 + lane)] = rs;
+      double pv = rs * vr[lane];
+      if (!diag) {
+        const double cs = red[1][0][lane] + red[1][1][lane] + red[1][2][lane] + red[1][3][lane];
+        if (C0 + lane < m) part[rb * ln + (i + 1 + C0 + lane)] = cs;
+        pv += cs * vc[lane];
+      }
+      pv = wave_sum(pv);
+      if (lane == 0) vpart[q] = pv;
     }
   } else {
-    const int rb = blockIdx.x;
-    const int c = 4 * (blockIdx.y - nch) + wv;
-    if (rb < ndb && c < 2 * j) {
+    const int d = blockIdx.x - nt;
+    const int rb = d % ndb;
+    const int c = 4 * (d / ndb) + wv;
+    if (c < 2 * j) {
       const double* y = c < j ? W + c * ln : A + (i0 + c - j) * ln;
       double s = 0.0;
 #pragma unroll 4
@@ -216,6 +248,30 @@ __global__ void k_hh_last(const double* A, int n, double* d, double* e, double* 
 __global__ void k_zero_u32(unsigned* p, int n) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) p[k] = 0u;
+}
+
+// Upper triangle of the n x n matrix A (leading dimension ld) from its lower triangle, by 32 x 32
+// tiles through LDS (workgroup t takes lower tile (rb, cb), t = rb (rb + 1) / 2 + cb).
+__global__ __launch_bounds__(256) void k_mirror_lower(double* A, int n, long long ld) {
+  __shared__ double tl[32][33];
+  const int q = blockIdx.x;
+  int rb = (int)((sqrt(8.0 * q + 1.0) - 1.0) * 0.5);
+  while ((rb + 1) * (rb + 2) / 2 <= q) rb++;
+  while (rb * (rb + 1) / 2 > q) rb--;
+  const int cb = q - rb * (rb + 1) / 2;
+  const int R0 = rb * 32, C0 = cb * 32;
+  for (int e = threadIdx.x; e < 32 * 32; e += 256) {
+    const int r = e & 31, c = e >> 5;
+    if (R0 + r < n && C0 + c < n) tl[c][r] = A[(R0 + r) + (C0 + c) * ld];
+  }
+  __syncthreads();
+  // element (C0 + c', R0 + r') of the upper triangle = (R0 + r', C0 + c'); consecutive threads
+  // write consecutive rows C0 + c'
+  for (int e = threadIdx.x; e < 32 * 32; e += 256) {
+    const int cp = e & 31, rp = e >> 5;
+    const int gr = C0 + cp, gc = R0 + rp;
+    if (gr < n && gc < n && gc > gr) A[gr + gc * ld] = tl[cp][rp];
+  }
 }
 
 // ------------------------------------------------------------------ divide and conquer

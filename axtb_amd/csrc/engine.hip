@@ -231,6 +231,8 @@ struct Engine {
     if (const char* v = std::getenv("AXTB_HIP_OA_STEPS")) oa_steps = std::min(4, std::max(1, std::atoi(v)));
     if (const char* v = std::getenv("AXTB_HIP_DENSE_FULL")) dense_full = std::max(1, std::atoi(v));
     if (const char* v = std::getenv("AXTB_HIP_DENSE_REFINE")) dense_refine = std::atoi(v) != 0;
+    if (const char* v = std::getenv("AXTB_HIP_F32_SWEEPS")) f32_sweeps = std::min(60, std::max(1, std::atoi(v)));
+    if (const char* v = std::getenv("AXTB_HIP_DENSE_ACCEPT")) dense_accept = std::min(1e-3, std::max(1e-12, std::atof(v)));
     if (const char* v = std::getenv("AXTB_HIP_GEMM128_BATCH")) gemm128_batch = std::atoi(v) != 0;
   }
   bool dense(int n) const { return n > dense_min; }
@@ -305,6 +307,8 @@ struct Engine {
       dim3 grid(blocks(nn, BT), blocks(mm, BT), (unsigned)jobs.size());
       if (two) {
         if (jobs[0].ta != 0 || jobs[0].tb != 1) throw DeviceError("gemm: two-term products are A B^T + A2 B2^T");
+        for (auto& j : jobs)
+          if (j.dk) throw DeviceError("gemm: two-term products take no inner scaling");
         LAUNCHW(flops, k_gemm128_nt2, grid, dim3(256), 0, gj.p);
       } else {
         launch128(jobs[0].ta * 2 + jobs[0].tb, grid, flops);
@@ -345,14 +349,19 @@ struct Engine {
       if (max_lds <= 0) max_lds = 64 * 1024;
     }
     const size_t es = fp32 ? 4 : 8;
-    std::vector<EigJob> groups[3];
-    for (auto& j : jobs) {
+    std::vector<int> gidx[3];
+    for (size_t q = 0; q < jobs.size(); q++) {
+      const EigJob& j = jobs[q];
       if (j.n > 200) throw DeviceError("Jacobi eigensolver called for n > 200");
       int mode = jacobi_lds(j.n, 2, es) <= (size_t)max_lds ? 2 : (jacobi_lds(j.n, 1, es) <= (size_t)max_lds ? 1 : 0);
       if (fp32 && mode == 0) throw DeviceError("FP32 Jacobi: matrix does not fit in LDS");
-      groups[mode].push_back(j);
+      gidx[mode].push_back((int)q);
     }
-    for (auto& g : groups) std::stable_sort(g.begin(), g.end(), [](const EigJob& a, const EigJob& b) { return a.n > b.n; });
+    std::vector<EigJob> groups[3];
+    for (int g = 0; g < 3; g++) {
+      std::stable_sort(gidx[g].begin(), gidx[g].end(), [&](int a, int b) { return jobs[a].n > jobs[b].n; });
+      for (int q : gidx[g]) groups[g].push_back(jobs[q]);
+    }
     jstat.ensure(jobs.size());
     const bool side = !groups[2].empty() && (!groups[1].empty() || !groups[0].empty());
     if (side) {
@@ -438,12 +447,12 @@ struct Engine {
     for (auto* b : {&dperm, &dkind, &dkcol, &dorg, &dsrc}) b->ensure(n);
     drot_ij.ensure(2 * (size_t)n);
     // 1. blocked Householder tridiagonalisation
-    const int nrbmax = (n + TRD_SR - 1) / TRD_SR, nchmax = (n + TRD_CH - 1) / TRD_CH;
+    const int nchmax = (n + TRD_CH - 1) / TRD_CH;
     const int ndbmax = (n + TRD_DR - 1) / TRD_DR, nsq = (n + TRD_CB - 1) / TRD_CB;
     dpart.ensure((size_t)nchmax * n);
     dtrW.ensure((size_t)n * TRD_NB);
     dtrDots.ensure((size_t)ndbmax * 2 * TRD_NB);
-    dtrVp.ensure((size_t)nrbmax * nchmax);
+    dtrVp.ensure((size_t)nchmax * (nchmax + 1) / 2);
     dtrSq.ensure(2 * (size_t)nsq);
     dtrAx.ensure(n);
     for (int i0 = 0; i0 + 1 < n; i0 += TRD_NB) {
@@ -457,8 +466,9 @@ struct Engine {
                j == b ? 1 : 0);
         if (j == b) break;
         const int m = n - i - 1;
-        const double bytes = 8.0 * m * (double)m;
-        LAUNCHW(bytes, k_trd_symv, dim3(blocks(m, TRD_SR), blocks(m, TRD_CH) + blocks(2 * j, 4)), dim3(256), 0,
+        const double bytes = 4.0 * m * (double)(m + 1);  // the lower triangle of A22
+        const int nbt = blocks(m, TRD_CH);
+        LAUNCHW(bytes, k_trd_symv, dim3(nbt * (nbt + 1) / 2 + blocks(m, TRD_DR) * blocks(2 * j, 4)), dim3(256), 0,
                 (const double*)A, n, i0, j, (const double*)dtrW.p, (const double*)sq_cur, (const double*)dtrAx.p,
                 dpart.p, dtrDots.p, dtrVp.p);
       }
@@ -466,11 +476,12 @@ struct Engine {
       double* A22 = A + (long long)r0 * n + r0;
       const double* V = A + (long long)i0 * n + r0;
       const double* Wt = dtrW.p + r0;
-      // A22 -= V W^T + W V^T in one pass over A22
+      // A22 -= V W^T + W V^T in one pass over A22, lower triangle (the only one read later)
       GemmJob g{mt, mt, b, 0, 1, V, Wt, A22, nullptr, n, n, n, -1.0, 1.0};
       g.A2 = Wt;
       g.B2 = V;
       g.k2 = b;
+      g.lower = 1;
       std::vector<GemmJob> g1{g};
       gemm(g1);
     }
@@ -694,11 +705,27 @@ struct Engine {
   }
 
   // the prepared batch on the device: setup, SCC and gradient (inputs resident in HBM)
+  bool scf_done = false, grad_done = false;
   void compute() {
+    scf_done = grad_done = false;
     if (mols.empty()) return;
     setup(opts_);
     scf(opts_);
-    if (opts_.want_gradient) gradient();
+    scf_done = true;
+    if (opts_.want_gradient) {
+      gradient();
+      grad_done = true;
+    }
+    HIP_OK(hipStreamSynchronize(st));
+  }
+
+  // the analytic gradient of the state the last compute() left (P, W's orbitals, potentials);
+  // a compute() without gradient followed by this equals a compute() with it
+  void gradient_of_state() {
+    if (mols.empty() || grad_done) return;
+    if (!scf_done) throw Fail{AXTB_ERR_INPUT, "no computed batch: batch_run first"};
+    gradient();
+    grad_done = true;
     HIP_OK(hipStreamSynchronize(st));
   }
 
@@ -1148,24 +1175,31 @@ struct Engine {
   // oa_steps (even) steps, a matrix whose last step still corrected by more than 1e-7 (max |e_ij|;
   // the error left is about its square) two more; one still above that is solved again by the
   // FP64 Jacobi (returned list). Scratch: C (A U), F (S), Vw (G), Wm (I + E).
-  std::vector<EigJob> refine(const std::vector<EigJob>& ea) {
+  // Steps alternate U between Z and T; round 0 is oa_steps steps (rounded up to a pair), round 1 a
+  // pair, each accepted at max |e_ij| <= 1e-7. in_t[q] tells whether job q's vectors ended in T (odd
+  // number of steps) or in Z. Returns the jobs not accepted.
+  std::vector<EigJob> refine(const std::vector<EigJob>& ea, std::vector<char>& in_t) {
     std::vector<EigJob> cur = ea, redo;
+    in_t.assign(ea.size(), 0);
     if (ea.empty()) return redo;
     std::vector<int> idx(ea.size());
     for (size_t q = 0; q < ea.size(); q++) idx[q] = (int)q;
     oa_delta.ensure(ea.size());
     long steps_total = 0;
     in_refine = true;
+    int gstep = 0;  // steps done by every job still in `cur`
     for (int round = 0; round < 2 && !cur.empty(); round++) {
       const int nsteps = round == 0 ? std::max(2, oa_steps + (oa_steps & 1)) : 2;
-      for (int step = 0; step < nsteps; step++) {
+      const double accept = 1e-7;
+      for (int step = 0; step < nsteps; step++, gstep++) {
         std::vector<GemmJob> gau, gsg, gu;
         std::vector<EigJob> oj;
-        for (const EigJob& j : cur) {
+        for (size_t q = 0; q < cur.size(); q++) {
+          const EigJob& j = cur[q];
           const long long o0 = j.Z - Z.p;
           const int n = j.n;
-          double* U = (step % 2 == 0 ? Z.p : T.p) + o0;
-          double* out = (step % 2 == 0 ? T.p : Z.p) + o0;
+          double* U = (gstep % 2 == 0 ? Z.p : T.p) + o0;
+          double* out = (gstep % 2 == 0 ? T.p : Z.p) + o0;
           gau.push_back(GemmJob{n, n, n, 0, 0, j.A, U, C.p + o0, nullptr, n, n, n, 1.0, 0.0});        // A U
           gsg.push_back(GemmJob{n, n, n, 1, 0, U, C.p + o0, F.p + o0, nullptr, n, n, n, 1.0, 0.0});   // S
           gsg.push_back(GemmJob{n, n, n, 1, 0, U, U, Vw.p + o0, nullptr, n, n, n, 1.0, 0.0});         // G
@@ -1178,7 +1212,7 @@ struct Engine {
         gemm(gau);
         gemm(gsg);
         LAUNCH(k_oa_step, dim3(oj.size()), dim3(256), 0, (const EigJob*)oa_jobs.p, oa_emax.p, oa_delta.p,
-               (const int*)oa_slot.p, round == 0 && step == 0 ? 1 : 0);
+               (const int*)oa_slot.p, gstep == 0 ? 1 : 0);
         gemm(gu);
         steps_total += (long)cur.size();
       }
@@ -1187,11 +1221,14 @@ struct Engine {
       HIP_OK(hipStreamSynchronize(st));
       std::vector<EigJob> next;
       std::vector<int> nidx;
-      for (size_t q = 0; q < cur.size(); q++)
-        if (!(h[q] <= 1e-7)) {
+      for (size_t q = 0; q < cur.size(); q++) {
+        if (h[q] <= accept) {
+          in_t[idx[q]] = gstep % 2;
+        } else {
           next.push_back(cur[q]);
           nidx.push_back(idx[q]);
         }
+      }
       cur.swap(next);
       idx.swap(nidx);
     }
@@ -1205,7 +1242,7 @@ struct Engine {
   // kernels, the elementwise step over many workgroups. The refined vectors end in Z; returns the
   // last max |e_ij|.
   DBuf<double> oab_part, oab_emax, oab_delta;
-  double refine_big(const EigJob& j) {
+  double refine_big(const EigJob& j, const int* cls) {
     const long long o0 = j.Z - Z.p;
     const int n = j.n;
     const int nb = 1024;
@@ -1215,38 +1252,55 @@ struct Engine {
     const EigJob oj{n, F.p + o0, Vw.p + o0, j.w, Wm.p + o0};
     double emax = 0.0;
     int step = 0;
-    for (; step < oa_steps; step++) {
+    while (step < dense_steps) {
       double* U = (step % 2 == 0 ? Z.p : T.p) + o0;
       double* out = (step % 2 == 0 ? T.p : Z.p) + o0;
       std::vector<GemmJob> gau{GemmJob{n, n, n, 0, 0, j.A, U, C.p + o0, nullptr, n, n, n, 1.0, 0.0}};
       gemm(gau);
+      // S = U^T (A U) and G = U^T U are symmetric: their lower tiles, then mirrored
       std::vector<GemmJob> gsg{GemmJob{n, n, n, 1, 0, U, C.p + o0, F.p + o0, nullptr, n, n, n, 1.0, 0.0},
                                GemmJob{n, n, n, 1, 0, U, U, Vw.p + o0, nullptr, n, n, n, 1.0, 0.0}};
+      gsg[0].lower = gsg[1].lower = 1;
       gemm(gsg);
+      const int nbm = blocks(n, 32);
+      LAUNCH(k_mirror_lower, dim3(nbm * (nbm + 1) / 2), dim3(256), 0, F.p + o0, n, (long long)n);
+      LAUNCH(k_mirror_lower, dim3(nbm * (nbm + 1) / 2), dim3(256), 0, Vw.p + o0, n, (long long)n);
       LAUNCH(k_oa_big_norms, dim3(nb), dim3(256), 0, oj, oab_part.p);
       LAUNCH(k_oa_big_e, dim3(nb), dim3(256), 0, oj, (const double*)oab_part.p, nb, oab_delta.p, step == 0 ? 1 : 0,
-             oab_emax.p);
+             oab_emax.p, cls);
       std::vector<GemmJob> gu{GemmJob{n, n, n, 0, 0, U, Wm.p + o0, out, nullptr, n, n, n, 1.0, 0.0}};
       gemm(gu);
+      step++;
       std::vector<double> h(nb);
       HIP_OK(hipMemcpyAsync(h.data(), oab_emax.p, sizeof(double) * nb, hipMemcpyDeviceToHost, st));
       HIP_OK(hipStreamSynchronize(st));
       emax = 0.0;
       for (double v : h) emax = std::max(emax, v);
+      if (emax <= dense_accept || emax >= 1.0) break;
     }
     if (step % 2 == 1)  // odd number of steps: the result is in T
       HIP_OK(hipMemcpyAsync(j.Z, T.p + o0, sizeof(double) * n * n, hipMemcpyDeviceToDevice, st));
     prof.count("refine_steps_dense", 1, (double)step);
     return emax;
   }
+  DBuf<int> dcls, dnfrac;
+  std::vector<int> dense_pfp;  // per molecule: its last eigenproblem was refined (W = P F P / 2)
+  int dense_steps = 4;         // mixed, dense: refinement steps at most per SCC iteration
+  // mixed, dense: a refinement is accepted once its step's largest correction max |e_ij| (over the
+  // occupied-empty pairs and the orthogonality terms) is <= dense_accept; the error it leaves is of
+  // second order in that correction (AXTB_HIP_DENSE_ACCEPT)
+  double dense_accept = 1e-5;
   int dense_full = 3;  // mixed: SCC iterations solved in full before refinement (AXTB_HIP_DENSE_FULL)
   // mixed: refine the previous iteration's vectors of dense-solver molecules instead of a full
-  // solve (AXTB_HIP_DENSE_REFINE=1; off: on spectra with bands of near-degenerate levels the
-  // unresolved clusters leave the orbital energies and W inexact)
-  bool dense_refine = false;
+  // solve (AXTB_HIP_DENSE_REFINE=0 turns it off). Only the occupied-empty pairs are resolved: P
+  // depends only on the occupied subspace, and W is formed as P F P / 2 (exact when every
+  // occupation is 0 or 2, the condition for refining); the orbital energies are the Rayleigh
+  // quotients of the refined vectors
+  bool dense_refine = true;
   bool gemm128_batch = false;  // batches of per-molecule products on the 128-tile kernel (AXTB_HIP_GEMM128_BATCH)
   bool in_refine = false;      // the refinement's batches always take the 128-tile kernel (measured faster)
   int oa_steps = 2;  // refinement steps per SCC iteration (AXTB_HIP_OA_STEPS)
+  int f32_sweeps = 60;  // mixed: FP32 Jacobi sweeps at most after the first iteration (AXTB_HIP_F32_SWEEPS)
 
   // ------------------------------------------------------------------------------ SCC loop
   std::vector<int> hactive;
@@ -1293,6 +1347,7 @@ struct Engine {
     // the refinement) or by refining the previous iteration's vectors
     const bool mixed = o.precision == 1;
     std::vector<int> dfull(NM, 1), drefined(NM, 0);
+    dense_pfp.assign(NM, 0);
     HIP_OK(hipMemcpyAsync(Bm.p, X.p, sizeof(double) * tot_mat, hipMemcpyDeviceToDevice, st));
     int it = 0;
     for (it = 1; it <= o.max_iter; it++) {
@@ -1333,9 +1388,14 @@ struct Engine {
       if (o.precision == 1) {
         // mixed: Jacobi in FP32, then one FP64 Ogita-Aishima step U' = U (I + E) from
         // S = U^T A U and G = U^T U (U in Z, A in Am; S -> F, G -> Vw, I + E -> Wm, U' -> T)
-        eigh(ea, true);
-        std::vector<EigJob> redo = refine(ea);
-        eigh(redo);  // not converged by the refinement: FP64 Jacobi
+        // after the first iteration A is nearly diagonal: f32_sweeps sweeps, the refinement does the rest
+        eigh(ea, true, it == 1 ? 60 : f32_sweeps);
+        std::vector<char> in_t;
+        std::vector<EigJob> redo = refine(ea, in_t);
+        eigh(redo);  // not converged by the refinement: FP64 Jacobi (vectors in Z)
+        // C = B U from wherever U ended (a3 and ea list the same molecules in the same order)
+        for (size_t q = 0; q < ea.size(); q++)
+          if (in_t[q]) a3[q].B = T.p + (ea[q].Z - Z.p);
       } else {
         eigh(ea);
       }
@@ -1343,13 +1403,26 @@ struct Engine {
         if (!hactive[m] || !isdense[m]) continue;
         drefined[m] = mixed && dense_refine && !dfull[m];
         if (drefined[m]) {
-          const double emax = refine_big(ejobs[m]);
-          // a refinement whose last correction is not small is followed by a full solve
-          dfull[m] = emax > 1e-6 ? 1 : 0;
-        } else {
-          dense_values(ejobs[m].A, ejobs[m].n, ejobs[m].w, mtau.p + mols[m].ao0, ejobs[m].V);
-          dfull[m] = it < dense_full ? 1 : 0;
+          // the previous iteration's vectors refined for the new matrix (classes from its
+          // occupations); not converged in dense_steps steps: a full solve instead
+          const int n = mols[m].nao;
+          dcls.ensure(n);
+          dnfrac.ensure(1);
+          LAUNCH(k_zero_u32, dim3(1), dim3(64), 0, (unsigned*)dnfrac.p, 1);
+          LAUNCH(k_occ_class, dim3(blocks(n, 256)), dim3(256), 0, (const double*)focc.p + mols[m].ao0, n, dcls.p,
+                 dnfrac.p);
+          int nfrac = 0;
+          HIP_OK(hipMemcpyAsync(&nfrac, dnfrac.p, sizeof(int), hipMemcpyDeviceToHost, st));
+          HIP_OK(hipStreamSynchronize(st));
+          const double emax = nfrac ? 1.0 : refine_big(ejobs[m], dcls.p);
+          if (!(emax <= dense_accept)) {
+            drefined[m] = 0;
+            prof.count("refine_dense_fallbacks", 1, 1.0);
+          }
         }
+        if (!drefined[m]) dense_values(ejobs[m].A, ejobs[m].n, ejobs[m].w, mtau.p + mols[m].ao0, ejobs[m].V);
+        dfull[m] = it < dense_full ? 1 : 0;
+        dense_pfp[m] = drefined[m];
       }
       gemm(a3);
       HIP_OK(hipMemcpyAsync(Bm.p, C.p, sizeof(double) * tot_mat, hipMemcpyDeviceToDevice, st));
@@ -1370,7 +1443,8 @@ struct Engine {
           if (!drefined[m]) {
             // mixed: all vectors, the start of the next iteration's refinement (refine_big reads and
             // leaves the vectors in Z)
-            dense_vectors(ejobs[m].A, n, mtau.p + mols[m].ao0, ejobs[m].V, mixed && dense_refine ? n : k, ejobs[m].Z);
+            const bool keep_all = mixed && dense_refine && it >= dense_full;  // the next iteration refines
+            dense_vectors(ejobs[m].A, n, mtau.p + mols[m].ao0, ejobs[m].V, keep_all ? n : k, ejobs[m].Z);
           }
           c.B = ejobs[m].Z;
           c.n = k;
@@ -1420,7 +1494,10 @@ struct Engine {
     HIP_OK(hipMemcpyAsync(hst.data(), status.p, sizeof(int) * NM, hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
     hok.assign(NM, 0);
-    std::vector<GemmJob> gW;
+    std::vector<GemmJob> gW, gF, gW2;
+    std::vector<int> hpfp(NM, 0);
+    bool anypfp = false;
+    int naomax = 0;
     std::vector<int> hk(NM);
     HIP_OK(hipMemcpyAsync(hk.data(), kocc.p, sizeof(int) * NM, hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
@@ -1429,9 +1506,28 @@ struct Engine {
       if (!hok[m]) continue;
       const Mol& M = mols[m];
       int n = M.nao;
+      if (m < (int)dense_pfp.size() && dense_pfp[m]) {
+        // refined occupied subspace (occupations 0 or 2): W = P F P / 2 with F rebuilt from the last
+        // input potentials (the refinement used the F pool as work space)
+        hpfp[m] = 1;
+        anypfp = true;
+        gF.push_back(GemmJob{n, n, n, 0, 0, F.p + M.mat0, P.p + M.mat0, T.p + M.mat0, nullptr, n, n, n, 1.0, 0.0});
+        gW2.push_back(GemmJob{n, n, n, 0, 0, P.p + M.mat0, T.p + M.mat0, Wm.p + M.mat0, nullptr, n, n, n, 0.5, 0.0});
+        naomax = std::max(naomax, n);
+        continue;
+      }
       // dense molecules hold only the first kocc orbitals (the others have occupation <= 1e-16)
       int k = dense(n) ? hk[m] : n;
       gW.push_back(GemmJob{n, n, k, 0, 1, C.p + M.mat0, C.p + M.mat0, Wm.p + M.mat0, fwork.p + M.ao0, n, n, n, 1.0, 0.0});
+    }
+    if (anypfp) {
+      DBuf<int> pfd;
+      pfd.upload(hpfp, st);
+      LAUNCH(k_fock, dim3(blocks((long long)naomax * naomax, 256), NM), dim3(256), 0, b, (const int*)pfd.p, pool.p,
+             stride, vsh.p, vat.p, vdp.p, vqp.p, F.p);
+      gemm(gF);
+      gemm(gW2);
+      HIP_OK(hipStreamSynchronize(st));
     }
     DBuf<int> okd;
     okd.upload(hok, st);
@@ -1478,7 +1574,7 @@ struct Engine {
     auto er = get(erep, NA), e3h = get(e3, NA), eelh = get(eel, NA), et = get(eterm, 8 * (size_t)NA);
     auto qa = get(qat_o, NA), qs = get(qsh_o, NS), d = get(dp_o, 3 * (size_t)NA), q = get(qp_o, 6 * (size_t)NA);
     auto e = get(w, NO);
-    auto g = o.want_gradient ? get(grad, 3 * (size_t)NA) : std::vector<double>();
+    auto g = grad_done ? get(grad, 3 * (size_t)NA) : std::vector<double>();
     std::vector<int> hst(NM), hef(NM);
     HIP_OK(hipMemcpyAsync(hst.data(), status.p, sizeof(int) * NM, hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
@@ -1526,7 +1622,8 @@ struct Engine {
       r.dp.assign(d.begin() + 3 * M.at0, d.begin() + 3 * (M.at0 + M.nat));
       r.qp.assign(q.begin() + 6 * M.at0, q.begin() + 6 * (M.at0 + M.nat));
       r.emo.assign(e.begin() + M.ao0, e.begin() + M.ao0 + M.nao);
-      if (o.want_gradient && r.status == 0) r.grad.assign(g.begin() + 3 * M.at0, g.begin() + 3 * (M.at0 + M.nat));
+      std::sort(r.emo.begin(), r.emo.end());
+      if (grad_done && r.status == 0) r.grad.assign(g.begin() + 3 * M.at0, g.begin() + 3 * (M.at0 + M.nat));
     }
   }
 };
@@ -1656,6 +1753,20 @@ int axtbhip_batch_run(axtbhip_context* ctx) {
   if (!ctx) return AXTB_ERR_INPUT;
   try {
     ctx->eng->compute();
+    return 0;
+  } catch (axtb::Fail& f) {
+    ctx->err = f.msg;
+    return f.code;
+  } catch (std::exception& e) {
+    ctx->err = e.what();
+    return AXTB_ERR_DEVICE;
+  }
+}
+
+int axtbhip_batch_gradient(axtbhip_context* ctx) {
+  if (!ctx) return AXTB_ERR_INPUT;
+  try {
+    ctx->eng->gradient_of_state();
     return 0;
   } catch (axtb::Fail& f) {
     ctx->err = f.msg;

@@ -19,9 +19,18 @@ struct GemmJob {
   const double* A2 = nullptr;
   const double* B2 = nullptr;
   int k2 = 0;
+  // 128-tile kernel only: compute only the tiles that hold elements of the lower triangle (row >=
+  // column) of C
+  int lower = 0;
 };
 
 typedef double d4v __attribute__((ext_vector_type(4)));
+
+// Load through the global address space: the GEMM operands' pointers come from job structs in
+// memory and would otherwise be generic (flat) loads, which the LDS waits of the MFMA loop also
+// wait for.
+typedef const double __attribute__((address_space(1)))* gdp;
+__device__ __forceinline__ double gload(const double* p) { return *(gdp)p; }
 
 constexpr int GT = 64;   // tile edge
 constexpr int GK = 16;   // k step
@@ -32,57 +41,96 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmJob* jobs) {
   const GemmJob J = jobs[blockIdx.z];
   const int row0 = blockIdx.y * GT, col0 = blockIdx.x * GT;
   if (row0 >= J.m || col0 >= J.n) return;
-  __shared__ double As[GK][GT + 1];
-  __shared__ double Bs[GK][GT + 1];
+  __shared__ double As[2][GK][GT + 1];
+  __shared__ double Bs[2][GK][GT + 1];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wr = (w & 1) * 32, wc = (w >> 1) * 32;
   d4v acc[2][2];
   for (int i = 0; i < 2; i++)
     for (int j = 0; j < 2; j++) acc[i][j] = (d4v){0.0, 0.0, 0.0, 0.0};
-  for (int k0 = 0; k0 < J.k; k0 += GK) {
-    // stage op(A)[row0:+64, k0:+16] and op(B)[k0:+16, col0:+64] (zero padded)
-    for (int e = tid; e < GT * GK; e += 256) {
-      int r, kk;
-      if (J.ta == 0) {
-        r = e % GT;
-        kk = e / GT;
-      } else {
-        kk = e % GK;
-        r = e / GK;
-      }
-      int gr = row0 + r, gk = k0 + kk;
-      double v = 0.0;
-      if (gr < J.m && gk < J.k) {
-        v = J.ta == 0 ? J.A[gr + gk * J.lda] : J.A[gk + (long long)gr * J.lda];
-        if (J.dk) v *= J.dk[gk];
-      }
-      As[kk][r] = v;
+  // element q (of 4) of this thread's share of a k-step of op(A) (row r, k) and op(B) (col c, k);
+  // loads read clamped in-bounds addresses, masks and the inner scaling are applied at the store
+  // to LDS (the next step's loads are in flight during this step's products)
+  auto a_rk = [&](int q, int& r, int& k) {
+    const int e = tid + 256 * q;
+    if (J.ta == 0) {
+      r = e % GT;
+      k = e / GT;
+    } else {
+      k = e % GK;
+      r = e / GK;
     }
-    for (int e = tid; e < GT * GK; e += 256) {
-      int c, kk;
-      if (J.tb == 0) {
-        kk = e % GK;
-        c = e / GK;
-      } else {
-        c = e % GT;
-        kk = e / GT;
-      }
-      int gc = col0 + c, gk = k0 + kk;
-      double v = 0.0;
-      if (gc < J.n && gk < J.k) v = J.tb == 0 ? J.B[gk + (long long)gc * J.ldb] : J.B[gc + (long long)gk * J.ldb];
-      Bs[kk][c] = v;
+  };
+  auto b_ck = [&](int q, int& c, int& k) {
+    const int e = tid + 256 * q;
+    if (J.tb == 0) {
+      k = e % GK;
+      c = e / GK;
+    } else {
+      c = e % GT;
+      k = e / GT;
     }
-    __syncthreads();
+  };
+  double ra[4], rb[4], rd[4];
+  unsigned ma = 0u, mb = 0u;
+  auto load = [&](int k0) {
+    ma = mb = 0u;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      int r, k;
+      a_rk(q, r, k);
+      const int gr = row0 + r, gk = k0 + k;
+      const int grc = min(gr, J.m - 1), gkc = min(gk, J.k - 1);
+      ra[q] = gload(J.ta == 0 ? J.A + grc + (long long)gkc * J.lda : J.A + gkc + (long long)grc * J.lda);
+      ma |= (gr < J.m && gk < J.k) ? 1u << q : 0u;
+      int c;
+      b_ck(q, c, k);
+      const int gc = col0 + c, gkb = k0 + k;
+      const int gcc = min(gc, J.n - 1), gkbc = min(gkb, J.k - 1);
+      rb[q] = gload(J.tb == 0 ? J.B + gkbc + (long long)gcc * J.ldb : J.B + gcc + (long long)gkbc * J.ldb);
+      mb |= (gc < J.n && gkb < J.k) ? 1u << q : 0u;
+    }
+    if (J.dk) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        int r, k;
+        a_rk(q, r, k);
+        rd[q] = gload(J.dk + min(k0 + k, J.k - 1));
+      }
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      int r, k;
+      a_rk(q, r, k);
+      const double av = J.dk ? ra[q] * rd[q] : ra[q];
+      As[buf][k][r] = (ma >> q) & 1u ? av : 0.0;
+      int c;
+      b_ck(q, c, k);
+      Bs[buf][k][c] = (mb >> q) & 1u ? rb[q] : 0.0;
+    }
+  };
+  const int nk = (J.k + GK - 1) / GK;
+  if (nk > 0) {
+    load(0);
+    store(0);
+  }
+  __syncthreads();
+  for (int kt = 0; kt < nk; kt++) {
+    const int buf = kt & 1;
+    if (kt + 1 < nk) load((kt + 1) * GK);
 #pragma unroll
     for (int kk = 0; kk < GK; kk += 4) {
       const int kl = kk + (lane >> 4), li = lane & 15;
-      double a0 = As[kl][wr + li], a1 = As[kl][wr + 16 + li];
-      double b0 = Bs[kl][wc + li], b1 = Bs[kl][wc + 16 + li];
+      const double a0 = As[buf][kl][wr + li], a1 = As[buf][kl][wr + 16 + li];
+      const double b0 = Bs[buf][kl][wc + li], b1 = Bs[buf][kl][wc + 16 + li];
       acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
       acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
       acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
       acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
     }
+    if (kt + 1 < nk) store(buf ^ 1);
     __syncthreads();
   }
   // lane l, element r of a 16 x 16 block holds row l / 16 + 4 r, column l % 16 (operands: lane l
@@ -113,6 +161,7 @@ __global__ __launch_bounds__(256) void k_gemm128(const GemmJob* jobs) {
   const GemmJob J = jobs[blockIdx.z];
   const int row0 = blockIdx.y * BT, col0 = blockIdx.x * BT;
   if (row0 >= J.m || col0 >= J.n) return;
+  if (J.lower && col0 > row0 + BT - 1) return;
   __shared__ double As[2][BK][BP];
   __shared__ double Bs[2][BK][BP];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -137,39 +186,43 @@ __global__ __launch_bounds__(256) void k_gemm128(const GemmJob* jobs) {
       c = (tid >> 4) + 16 * q;
     }
   };
-  // inner index gk < k: first term; k <= gk < k + k2: second term at gk - k
+  // inner index gk < k: first term; k <= gk < k + k2: second term at gk - k. Branch-free: every
+  // load reads a clamped (in-bounds) address; the masks ma / mb (bit q: element q inside op(A),
+  // op(B)) and the inner scaling dk of the first term are applied when the step is stored to LDS,
+  // so nothing waits for the loads before the products of the current step.
+  const int kt = J.k + (TWO ? J.k2 : 0);
+  double rd[8];
+  unsigned ma = 0u, mb = 0u;
   auto load = [&](int k0) {
+    ma = mb = 0u;
 #pragma unroll
     for (int q = 0; q < 8; q++) {
       int r, k;
       a_rk(q, r, k);
-      const int gr = row0 + r;
-      int gk = k0 + k;
-      double v = 0.0;
-      if (gr < J.m) {
-        if (gk < J.k) {
-          v = TA == 0 ? J.A[gr + (long long)gk * J.lda] : J.A[gk + (long long)gr * J.lda];
-          if (J.dk) v *= J.dk[gk];
-        } else if (TWO && gk - J.k < J.k2) {
-          gk -= J.k;
-          v = TA == 0 ? J.A2[gr + (long long)gk * J.lda] : J.A2[gk + (long long)gr * J.lda];
-        }
-      }
-      ra[q] = v;
+      const int gr = row0 + r, gk = k0 + k;
+      const int grc = min(gr, J.m - 1);
+      const bool first = !TWO || gk < J.k || J.k2 <= 0;
+      const double* pa = first ? J.A : J.A2;
+      const int gkc = first ? min(gk, J.k - 1) : min(gk - J.k, J.k2 - 1);
+      ra[q] = gload(TA == 0 ? pa + grc + (long long)gkc * J.lda : pa + gkc + (long long)grc * J.lda);
+      ma |= (gr < J.m && gk < kt) ? 1u << q : 0u;
       int c;
       b_ck(q, c, k);
-      const int gc = col0 + c;
-      int gkb = k0 + k;
-      double u = 0.0;
-      if (gc < J.n) {
-        if (gkb < J.k) {
-          u = TB == 0 ? J.B[gkb + (long long)gc * J.ldb] : J.B[gc + (long long)gkb * J.ldb];
-        } else if (TWO && gkb - J.k < J.k2) {
-          gkb -= J.k;
-          u = TB == 0 ? J.B2[gkb + (long long)gc * J.ldb] : J.B2[gc + (long long)gkb * J.ldb];
-        }
+      const int gc = col0 + c, gkb = k0 + k;
+      const int gcc = min(gc, J.n - 1);
+      const bool firstb = !TWO || gkb < J.k || J.k2 <= 0;
+      const double* pb = firstb ? J.B : J.B2;
+      const int gkbc = firstb ? min(gkb, J.k - 1) : min(gkb - J.k, J.k2 - 1);
+      rb[q] = gload(TB == 0 ? pb + gkbc + (long long)gcc * J.ldb : pb + gcc + (long long)gkbc * J.ldb);
+      mb |= (gc < J.n && gkb < kt) ? 1u << q : 0u;
+    }
+    if (!TWO && J.dk) {  // two-term products carry no inner scaling
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        int r, k;
+        a_rk(q, r, k);
+        rd[q] = gload(J.dk + min(k0 + k, J.k - 1));
       }
-      rb[q] = u;
     }
   };
   auto store = [&](int buf) {
@@ -177,10 +230,11 @@ __global__ __launch_bounds__(256) void k_gemm128(const GemmJob* jobs) {
     for (int q = 0; q < 8; q++) {
       int r, k;
       a_rk(q, r, k);
-      As[buf][k][r] = ra[q];
+      const double av = (!TWO && J.dk) ? ra[q] * rd[q] : ra[q];
+      As[buf][k][r] = (ma >> q) & 1u ? av : 0.0;
       int c;
       b_ck(q, c, k);
-      Bs[buf][k][c] = rb[q];
+      Bs[buf][k][c] = (mb >> q) & 1u ? rb[q] : 0.0;
     }
   };
   d4v acc[4][4];
@@ -188,9 +242,11 @@ __global__ __launch_bounds__(256) void k_gemm128(const GemmJob* jobs) {
   for (int i = 0; i < 4; i++)
 #pragma unroll
     for (int j = 0; j < 4; j++) acc[i][j] = (d4v){0.0, 0.0, 0.0, 0.0};
-  const int nk = (J.k + (TWO ? J.k2 : 0) + BK - 1) / BK;
-  load(0);
-  store(0);
+  const int nk = (kt + BK - 1) / BK;
+  if (nk > 0) {
+    load(0);
+    store(0);
+  }
   __syncthreads();
   for (int kt = 0; kt < nk; kt++) {
     const int buf = kt & 1;
@@ -532,16 +588,19 @@ __global__ __launch_bounds__(256) void k_oa_step(const EigJob* jobs, double* ema
   } else {
     delta = fmax(delta, delta_keep[slot[blockIdx.x]]);
   }
+  // emax: the largest correction (pairs left unrotated as clusters are not counted: a coupling
+  // there changes neither P nor, beyond its own size, W)
   double emx = 0.0;
   for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
     const int i = e % n, j = e / n;
+    const double g = G[e];
     double v;
     if (i == j) {
-      v = 0.5 * (1.0 - G[e]);
+      v = 0.5 * (1.0 - g);
     } else {
       const double gap = lam[j] - lam[i];
-      const double num = S[e] - lam[j] * G[e];
-      v = (fabs(gap) > delta && fabs(num) < 0.05 * fabs(gap)) ? num / gap : -0.5 * G[e];
+      const double num = S[e] - lam[j] * g;
+      v = (fabs(gap) > delta && fabs(num) < 0.05 * fabs(gap)) ? num / gap : -0.5 * g;
     }
     emx = fmax(emx, fabs(v));
     J.Z[e] = (i == j ? 1.0 : 0.0) + v;
@@ -596,8 +655,13 @@ __global__ __launch_bounds__(256) void k_oa_big_norms(const EigJob J, double* pa
   }
 }
 
+// cls (nullable): per column, 0 = fully occupied, 1 = empty, 2 = fractional occupation. With it,
+// an occupied-empty pair is always rotated (its gap is the occupied-virtual gap; a first-order
+// mixing >= 0.05 marks the step as not convergent, emax >= 1), pairs inside the occupied or the
+// empty block are only orthogonalised (rotations there leave P = C f C^T unchanged when f is
+// constant on the block), and pairs with a fractional column follow the delta rule.
 __global__ __launch_bounds__(256) void k_oa_big_e(const EigJob J, const double* part, int nparts, double* delta_keep,
-                                                  int first, double* emax_part) {
+                                                  int first, double* emax_part, const int* cls) {
   __shared__ double red[16];
   __shared__ double s_delta;
   const int n = J.n;
@@ -626,7 +690,22 @@ __global__ __launch_bounds__(256) void k_oa_big_e(const EigJob J, const double* 
     } else {
       const double gap = J.w[j] - J.w[i];
       const double num = J.A[e] - J.w[j] * g;
-      v = (fabs(gap) > delta && fabs(num) < 0.05 * fabs(gap)) ? num / gap : -0.5 * g;
+      const int ci = cls ? cls[i] : 2, cj = cls ? cls[j] : 2;
+      if (ci != 2 && cj != 2) {
+        if (ci == cj) {
+          v = -0.5 * g;
+        } else if (fabs(num) < 0.05 * fabs(gap)) {
+          v = num / gap;
+        } else {
+          v = -0.5 * g;
+          emx = fmax(emx, 1.0);
+        }
+      } else if (fabs(gap) > delta && fabs(num) < 0.05 * fabs(gap)) {
+        v = num / gap;
+      } else {
+        v = -0.5 * g;
+        emx = fmax(emx, fabs(num));
+      }
     }
     emx = fmax(emx, fabs(v));
     J.Z[e] = (i == j ? 1.0 : 0.0) + v;
@@ -640,6 +719,16 @@ __global__ __launch_bounds__(256) void k_oa_big_e(const EigJob J, const double* 
     for (int w = 0; w < (int)(blockDim.x >> 6); w++) m = fmax(m, red[w]);
     emax_part[blockIdx.x] = m;
   }
+}
+
+// Occupation class of each orbital column for k_oa_big_e: 0 if f > 2 - 1e-12, 1 if f < 1e-12,
+// else 2; *nfrac counts the class-2 columns.
+__global__ void k_occ_class(const double* f, int n, int* cls, int* nfrac) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c = f[i] > 2.0 - 1e-12 ? 0 : (f[i] < 1e-12 ? 1 : 2);
+  cls[i] = c;
+  if (c == 2) atomicAdd(nfrac, 1);
 }
 
 // S-orthonormal start basis B0 = L^-T from the Cholesky factor S = L L^T (the reduction of

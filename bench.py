@@ -7,11 +7,12 @@ orientations) or an all-trans n-alkane C3..C20, with 0.05 A Gaussian coordinate 
 default conventions, SCC energy convergence 1e-8 Eh (accuracy 1e-2), 300 K, D4, EEQ start.
 A step is one energy + gradient pass over the whole batch (setup, SCC with per-molecule
 convergence masks, gradient), inputs resident on the device. Other workloads: --workload
-water1000 / water200 / c100h202 / water20 / c30h62 (one molecule per step).
+water1000 / water200 / c100h202 / water20 / c30h62 (one molecule per step), conformers2048
+(2048 C30H62 conformers, 0.1 A noise, one batch per step).
 
 Precision (default both): the workload is run in plain FP64 and in mixed precision (FP32 Jacobi
-eigensolves and FP64 Ogita-Aishima refinement; for large molecules refinement of the previous
-SCC iteration's eigenvectors), each prepared once and timed separately. `value` is the mixed
+eigensolves and FP64 Ogita-Aishima refinement for the batched eigenproblems, n <= 200; larger
+molecules on the FP64 dense solver in both modes), each prepared once and timed separately. `value` is the mixed
 E+grad/s; `config` holds the FP64 E+grad/s, the speedup and the largest differences of the
 mixed results from the FP64 results over all molecules, and both against the CPU oracle on the
 first molecules.
@@ -44,6 +45,8 @@ def workload(name, seed=0):
         return S.batch(4096, seed=seed)
     if name.startswith("batch"):
         return S.batch(int(name[5:]), seed=seed)
+    if name.startswith("conformers"):  # C30H62 conformers, 0.1 A noise (conformers2048)
+        return S.conformers(int(name[10:]), seed=seed)
     return [S.config(name, seed=seed)]
 
 
@@ -168,7 +171,7 @@ def main():
     counters = {k: v for k, v in prof.items() if not k.startswith("k_")}
     prof = {k: v for k, v in prof.items() if k.startswith("k_")}
     sweeps = {k: round(v[2] / v[0], 3) for k, v in counters.items() if k.startswith("sweeps_") and v[0]}
-    refinement = {k: round(v[2] / v[0], 3) if k == "refine_steps" else v[0] for k, v in counters.items()
+    refinement = {k: round(v[2] / v[0], 3) if k.startswith("refine_steps") else v[0] for k, v in counters.items()
                   if k.startswith("refine") and v[0]}
     # dominant kernel and its roofline from the live HIP-event timings
     total_gpu_ms = sum(v[1] for v in prof.values())

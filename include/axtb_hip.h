@@ -90,6 +90,13 @@ int axtbhip_gfn2_engrad_batch(axtbhip_context* ctx, const axtb_mol* mols, int nm
 int axtbhip_batch_prepare(axtbhip_context* ctx, const axtb_mol* mols, int nmol, const axtb_opts* opts);
 int axtbhip_batch_run(axtbhip_context* ctx);
 int axtbhip_batch_results(axtbhip_context* ctx, const axtb_mol* mols, int nmol, axtb_result* outs);
+/* The adjoint step on the resident batch: after a batch_run with want_gradient = 0, the analytic
+ * gradient of the converged state (W and P contracted with dS/dR and dH0/dR recomputed per
+ * shell-pair block, plus the ES2/ES3/AES/D4/repulsion and CN-chain terms), equal to a run with
+ * want_gradient = 1; batch_results then returns it. Replaces XtbModel::gradient_functional
+ * (crates/axtb/src/xtb.rs:392-421) applied to the state of the last single point. Returns
+ * AXTB_ERR_INPUT before any batch_run. */
+int axtbhip_batch_gradient(axtbhip_context* ctx);
 
 /* Per-kernel timing with HIP events on the context's stream (the counterpart of the reference's
  * phase timers, crates/base/src/timing.rs:11-54): report lines "kernel launches total_ms work",
