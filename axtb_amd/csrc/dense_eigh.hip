@@ -250,6 +250,82 @@ __global__ void k_zero_u32(unsigned* p, int n) {
   if (k < n) p[k] = 0u;
 }
 
+// ------------------------------------------------------------------ blocked Cholesky start basis
+// Diagonal block of the right-looking blocked Cholesky factorisation S = L L^T (the reduction of
+// GenEigSolver, base/src/linalg/eigen.rs:72-113): the b x b block at (k0, k0) of W (lower
+// triangle, already updated by the previous panels) is factored in LDS and written back, and its
+// inverse L11^-1 (lower, zero above) is written to M at (k0, k0). fail = 1 on a pivot <= 0.
+__global__ __launch_bounds__(256) void k_chol_diag(double* W, int n, int k0, int b, double* M, int* fail) {
+  __shared__ double a[64][65];  // a[c][r]
+  __shared__ double mi[64][65];
+  __shared__ int bad;
+  const int tid = threadIdx.x;
+  const long long ln = n;
+  if (tid == 0) bad = 0;
+  for (int e = tid; e < 64 * 64; e += 256) {
+    const int r = e & 63, c = e >> 6;
+    a[c][r] = (r < b && c < b && r >= c) ? W[(k0 + r) + (k0 + c) * ln] : 0.0;
+  }
+  __syncthreads();
+  for (int j = 0; j < b; j++) {
+    if (tid == 0) {
+      const double d = a[j][j];
+      if (!(d > 0.0)) bad = 1;
+      a[j][j] = d > 0.0 ? sqrt(d) : 1.0;
+    }
+    __syncthreads();
+    const double dj = a[j][j];
+    for (int r = j + 1 + tid; r < b; r += 256) a[j][r] /= dj;
+    __syncthreads();
+    // trailing lower block: a(r, c) -= a(r, j) a(c, j), j < c <= r
+    const int m = b - j - 1;
+    for (int e = tid; e < m * m; e += 256) {
+      const int r = j + 1 + e % m, c = j + 1 + e / m;
+      if (r >= c) a[c][r] -= a[j][r] * a[j][c];
+    }
+    __syncthreads();
+  }
+  // inverse by forward substitution, one column per thread: L x = e_c
+  if (tid < b) {
+    const int c = tid;
+    for (int r = 0; r < b; r++) {
+      double v = 0.0;
+      if (r == c) v = 1.0;
+      if (r >= c) {
+        for (int k = c; k < r; k++) v -= a[k][r] * mi[c][k];
+        v /= a[r][r];
+      }
+      mi[c][r] = v;
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < b * b; e += 256) {
+    const int r = e % b, c = e / b;
+    if (r >= c) W[(k0 + r) + (k0 + c) * ln] = a[c][r];
+    M[(k0 + r) + (k0 + c) * ln] = r >= c ? mi[c][r] : 0.0;
+  }
+  if (tid == 0 && bad) *fail = 1;
+}
+
+// X = M^T for lower triangular M (X upper triangular, zero below), by 32 x 32 tiles through LDS.
+__global__ __launch_bounds__(256) void k_lower_to_upper_t(const double* M, double* X, int n) {
+  __shared__ double tl[32][33];
+  const int R0 = blockIdx.y * 32, C0 = blockIdx.x * 32;  // tile of X
+  const long long ln = n;
+  // X(R0 + i, C0 + j) = M(C0 + j, R0 + i): read M's tile (rows C0.., cols R0..) coalesced
+  for (int e = threadIdx.x; e < 32 * 32; e += 256) {
+    const int r = e & 31, c = e >> 5;  // M(C0 + r, R0 + c)
+    const int gr = C0 + r, gc = R0 + c;
+    tl[c][r] = (gr < n && gc < n && gr >= gc) ? M[gr + gc * ln] : 0.0;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 32 * 32; e += 256) {
+    const int i = e & 31, j = e >> 5;  // X(R0 + i, C0 + j) = M(C0 + j, R0 + i) = tl[i][j]
+    const int gr = R0 + i, gc = C0 + j;
+    if (gr < n && gc < n) X[gr + gc * ln] = tl[i][j];
+  }
+}
+
 // Upper triangle of the n x n matrix A (leading dimension ld) from its lower triangle, by 32 x 32
 // tiles through LDS (workgroup t takes lower tile (rb, cb), t = rb (rb + 1) / 2 + cb).
 __global__ __launch_bounds__(256) void k_mirror_lower(double* A, int n, long long ld) {

@@ -231,6 +231,7 @@ struct Engine {
     if (const char* v = std::getenv("AXTB_HIP_OA_STEPS")) oa_steps = std::min(4, std::max(1, std::atoi(v)));
     if (const char* v = std::getenv("AXTB_HIP_DENSE_FULL")) dense_full = std::max(1, std::atoi(v));
     if (const char* v = std::getenv("AXTB_HIP_DENSE_REFINE")) dense_refine = std::atoi(v) != 0;
+    if (const char* v = std::getenv("AXTB_HIP_F32_TOL")) f32_tol = (float)std::min(1e3, std::max(1.0, std::atof(v)));
     if (const char* v = std::getenv("AXTB_HIP_F32_SWEEPS")) f32_sweeps = std::min(60, std::max(1, std::atoi(v)));
     if (const char* v = std::getenv("AXTB_HIP_DENSE_ACCEPT")) dense_accept = std::min(1e-3, std::max(1e-12, std::atof(v)));
     if (const char* v = std::getenv("AXTB_HIP_GEMM128_BATCH")) gemm128_batch = std::atoi(v) != 0;
@@ -342,7 +343,8 @@ struct Engine {
   // (mode 0). Within a launch the largest matrices go first (shorter tail); the mode 1/0 launches
   // run on a second stream beside the mode 2 launch, both joined back into the engine stream.
   // fp32: the rotations in FP32 (A and V in LDS; Z and w returned in FP64 for the refinement).
-  void eigh(std::vector<EigJob>& jobs, bool fp32 = false, int max_sweeps = 60) {
+  // tol_scale multiplies the rotation thresholds (the mixed path's FP32 solves, refined afterwards)
+  void eigh(std::vector<EigJob>& jobs, bool fp32 = false, int max_sweeps = 60, float tol_scale = 1.0f) {
     if (jobs.empty()) return;
     if (!max_lds) {
       HIP_OK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
@@ -386,7 +388,7 @@ struct Engine {
       off += js.size();
       auto go = [&](const void* fn, auto kern) {
         HIP_OK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-        hipLaunchKernelGGL(kern, dim3(js.size()), dim3(512), shm, s, buf.p, max_sweeps, stat);
+        hipLaunchKernelGGL(kern, dim3(js.size()), dim3(512), shm, s, buf.p, max_sweeps, stat, tol_scale);
       };
       {
         const char* nm = fp32 ? (mode == 2 ? "k_jacobi2_f32" : "k_jacobi1_f32")
@@ -1108,8 +1110,9 @@ struct Engine {
                eeqy.p, qstart.p, eeqfail.p);
       }
     }
-    // S-orthonormal start basis X: L^-T from a Cholesky factor in LDS where S fits, else the
-    // Loewdin orthogonaliser U s^-1/2 U^T from the eigenpairs of S
+    // S-orthonormal start basis X: L^-T from a Cholesky factor in LDS where S fits or blocked on
+    // the matrix cores for dense-solver molecules, else the Loewdin orthogonaliser U s^-1/2 U^T
+    // from the eigenpairs of S
     {
       std::vector<EigJob> jobs, cj;
       std::vector<int> lw;
@@ -1122,13 +1125,14 @@ struct Engine {
           nch = std::max(nch, n);
           continue;
         }
+        if (dense(n)) {
+          if (!dense_chol_basis(pool.p + o0, n, Am.p + o0, Vw.p + o0, T.p + o0, X.p + o0))
+            throw Fail{AXTB_ERR_NUMERICAL, "overlap matrix is not positive definite"};
+          continue;
+        }
         lw.push_back(m);
         HIP_OK(hipMemcpyAsync(Am.p + o0, pool.p + o0, sizeof(double) * n * n, hipMemcpyDeviceToDevice, st));
-        EigJob j{n, Am.p + o0, Vw.p + o0, w.p + mols[m].ao0, Z.p + o0};
-        if (dense(j.n))
-          dense_eigh(j.A, j.n, j.w, j.V, j.Z);
-        else
-          jobs.push_back(j);
+        jobs.push_back(EigJob{n, Am.p + o0, Vw.p + o0, w.p + mols[m].ao0, Z.p + o0});
       }
       if (!cj.empty()) {
         cjobs.upload(cj, st);
@@ -1164,6 +1168,67 @@ struct Engine {
         gemm(gjobs);
       }
     }
+  }
+  // X = L^-T for the n x n overlap S (n > 200): blocked right-looking Cholesky S = L L^T in W
+  // (panels of 64: k_chol_diag factors and inverts the diagonal block, L21 = A21 L11^-T in place,
+  // A22 -= L21 L21^T on its lower tiles), then M = L^-1 by recursive halving over the 64-row blocks
+  // (M21 = -M22 (L21 M11), all nodes of a level in one batch; M, zero above the diagonal, in Mb),
+  // X = M^T. Tt is n x n work. Returns false if a pivot is not positive.
+  bool dense_chol_basis(const double* S, int n, double* W, double* Mb, double* Tt, double* X) {
+    HIP_OK(hipMemcpyAsync(W, S, sizeof(double) * n * (size_t)n, hipMemcpyDeviceToDevice, st));
+    fill(Mb, (long long)n * n);
+    cfail.ensure(1);
+    LAUNCH(k_zero_u32, dim3(1), dim3(64), 0, (unsigned*)cfail.p, 1);
+    const long long ln = n;
+    for (int k0 = 0; k0 < n; k0 += 64) {
+      const int b = std::min(64, n - k0), rest = n - k0 - b;
+      LAUNCH(k_chol_diag, dim3(1), dim3(256), 0, W, n, k0, b, Mb, cfail.p);
+      if (rest == 0) break;
+      double* A21 = W + (k0 + b) + k0 * ln;
+      // L21 = A21 (L11^-1)^T, in place: each output tile reads its own rows, all b columns fit one tile
+      std::vector<GemmJob> gp{GemmJob{rest, b, b, 0, 1, A21, Mb + k0 + k0 * ln, A21, nullptr, ln, ln, ln, 1.0, 0.0}};
+      gemm(gp);
+      GemmJob t{rest, rest, b, 0, 1, A21, A21, W + (k0 + b) + (k0 + b) * ln, nullptr, ln, ln, ln, -1.0, 1.0};
+      t.lower = 1;
+      std::vector<GemmJob> gt{t};
+      gemm(gt);
+    }
+    // the inverse: nodes [s, e) of 64-row blocks, split at the middle; leaves (one block) are done
+    const int nb = (n + 63) / 64;
+    std::vector<std::vector<std::pair<int, int>>> levels;
+    std::vector<std::pair<int, int>> cur{{0, nb}};
+    while (!cur.empty()) {
+      std::vector<std::pair<int, int>> inner, next;
+      for (auto& nd : cur)
+        if (nd.second - nd.first > 1) {
+          inner.push_back(nd);
+          const int mid = (nd.first + nd.second) / 2;
+          next.push_back({nd.first, mid});
+          next.push_back({mid, nd.second});
+        }
+      if (!inner.empty()) levels.push_back(inner);
+      cur.swap(next);
+    }
+    for (int lv = (int)levels.size() - 1; lv >= 0; lv--) {
+      std::vector<GemmJob> j1, j2;
+      for (auto& nd : levels[lv]) {
+        const int mid = (nd.first + nd.second) / 2;
+        const int r1 = nd.first * 64, r2 = mid * 64, r3 = std::min(nd.second * 64, n);
+        const int h1 = r2 - r1, h2 = r3 - r2;
+        // T21 = L21 M11; M21 = -M22 T21
+        j1.push_back(GemmJob{h2, h1, h1, 0, 0, W + r2 + r1 * ln, Mb + r1 + r1 * ln, Tt + r2 + r1 * ln, nullptr, ln, ln,
+                             ln, 1.0, 0.0});
+        j2.push_back(GemmJob{h2, h1, h2, 0, 0, Mb + r2 + r2 * ln, Tt + r2 + r1 * ln, Mb + r2 + r1 * ln, nullptr, ln, ln,
+                             ln, -1.0, 0.0});
+      }
+      gemm(j1);
+      gemm(j2);
+    }
+    LAUNCH(k_lower_to_upper_t, dim3(blocks(n, 32), blocks(n, 32)), dim3(256), 0, (const double*)Mb, X, n);
+    int hf = 0;
+    HIP_OK(hipMemcpyAsync(&hf, cfail.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    return hf == 0;
   }
   DBuf<int> mlu, mev, cfail;
   DBuf<EigJob> cjobs;
@@ -1290,7 +1355,7 @@ struct Engine {
   // occupied-empty pairs and the orthogonality terms) is <= dense_accept; the error it leaves is of
   // second order in that correction (AXTB_HIP_DENSE_ACCEPT)
   double dense_accept = 1e-5;
-  int dense_full = 3;  // mixed: SCC iterations solved in full before refinement (AXTB_HIP_DENSE_FULL)
+  int dense_full = 1;  // mixed: SCC iterations solved in full before refinement (AXTB_HIP_DENSE_FULL)
   // mixed: refine the previous iteration's vectors of dense-solver molecules instead of a full
   // solve (AXTB_HIP_DENSE_REFINE=0 turns it off). Only the occupied-empty pairs are resolved: P
   // depends only on the occupied subspace, and W is formed as P F P / 2 (exact when every
@@ -1301,6 +1366,7 @@ struct Engine {
   bool in_refine = false;      // the refinement's batches always take the 128-tile kernel (measured faster)
   int oa_steps = 2;  // refinement steps per SCC iteration (AXTB_HIP_OA_STEPS)
   int f32_sweeps = 60;  // mixed: FP32 Jacobi sweeps at most after the first iteration (AXTB_HIP_F32_SWEEPS)
+  float f32_tol = 1.0f;  // mixed: scale of the FP32 Jacobi rotation thresholds (AXTB_HIP_F32_TOL)
 
   // ------------------------------------------------------------------------------ SCC loop
   std::vector<int> hactive;
@@ -1389,7 +1455,7 @@ struct Engine {
         // mixed: Jacobi in FP32, then one FP64 Ogita-Aishima step U' = U (I + E) from
         // S = U^T A U and G = U^T U (U in Z, A in Am; S -> F, G -> Vw, I + E -> Wm, U' -> T)
         // after the first iteration A is nearly diagonal: f32_sweeps sweeps, the refinement does the rest
-        eigh(ea, true, it == 1 ? 60 : f32_sweeps);
+        eigh(ea, true, it == 1 ? 60 : f32_sweeps, f32_tol);
         std::vector<char> in_t;
         std::vector<EigJob> redo = refine(ea, in_t);
         eigh(redo);  // not converged by the refinement: FP64 Jacobi (vectors in Z)

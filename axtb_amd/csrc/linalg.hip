@@ -349,7 +349,7 @@ template <> struct JacobiTol<float> {
 };
 
 template <int MODE, typename T>
-__global__ __launch_bounds__(512) void k_jacobi(const EigJob* jobs, int max_sweeps, int* status) {
+__global__ __launch_bounds__(512) void k_jacobi(const EigJob* jobs, int max_sweeps, int* status, float tol_scale) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T* lds = (T*)lds_raw;
   __shared__ int rotated;
@@ -378,7 +378,7 @@ __global__ __launch_bounds__(512) void k_jacobi(const EigJob* jobs, int max_swee
   }
   __syncthreads();
   const T an = anorm;
-  constexpr T REL = JacobiTol<T>::REL, ABS = JacobiTol<T>::ABS;
+  const T REL = JacobiTol<T>::REL * (T)tol_scale, ABS = JacobiTol<T>::ABS * (T)tol_scale;
   const int nblk = np * (np + 1) / 2;
   // this thread's lower 2 x 2 blocks (k1 >= k2; block b = k1 (k1 + 1) / 2 + k2, b = tid + j nt),
   // the same in every round
@@ -531,11 +531,11 @@ __global__ __launch_bounds__(512) void k_jacobi(const EigJob* jobs, int max_swee
   if (tid == 0) status[blockIdx.x] = conv ? sweep : -sweep;
 }
 
-template __global__ void k_jacobi<0, double>(const EigJob*, int, int*);
-template __global__ void k_jacobi<1, double>(const EigJob*, int, int*);
-template __global__ void k_jacobi<2, double>(const EigJob*, int, int*);
-template __global__ void k_jacobi<1, float>(const EigJob*, int, int*);
-template __global__ void k_jacobi<2, float>(const EigJob*, int, int*);
+template __global__ void k_jacobi<0, double>(const EigJob*, int, int*, float);
+template __global__ void k_jacobi<1, double>(const EigJob*, int, int*, float);
+template __global__ void k_jacobi<2, double>(const EigJob*, int, int*, float);
+template __global__ void k_jacobi<1, float>(const EigJob*, int, int*, float);
+template __global__ void k_jacobi<2, float>(const EigJob*, int, int*, float);
 
 static constexpr auto k_jacobi0 = k_jacobi<0, double>;
 static constexpr auto k_jacobi1 = k_jacobi<1, double>;
