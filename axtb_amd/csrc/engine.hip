@@ -231,6 +231,8 @@ struct Engine {
     if (const char* v = std::getenv("AXTB_HIP_OA_STEPS")) oa_steps = std::min(4, std::max(1, std::atoi(v)));
     if (const char* v = std::getenv("AXTB_HIP_DENSE_FULL")) dense_full = std::max(1, std::atoi(v));
     if (const char* v = std::getenv("AXTB_HIP_DENSE_REFINE")) dense_refine = std::atoi(v) != 0;
+    if (const char* v = std::getenv("AXTB_HIP_GEMM_SMALL")) gemm_small = std::atoi(v) != 0;
+    if (const char* v = std::getenv("AXTB_HIP_EXACT_LEVELS")) exact_levels = std::atoi(v) != 0;
     if (const char* v = std::getenv("AXTB_HIP_F32_TOL")) f32_tol = (float)std::min(1e3, std::max(1.0, std::atof(v)));
     if (const char* v = std::getenv("AXTB_HIP_F32_SWEEPS")) f32_sweeps = std::min(60, std::max(1, std::atoi(v)));
     if (const char* v = std::getenv("AXTB_HIP_DENSE_ACCEPT")) dense_accept = std::min(1e-3, std::max(1e-12, std::atof(v)));
@@ -299,8 +301,24 @@ struct Engine {
         return;
       }
     }
+    // products with m, n <= 128: one workgroup per job, 16 x 16 granularity
+    if (gemm_small && same && !two && mm <= 128 && nn <= 128) {
+      gj.upload(jobs, st);
+      const int t = jobs[0].ta * 2 + jobs[0].tb;
+      const dim3 grid((unsigned)jobs.size());
+      if (t == 0)
+        LAUNCHW(flops, k_gemm_small_nn, grid, dim3(512), 0, gj.p);
+      else if (t == 1)
+        LAUNCHW(flops, k_gemm_small_nt, grid, dim3(512), 0, gj.p);
+      else if (t == 2)
+        LAUNCHW(flops, k_gemm_small_tn, grid, dim3(512), 0, gj.p);
+      else
+        LAUNCHW(flops, k_gemm_small_tt, grid, dim3(512), 0, gj.p);
+      HIP_OK(hipGetLastError());
+      return;
+    }
     // large products on 128 x 128 tiles (one transpose pair per launch; two-term products always),
-    // the batch of small per-molecule products on 64 x 64 tiles
+    // the other batches of per-molecule products on 64 x 64 tiles
     if (two || (same && (long long)mm * nn >= 256LL * 256 && jobs.size() <= 8) ||
         (same && (gemm128_batch || in_refine) && std::max(mm, nn) > 64)) {
       if (!same) throw DeviceError("gemm: mixed transposes in one launch");
@@ -1089,6 +1107,7 @@ struct Engine {
           nlu = std::max(nlu, n1);
           continue;
         }
+        if (dense(n1) && eeq_cholesky(m, heq0[m])) continue;
         ev.push_back(m);
         EigJob j{n1, eeqA.p + heq0[m], eeqV.p + heq0[m], eeqw.p + mols[m].at0 + m, eeqZ.p + heq0[m]};
         if (dense(n1))
@@ -1230,6 +1249,28 @@ struct Engine {
     HIP_OK(hipStreamSynchronize(st));
     return hf == 0;
   }
+  // Large EEQ systems (N + 1 > 200): the N x N block A of [A 1; 1^T 0] by the blocked Cholesky
+  // (A^-1 = X X^T, X = L^-T), two solves and the Schur complement of the constraint (k_eeq_schur).
+  // Work in the matrix pools at the molecule's offset (free during setup; N <= n_ao). Returns
+  // false, leaving nothing changed, if A is not positive definite (the eigensolver path follows).
+  bool eeq_cholesky(int m, long long e0) {
+    const int N = mols[m].nat, n1 = N + 1;
+    const long long o0 = mols[m].mat0;
+    HIP_OK(hipMemcpy2DAsync(T.p + o0, sizeof(double) * N, eeqA.p + e0, sizeof(double) * n1, sizeof(double) * N, N,
+                            hipMemcpyDeviceToDevice, st));
+    if (!dense_chol_basis(T.p + o0, N, Am.p + o0, Vw.p + o0, C.p + o0, X.p + o0)) return false;
+    Batch b = batch();
+    double* R = Z.p + o0;      // [r | 1], N x 2
+    double* Y = F.p + o0;      // X^T R
+    double* Xs = Wm.p + o0;    // X X^T R
+    LAUNCH(k_eeq_rhs, dim3(blocks(N, 256)), dim3(256), 0, b, m, (const double*)eeqr.p, R);
+    std::vector<GemmJob> g1{GemmJob{N, 2, N, 1, 0, X.p + o0, R, Y, nullptr, N, N, N, 1.0, 0.0}};
+    gemm(g1);
+    std::vector<GemmJob> g2{GemmJob{N, 2, N, 0, 0, X.p + o0, Y, Xs, nullptr, N, N, N, 1.0, 0.0}};
+    gemm(g2);
+    LAUNCH(k_eeq_schur, dim3(1), dim3(256), 0, b, m, (const double*)Xs, (const double*)eeqr.p, qstart.p, eeqfail.p);
+    return true;
+  }
   DBuf<int> mlu, mev, cfail;
   DBuf<EigJob> cjobs;
   DBuf<double> s12_, mtau, oa_emax, oa_delta;
@@ -1247,6 +1288,8 @@ struct Engine {
     std::vector<EigJob> cur = ea, redo;
     in_t.assign(ea.size(), 0);
     if (ea.empty()) return redo;
+    for (const EigJob& j : ea)
+      if (j.n > 200) throw DeviceError("refine: batched refinement called for n > 200");
     std::vector<int> idx(ea.size());
     for (size_t q = 0; q < ea.size(); q++) idx[q] = (int)q;
     oa_delta.ensure(ea.size());
@@ -1354,7 +1397,7 @@ struct Engine {
   // mixed, dense: a refinement is accepted once its step's largest correction max |e_ij| (over the
   // occupied-empty pairs and the orthogonality terms) is <= dense_accept; the error it leaves is of
   // second order in that correction (AXTB_HIP_DENSE_ACCEPT)
-  double dense_accept = 1e-5;
+  double dense_accept = 1e-4;
   int dense_full = 1;  // mixed: SCC iterations solved in full before refinement (AXTB_HIP_DENSE_FULL)
   // mixed: refine the previous iteration's vectors of dense-solver molecules instead of a full
   // solve (AXTB_HIP_DENSE_REFINE=0 turns it off). Only the occupied-empty pairs are resolved: P
@@ -1363,6 +1406,7 @@ struct Engine {
   // quotients of the refined vectors
   bool dense_refine = true;
   bool gemm128_batch = false;  // batches of per-molecule products on the 128-tile kernel (AXTB_HIP_GEMM128_BATCH)
+  bool gemm_small = true;      // products with m, n <= 128 on the one-workgroup-per-job kernel (AXTB_HIP_GEMM_SMALL)
   bool in_refine = false;      // the refinement's batches always take the 128-tile kernel (measured faster)
   int oa_steps = 2;  // refinement steps per SCC iteration (AXTB_HIP_OA_STEPS)
   int f32_sweeps = 60;  // mixed: FP32 Jacobi sweeps at most after the first iteration (AXTB_HIP_F32_SWEEPS)
@@ -1544,7 +1588,14 @@ struct Engine {
       if (!any) break;
     }
     iters_ = it;
+    // refined dense molecules: the orbital energies of the last iteration's matrix by an
+    // eigenvalue-only FP64 dense solve (the refinement leaves Rayleigh quotients inside the
+    // occupied and empty blocks; P, W and the energy do not use them)
+    if (exact_levels)
+      for (int m = 0; m < NM; m++)
+        if (dense_pfp[m]) dense_values(ejobs[m].A, ejobs[m].n, ejobs[m].w, mtau.p + mols[m].ao0, ejobs[m].V);
   }
+  bool exact_levels = true;  // mixed, dense: exact orbital energies at the end (AXTB_HIP_EXACT_LEVELS)
 
   // ------------------------------------------------------------------------------ gradient
   std::vector<int> hok;

@@ -188,6 +188,44 @@ __global__ void k_eeq_solve(Batch b, const int* mlist, const long long* eq0, con
   }
 }
 
+// EEQ start charges of molecule m from A^-1 r (column 0 of x) and A^-1 1 (column 1), A the N x N
+// block of the EEQ system [A 1; 1^T 0] [q; lambda] = [r; Q]: lambda = (sum x1 - Q) / sum x2,
+// q = x1 - lambda x2 (the bordered system of eeq.rs:43-77 by its Schur complement), split over the
+// shells as k_eeq_solve does. One workgroup; sums in a fixed order.
+__global__ void k_eeq_schur(Batch b, int m, const double* x, const double* rhs, double* qstart, int* eeq_failed) {
+  __shared__ double red[16];
+  const Mol M = b.mol[m];
+  const int n = M.nat;
+  const double* r = rhs + (long long)M.at0 + m;
+  double s1 = 0.0, s2 = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    s1 += x[i];
+    s2 += x[n + i];
+  }
+  s1 = block_sum(s1, red);
+  s2 = block_sum(s2, red);
+  const double lam = (s1 - r[n]) / s2;
+  bool bad = !isfinite(lam);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const double q = x[i] - lam * x[n + i];
+    const int a = M.at0 + i;
+    double tot = 0.0;
+    for (int s = b.a.ish[a]; s < b.a.ish[a] + b.a.nsh[a]; s++) tot += b.s.refocc[s];
+    for (int s = b.a.ish[a]; s < b.a.ish[a] + b.a.nsh[a]; s++)
+      qstart[s] = (tot > 0.0 && !bad) ? q * b.s.refocc[s] / tot : 0.0;
+  }
+  if (threadIdx.x == 0 && bad) eeq_failed[m] = 1;
+}
+
+// Right-hand sides [r | 1] (N x 2) of k_eeq_schur's solves.
+__global__ void k_eeq_rhs(Batch b, int m, const double* rhs, double* R) {
+  const Mol M = b.mol[m];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M.nat) return;
+  R[i] = rhs[(long long)M.at0 + m + i];
+  R[M.nat + i] = 1.0;
+}
+
 // ------------------------------------------------------------------ integrals
 // hscale of shells (sa on atom A, sb on atom B) (gfn2.rs:24-30,76-98)
 __device__ __forceinline__ double hscale(const Batch& b, int sa, int sb) {

@@ -297,6 +297,126 @@ __global__ void k_gemm_reduce(const double* P, int splits, int m, int n, double*
   *out = beta == 0.0 ? s : s + beta * *out;
 }
 
+// Batched products of small matrices (m, n <= 128): one workgroup of 8 wavefronts per job, the
+// output as ceil(m/16) x ceil(n/16) MFMA blocks dealt round-robin to the wavefronts (so no
+// wavefront works on padding beyond the next multiple of 16), K in steps of 16 through
+// double-buffered LDS with the next step's loads in registers (clamped addresses, masks applied at
+// the store, as in k_gemm128). Few registers per thread: two workgroups share a CU.
+constexpr int SP = 132;  // padded LDS row of the small-matrix kernel
+template <int TA, int TB>
+__global__ __launch_bounds__(512) void k_gemm_small(const GemmJob* jobs) {
+  const GemmJob J = jobs[blockIdx.x];
+  __shared__ double As[2][BK][SP];
+  __shared__ double Bs[2][BK][SP];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int mb = (J.m + 15) >> 4, nbk = (J.n + 15) >> 4, nblk = mb * nbk;
+  // element q (of 4) of this thread's share of a k-step: (row / col within 128, k within 16)
+  auto a_rk = [&](int q, int& r, int& k) {
+    if (TA == 0) {
+      r = tid & 127;
+      k = (tid >> 7) * 4 + q;
+    } else {
+      k = tid & 15;
+      r = (tid >> 4) + 32 * q;
+    }
+  };
+  auto b_ck = [&](int q, int& c, int& k) {
+    if (TB == 1) {
+      c = tid & 127;
+      k = (tid >> 7) * 4 + q;
+    } else {
+      k = tid & 15;
+      c = (tid >> 4) + 32 * q;
+    }
+  };
+  double ra[4], rb[4], rd[4];
+  unsigned ma = 0u, mbk = 0u;
+  auto load = [&](int k0) {
+    ma = mbk = 0u;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      int r, k;
+      a_rk(q, r, k);
+      const int gk = k0 + k, grc = min(r, J.m - 1), gkc = min(gk, J.k - 1);
+      ra[q] = gload(TA == 0 ? J.A + grc + (long long)gkc * J.lda : J.A + gkc + (long long)grc * J.lda);
+      ma |= (r < J.m && gk < J.k) ? 1u << q : 0u;
+      int c;
+      b_ck(q, c, k);
+      const int gkb = k0 + k, gcc = min(c, J.n - 1), gkbc = min(gkb, J.k - 1);
+      rb[q] = gload(TB == 0 ? J.B + gkbc + (long long)gcc * J.ldb : J.B + gcc + (long long)gkbc * J.ldb);
+      mbk |= (c < J.n && gkb < J.k) ? 1u << q : 0u;
+    }
+    if (J.dk) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        int r, k;
+        a_rk(q, r, k);
+        rd[q] = gload(J.dk + min(k0 + k, J.k - 1));
+      }
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      int r, k;
+      a_rk(q, r, k);
+      const double av = J.dk ? ra[q] * rd[q] : ra[q];
+      As[buf][k][r] = (ma >> q) & 1u ? av : 0.0;
+      int c;
+      b_ck(q, c, k);
+      Bs[buf][k][c] = (mbk >> q) & 1u ? rb[q] : 0.0;
+    }
+  };
+  d4v acc[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) acc[q] = (d4v){0.0, 0.0, 0.0, 0.0};
+  const int nk = (J.k + BK - 1) / BK;
+  if (nk > 0) {
+    load(0);
+    store(0);
+  }
+  __syncthreads();
+  for (int kt = 0; kt < nk; kt++) {
+    const int buf = kt & 1;
+    if (kt + 1 < nk) load((kt + 1) * BK);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      const int kl = kk + (lane >> 4), li = lane & 15;
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        const int bq = w + 8 * q;
+        if (bq < nblk) {
+          const int bi = bq % mb, bj = bq / mb;
+          const double a = As[buf][kl][16 * bi + li], b = Bs[buf][kl][16 * bj + li];
+          acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[q], 0, 0, 0);
+        }
+      }
+    }
+    if (kt + 1 < nk) store(buf ^ 1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const int bq = w + 8 * q;
+    if (bq >= nblk) continue;
+    const int bi = bq % mb, bj = bq / mb;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int gr = 16 * bi + (lane >> 4) + 4 * r, gc = 16 * bj + (lane & 15);
+      if (gr < J.m && gc < J.n) {
+        double* c = &J.C[gr + (long long)gc * J.ldc];
+        const double v = J.alpha * acc[q][r];
+        *c = J.beta == 0.0 ? v : v + J.beta * *c;
+      }
+    }
+  }
+}
+
+static constexpr auto k_gemm_small_nn = k_gemm_small<0, 0>;
+static constexpr auto k_gemm_small_nt = k_gemm_small<0, 1>;
+static constexpr auto k_gemm_small_tn = k_gemm_small<1, 0>;
+static constexpr auto k_gemm_small_tt = k_gemm_small<1, 1>;
+
 static constexpr auto k_gemm128_nn = k_gemm128<0, 0, false>;
 static constexpr auto k_gemm128_nt = k_gemm128<0, 1, false>;
 static constexpr auto k_gemm128_tn = k_gemm128<1, 0, false>;
@@ -554,64 +674,85 @@ static constexpr auto k_jacobi2_f32 = k_jacobi<2, float>;
 __global__ __launch_bounds__(256) void k_oa_step(const EigJob* jobs, double* emax, double* delta_keep, const int* slot,
                                                  int first) {
   __shared__ double red[16];
+  __shared__ double lam[200];  // batched eigenproblems have n <= 200
   const EigJob J = jobs[blockIdx.x];
-  const int n = J.n;
+  const int n = J.n, tid = threadIdx.x, nt = blockDim.x;
   const double* S = J.A;
   const double* G = J.V;
-  double* lam = J.w;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) lam[i] = S[i + (long long)i * n] / G[i + (long long)i * n];
-  __syncthreads();
-  double so = 0.0, ro = 0.0, am = 0.0;
-  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
-    const int i = e % n, j = e / n;
-    const double sd = S[e] - (i == j ? lam[i] : 0.0);
-    const double rd = (i == j ? 1.0 : 0.0) - G[e];
-    so += sd * sd;
-    ro += rd * rd;
+  for (int i = tid; i < n; i += nt) {
+    lam[i] = S[i + (long long)i * n] / G[i + (long long)i * n];
+    J.w[i] = lam[i];
   }
-  for (int i = threadIdx.x; i < n; i += blockDim.x) am = fmax(am, fabs(lam[i]));
+  __syncthreads();
+  // element e = i + j n of this thread, advanced without division
+  const int di = nt % n, dj = nt / n;
+  double so = 0.0, ro = 0.0, am = 0.0;
+  {
+    int i = tid % n, j = tid / n;
+    for (int e = tid; e < n * n; e += nt) {
+      const double sd = S[e] - (i == j ? lam[i] : 0.0);
+      const double rd = (i == j ? 1.0 : 0.0) - G[e];
+      so += sd * sd;
+      ro += rd * rd;
+      i += di;
+      j += dj;
+      if (i >= n) {
+        i -= n;
+        j++;
+      }
+    }
+  }
+  for (int i = tid; i < n; i += nt) am = fmax(am, fabs(lam[i]));
   so = block_sum(so, red);
   ro = block_sum(ro, red);
   // max |lambda| over the workgroup
   for (int o = 32; o > 0; o >>= 1) am = fmax(am, __shfl_xor(am, o, 64));
   __syncthreads();
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = am;
+  if ((tid & 63) == 0) red[tid >> 6] = am;
   __syncthreads();
   am = 0.0;
-  for (int w = 0; w < (int)(blockDim.x >> 6); w++) am = fmax(am, red[w]);
+  for (int w = 0; w < (int)(nt >> 6); w++) am = fmax(am, red[w]);
   // the first step's cluster threshold is kept for the later steps: a pair left unrotated as a
   // cluster is not rotated later from a smaller delta; pairs whose first-order mixing would
   // exceed 0.05 are treated as clusters too
   double delta = 2.0 * (sqrt(so) + am * sqrt(ro));
   if (first) {
-    if (threadIdx.x == 0) delta_keep[slot[blockIdx.x]] = delta;
+    if (tid == 0) delta_keep[slot[blockIdx.x]] = delta;
   } else {
     delta = fmax(delta, delta_keep[slot[blockIdx.x]]);
   }
   // emax: the largest correction (pairs left unrotated as clusters are not counted: a coupling
   // there changes neither P nor, beyond its own size, W)
   double emx = 0.0;
-  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
-    const int i = e % n, j = e / n;
-    const double g = G[e];
-    double v;
-    if (i == j) {
-      v = 0.5 * (1.0 - g);
-    } else {
-      const double gap = lam[j] - lam[i];
-      const double num = S[e] - lam[j] * g;
-      v = (fabs(gap) > delta && fabs(num) < 0.05 * fabs(gap)) ? num / gap : -0.5 * g;
+  {
+    int i = tid % n, j = tid / n;
+    for (int e = tid; e < n * n; e += nt) {
+      const double g = G[e];
+      double v;
+      if (i == j) {
+        v = 0.5 * (1.0 - g);
+      } else {
+        const double gap = lam[j] - lam[i];
+        const double num = S[e] - lam[j] * g;
+        v = (fabs(gap) > delta && fabs(num) < 0.05 * fabs(gap)) ? num / gap : -0.5 * g;
+      }
+      emx = fmax(emx, fabs(v));
+      J.Z[e] = (i == j ? 1.0 : 0.0) + v;
+      i += di;
+      j += dj;
+      if (i >= n) {
+        i -= n;
+        j++;
+      }
     }
-    emx = fmax(emx, fabs(v));
-    J.Z[e] = (i == j ? 1.0 : 0.0) + v;
   }
   for (int o = 32; o > 0; o >>= 1) emx = fmax(emx, __shfl_xor(emx, o, 64));
   __syncthreads();
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = emx;
+  if ((tid & 63) == 0) red[tid >> 6] = emx;
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (tid == 0) {
     double m = 0.0;
-    for (int w = 0; w < (int)(blockDim.x >> 6); w++) m = fmax(m, red[w]);
+    for (int w = 0; w < (int)(nt >> 6); w++) m = fmax(m, red[w]);
     emax[blockIdx.x] = m;
   }
 }
