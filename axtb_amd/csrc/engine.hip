@@ -1588,12 +1588,29 @@ struct Engine {
       if (!any) break;
     }
     iters_ = it;
-    // refined dense molecules: the orbital energies of the last iteration's matrix by an
-    // eigenvalue-only FP64 dense solve (the refinement leaves Rayleigh quotients inside the
-    // occupied and empty blocks; P, W and the energy do not use them)
-    if (exact_levels)
-      for (int m = 0; m < NM; m++)
-        if (dense_pfp[m]) dense_values(ejobs[m].A, ejobs[m].n, ejobs[m].w, mtau.p + mols[m].ao0, ejobs[m].V);
+    // refined dense molecules: the orbital energies of the last iteration's matrix (the refinement
+    // leaves Rayleigh quotients inside the occupied and empty blocks; P, W and the energy do not
+    // use them) as the eigenvalues of the occupied and the empty block of S = U^T A U from the last
+    // refinement step (in F; the first kocc columns are the occupied ones, and the coupling between
+    // the blocks is below the acceptance bound, so the blocks' eigenvalues are A's to second order
+    // in it), each by an eigenvalue-only FP64 dense solve
+    if (exact_levels && mixed) {
+      std::vector<int> hk(NM);
+      HIP_OK(hipMemcpyAsync(hk.data(), kocc.p, sizeof(int) * NM, hipMemcpyDeviceToHost, st));
+      HIP_OK(hipStreamSynchronize(st));
+      for (int m = 0; m < NM; m++) {
+        if (!dense_pfp[m]) continue;
+        const int n = mols[m].nao, k = hk[m];
+        const long long o0 = mols[m].mat0, a0 = mols[m].ao0;
+        for (int blk = 0; blk < 2; blk++) {
+          const int s0 = blk == 0 ? 0 : k, nb = blk == 0 ? k : n - k;
+          if (nb <= 0) continue;
+          HIP_OK(hipMemcpy2DAsync(T.p + o0, sizeof(double) * nb, F.p + o0 + s0 + (long long)s0 * n, sizeof(double) * n,
+                                  sizeof(double) * nb, nb, hipMemcpyDeviceToDevice, st));
+          dense_values(T.p + o0, nb, w.p + a0 + s0, mtau.p + a0 + s0, Vw.p + o0);
+        }
+      }
+    }
   }
   bool exact_levels = true;  // mixed, dense: exact orbital energies at the end (AXTB_HIP_EXACT_LEVELS)
 
