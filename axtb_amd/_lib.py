@@ -16,8 +16,8 @@ ERROR_KINDS = {1: "input", 2: "numerical", 3: "scf_not_converged", 4: "unsupport
 TERMS = ("repulsion", "dispersion", "electronic", "isotropic_es", "third_order", "anisotropic", "entropy", "band")
 EXPORTS = ("axtbhip_version", "axtbhip_context_create", "axtbhip_context_destroy", "axtbhip_last_error",
            "axtbhip_basis_size", "axtbhip_gfn2_engrad", "axtbhip_gfn2_engrad_batch", "axtbhip_batch_prepare",
-           "axtbhip_batch_run", "axtbhip_batch_results", "axtbhip_profile_enable", "axtbhip_profile_report",
-           "axtbhip_synchronize")
+           "axtbhip_batch_run", "axtbhip_batch_gradient", "axtbhip_batch_results", "axtbhip_profile_enable",
+           "axtbhip_profile_report", "axtbhip_synchronize")
 
 
 class AxtbError(RuntimeError):

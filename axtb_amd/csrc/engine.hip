@@ -231,6 +231,7 @@ struct Engine {
     if (const char* v = std::getenv("AXTB_HIP_OA_STEPS")) oa_steps = std::min(4, std::max(1, std::atoi(v)));
     if (const char* v = std::getenv("AXTB_HIP_DENSE_FULL")) dense_full = std::max(1, std::atoi(v));
     if (const char* v = std::getenv("AXTB_HIP_DENSE_REFINE")) dense_refine = std::atoi(v) != 0;
+    if (const char* v = std::getenv("AXTB_HIP_DENSE_OVMIX")) dense_ovmix = std::min(0.5, std::max(0.01, std::atof(v)));
     if (const char* v = std::getenv("AXTB_HIP_GEMM_SMALL")) gemm_small = std::atoi(v) != 0;
     if (const char* v = std::getenv("AXTB_HIP_EXACT_LEVELS")) exact_levels = std::atoi(v) != 0;
     if (const char* v = std::getenv("AXTB_HIP_F32_TOL")) f32_tol = (float)std::min(1e3, std::max(1.0, std::atof(v)));
@@ -1080,7 +1081,10 @@ struct Engine {
     LAUNCH(k_atm_pairs, dim3(blocks((long long)natmax * natmax, 256), NM), dim3(256), 0, b, gw.p,
                        dgw.p, atmP.p);
     if (o.dispersion) {
-      LAUNCH(k_atm, dim3(blocks(NA, 4)), dim3(256), 0, b, atmP.p, e3.p, g3.p, dcn3.p);
+      atmGD.ensure(2 * (size_t)tot_pair);
+      LAUNCH(k_atm_pairsum, dim3(blocks(NA, 4)), dim3(256), 0, b, (const double*)atmP.p, atmGD.p);
+      LAUNCH(k_atm, dim3(blocks(NA, 4)), dim3(256), 0, b, (const double*)atmP.p, (const double*)atmGD.p, e3.p, g3.p,
+             dcn3.p);
     } else {
       fill(e3.p, NA);
       fill(g3.p, 3LL * NA);
@@ -1375,7 +1379,7 @@ struct Engine {
       LAUNCH(k_mirror_lower, dim3(nbm * (nbm + 1) / 2), dim3(256), 0, Vw.p + o0, n, (long long)n);
       LAUNCH(k_oa_big_norms, dim3(nb), dim3(256), 0, oj, oab_part.p);
       LAUNCH(k_oa_big_e, dim3(nb), dim3(256), 0, oj, (const double*)oab_part.p, nb, oab_delta.p, step == 0 ? 1 : 0,
-             oab_emax.p, cls);
+             oab_emax.p, cls, dense_ovmix);
       std::vector<GemmJob> gu{GemmJob{n, n, n, 0, 0, U, Wm.p + o0, out, nullptr, n, n, n, 1.0, 0.0}};
       gemm(gu);
       step++;
@@ -1391,6 +1395,15 @@ struct Engine {
     prof.count("refine_steps_dense", 1, (double)step);
     return emax;
   }
+  // symmetric products of dense-solver molecules are formed on their lower 128-tiles and mirrored
+  void mirror_dense(const std::vector<GemmJob>& jobs) {
+    for (const GemmJob& j : jobs)
+      if (j.lower) {
+        const int nbm = blocks(j.m, 32);
+        LAUNCH(k_mirror_lower, dim3(nbm * (nbm + 1) / 2), dim3(256), 0, j.C, j.m, j.ldc);
+      }
+  }
+  DBuf<double> atmGD;  // ATM pair sums (2 x the atom-pair buffer)
   DBuf<int> dcls, dnfrac;
   std::vector<int> dense_pfp;  // per molecule: its last eigenproblem was refined (W = P F P / 2)
   int dense_steps = 4;         // mixed, dense: refinement steps at most per SCC iteration
@@ -1398,6 +1411,9 @@ struct Engine {
   // occupied-empty pairs and the orthogonality terms) is <= dense_accept; the error it leaves is of
   // second order in that correction (AXTB_HIP_DENSE_ACCEPT)
   double dense_accept = 1e-4;
+  // mixed, dense: largest first-order mixing of an occupied-empty pair the refinement takes on; a
+  // larger one ends it (a full solve follows) (AXTB_HIP_DENSE_OVMIX)
+  double dense_ovmix = 0.05;
   int dense_full = 1;  // mixed: SCC iterations solved in full before refinement (AXTB_HIP_DENSE_FULL)
   // mixed: refine the previous iteration's vectors of dense-solver molecules instead of a full
   // solve (AXTB_HIP_DENSE_REFINE=0 turns it off). Only the occupied-empty pairs are resolved: P
@@ -1494,7 +1510,9 @@ struct Engine {
                          pool.p, stride, vsh.p, vat.p, vdp.p, vqp.p, F.p);
       // generalized eigenproblem in the Loewdin basis
       gemm(a1);
+      for (auto& j : a2) j.lower = dense(j.n) ? 1 : 0;  // A = B^T (F B) is symmetric
       gemm(a2);
+      mirror_dense(a2);
       if (o.precision == 1) {
         // mixed: Jacobi in FP32, then one FP64 Ogita-Aishima step U' = U (I + E) from
         // S = U^T A U and G = U^T U (U in Z, A in Am; S -> F, G -> Vw, I + E -> Wm, U' -> T)
@@ -1564,7 +1582,9 @@ struct Engine {
         a4.push_back(p);
       }
       gemm(a5);
+      for (auto& j : a4) j.lower = dense(j.n) ? 1 : 0;
       gemm(a4);  // P = C diag(f) C^T
+      mirror_dense(a4);
       LAUNCHW(sbytes, k_mulliken_cols, dim3(blocks(NO, 4)), dim3(256), 0, b, active.p, pool.p, stride, P.p,
                          cols.p);
       LAUNCH(k_mulliken_atoms, dim3(blocks(NA, 128)), dim3(128), 0, b, active.p, cols.p, qsh_o.p,
@@ -1660,7 +1680,9 @@ struct Engine {
       LAUNCH(k_fock, dim3(blocks((long long)naomax * naomax, 256), NM), dim3(256), 0, b, (const int*)pfd.p, pool.p,
              stride, vsh.p, vat.p, vdp.p, vqp.p, F.p);
       gemm(gF);
+      for (auto& j : gW2) j.lower = 1;  // W = P (F P) / 2 is symmetric
       gemm(gW2);
+      mirror_dense(gW2);
       HIP_OK(hipStreamSynchronize(st));
     }
     DBuf<int> okd;
@@ -1672,7 +1694,9 @@ struct Engine {
     HIP_OK(hipStreamSynchronize(st));
     for (int i = 0; i < naos(); i++) hf[i] *= hw[i];
     HIP_OK(hipMemcpyAsync(fwork.p, hf.data(), sizeof(double) * naos(), hipMemcpyHostToDevice, st));
+    for (auto& j : gW) j.lower = dense(j.n) ? 1 : 0;
     gemm(gW);
+    mirror_dense(gW);
     // gradient = repulsion + three-body (setup) + pair terms + CN chains
     HIP_OK(hipMemcpyAsync(grad.p, grep.p, sizeof(double) * 3 * NA, hipMemcpyDeviceToDevice, st));
     std::vector<double> hg3(3 * NA), hgr(3 * NA);

@@ -520,25 +520,24 @@ __global__ void k_atm_pairs(Batch b, const double* gw, const double* dgw, double
   P[7 * tot + o] = dswv;
 }
 
-// ATM three-body energy and gradient by pair sums (disp.rs:607-720, axtb's d4_atm_pair_sums):
-// one wavefront per atom a sums, over its partners b with sw > 0, the pair's (g, d) over third
-// atoms k, and keeps its own share: E_a -= sw d / 6 per pair, the pair force along R_b - R_a and
-// dE/dCN_a. Writes e3[a], g3[3a], dcn3[a].
-__global__ void k_atm(Batch b, const double* P, double* e3, double* g3, double* dcn3) {
+// ATM three-body energy and gradient by pair sums (disp.rs:607-720, axtb's d4_atm_pair_sums), in
+// two passes. k_atm_pairsum: the pair's (g, d) summed over third atoms k for every unordered atom
+// pair (a, b), a < b, written for both orders: for every k,
+// the damped angular term and its derivative along r_ab (the triple sum of the pair is symmetric
+// in a and b, so each pair is summed once). GD[o] = cab * sum_k (derivative part),
+// GD[tot + o] = cab * sum_k (energy part), cab = -5 sqrt|C6_ab|. One wavefront per atom a.
+__global__ void k_atm_pairsum(Batch b, const double* P, double* GD) {
   int a = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
   int lane = threadIdx.x & 63;
   if (a >= b.nat) return;
   int m = b.a.mol[a];
   const Mol M = b.mol[m];
   long long tot = b.mol[b.nmol - 1].pair0 + (long long)b.mol[b.nmol - 1].nat * b.mol[b.nmol - 1].nat;
-  const double *R2 = P, *IR = P + tot, *T = P + 2 * tot, *W = P + 3 * tot, *C6 = P + 4 * tot, *QA = P + 5 * tot,
-               *SW = P + 6 * tot, *DSW = P + 7 * tot;
+  const double *R2 = P, *IR = P + tot, *T = P + 2 * tot, *W = P + 3 * tot, *C6 = P + 4 * tot, *SW = P + 6 * tot;
   int la = a - M.at0;
   long long rowa = M.pair0 + (long long)la * M.nat;
-  double es = 0.0, gx = 0.0, gy = 0.0, gz = 0.0, dc = 0.0;
   const double alp3 = 16.0 / 3.0;
-  for (int lb = 0; lb < M.nat; lb++) {
-    if (lb == la) continue;
+  for (int lb = la + 1; lb < M.nat; lb++) {
     double swab = SW[rowa + lb];
     if (!(swab > 0.0)) continue;
     long long rowb = M.pair0 + (long long)lb * M.nat;
@@ -562,19 +561,51 @@ __global__ void k_atm(Batch b, const double* P, double* e3, double* g3, double* 
     }
     gs = wave_sum(gs);
     ds = wave_sum(ds);
-    double cab = -(5.0 * sqrt(fabs(C6[rowa + lb])));
-    double g = gs * cab, d = ds * cab;
+    if (lane == 0) {
+      double cab = -(5.0 * sqrt(fabs(C6[rowa + lb])));
+      GD[rowa + lb] = gs * cab;
+      GD[rowb + la] = gs * cab;
+      GD[tot + rowa + lb] = ds * cab;
+      GD[tot + rowb + la] = ds * cab;
+    }
+  }
+}
+
+// k_atm: each atom a keeps its share of its pairs with sw > 0: E_a -= sw d / 6 per pair, the pair
+// force along R_b - R_a and dE/dCN_a (writes e3[a], g3[3a], dcn3[a]); one wavefront per atom,
+// lanes over the partners b.
+__global__ void k_atm(Batch b, const double* P, const double* GD, double* e3, double* g3, double* dcn3) {
+  int a = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  int lane = threadIdx.x & 63;
+  if (a >= b.nat) return;
+  int m = b.a.mol[a];
+  const Mol M = b.mol[m];
+  long long tot = b.mol[b.nmol - 1].pair0 + (long long)b.mol[b.nmol - 1].nat * b.mol[b.nmol - 1].nat;
+  const double *R2 = P, *QA = P + 5 * tot, *SW = P + 6 * tot, *DSW = P + 7 * tot;
+  int la = a - M.at0;
+  long long rowa = M.pair0 + (long long)la * M.nat;
+  double es = 0.0, gx = 0.0, gy = 0.0, gz = 0.0, dc = 0.0;
+  const double* xa = &b.a.xyz[3 * a];
+  for (int lb = lane; lb < M.nat; lb += 64) {
+    if (lb == la) continue;
+    double swab = SW[rowa + lb];
+    if (!(swab > 0.0)) continue;
+    const double g = GD[rowa + lb], d = GD[tot + rowa + lb];
     double de = swab * d;
     es -= de / 6.0;
     // force on a from the pair (a, b): -(f) (R_b - R_a) with f = sw g / r2 - dsw/r d
-    double f = swab * g / r2 - DSW[rowa + lb] * d;
-    const double* xa = &b.a.xyz[3 * a];
+    double f = swab * g / R2[rowa + lb] - DSW[rowa + lb] * d;
     const double* xb = &b.a.xyz[3 * (M.at0 + lb)];
     gx -= f * (xb[0] - xa[0]);
     gy -= f * (xb[1] - xa[1]);
     gz -= f * (xb[2] - xa[2]);
     dc -= 0.5 * de * QA[rowa + lb];
   }
+  es = wave_sum(es);
+  gx = wave_sum(gx);
+  gy = wave_sum(gy);
+  gz = wave_sum(gz);
+  dc = wave_sum(dc);
   if (lane == 0) {
     e3[a] = es;
     g3[3 * a] = gx;

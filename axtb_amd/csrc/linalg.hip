@@ -468,7 +468,7 @@ template <> struct JacobiTol<float> {
   static constexpr float REL = 2.5e-7f, ABS = 2.5e-7f;
 };
 
-template <int MODE, typename T>
+template <int MODE, typename T, bool PACK>
 __global__ __launch_bounds__(512) void k_jacobi(const EigJob* jobs, int max_sweeps, int* status, float tol_scale) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T* lds = (T*)lds_raw;
@@ -479,20 +479,34 @@ __global__ __launch_bounds__(512) void k_jacobi(const EigJob* jobs, int max_swee
   if (n <= 0) return;
   const int nn = n + (n & 1), np = nn / 2;
   const int lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
-  // dynamic LDS: [A (n*n, MODE >= 1)] [V (n*n, MODE 2)] [c, s per pair: 2 buffers] [p, q per pair:
-  // 2 buffers]; MODE 0 works on J.A, J.V in place (FP64 only)
+  // dynamic LDS: [A (MODE >= 1; PACK: the lower triangle packed by rows, element (r, c), r >= c,
+  // at r (r + 1) / 2 + c; else full n*n)] [V (n*n, MODE 2)] [c, s per pair: 2 buffers] [p, q per
+  // pair: 2 buffers]; MODE 0 works on J.A (full, both triangles kept) and J.V in place (FP64 only)
+  const int nl = PACK ? n * (n + 1) / 2 : n * n;
   T* A = MODE >= 1 ? lds : (T*)J.A;
-  T* V = MODE == 2 ? lds + n * n : (T*)J.V;
-  T* csb0 = lds + (MODE >= 1 ? n * n : 0) + (MODE == 2 ? n * n : 0);
+  T* V = MODE == 2 ? lds + nl : (T*)J.V;
+  T* csb0 = lds + (MODE >= 1 ? nl : 0) + (MODE == 2 ? n * n : 0);
   int* pqb0 = (int*)(csb0 + 4 * np);
+  // element (r, c) of the symmetric A
+  auto aix = [&](int r, int c) -> int {
+    if (!PACK) return r + c * n;
+    const int hi = max(r, c), lo = min(r, c);
+    return hi * (hi + 1) / 2 + lo;
+  };
   if (MODE >= 1)
-    for (int e = tid; e < n * n; e += nt) A[e] = (T)J.A[e];
+    for (int e = tid; e < n * n; e += nt) {
+      const int r = e % n, c = e / n;
+      if (!PACK)
+        A[e] = (T)J.A[e];
+      else if (r >= c)
+        A[r * (r + 1) / 2 + c] = (T)J.A[e];
+    }
   for (int e = tid; e < n * n; e += nt) V[e] = (e % n == e / n) ? (T)1 : (T)0;
   if (tid == 0) rotated = 0;
   __syncthreads();
   if (tid < 64) {
     T s = 0;
-    for (int e = tid; e < n * n; e += 64) s = fmax(s, fabs(A[e]));
+    for (int e = tid; e < nl; e += 64) s = fmax(s, fabs(A[e]));
     for (int o = 32; o > 0; o >>= 1) s = fmax(s, __shfl_xor(s, o, 64));
     if (tid == 0) anorm = s;
   }
@@ -521,16 +535,21 @@ __global__ __launch_bounds__(512) void k_jacobi(const EigJob* jobs, int max_swee
       }
     }
   }
-  // rotation (c, s) and pair (p, q) of every pair k of round r into buffer `buf`
+  // rotation (c, s) and pair (p, q) of every pair k of round r into buffer `buf`. With V in LDS
+  // (MODE 2) the last ceil(np / 64) wavefronts compute them (pair k on thread nt - 1 - k) while
+  // the first nvw wavefronts update V; with V in global memory every wavefront updates V and the
+  // first threads compute the rotations
+  const int nvw = MODE == 2 ? max(1, nw - (np + 63) / 64) : nw;
+  const int rt = MODE == 2 ? nt - 1 - tid : tid;
   auto rotations = [&](int r, int buf) {
     T* csb = csb0 + buf * 2 * np;
     int* pqb = pqb0 + buf * 2 * np;
-    for (int k = tid; k < np; k += nt) {
+    for (int k = rt; k < np; k += nt) {
       int p, q;
       tour_pair(r, k, nn, p, q);
       T c = 1, s = 0;
       if (q < n) {
-        const T app = A[p + p * n], aqq = A[q + q * n], apq = A[p + q * n];
+        const T app = A[aix(p, p)], aqq = A[aix(q, q)], apq = A[aix(p, q)];
         if (fabs(apq) > REL * sqrt(fabs(app) * fabs(aqq)) && fabs(apq) > ABS * an) {
           const T tau = (aqq - app) / (2 * apq);
           const T t = (tau >= 0 ? (T)1 : (T)-1) / (fabs(tau) + sqrt(1 + tau * tau));
@@ -553,8 +572,8 @@ __global__ __launch_bounds__(512) void k_jacobi(const EigJob* jobs, int max_swee
       const int dj = nt / n, dii = nt % n;
       for (int e = tid; e < n * n; e += nt) {
         if (i < j) {
-          const T aij = A[e];
-          if (fabs(aij) > REL * sqrt(fabs(A[i + i * n]) * fabs(A[j + j * n])) && fabs(aij) > ABS * an)
+          const T aij = A[aix(i, j)];
+          if (fabs(aij) > REL * sqrt(fabs(A[aix(i, i)]) * fabs(A[aix(j, j)])) && fabs(aij) > ABS * an)
             rotated = 1;
         }
         i += dii;
@@ -586,43 +605,46 @@ __global__ __launch_bounds__(512) void k_jacobi(const EigJob* jobs, int max_swee
         const int p1 = pqb[2 * k1], q1 = pqb[2 * k1 + 1], p2 = pqb[2 * k2], q2 = pqb[2 * k2 + 1];
         const T c1 = csb[2 * k1], c2 = csb[2 * k2];
         const bool vq1 = q1 < n, vq2 = q2 < n;
-        const T b00 = A[p1 + p2 * n];
-        const T b01 = vq2 ? A[p1 + q2 * n] : (T)0;
-        const T b10 = vq1 ? A[q1 + p2 * n] : (T)0;
-        const T b11 = (vq1 && vq2) ? A[q1 + q2 * n] : (T)0;
+        const T b00 = A[aix(p1, p2)];
+        const T b01 = vq2 ? A[aix(p1, q2)] : (T)0;
+        const T b10 = vq1 ? A[aix(q1, p2)] : (T)0;
+        const T b11 = (vq1 && vq2) ? A[aix(q1, q2)] : (T)0;
         const T t00 = c2 * b00 - s2 * b01, t01 = s2 * b00 + c2 * b01;
         const T t10 = c2 * b10 - s2 * b11, t11 = s2 * b10 + c2 * b11;
         const T n00 = c1 * t00 - s1 * t10, n01 = c1 * t01 - s1 * t11;
         const T n10 = s1 * t00 + c1 * t10, n11 = s1 * t01 + c1 * t11;
         if (k1 == k2) {
-          A[p1 + p1 * n] = n00;
+          A[aix(p1, p1)] = n00;
           if (vq1) {
-            A[q1 + q1 * n] = n11;
-            A[p1 + q1 * n] = 0;
-            A[q1 + p1 * n] = 0;
+            A[aix(q1, q1)] = n11;
+            A[aix(p1, q1)] = 0;
+            if (!PACK) A[q1 + p1 * n] = 0;
           }
           continue;
         }
-        A[p1 + p2 * n] = n00;
-        A[p2 + p1 * n] = n00;
+        // (r, c) and (c, r) are one element in the packed lower triangle; the full storage keeps
+        // the mirror too
+        A[aix(p1, p2)] = n00;
+        if (!PACK) A[p2 + p1 * n] = n00;
         if (vq2) {
-          A[p1 + q2 * n] = n01;
-          A[q2 + p1 * n] = n01;
+          A[aix(p1, q2)] = n01;
+          if (!PACK) A[q2 + p1 * n] = n01;
         }
         if (vq1) {
-          A[q1 + p2 * n] = n10;
-          A[p2 + q1 * n] = n10;
+          A[aix(q1, p2)] = n10;
+          if (!PACK) A[p2 + q1 * n] = n10;
         }
         if (vq1 && vq2) {
-          A[q1 + q2 * n] = n11;
-          A[q2 + q1 * n] = n11;
+          A[aix(q1, q2)] = n11;
+          if (!PACK) A[q2 + q1 * n] = n11;
         }
       }
       __syncthreads();
       // the next round's rotations (from the updated A) beside this round's V <- V J
       if (r + 1 < nn - 1) rotations(r + 1, buf ^ 1);
-      // V: one pair per wavefront at a time, the lanes over consecutive rows
-      for (int k = wv; k < np; k += nw) {
+      // V: one pair per wavefront at a time, the lanes over consecutive rows; the last wavefronts
+      // compute the next round's rotations instead
+      for (int k = wv; k < np && wv < nvw; k += nvw) {
         const T s = csb[2 * k + 1];
         if (s == 0) continue;
         const T c = csb[2 * k];
@@ -639,10 +661,10 @@ __global__ __launch_bounds__(512) void k_jacobi(const EigJob* jobs, int max_swee
   }
   // eigenvalues ascending (ties by index), vectors permuted alike
   for (int i = tid; i < n; i += nt) {
-    const T di = A[i + i * n];
+    const T di = A[aix(i, i)];
     int rank = 0;
     for (int j = 0; j < n; j++) {
-      const T dj = A[j + j * n];
+      const T dj = A[aix(j, j)];
       if (dj < di || (dj == di && j < i)) rank++;
     }
     J.w[rank] = (double)di;
@@ -651,17 +673,17 @@ __global__ __launch_bounds__(512) void k_jacobi(const EigJob* jobs, int max_swee
   if (tid == 0) status[blockIdx.x] = conv ? sweep : -sweep;
 }
 
-template __global__ void k_jacobi<0, double>(const EigJob*, int, int*, float);
-template __global__ void k_jacobi<1, double>(const EigJob*, int, int*, float);
-template __global__ void k_jacobi<2, double>(const EigJob*, int, int*, float);
-template __global__ void k_jacobi<1, float>(const EigJob*, int, int*, float);
-template __global__ void k_jacobi<2, float>(const EigJob*, int, int*, float);
+template __global__ void k_jacobi<0, double, false>(const EigJob*, int, int*, float);
+template __global__ void k_jacobi<1, double, true>(const EigJob*, int, int*, float);
+template __global__ void k_jacobi<2, double, true>(const EigJob*, int, int*, float);
+template __global__ void k_jacobi<1, float, true>(const EigJob*, int, int*, float);
+template __global__ void k_jacobi<2, float, false>(const EigJob*, int, int*, float);
 
-static constexpr auto k_jacobi0 = k_jacobi<0, double>;
-static constexpr auto k_jacobi1 = k_jacobi<1, double>;
-static constexpr auto k_jacobi2 = k_jacobi<2, double>;
-static constexpr auto k_jacobi1_f32 = k_jacobi<1, float>;
-static constexpr auto k_jacobi2_f32 = k_jacobi<2, float>;
+static constexpr auto k_jacobi0 = k_jacobi<0, double, false>;
+static constexpr auto k_jacobi1 = k_jacobi<1, double, true>;
+static constexpr auto k_jacobi2 = k_jacobi<2, double, true>;
+static constexpr auto k_jacobi1_f32 = k_jacobi<1, float, true>;
+static constexpr auto k_jacobi2_f32 = k_jacobi<2, float, false>;
 
 // One Ogita-Aishima refinement step of approximate eigenvectors U of A (FP64), given
 // S = U^T A U and G = U^T U (R = I - G): lambda_i = s_ii / g_ii;
@@ -798,11 +820,11 @@ __global__ __launch_bounds__(256) void k_oa_big_norms(const EigJob J, double* pa
 
 // cls (nullable): per column, 0 = fully occupied, 1 = empty, 2 = fractional occupation. With it,
 // an occupied-empty pair is always rotated (its gap is the occupied-virtual gap; a first-order
-// mixing >= 0.05 marks the step as not convergent, emax >= 1), pairs inside the occupied or the
+// mixing >= ovmix marks the step as not convergent, emax >= 1), pairs inside the occupied or the
 // empty block are only orthogonalised (rotations there leave P = C f C^T unchanged when f is
 // constant on the block), and pairs with a fractional column follow the delta rule.
 __global__ __launch_bounds__(256) void k_oa_big_e(const EigJob J, const double* part, int nparts, double* delta_keep,
-                                                  int first, double* emax_part, const int* cls) {
+                                                  int first, double* emax_part, const int* cls, double ovmix) {
   __shared__ double red[16];
   __shared__ double s_delta;
   const int n = J.n;
@@ -835,7 +857,7 @@ __global__ __launch_bounds__(256) void k_oa_big_e(const EigJob J, const double* 
       if (ci != 2 && cj != 2) {
         if (ci == cj) {
           v = -0.5 * g;
-        } else if (fabs(num) < 0.05 * fabs(gap)) {
+        } else if (fabs(num) < ovmix * fabs(gap)) {
           v = num / gap;
         } else {
           v = -0.5 * g;
@@ -1046,8 +1068,10 @@ __global__ __launch_bounds__(512) void k_eeq_lu(Batch b, const int* mlist, const
 
 // dynamic LDS bytes of k_jacobi for an n x n matrix in elements of `es` bytes
 inline size_t jacobi_lds(int n, int mode, size_t es = 8) {
-  size_t np = (n + 1) / 2;
-  return (size_t)(mode >= 1 ? 1 : 0) * n * n * es + (size_t)(mode == 2 ? 1 : 0) * n * n * es + np * 4 * es + np * 4 * 4;
+  // A packed (lower triangle) except in the FP32 LDS mode 2, which keeps it full (k_jacobi's PACK)
+  const bool pack = !(mode == 2 && es == 4);
+  const size_t np = (n + 1) / 2, na = pack ? (size_t)n * (n + 1) / 2 : (size_t)n * n;
+  return (size_t)(mode >= 1 ? 1 : 0) * na * es + (size_t)(mode == 2 ? 1 : 0) * n * n * es + np * 4 * es + np * 4 * 4;
 }
 
 }  // namespace axtb

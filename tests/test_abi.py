@@ -61,3 +61,24 @@ def test_workloads():
     assert len(z) == 302 and (z == 6).sum() == 100
     b = S.batch(64, seed=1)
     assert len(b) == 64 and all(9 <= len(m[0]) <= 62 for m in b)
+
+
+def test_bench_single_gpu_only():
+    """bench.py measures one GPU: --gpus N > 1 exits with a message, without touching a GPU."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode != 0
+    assert "one GPU" in r.stderr
+
+
+def test_conformer_workload():
+    """The differentiation workload: C30H62 conformers, all-trans plus 0.1 A Gaussian noise."""
+    import numpy as np
+    from axtb_amd import structures as S
+    c = S.conformers(4, seed=3)
+    assert len(c) == 4 and all(len(z) == 92 for z, _ in c)
+    ref = S.alkane(30)[1]
+    dev = np.concatenate([x - ref for _, x in c]).ravel() / S.AATOAU
+    assert 0.07 < dev.std() < 0.13
