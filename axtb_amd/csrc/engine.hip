@@ -65,6 +65,39 @@ bool sp_element(int z) {
   return true;
 }
 
+// Pinned host staging for the small per-launch uploads (job lists): a copy from pageable memory
+// makes the host wait for the stream to drain, one from pinned memory does not. A ring; when it
+// wraps, the device is synchronised so that no copy still reads the part being reused.
+struct Staging {
+  char* base = nullptr;
+  size_t cap = 0, off = 0;
+  const void* put(const void* src, size_t bytes) {
+    const size_t need = (bytes + 255) & ~(size_t)255;
+    if (!base) {
+      cap = (size_t)64 << 20;
+      if (hipHostMalloc((void**)&base, cap, hipHostMallocDefault) != hipSuccess) {
+        base = nullptr;
+        cap = 0;
+      }
+    }
+    if (!base || need > cap) return nullptr;  // the caller copies from pageable memory
+    if (off + need > cap) {
+      HIP_OK(hipDeviceSynchronize());
+      off = 0;
+    }
+    char* d = base + off;
+    off += need;
+    std::memcpy(d, src, bytes);
+    return d;
+  }
+  ~Staging() {
+    if (base) {
+      (void)hipDeviceSynchronize();
+      (void)hipHostFree(base);
+    }
+  }
+};
+
 template <class T>
 struct DBuf {
   T* p = nullptr;
@@ -76,9 +109,11 @@ struct DBuf {
     HIP_OK(hipMalloc(&p, std::max<size_t>(m, 1) * sizeof(T)));
     n = std::max<size_t>(m, 1);
   }
-  void upload(const std::vector<T>& v, hipStream_t s) {
+  void upload(const std::vector<T>& v, hipStream_t s, Staging* stage = nullptr) {
     ensure(v.size());
-    if (!v.empty()) HIP_OK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    if (v.empty()) return;
+    const void* src = stage ? stage->put(v.data(), v.size() * sizeof(T)) : nullptr;
+    HIP_OK(hipMemcpyAsync(p, src ? src : (const void*)v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
   }
   ~DBuf() {
     if (p) (void)hipFree(p);
@@ -192,6 +227,7 @@ struct MolOut {
 };
 
 struct Engine {
+  Staging stage;  // first member: destroyed last, after every buffer and stream
   int device = 0;
   hipStream_t st = nullptr;
   hipStream_t st2 = nullptr;  // side stream (default priority) for overlapped eigensolver launches
@@ -231,6 +267,7 @@ struct Engine {
     if (const char* v = std::getenv("AXTB_HIP_OA_STEPS")) oa_steps = std::min(4, std::max(1, std::atoi(v)));
     if (const char* v = std::getenv("AXTB_HIP_DENSE_FULL")) dense_full = std::max(1, std::atoi(v));
     if (const char* v = std::getenv("AXTB_HIP_DENSE_REFINE")) dense_refine = std::atoi(v) != 0;
+    if (const char* v = std::getenv("AXTB_HIP_DENSE_STEPS")) dense_steps = std::min(12, std::max(1, std::atoi(v)));
     if (const char* v = std::getenv("AXTB_HIP_DENSE_OVMIX")) dense_ovmix = std::min(0.5, std::max(0.01, std::atof(v)));
     if (const char* v = std::getenv("AXTB_HIP_GEMM_SMALL")) gemm_small = std::atoi(v) != 0;
     if (const char* v = std::getenv("AXTB_HIP_EXACT_LEVELS")) exact_levels = std::atoi(v) != 0;
@@ -293,7 +330,7 @@ struct Engine {
           P.beta = 0.0;
           parts.push_back(P);
         }
-        gj.upload(parts, st);
+        gj.upload(parts, st, &stage);
         dim3 grid(blocks(J.n, BT), blocks(J.m, BT), (unsigned)parts.size());
         launch128(J.ta * 2 + J.tb, grid, flops);
         LAUNCH(k_gemm_reduce, dim3(blocks((long long)J.m * J.n, 256)), dim3(256), 0, (const double*)gpart.p,
@@ -304,7 +341,7 @@ struct Engine {
     }
     // products with m, n <= 128: one workgroup per job, 16 x 16 granularity
     if (gemm_small && same && !two && mm <= 128 && nn <= 128) {
-      gj.upload(jobs, st);
+      gj.upload(jobs, st, &stage);
       const int t = jobs[0].ta * 2 + jobs[0].tb;
       const dim3 grid((unsigned)jobs.size());
       if (t == 0)
@@ -323,7 +360,7 @@ struct Engine {
     if (two || (same && (long long)mm * nn >= 256LL * 256 && jobs.size() <= 8) ||
         (same && (gemm128_batch || in_refine) && std::max(mm, nn) > 64)) {
       if (!same) throw DeviceError("gemm: mixed transposes in one launch");
-      gj.upload(jobs, st);
+      gj.upload(jobs, st, &stage);
       dim3 grid(blocks(nn, BT), blocks(mm, BT), (unsigned)jobs.size());
       if (two) {
         if (jobs[0].ta != 0 || jobs[0].tb != 1) throw DeviceError("gemm: two-term products are A B^T + A2 B2^T");
@@ -336,7 +373,7 @@ struct Engine {
       HIP_OK(hipGetLastError());
       return;
     }
-    gj.upload(jobs, st);
+    gj.upload(jobs, st, &stage);
     dim3 grid(blocks(nn, GT), blocks(mm, GT), (unsigned)jobs.size());
     LAUNCHW(flops, k_gemm, grid, dim3(256), 0, gj.p);
     HIP_OK(hipGetLastError());
@@ -394,25 +431,32 @@ struct Engine {
       auto& js = groups[mode];
       if (js.empty()) continue;
       hipStream_t s = (side && mode < 2) ? st2 : st;
-      int nmax = 0;
-      double flops = 0.0;
-      for (auto& j : js) {
-        nmax = std::max(nmax, j.n);
-        flops += 4.0 / 3.0 * j.n * (double)j.n * j.n + 2.0 * j.n * (double)j.n * j.n;
-      }
       DBuf<EigJob>& buf = ejb[mode];
-      buf.upload(js, s);
-      size_t shm = jacobi_lds(nmax, mode, es);
+      buf.upload(js, s, &stage);
       int* stat = jstat.p + off;
       off += js.size();
-      auto go = [&](const void* fn, auto kern) {
-        HIP_OK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-        hipLaunchKernelGGL(kern, dim3(js.size()), dim3(512), shm, s, buf.p, max_sweeps, stat, tol_scale);
+      const char* nm = fp32 ? (mode == 2 ? "k_jacobi2_f32" : "k_jacobi1_f32")
+                            : (mode == 2 ? "k_jacobi2" : (mode == 1 ? "k_jacobi1" : "k_jacobi0"));
+      // the jobs (largest first) in launches of one LDS size class each: a launch reserves the LDS
+      // of its largest matrix in every workgroup, so small matrices in a launch of their own fit 2-4
+      // workgroups per CU
+      auto per_cu = [&](int n) {
+        return std::max<size_t>(1, std::min<size_t>(4, (size_t)max_lds / std::max<size_t>(jacobi_lds(n, mode, es), 1)));
       };
-      {
-        const char* nm = fp32 ? (mode == 2 ? "k_jacobi2_f32" : "k_jacobi1_f32")
-                              : (mode == 2 ? "k_jacobi2" : (mode == 1 ? "k_jacobi1" : "k_jacobi0"));
+      size_t b0 = 0;
+      while (b0 < js.size()) {
+        const size_t shm = jacobi_lds(js[b0].n, mode, es);
+        const size_t pc = mode == 0 ? 1 : per_cu(js[b0].n);
+        size_t b1 = b0 + 1;
+        while (b1 < js.size() && (mode == 0 || per_cu(js[b1].n) <= pc)) b1++;
+        double flops = 0.0;  // dense eigendecompositions with all vectors
+        for (size_t q = b0; q < b1; q++) flops += 4.0 / 3.0 * js[q].n * (double)js[q].n * js[q].n + 2.0 * js[q].n * (double)js[q].n * js[q].n;
         ProfScope ps_(&prof, s, nm, flops);
+        auto go = [&](const void* fn, auto kern) {
+          HIP_OK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+          hipLaunchKernelGGL(kern, dim3((unsigned)(b1 - b0)), dim3(512), shm, s, buf.p + b0, max_sweeps, stat + b0,
+                             tol_scale);
+        };
         if (fp32) {
           if (mode == 2)
             go((const void*)k_jacobi2_f32, k_jacobi2_f32);
@@ -425,6 +469,7 @@ struct Engine {
         } else {
           go((const void*)k_jacobi0, k_jacobi0);
         }
+        b0 = b1;
       }
       HIP_OK(hipGetLastError());
       if (prof.on) {
@@ -1315,12 +1360,13 @@ struct Engine {
           gau.push_back(GemmJob{n, n, n, 0, 0, j.A, U, C.p + o0, nullptr, n, n, n, 1.0, 0.0});        // A U
           gsg.push_back(GemmJob{n, n, n, 1, 0, U, C.p + o0, F.p + o0, nullptr, n, n, n, 1.0, 0.0});   // S
           gsg.push_back(GemmJob{n, n, n, 1, 0, U, U, Vw.p + o0, nullptr, n, n, n, 1.0, 0.0});         // G
+          gsg[gsg.size() - 2].lower = gsg.back().lower = sym_lower(n, false);  // both symmetric
           gu.push_back(GemmJob{n, n, n, 0, 0, U, Wm.p + o0, out, nullptr, n, n, n, 1.0, 0.0});        // U'
           oj.push_back(EigJob{n, F.p + o0, Vw.p + o0, j.w, Wm.p + o0});
         }
-        oa_jobs.upload(oj, st);
+        oa_jobs.upload(oj, st, &stage);
         oa_emax.ensure(oj.size());
-        oa_slot.upload(idx, st);  // the kept cluster threshold of a job lives at its index in `ea`
+        oa_slot.upload(idx, st, &stage);  // the kept cluster threshold of a job lives at its index in `ea`
         gemm(gau);
         gemm(gsg);
         LAUNCH(k_oa_step, dim3(oj.size()), dim3(256), 0, (const EigJob*)oa_jobs.p, oa_emax.p, oa_delta.p,
@@ -1395,10 +1441,12 @@ struct Engine {
     prof.count("refine_steps_dense", 1, (double)step);
     return emax;
   }
-  // symmetric products of dense-solver molecules are formed on their lower 128-tiles and mirrored
+  // symmetric products: small ones (n <= 128) on the lower 16 x 16 blocks of k_gemm_small, which
+  // writes the mirror itself; dense-solver molecules' on their lower 128-tiles, mirrored here
+  static int sym_lower(int n, bool dense_mol) { return (dense_mol || n <= 128) ? 1 : 0; }
   void mirror_dense(const std::vector<GemmJob>& jobs) {
     for (const GemmJob& j : jobs)
-      if (j.lower) {
+      if (j.lower && j.m > 128) {
         const int nbm = blocks(j.m, 32);
         LAUNCH(k_mirror_lower, dim3(nbm * (nbm + 1) / 2), dim3(256), 0, j.C, j.m, j.ldc);
       }
@@ -1406,7 +1454,7 @@ struct Engine {
   DBuf<double> atmGD;  // ATM pair sums (2 x the atom-pair buffer)
   DBuf<int> dcls, dnfrac;
   std::vector<int> dense_pfp;  // per molecule: its last eigenproblem was refined (W = P F P / 2)
-  int dense_steps = 4;         // mixed, dense: refinement steps at most per SCC iteration
+  int dense_steps = 4;         // mixed, dense: refinement steps at most per SCC iteration (AXTB_HIP_DENSE_STEPS)
   // mixed, dense: a refinement is accepted once its step's largest correction max |e_ij| (over the
   // occupied-empty pairs and the orthogonality terms) is <= dense_accept; the error it leaves is of
   // second order in that correction (AXTB_HIP_DENSE_ACCEPT)
@@ -1510,7 +1558,7 @@ struct Engine {
                          pool.p, stride, vsh.p, vat.p, vdp.p, vqp.p, F.p);
       // generalized eigenproblem in the Loewdin basis
       gemm(a1);
-      for (auto& j : a2) j.lower = dense(j.n) ? 1 : 0;  // A = B^T (F B) is symmetric
+      for (auto& j : a2) j.lower = sym_lower(j.n, dense(j.n));  // A = B^T (F B) is symmetric
       gemm(a2);
       mirror_dense(a2);
       if (o.precision == 1) {
@@ -1582,7 +1630,7 @@ struct Engine {
         a4.push_back(p);
       }
       gemm(a5);
-      for (auto& j : a4) j.lower = dense(j.n) ? 1 : 0;
+      for (auto& j : a4) j.lower = sym_lower(j.n, dense(j.n));
       gemm(a4);  // P = C diag(f) C^T
       mirror_dense(a4);
       LAUNCHW(sbytes, k_mulliken_cols, dim3(blocks(NO, 4)), dim3(256), 0, b, active.p, pool.p, stride, P.p,
@@ -1694,7 +1742,7 @@ struct Engine {
     HIP_OK(hipStreamSynchronize(st));
     for (int i = 0; i < naos(); i++) hf[i] *= hw[i];
     HIP_OK(hipMemcpyAsync(fwork.p, hf.data(), sizeof(double) * naos(), hipMemcpyHostToDevice, st));
-    for (auto& j : gW) j.lower = dense(j.n) ? 1 : 0;
+    for (auto& j : gW) j.lower = sym_lower(j.n, dense(j.n));
     gemm(gW);
     mirror_dense(gW);
     // gradient = repulsion + three-body (setup) + pair terms + CN chains

@@ -309,7 +309,25 @@ __global__ __launch_bounds__(512) void k_gemm_small(const GemmJob* jobs) {
   __shared__ double As[2][BK][SP];
   __shared__ double Bs[2][BK][SP];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int mb = (J.m + 15) >> 4, nbk = (J.n + 15) >> 4, nblk = mb * nbk;
+  const int mb = (J.m + 15) >> 4, nbk = (J.n + 15) >> 4;
+  // lower (square C): only the blocks on or below the diagonal, the others written as their mirror
+  const bool low = J.lower && J.m == J.n && J.beta == 0.0;
+  const int nblk = low ? mb * (mb + 1) / 2 : mb * nbk;
+  int bi_q[8], bj_q[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const int bq = w + 8 * q;
+    if (low) {
+      int bi = (int)((sqrt(8.0 * bq + 1.0) - 1.0) * 0.5);
+      while ((bi + 1) * (bi + 2) / 2 <= bq) bi++;
+      while (bi * (bi + 1) / 2 > bq) bi--;
+      bi_q[q] = bi;
+      bj_q[q] = bq - bi * (bi + 1) / 2;
+    } else {
+      bi_q[q] = bq % mb;
+      bj_q[q] = bq / mb;
+    }
+  }
   // element q (of 4) of this thread's share of a k-step: (row / col within 128, k within 16)
   auto a_rk = [&](int q, int& r, int& k) {
     if (TA == 0) {
@@ -386,8 +404,7 @@ __global__ __launch_bounds__(512) void k_gemm_small(const GemmJob* jobs) {
       for (int q = 0; q < 8; q++) {
         const int bq = w + 8 * q;
         if (bq < nblk) {
-          const int bi = bq % mb, bj = bq / mb;
-          const double a = As[buf][kl][16 * bi + li], b = Bs[buf][kl][16 * bj + li];
+          const double a = As[buf][kl][16 * bi_q[q] + li], b = Bs[buf][kl][16 * bj_q[q] + li];
           acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[q], 0, 0, 0);
         }
       }
@@ -399,14 +416,16 @@ __global__ __launch_bounds__(512) void k_gemm_small(const GemmJob* jobs) {
   for (int q = 0; q < 8; q++) {
     const int bq = w + 8 * q;
     if (bq >= nblk) continue;
-    const int bi = bq % mb, bj = bq / mb;
+    const int bi = bi_q[q], bj = bj_q[q];
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       const int gr = 16 * bi + (lane >> 4) + 4 * r, gc = 16 * bj + (lane & 15);
       if (gr < J.m && gc < J.n) {
         double* c = &J.C[gr + (long long)gc * J.ldc];
         const double v = J.alpha * acc[q][r];
-        *c = J.beta == 0.0 ? v : v + J.beta * *c;
+        const double o = J.beta == 0.0 ? v : v + J.beta * *c;
+        *c = o;
+        if (low && bi != bj) J.C[gc + (long long)gr * J.ldc] = o;  // the mirror above the diagonal
       }
     }
   }
@@ -1066,12 +1085,13 @@ __global__ __launch_bounds__(512) void k_eeq_lu(Batch b, const int* mlist, const
   if (tid == 0 && bad) eeq_failed[m] = 1;
 }
 
-// dynamic LDS bytes of k_jacobi for an n x n matrix in elements of `es` bytes
-inline size_t jacobi_lds(int n, int mode, size_t es = 8) {
-  // A packed (lower triangle) except in the FP32 LDS mode 2, which keeps it full (k_jacobi's PACK)
-  const bool pack = !(mode == 2 && es == 4);
+// dynamic LDS bytes of k_jacobi<mode, T, pack> for an n x n matrix in elements of `es` bytes
+inline size_t jacobi_lds_p(int n, int mode, size_t es, bool pack) {
   const size_t np = (n + 1) / 2, na = pack ? (size_t)n * (n + 1) / 2 : (size_t)n * n;
   return (size_t)(mode >= 1 ? 1 : 0) * na * es + (size_t)(mode == 2 ? 1 : 0) * n * n * es + np * 4 * es + np * 4 * 4;
 }
+// the layout of the instantiations: A packed except in the FP32 LDS mode 2 (k_jacobi2_f32, where the
+// packed index arithmetic costs more than two workgroups per CU gain for n = 100-115, measured)
+inline size_t jacobi_lds(int n, int mode, size_t es = 8) { return jacobi_lds_p(n, mode, es, !(mode == 2 && es == 4)); }
 
 }  // namespace axtb
