@@ -64,12 +64,28 @@ __global__ __launch_bounds__(256) void k_trd_col(double* A, int n, int i0, int j
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int r = i + blockIdx.x * TRD_CB + lane;  // global row
   const long long ln = n;
+  const int mp = n - i;  // rows of the previous reflector and of its symv
+  const int nch = j > 0 ? (mp + TRD_CH - 1) / TRD_CH : 0;
+  // this row's loads first (none depends on the reductions below, so they overlap them): its
+  // share of the symv partials, V(r, c) and W(r, c) for c = wv + 4k < jj, and A(r, ip), A(r, i)
+  constexpr int KR = (TRD_NB + 3) / 4;
+  double ps = 0.0, vr[KR], wr_[KR], a_ip = 0.0, a_i = 0.0;
+  if (r < n) {
+    for (int ch = wv; ch < nch; ch += 4) ps += part[ch * ln + r];
+#pragma unroll
+    for (int k = 0; k < KR; k++) {
+      const int c = wv + 4 * k;
+      vr[k] = c < jj ? A[(i0 + c) * ln + r] : 0.0;
+      wr_[k] = c < jj ? W[c * ln + r] : 0.0;
+    }
+    if (wv == 0) {
+      if (j > 0) a_ip = A[ip * ln + r];
+      a_i = A[i * ln + r];
+    }
+  }
   double t = 0.0, alpha = 0.0, scal = 1.0;
-  int nch = 0;
   if (j > 0) {
-    const int mp = n - i;  // rows of the previous reflector and of its symv
     const int ndb = (mp + TRD_DR - 1) / TRD_DR;
-    nch = (mp + TRD_CH - 1) / TRD_CH;
     const int ntile = nch * (nch + 1) / 2;
     double beta;
     trd_reflector(axv[ip], trd_sumsq(sq_prev, n, ip, lane), t, beta, scal);
@@ -89,8 +105,7 @@ __global__ __launch_bounds__(256) void k_trd_col(double* A, int n, int i0, int j
     double va = 0.0;
     for (int k = threadIdx.x; k < ntile; k += 256) va += vpart[k];
     va = block_sum(va, rs);  // its barriers also publish y1, y2
-    double yy = 0.0;
-    for (int c = 0; c < jj; c++) yy += y1[c] * y2[c];
+    const double yy = wave_sum(lane < jj ? y1[lane] * y2[lane] : 0.0);
     alpha = -0.5 * t * t * (va - 2.0 * yy);
   }
   if (!finalize_only) {
@@ -113,15 +128,16 @@ __global__ __launch_bounds__(256) void k_trd_col(double* A, int n, int i0, int j
     }
     __syncthreads();
   }
-  // per wavefront: its share of p_r, of sw_r = V y1 + W y2 and of u_r = V wi + W vi
-  double ps = 0.0, sw = 0.0, u = 0.0;
+  // per wavefront: its share of sw_r = V y1 + W y2 and of u_r = V wi + W vi
+  double sw = 0.0, u = 0.0;
   if (r < n) {
-    for (int ch = wv; ch < nch; ch += 4) ps += part[ch * ln + r];
-#pragma unroll 4
-    for (int c = wv; c < jj; c += 4) {
-      const double v = A[(i0 + c) * ln + r], w = W[c * ln + r];
-      sw += v * y1[c] + w * y2[c];
-      u += v * wi[c] + w * vi[c];
+#pragma unroll
+    for (int k = 0; k < KR; k++) {
+      const int c = wv + 4 * k;
+      if (c < jj) {
+        sw += vr[k] * y1[c] + wr_[k] * y2[c];
+        u += vr[k] * wi[c] + wr_[k] * vi[c];
+      }
     }
   }
   red[0][wv][lane] = ps;
@@ -135,14 +151,14 @@ __global__ __launch_bounds__(256) void k_trd_col(double* A, int n, int i0, int j
     const double swt = red[1][0][lane] + red[1][1][lane] + red[1][2][lane] + red[1][3][lane];
     double ut = red[2][0][lane] + red[2][1][lane] + red[2][2][lane] + red[2][3][lane];
     if (j > 0) {
-      const double vp = r == i ? 1.0 : A[ip * ln + r] * scal;
+      const double vp = r == i ? 1.0 : a_ip * scal;
       A[ip * ln + r] = vp;
       const double wr = t * (p - swt) + alpha * vp;
       W[jj * ln + r] = wr;
       if (!finalize_only) ut += vp * wi[jj] + wr * vi[jj];
     }
     if (!finalize_only) {
-      a = A[i * ln + r] - ut;
+      a = a_i - ut;
       A[i * ln + r] = a;
       if (r == i + 1) axv[i] = a;
     }
