@@ -267,14 +267,9 @@ struct Engine {
     if (const char* v = std::getenv("AXTB_HIP_OA_STEPS")) oa_steps = std::min(4, std::max(1, std::atoi(v)));
     if (const char* v = std::getenv("AXTB_HIP_DENSE_FULL")) dense_full = std::max(1, std::atoi(v));
     if (const char* v = std::getenv("AXTB_HIP_DENSE_REFINE")) dense_refine = std::atoi(v) != 0;
-    if (const char* v = std::getenv("AXTB_HIP_DENSE_STEPS")) dense_steps = std::min(12, std::max(1, std::atoi(v)));
-    if (const char* v = std::getenv("AXTB_HIP_DENSE_OVMIX")) dense_ovmix = std::min(0.5, std::max(0.01, std::atof(v)));
     if (const char* v = std::getenv("AXTB_HIP_GEMM_SMALL")) gemm_small = std::atoi(v) != 0;
     if (const char* v = std::getenv("AXTB_HIP_EXACT_LEVELS")) exact_levels = std::atoi(v) != 0;
-    if (const char* v = std::getenv("AXTB_HIP_F32_TOL")) f32_tol = (float)std::min(1e3, std::max(1.0, std::atof(v)));
-    if (const char* v = std::getenv("AXTB_HIP_F32_SWEEPS")) f32_sweeps = std::min(60, std::max(1, std::atoi(v)));
     if (const char* v = std::getenv("AXTB_HIP_DENSE_ACCEPT")) dense_accept = std::min(1e-3, std::max(1e-12, std::atof(v)));
-    if (const char* v = std::getenv("AXTB_HIP_GEMM128_BATCH")) gemm128_batch = std::atoi(v) != 0;
   }
   bool dense(int n) const { return n > dense_min; }
 
@@ -358,7 +353,7 @@ struct Engine {
     // large products on 128 x 128 tiles (one transpose pair per launch; two-term products always),
     // the other batches of per-molecule products on 64 x 64 tiles
     if (two || (same && (long long)mm * nn >= 256LL * 256 && jobs.size() <= 8) ||
-        (same && (gemm128_batch || in_refine) && std::max(mm, nn) > 64)) {
+        (same && in_refine && std::max(mm, nn) > 64)) {
       if (!same) throw DeviceError("gemm: mixed transposes in one launch");
       gj.upload(jobs, st, &stage);
       dim3 grid(blocks(nn, BT), blocks(mm, BT), (unsigned)jobs.size());
@@ -1454,13 +1449,13 @@ struct Engine {
   DBuf<double> atmGD;  // ATM pair sums (2 x the atom-pair buffer)
   DBuf<int> dcls, dnfrac;
   std::vector<int> dense_pfp;  // per molecule: its last eigenproblem was refined (W = P F P / 2)
-  int dense_steps = 4;         // mixed, dense: refinement steps at most per SCC iteration (AXTB_HIP_DENSE_STEPS)
+  int dense_steps = 4;         // mixed, dense: refinement steps at most per SCC iteration
   // mixed, dense: a refinement is accepted once its step's largest correction max |e_ij| (over the
   // occupied-empty pairs and the orthogonality terms) is <= dense_accept; the error it leaves is of
   // second order in that correction (AXTB_HIP_DENSE_ACCEPT)
   double dense_accept = 1e-4;
   // mixed, dense: largest first-order mixing of an occupied-empty pair the refinement takes on; a
-  // larger one ends it (a full solve follows) (AXTB_HIP_DENSE_OVMIX)
+  // larger one ends it (a full solve follows)
   double dense_ovmix = 0.05;
   int dense_full = 1;  // mixed: SCC iterations solved in full before refinement (AXTB_HIP_DENSE_FULL)
   // mixed: refine the previous iteration's vectors of dense-solver molecules instead of a full
@@ -1469,12 +1464,9 @@ struct Engine {
   // occupation is 0 or 2, the condition for refining); the orbital energies are the Rayleigh
   // quotients of the refined vectors
   bool dense_refine = true;
-  bool gemm128_batch = false;  // batches of per-molecule products on the 128-tile kernel (AXTB_HIP_GEMM128_BATCH)
   bool gemm_small = true;      // products with m, n <= 128 on the one-workgroup-per-job kernel (AXTB_HIP_GEMM_SMALL)
   bool in_refine = false;      // the refinement's batches always take the 128-tile kernel (measured faster)
   int oa_steps = 2;  // refinement steps per SCC iteration (AXTB_HIP_OA_STEPS)
-  int f32_sweeps = 60;  // mixed: FP32 Jacobi sweeps at most after the first iteration (AXTB_HIP_F32_SWEEPS)
-  float f32_tol = 1.0f;  // mixed: scale of the FP32 Jacobi rotation thresholds (AXTB_HIP_F32_TOL)
 
   // ------------------------------------------------------------------------------ SCC loop
   std::vector<int> hactive;
@@ -1564,8 +1556,7 @@ struct Engine {
       if (o.precision == 1) {
         // mixed: Jacobi in FP32, then one FP64 Ogita-Aishima step U' = U (I + E) from
         // S = U^T A U and G = U^T U (U in Z, A in Am; S -> F, G -> Vw, I + E -> Wm, U' -> T)
-        // after the first iteration A is nearly diagonal: f32_sweeps sweeps, the refinement does the rest
-        eigh(ea, true, it == 1 ? 60 : f32_sweeps, f32_tol);
+        eigh(ea, true);
         std::vector<char> in_t;
         std::vector<EigJob> redo = refine(ea, in_t);
         eigh(redo);  // not converged by the refinement: FP64 Jacobi (vectors in Z)
