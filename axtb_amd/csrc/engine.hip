@@ -447,23 +447,10 @@ struct Engine {
         double flops = 0.0;  // dense eigendecompositions with all vectors
         for (size_t q = b0; q < b1; q++) flops += 4.0 / 3.0 * js[q].n * (double)js[q].n * js[q].n + 2.0 * js[q].n * (double)js[q].n * js[q].n;
         ProfScope ps_(&prof, s, nm, flops);
-        auto go = [&](const void* fn, auto kern) {
-          HIP_OK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-          hipLaunchKernelGGL(kern, dim3((unsigned)(b1 - b0)), dim3(512), shm, s, buf.p + b0, max_sweeps, stat + b0,
-                             tol_scale);
-        };
-        if (fp32) {
-          if (mode == 2)
-            go((const void*)k_jacobi2_f32, k_jacobi2_f32);
-          else
-            go((const void*)k_jacobi1_f32, k_jacobi1_f32);
-        } else if (mode == 2) {
-          go((const void*)k_jacobi2, k_jacobi2);
-        } else if (mode == 1) {
-          go((const void*)k_jacobi1, k_jacobi1);
-        } else {
-          go((const void*)k_jacobi0, k_jacobi0);
-        }
+        const JacobiKernel kern = jacobi_kernel(mode, fp32);
+        HIP_OK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+        hipLaunchKernelGGL(kern, dim3((unsigned)(b1 - b0)), dim3(512), shm, s, buf.p + b0, max_sweeps, stat + b0,
+                           tol_scale);
         b0 = b1;
       }
       HIP_OK(hipGetLastError());

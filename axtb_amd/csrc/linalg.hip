@@ -480,6 +480,7 @@ __device__ __forceinline__ void tour_pair(int r, int k, int nn, int& p, int& q) 
 // and |a_pq| > ABS max|A|; in FP32 the stop is at the rounding level (the result is refined in
 // FP64 afterwards).
 template <typename T> struct JacobiTol;
+
 template <> struct JacobiTol<double> {
   static constexpr double REL = 1e-15, ABS = 1e-17;
 };
@@ -487,11 +488,18 @@ template <> struct JacobiTol<float> {
   static constexpr float REL = 2.5e-7f, ABS = 2.5e-7f;
 };
 
+// rotation of one pair of a round: one 16-byte LDS record in FP32 (32 bytes in FP64)
+template <typename T> struct alignas(16) JRot {
+  T c, s;
+  int p, q;
+};
+
 template <int MODE, typename T, bool PACK>
 __global__ __launch_bounds__(512) void k_jacobi(const EigJob* jobs, int max_sweeps, int* status, float tol_scale) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T* lds = (T*)lds_raw;
   __shared__ int rotated;
+  __shared__ int rflag[2][2];  // per rotation buffer and rotation wavefront: any pair rotated
   __shared__ T anorm;
   const EigJob J = jobs[blockIdx.x];
   const int n = J.n, tid = threadIdx.x, nt = blockDim.x;
@@ -499,29 +507,42 @@ __global__ __launch_bounds__(512) void k_jacobi(const EigJob* jobs, int max_swee
   const int nn = n + (n & 1), np = nn / 2;
   const int lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
   // dynamic LDS: [A (MODE >= 1; PACK: the lower triangle packed by rows, element (r, c), r >= c,
-  // at r (r + 1) / 2 + c; else full n*n)] [V (n*n, MODE 2)] [c, s per pair: 2 buffers] [p, q per
-  // pair: 2 buffers]; MODE 0 works on J.A (full, both triangles kept) and J.V in place (FP64 only)
-  const int nl = PACK ? n * (n + 1) / 2 : n * n;
+  // at r (r + 1) / 2 + c; else full storage, columns ld = n | 1 apart)] [V (MODE 2; columns ldv
+  // apart)] [JRot per pair: 2 buffers, 16-byte aligned]; MODE 0 works on J.A (full storage) and
+  // J.V in place (FP64 only). Only the lower triangle of A is kept up to date: element (r, c) is
+  // read and written at (max, min). An odd column distance in LDS keeps the column-strided
+  // accesses of a round (consecutive threads on consecutive pairs) on distinct banks.
+  const int ld = (PACK || MODE == 0) ? n : (n | 1);
+  const int ldv = (MODE == 2 && !PACK) ? ld : n;
+  const int nl = PACK ? n * (n + 1) / 2 : n * ld;
   T* A = MODE >= 1 ? lds : (T*)J.A;
   T* V = MODE == 2 ? lds + nl : (T*)J.V;
-  T* csb0 = lds + (MODE >= 1 ? nl : 0) + (MODE == 2 ? n * n : 0);
-  int* pqb0 = (int*)(csb0 + 4 * np);
+  const int rot_off = ((((MODE >= 1 ? nl : 0) + (MODE == 2 ? n * ldv : 0)) * (int)sizeof(T) + 15) / 16) * 16;
+  JRot<T>* rot0 = (JRot<T>*)(lds_raw + rot_off);
   // element (r, c) of the symmetric A
   auto aix = [&](int r, int c) -> int {
-    if (!PACK) return r + c * n;
     const int hi = max(r, c), lo = min(r, c);
-    return hi * (hi + 1) / 2 + lo;
+    return PACK ? hi * (hi + 1) / 2 + lo : hi + lo * ld;
   };
-  if (MODE >= 1)
-    for (int e = tid; e < n * n; e += nt) {
-      const int r = e % n, c = e / n;
-      if (!PACK)
-        A[e] = (T)J.A[e];
-      else if (r >= c)
-        A[r * (r + 1) / 2 + c] = (T)J.A[e];
+  if (MODE >= 1) {
+    if (PACK) {
+      for (int e = tid; e < n * n; e += nt) {
+        const int r = e % n, c = e / n;
+        if (r >= c) A[r * (r + 1) / 2 + c] = (T)J.A[e];
+      }
+    } else {
+      for (int e = tid; e < n * ld; e += nt) {  // the padding row (n even) zeroed
+        const int r = e % ld, c = e / ld;
+        A[e] = r < n ? (T)J.A[r + c * n] : (T)0;
+      }
     }
-  for (int e = tid; e < n * n; e += nt) V[e] = (e % n == e / n) ? (T)1 : (T)0;
+  }
+  for (int e = tid; e < n * ldv; e += nt) {
+    const int r = e % ldv, c = e / ldv;
+    V[e] = r == c ? (T)1 : (T)0;
+  }
   if (tid == 0) rotated = 0;
+  if (tid < 4) rflag[tid >> 1][tid & 1] = 0;
   __syncthreads();
   if (tid < 64) {
     T s = 0;
@@ -532,6 +553,10 @@ __global__ __launch_bounds__(512) void k_jacobi(const EigJob* jobs, int max_swee
   __syncthreads();
   const T an = anorm;
   const T REL = JacobiTol<T>::REL * (T)tol_scale, ABS = JacobiTol<T>::ABS * (T)tol_scale;
+  // the rotation test of a pair (the same at the start of a sweep and in a round)
+  auto need = [&](T app, T aqq, T apq) {
+    return fabs(apq) > REL * sqrt(fabs(app) * fabs(aqq)) && fabs(apq) > ABS * an;
+  };
   const int nblk = np * (np + 1) / 2;
   // this thread's lower 2 x 2 blocks (k1 >= k2; block b = k1 (k1 + 1) / 2 + k2, b = tid + j nt),
   // the same in every round
@@ -560,27 +585,31 @@ __global__ __launch_bounds__(512) void k_jacobi(const EigJob* jobs, int max_swee
   // first threads compute the rotations
   const int nvw = MODE == 2 ? max(1, nw - (np + 63) / 64) : nw;
   const int rt = MODE == 2 ? nt - 1 - tid : tid;
+  // Pair k is on thread rt == k (np <= 100 < nt, so pairs k < 64 share one wavefront and the
+  // others a second); each wavefront that holds pairs sets rflag[buf][k / 64] to whether any of its
+  // pairs rotates, so that a round with no rotation skips its updates. Called by every thread.
   auto rotations = [&](int r, int buf) {
-    T* csb = csb0 + buf * 2 * np;
-    int* pqb = pqb0 + buf * 2 * np;
-    for (int k = rt; k < np; k += nt) {
+    JRot<T>* rb = rot0 + buf * np;
+    const int k = rt;
+    bool rotd = false;
+    if (k < np) {
       int p, q;
       tour_pair(r, k, nn, p, q);
       T c = 1, s = 0;
       if (q < n) {
         const T app = A[aix(p, p)], aqq = A[aix(q, q)], apq = A[aix(p, q)];
-        if (fabs(apq) > REL * sqrt(fabs(app) * fabs(aqq)) && fabs(apq) > ABS * an) {
+        if (need(app, aqq, apq)) {
           const T tau = (aqq - app) / (2 * apq);
           const T t = (tau >= 0 ? (T)1 : (T)-1) / (fabs(tau) + sqrt(1 + tau * tau));
           c = 1 / sqrt(1 + t * t);
           s = t * c;
         }
       }
-      csb[2 * k] = c;
-      csb[2 * k + 1] = s;
-      pqb[2 * k] = p;
-      pqb[2 * k + 1] = q;
+      rb[k] = JRot<T>{c, s, p, q};
+      rotd = s != 0;
     }
+    const bool any = __ballot(rotd) != 0ull;
+    if (k < np && (k & 63) == 0) rflag[buf][k >> 6] = any ? 1 : 0;
   };
   int sweep = 0;
   bool conv = n == 1;
@@ -592,8 +621,7 @@ __global__ __launch_bounds__(512) void k_jacobi(const EigJob* jobs, int max_swee
       for (int e = tid; e < n * n; e += nt) {
         if (i < j) {
           const T aij = A[aix(i, j)];
-          if (fabs(aij) > REL * sqrt(fabs(A[aix(i, i)]) * fabs(A[aix(j, j)])) && fabs(aij) > ABS * an)
-            rotated = 1;
+          if (need(A[aix(i, i)], A[aix(j, j)], aij)) rotated = 1;
         }
         i += dii;
         j += dj;
@@ -612,67 +640,58 @@ __global__ __launch_bounds__(512) void k_jacobi(const EigJob* jobs, int max_swee
     __syncthreads();
     for (int r = 0; r < nn - 1; r++) {
       const int buf = r & 1;
-      const T* csb = csb0 + buf * 2 * np;
-      const int* pqb = pqb0 + buf * 2 * np;
-      // A <- J^T A J on the lower 2 x 2 blocks, mirrored
+      const JRot<T>* rb = rot0 + buf * np;
+      const bool any = (rflag[buf][0] | rflag[buf][1]) != 0;  // a round with no rotation: no updates
+      // A <- J^T A J on the lower 2 x 2 blocks of this thread. A block whose pairs both rotate by
+      // identity is skipped; the four elements are read unconditionally (an index q >= n, odd n, is
+      // clamped to a valid element whose value is never used or written).
 #pragma unroll
       for (int jb = 0; jb < MAXB; jb++) {
-        if (tid + jb * nt >= nblk) break;
-        const int k1 = bk1[jb], k2 = bk2[jb];
-        const T s1 = csb[2 * k1 + 1], s2 = csb[2 * k2 + 1];
+        if (!any || tid + jb * nt >= nblk) break;
+        const JRot<T> r1 = rb[bk1[jb]], r2 = rb[bk2[jb]];
+        const T s1 = r1.s, s2 = r2.s;
         if (s1 == 0 && s2 == 0) continue;
-        const int p1 = pqb[2 * k1], q1 = pqb[2 * k1 + 1], p2 = pqb[2 * k2], q2 = pqb[2 * k2 + 1];
-        const T c1 = csb[2 * k1], c2 = csb[2 * k2];
-        const bool vq1 = q1 < n, vq2 = q2 < n;
-        const T b00 = A[aix(p1, p2)];
-        const T b01 = vq2 ? A[aix(p1, q2)] : (T)0;
-        const T b10 = vq1 ? A[aix(q1, p2)] : (T)0;
-        const T b11 = (vq1 && vq2) ? A[aix(q1, q2)] : (T)0;
+        const T c1 = r1.c, c2 = r2.c;
+        const int p1 = r1.p, p2 = r2.p;
+        const bool vq1 = r1.q < n, vq2 = r2.q < n;
+        const int q1 = vq1 ? r1.q : p1, q2 = vq2 ? r2.q : p2;
+        const int i00 = aix(p1, p2), i01 = aix(p1, q2), i10 = aix(q1, p2), i11 = aix(q1, q2);
+        const T e00 = A[i00], e01 = A[i01], e10 = A[i10], e11 = A[i11];
+        const T b00 = e00, b01 = vq2 ? e01 : (T)0, b10 = vq1 ? e10 : (T)0, b11 = (vq1 && vq2) ? e11 : (T)0;
         const T t00 = c2 * b00 - s2 * b01, t01 = s2 * b00 + c2 * b01;
         const T t10 = c2 * b10 - s2 * b11, t11 = s2 * b10 + c2 * b11;
         const T n00 = c1 * t00 - s1 * t10, n01 = c1 * t01 - s1 * t11;
         const T n10 = s1 * t00 + c1 * t10, n11 = s1 * t01 + c1 * t11;
-        if (k1 == k2) {
-          A[aix(p1, p1)] = n00;
+        if (p1 == p2) {  // diagonal block (k1 == k2)
+          A[i00] = n00;
           if (vq1) {
-            A[aix(q1, q1)] = n11;
-            A[aix(p1, q1)] = 0;
-            if (!PACK) A[q1 + p1 * n] = 0;
+            A[i11] = n11;
+            A[i01] = 0;
           }
           continue;
         }
-        // (r, c) and (c, r) are one element in the packed lower triangle; the full storage keeps
-        // the mirror too
-        A[aix(p1, p2)] = n00;
-        if (!PACK) A[p2 + p1 * n] = n00;
-        if (vq2) {
-          A[aix(p1, q2)] = n01;
-          if (!PACK) A[q2 + p1 * n] = n01;
-        }
-        if (vq1) {
-          A[aix(q1, p2)] = n10;
-          if (!PACK) A[p2 + q1 * n] = n10;
-        }
-        if (vq1 && vq2) {
-          A[aix(q1, q2)] = n11;
-          if (!PACK) A[q2 + q1 * n] = n11;
-        }
+        A[i00] = n00;
+        if (vq2) A[i01] = n01;
+        if (vq1) A[i10] = n10;
+        if (vq1 && vq2) A[i11] = n11;
       }
       __syncthreads();
       // the next round's rotations (from the updated A) beside this round's V <- V J
       if (r + 1 < nn - 1) rotations(r + 1, buf ^ 1);
-      // V: one pair per wavefront at a time, the lanes over consecutive rows; the last wavefronts
-      // compute the next round's rotations instead
-      for (int k = wv; k < np && wv < nvw; k += nvw) {
-        const T s = csb[2 * k + 1];
-        if (s == 0) continue;
-        const T c = csb[2 * k];
-        T* vp = V + pqb[2 * k] * n;
-        T* vq = V + pqb[2 * k + 1] * n;
+      // V: one pair per wavefront at a time (pairs wv, wv + nvw, ...), the lanes over consecutive
+      // rows; a pair that does not rotate costs one record read. With V in LDS the last wavefronts
+      // compute the next round's rotations instead.
+      for (int k = wv; any && k < np && wv < nvw; k += nvw) {
+        const JRot<T> rk = rb[k];
+        const T sn = rk.s;
+        if (sn == 0) continue;
+        const T c = rk.c;
+        T* vp = V + rk.p * ldv;
+        T* vq = V + rk.q * ldv;
         for (int i = lane; i < n; i += 64) {
           const T a = vp[i], b = vq[i];
-          vp[i] = c * a - s * b;
-          vq[i] = s * a + c * b;
+          vp[i] = c * a - sn * b;
+          vq[i] = sn * a + c * b;
         }
       }
       __syncthreads();
@@ -687,7 +706,7 @@ __global__ __launch_bounds__(512) void k_jacobi(const EigJob* jobs, int max_swee
       if (dj < di || (dj == di && j < i)) rank++;
     }
     J.w[rank] = (double)di;
-    for (int r = 0; r < n; r++) J.Z[r + (long long)rank * n] = (double)V[r + (long long)i * n];
+    for (int r = 0; r < n; r++) J.Z[r + (long long)rank * n] = (double)V[r + (long long)i * ldv];
   }
   if (tid == 0) status[blockIdx.x] = conv ? sweep : -sweep;
 }
@@ -698,11 +717,14 @@ template __global__ void k_jacobi<2, double, true>(const EigJob*, int, int*, flo
 template __global__ void k_jacobi<1, float, true>(const EigJob*, int, int*, float);
 template __global__ void k_jacobi<2, float, false>(const EigJob*, int, int*, float);
 
-static constexpr auto k_jacobi0 = k_jacobi<0, double, false>;
-static constexpr auto k_jacobi1 = k_jacobi<1, double, true>;
-static constexpr auto k_jacobi2 = k_jacobi<2, double, true>;
-static constexpr auto k_jacobi1_f32 = k_jacobi<1, float, true>;
-static constexpr auto k_jacobi2_f32 = k_jacobi<2, float, false>;
+using JacobiKernel = void (*)(const EigJob*, int, int*, float);
+// the instantiation for (mode, arithmetic); A packed except in the FP32 LDS mode 2
+inline JacobiKernel jacobi_kernel(int mode, bool fp32) {
+  if (fp32) return mode == 2 ? k_jacobi<2, float, false> : k_jacobi<1, float, true>;
+  if (mode == 2) return k_jacobi<2, double, true>;
+  if (mode == 1) return k_jacobi<1, double, true>;
+  return k_jacobi<0, double, false>;
+}
 
 // One Ogita-Aishima refinement step of approximate eigenvectors U of A (FP64), given
 // S = U^T A U and G = U^T U (R = I - G): lambda_i = s_ii / g_ii;
@@ -1087,8 +1109,10 @@ __global__ __launch_bounds__(512) void k_eeq_lu(Batch b, const int* mlist, const
 
 // dynamic LDS bytes of k_jacobi<mode, T, pack> for an n x n matrix in elements of `es` bytes
 inline size_t jacobi_lds_p(int n, int mode, size_t es, bool pack) {
-  const size_t np = (n + 1) / 2, na = pack ? (size_t)n * (n + 1) / 2 : (size_t)n * n;
-  return (size_t)(mode >= 1 ? 1 : 0) * na * es + (size_t)(mode == 2 ? 1 : 0) * n * n * es + np * 4 * es + np * 4 * 4;
+  const size_t ld = pack ? n : (n | 1), ldv = (mode == 2 && !pack) ? ld : n;
+  const size_t np = (n + 1) / 2, na = pack ? (size_t)n * (n + 1) / 2 : (size_t)n * ld;
+  const size_t mats = (size_t)(mode >= 1 ? 1 : 0) * na * es + (size_t)(mode == 2 ? 1 : 0) * n * ldv * es;
+  return (mats + 15) / 16 * 16 + 2 * np * (es == 4 ? 16 : 32);  // + two buffers of JRot records
 }
 // the layout of the instantiations: A packed except in the FP32 LDS mode 2 (k_jacobi2_f32, where the
 // packed index arithmetic costs more than two workgroups per CU gain for n = 100-115, measured)
