@@ -423,17 +423,26 @@ __global__ void k_d4_weights(Batch b, const int* active, const double* cn_d4, co
 // C6 of a pair and its derivative contractions (model.rs:607-627)
 __device__ __forceinline__ void pair_c6(const Batch& b, int i, int j, const double* gw, const double* dgw, double& c6,
                                         double& di, double& dj) {
+  // the references of j over a fixed 7 (every load in bounds, unused ones masked to zero), so
+  // that a row's loads are issued together; the sums run in the order of the ni x nj loops (the
+  // added zeros leave them unchanged)
   const double* tab = &b.c6ref[((long long)b.a.sp[i] * b.nsp + b.a.sp[j]) * 49];
-  int ni = b.a.d4_nref[i], nj = b.a.d4_nref[j];
+  const int ni = b.a.d4_nref[i], nj = b.a.d4_nref[j];
+  double wj[MAXREF], dwj[MAXREF];
+#pragma unroll
+  for (int r = 0; r < MAXREF; r++) {
+    wj[r] = r < nj ? gw[MAXREF * j + r] : 0.0;
+    dwj[r] = r < nj ? dgw[MAXREF * j + r] : 0.0;
+  }
   c6 = di = dj = 0.0;
   for (int ri = 0; ri < ni; ri++) {
-    double wi = gw[MAXREF * i + ri], dwi = dgw[MAXREF * i + ri];
-    for (int rj = 0; rj < nj; rj++) {
-      double r = tab[ri * 7 + rj];
-      double wj = gw[MAXREF * j + rj];
-      c6 += wi * wj * r;
-      di += dwi * wj * r;
-      dj += wi * dgw[MAXREF * j + rj] * r;
+    const double wi = gw[MAXREF * i + ri], dwi = dgw[MAXREF * i + ri];
+#pragma unroll
+    for (int rj = 0; rj < MAXREF; rj++) {
+      const double r = rj < nj ? tab[ri * 7 + rj] : 0.0;
+      c6 += wi * wj[rj] * r;
+      di += dwi * wj[rj] * r;
+      dj += wi * dwj[rj] * r;
     }
   }
 }
@@ -691,7 +700,7 @@ __device__ __forceinline__ void aes_g(double r1, double mri, double mrj, double&
 
 // Atom-resolved potentials of one wavefront per atom: AES (coulomb.rs:407-502) into vat, vdp,
 // vqp and the D4 dE/dq (disp.rs:420-455) into vat.
-__global__ void k_atom_pot(Batch b, const int* active, const double* mrad, const double* qat, const double* dp,
+__global__ __launch_bounds__(256) void k_atom_pot(Batch b, const int* active, const double* mrad, const double* qat, const double* dp,
                            const double* qp, const double* gw, const double* dgwdq, double* vat, double* vdp,
                            double* vqp) {
   int a = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
@@ -929,7 +938,7 @@ __global__ void k_mulliken_atoms(Batch b, const int* active, const double* cols,
 // Electronic energy per atom of the output moments (xtb.rs:1058-1146): ES2 from gq = gamma q_out,
 // ES3, AES (coulomb.rs:668-731), D4 two-body (disp.rs:380-399), -TS / nat and the band energy.
 // One wavefront per atom.
-__global__ void k_energy_atoms(Batch b, const int* active, const double* mrad, const double* gq, const double* qsh,
+__global__ __launch_bounds__(256) void k_energy_atoms(Batch b, const int* active, const double* mrad, const double* gq, const double* qsh,
                                const double* qat, const double* dp, const double* qp, const double* gw,
                                const double* dgw, const double* eband, const double* st, double* eel, double* eterm) {
   int a = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);

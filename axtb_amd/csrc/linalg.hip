@@ -304,11 +304,12 @@ __global__ void k_gemm_reduce(const double* P, int splits, int m, int n, double*
 // the store, as in k_gemm128). Few registers per thread: two workgroups share a CU.
 constexpr int SP = 132;  // padded LDS row of the small-matrix kernel
 template <int TA, int TB>
-__global__ __launch_bounds__(512) void k_gemm_small(const GemmJob* jobs) {
+__global__ __launch_bounds__(512, 4) void k_gemm_small(const GemmJob* jobs) {
   const GemmJob J = jobs[blockIdx.x];
   __shared__ double As[2][BK][SP];
   __shared__ double Bs[2][BK][SP];
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  // the wavefront index as a scalar: the block lists and the block branches are then uniform
+  const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int mb = (J.m + 15) >> 4, nbk = (J.n + 15) >> 4;
   // lower (square C): only the blocks on or below the diagonal, the others written as their mirror
   const bool low = J.lower && J.m == J.n && J.beta == 0.0;
