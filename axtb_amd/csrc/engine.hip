@@ -266,6 +266,8 @@ struct Engine {
     if (const char* v = std::getenv("AXTB_HIP_DENSE_MIN")) dense_min = std::min(200, std::max(1, std::atoi(v)));
     if (const char* v = std::getenv("AXTB_HIP_OA_STEPS")) oa_steps = std::min(4, std::max(1, std::atoi(v)));
     if (const char* v = std::getenv("AXTB_HIP_DENSE_FULL")) dense_full = std::max(1, std::atoi(v));
+    if (const char* v = std::getenv("AXTB_HIP_SPLIT_REFINE")) split_refine = std::atoi(v) != 0;
+    if (const char* v = std::getenv("AXTB_HIP_SPLIT_RESIDUAL")) split_residual = std::atof(v);
     if (const char* v = std::getenv("AXTB_HIP_DENSE_REFINE")) dense_refine = std::atoi(v) != 0;
     if (const char* v = std::getenv("AXTB_HIP_GEMM_SMALL")) gemm_small = std::atoi(v) != 0;
     if (const char* v = std::getenv("AXTB_HIP_EXACT_LEVELS")) exact_levels = std::atoi(v) != 0;
@@ -1314,9 +1316,13 @@ struct Engine {
   // FP64 Jacobi (returned list). Scratch: C (A U), F (S), Vw (G), Wm (I + E).
   // Steps alternate U between Z and T; round 0 is oa_steps steps (rounded up to a pair), round 1 a
   // pair, each accepted at max |e_ij| <= 1e-7. in_t[q] tells whether job q's vectors ended in T (odd
-  // number of steps) or in Z. Returns the jobs not accepted.
-  std::vector<EigJob> refine(const std::vector<EigJob>& ea, std::vector<char>& in_t) {
-    std::vector<EigJob> cur = ea, redo;
+  // number of steps) or in Z. Returns the positions in `ea` of the jobs not accepted.
+  std::vector<int> refine(const std::vector<EigJob>& ea, std::vector<char>& in_t, const int* cls = nullptr) {
+    // cls: the split of the orbitals into occupied / empty (k_oa_step), refined from U = I: the
+    // first step is k_oa_step alone (S = A, G = I, I + E is U), the later rounds are single steps,
+    // and a job not converged within its four steps is returned for a full solve
+    std::vector<EigJob> cur = ea;
+    std::vector<int> redo;
     in_t.assign(ea.size(), 0);
     if (ea.empty()) return redo;
     for (const EigJob& j : ea)
@@ -1327,10 +1333,15 @@ struct Engine {
     long steps_total = 0;
     in_refine = true;
     int gstep = 0;  // steps done by every job still in `cur`
-    for (int round = 0; round < 2 && !cur.empty(); round++) {
-      const int nsteps = round == 0 ? std::max(2, oa_steps + (oa_steps & 1)) : 2;
-      const double accept = 1e-7;
+    // the split refinement: rounds of 2 (U = I and one product step), 1 and 1 steps
+    const int nrounds = cls ? 3 : 2;
+    for (int round = 0; round < nrounds && !cur.empty(); round++) {
+      const int nsteps = round == 0 ? std::max(2, oa_steps + (oa_steps & 1)) : (cls ? 1 : 2);
+      // the split refinement converges linearly at the rate (coupling inside the blocks) / gap, so
+      // it is accepted at a tighter bound
+      const double accept = cls ? split_accept : 1e-7;
       for (int step = 0; step < nsteps; step++, gstep++) {
+        const bool ident = cls && gstep == 0;
         std::vector<GemmJob> gau, gsg, gu;
         std::vector<EigJob> oj;
         for (size_t q = 0; q < cur.size(); q++) {
@@ -1339,6 +1350,10 @@ struct Engine {
           const int n = j.n;
           double* U = (gstep % 2 == 0 ? Z.p : T.p) + o0;
           double* out = (gstep % 2 == 0 ? T.p : Z.p) + o0;
+          if (ident) {  // S = A, G = I: I + E goes straight to the output
+            oj.push_back(EigJob{n, j.A, nullptr, j.w, out});
+            continue;
+          }
           gau.push_back(GemmJob{n, n, n, 0, 0, j.A, U, C.p + o0, nullptr, n, n, n, 1.0, 0.0});        // A U
           gsg.push_back(GemmJob{n, n, n, 1, 0, U, C.p + o0, F.p + o0, nullptr, n, n, n, 1.0, 0.0});   // S
           gsg.push_back(GemmJob{n, n, n, 1, 0, U, U, Vw.p + o0, nullptr, n, n, n, 1.0, 0.0});         // G
@@ -1352,9 +1367,9 @@ struct Engine {
         gemm(gau);
         gemm(gsg);
         LAUNCH(k_oa_step, dim3(oj.size()), dim3(256), 0, (const EigJob*)oa_jobs.p, oa_emax.p, oa_delta.p,
-               (const int*)oa_slot.p, gstep == 0 ? 1 : 0);
+               (const int*)oa_slot.p, gstep == 0 ? 1 : 0, ident ? 1 : 0, cls, (const double*)w.p, oa_ovmix);
         gemm(gu);
-        steps_total += (long)cur.size();
+        if (!ident) steps_total += (long)cur.size();
       }
       std::vector<double> h(cur.size());
       HIP_OK(hipMemcpyAsync(h.data(), oa_emax.p, sizeof(double) * cur.size(), hipMemcpyDeviceToHost, st));
@@ -1373,9 +1388,11 @@ struct Engine {
       idx.swap(nidx);
     }
     in_refine = false;
-    prof.count("refine_steps", (long)ea.size(), (double)steps_total);
-    if (!cur.empty()) prof.count("refine_fallbacks", (long)cur.size(), (double)cur.size());
-    return cur;
+    const char* nsteps = cls ? "refine_steps_split" : "refine_steps";
+    const char* nfall = cls ? "refine_split_fallbacks" : "refine_fallbacks";
+    prof.count(nsteps, (long)ea.size(), (double)steps_total);
+    if (!idx.empty()) prof.count(nfall, (long)idx.size(), (double)idx.size());
+    return idx;
   }
 
   // The same refinement for one large matrix (the dense-solver molecules): the GEMMs on 128-tile
@@ -1454,6 +1471,16 @@ struct Engine {
   bool gemm_small = true;      // products with m, n <= 128 on the one-workgroup-per-job kernel (AXTB_HIP_GEMM_SMALL)
   bool in_refine = false;      // the refinement's batches always take the 128-tile kernel (measured faster)
   int oa_steps = 2;  // refinement steps per SCC iteration (AXTB_HIP_OA_STEPS)
+  // mixed, batched molecules: SCC iterations after the first refine the previous orbitals for the
+  // occupied / empty split when the occupations are all 0 or 2 (AXTB_HIP_SPLIT_REFINE)
+  bool split_refine = true;
+  double oa_ovmix = 0.05;  // largest first-order occupied / empty mixing a split step corrects
+  double split_accept = 1e-9;  // max |e_ij| of the last split step
+  // the split refinement starts once the previous iteration's residual is below this: the coupling
+  // inside the blocks (the change of F since the last full solve) sets its convergence rate
+  double split_residual = 1e-2;
+  DBuf<int> dmlist, ocls, onfrac;
+  DBuf<BlockCopy> dbcopy;
 
   // ------------------------------------------------------------------------------ SCC loop
   std::vector<int> hactive;
@@ -1506,6 +1533,7 @@ struct Engine {
     for (it = 1; it <= o.max_iter; it++) {
       std::vector<GemmJob> a1, a2, a3;
       std::vector<EigJob> ea;
+      std::vector<int> eam;  // molecule of each entry of ea
       for (int m = 0; m < NM; m++)
         if (hactive[m]) {
           a1.push_back(g1[m]);
@@ -1513,6 +1541,7 @@ struct Engine {
           if (!isdense[m]) {
             a3.push_back(g3v[m]);
             ea.push_back(ejobs[m]);
+            eam.push_back(m);
           }
         }
       if (a1.empty()) break;
@@ -1543,13 +1572,52 @@ struct Engine {
       if (o.precision == 1) {
         // mixed: Jacobi in FP32, then one FP64 Ogita-Aishima step U' = U (I + E) from
         // S = U^T A U and G = U^T U (U in Z, A in Am; S -> F, G -> Vw, I + E -> Wm, U' -> T)
-        eigh(ea, true);
-        std::vector<char> in_t;
-        std::vector<EigJob> redo = refine(ea, in_t);
-        eigh(redo);  // not converged by the refinement: FP64 Jacobi (vectors in Z)
+        // Molecules whose previous occupations were all 0 or 2: the previous orbitals (A nearly
+        // block diagonal) refined for the occupied / empty split alone, from U = I, no Jacobi; P
+        // needs no more, W is then P F P / 2 and the orbital energies come from the blocks at the
+        // end. The others, and those not converged that way: the FP32 Jacobi and the full
+        // refinement.
+        std::vector<int> qs, qf;  // positions in ea (and a3: the same molecules in the same order)
+        if (it > 2 && split_refine && !ea.empty()) {
+          std::vector<int> ml;
+          for (int m = 0; m < NM; m++)
+            if (hactive[m] && !isdense[m]) ml.push_back(m);
+          dmlist.upload(ml, st, &stage);
+          ocls.ensure(NO);
+          onfrac.ensure(NM);
+          LAUNCH(k_occ_class_batch, dim3(ml.size()), dim3(256), 0, b, (const int*)dmlist.p, (const double*)focc.p,
+                 ocls.p, onfrac.p);
+          std::vector<int> hnf(NM);
+          std::vector<double> hm8(8 * (size_t)NM);
+          HIP_OK(hipMemcpyAsync(hnf.data(), onfrac.p, sizeof(int) * NM, hipMemcpyDeviceToHost, st));
+          HIP_OK(hipMemcpyAsync(hm8.data(), mst.p, sizeof(double) * 8 * NM, hipMemcpyDeviceToHost, st));
+          HIP_OK(hipStreamSynchronize(st));
+          for (size_t q = 0; q < ml.size(); q++)
+            (hnf[ml[q]] == 0 && hm8[8 * ml[q] + 2] < split_residual ? qs : qf).push_back((int)q);
+        } else {
+          for (size_t q = 0; q < ea.size(); q++) qf.push_back((int)q);
+        }
+        std::vector<char> in_t(ea.size(), 0), ts, tf;
+        std::vector<EigJob> es, ef;
+        for (int q : qs) es.push_back(ea[q]);
+        for (int f : refine(es, ts, ocls.p)) qf.push_back(qs[f]);
+        std::vector<char> ok_split(ea.size(), 0);
+        for (size_t t = 0; t < qs.size(); t++) ok_split[qs[t]] = 1;
+        for (int q : qf) ok_split[q] = 0;
+        for (size_t t = 0; t < qs.size(); t++)
+          if (ok_split[qs[t]]) in_t[qs[t]] = ts[t];
+        std::sort(qf.begin(), qf.end());
+        for (int q : qf) ef.push_back(ea[q]);
+        eigh(ef, true);
+        std::vector<EigJob> redo;
+        for (int f : refine(ef, tf)) redo.push_back(ef[f]);
+        for (size_t t = 0; t < qf.size(); t++) in_t[qf[t]] = tf[t];
+        eigh(redo);  // not converged by the refinement: FP64 Jacobi (vectors in Z; in_t is 0 for them)
         // C = B U from wherever U ended (a3 and ea list the same molecules in the same order)
-        for (size_t q = 0; q < ea.size(); q++)
+        for (size_t q = 0; q < ea.size(); q++) {
           if (in_t[q]) a3[q].B = T.p + (ea[q].Z - Z.p);
+          dense_pfp[eam[q]] = ok_split[q];
+        }
       } else {
         eigh(ea);
       }
@@ -1640,12 +1708,40 @@ struct Engine {
     // refinement step (in F; the first kocc columns are the occupied ones, and the coupling between
     // the blocks is below the acceptance bound, so the blocks' eigenvalues are A's to second order
     // in it), each by an eigenvalue-only FP64 dense solve
-    if (exact_levels && mixed) {
-      std::vector<int> hk(NM);
-      HIP_OK(hipMemcpyAsync(hk.data(), kocc.p, sizeof(int) * NM, hipMemcpyDeviceToHost, st));
-      HIP_OK(hipStreamSynchronize(st));
+    bool anypfp = false;
+    for (int m = 0; m < NM; m++) anypfp |= dense_pfp[m] != 0;
+    if (!mixed || !anypfp) return;
+    std::vector<int> hk(NM);
+    HIP_OK(hipMemcpyAsync(hk.data(), kocc.p, sizeof(int) * NM, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    // batched molecules refined for the occupied / empty split in their last iteration (always, as
+    // their orbital energies are Rayleigh quotients otherwise): both blocks of S of every one of
+    // them, by the batched FP64 Jacobi (blocks in T, vectors to Vw, work in C: free now)
+    {
+      std::vector<EigJob> bj;
+      std::vector<BlockCopy> bc;
       for (int m = 0; m < NM; m++) {
-        if (!dense_pfp[m]) continue;
+        if (!dense_pfp[m] || dense(mols[m].nao)) continue;
+        const int n = mols[m].nao, k = hk[m];
+        const long long o0 = mols[m].mat0, a0 = mols[m].ao0;
+        long long off = 0;
+        for (int blk = 0; blk < 2; blk++) {
+          const int s0 = blk == 0 ? 0 : k, nb = blk == 0 ? k : n - k;
+          if (nb <= 0) continue;
+          bc.push_back(BlockCopy{F.p + o0 + s0 + (long long)s0 * n, T.p + o0 + off, nb, n});
+          bj.push_back(EigJob{nb, T.p + o0 + off, C.p + o0 + off, w.p + a0 + s0, Vw.p + o0 + off});
+          off += (long long)nb * nb;
+        }
+      }
+      if (!bc.empty()) {
+        dbcopy.upload(bc, st, &stage);
+        LAUNCH(k_copy_blocks, dim3(bc.size()), dim3(256), 0, (const BlockCopy*)dbcopy.p);
+      }
+      eigh(bj);
+    }
+    if (exact_levels) {
+      for (int m = 0; m < NM; m++) {
+        if (!dense_pfp[m] || !dense(mols[m].nao)) continue;
         const int n = mols[m].nao, k = hk[m];
         const long long o0 = mols[m].mat0, a0 = mols[m].ao0;
         for (int blk = 0; blk < 2; blk++) {
@@ -1657,6 +1753,15 @@ struct Engine {
         }
       }
     }
+    // the Fermi level and -TS of the refined molecules from their exact levels (the occupations
+    // stay 0 or 2), into the iteration record
+    std::vector<int> hp(NM, 0);
+    for (int m = 0; m < NM; m++) hp[m] = dense_pfp[m] && (exact_levels || !dense(mols[m].nao)) ? 1 : 0;
+    dmlist.upload(hp, st, &stage);
+    LAUNCH(k_fermi, dim3(NM), dim3(256), 0, b, (const int*)dmlist.p, w.p, focc.p, fwork.p, fst.p, kocc.p, status.p);
+    HIP_OK(hipMemcpy2DAsync(mst.p + 3, 8 * sizeof(double), fst.p, 2 * sizeof(double), 2 * sizeof(double), NM,
+                            hipMemcpyDeviceToDevice, st));
+    HIP_OK(hipStreamSynchronize(st));
   }
   bool exact_levels = true;  // mixed, dense: exact orbital energies at the end (AXTB_HIP_EXACT_LEVELS)
 

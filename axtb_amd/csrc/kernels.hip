@@ -787,12 +787,36 @@ __global__ void k_fock(Batch b, const int* active, const double* pool, long long
 // st: per molecule {efermi, ts} into dst[2*m]; kocc (last index with focc > 1e-16, +1) into kocc.
 __device__ double fermi_fill(const double* w, int n, double nel, double kt, double* occ, double* red, double& ef_out,
                              bool& ok) {
-  // aufbau start (occupations.rs:55-67)
+  // aufbau start (occupations.rs:55-67): midway between w[i0] and w[i1]; taken as max w[0..i0]
+  // and min w[i1..n-1], which are those for sorted w and the highest occupied and lowest empty
+  // level for orbitals refined for the occupied / empty split (occupied first, each block in the
+  // order of the previous iteration)
   int full = (int)floor(nel);
   double rem = fmod(nel, 1.0);
   int homo = full + (rem > 0.5 ? 1 : 0);
   int i0 = max(homo, 1) - 1, i1 = min(homo + 1, n) - 1;
-  double ef = 0.5 * (w[i0] + w[i1]), ef_used = ef;
+  double whi = -INFINITY, wlo = INFINITY;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    if (i <= i0) whi = fmax(whi, w[i]);
+    if (i >= i1) wlo = fmin(wlo, w[i]);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    whi = fmax(whi, __shfl_xor(whi, o, 64));
+    wlo = fmin(wlo, __shfl_xor(wlo, o, 64));
+  }
+  __shared__ double wsel[2][16];
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) {
+    wsel[0][threadIdx.x >> 6] = whi;
+    wsel[1][threadIdx.x >> 6] = wlo;
+  }
+  __syncthreads();
+  for (int q = 0; q < (int)(blockDim.x >> 6); q++) {
+    whi = fmax(whi, wsel[0][q]);
+    wlo = fmin(wlo, wsel[1][q]);
+  }
+  __syncthreads();
+  double ef = 0.5 * (whi + wlo), ef_used = ef;
   for (int it = 0; it < 200; it++) {
     ef_used = ef;
     double tot = 0.0, dtot = 0.0;
@@ -875,6 +899,27 @@ __global__ void k_fermi(Batch b, const int* active, const double* w, double* foc
     st[2 * m + 1] = ts;
     if (!ok) atomicOr(&status[m], 2);
   }
+}
+
+// Occupation class of every orbital of the molecules in mlist (one workgroup each): 0 if
+// f > 2 - 1e-12, 1 if f < 1e-12, else 2, at the orbital's AO index; nfrac[m] counts the class-2
+// orbitals of molecule m.
+__global__ void k_occ_class_batch(Batch b, const int* mlist, const double* focc, int* cls, int* nfrac) {
+  __shared__ int cnt;
+  const int m = mlist[blockIdx.x];
+  const Mol M = b.mol[m];
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  int c2 = 0;
+  for (int i = threadIdx.x; i < M.nao; i += blockDim.x) {
+    const double f = focc[M.ao0 + i];
+    const int c = f > 2.0 - 1e-12 ? 0 : (f < 1e-12 ? 1 : 2);
+    cls[M.ao0 + i] = c;
+    c2 += c == 2 ? 1 : 0;
+  }
+  if (c2) atomicAdd(&cnt, c2);
+  __syncthreads();
+  if (threadIdx.x == 0) nfrac[m] = cnt;
 }
 
 // Mulliken column sums (scf/mulliken.rs:19-59) and the band energy Tr(H0 P) per column: one

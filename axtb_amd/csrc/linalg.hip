@@ -735,17 +735,25 @@ inline JacobiKernel jacobi_kernel(int mode, bool fp32) {
 // and writes I + E (so that U' = U (I + E)) and lambda into w (T. Ogita, K. Aishima, Japan J. Indust.
 // Appl. Math. 35 (2018) 1007-1035, Algorithm 1). One workgroup per matrix; job.A = S, job.V = G,
 // job.Z = I + E, job.w = lambda.
+// Options: ident: U = I (S = A in J.A, G = I, not read), so that I + E is the refined U itself;
+// cls (with wbase): occupation classes of the orbitals, orbital i of job J at cls[J.w - wbase + i],
+// 0 occupied, 1 empty (no fractional ones): a pair within a class is only orthonormalised, a pair
+// across is corrected if its first-order mixing is below ovmix (else emax = 1: not converged).
 __global__ __launch_bounds__(256) void k_oa_step(const EigJob* jobs, double* emax, double* delta_keep, const int* slot,
-                                                 int first) {
+                                                 int first, int ident, const int* cls, const double* wbase,
+                                                 double ovmix) {
   __shared__ double red[16];
   __shared__ double lam[200];  // batched eigenproblems have n <= 200
+  __shared__ signed char cl[200];
   const EigJob J = jobs[blockIdx.x];
   const int n = J.n, tid = threadIdx.x, nt = blockDim.x;
   const double* S = J.A;
   const double* G = J.V;
+  auto gel = [&](int e, bool diag) { return ident ? (diag ? 1.0 : 0.0) : G[e]; };
   for (int i = tid; i < n; i += nt) {
-    lam[i] = S[i + (long long)i * n] / G[i + (long long)i * n];
+    lam[i] = S[i + (long long)i * n] / gel(i + i * n, true);
     J.w[i] = lam[i];
+    cl[i] = cls ? (signed char)cls[(J.w - wbase) + i] : (signed char)-1;
   }
   __syncthreads();
   // element e = i + j n of this thread, advanced without division
@@ -755,7 +763,7 @@ __global__ __launch_bounds__(256) void k_oa_step(const EigJob* jobs, double* ema
     int i = tid % n, j = tid / n;
     for (int e = tid; e < n * n; e += nt) {
       const double sd = S[e] - (i == j ? lam[i] : 0.0);
-      const double rd = (i == j ? 1.0 : 0.0) - G[e];
+      const double rd = (i == j ? 1.0 : 0.0) - gel(e, i == j);
       so += sd * sd;
       ro += rd * rd;
       i += di;
@@ -791,14 +799,25 @@ __global__ __launch_bounds__(256) void k_oa_step(const EigJob* jobs, double* ema
   {
     int i = tid % n, j = tid / n;
     for (int e = tid; e < n * n; e += nt) {
-      const double g = G[e];
+      const double g = gel(e, i == j);
       double v;
       if (i == j) {
         v = 0.5 * (1.0 - g);
       } else {
         const double gap = lam[j] - lam[i];
         const double num = S[e] - lam[j] * g;
-        v = (fabs(gap) > delta && fabs(num) < 0.05 * fabs(gap)) ? num / gap : -0.5 * g;
+        if (cl[i] >= 0) {
+          if (cl[i] == cl[j]) {
+            v = -0.5 * g;
+          } else if (fabs(num) < ovmix * fabs(gap)) {
+            v = num / gap;
+          } else {
+            v = -0.5 * g;
+            emx = 1.0;
+          }
+        } else {
+          v = (fabs(gap) > delta && fabs(num) < 0.05 * fabs(gap)) ? num / gap : -0.5 * g;
+        }
       }
       emx = fmax(emx, fabs(v));
       J.Z[e] = (i == j ? 1.0 : 0.0) + v;
@@ -924,6 +943,19 @@ __global__ __launch_bounds__(256) void k_oa_big_e(const EigJob J, const double* 
     for (int w = 0; w < (int)(blockDim.x >> 6); w++) m = fmax(m, red[w]);
     emax_part[blockIdx.x] = m;
   }
+}
+
+// Square sub-blocks copied out of column-major matrices, one workgroup per block:
+// dst[r + c nb] = src[r + c ld] for r, c < nb.
+struct BlockCopy {
+  const double* src;
+  double* dst;
+  int nb, ld;
+};
+__global__ void k_copy_blocks(const BlockCopy* jobs) {
+  const BlockCopy J = jobs[blockIdx.x];
+  const int nn = J.nb * J.nb;
+  for (int e = threadIdx.x; e < nn; e += blockDim.x) J.dst[e] = J.src[(e % J.nb) + (long long)(e / J.nb) * J.ld];
 }
 
 // Occupation class of each orbital column for k_oa_big_e: 0 if f > 2 - 1e-12, 1 if f < 1e-12,

@@ -195,6 +195,9 @@ def main():
         fp64_rate = len(mols) * args.steps / runs["fp64"][0]
         cfg_extra.update({"fp64_value": fp64_rate, "mixed_speedup_vs_fp64": value / fp64_rate,
                           "mixed_vs_fp64_max_abs_all_molecules": dict(zip(names, d)),
+                          "mixed_vs_fp64_max_abs_efermi": max((abs(r["efermi"] - o["efermi"])
+                                                               for r, o in zip(runs["mixed"][2], runs["fp64"][2])
+                                                               if not r["status"] and not o["status"]), default=0.0),
                           "mixed_within_1e-6_Eh": d[0] <= 1e-6})
     line = {
         "metric": "E+grad/s (GFN2-xTB energy + analytic gradient)",
