@@ -750,6 +750,17 @@ struct Engine {
       mols.clear();
       return;
     }
+    // largest molecules first: every per-molecule launch (products, eigensolvers, refinement) then
+    // deals its longest jobs out first and ends on the short ones
+    {
+      std::vector<int> nao(in.size(), 0);
+      for (int i : use)
+        for (int a = 0; a < in[i].nat; a++) {
+          const int iz = in[i].z[a] - 1;
+          for (int s = 0; s < GFN2_NSHELL[iz]; s++) nao[i] += 2 * GFN2_ANG_SHELL[iz][s] + 1;
+        }
+      std::stable_sort(use.begin(), use.end(), [&](int a, int b) { return nao[a] > nao[b]; });
+    }
     build(in, use, o, out);
     opts_ = o;
   }
@@ -1063,6 +1074,15 @@ struct Engine {
     cols.ensure(11 * (size_t)NO);
     gam.ensure(tot_shp);
     pool.ensure(11 * (size_t)tot_mat);
+    {
+      std::vector<int2> tl;  // 256-element tiles of the per-molecule matrices (k_fock)
+      for (int m = 0; m < (int)mols.size(); m++) {
+        const long long nn = (long long)mols[m].nao * mols[m].nao;
+        for (long long e = 0; e < nn; e += 256) tl.push_back(make_int2(m, (int)e));
+      }
+      mat_tiles.upload(tl, st);
+      n_mat_tiles = (int)tl.size();
+    }
     for (auto* d : {&X, &Bm, &F, &T, &Am, &Vw, &Z, &C, &P, &Wm}) d->ensure(tot_mat);
     atmP.ensure(8 * (size_t)tot_pair);
     for (auto* d : {&eeqA, &eeqV, &eeqZ}) d->ensure(tot_eq);
@@ -1481,6 +1501,10 @@ struct Engine {
   double split_residual = 1e-2;
   DBuf<int> dmlist, ocls, onfrac;
   DBuf<BlockCopy> dbcopy;
+  DBuf<int2> mat_tiles, h0list;
+  DBuf<int> h0cnt;
+  DBuf<long long> h0goff;
+  int n_mat_tiles = 0;
 
   // ------------------------------------------------------------------------------ SCC loop
   std::vector<int> hactive;
@@ -1562,7 +1586,7 @@ struct Engine {
       double sbytes = 0.0;
       for (int m = 0; m < NM; m++)
         if (hactive[m]) sbytes += 12.0 * 8.0 * mols[m].nao * (double)mols[m].nao;
-      LAUNCHW(sbytes, k_fock, dim3(blocks((long long)naomax * naomax, 256), NM), dim3(256), 0, b, active.p,
+      LAUNCHW(sbytes, k_fock, dim3(n_mat_tiles), dim3(256), 0, b, active.p, (const int2*)mat_tiles.p,
                          pool.p, stride, vsh.p, vat.p, vdp.p, vqp.p, F.p);
       // generalized eigenproblem in the Loewdin basis
       gemm(a1);
@@ -1808,7 +1832,7 @@ struct Engine {
     if (anypfp) {
       DBuf<int> pfd;
       pfd.upload(hpfp, st);
-      LAUNCH(k_fock, dim3(blocks((long long)naomax * naomax, 256), NM), dim3(256), 0, b, (const int*)pfd.p, pool.p,
+      LAUNCH(k_fock, dim3(n_mat_tiles), dim3(256), 0, b, (const int*)pfd.p, (const int2*)mat_tiles.p, pool.p,
              stride, vsh.p, vat.p, vdp.p, vqp.p, F.p);
       gemm(gF);
       for (auto& j : gW2) j.lower = 1;  // W = P (F P) / 2 is symmetric
@@ -1840,8 +1864,38 @@ struct Engine {
                        gw.p, dgw.p, 1);
     int natmax = 0;
     for (auto& M : mols) natmax = std::max(natmax, M.nat);
-    LAUNCH(k_h0_grad, dim3(blocks((long long)natmax * natmax, 64), NM), dim3(64), 0, b, (const int*)okd.p, P.p, Wm.p,
-           se.p, dsedcn.p, vsh.p, vat.p, vdp.p, vqp.p, atmP.p, (long long)tot_pair);
+    // H0 / overlap terms per atom pair by shell-pair type (hg in atmP: the ATM table is used up);
+    // the lists of group g (molecules m % H0_GROUPS == g) hold up to the group's sum of nat^2
+    {
+      std::vector<long long> go(H0_GROUPS, 0);
+      for (int m = 0; m < NM; m++)
+        if (m % H0_GROUPS + 1 < H0_GROUPS) go[m % H0_GROUPS + 1] += (long long)mols[m].nat * mols[m].nat;
+      for (int g = 1; g < H0_GROUPS; g++) go[g] += go[g - 1];
+      h0goff.upload(go, st, &stage);
+    }
+    h0cnt.ensure(4 * H0_GROUPS);
+    h0list.ensure(4 * (size_t)tot_pair);
+    LAUNCH(k_zero_u32, dim3(1), dim3(256), 0, (unsigned*)h0cnt.p, 4 * H0_GROUPS);
+    LAUNCH(k_h0_keys, dim3(blocks((long long)natmax * natmax, 64), NM), dim3(64), 0, b, (const int*)okd.p, h0cnt.p,
+           h0list.p, (long long)tot_pair, (const long long*)h0goff.p);
+    fill(atmP.p, 5 * (long long)tot_pair);
+    std::vector<int> hcnt(4 * H0_GROUPS);
+    HIP_OK(hipMemcpyAsync(hcnt.data(), h0cnt.p, sizeof(int) * hcnt.size(), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    auto h0key = [&](int k, auto kern) {
+      int mx = 0;
+      for (int g = 0; g < H0_GROUPS; g++) mx = std::max(mx, hcnt[k * H0_GROUPS + g]);
+      if (mx <= 0) return;
+      ProfScope ps_(&prof, st, "k_h0_grad", 0.0);
+      hipLaunchKernelGGL(kern, dim3(blocks(mx, 64), H0_GROUPS), dim3(64), 0, st, b,
+                         (const int2*)h0list.p + (long long)k * tot_pair, (const int*)h0cnt.p + k * H0_GROUPS,
+                         (const long long*)h0goff.p, P.p, Wm.p, se.p, dsedcn.p, vsh.p, vat.p, vdp.p, vqp.p, atmP.p,
+                         (long long)tot_pair);
+    };
+    h0key(0, k_h0_grad_key<0, 0>);
+    h0key(1, k_h0_grad_key<0, 1>);
+    h0key(2, k_h0_grad_key<1, 0>);
+    h0key(3, k_h0_grad_key<1, 1>);
     LAUNCH(k_pair_grad, dim3(blocks(NA, 4)), dim3(256), 0, b, (const int*)okd.p, pool.p, stride, P.p, Wm.p, se.p,
            dsedcn.p, vsh.p, vat.p, vdp.p, vqp.p, qsh_o.p, qat_o.p, dp_o.p, qp_o.p, mrad.p, dmrdcn.p, gw.p, dgw.p,
            dcn3.p, (const double*)atmP.p, (long long)tot_pair, grad.p, dedcn.p, dedcn4.p);

@@ -751,12 +751,16 @@ __global__ __launch_bounds__(256) void k_atom_pot(Batch b, const int* active, co
 // Fock matrix F = H0 - 1/2 (B + B^T), B(mu,nu) = S v(nu) + D.vd(A(nu)) + Q.vq(A(nu))
 // (scf/fock.rs:22-104). B(nu,mu) is rebuilt from the (mu,nu) integrals by shifting their moments
 // to the atom of mu, so every read is coalesced. `out` has the layout of the integral pools.
-__global__ void k_fock(Batch b, const int* active, const double* pool, long long stride, const double* vsh,
-                       const double* vat, const double* vdp, const double* vqp, double* F) {
-  int m = blockIdx.y;
+// Launched over the 256-element tiles of all matrices: tile t covers elements tiles[t].y .. + 255 of
+// molecule tiles[t].x (no workgroup beyond a molecule's n^2 elements).
+__global__ __launch_bounds__(256) void k_fock(Batch b, const int* active, const int2* tiles, const double* pool,
+                                              long long stride, const double* vsh, const double* vat,
+                                              const double* vdp, const double* vqp, double* F) {
+  const int2 tl = tiles[blockIdx.x];
+  const int m = tl.x;
   if (active && !active[m]) return;
   const Mol M = b.mol[m];
-  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long idx = (long long)tl.y + threadIdx.x;
   long long nn = (long long)M.nao * M.nao;
   if (idx >= nn) return;
   int mu = (int)(idx % M.nao), nu = (int)(idx / M.nao);
@@ -1372,47 +1376,101 @@ __device__ __forceinline__ void h0_grad_shell(const Batch& b, const Mol& M, int 
   dcnj += dsedcn[sj] * hs * ps;
 }
 
-__global__ __launch_bounds__(64) void k_h0_grad(Batch b, const int* okm, const double* P, const double* W, const double* se,
-                          const double* dsedcn, const double* vsh, const double* vat, const double* vdp,
-                          const double* vqp, double* hg, long long hstride) {
-  int m = blockIdx.y;
-  if (okm && !okm[m]) return;
+// The H0 / overlap part of the gradient per atom pair li > lj into hg (stride hstride: dE/dR_ij (3),
+// dE/dCN_i, dE/dCN_j), split by shell-pair type so that a wavefront runs one type: k_h0_keys lists,
+// per key 2 l_i + l_j (s and p shells), the atom pairs that hold such a shell pair after the
+// integral screening, in H0_GROUPS lists per key (molecule m in group m % H0_GROUPS, whose list
+// starts at goff[group] within the key's region of cap entries; slots taken per wavefront, counted
+// in cnt[key * H0_GROUPS + group]: many counters, little contention); k_h0_grad_key<LI, LJ>
+// (grid.y = group) then adds that shell pair's terms to hg (zeroed before the first key).
+// Launched in key order, every atom pair accumulates its shell pairs in the order of a loop over
+// the shells of i, then of j (s before p on each atom).
+constexpr int H0_GROUPS = 64;
+__global__ __launch_bounds__(64) void k_h0_keys(Batch b, const int* okm, int* cnt, int2* list, long long cap,
+                                                const long long* goff) {
+  const int m = blockIdx.y;
   const Mol M = b.mol[m];
-  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (long long)M.nat * M.nat) return;
-  int li = (int)(idx / M.nat), lj = (int)(idx % M.nat);
-  if (li <= lj) return;
+  const long long idx = (long long)blockIdx.x * 64 + threadIdx.x;
+  int mask = 0;
+  if ((!okm || okm[m]) && idx < (long long)M.nat * M.nat) {
+    const int li = (int)(idx / M.nat), lj = (int)(idx % M.nat);
+    if (li > lj) {
+      const int iat = M.at0 + li, jat = M.at0 + lj;
+      const double* xi = &b.a.xyz[3 * iat];
+      const double* xj = &b.a.xyz[3 * jat];
+      const double v[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+      const double r2 = dot3(v, v);
+      double amin_i = 1e300, amin_j = 1e300;
+      for (int s = b.a.ish[iat]; s < b.a.ish[iat] + b.a.nsh[iat]; s++) amin_i = fmin(amin_i, b.s.minalpha[s]);
+      for (int s = b.a.ish[jat]; s < b.a.ish[jat] + b.a.nsh[jat]; s++) amin_j = fmin(amin_j, b.s.minalpha[s]);
+      if (!(amin_i * amin_j * r2 / (amin_i + amin_j) > b.intcut * (1.0 + 1e-10)) && r2 >= 2.220446049250313e-16)
+        for (int si = b.a.ish[iat]; si < b.a.ish[iat] + b.a.nsh[iat]; si++)
+          for (int sj = b.a.ish[jat]; sj < b.a.ish[jat] + b.a.nsh[jat]; sj++) {
+            const double a1 = b.s.minalpha[si], a2 = b.s.minalpha[sj];
+            if (a1 * a2 * r2 / (a1 + a2) > b.intcut * (1.0 + 1e-10)) continue;
+            mask |= 1 << (2 * b.s.l[si] + b.s.l[sj]);
+          }
+    }
+  }
+  const int lane = threadIdx.x;
+  for (int k = 0; k < 4; k++) {
+    const bool has = (mask >> k) & 1;
+    const unsigned long long bal = __ballot(has);
+    if (!bal) continue;
+    int base = 0;
+    const int g = m % H0_GROUPS;
+    if (lane == 0) base = atomicAdd(&cnt[k * H0_GROUPS + g], (int)__popcll(bal));
+    base = __shfl(base, 0, 64);
+    if (has) list[k * cap + goff[g] + base + __popcll(bal & ((1ull << lane) - 1ull))] = make_int2(m, (int)idx);
+  }
+}
+
+template <int LI, int LJ>
+__global__ __launch_bounds__(64) void k_h0_grad_key(Batch b, const int2* list, const int* count,
+                                                    const long long* goff, const double* P, const double* W,
+                                                    const double* se, const double* dsedcn, const double* vsh,
+                                                    const double* vat, const double* vdp, const double* vqp,
+                                                    double* hg, long long hstride) {
+  const int g = blockIdx.y, t = blockIdx.x * 64 + threadIdx.x;
+  if (t >= count[g]) return;
+  const int2 it = list[goff[g] + t];
+  const int m = it.x;
+  const Mol M = b.mol[m];
+  const int idx = it.y, li = idx / M.nat, lj = idx % M.nat;
   const int iat = M.at0 + li, jat = M.at0 + lj;
   const double* xi = &b.a.xyz[3 * iat];
   const double* xj = &b.a.xyz[3 * jat];
   double v[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
-  double r2 = dot3(v, v);
-  double dg[3] = {0, 0, 0}, dcni = 0.0, dcnj = 0.0;
-  double amin_i = 1e300, amin_j = 1e300;
-  for (int s = b.a.ish[iat]; s < b.a.ish[iat] + b.a.nsh[iat]; s++) amin_i = fmin(amin_i, b.s.minalpha[s]);
-  for (int s = b.a.ish[jat]; s < b.a.ish[jat] + b.a.nsh[jat]; s++) amin_j = fmin(amin_j, b.s.minalpha[s]);
-  if (!(amin_i * amin_j * r2 / (amin_i + amin_j) > b.intcut * (1.0 + 1e-10)) && r2 >= 2.220446049250313e-16) {
-    double rr = sqrt(sqrt(r2) / (b.a.rad[jat] + b.a.rad[iat]));
-    for (int si = b.a.ish[iat]; si < b.a.ish[iat] + b.a.nsh[iat]; si++)
-      for (int sj = b.a.ish[jat]; sj < b.a.ish[jat] + b.a.nsh[jat]; sj++) {
-        double a1 = b.s.minalpha[si], a2 = b.s.minalpha[sj];
-        if (a1 * a2 * r2 / (a1 + a2) > b.intcut * (1.0 + 1e-10)) continue;
-        const int key = 2 * b.s.l[si] + b.s.l[sj];
-        if (key == 0)
-          h0_grad_shell<0, 0>(b, M, si, sj, iat, jat, v, r2, rr, P, W, se, dsedcn, vsh, vat, vdp, vqp, dg, dcni, dcnj);
-        else if (key == 1)
-          h0_grad_shell<0, 1>(b, M, si, sj, iat, jat, v, r2, rr, P, W, se, dsedcn, vsh, vat, vdp, vqp, dg, dcni, dcnj);
-        else if (key == 2)
-          h0_grad_shell<1, 0>(b, M, si, sj, iat, jat, v, r2, rr, P, W, se, dsedcn, vsh, vat, vdp, vqp, dg, dcni, dcnj);
-        else
-          h0_grad_shell<1, 1>(b, M, si, sj, iat, jat, v, r2, rr, P, W, se, dsedcn, vsh, vat, vdp, vqp, dg, dcni, dcnj);
-      }
+  const double r2 = dot3(v, v);
+  const double rr = sqrt(sqrt(r2) / (b.a.rad[jat] + b.a.rad[iat]));
+  const long long e = M.pair0 + idx;
+  double dg[3] = {hg[e], hg[hstride + e], hg[2 * hstride + e]};
+  double dcni = hg[3 * hstride + e], dcnj = hg[4 * hstride + e];
+  for (int si = b.a.ish[iat]; si < b.a.ish[iat] + b.a.nsh[iat]; si++) {
+    if (b.s.l[si] != LI) continue;
+    for (int sj = b.a.ish[jat]; sj < b.a.ish[jat] + b.a.nsh[jat]; sj++) {
+      if (b.s.l[sj] != LJ) continue;
+      const double a1 = b.s.minalpha[si], a2 = b.s.minalpha[sj];
+      if (a1 * a2 * r2 / (a1 + a2) > b.intcut * (1.0 + 1e-10)) continue;
+      h0_grad_shell<LI, LJ>(b, M, si, sj, iat, jat, v, r2, rr, P, W, se, dsedcn, vsh, vat, vdp, vqp, dg, dcni, dcnj);
+    }
   }
-  long long e = M.pair0 + idx;
   for (int k = 0; k < 3; k++) hg[k * hstride + e] = dg[k];
   hg[3 * hstride + e] = dcni;
   hg[4 * hstride + e] = dcnj;
 }
+template __global__ void k_h0_grad_key<0, 0>(Batch, const int2*, const int*, const long long*, const double*,
+                                             const double*, const double*, const double*, const double*,
+                                             const double*, const double*, const double*, double*, long long);
+template __global__ void k_h0_grad_key<0, 1>(Batch, const int2*, const int*, const long long*, const double*,
+                                             const double*, const double*, const double*, const double*,
+                                             const double*, const double*, const double*, double*, long long);
+template __global__ void k_h0_grad_key<1, 0>(Batch, const int2*, const int*, const long long*, const double*,
+                                             const double*, const double*, const double*, const double*,
+                                             const double*, const double*, const double*, double*, long long);
+template __global__ void k_h0_grad_key<1, 1>(Batch, const int2*, const int*, const long long*, const double*,
+                                             const double*, const double*, const double*, const double*,
+                                             const double*, const double*, const double*, double*, long long);
 
 // Chain rules of the two coordination numbers (base/src/ncoord.rs:486-519): the GFN2 CN with
 // dE/dCN from the levels and the AES radii, and the D4 CN (Jacobian order, zero weights skipped)

@@ -759,7 +759,8 @@ __global__ __launch_bounds__(256) void k_oa_step(const EigJob* jobs, double* ema
   // element e = i + j n of this thread, advanced without division
   const int di = nt % n, dj = nt / n;
   double so = 0.0, ro = 0.0, am = 0.0;
-  {
+  double delta = 0.0;  // the cluster threshold (not used with classes)
+  if (!cls) {
     int i = tid % n, j = tid / n;
     for (int e = tid; e < n * n; e += nt) {
       const double sd = S[e] - (i == j ? lam[i] : 0.0);
@@ -774,24 +775,26 @@ __global__ __launch_bounds__(256) void k_oa_step(const EigJob* jobs, double* ema
       }
     }
   }
-  for (int i = tid; i < n; i += nt) am = fmax(am, fabs(lam[i]));
-  so = block_sum(so, red);
-  ro = block_sum(ro, red);
-  // max |lambda| over the workgroup
-  for (int o = 32; o > 0; o >>= 1) am = fmax(am, __shfl_xor(am, o, 64));
-  __syncthreads();
-  if ((tid & 63) == 0) red[tid >> 6] = am;
-  __syncthreads();
-  am = 0.0;
-  for (int w = 0; w < (int)(nt >> 6); w++) am = fmax(am, red[w]);
-  // the first step's cluster threshold is kept for the later steps: a pair left unrotated as a
-  // cluster is not rotated later from a smaller delta; pairs whose first-order mixing would
-  // exceed 0.05 are treated as clusters too
-  double delta = 2.0 * (sqrt(so) + am * sqrt(ro));
-  if (first) {
-    if (tid == 0) delta_keep[slot[blockIdx.x]] = delta;
-  } else {
-    delta = fmax(delta, delta_keep[slot[blockIdx.x]]);
+  if (!cls) {
+    for (int i = tid; i < n; i += nt) am = fmax(am, fabs(lam[i]));
+    so = block_sum(so, red);
+    ro = block_sum(ro, red);
+    // max |lambda| over the workgroup
+    for (int o = 32; o > 0; o >>= 1) am = fmax(am, __shfl_xor(am, o, 64));
+    __syncthreads();
+    if ((tid & 63) == 0) red[tid >> 6] = am;
+    __syncthreads();
+    am = 0.0;
+    for (int w = 0; w < (int)(nt >> 6); w++) am = fmax(am, red[w]);
+    // the first step's cluster threshold is kept for the later steps: a pair left unrotated as a
+    // cluster is not rotated later from a smaller delta; pairs whose first-order mixing would
+    // exceed 0.05 are treated as clusters too
+    delta = 2.0 * (sqrt(so) + am * sqrt(ro));
+    if (first) {
+      if (tid == 0) delta_keep[slot[blockIdx.x]] = delta;
+    } else {
+      delta = fmax(delta, delta_keep[slot[blockIdx.x]]);
+    }
   }
   // emax: the largest correction (pairs left unrotated as clusters are not counted: a coupling
   // there changes neither P nor, beyond its own size, W)
