@@ -1117,8 +1117,35 @@ struct Engine {
     fill(pool.p, 11 * tot_mat);
     double ibytes = 0.0;  // the 11 matrices written once
     for (auto& M : mols) ibytes += 11.0 * 8.0 * M.nao * (double)M.nao;
-    LAUNCHW(ibytes, k_integrals, dim3(blocks((long long)nshmax * nshmax, 128), NM), dim3(128), 0, b, se.p,
-                       pool.p, (long long)tot_mat);
+    {
+      // shell-pair lists by type (k_int_keys), grouped by molecule m % INT_GROUPS
+      std::vector<long long> go(INT_GROUPS, 0);
+      for (int m = 0; m < NM; m++)
+        if (m % INT_GROUPS + 1 < INT_GROUPS) go[m % INT_GROUPS + 1] += (long long)mols[m].nsh * mols[m].nsh;
+      for (int g = 1; g < INT_GROUPS; g++) go[g] += go[g - 1];
+      intgoff.upload(go, st, &stage);
+      intcnt.ensure(4 * INT_GROUPS);
+      intlist.ensure(4 * (size_t)tot_shp);
+      LAUNCH(k_zero_u32, dim3(1), dim3(256), 0, (unsigned*)intcnt.p, 4 * INT_GROUPS);
+      LAUNCH(k_int_keys, dim3(blocks((long long)nshmax * nshmax, 64), NM), dim3(64), 0, b, intcnt.p, intlist.p,
+             (long long)tot_shp, (const long long*)intgoff.p);
+      std::vector<int> hc(4 * INT_GROUPS);
+      HIP_OK(hipMemcpyAsync(hc.data(), intcnt.p, sizeof(int) * hc.size(), hipMemcpyDeviceToHost, st));
+      HIP_OK(hipStreamSynchronize(st));
+      auto ikey = [&](int k, auto kern) {
+        int mx = 0;
+        for (int g = 0; g < INT_GROUPS; g++) mx = std::max(mx, hc[k * INT_GROUPS + g]);
+        if (mx <= 0) return;
+        ProfScope ps_(&prof, st, "k_integrals", k == 0 ? ibytes : 0.0);
+        hipLaunchKernelGGL(kern, dim3(blocks(mx, 64), INT_GROUPS), dim3(64), 0, st, b,
+                           (const int2*)intlist.p + (long long)k * tot_shp, (const int*)intcnt.p + k * INT_GROUPS,
+                           (const long long*)intgoff.p, (const double*)se.p, pool.p, (long long)tot_mat);
+      };
+      ikey(0, k_integrals_key<0, 0>);
+      ikey(1, k_integrals_key<0, 1>);
+      ikey(2, k_integrals_key<1, 0>);
+      ikey(3, k_integrals_key<1, 1>);
+    }
     // repulsion
     LAUNCH(k_repulsion, dim3(blocks(NA, 4)), dim3(256), 0, b, (const int*)nullptr, erep.p, grep.p);
     // D4 three-body at q = 0 (weights with dgw/dCN)
@@ -1503,7 +1530,9 @@ struct Engine {
   DBuf<BlockCopy> dbcopy;
   DBuf<int2> mat_tiles, h0list;
   DBuf<int> h0cnt;
-  DBuf<long long> h0goff;
+  DBuf<long long> h0goff, intgoff;
+  DBuf<int2> intlist;
+  DBuf<int> intcnt;
   int n_mat_tiles = 0;
 
   // ------------------------------------------------------------------------------ SCC loop

@@ -278,44 +278,81 @@ __device__ __forceinline__ void integral_block(const Batch& b, const Mol& M, int
 
 // S, H0, D (3), Q (6) of every shell pair (ish of atom A >= jsh of atom B): each AO element
 // written once (ao.rs:195-291, hamiltonian.rs:77-94). Pools are `stride` elements apart:
-// slot 0 S, 1 H0, 2-4 D, 5-10 Q; element (mu, nu) of molecule m at mat0 + mu + nu nao.
-__global__ __launch_bounds__(128) void k_integrals(Batch b, const double* se, double* pool, long long stride) {
-  int m = blockIdx.y;
+// slot 0 S, 1 H0, 2-4 D, 5-10 Q; element (mu, nu) of molecule m at mat0 + mu + nu nao; elements of
+// screened shell pairs stay as they are (zero). Split by shell-pair type, so that a wavefront runs
+// one type: k_int_keys lists the shell pairs (i, j) of each key 2 l_i + l_j that pass the screening
+// (entry (m, i nsh + j); the lists grouped by molecule like k_h0_keys' lists, capacity of a group the
+// sum of its nsh^2), k_integrals_key<LI, LJ> (grid.y = group) computes them.
+constexpr int INT_GROUPS = 64;
+__global__ __launch_bounds__(64) void k_int_keys(Batch b, int* cnt, int2* list, long long cap, const long long* goff) {
+  const int m = blockIdx.y;
   const Mol M = b.mol[m];
-  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (long long)M.nsh * M.nsh) return;
-  int i = (int)(idx / M.nsh), j = (int)(idx % M.nsh);  // ket shell i (column), bra shell j (row)
-  int si = M.sh0 + i, sj = M.sh0 + j;
-  int ai = b.s.at[si], aj = b.s.at[sj];
-  if (ai < aj) return;
+  const long long idx = (long long)blockIdx.x * 64 + threadIdx.x;
+  int key = -1;
+  if (idx < (long long)M.nsh * M.nsh) {
+    const int i = (int)(idx / M.nsh), j = (int)(idx % M.nsh);  // ket shell i (column), bra shell j (row)
+    const int si = M.sh0 + i, sj = M.sh0 + j;
+    const int ai = b.s.at[si], aj = b.s.at[sj];
+    if (ai >= aj) {
+      bool keep = true;
+      if (ai != aj) {
+        // atom-pair and shell-pair exponent screening (integrals.rs:216-218)
+        const double* xi = &b.a.xyz[3 * ai];
+        const double* xj = &b.a.xyz[3 * aj];
+        const double vec[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+        const double r2 = dot3(vec, vec);
+        const double a1 = b.s.minalpha[si], a2 = b.s.minalpha[sj];
+        keep = !(a1 * a2 * r2 / (a1 + a2) > b.intcut * (1.0 + 1e-10));
+      }
+      if (keep) key = 2 * b.s.l[si] + b.s.l[sj];
+    }
+  }
+  const int lane = threadIdx.x, g = m % INT_GROUPS;
+  for (int k = 0; k < 4; k++) {
+    const bool has = key == k;
+    const unsigned long long bal = __ballot(has);
+    if (!bal) continue;
+    int base = 0;
+    if (lane == 0) base = atomicAdd(&cnt[k * INT_GROUPS + g], (int)__popcll(bal));
+    base = __shfl(base, 0, 64);
+    if (has) list[k * cap + goff[g] + base + __popcll(bal & ((1ull << lane) - 1ull))] = make_int2(m, (int)idx);
+  }
+}
+
+template <int LI, int LJ>
+__global__ __launch_bounds__(64) void k_integrals_key(Batch b, const int2* list, const int* count,
+                                                      const long long* goff, const double* se, double* pool,
+                                                      long long stride) {
+  const int g = blockIdx.y, t = blockIdx.x * 64 + threadIdx.x;
+  if (t >= count[g]) return;
+  const int2 it = list[goff[g] + t];
+  const Mol M = b.mol[it.x];
+  const int i = it.y / M.nsh, j = it.y % M.nsh;
+  const int si = M.sh0 + i, sj = M.sh0 + j;
+  const int ai = b.s.at[si], aj = b.s.at[sj];
   const double* xi = &b.a.xyz[3 * ai];
   const double* xj = &b.a.xyz[3 * aj];
   double vec[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
-  double r2 = dot3(vec, vec);
-  bool onsite = ai == aj;
-  if (!onsite) {
-    // atom-pair and shell-pair exponent screening (integrals.rs:216-218)
-    double a1 = b.s.minalpha[si], a2 = b.s.minalpha[sj];
-    if (a1 * a2 * r2 / (a1 + a2) > b.intcut * (1.0 + 1e-10)) return;
-  }
+  const double r2 = dot3(vec, vec);
+  const bool onsite = ai == aj;
   double hij;
   if (onsite) {
     hij = 0.5 * (se[si] + se[sj]);
   } else {
-    double rr = sqrt(sqrt(r2) / (b.a.rad[aj] + b.a.rad[ai]));
-    double sp = (1.0 + b.s.shpoly[si] * rr) * (1.0 + b.s.shpoly[sj] * rr);
+    const double rr = sqrt(sqrt(r2) / (b.a.rad[aj] + b.a.rad[ai]));
+    const double sp = (1.0 + b.s.shpoly[si] * rr) * (1.0 + b.s.shpoly[sj] * rr);
     hij = 0.5 * (se[si] + se[sj]) * hscale(b, sj, si) * sp;
   }
-  const int key = 2 * b.s.l[si] + b.s.l[sj];
-  if (key == 0)
-    integral_block<0, 0>(b, M, si, sj, vec, onsite, hij, pool, stride);
-  else if (key == 1)
-    integral_block<0, 1>(b, M, si, sj, vec, onsite, hij, pool, stride);
-  else if (key == 2)
-    integral_block<1, 0>(b, M, si, sj, vec, onsite, hij, pool, stride);
-  else
-    integral_block<1, 1>(b, M, si, sj, vec, onsite, hij, pool, stride);
+  integral_block<LI, LJ>(b, M, si, sj, vec, onsite, hij, pool, stride);
 }
+template __global__ void k_integrals_key<0, 0>(Batch, const int2*, const int*, const long long*, const double*,
+                                               double*, long long);
+template __global__ void k_integrals_key<0, 1>(Batch, const int2*, const int*, const long long*, const double*,
+                                               double*, long long);
+template __global__ void k_integrals_key<1, 0>(Batch, const int2*, const int*, const long long*, const double*,
+                                               double*, long long);
+template __global__ void k_integrals_key<1, 1>(Batch, const int2*, const int*, const long long*, const double*,
+                                               double*, long long);
 
 // ------------------------------------------------------------------ repulsion (repulsion.rs:44-73)
 // One wavefront per atom: E_A (half of each pair) and dE/dR_A, cutoff 40 bohr.
