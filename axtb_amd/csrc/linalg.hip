@@ -154,7 +154,7 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmJob* jobs) {
 // op(A) = A^T / op(B) = B^T at compile time. Every job of a launch has the same TA, TB.
 constexpr int BT = 128;
 constexpr int BK = 16;
-constexpr int BP = BT + 4;  // padded LDS row
+constexpr int BP = BT + 1;  // padded LDS row (an odd row length measured fastest)
 
 template <int TA, int TB, bool TWO>
 __global__ __launch_bounds__(256) void k_gemm128(const GemmJob* jobs) {
@@ -302,7 +302,7 @@ __global__ void k_gemm_reduce(const double* P, int splits, int m, int n, double*
 // wavefront works on padding beyond the next multiple of 16), K in steps of 16 through
 // double-buffered LDS with the next step's loads in registers (clamped addresses, masks applied at
 // the store, as in k_gemm128). Few registers per thread: two workgroups share a CU.
-constexpr int SP = 132;  // padded LDS row of the small-matrix kernel
+constexpr int SP = 129;  // padded LDS row of the small-matrix kernel (odd, as BP)
 template <int TA, int TB>
 __global__ __launch_bounds__(512, 4) void k_gemm_small(const GemmJob* jobs) {
   const GemmJob J = jobs[blockIdx.x];
