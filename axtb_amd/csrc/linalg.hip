@@ -157,7 +157,7 @@ constexpr int BK = 16;
 constexpr int BP = BT + 1;  // padded LDS row (an odd row length measured fastest)
 
 template <int TA, int TB, bool TWO>
-__global__ __launch_bounds__(256) void k_gemm128(const GemmJob* jobs) {
+__global__ __launch_bounds__(256, 2) void k_gemm128(const GemmJob* jobs) {
   const GemmJob J = jobs[blockIdx.z];
   const int row0 = blockIdx.y * BT, col0 = blockIdx.x * BT;
   if (row0 >= J.m || col0 >= J.n) return;
