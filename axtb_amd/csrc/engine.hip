@@ -1075,10 +1075,11 @@ struct Engine {
     gam.ensure(tot_shp);
     pool.ensure(11 * (size_t)tot_mat);
     {
-      std::vector<int2> tl;  // 256-element tiles of the per-molecule matrices (k_fock)
+      std::vector<int2> tl;  // the lower 32 x 32 tiles (I >= J) of the per-molecule matrices (k_fock)
       for (int m = 0; m < (int)mols.size(); m++) {
-        const long long nn = (long long)mols[m].nao * mols[m].nao;
-        for (long long e = 0; e < nn; e += 256) tl.push_back(make_int2(m, (int)e));
+        const int nt = (mols[m].nao + 31) / 32;
+        for (int I = 0; I < nt; I++)
+          for (int J = 0; J <= I; J++) tl.push_back(make_int2(m, (I << 16) | J));
       }
       mat_tiles.upload(tl, st);
       n_mat_tiles = (int)tl.size();
@@ -1610,12 +1611,16 @@ struct Engine {
       if (!o.dispersion) fill(gw.p, MAXREF * (long long)NA);
       LAUNCH(k_atom_pot, dim3(blocks(NA, 4)), dim3(256), 0, b, active.p, mrad.p, qat.p, dp.p, qp.p,
                          gw.p, dgw.p, vat.p, vdp.p, vqp.p);
-      // HBM bytes of the streaming kernels: Fock reads S, H0, D, Q and writes F (12 n^2 doubles);
-      // Mulliken reads P, S, D, Q, H0 (12 n^2 doubles)
-      double sbytes = 0.0;
+      // HBM bytes of the streaming kernels: Fock reads the lower triangles of S, H0, D, Q and writes
+      // F (11 n (n + 1) / 2 + n^2 doubles); Mulliken reads P, S, D, Q, H0 (12 n^2 doubles)
+      double sbytes = 0.0, fbytes = 0.0;
       for (int m = 0; m < NM; m++)
-        if (hactive[m]) sbytes += 12.0 * 8.0 * mols[m].nao * (double)mols[m].nao;
-      LAUNCHW(sbytes, k_fock, dim3(n_mat_tiles), dim3(256), 0, b, active.p, (const int2*)mat_tiles.p,
+        if (hactive[m]) {
+          const double n = mols[m].nao;
+          sbytes += 12.0 * 8.0 * n * n;
+          fbytes += 8.0 * (11.0 * n * (n + 1.0) / 2.0 + n * n);
+        }
+      LAUNCHW(fbytes, k_fock, dim3(n_mat_tiles), dim3(256), 0, b, active.p, (const int2*)mat_tiles.p,
                          pool.p, stride, vsh.p, vat.p, vdp.p, vqp.p, F.p);
       // generalized eigenproblem in the Loewdin basis
       gemm(a1);

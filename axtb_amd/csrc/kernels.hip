@@ -788,39 +788,59 @@ __global__ __launch_bounds__(256) void k_atom_pot(Batch b, const int* active, co
 // Fock matrix F = H0 - 1/2 (B + B^T), B(mu,nu) = S v(nu) + D.vd(A(nu)) + Q.vq(A(nu))
 // (scf/fock.rs:22-104). B(nu,mu) is rebuilt from the (mu,nu) integrals by shifting their moments
 // to the atom of mu, so every read is coalesced. `out` has the layout of the integral pools.
-// Launched over the 256-element tiles of all matrices: tile t covers elements tiles[t].y .. + 255 of
-// molecule tiles[t].x (no workgroup beyond a molecule's n^2 elements).
+// F is symmetric, and F(mu, nu) is formed from the (mu, nu) integrals alone (the moments of (nu, mu)
+// are those of (mu, nu) shifted to the atom of mu), so only the lower triangle of the integral pools
+// is read: one workgroup per 32 x 32 tile (I, J), I >= J, of a molecule (tiles[t] = (m, I << 16 | J)),
+// each thread four elements; the tile is written as it is and, through LDS, as its transpose
+// (both coalesced). In a diagonal tile the elements above the diagonal come from the transpose.
 __global__ __launch_bounds__(256) void k_fock(Batch b, const int* active, const int2* tiles, const double* pool,
                                               long long stride, const double* vsh, const double* vat,
                                               const double* vdp, const double* vqp, double* F) {
+  __shared__ double tt[32][33];
   const int2 tl = tiles[blockIdx.x];
   const int m = tl.x;
   if (active && !active[m]) return;
   const Mol M = b.mol[m];
-  long long idx = (long long)tl.y + threadIdx.x;
-  long long nn = (long long)M.nao * M.nao;
-  if (idx >= nn) return;
-  int mu = (int)(idx % M.nao), nu = (int)(idx / M.nao);
-  int gmu = M.ao0 + mu, gnu = M.ao0 + nu;
-  int A = b.ao_at[gmu], B = b.ao_at[gnu];
-  long long e = M.mat0 + idx;
-  double s = pool[e], h = pool[stride + e];
-  double d[3], q[6];
-  for (int k = 0; k < 3; k++) d[k] = pool[(2 + k) * stride + e];
-  for (int k = 0; k < 6; k++) q[k] = pool[(5 + k) * stride + e];
-  double vmu = vsh[b.ao_sh[gmu]] + vat[A], vnu = vsh[b.ao_sh[gnu]] + vat[B];
-  double bsum = s * (vmu + vnu);
-  for (int k = 0; k < 3; k++) bsum += d[k] * vdp[3 * B + k];
-  for (int k = 0; k < 6; k++) bsum += q[k] * vqp[6 * B + k];
-  // moments of (nu, mu): around the atom of mu
-  const double* xa = &b.a.xyz[3 * A];
-  const double* xb = &b.a.xyz[3 * B];
-  double vec[3] = {xb[0] - xa[0], xb[1] - xa[1], xb[2] - xa[2]};
-  double dt[3], qt[6];
-  shift_to_bra(vec, s, d, q, dt, qt);
-  for (int k = 0; k < 3; k++) bsum += dt[k] * vdp[3 * A + k];
-  for (int k = 0; k < 6; k++) bsum += qt[k] * vqp[6 * A + k];
-  F[M.mat0 + idx] = h - 0.5 * bsum;
+  const int n = M.nao, I = tl.y >> 16, J = tl.y & 0xffff;
+  const int r = threadIdx.x & 31, c0 = threadIdx.x >> 5;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int c = c0 + 8 * k;
+    const int mu = 32 * I + r, nu = 32 * J + c;
+    double f = 0.0;
+    if (mu < n && nu < n && mu >= nu) {
+      const long long e = M.mat0 + mu + (long long)nu * n;
+      const int gmu = M.ao0 + mu, gnu = M.ao0 + nu;
+      const int A = b.ao_at[gmu], B = b.ao_at[gnu];
+      const double s = pool[e], h = pool[stride + e];
+      double d[3], q[6];
+      for (int x = 0; x < 3; x++) d[x] = pool[(2 + x) * stride + e];
+      for (int x = 0; x < 6; x++) q[x] = pool[(5 + x) * stride + e];
+      const double vmu = vsh[b.ao_sh[gmu]] + vat[A], vnu = vsh[b.ao_sh[gnu]] + vat[B];
+      double bsum = s * (vmu + vnu);
+      for (int x = 0; x < 3; x++) bsum += d[x] * vdp[3 * B + x];
+      for (int x = 0; x < 6; x++) bsum += q[x] * vqp[6 * B + x];
+      // moments of (nu, mu): around the atom of mu
+      const double* xa = &b.a.xyz[3 * A];
+      const double* xb = &b.a.xyz[3 * B];
+      double vec[3] = {xb[0] - xa[0], xb[1] - xa[1], xb[2] - xa[2]};
+      double dt[3], qt[6];
+      shift_to_bra(vec, s, d, q, dt, qt);
+      for (int x = 0; x < 3; x++) bsum += dt[x] * vdp[3 * A + x];
+      for (int x = 0; x < 6; x++) bsum += qt[x] * vqp[6 * A + x];
+      f = h - 0.5 * bsum;
+      F[e] = f;
+    }
+    tt[c][r] = f;
+  }
+  __syncthreads();
+  // the transpose: element (32 J + r, 32 I + c) = F(mu = 32 I + c, nu = 32 J + r)
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int c = c0 + 8 * k;
+    const int row = 32 * J + r, col = 32 * I + c;
+    if (row < n && col < n && col >= row && (I != J || col > row)) F[M.mat0 + row + (long long)col * n] = tt[r][c];
+  }
 }
 
 // Fermi filling of each molecule (base/src/occupations.rs:94-125, Newton::XTB680), entropy term
