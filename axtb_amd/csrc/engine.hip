@@ -267,6 +267,7 @@ struct Engine {
     if (const char* v = std::getenv("AXTB_HIP_OA_STEPS")) oa_steps = std::min(4, std::max(1, std::atoi(v)));
     if (const char* v = std::getenv("AXTB_HIP_DENSE_FULL")) dense_full = std::max(1, std::atoi(v));
     if (const char* v = std::getenv("AXTB_HIP_SPLIT_REFINE")) split_refine = std::atoi(v) != 0;
+    if (const char* v = std::getenv("AXTB_HIP_JACOBI_WIDE")) jacobi_wide = std::atoi(v) != 0;
     if (const char* v = std::getenv("AXTB_HIP_SPLIT_RESIDUAL")) split_residual = std::atof(v);
     if (const char* v = std::getenv("AXTB_HIP_DENSE_REFINE")) dense_refine = std::atoi(v) != 0;
     if (const char* v = std::getenv("AXTB_HIP_GEMM_SMALL")) gemm_small = std::atoi(v) != 0;
@@ -451,7 +452,10 @@ struct Engine {
         ProfScope ps_(&prof, s, nm, flops);
         const JacobiKernel kern = jacobi_kernel(mode, fp32);
         HIP_OK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-        hipLaunchKernelGGL(kern, dim3((unsigned)(b1 - b0)), dim3(512), shm, s, buf.p + b0, max_sweeps, stat + b0,
+        // a launch with one workgroup per CU (its matrices fill the LDS) runs 16 wavefronts per
+        // matrix, the others 8
+        const unsigned nthr = (mode != 0 && pc == 1 && jacobi_wide) ? 1024u : 512u;
+        hipLaunchKernelGGL(kern, dim3((unsigned)(b1 - b0)), dim3(nthr), shm, s, buf.p + b0, max_sweeps, stat + b0,
                            tol_scale);
         b0 = b1;
       }
@@ -1519,6 +1523,7 @@ struct Engine {
   bool gemm_small = true;      // products with m, n <= 128 on the one-workgroup-per-job kernel (AXTB_HIP_GEMM_SMALL)
   bool in_refine = false;      // the refinement's batches always take the 128-tile kernel (measured faster)
   int oa_steps = 2;  // refinement steps per SCC iteration (AXTB_HIP_OA_STEPS)
+  bool jacobi_wide = true;  // 1024-thread Jacobi workgroups when one fills a CU (AXTB_HIP_JACOBI_WIDE)
   // mixed, batched molecules: SCC iterations after the first refine the previous orbitals for the
   // occupied / empty split when the occupations are all 0 or 2 (AXTB_HIP_SPLIT_REFINE)
   bool split_refine = true;

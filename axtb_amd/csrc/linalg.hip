@@ -496,7 +496,7 @@ template <typename T> struct alignas(16) JRot {
 };
 
 template <int MODE, typename T, bool PACK>
-__global__ __launch_bounds__(512) void k_jacobi(const EigJob* jobs, int max_sweeps, int* status, float tol_scale) {
+__global__ __launch_bounds__(1024) void k_jacobi(const EigJob* jobs, int max_sweeps, int* status, float tol_scale) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T* lds = (T*)lds_raw;
   __shared__ int rotated;
