@@ -452,9 +452,9 @@ struct Engine {
         ProfScope ps_(&prof, s, nm, flops);
         const JacobiKernel kern = jacobi_kernel(mode, fp32);
         HIP_OK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-        // a launch with one workgroup per CU (its matrices fill the LDS) runs 16 wavefronts per
-        // matrix, the others 8
-        const unsigned nthr = (mode != 0 && pc == 1 && jacobi_wide) ? 1024u : 512u;
+        // 16 wavefronts per matrix while they fit (32 per CU: one workgroup per CU, or two of the FP32
+        // kernels, which use at most 64 registers), else 8
+        const unsigned nthr = (mode != 0 && jacobi_wide && (pc == 1 || (pc == 2 && fp32))) ? 1024u : 512u;
         hipLaunchKernelGGL(kern, dim3((unsigned)(b1 - b0)), dim3(nthr), shm, s, buf.p + b0, max_sweeps, stat + b0,
                            tol_scale);
         b0 = b1;
