@@ -409,7 +409,8 @@ struct Engine {
     for (size_t q = 0; q < jobs.size(); q++) {
       const EigJob& j = jobs[q];
       if (j.n > 200) throw DeviceError("Jacobi eigensolver called for n > 200");
-      int mode = jacobi_lds(j.n, 2, es) <= (size_t)max_lds ? 2 : (jacobi_lds(j.n, 1, es) <= (size_t)max_lds ? 1 : 0);
+      // eigenvalues only (Z null): no V, so A alone in LDS (mode 1) when it fits
+      int mode = (j.Z && jacobi_lds(j.n, 2, es) <= (size_t)max_lds) ? 2 : (jacobi_lds(j.n, 1, es) <= (size_t)max_lds ? 1 : 0);
       if (fp32 && mode == 0) throw DeviceError("FP32 Jacobi: matrix does not fit in LDS");
       gidx[mode].push_back((int)q);
     }
@@ -1792,7 +1793,7 @@ struct Engine {
           const int s0 = blk == 0 ? 0 : k, nb = blk == 0 ? k : n - k;
           if (nb <= 0) continue;
           bc.push_back(BlockCopy{F.p + o0 + s0 + (long long)s0 * n, T.p + o0 + off, nb, n});
-          bj.push_back(EigJob{nb, T.p + o0 + off, C.p + o0 + off, w.p + a0 + s0, Vw.p + o0 + off});
+          bj.push_back(EigJob{nb, T.p + o0 + off, nullptr, w.p + a0 + s0, nullptr});  // eigenvalues only
           off += (long long)nb * nb;
         }
       }

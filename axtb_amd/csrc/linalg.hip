@@ -449,7 +449,7 @@ struct EigJob {
   double* A;      // n x n (ld n), destroyed
   double* V;      // n x n work (ld n)
   double* w;      // eigenvalues, ascending
-  double* Z;      // eigenvectors (ld n), columns in the order of w
+  double* Z;      // eigenvectors (ld n), columns in the order of w; null: eigenvalues only (V unused)
 };
 
 // Pair k of round r of the round-robin tournament over nn (even) indices: index 0 stays, the
@@ -538,7 +538,9 @@ __global__ __launch_bounds__(1024) void k_jacobi(const EigJob* jobs, int max_swe
       }
     }
   }
-  for (int e = tid; e < n * ldv; e += nt) {
+  // eigenvalues only (J.Z null): V is neither formed nor read
+  const bool vec = J.Z != nullptr;
+  for (int e = tid; vec && e < n * ldv; e += nt) {
     const int r = e % ldv, c = e / ldv;
     V[e] = r == c ? (T)1 : (T)0;
   }
@@ -682,7 +684,7 @@ __global__ __launch_bounds__(1024) void k_jacobi(const EigJob* jobs, int max_swe
       // V: one pair per wavefront at a time (pairs wv, wv + nvw, ...), the lanes over consecutive
       // rows; a pair that does not rotate costs one record read. With V in LDS the last wavefronts
       // compute the next round's rotations instead.
-      for (int k = wv; any && k < np && wv < nvw; k += nvw) {
+      for (int k = wv; vec && any && k < np && wv < nvw; k += nvw) {
         const JRot<T> rk = rb[k];
         const T sn = rk.s;
         if (sn == 0) continue;
@@ -707,7 +709,7 @@ __global__ __launch_bounds__(1024) void k_jacobi(const EigJob* jobs, int max_swe
       if (dj < di || (dj == di && j < i)) rank++;
     }
     J.w[rank] = (double)di;
-    for (int r = 0; r < n; r++) J.Z[r + (long long)rank * n] = (double)V[r + (long long)i * ldv];
+    for (int r = 0; vec && r < n; r++) J.Z[r + (long long)rank * n] = (double)V[r + (long long)i * ldv];
   }
   if (tid == 0) status[blockIdx.x] = conv ? sweep : -sweep;
 }
