@@ -1179,6 +1179,7 @@ struct Engine {
       if (max_lds <= 0) max_lds = 64 * 1024;
     }
     auto fits = [&](int n) { return ((size_t)n * n + n) * sizeof(double) <= (size_t)max_lds; };
+    auto fits_packed = [&](int n) { return ((size_t)n * (n + 1) / 2 + n) * sizeof(double) <= (size_t)max_lds; };
     fill(qstart.p, NS);
     if (o.guess == 0) {
       LAUNCH(k_eeq_system, dim3(blocks((long long)(natmax + 1) * (natmax + 1), 256), NM), dim3(256), 0,
@@ -1215,19 +1216,24 @@ struct Engine {
                eeqy.p, qstart.p, eeqfail.p);
       }
     }
-    // S-orthonormal start basis X: L^-T from a Cholesky factor in LDS where S fits or blocked on
-    // the matrix cores for dense-solver molecules, else the Loewdin orthogonaliser U s^-1/2 U^T
-    // from the eigenpairs of S
+    // S-orthonormal start basis X: L^-T from a Cholesky factor in LDS where S fits (its lower
+    // triangle packed where the square does not) or blocked on the matrix cores for dense-solver
+    // molecules, else the Loewdin orthogonaliser U s^-1/2 U^T from the eigenpairs of S
     {
-      std::vector<EigJob> jobs, cj;
+      std::vector<EigJob> jobs, cj, cjp;
       std::vector<int> lw;
-      int nch = 0;
+      int nch = 0, nchp = 0;
       for (int m = 0; m < NM; m++) {
         long long o0 = mols[m].mat0;
         const int n = mols[m].nao;
         if (!dense(n) && fits(n)) {
           cj.push_back(EigJob{n, pool.p + o0, nullptr, nullptr, X.p + o0});
           nch = std::max(nch, n);
+          continue;
+        }
+        if (!dense(n) && fits_packed(n)) {
+          cjp.push_back(EigJob{n, pool.p + o0, nullptr, nullptr, X.p + o0});
+          nchp = std::max(nchp, n);
           continue;
         }
         if (dense(n)) {
@@ -1247,6 +1253,19 @@ struct Engine {
         LAUNCH(k_chol_basis, dim3(cj.size()), dim3(512), shm, (const EigJob*)cjobs.p, cfail.p);
         std::vector<int> hf(cj.size());
         HIP_OK(hipMemcpyAsync(hf.data(), cfail.p, sizeof(int) * cj.size(), hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+        for (int f : hf)
+          if (f) throw Fail{AXTB_ERR_NUMERICAL, "overlap matrix is not positive definite"};
+      }
+      if (!cjp.empty()) {
+        cjobs.upload(cjp, st);
+        cfail.ensure(cjp.size());
+        const size_t shm = ((size_t)nchp * (nchp + 1) / 2 + nchp) * sizeof(double);
+        HIP_OK(hipFuncSetAttribute((const void*)k_chol_basis_packed, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)shm));
+        LAUNCH(k_chol_basis_packed, dim3(cjp.size()), dim3(512), shm, (const EigJob*)cjobs.p, cfail.p);
+        std::vector<int> hf(cjp.size());
+        HIP_OK(hipMemcpyAsync(hf.data(), cfail.p, sizeof(int) * cjp.size(), hipMemcpyDeviceToHost, st));
         HIP_OK(hipStreamSynchronize(st));
         for (int f : hf)
           if (f) throw Fail{AXTB_ERR_NUMERICAL, "overlap matrix is not positive definite"};

@@ -1038,6 +1038,70 @@ __global__ __launch_bounds__(512) void k_chol_basis(const EigJob* jobs, int* fai
   if (tid == 0) fail[blockIdx.x] = bad;
 }
 
+// The same for matrices whose full square does not fit in LDS (n <= 200): the lower triangle packed
+// by columns (element (r, c), r >= c, at c (2n - c + 1) / 2 + r - c) plus one column, n (n + 1) / 2 + n
+// doubles. L by the right-looking Cholesky, then X = L^-1 in place, columns from the last:
+// X(j, j) = 1 / L(j, j), X(i, j) = -X(j, j) sum_{k = j+1..i} X(i, k) L(k, j) for i > j (column j of L
+// copied aside first; the columns k > j of X are final). job.Z = B0 = X^T (upper triangular).
+__global__ __launch_bounds__(512) void k_chol_basis_packed(const EigJob* jobs, int* fail) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  __shared__ int bad;
+  const EigJob J = jobs[blockIdx.x];
+  const int n = J.n, tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
+  double* M = (double*)lds_raw;
+  double* col = M + n * (n + 1) / 2;
+  auto pidx = [&](int r, int c) { return c * (2 * n - c + 1) / 2 + (r - c); };
+  for (int c = wv; c < n; c += nw)
+    for (int r = c + lane; r < n; r += 64) M[pidx(r, c)] = J.A[r + (long long)c * n];
+  if (tid == 0) bad = 0;
+  __syncthreads();
+  for (int k = 0; k < n; k++) {
+    const int pkk = pidx(k, k);
+    if (tid == 0) {
+      double d = M[pkk];
+      if (!(d > 0.0)) {
+        bad = 1;
+        d = 1.0;
+      }
+      M[pkk] = sqrt(d);
+    }
+    __syncthreads();
+    const double lkk = M[pkk];
+    for (int i = k + 1 + tid; i < n; i += nt) M[pkk + (i - k)] /= lkk;
+    __syncthreads();
+    // trailing lower triangle: column j by one wavefront, rows i >= j over the lanes
+    for (int j = k + 1 + wv; j < n; j += nw) {
+      const int pj = pidx(j, j), pk = pkk + (j - k);  // (j, j) and (j, k): rows i >= j follow
+      const double ljk = M[pk];
+      for (int i = j + lane; i < n; i += 64) M[pj + (i - j)] -= M[pk + (i - j)] * ljk;
+    }
+    __syncthreads();
+  }
+  for (int j = n - 1; j >= 0; j--) {
+    const int pjj = pidx(j, j);
+    for (int i = j + 1 + tid; i < n; i += nt) col[i] = M[pjj + (i - j)];
+    __syncthreads();
+    const double xjj = 1.0 / M[pjj];
+    for (int i = j + 1 + tid; i < n; i += nt) {
+      double s = 0.0;
+      int p = pidx(i, j + 1);
+      for (int k = j + 1; k <= i; k++) {
+        s += M[p] * col[k];
+        p += n - k - 1;
+      }
+      M[pjj + (i - j)] = -xjj * s;
+    }
+    __syncthreads();
+    if (tid == 0) M[pjj] = xjj;
+    __syncthreads();
+  }
+  for (int e = tid; e < n * n; e += nt) {
+    const int r = e % n, c = e / n;
+    J.Z[e] = c >= r ? M[pidx(c, r)] : 0.0;
+  }
+  if (tid == 0) fail[blockIdx.x] = bad;
+}
+
 // EEQ charges of the listed molecules by LU with partial pivoting (the first maximum, as the
 // reference's dense LU in base/src/eeq.rs), in LDS ((n+1)^2 + n+1 doubles), then the start shell
 // charges as in k_eeq_solve; a zero pivot or a non-finite charge gives the neutral start.
