@@ -113,6 +113,36 @@ def test_mixed_precision_batch(ctx):
     print("mixed: max |dE| %.2e |dg| %.2e |dq| %.2e |deps| %.2e |defermi| %.2e" % tuple(worst), flush=True)
 
 
+def test_mixed_split_edge_cases(ctx):
+    """Mixed precision where the occupied / empty split refinement does not apply or works at its
+    edges: zero temperature (integral occupations by index), an open-shell cation (a singly
+    occupied orbital: every iteration solved in full), a doubly charged closed-shell molecule (its
+    SCC need not converge, but then in both precisions alike), and a repeated run (the same bits);
+    each against the FP64 engine."""
+    from axtb_amd import _lib, structures as S
+    z, x = structure("caffeine")
+    wz, wx = S.config("water20")
+    mols = [(z, x), (z, x, 1.0, 1), (z, x, 2.0, 0), (wz, wx)]
+    for etemp in (0.0, 300.0):
+        om = _lib.make_opts(accuracy=1e-2, etemp=etemp, precision="mixed")
+        o64 = _lib.make_opts(accuracy=1e-2, etemp=etemp, precision="fp64")
+        rm = ctx.engrad_batch(mols, om)
+        r64 = ctx.engrad_batch(mols, o64)
+        again = ctx.engrad_batch(mols, om)
+        print("etemp", etemp, "status mixed", [r["status"] for r in rm], "fp64", [r["status"] for r in r64],
+              flush=True)
+        for a, b, c in zip(rm, r64, again):
+            assert a["status"] == b["status"]  # a molecule that does not converge fails in both
+            if a["status"]:
+                continue
+            assert abs(a["energy"] - b["energy"]) <= 1e-8
+            assert np.abs(a["gradient"] - b["gradient"]).max() <= 1e-7
+            assert np.abs(np.sort(a["orbital_energies"]) - np.sort(b["orbital_energies"])).max() <= 1e-7
+            assert abs(a["efermi"] - b["efermi"]) <= 1e-7
+            assert a["energy"] == c["energy"] and np.array_equal(a["gradient"], c["gradient"])
+        assert rm[0]["status"] == 0 and rm[1]["status"] == 0 and rm[3]["status"] == 0
+
+
 def test_charged_and_invariants(ctx):
     z, x = structure("caffeine")
     r, o = check(ctx, z, x, 1e-6, charge=1.0)
