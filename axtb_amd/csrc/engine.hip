@@ -269,6 +269,8 @@ struct Engine {
     if (const char* v = std::getenv("AXTB_HIP_SPLIT_REFINE")) split_refine = std::atoi(v) != 0;
     if (const char* v = std::getenv("AXTB_HIP_JACOBI_WIDE")) jacobi_wide = std::atoi(v) != 0;
     if (const char* v = std::getenv("AXTB_HIP_SPLIT_RESIDUAL")) split_residual = std::atof(v);
+    if (const char* v = std::getenv("AXTB_HIP_SPLIT_STEPS")) split_steps = std::max(3, std::atoi(v));
+    if (const char* v = std::getenv("AXTB_HIP_SPLIT_FROM")) split_from = std::max(2, std::atoi(v));
     if (const char* v = std::getenv("AXTB_HIP_DENSE_REFINE")) dense_refine = std::atoi(v) != 0;
     if (const char* v = std::getenv("AXTB_HIP_GEMM_SMALL")) gemm_small = std::atoi(v) != 0;
     if (const char* v = std::getenv("AXTB_HIP_EXACT_LEVELS")) exact_levels = std::atoi(v) != 0;
@@ -1392,7 +1394,7 @@ struct Engine {
   std::vector<int> refine(const std::vector<EigJob>& ea, std::vector<char>& in_t, const int* cls = nullptr) {
     // cls: the split of the orbitals into occupied / empty (k_oa_step), refined from U = I: the
     // first step is k_oa_step alone (S = A, G = I, I + E is U), the later rounds are single steps,
-    // and a job not converged within its four steps is returned for a full solve
+    // and a job not converged within split_steps steps is returned for a full solve
     std::vector<EigJob> cur = ea;
     std::vector<int> redo;
     in_t.assign(ea.size(), 0);
@@ -1406,7 +1408,7 @@ struct Engine {
     in_refine = true;
     int gstep = 0;  // steps done by every job still in `cur`
     // the split refinement: rounds of 2 (U = I and one product step), 1 and 1 steps
-    const int nrounds = cls ? 3 : 2;
+    const int nrounds = cls ? std::max(2, split_steps - 1) : 2;
     for (int round = 0; round < nrounds && !cur.empty(); round++) {
       const int nsteps = round == 0 ? std::max(2, oa_steps + (oa_steps & 1)) : (cls ? 1 : 2);
       // the split refinement converges linearly at the rate (coupling inside the blocks) / gap, so
@@ -1549,9 +1551,11 @@ struct Engine {
   bool split_refine = true;
   double oa_ovmix = 0.05;  // largest first-order occupied / empty mixing a split step corrects
   double split_accept = 1e-9;  // max |e_ij| of the last split step
+  int split_steps = 8;  // steps of the split refinement before a full solve (AXTB_HIP_SPLIT_STEPS)
+  int split_from = 3;   // first SCC iteration that may use it (AXTB_HIP_SPLIT_FROM)
   // the split refinement starts once the previous iteration's residual is below this: the coupling
   // inside the blocks (the change of F since the last full solve) sets its convergence rate
-  double split_residual = 1e-2;
+  double split_residual = 5e-2;
   DBuf<int> dmlist, ocls, onfrac;
   DBuf<BlockCopy> dbcopy;
   DBuf<int2> mat_tiles, h0list;
@@ -1661,7 +1665,7 @@ struct Engine {
         // end. The others, and those not converged that way: the FP32 Jacobi and the full
         // refinement.
         std::vector<int> qs, qf;  // positions in ea (and a3: the same molecules in the same order)
-        if (it > 2 && split_refine && !ea.empty()) {
+        if (it >= split_from && split_refine && !ea.empty()) {
           std::vector<int> ml;
           for (int m = 0; m < NM; m++)
             if (hactive[m] && !isdense[m]) ml.push_back(m);
