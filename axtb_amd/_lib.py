@@ -15,7 +15,8 @@ OK, ERR_INPUT, ERR_NUMERICAL, ERR_SCF, ERR_ELEMENT, ERR_DEVICE = range(6)
 ERROR_KINDS = {1: "input", 2: "numerical", 3: "scf_not_converged", 4: "unsupported_element", 5: "device"}
 TERMS = ("repulsion", "dispersion", "electronic", "isotropic_es", "third_order", "anisotropic", "entropy", "band")
 EXPORTS = ("axtbhip_version", "axtbhip_context_create", "axtbhip_context_destroy", "axtbhip_last_error",
-           "axtbhip_basis_size", "axtbhip_gfn2_engrad", "axtbhip_gfn2_engrad_batch", "axtbhip_batch_prepare",
+           "axtbhip_basis_size", "axtbhip_gfn2_engrad", "axtbhip_gfn2_engrad_batch", "axtbhip_gfn2_engrad_batch_guess",
+           "axtbhip_batch_prepare",
            "axtbhip_batch_run", "axtbhip_batch_gradient", "axtbhip_batch_results", "axtbhip_profile_enable",
            "axtbhip_profile_report", "axtbhip_synchronize")
 
@@ -32,6 +33,10 @@ class AxtbError(RuntimeError):
 class Mol(ctypes.Structure):
     _fields_ = [("nat", ctypes.c_int), ("numbers", ctypes.c_void_p), ("positions", ctypes.c_void_p),
                 ("charge", ctypes.c_double), ("uhf", ctypes.c_int)]
+
+
+class Guess(ctypes.Structure):
+    _fields_ = [("shell_charges", ctypes.c_void_p), ("dipoles", ctypes.c_void_p), ("quadrupoles", ctypes.c_void_p)]
 
 
 class Opts(ctypes.Structure):
@@ -67,6 +72,9 @@ def lib():
                                           ctypes.POINTER(Result)]
         L.axtbhip_gfn2_engrad_batch.argtypes = [ctypes.c_void_p, ctypes.POINTER(Mol), ctypes.c_int,
                                                 ctypes.POINTER(Opts), ctypes.POINTER(Result)]
+        L.axtbhip_gfn2_engrad_batch_guess.argtypes = [ctypes.c_void_p, ctypes.POINTER(Mol), ctypes.c_int,
+                                                      ctypes.POINTER(Guess), ctypes.POINTER(Opts),
+                                                      ctypes.POINTER(Result)]
         L.axtbhip_batch_prepare.argtypes = [ctypes.c_void_p, ctypes.POINTER(Mol), ctypes.c_int, ctypes.POINTER(Opts)]
         L.axtbhip_batch_run.argtypes = [ctypes.c_void_p]
         L.axtbhip_batch_gradient.argtypes = [ctypes.c_void_p]
@@ -110,10 +118,33 @@ class Context:
         lib().axtbhip_last_error(self._h, buf, 1024)
         return buf.value.decode()
 
-    def engrad_batch(self, molecules, opts, raise_on_error=False):
-        """molecules: list of (numbers, positions_bohr[, charge[, uhf]]). Returns a list of dicts."""
+    def engrad_batch(self, molecules, opts, raise_on_error=False, starts=None):
+        """molecules: list of (numbers, positions_bohr[, charge[, uhf]]). Returns a list of dicts.
+
+        starts: None, or one given SCF start per molecule (None or a dict with "shell_charges",
+        "dipoles", "quadrupoles", e.g. a previous result); a warm start that does not converge is
+        computed once more from the fresh start."""
         mols, res, keep, outs = self._marshal(molecules)
-        rc = lib().axtbhip_gfn2_engrad_batch(self._h, mols, len(molecules), ctypes.byref(opts), res)
+        if starts is None:
+            rc = lib().axtbhip_gfn2_engrad_batch(self._h, mols, len(molecules), ctypes.byref(opts), res)
+        else:
+            if len(starts) != len(molecules):
+                raise AxtbError(ERR_INPUT, "one start per molecule")
+            gs = (Guess * len(molecules))()
+            for i, st in enumerate(starts):
+                if st is None:
+                    continue
+                nat = mols[i].nat
+                arrs = [np.ascontiguousarray(st[k], dtype=np.float64).reshape(-1)
+                        for k in ("shell_charges", "dipoles", "quadrupoles")]
+                nsh = ctypes.c_int()
+                lib().axtbhip_basis_size(ctypes.byref(mols[i]), ctypes.byref(nsh), None)
+                if arrs[0].size != nsh.value or arrs[1].size != 3 * nat or arrs[2].size != 6 * nat:
+                    raise AxtbError(ERR_INPUT, f"start {i} needs {nsh.value} shell charges and {nat} atomic "
+                                               f"dipoles/quadrupoles")
+                keep += arrs
+                gs[i] = Guess(arrs[0].ctypes.data, arrs[1].ctypes.data, arrs[2].ctypes.data)
+            rc = lib().axtbhip_gfn2_engrad_batch_guess(self._h, mols, len(molecules), gs, ctypes.byref(opts), res)
         return self._unmarshal(rc, res, outs, opts, raise_on_error)
 
     def prepare(self, molecules, opts):
@@ -208,3 +239,14 @@ class Context:
         if r["status"]:
             raise AxtbError(r["status"], self.last_error())
         return r
+
+
+def basis_size(numbers):
+    """(nsh, nao) of a molecule's GFN2 basis (axtb::basis::Basis, crates/axtb/src/basis.rs:165-229)."""
+    z = np.ascontiguousarray(numbers, dtype=np.int32)
+    m = Mol(len(z), z.ctypes.data, None, 0.0, -1)
+    nsh, nao = ctypes.c_int(), ctypes.c_int()
+    rc = lib().axtbhip_basis_size(ctypes.byref(m), ctypes.byref(nsh), ctypes.byref(nao))
+    if rc:
+        raise AxtbError(rc, "unsupported element")
+    return nsh.value, nao.value

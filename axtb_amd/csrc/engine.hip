@@ -215,6 +215,11 @@ struct MolIn {
   const double* xyz;
   double charge;
   int uhf;
+  // a given SCF start (xtb.rs:204-208): shell charges [nsh] (null: none), atomic dipoles
+  // [nat][3] and quadrupoles [nat][6] (null: zero)
+  const double* gq = nullptr;
+  const double* gdp = nullptr;
+  const double* gqp = nullptr;
 };
 
 struct MolOut {
@@ -751,6 +756,20 @@ struct Engine {
         r.msg = "the charge must be an integer";
         continue;
       }
+      if (m.gq) {
+        // a given start must be finite (calculator.rs:482-509)
+        int nsh = 0;
+        for (int a = 0; a < m.nat; a++) nsh += GFN2_NSHELL[m.z[a] - 1];
+        bool fin = true;
+        for (int k = 0; k < nsh; k++) fin = fin && std::isfinite(m.gq[k]);
+        for (int k = 0; m.gdp && k < 3 * m.nat; k++) fin = fin && std::isfinite(m.gdp[k]);
+        for (int k = 0; m.gqp && k < 6 * m.nat; k++) fin = fin && std::isfinite(m.gqp[k]);
+        if (!fin) {
+          r.status = AXTB_ERR_INPUT;
+          r.msg = "the SCF start has non-finite values";
+          continue;
+        }
+      }
       use.push_back((int)i);
     }
     if (use.empty()) {
@@ -809,6 +828,12 @@ struct Engine {
   std::vector<double> hxyz, hrcov, hen, hrad, hra, hrz, hdk, hqk, har, hav, hchi, heta, hkcn, herad, hreeq, hd4rcov,
       hd4zeff, hd4gi, hd4r4r2, hd4refcn, hd4refq, hsal, hsco, hse0, hskcn, hsshp, hsref, hshard, hsgam3, hsmin, hszeta;
   std::vector<int> map_use;  // batch molecule -> input index
+  // given starts of the batch molecules (flag per molecule; moments in the batch layout)
+  std::vector<int> hgflag;
+  std::vector<double> hgq, hgdp, hgqp;
+  DBuf<int> g_flag;
+  DBuf<double> g_q, g_dp, g_qp;
+  bool any_guess = false;
   int cap = 64;
   long long tot_mat = 0, tot_pair = 0, tot_shp = 0, tot_eq = 0, tot_hist = 0;
   int tot_vec = 0;
@@ -822,6 +847,10 @@ struct Engine {
       v->clear();
     mols.clear();
     map_use.clear();
+    hgflag.clear();
+    hgq.clear();
+    hgdp.clear();
+    hgqp.clear();
     cap = std::max(1, std::min(o.max_iter, 64));
     std::map<int, int> species;
     tot_mat = tot_pair = tot_shp = tot_eq = tot_hist = 0;
@@ -966,6 +995,10 @@ struct Engine {
       tot_hist += (long long)cap * M.ndim;
       mols.push_back(M);
       map_use.push_back(ii);
+      hgflag.push_back(m.gq != nullptr);
+      for (int k = 0; k < nsh; k++) hgq.push_back(m.gq ? m.gq[k] : 0.0);
+      for (int k = 0; k < 3 * m.nat; k++) hgdp.push_back(m.gq && m.gdp ? m.gdp[k] : 0.0);
+      for (int k = 0; k < 6 * m.nat; k++) hgqp.push_back(m.gq && m.gqp ? m.gqp[k] : 0.0);
       at0 += m.nat;
       sh0 += nsh;
       ao0 += nao;
@@ -1027,6 +1060,13 @@ struct Engine {
     s_alpha.upload(hsal, st); s_coeff.upload(hsco, st); s_se0.upload(hse0, st); s_kcn.upload(hskcn, st);
     s_shpoly.upload(hsshp, st); s_refocc.upload(hsref, st); s_hard.upload(hshard, st); s_gam3.upload(hsgam3, st);
     s_minal.upload(hsmin, st); s_zeta.upload(hszeta, st);
+    any_guess = std::find(hgflag.begin(), hgflag.end(), 1) != hgflag.end();
+    if (any_guess) {
+      g_flag.upload(hgflag, st);
+      g_q.upload(hgq, st);
+      g_dp.upload(hgdp, st);
+      g_qp.upload(hgqp, st);
+    }
     intcut_ = std::max(25.0 - 10.0 * std::log10(o.accuracy), 20.0);  // basis.rs:135-137
   }
   int nsp_ = 0;
@@ -1575,9 +1615,12 @@ struct Engine {
     int naomax = 0;
     for (auto& M : mols) naomax = std::max(naomax, M.nao);
     const long long stride = tot_mat;
-    // start moments -> q_in (dipoles and quadrupoles zero)
+    // start moments -> q_in (dipoles and quadrupoles zero); a given start replaces them
     fill(dp.p, 3LL * NA);
     fill(qp.p, 6LL * NA);
+    if (any_guess)
+      LAUNCH(k_start_guess, dim3(blocks(NA, 128)), dim3(128), 0, b, (const int*)g_flag.p, (const double*)g_q.p,
+             (const double*)g_dp.p, (const double*)g_qp.p, qstart.p, dp.p, qp.p);
     LAUNCH(k_pack, dim3(blocks(NA, 128)), dim3(128), 0, b, (const int*)nullptr, qstart.p, dp.p, qp.p,
                        qin.p, (const double*)nullptr, (double*)nullptr);
     double econv = 1e-6 * o.accuracy, pconv = 2e-5 * o.accuracy;
@@ -2105,14 +2148,35 @@ int axtbhip_basis_size(const axtb_mol* mol, int* nsh, int* nao) {
   return 0;
 }
 
-int axtbhip_gfn2_engrad_batch(axtbhip_context* ctx, const axtb_mol* mols, int nmol, const axtb_opts* opts,
-                              axtb_result* outs) {
+int axtbhip_gfn2_engrad_batch_guess(axtbhip_context* ctx, const axtb_mol* mols, int nmol, const axtb_guess* guesses,
+                                    const axtb_opts* opts, axtb_result* outs) {
   if (!ctx || !mols || !opts || !outs || nmol < 0) return AXTB_ERR_INPUT;
   std::vector<axtb::MolIn> in(nmol);
-  for (int i = 0; i < nmol; i++) in[i] = axtb::MolIn{mols[i].nat, mols[i].numbers, mols[i].positions, mols[i].charge, mols[i].uhf};
+  for (int i = 0; i < nmol; i++) {
+    in[i] = axtb::MolIn{mols[i].nat, mols[i].numbers, mols[i].positions, mols[i].charge, mols[i].uhf};
+    if (guesses && guesses[i].shell_charges) {
+      in[i].gq = guesses[i].shell_charges;
+      in[i].gdp = guesses[i].dipoles;
+      in[i].gqp = guesses[i].quadrupoles;
+    }
+  }
   std::vector<axtb::MolOut> res;
   try {
     ctx->eng->run(in, *opts, res);
+    // the warm start that did not converge: once more from the fresh start
+    std::vector<int> again;
+    for (int i = 0; i < nmol; i++)
+      if (in[i].gq && res[i].status == AXTB_ERR_SCF_NOT_CONVERGED) again.push_back(i);
+    if (!again.empty()) {
+      std::vector<axtb::MolIn> in2;
+      for (int i : again) {
+        in2.push_back(in[i]);
+        in2.back().gq = in2.back().gdp = in2.back().gqp = nullptr;
+      }
+      std::vector<axtb::MolOut> res2;
+      ctx->eng->run(in2, *opts, res2);
+      for (size_t k = 0; k < again.size(); k++) res[again[k]] = res2[k];
+    }
   } catch (axtb::Fail& f) {
     ctx->err = f.msg;
     for (int i = 0; i < nmol; i++) outs[i].status = f.code;
@@ -2132,6 +2196,11 @@ int axtbhip_gfn2_engrad_batch(axtbhip_context* ctx, const axtb_mol* mols, int nm
     }
   }
   return worst;
+}
+
+int axtbhip_gfn2_engrad_batch(axtbhip_context* ctx, const axtb_mol* mols, int nmol, const axtb_opts* opts,
+                              axtb_result* outs) {
+  return axtbhip_gfn2_engrad_batch_guess(ctx, mols, nmol, nullptr, opts, outs);
 }
 
 int axtbhip_gfn2_engrad(axtbhip_context* ctx, const axtb_mol* mol, const axtb_opts* opts, axtb_result* out) {

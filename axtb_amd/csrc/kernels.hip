@@ -707,6 +707,18 @@ __global__ void k_pack(Batch b, const int* active, const double* qsh, const doub
   for (int c = 0; c < 6; c++) put(M.nsh + 3 * M.nat + 6 * la + c, qp[6 * a + c] * ((c == 1 || c == 3 || c == 4) ? sq2 : 1.0));
 }
 
+// A given SCF start (xtb.rs:519-529, the moments of a previous geometry): the shell charges,
+// atomic dipoles and quadrupoles of the molecules with gflag set replace their EEQ or neutral
+// start. One thread per atom.
+__global__ void k_start_guess(Batch b, const int* gflag, const double* gq, const double* gdp, const double* gqp,
+                              double* qsh, double* dp, double* qp) {
+  int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= b.nat || !gflag[b.a.mol[a]]) return;
+  for (int s = b.a.ish[a]; s < b.a.ish[a] + b.a.nsh[a]; s++) qsh[s] = gq[s];
+  for (int k = 0; k < 3; k++) dp[3 * a + k] = gdp[3 * a + k];
+  for (int k = 0; k < 6; k++) qp[6 * a + k] = gqp[6 * a + k];
+}
+
 // v_sh = gamma q (+ Gamma q^2 with `third`): one wavefront per shell row (coulomb.rs:1006-1021,
 // 1141-1143).
 __global__ void k_shell_pot(Batch b, const int* active, const double* gam, const double* qsh, double* out, int third) {

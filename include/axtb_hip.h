@@ -83,6 +83,24 @@ int axtbhip_gfn2_engrad(axtbhip_context* ctx, const axtb_mol* mol, const axtb_op
 int axtbhip_gfn2_engrad_batch(axtbhip_context* ctx, const axtb_mol* mols, int nmol, const axtb_opts* opts,
                               axtb_result* outs);
 
+/* A given SCF start of one molecule: the converged moments of a previous geometry, as a
+ * Calculator keeps them between single points (ScfGuess, crates/axtb/src/xtb.rs:204-215; used
+ * at xtb.rs:519-529). Array sizes are those of the molecule (nsh from axtbhip_basis_size) and
+ * their layouts those of axtb_result. */
+typedef struct axtb_guess {
+  const double* shell_charges; /* [nsh]; NULL: no given start, opts->guess decides */
+  const double* dipoles;       /* [nat][3]; NULL: zero */
+  const double* quadrupoles;   /* [nat][6] (xx xy yy xz yz zz); NULL: zero */
+} axtb_guess;
+
+/* axtbhip_gfn2_engrad_batch with a given start per molecule (guesses may be NULL: none). A
+ * molecule that started from its given moments and did not converge is computed once more from
+ * the start opts->guess names, as Calculator::singlepoint_inner does with
+ * RetryLadder::warm_then_fresh (crates/axtb/src/calculator.rs:194-206,
+ * crates/base/src/method/retry.rs:150-158). */
+int axtbhip_gfn2_engrad_batch_guess(axtbhip_context* ctx, const axtb_mol* mols, int nmol, const axtb_guess* guesses,
+                                    const axtb_opts* opts, axtb_result* outs);
+
 /* A batch kept on the device between calls, for repeated or timed runs: prepare validates and
  * uploads the molecules (host arrays may be released afterwards), run computes on the resident
  * inputs, results downloads them (mols gives the array sizes). No counterpart in the reference:

@@ -52,8 +52,9 @@ TERMS = ("repulsion", "dispersion", "electronic", "isotropic_es", "third_order",
 
 
 def engrad(numbers, positions, charge=0.0, acc=1.0, etemp=300.0, max_iter=250, guess="eeq", gradient=True,
-           dispersion=True):
-    """GFN2-xTB energy (+ gradient) of one molecule; positions in bohr [N,3]."""
+           dispersion=True, start=None):
+    """GFN2-xTB energy (+ gradient) of one molecule; positions in bohr [N,3]. ``start``: a given
+    SCF start {"qsh", "dipoles", "quadrupoles"} (the moments of a previous result), else ``guess``."""
     z = np.ascontiguousarray(numbers, dtype=np.int32)
     x = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
     nat = len(z)
@@ -67,10 +68,15 @@ def engrad(numbers, positions, charge=0.0, acc=1.0, etemp=300.0, max_iter=250, g
     dp = np.zeros((nat, 3))
     qp = np.zeros((nat, 6))
     emo = np.zeros(nao)
-    rc = lib().oracle_engrad(nat, _p(z), _p(x), ctypes.c_double(charge), ctypes.c_double(acc),
-                             ctypes.c_double(etemp), max_iter, 0 if guess == "eeq" else 1, int(gradient),
-                             int(dispersion), ctypes.byref(e), _p(terms), _p(g), _p(qat), _p(qsh), _p(dp), _p(qp),
-                             _p(emo), ctypes.byref(it))
+    gs = [None, None, None]
+    if start is not None:
+        gs = [np.ascontiguousarray(start[k], dtype=np.float64).reshape(-1) for k in ("qsh", "dipoles", "quadrupoles")]
+        if gs[0].size != nsh or gs[1].size != 3 * nat or gs[2].size != 6 * nat:
+            raise OracleError(1, "the start does not have the molecule's sizes")
+    rc = lib().oracle_engrad_guess(nat, _p(z), _p(x), ctypes.c_double(charge), ctypes.c_double(acc),
+                                   ctypes.c_double(etemp), max_iter, 0 if guess == "eeq" else 1, int(gradient),
+                                   int(dispersion), _p(gs[0]), _p(gs[1]), _p(gs[2]), ctypes.byref(e), _p(terms),
+                                   _p(g), _p(qat), _p(qsh), _p(dp), _p(qp), _p(emo), ctypes.byref(it))
     _check(rc)
     return {"energy": e.value, "terms": dict(zip(TERMS, terms)), "gradient": g if gradient else None,
             "qat": qat, "qsh": qsh, "dipoles": dp, "quadrupoles": qp, "emo": emo, "iterations": it.value}

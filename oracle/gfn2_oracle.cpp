@@ -1577,6 +1577,9 @@ struct Broyden {
 struct Options {
   double acc = 1.0, etemp = 300.0, damping = 0.4;
   int max_iter = 250, guess = 0 /*0 EEQ, 1 SAD*/, want_grad = 1, dispersion = 1;
+  // a given start (ScfGuess, xtb.rs:204-208): shell charges [nsh] (null: none), dipoles [3N],
+  // quadrupoles [6N] (null: zero)
+  const double *gq = nullptr, *gdp = nullptr, *gqp = nullptr;
 };
 
 struct Result {
@@ -1702,7 +1705,12 @@ Result singlepoint(int nat, const int* z, const double* xyz, double charge, cons
   start.qsh.assign(nsh, 0.0);
   start.dp.assign(3 * nat, 0.0);
   start.qp.assign(6 * nat, 0.0);
-  if (opt.guess == 0) {
+  if (opt.gq) {
+    // the given moments (xtb.rs:522-528)
+    for (int s = 0; s < nsh; s++) start.qsh[s] = opt.gq[s];
+    if (opt.gdp) start.dp.assign(opt.gdp, opt.gdp + 3 * nat);
+    if (opt.gqp) start.qp.assign(opt.gqp, opt.gqp + 6 * nat);
+  } else if (opt.guess == 0) {
     try {
       vec q = eeq_charges(nat, z, xyz, charge);
       for (int iat = 0; iat < nat; iat++) {
@@ -2028,11 +2036,15 @@ const char* oracle_last_error() { return last_error.c_str(); }
 // GFN2-xTB single point. Outputs (caller-owned, may be null): energy, terms[8], grad[3N], qat[N],
 // qsh[nsh], dp[3N], qp[6N], emo[nao], iterations. Returns 0 ok, 1 input, 2 numerical,
 // 3 SCF not converged, 4 unsupported element.
-int oracle_engrad(int nat, const int* z, const double* xyz, double charge, double acc, double etemp, int max_iter,
-                  int guess, int want_grad, int dispersion, double* energy, double* terms, double* grad, double* qat,
-                  double* qsh, double* dp, double* qp, double* emo, int* iterations) {
+int oracle_engrad_guess(int nat, const int* z, const double* xyz, double charge, double acc, double etemp,
+                        int max_iter, int guess, int want_grad, int dispersion, const double* gq, const double* gdp,
+                        const double* gqp, double* energy, double* terms, double* grad, double* qat, double* qsh,
+                        double* dp, double* qp, double* emo, int* iterations) {
   try {
     Options o;
+    o.gq = gq;
+    o.gdp = gdp;
+    o.gqp = gqp;
     o.acc = acc;
     o.etemp = etemp;
     o.max_iter = max_iter;
@@ -2057,6 +2069,13 @@ int oracle_engrad(int nat, const int* z, const double* xyz, double charge, doubl
     last_error = e.what();
     return 2;
   }
+}
+
+int oracle_engrad(int nat, const int* z, const double* xyz, double charge, double acc, double etemp, int max_iter,
+                  int guess, int want_grad, int dispersion, double* energy, double* terms, double* grad, double* qat,
+                  double* qsh, double* dp, double* qp, double* emo, int* iterations) {
+  return oracle_engrad_guess(nat, z, xyz, charge, acc, etemp, max_iter, guess, want_grad, dispersion, nullptr, nullptr,
+                             nullptr, energy, terms, grad, qat, qsh, dp, qp, emo, iterations);
 }
 
 // EEQ-2019 charges of the SCF start (axtb/src/eeq.rs:43-77)

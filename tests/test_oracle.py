@@ -174,3 +174,36 @@ def test_errors():
     with pytest.raises(oracle.OracleError) as e:
         oracle.engrad(z, x, charge=0.5)
     assert e.value.code == 1
+
+
+WATER_RS = ([8, 1, 1], np.array([[0.0, 0.0, 0.22], [0.0, 1.43, -0.89], [0.0, -1.43, -0.89]]))
+
+
+def bad_start(nsh, nat):
+    """The start the reference's calculator test cannot recover from in 9 iterations
+    (crates/axtb/tests/calculator_threads.rs:26-36)."""
+    return {"qsh": np.array([4.0 if i % 2 == 0 else -4.0 for i in range(nsh)]),
+            "dipoles": np.array([[3.0, -3.0, 3.0]] * nat), "quadrupoles": np.array([[5.0, 0.0, -5.0, 0.0, 0.0, 0.0]] * nat)}
+
+
+def test_warm_start_golden_iterations():
+    """A given SCF start (xtb.rs:519-529) against the counts of the reference's own test
+    (calculator_threads.rs:19): water converges in 8 iterations from the EEQ start and needs 11
+    from the bad start (so it fails at max_iter 9), to the same energy; its own converged moments
+    are a fixed point (2 iterations, the same energy)."""
+    z, x = WATER_RS
+    f = oracle.engrad(z, x)
+    assert f["iterations"] == 8
+    b = oracle.engrad(z, x, start=bad_start(len(f["qsh"]), 3))
+    assert b["iterations"] == 11
+    assert abs(b["energy"] - f["energy"]) < 1e-9
+    with pytest.raises(oracle.OracleError) as e:
+        oracle.engrad(z, x, start=bad_start(len(f["qsh"]), 3), max_iter=9)
+    assert e.value.code == 3
+    w = oracle.engrad(z, x, start={"qsh": f["qsh"], "dipoles": f["dipoles"], "quadrupoles": f["quadrupoles"]})
+    assert w["iterations"] <= 2
+    assert abs(w["energy"] - f["energy"]) < 1e-10
+    # a start that drops the multipoles is no fixed point: more iterations, the same energy
+    q = oracle.engrad(z, x, start={"qsh": f["qsh"], "dipoles": 0 * f["dipoles"], "quadrupoles": 0 * f["quadrupoles"]})
+    assert q["iterations"] > w["iterations"]
+    assert abs(q["energy"] - f["energy"]) < 1e-9
