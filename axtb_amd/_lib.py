@@ -16,7 +16,7 @@ ERROR_KINDS = {1: "input", 2: "numerical", 3: "scf_not_converged", 4: "unsupport
 TERMS = ("repulsion", "dispersion", "electronic", "isotropic_es", "third_order", "anisotropic", "entropy", "band")
 EXPORTS = ("axtbhip_version", "axtbhip_context_create", "axtbhip_context_destroy", "axtbhip_last_error",
            "axtbhip_basis_size", "axtbhip_gfn2_engrad", "axtbhip_gfn2_engrad_batch", "axtbhip_gfn2_engrad_batch_guess",
-           "axtbhip_batch_prepare",
+           "axtbhip_batch_prepare", "axtbhip_batch_prepare_guess",
            "axtbhip_batch_run", "axtbhip_batch_gradient", "axtbhip_batch_results", "axtbhip_profile_enable",
            "axtbhip_profile_report", "axtbhip_synchronize")
 
@@ -76,6 +76,8 @@ def lib():
                                                       ctypes.POINTER(Guess), ctypes.POINTER(Opts),
                                                       ctypes.POINTER(Result)]
         L.axtbhip_batch_prepare.argtypes = [ctypes.c_void_p, ctypes.POINTER(Mol), ctypes.c_int, ctypes.POINTER(Opts)]
+        L.axtbhip_batch_prepare_guess.argtypes = [ctypes.c_void_p, ctypes.POINTER(Mol), ctypes.c_int,
+                                                  ctypes.POINTER(Guess), ctypes.POINTER(Opts)]
         L.axtbhip_batch_run.argtypes = [ctypes.c_void_p]
         L.axtbhip_batch_gradient.argtypes = [ctypes.c_void_p]
         L.axtbhip_batch_results.argtypes = [ctypes.c_void_p, ctypes.POINTER(Mol), ctypes.c_int,
@@ -130,27 +132,39 @@ class Context:
         else:
             if len(starts) != len(molecules):
                 raise AxtbError(ERR_INPUT, "one start per molecule")
-            gs = (Guess * len(molecules))()
-            for i, st in enumerate(starts):
-                if st is None:
-                    continue
-                nat = mols[i].nat
-                arrs = [np.ascontiguousarray(st[k], dtype=np.float64).reshape(-1)
-                        for k in ("shell_charges", "dipoles", "quadrupoles")]
-                nsh = ctypes.c_int()
-                lib().axtbhip_basis_size(ctypes.byref(mols[i]), ctypes.byref(nsh), None)
-                if arrs[0].size != nsh.value or arrs[1].size != 3 * nat or arrs[2].size != 6 * nat:
-                    raise AxtbError(ERR_INPUT, f"start {i} needs {nsh.value} shell charges and {nat} atomic "
-                                               f"dipoles/quadrupoles")
-                keep += arrs
-                gs[i] = Guess(arrs[0].ctypes.data, arrs[1].ctypes.data, arrs[2].ctypes.data)
+            gs = self._starts(mols, starts, keep)
             rc = lib().axtbhip_gfn2_engrad_batch_guess(self._h, mols, len(molecules), gs, ctypes.byref(opts), res)
         return self._unmarshal(rc, res, outs, opts, raise_on_error)
 
-    def prepare(self, molecules, opts):
-        """Upload a batch once; run() then computes on the resident inputs, results() downloads."""
+    def _starts(self, mols, starts, keep):
+        """The axtb_guess array of per-molecule starts (None: no start); checks the sizes."""
+        if len(starts) != len(mols):
+            raise AxtbError(ERR_INPUT, "one start per molecule")
+        gs = (Guess * len(mols))()
+        for i, st in enumerate(starts):
+            if st is None:
+                continue
+            nat = mols[i].nat
+            arrs = [np.ascontiguousarray(st[k], dtype=np.float64).reshape(-1)
+                    for k in ("shell_charges", "dipoles", "quadrupoles")]
+            nsh = ctypes.c_int()
+            lib().axtbhip_basis_size(ctypes.byref(mols[i]), ctypes.byref(nsh), None)
+            if arrs[0].size != nsh.value or arrs[1].size != 3 * nat or arrs[2].size != 6 * nat:
+                raise AxtbError(ERR_INPUT, f"start {i} needs {nsh.value} shell charges and {nat} atomic "
+                                           f"dipoles/quadrupoles")
+            keep += arrs
+            gs[i] = Guess(arrs[0].ctypes.data, arrs[1].ctypes.data, arrs[2].ctypes.data)
+        return gs
+
+    def prepare(self, molecules, opts, starts=None):
+        """Upload a batch once (with optional per-molecule starts); run() then computes on the
+        resident inputs, results() downloads."""
         self._prep = self._marshal(molecules) + (opts,)
-        rc = lib().axtbhip_batch_prepare(self._h, self._prep[0], len(molecules), ctypes.byref(opts))
+        if starts is None:
+            rc = lib().axtbhip_batch_prepare(self._h, self._prep[0], len(molecules), ctypes.byref(opts))
+        else:
+            gs = self._starts(self._prep[0], starts, self._prep[2])
+            rc = lib().axtbhip_batch_prepare_guess(self._h, self._prep[0], len(molecules), gs, ctypes.byref(opts))
         if rc:
             raise AxtbError(rc, self.last_error())
 

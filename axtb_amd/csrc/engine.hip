@@ -2207,10 +2207,18 @@ int axtbhip_gfn2_engrad(axtbhip_context* ctx, const axtb_mol* mol, const axtb_op
   return axtbhip_gfn2_engrad_batch(ctx, mol, 1, opts, out);
 }
 
-int axtbhip_batch_prepare(axtbhip_context* ctx, const axtb_mol* mols, int nmol, const axtb_opts* opts) {
+int axtbhip_batch_prepare_guess(axtbhip_context* ctx, const axtb_mol* mols, int nmol, const axtb_guess* guesses,
+                                const axtb_opts* opts) {
   if (!ctx || !mols || !opts || nmol < 0) return AXTB_ERR_INPUT;
   std::vector<axtb::MolIn> in(nmol);
-  for (int i = 0; i < nmol; i++) in[i] = axtb::MolIn{mols[i].nat, mols[i].numbers, mols[i].positions, mols[i].charge, mols[i].uhf};
+  for (int i = 0; i < nmol; i++) {
+    in[i] = axtb::MolIn{mols[i].nat, mols[i].numbers, mols[i].positions, mols[i].charge, mols[i].uhf};
+    if (guesses && guesses[i].shell_charges) {
+      in[i].gq = guesses[i].shell_charges;
+      in[i].gdp = guesses[i].dipoles;
+      in[i].gqp = guesses[i].quadrupoles;
+    }
+  }
   try {
     ctx->eng->prepare(in, *opts, ctx->prepared);
     ctx->err.clear();
@@ -2222,6 +2230,10 @@ int axtbhip_batch_prepare(axtbhip_context* ctx, const axtb_mol* mols, int nmol, 
     ctx->err = e.what();
     return AXTB_ERR_DEVICE;
   }
+}
+
+int axtbhip_batch_prepare(axtbhip_context* ctx, const axtb_mol* mols, int nmol, const axtb_opts* opts) {
+  return axtbhip_batch_prepare_guess(ctx, mols, nmol, nullptr, opts);
 }
 
 int axtbhip_batch_run(axtbhip_context* ctx) {

@@ -8,7 +8,9 @@ default conventions, SCC energy convergence 1e-8 Eh (accuracy 1e-2), 300 K, D4, 
 A step is one energy + gradient pass over the whole batch (setup, SCC with per-molecule
 convergence masks, gradient), inputs resident on the device. Other workloads: --workload
 water1000 / water200 / c100h202 / water20 / c30h62 (one molecule per step), conformers2048
-(2048 C30H62 conformers, 0.1 A noise, one batch per step).
+(2048 C30H62 conformers, 0.1 A noise, one batch per step). --warm: a trajectory step of the
+workload (every coordinate moved by 0.02 bohr, each molecule started from the converged moments
+of its previous geometry).
 
 Precision (default both): the workload is run in plain FP64 and in mixed precision (FP32 Jacobi
 eigensolves and FP64 Ogita-Aishima refinement for the batched eigenproblems, n <= 200; larger
@@ -76,13 +78,13 @@ def cpu_baseline(mols, budget_s, acc):
     return n / dt, n, dt
 
 
-def measure(mols, precision, steps, warmup, acc):
+def measure(mols, precision, steps, warmup, acc, starts=None):
     """Prepare the batch once, run `warmup` untimed and `steps` timed passes; returns the wall time of
     the timed passes, the HIP-event kernel profile of those passes and the results."""
     from axtb_amd import _lib
     opts = _lib.make_opts(accuracy=acc, etemp=300.0, precision=precision, gradient=True)
     ctx = _lib.Context(0)
-    ctx.prepare(mols, opts)
+    ctx.prepare(mols, opts, starts=starts)
     for i in range(warmup):
         ctx.run()
         print(f"[bench] {precision} warmup {i + 1}/{warmup} done", file=sys.stderr, flush=True)
@@ -124,6 +126,9 @@ def main():
     ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed"],
                     help="both (default): FP64 and mixed on the same inputs, value = mixed, with the FP64 "
                          "rate and the mixed-vs-FP64 differences; or one of fp64 / mixed alone")
+    ap.add_argument("--warm", action="store_true",
+                    help="a trajectory step: every molecule moved by 0.02 bohr (seeded) and started from the "
+                         "converged moments of its previous geometry (computed untimed)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check", type=int, default=8, help="molecules checked against the oracle after timing")
@@ -149,9 +154,19 @@ def main():
     mols = workload(args.workload)
     print(f"[bench] workload {args.workload}: {len(mols)} molecules, {sum(len(m[0]) for m in mols)} atoms",
           file=sys.stderr, flush=True)
+    starts = None
+    if args.warm:
+        from axtb_amd import _lib
+        c0 = _lib.Context(0)
+        r0 = c0.engrad_batch(mols, _lib.make_opts(accuracy=acc), raise_on_error=True)
+        c0.close()
+        starts = [{k: r[k] for k in ("shell_charges", "dipoles", "quadrupoles")} for r in r0]
+        rng = np.random.default_rng(1)
+        mols = [(m[0], np.asarray(m[1]) + rng.normal(scale=0.02, size=np.shape(m[1]))) + tuple(m[2:]) for m in mols]
+        print("[bench] trajectory step: previous geometry computed, starts taken", file=sys.stderr, flush=True)
     runs = {}
     for prec in (["fp64", "mixed"] if args.precision == "both" else [args.precision]):
-        runs[prec] = measure(mols, prec, args.steps, args.warmup, acc)
+        runs[prec] = measure(mols, prec, args.steps, args.warmup, acc, starts)
     main_prec = "mixed" if "mixed" in runs else "fp64"
     dt, prof, res = runs[main_prec]
     bad = [i for i, r in enumerate(res) if r["status"] != 0]
@@ -161,7 +176,9 @@ def main():
     import oracle
     checked = {p: [0.0] * 4 for p in runs}
     for i in range(min(args.check, len(mols))):
-        o = oracle.engrad(mols[i][0], mols[i][1], acc=acc)
+        st = None if starts is None else {"qsh": starts[i]["shell_charges"], "dipoles": starts[i]["dipoles"],
+                                          "quadrupoles": starts[i]["quadrupoles"]}
+        o = oracle.engrad(mols[i][0], mols[i][1], acc=acc, start=st)
         ref = {"status": 0, "energy": o["energy"], "gradient": o["gradient"], "charges": o["qat"],
                "orbital_energies": o["emo"]}
         for p, (_, _, rs) in runs.items():
@@ -212,7 +229,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if main_prec == "fp64" else "f32+f64",
         "data": "synthetic (seeded molecules of the benchmark configuration)",
-        "config": {"workload": args.workload, "molecules": len(mols), "precision": main_prec,
+        "config": {"workload": args.workload + (" trajectory step (0.02 bohr, warm start)" if args.warm else ""),
+                   "molecules": len(mols), "precision": main_prec,
                    "accuracy": acc, "etemp_K": 300.0, "mean_scf_iterations": float(np.mean(iters)) if iters else None,
                    "failed": len(bad), "parity_checked": min(args.check, len(mols)), **cfg_extra,
                    "mean_jacobi_sweeps": sweeps, "refinement": refinement},

@@ -106,6 +106,12 @@ int axtbhip_gfn2_engrad_batch_guess(axtbhip_context* ctx, const axtb_mol* mols, 
  * inputs, results downloads them (mols gives the array sizes). No counterpart in the reference:
  * its calculators own their inputs the same way (crates/axtb/src/calculator.rs:31-52). */
 int axtbhip_batch_prepare(axtbhip_context* ctx, const axtb_mol* mols, int nmol, const axtb_opts* opts);
+/* batch_prepare with a given start per molecule (guesses may be NULL), copied with the batch: every
+ * following batch_run starts those molecules from them (a trajectory step of many molecules with
+ * its inputs resident). No retry here: a warm start that does not converge reports
+ * AXTB_ERR_SCF_NOT_CONVERGED, and the caller runs it again without a start. */
+int axtbhip_batch_prepare_guess(axtbhip_context* ctx, const axtb_mol* mols, int nmol, const axtb_guess* guesses,
+                                const axtb_opts* opts);
 int axtbhip_batch_run(axtbhip_context* ctx);
 int axtbhip_batch_results(axtbhip_context* ctx, const axtb_mol* mols, int nmol, axtb_result* outs);
 /* The adjoint step on the resident batch: after a batch_run with want_gradient = 0, the analytic

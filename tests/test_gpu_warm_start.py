@@ -118,6 +118,13 @@ def test_trajectory_batch(ctx, precision):
     de = max(abs(w["energy"] - f["energy"]) for w, f in zip(warm, fresh))
     print(f"iterations warm {it_w:.2f} fresh {it_f:.2f}; warm vs fresh max |dE| {de:.1e}", flush=True)
     assert it_w < it_f - 1.0  # the oracle: 9.4 against 11.8 on every 16th molecule
+    # the resident batch with the same starts (axtbhip_batch_prepare_guess): the same bits, twice
+    ctx.prepare(mols1, opts, starts=starts)
+    for _ in range(2):
+        ctx.run()
+        res = ctx.results(raise_on_error=True)
+        assert all(a["energy"] == b["energy"] and np.array_equal(a["gradient"], b["gradient"])
+                   and a["iterations"] == b["iterations"] for a, b in zip(res, warm))
     if precision == "mixed":
         w64 = ctx.engrad_batch(mols1, opts64, raise_on_error=True, starts=starts)
         for w, f in zip(warm, w64):
