@@ -201,7 +201,7 @@ def main():
         pk = PEAK_FP32_MFMA if name.endswith("_f32") else PEAK_FP64_MFMA
         bound, peak, unit, achieved = "mfma", pk / 1e12, "TFLOP/s", per_launch / avg_s / 1e12
 
-    traffic = measured_traffic(name, args.workload)
+    traffic = None if args.warm else measured_traffic(name, args.workload)  # counters exist for fresh starts only
     value = len(mols) * args.steps / dt
     names = ("dE", "dg", "dq", "deps")
     cfg_extra = {}
