@@ -16,7 +16,7 @@ ERROR_KINDS = {1: "input", 2: "numerical", 3: "scf_not_converged", 4: "unsupport
 TERMS = ("repulsion", "dispersion", "electronic", "isotropic_es", "third_order", "anisotropic", "entropy", "band")
 EXPORTS = ("axtbhip_version", "axtbhip_context_create", "axtbhip_context_destroy", "axtbhip_last_error",
            "axtbhip_basis_size", "axtbhip_gfn2_engrad", "axtbhip_gfn2_engrad_batch", "axtbhip_gfn2_engrad_batch_guess",
-           "axtbhip_batch_prepare", "axtbhip_batch_prepare_guess",
+           "axtbhip_batch_prepare", "axtbhip_batch_prepare_guess", "axtbhip_batch_update",
            "axtbhip_batch_run", "axtbhip_batch_gradient", "axtbhip_batch_results", "axtbhip_profile_enable",
            "axtbhip_profile_report", "axtbhip_synchronize")
 
@@ -78,6 +78,7 @@ def lib():
         L.axtbhip_batch_prepare.argtypes = [ctypes.c_void_p, ctypes.POINTER(Mol), ctypes.c_int, ctypes.POINTER(Opts)]
         L.axtbhip_batch_prepare_guess.argtypes = [ctypes.c_void_p, ctypes.POINTER(Mol), ctypes.c_int,
                                                   ctypes.POINTER(Guess), ctypes.POINTER(Opts)]
+        L.axtbhip_batch_update.argtypes = [ctypes.c_void_p, ctypes.POINTER(Mol), ctypes.c_int, ctypes.POINTER(Guess)]
         L.axtbhip_batch_run.argtypes = [ctypes.c_void_p]
         L.axtbhip_batch_gradient.argtypes = [ctypes.c_void_p]
         L.axtbhip_batch_results.argtypes = [ctypes.c_void_p, ctypes.POINTER(Mol), ctypes.c_int,
@@ -165,6 +166,15 @@ class Context:
         else:
             gs = self._starts(self._prep[0], starts, self._prep[2])
             rc = lib().axtbhip_batch_prepare_guess(self._h, self._prep[0], len(molecules), gs, ctypes.byref(opts))
+        if rc:
+            raise AxtbError(rc, self.last_error())
+
+    def update(self, molecules, starts=None):
+        """New positions (and starts) for the prepared batch's molecules (same atoms, same order);
+        the next run() computes them. Raises on any mismatch and leaves the batch unchanged."""
+        mols, res, keep, outs = self._marshal(molecules)
+        gs = self._starts(mols, starts, keep) if starts is not None else None
+        rc = lib().axtbhip_batch_update(self._h, mols, len(molecules), gs)
         if rc:
             raise AxtbError(rc, self.last_error())
 

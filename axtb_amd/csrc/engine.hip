@@ -708,6 +708,62 @@ struct Engine {
   }
 
   // ------------------------------------------------------------------------------ one batch
+  // a given start must be finite (calculator.rs:482-509); the elements are checked before
+  static bool finite_start(const MolIn& m) {
+    if (!m.gq) return true;
+    int nsh = 0;
+    for (int a = 0; a < m.nat; a++) nsh += GFN2_NSHELL[m.z[a] - 1];
+    bool fin = true;
+    for (int k = 0; k < nsh; k++) fin = fin && std::isfinite(m.gq[k]);
+    for (int k = 0; m.gdp && k < 3 * m.nat; k++) fin = fin && std::isfinite(m.gdp[k]);
+    for (int k = 0; m.gqp && k < 6 * m.nat; k++) fin = fin && std::isfinite(m.gqp[k]);
+    return fin;
+  }
+
+  // new positions and starts for the molecules of the prepared batch (same atoms in the same
+  // order; `in` indexed as at prepare). Everything is checked before anything changes: a
+  // mismatch or an invalid geometry or start fails the call and leaves the batch as it was.
+  void update(const std::vector<MolIn>& in) {
+    HIP_OK(hipSetDevice(device));
+    for (int m = 0; m < (int)mols.size(); m++) {
+      const Mol& M = mols[m];
+      const MolIn& x = in[map_use[m]];
+      const std::string id = "molecule " + std::to_string(map_use[m]) + ": ";
+      if (x.nat != M.nat) throw Fail{AXTB_ERR_INPUT, id + "the atom count differs from the prepared batch"};
+      for (int a = 0; a < M.nat; a++) {
+        if (x.z[a] != hz[M.at0 + a]) throw Fail{AXTB_ERR_INPUT, id + "the elements differ from the prepared batch"};
+        for (int k = 0; k < 3; k++)
+          if (!std::isfinite(x.xyz[3 * a + k])) throw Fail{AXTB_ERR_INPUT, id + "non-finite coordinate"};
+        for (int c = 0; c < a; c++) {
+          double d2 = 0;
+          for (int k = 0; k < 3; k++) d2 += std::pow(x.xyz[3 * a + k] - x.xyz[3 * c + k], 2);
+          if (!(d2 >= 1e-6)) throw Fail{AXTB_ERR_INPUT, id + "atoms " + std::to_string(c) + " and " + std::to_string(a) + " coincide"};
+        }
+      }
+      if (!finite_start(x)) throw Fail{AXTB_ERR_INPUT, id + "the SCF start has non-finite values"};
+    }
+    for (int m = 0; m < (int)mols.size(); m++) {
+      const Mol& M = mols[m];
+      const MolIn& x = in[map_use[m]];
+      std::copy(x.xyz, x.xyz + 3 * M.nat, hxyz.begin() + 3 * M.at0);
+      hgflag[m] = x.gq != nullptr;
+      for (int k = 0; k < M.nsh; k++) hgq[M.sh0 + k] = x.gq ? x.gq[k] : 0.0;
+      for (int k = 0; k < 3 * M.nat; k++) hgdp[3 * M.at0 + k] = x.gq && x.gdp ? x.gdp[k] : 0.0;
+      for (int k = 0; k < 6 * M.nat; k++) hgqp[6 * M.at0 + k] = x.gq && x.gqp ? x.gqp[k] : 0.0;
+    }
+    if (mols.empty()) return;
+    a_xyz.upload(hxyz, st);
+    any_guess = std::find(hgflag.begin(), hgflag.end(), 1) != hgflag.end();
+    if (any_guess) {
+      g_flag.upload(hgflag, st);
+      g_q.upload(hgq, st);
+      g_dp.upload(hgdp, st);
+      g_qp.upload(hgqp, st);
+    }
+    HIP_OK(hipStreamSynchronize(st));
+    scf_done = grad_done = false;
+  }
+
   // validate and upload a batch (per-molecule input errors are reported in `out`)
   void prepare(const std::vector<MolIn>& in, const axtb_opts& o, std::vector<MolOut>& out) {
     HIP_OK(hipSetDevice(device));
@@ -756,19 +812,10 @@ struct Engine {
         r.msg = "the charge must be an integer";
         continue;
       }
-      if (m.gq) {
-        // a given start must be finite (calculator.rs:482-509)
-        int nsh = 0;
-        for (int a = 0; a < m.nat; a++) nsh += GFN2_NSHELL[m.z[a] - 1];
-        bool fin = true;
-        for (int k = 0; k < nsh; k++) fin = fin && std::isfinite(m.gq[k]);
-        for (int k = 0; m.gdp && k < 3 * m.nat; k++) fin = fin && std::isfinite(m.gdp[k]);
-        for (int k = 0; m.gqp && k < 6 * m.nat; k++) fin = fin && std::isfinite(m.gqp[k]);
-        if (!fin) {
-          r.status = AXTB_ERR_INPUT;
-          r.msg = "the SCF start has non-finite values";
-          continue;
-        }
+      if (!finite_start(m)) {
+        r.status = AXTB_ERR_INPUT;
+        r.msg = "the SCF start has non-finite values";
+        continue;
       }
       use.push_back((int)i);
     }
@@ -2234,6 +2281,30 @@ int axtbhip_batch_prepare_guess(axtbhip_context* ctx, const axtb_mol* mols, int 
 
 int axtbhip_batch_prepare(axtbhip_context* ctx, const axtb_mol* mols, int nmol, const axtb_opts* opts) {
   return axtbhip_batch_prepare_guess(ctx, mols, nmol, nullptr, opts);
+}
+
+int axtbhip_batch_update(axtbhip_context* ctx, const axtb_mol* mols, int nmol, const axtb_guess* guesses) {
+  if (!ctx || !mols || nmol != (int)ctx->prepared.size()) return AXTB_ERR_INPUT;
+  std::vector<axtb::MolIn> in(nmol);
+  for (int i = 0; i < nmol; i++) {
+    in[i] = axtb::MolIn{mols[i].nat, mols[i].numbers, mols[i].positions, mols[i].charge, mols[i].uhf};
+    if (guesses && guesses[i].shell_charges) {
+      in[i].gq = guesses[i].shell_charges;
+      in[i].gdp = guesses[i].dipoles;
+      in[i].gqp = guesses[i].quadrupoles;
+    }
+  }
+  try {
+    ctx->eng->update(in);
+    ctx->err.clear();
+    return 0;
+  } catch (axtb::Fail& f) {
+    ctx->err = f.msg;
+    return f.code;
+  } catch (std::exception& e) {
+    ctx->err = e.what();
+    return AXTB_ERR_DEVICE;
+  }
 }
 
 int axtbhip_batch_run(axtbhip_context* ctx) {

@@ -112,6 +112,13 @@ int axtbhip_batch_prepare(axtbhip_context* ctx, const axtb_mol* mols, int nmol, 
  * AXTB_ERR_SCF_NOT_CONVERGED, and the caller runs it again without a start. */
 int axtbhip_batch_prepare_guess(axtbhip_context* ctx, const axtb_mol* mols, int nmol, const axtb_guess* guesses,
                                 const axtb_opts* opts);
+/* New positions (and starts; guesses may be NULL: none) for the molecules of the prepared batch, the
+ * next step of a trajectory without a new prepare: mols must hold the prepared molecules' atoms in
+ * the same order (nmol as at prepare; molecules that failed there are skipped). Checked as at
+ * prepare before anything changes; any mismatch returns AXTB_ERR_INPUT and keeps the batch. The
+ * counterpart of Method::singlepoint(new positions) on a calculator that keeps its molecule
+ * (crates/axtb/src/calculator.rs:194-206). */
+int axtbhip_batch_update(axtbhip_context* ctx, const axtb_mol* mols, int nmol, const axtb_guess* guesses);
 int axtbhip_batch_run(axtbhip_context* ctx);
 int axtbhip_batch_results(axtbhip_context* ctx, const axtb_mol* mols, int nmol, axtb_result* outs);
 /* The adjoint step on the resident batch: after a batch_run with want_gradient = 0, the analytic

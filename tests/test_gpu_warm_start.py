@@ -125,6 +125,20 @@ def test_trajectory_batch(ctx, precision):
         res = ctx.results(raise_on_error=True)
         assert all(a["energy"] == b["energy"] and np.array_equal(a["gradient"], b["gradient"])
                    and a["iterations"] == b["iterations"] for a, b in zip(res, warm))
+    # the same step as an update of the resident batch of the previous geometry
+    # (axtbhip_batch_update): the same bits; a mismatched update fails and keeps the batch
+    ctx.prepare(mols, opts)
+    ctx.run()
+    ctx.update(mols1, starts=starts)
+    bad = list(mols1)
+    bad[3] = (bad[3][0][:-1], bad[3][1][:-1])
+    with pytest.raises(_lib.AxtbError) as e:
+        ctx.update(bad, starts=None)
+    assert e.value.code == _lib.ERR_INPUT
+    ctx.run()
+    res = ctx.results(raise_on_error=True)
+    assert all(a["energy"] == b["energy"] and np.array_equal(a["gradient"], b["gradient"])
+               and a["iterations"] == b["iterations"] for a, b in zip(res, warm))
     if precision == "mixed":
         w64 = ctx.engrad_batch(mols1, opts64, raise_on_error=True, starts=starts)
         for w, f in zip(warm, w64):
