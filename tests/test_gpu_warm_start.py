@@ -151,3 +151,27 @@ def test_trajectory_batch(ctx, precision):
         assert warm[i]["iterations"] == o["iterations"]
         assert abs(warm[i]["energy"] - o["energy"]) <= 1e-8
         assert np.abs(warm[i]["gradient"] - o["gradient"]).max() <= 1e-7
+
+
+def test_session_lineages(ctx):
+    """Session.compute_many (crates/axtb-py/src/engine.rs:403-470, 836-843): two trajectories of
+    C10H22 as two lineages, advanced side by side in GPU batches, give the results of two
+    calculators stepped one geometry at a time (each from its previous moments)."""
+    from axtb_amd.calculator import Calculator
+    from axtb_amd.session import Session
+    from axtb_amd import structures as S
+    z, x0 = S.alkane(10, rng=np.random.default_rng(7), noise=0.05)
+    rng = np.random.default_rng(8)
+    traj = {l: [x0 + rng.normal(scale=0.03, size=x0.shape) for _ in range(3)] for l in ("p", "q")}
+    print("session of two C10H22 lineages", flush=True)
+    reqs = [dict(positions=traj[l][k], lineage=l) for k in range(3) for l in ("p", "q")]
+    out = Session(z, context=ctx).compute_many(reqs)
+    assert [r["from"] for r in out] == [None, None, "p", "q", "p", "q"]
+    for l in ("p", "q"):
+        calc = Calculator(z, x0, context=ctx)
+        got = [r for q, r in zip(reqs, out) if q["lineage"] == l]
+        for x, r in zip(traj[l], got):
+            sp = calc.singlepoint(x)
+            assert sp.electronic.scf.iterations == r["iterations"]
+            assert abs(sp.energy - r["energy"]) <= 1e-12
+            assert np.abs(sp.gradient - r["gradient"]).max() <= 1e-12
