@@ -51,16 +51,22 @@ def sizes(numbers):
 TERMS = ("repulsion", "dispersion", "electronic", "isotropic_es", "third_order", "anisotropic", "entropy", "band")
 
 
+MIXERS = {"broyden": 0, "diis": 1, "auto": 2}
+MIXER_USED = {0: "broyden", 1: "diis", 2: "diis-fallback"}
+
+
 def engrad(numbers, positions, charge=0.0, acc=1.0, etemp=300.0, max_iter=250, guess="eeq", gradient=True,
-           dispersion=True, start=None):
+           dispersion=True, start=None, mixer="auto", report=None):
     """GFN2-xTB energy (+ gradient) of one molecule; positions in bohr [N,3]. ``start``: a given
-    SCF start {"qsh", "dipoles", "quadrupoles"} (the moments of a previous result), else ``guess``."""
+    SCF start {"qsh", "dipoles", "quadrupoles"} (the moments of a previous result), else ``guess``.
+    ``mixer``: the SCF mixer plan (auto = Broyden, then DIIS from the start: the axtb conventions).
+    ``report``: a dict that receives the SCF report (iterations, mixer_used, attempts) also when
+    the calculation fails."""
     z = np.ascontiguousarray(numbers, dtype=np.int32)
     x = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
     nat = len(z)
     nsh, nao = sizes(z)
     e = ctypes.c_double()
-    it = ctypes.c_int()
     terms = np.zeros(8)
     g = np.zeros((nat, 3))
     qat = np.zeros(nat)
@@ -68,18 +74,42 @@ def engrad(numbers, positions, charge=0.0, acc=1.0, etemp=300.0, max_iter=250, g
     dp = np.zeros((nat, 3))
     qp = np.zeros((nat, 6))
     emo = np.zeros(nao)
+    info = np.zeros(7, dtype=np.int32)
+    scal = np.zeros(5)
     gs = [None, None, None]
     if start is not None:
         gs = [np.ascontiguousarray(start[k], dtype=np.float64).reshape(-1) for k in ("qsh", "dipoles", "quadrupoles")]
         if gs[0].size != nsh or gs[1].size != 3 * nat or gs[2].size != 6 * nat:
             raise OracleError(1, "the start does not have the molecule's sizes")
-    rc = lib().oracle_engrad_guess(nat, _p(z), _p(x), ctypes.c_double(charge), ctypes.c_double(acc),
-                                   ctypes.c_double(etemp), max_iter, 0 if guess == "eeq" else 1, int(gradient),
-                                   int(dispersion), _p(gs[0]), _p(gs[1]), _p(gs[2]), ctypes.byref(e), _p(terms),
-                                   _p(g), _p(qat), _p(qsh), _p(dp), _p(qp), _p(emo), ctypes.byref(it))
+    rc = lib().oracle_singlepoint(nat, _p(z), _p(x), ctypes.c_double(charge), ctypes.c_double(acc),
+                                  ctypes.c_double(etemp), max_iter, 0 if guess == "eeq" else 1, int(gradient),
+                                  int(dispersion), MIXERS[mixer], _p(gs[0]), _p(gs[1]), _p(gs[2]), ctypes.byref(e),
+                                  _p(terms), _p(g), _p(qat), _p(qsh), _p(dp), _p(qp), _p(emo), _p(info), _p(scal))
+    rep = {"iterations": int(info[0]), "mixer_used": MIXER_USED.get(int(info[1])),
+           "attempts": [(("broyden", "diis")[0 if (mixer != "diis" and a == 0) else 1], int(info[3 + 2 * a]),
+                         bool(info[4 + 2 * a])) for a in range(int(info[2]))]}
+    if report is not None:
+        report.update(rep)
     _check(rc)
     return {"energy": e.value, "terms": dict(zip(TERMS, terms)), "gradient": g if gradient else None,
-            "qat": qat, "qsh": qsh, "dipoles": dp, "quadrupoles": qp, "emo": emo, "iterations": it.value}
+            "qat": qat, "qsh": qsh, "dipoles": dp, "quadrupoles": qp, "emo": emo, "efermi": scal[0],
+            "dipole": scal[1:4].copy(), "gap": scal[4], **rep}
+
+
+def engrad_many(mols, acc=1.0, etemp=300.0, gradient=True):
+    """Energies (and gradients) of many neutral molecules [(numbers, bohr)], one molecule per
+    OpenMP thread (OMP_NUM_THREADS). Returns (energies, gradients or None, status)."""
+    nats = np.array([len(m[0]) for m in mols], dtype=np.int32)
+    off = np.zeros(len(mols), dtype=np.int64)
+    off[1:] = np.cumsum(nats)[:-1]
+    z = np.ascontiguousarray(np.concatenate([np.asarray(m[0], dtype=np.int32) for m in mols]))
+    x = np.ascontiguousarray(np.concatenate([np.asarray(m[1], dtype=np.float64).reshape(-1, 3) for m in mols]))
+    e = np.zeros(len(mols))
+    g = np.zeros_like(x) if gradient else None
+    st = np.zeros(len(mols), dtype=np.int32)
+    lib().oracle_engrad_many(len(mols), _p(nats), _p(off), _p(z), _p(x), ctypes.c_double(acc), ctypes.c_double(etemp),
+                             int(gradient), _p(e), _p(g), _p(st))
+    return e, g, st
 
 
 def eeq(numbers, positions, charge=0.0):
@@ -142,3 +172,20 @@ def debug(numbers, positions, charge=0.0, acc=1.0):
     o["D"] = o["D"].reshape(3, nao, nao).transpose(0, 2, 1).copy()
     o["Q"] = o["Q"].reshape(6, nao, nao).transpose(0, 2, 1).copy()
     return o
+
+
+def diis_linear(eig, b, damp=0.4, max_steps=100):
+    """The oracle's DIIS mixer on q = diag(eig) q + b: (steps, final input)."""
+    eig = np.ascontiguousarray(eig, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros(len(eig))
+    steps = lib().oracle_diis_linear(len(eig), _p(eig), _p(b), ctypes.c_double(damp), max_steps, _p(x))
+    return steps, x
+
+
+def diis_coefficients(residuals):
+    """Pulay coefficients of the residuals (rows, oldest first)."""
+    r = np.ascontiguousarray(residuals, dtype=np.float64)
+    c = np.zeros(r.shape[0])
+    lib().oracle_diis_coefficients(r.shape[0], r.shape[1], _p(r), _p(c))
+    return c

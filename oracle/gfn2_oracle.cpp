@@ -18,9 +18,12 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <stdexcept>
 #include <string>
 #include <vector>
+
+#include <omp.h>
 
 #include "../axtb_amd/csrc/gfn2_tables.h"
 
@@ -276,6 +279,68 @@ struct Mat {
   double operator()(int i, int j) const { return a[(size_t)j * n + i]; }
 };
 
+// C = alpha op(A) op(B) + beta C, column-major (op = transpose when ta / tb). Tiles of C over the
+// OpenMP threads; each tile packs its panels of A and B and sums K in ascending chunks, so the
+// result does not depend on the thread count. Plain blocked loops (the reference multiplies with
+// faer's blocked products; the bits of either are not pinned).
+void mm(bool ta, bool tb, int m, int n, int k, double alpha, const double* A, int lda, const double* B, int ldb,
+        double beta, double* C, int ldc) {
+  if (m <= 0 || n <= 0) return;
+  const int MC = 96, NC = 96, KC = 256, MR = 8, NR = 4;
+  const int tm = (m + MC - 1) / MC, tn = (n + NC - 1) / NC;
+#pragma omp parallel
+  {
+    vector<double> Ap((size_t)MC * KC), Bp((size_t)KC * NC);
+#pragma omp for collapse(2) schedule(static)
+    for (int bi = 0; bi < tm; bi++)
+      for (int bj = 0; bj < tn; bj++) {
+        const int i0 = bi * MC, j0 = bj * NC, mc = std::min(MC, m - i0), nc = std::min(NC, n - j0);
+        for (int j = 0; j < nc; j++) {
+          double* c = C + (size_t)(j0 + j) * ldc + i0;
+          for (int i = 0; i < mc; i++) c[i] = beta == 0.0 ? 0.0 : beta * c[i];
+        }
+        for (int p0 = 0; p0 < k; p0 += KC) {
+          const int kc = std::min(KC, k - p0);
+          for (int ir = 0; ir < mc; ir += MR)
+            for (int p = 0; p < kc; p++)
+              for (int i = 0; i < MR; i++) {
+                const int gi = i0 + ir + i, gp = p0 + p;
+                Ap[(size_t)ir * kc + p * MR + i] =
+                    ir + i < mc ? (ta ? A[(size_t)gi * lda + gp] : A[(size_t)gp * lda + gi]) : 0.0;
+              }
+          for (int jr = 0; jr < nc; jr += NR)
+            for (int p = 0; p < kc; p++)
+              for (int j = 0; j < NR; j++) {
+                const int gj = j0 + jr + j, gp = p0 + p;
+                Bp[(size_t)jr * kc + p * NR + j] =
+                    jr + j < nc ? (tb ? B[(size_t)gp * ldb + gj] : B[(size_t)gj * ldb + gp]) : 0.0;
+              }
+          for (int jr = 0; jr < nc; jr += NR)
+            for (int ir = 0; ir < mc; ir += MR) {
+              double acc[NR][MR] = {};
+              const double* a = &Ap[(size_t)ir * kc];
+              const double* b = &Bp[(size_t)jr * kc];
+              for (int p = 0; p < kc; p++)
+                for (int j = 0; j < NR; j++)
+                  for (int i = 0; i < MR; i++) acc[j][i] += a[p * MR + i] * b[p * NR + j];
+              for (int j = 0; j < NR && jr + j < nc; j++) {
+                double* c = C + (size_t)(j0 + jr + j) * ldc + i0 + ir;
+                for (int i = 0; i < MR && ir + i < mc; i++) c[i] += alpha * acc[j][i];
+              }
+            }
+        }
+      }
+  }
+}
+
+// the lower triangle of a column-major n x n matrix copied to the upper one
+void mirror_lower(Mat& M) {
+  const int n = M.n;
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < j; i++) M(i, j) = M(j, i);
+}
+
 // Gaussian elimination with partial pivoting on the last maximum (base/src/linalg/lu.rs:81-107)
 bool gauss_last_max(vector<vec>& a, vec& b) {
   int n = (int)b.size();
@@ -317,6 +382,7 @@ vec lu_solve(Mat A, vec b) {
     }
     double piv = A(k, k);
     for (int i = k + 1; i < n; i++) A(i, k) /= piv;
+#pragma omp parallel for schedule(static, 8) if (n - k > 256)
     for (int j = k + 1; j < n; j++) {
       double akj = A(k, j);
       if (akj == 0.0) continue;
@@ -633,8 +699,10 @@ Ints build_integrals(const Model& m, const vec& se) {
   I.H0 = Mat(n, n);
   for (int k = 0; k < 3; k++) I.D[k] = Mat(n, n);
   for (int k = 0; k < 6; k++) I.Q[k] = Mat(n, n);
-  double s[25], d[25][3], q[25][6];
+  // atom pairs write disjoint blocks (and their mirrors), so the rows of atoms run in parallel
+#pragma omp parallel for schedule(dynamic, 4)
   for (int iat = 0; iat < m.nat; iat++) {
+    double s[25], d[25][3], q[25][6];
     for (int jat = 0; jat <= iat; jat++) {
       double vec[3];
       for (int k = 0; k < 3; k++) vec[k] = m.xyz[3 * iat + k] - m.xyz[3 * jat + k];
@@ -1240,7 +1308,7 @@ void d4_gradient(const D4& d, const vec& q, vec& g) {
 
 // ATM three-body at q = 0, per triple (disp.rs:665-870; the pair-sum form gives the same
 // values up to rounding). Adds atom energies and, with g, the gradient incl. the D4 CN chain.
-void d4_three_body(const D4& d, vec& e, vec* g) {
+void d4_three_body(const D4& d, vec& e_out, vec* g) {
   int nat = d.nat;
   vec q0(nat, 0.0);
   vector<vec> gw, dgwdcn;
@@ -1275,6 +1343,16 @@ void d4_three_body(const D4& d, vec& e, vec* g) {
       }
     }
   vec dedcn(nat, 0.0);
+  // triples (i > j > k) by i over the threads, each with its own sums, added in thread order
+  const int nt = std::max(1, omp_get_max_threads());
+  vector<vec> te(nt, vec(nat, 0.0)), tg(nt, vec(g ? 3 * nat : 0, 0.0)), tc(nt, vec(nat, 0.0));
+#pragma omp parallel
+  {
+  const int tid = omp_get_thread_num();
+  vec& e = te[tid];
+  vec* gt = g ? &tg[tid] : nullptr;
+  vec& dedcn = tc[tid];
+#pragma omp for schedule(static, 1)
   for (int i = 0; i < nat; i++)
     for (int j = 0; j < i; j++) {
       if (!ok[(size_t)j * nat + i]) continue;
@@ -1297,7 +1375,7 @@ void d4_three_body(const D4& d, vec& e, vec* g) {
         e[i] -= de / 3.0;
         e[j] -= de / 3.0;
         e[k] -= de / 3.0;
-        if (!g) continue;
+        if (!gt) continue;
         double dfdmp = -(2.0 * ALP * rpow * (fdmp * fdmp));
         double swc9 = sw * c9;
         auto dangf = [&](double a, double b, double c) {  // derivative of ang wrt a = r2 of the pair
@@ -1312,9 +1390,9 @@ void d4_three_body(const D4& d, vec& e, vec* g) {
           double vik = d.xyz[3 * k + x] - d.xyz[3 * i + x];
           double vjk = d.xyz[3 * k + x] - d.xyz[3 * j + x];
           double dgij = fij * vij, dgik = fik * vik, dgjk = fjk * vjk;
-          (*g)[3 * i + x] -= dgij + dgik;
-          (*g)[3 * j + x] += dgij - dgjk;
-          (*g)[3 * k + x] += dgik + dgjk;
+          (*gt)[3 * i + x] -= dgij + dgik;
+          (*gt)[3 * j + x] += dgij - dgjk;
+          (*gt)[3 * k + x] += dgik + dgjk;
         }
         double h = de * 0.5;
         dedcn[i] -= h * (QA(i, j) + QA(i, k));
@@ -1322,6 +1400,15 @@ void d4_three_body(const D4& d, vec& e, vec* g) {
         dedcn[k] -= h * (QA(k, i) + QA(k, j));
       }
     }
+  }
+  for (int t = 0; t < nt; t++) {
+    for (int a = 0; a < nat; a++) {
+      e_out[a] += te[t][a];
+      dedcn[a] += tc[t][a];
+    }
+    if (g)
+      for (int a = 0; a < 3 * nat; a++) (*g)[a] += tg[t][a];
+  }
   if (g) d4_cn_gradient(d, dedcn, *g);
 }
 
@@ -1330,65 +1417,130 @@ void d4_three_body(const D4& d, vec& e, vec* g) {
 // with Wilkinson-type shifts (the textbook EISPACK tred2/tql2 scheme, own implementation).
 // In: a (n x n, column-major, symmetric; overwritten by the eigenvectors). Out: w ascending.
 void eigh(int n, vec& a, vec& w) {
-  auto A = [&](int i, int j) -> double& { return a[(size_t)j * n + i]; };
-  vec d(n), e(n);
-  // tridiagonalise from the last row up (work in row-major view: use A(i,j) symmetric)
+  // the matrix is symmetric, so its column-major storage read row by row is the same matrix:
+  // R(i, j) = a[i n + j] keeps every inner loop below on contiguous memory
+  auto R = [&](int i, int j) -> double& { return a[(size_t)i * n + j]; };
+  vec d(n), e(n), u(n);
+  const int nt = std::max(1, omp_get_max_threads());
+  vector<vec> part(nt, vec(n));
+  // Householder reduction, last row first: row i (k <= l) becomes the reflector u, the leading
+  // (l + 1) x (l + 1) lower triangle gets A - u p^T - p u^T
   for (int i = n - 1; i > 0; i--) {
-    int l = i - 1;
+    const int l = i - 1;
     double h = 0.0, scale = 0.0;
+    double* ri = &a[(size_t)i * n];
     if (l > 0) {
-      for (int k = 0; k <= l; k++) scale += std::fabs(A(i, k));
+      for (int k = 0; k <= l; k++) scale += std::fabs(ri[k]);
       if (scale == 0.0) {
-        e[i] = A(i, l);
+        e[i] = ri[l];
       } else {
         for (int k = 0; k <= l; k++) {
-          A(i, k) /= scale;
-          h += A(i, k) * A(i, k);
+          ri[k] /= scale;
+          h += ri[k] * ri[k];
         }
-        double f = A(i, l);
+        double f = ri[l];
         double g = f >= 0.0 ? -std::sqrt(h) : std::sqrt(h);
         e[i] = scale * g;
         h -= f * g;
-        A(i, l) = f - g;
+        ri[l] = f - g;
+        // p = A u / h from the lower triangle: row j's dot with u, and row j's entries scattered
+        // into the transposed products (per-thread sums, added in thread order)
+#pragma omp parallel
+        {
+          const int t = omp_get_thread_num();
+          double* acc = part[t].data();
+          std::fill(acc, acc + l + 1, 0.0);
+#pragma omp for schedule(static, 16)
+          for (int j = 0; j <= l; j++) {
+            const double* rj = &a[(size_t)j * n];
+            const double uj = ri[j];
+            double s = rj[j] * uj;
+            for (int k = 0; k < j; k++) {
+              s += rj[k] * ri[k];
+              acc[k] += rj[k] * uj;
+            }
+            acc[j] += s;
+          }
+        }
         f = 0.0;
         for (int j = 0; j <= l; j++) {
-          A(j, i) = A(i, j) / h;
-          g = 0.0;
-          for (int k = 0; k <= j; k++) g += A(j, k) * A(i, k);
-          for (int k = j + 1; k <= l; k++) g += A(k, j) * A(i, k);
-          e[j] = g / h;
-          f += e[j] * A(i, j);
+          double s = 0.0;
+          for (int t = 0; t < nt; t++) s += part[t][j];
+          e[j] = s / h;
+          f += e[j] * ri[j];
         }
-        double hh = f / (h + h);
+        const double hh = f / (h + h);
+        for (int j = 0; j <= l; j++) e[j] -= hh * ri[j];
+#pragma omp parallel for schedule(static, 16)
         for (int j = 0; j <= l; j++) {
-          f = A(i, j);
-          e[j] = g = e[j] - hh * f;
-          for (int k = 0; k <= j; k++) A(j, k) -= (f * e[k] + g * A(i, k));
+          double* rj = &a[(size_t)j * n];
+          const double fj = ri[j], gj = e[j];
+          for (int k = 0; k <= j; k++) rj[k] -= fj * e[k] + gj * ri[k];
         }
+        for (int j = 0; j <= l; j++) R(j, i) = ri[j] / h;
       }
     } else {
-      e[i] = A(i, l);
+      e[i] = ri[l];
     }
     d[i] = h;
   }
   d[0] = 0.0;
   e[0] = 0.0;
+  // the accumulated transformation: Q = H_1 ... H_{n-1}, built up from the first reflector
   for (int i = 0; i < n; i++) {
-    int l = i - 1;
+    const int l = i - 1;
     if (d[i] != 0.0) {
-      for (int j = 0; j <= l; j++) {
-        double g = 0.0;
-        for (int k = 0; k <= l; k++) g += A(i, k) * A(k, j);
-        for (int k = 0; k <= l; k++) A(k, j) -= g * A(k, i);
+      const double* ri = &a[(size_t)i * n];
+      // g_j = sum_k R(i, k) R(k, j), then R(k, j) -= g_j R(k, i), for j, k <= l
+#pragma omp parallel for schedule(static)
+      for (int j0 = 0; j0 <= l; j0 += 64) {
+        const int j1 = std::min(l + 1, j0 + 64);
+        for (int j = j0; j < j1; j++) u[j] = 0.0;
+        for (int k = 0; k <= l; k++) {
+          const double rik = ri[k];
+          const double* rk = &a[(size_t)k * n];
+          for (int j = j0; j < j1; j++) u[j] += rik * rk[j];
+        }
+      }
+#pragma omp parallel for schedule(static)
+      for (int k = 0; k <= l; k++) {
+        double* rk = &a[(size_t)k * n];
+        const double rki = rk[i];
+        for (int j = 0; j <= l; j++) rk[j] -= u[j] * rki;
       }
     }
-    d[i] = A(i, i);
-    A(i, i) = 1.0;
-    for (int j = 0; j <= l; j++) A(j, i) = A(i, j) = 0.0;
+    d[i] = R(i, i);
+    R(i, i) = 1.0;
+    for (int j = 0; j <= l; j++) R(j, i) = R(i, j) = 0.0;
   }
-  // implicit QL on (d, e) with vectors in A (rows = components)
+  // implicit QL on (d, e); the rotations of the vectors are recorded and applied in batches, row
+  // by row (a row of R holds one component of every vector; the recurrence on d and e does not
+  // read the vectors, so deferring them changes nothing)
   for (int i = 1; i < n; i++) e[i - 1] = e[i];
   if (n > 0) e[n - 1] = 0.0;
+  vector<int> rot_i;
+  vec rot_c, rot_s;
+  auto flush = [&]() {
+    const size_t nr = rot_i.size();
+    if (!nr) return;
+#pragma omp parallel for schedule(static)
+    for (int k0 = 0; k0 < n; k0 += 8) {
+      const int k1 = std::min(n, k0 + 8);
+      for (size_t r = 0; r < nr; r++) {
+        const int i = rot_i[r];
+        const double c = rot_c[r], s = rot_s[r];
+        for (int k = k0; k < k1; k++) {
+          double* rk = &a[(size_t)k * n];
+          const double f = rk[i + 1];
+          rk[i + 1] = s * rk[i] + c * f;
+          rk[i] = c * rk[i] - s * f;
+        }
+      }
+    }
+    rot_i.clear();
+    rot_c.clear();
+    rot_s.clear();
+  };
   for (int l = 0; l < n; l++) {
     int iter = 0, mm;
     do {
@@ -1417,12 +1569,11 @@ void eigh(int n, vec& a, vec& w) {
           r = (d[i] - g) * s + 2.0 * c * b;
           d[i + 1] = g + (p = s * r);
           g = c * r - b;
-          for (int k = 0; k < n; k++) {
-            f = A(k, i + 1);
-            A(k, i + 1) = s * A(k, i) + c * f;
-            A(k, i) = c * A(k, i) - s * f;
-          }
+          rot_i.push_back(i);
+          rot_c.push_back(c);
+          rot_s.push_back(s);
         }
+        if (rot_i.size() >= ((size_t)1 << 17)) flush();
         if (r == 0.0 && i >= l) continue;
         d[l] -= p;
         e[l] = g;
@@ -1430,16 +1581,17 @@ void eigh(int n, vec& a, vec& w) {
       }
     } while (mm != l);
   }
-  // sort ascending
+  flush();
+  // sort ascending; eigenvector j is column idx[j] of R
   vector<int> idx(n);
   for (int i = 0; i < n; i++) idx[i] = i;
   std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return d[x] < d[y]; });
   vec out((size_t)n * n);
   w.resize(n);
-  for (int j = 0; j < n; j++) {
-    w[j] = d[idx[j]];
-    std::memcpy(&out[(size_t)j * n], &a[(size_t)idx[j] * n], sizeof(double) * n);
-  }
+  for (int j = 0; j < n; j++) w[j] = d[idx[j]];
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j < n; j++)
+    for (int k = 0; k < n; k++) out[(size_t)j * n + k] = a[(size_t)k * n + idx[j]];
   a.swap(out);
 }
 
@@ -1573,10 +1725,136 @@ struct Broyden {
   }
 };
 
+// ------------------------------------------------------------------ DIIS (scf/mixing.rs:340-456)
+// Pulay mixing with restarts: the last DIIS_SIZE inputs and residuals, coefficients by modified
+// Gram-Schmidt on the differences to the newest residual (newest first, a nearly dependent one ends
+// the basis), restart from the best iterate with half the damping on a stall or a divergence.
+const int DIIS_SIZE = 30;                 // mixing.rs:341
+const double DIIS_DEPENDENCE = 1e-2;      // mixing.rs:346
+const int DIIS_STALL = 12;                // mixing.rs:349
+const double DIIS_RESTART_GROWTH = 1e3;   // mixing.rs:352
+const double DIIS_RESTART_FLOOR = 0.1;    // mixing.rs:354
+
+inline double dot_seq(const vec& x, const vec& y) {  // base/src/math.rs:69-71
+  double s = 0.0;
+  for (size_t i = 0; i < x.size(); i++) s += x[i] * y[i];
+  return s;
+}
+
+// mixing.rs:467-513
+vec diis_coefficients(const std::deque<vec>& r) {
+  const int m = (int)r.size();
+  const vec& last = r[m - 1];
+  const size_t n = last.size();
+  vector<vec> basis, rcol;
+  vector<int> kept;
+  for (int j = m - 2; j >= 0; j--) {
+    vec d(n);
+    for (size_t k = 0; k < n; k++) d[k] = last[k] - r[j][k];
+    const double norm0 = std::sqrt(dot_seq(d, d));
+    if (std::isnan(norm0) || norm0 <= 0.0) break;
+    vec col(basis.size() + 1, 0.0);
+    for (size_t l = 0; l < basis.size(); l++) {
+      const double h = dot_seq(basis[l], d);
+      col[l] = h;
+      for (size_t k = 0; k < n; k++) d[k] -= h * basis[l][k];
+    }
+    const double nr = std::sqrt(dot_seq(d, d));
+    if (nr < DIIS_DEPENDENCE * norm0) break;
+    col[basis.size()] = nr;
+    for (double& x : d) x /= nr;
+    basis.push_back(d);
+    rcol.push_back(col);
+    kept.push_back(j);
+  }
+  const int p = (int)basis.size();
+  vec rhs(p), gl(p, 0.0);
+  for (int l = 0; l < p; l++) rhs[l] = dot_seq(basis[l], last);
+  for (int l = p - 1; l >= 0; l--) {
+    double v = rhs[l];
+    for (int k = l + 1; k < p; k++) v -= rcol[k][l] * gl[k];
+    gl[l] = v / rcol[l][l];
+  }
+  vec c(m, 0.0);
+  c[m - 1] = 1.0;
+  for (int l = 0; l < p; l++) {
+    c[kept[l]] += gl[l];
+    c[m - 1] -= gl[l];
+  }
+  return c;
+}
+
+struct Diis {
+  double damp;
+  std::deque<vec> hq, hr;
+  bool has_best = false;
+  double best_norm = 0.0;
+  vec best_q, best_r, q_in, dq;
+  int since_best = 0;
+  Diis(int n, double d) : damp(d), q_in(n), dq(n) {}
+  // mixing.rs:410-448
+  bool next() {
+    const double norm = std::sqrt(dot_seq(dq, dq));
+    if (!std::isfinite(norm)) return false;
+    if (has_best) {
+      const bool diverging = norm > DIIS_RESTART_GROWTH * best_norm && norm > DIIS_RESTART_FLOOR;
+      if (diverging || since_best >= DIIS_STALL) {
+        since_best = 0;
+        hq.clear();
+        hr.clear();
+        damp *= 0.5;
+        for (size_t k = 0; k < q_in.size(); k++) q_in[k] = best_q[k] + damp * best_r[k];
+        return true;
+      }
+    }
+    if (!has_best || norm < best_norm) {
+      has_best = true;
+      best_norm = norm;
+      best_q = q_in;
+      best_r = dq;
+      since_best = 0;
+    } else {
+      since_best++;
+    }
+    if ((int)hq.size() == DIIS_SIZE) {
+      hq.pop_front();
+      hr.pop_front();
+    }
+    hq.push_back(q_in);
+    hr.push_back(dq);
+    const vec c = diis_coefficients(hr);
+    for (size_t k = 0; k < q_in.size(); k++) {
+      double x = 0.0;
+      for (size_t i = 0; i < c.size(); i++)
+        if (c[i] != 0.0) x += c[i] * (hq[i][k] + damp * hr[i][k]);
+      q_in[k] = x;
+    }
+    return true;
+  }
+};
+
+// The vector mixer of one SCF attempt: Broyden or DIIS (MixerKind, scf/mixing.rs:83-118).
+enum MixerKind { MIX_BROYDEN = 0, MIX_DIIS = 1, MIX_AUTO = 2 };
+struct VecMixer {
+  int kind;
+  Broyden br;
+  Diis di;
+  VecMixer(int k, int n, int memory, double damp) : kind(k), br(n, memory, damp), di(n, damp) {}
+  vec& q_in() { return kind == MIX_DIIS ? di.q_in : br.q_in; }
+  vec& dq() { return kind == MIX_DIIS ? di.dq : br.dq; }
+  void set_residual(const vec& out) {
+    vec& q = q_in();
+    vec& d = dq();
+    for (size_t k = 0; k < d.size(); k++) d[k] = out[k] - q[k];
+  }
+  bool next() { return kind == MIX_DIIS ? di.next() : br.next(); }
+};
+
 // ------------------------------------------------------------------ the calculation (xtb.rs:437-861)
 struct Options {
   double acc = 1.0, etemp = 300.0, damping = 0.4;
   int max_iter = 250, guess = 0 /*0 EEQ, 1 SAD*/, want_grad = 1, dispersion = 1;
+  int mixer = MIX_AUTO;  // the axtb conventions' plan (conventions.rs:260)
   // a given start (ScfGuess, xtb.rs:204-208): shell charges [nsh] (null: none), dipoles [3N],
   // quadrupoles [6N] (null: zero)
   const double *gq = nullptr, *gdp = nullptr, *gqp = nullptr;
@@ -1588,6 +1866,10 @@ struct Result {
   vec grad, qat, qsh, dp, qp, emo, focc;
   int iterations = 0, nao = 0, nsh = 0, converged = 0;
   double efermi = 0, residual = 0, energy_change = 0;
+  // the attempts of the mixer plan (iterations, converged) and the mixer of the result:
+  // 0 broyden, 1 diis, 2 diis-fallback (MixerPlan::used, scf/mixing.rs:150-157)
+  int nattempts = 0, attempt_iters[2] = {0, 0}, attempt_conv[2] = {0, 0}, mixer_used = 0;
+  double dipole[3] = {0, 0, 0}, gap = 0;
 };
 
 struct DebugOut {
@@ -1599,10 +1881,33 @@ inline void dbg_copy(double* dst, const double* src, size_t n) {
   if (debug_on && dst) std::memcpy(dst, src, n * sizeof(double));
 }
 
+// P = C diag(f) C^T over the orbitals with f > 1e-16 (density.rs:65-87: the lower triangle, mirrored);
+// with eps: W = C diag(f eps) C^T (xtb.rs:688-696). Tmp: n x n work.
+void weighted_outer(const Mat& C, const vec& focc, const vec* eps, Mat& Tmp, Mat& P) {
+  const int n = C.n;
+  vector<int> kept;
+  for (int k = 0; k < n; k++)
+    if (focc[k] > 1e-16) kept.push_back(k);
+  const int kk = (int)kept.size();
+  Mat Ck(n, std::max(kk, 1));
+#pragma omp parallel for schedule(static)
+  for (int q = 0; q < kk; q++) {
+    const int k = kept[q];
+    const double f = eps ? focc[k] * (*eps)[k] : focc[k];
+    for (int i = 0; i < n; i++) {
+      Ck(i, q) = C(i, k);
+      Tmp(i, q) = C(i, k) * f;
+    }
+  }
+  mm(false, true, n, n, kk, 1.0, Ck.a.data(), n, Tmp.a.data(), n, 0.0, P.a.data(), n);
+  mirror_lower(P);
+}
+
 const double SQRT2 = 1.4142135623730951;
 const bool QOFF[6] = {false, true, false, true, true, false};
 
-Result singlepoint(int nat, const int* z, const double* xyz, double charge, const Options& opt) {
+Result singlepoint(int nat, const int* z, const double* xyz, double charge, const Options& opt,
+                   Result* report = nullptr) {
   if (!(opt.acc > 0.0 && std::isfinite(opt.acc))) throw Error(1, "accuracy must be positive");
   if (!(opt.etemp >= 0.0 && std::isfinite(opt.etemp))) throw Error(1, "etemp must be >= 0");
   if (!(opt.damping > 0.0 && opt.damping <= 1.0)) throw Error(1, "damping must be in (0, 1]");
@@ -1655,28 +1960,39 @@ Result singlepoint(int nat, const int* z, const double* xyz, double charge, cons
     dbg_copy(debug_out.cn, cn.data(), nat);
   }
   // Cholesky S = L L^T (eigen.rs:72-113)
+  // (right-looking: column j scaled, then the trailing columns updated, each on contiguous memory)
   Mat L = I.S;
   for (int j = 0; j < n; j++) {
-    double s = L(j, j);
-    for (int k = 0; k < j; k++) s -= L(j, k) * L(j, k);
+    const double s = L(j, j);
     if (!(s > 0.0)) throw Error(2, "overlap matrix is not positive definite");
-    double ljj = std::sqrt(s);
-    L(j, j) = ljj;
-    for (int i = j + 1; i < n; i++) {
-      double t = L(i, j);
-      for (int k = 0; k < j; k++) t -= L(i, k) * L(j, k);
-      L(i, j) = t / ljj;
+    const double ljj = std::sqrt(s);
+    double* lj = &L.a[(size_t)j * n];
+    lj[j] = ljj;
+    for (int i = j + 1; i < n; i++) lj[i] /= ljj;
+#pragma omp parallel for schedule(static, 8)
+    for (int c = j + 1; c < n; c++) {
+      double* lc = &L.a[(size_t)c * n];
+      const double f = lj[c];
+      if (f != 0.0)
+        for (int i = c; i < n; i++) lc[i] -= lj[i] * f;
     }
-    for (int i = 0; i < j; i++) L(i, j) = 0.0;
   }
-  // Linv (lower)
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < j; i++) L(i, j) = 0.0;
+  // Linv (lower): L X = I by forward substitution, eight columns of X per pass over L
   Mat Li(n, n);
-  for (int j = 0; j < n; j++) {
-    Li(j, j) = 1.0 / L(j, j);
-    for (int i = j + 1; i < n; i++) {
-      double s = 0.0;
-      for (int k = j; k < i; k++) s -= L(i, k) * Li(k, j);
-      Li(i, j) = s / L(i, i);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int j0 = 0; j0 < n; j0 += 8) {
+    const int j1 = std::min(n, j0 + 8);
+    for (int j = j0; j < j1; j++) Li(j, j) = 1.0;
+    for (int k = j0; k < n; k++) {
+      const double* lk = &L.a[(size_t)k * n];
+      for (int j = j0; j < j1 && j <= k; j++) {
+        double* x = &Li.a[(size_t)j * n];
+        const double xk = (x[k] /= lk[k]);
+        if (xk != 0.0)
+          for (int i = k + 1; i < n; i++) x[i] -= lk[i] * xk;
+      }
     }
   }
 
@@ -1731,16 +2047,31 @@ Result singlepoint(int nat, const int* z, const double* xyz, double charge, cons
   vec emo(n), occa(n), occb(n), focc(n), vao(n);
   double ts = 0.0, elast = 0.0, efermi = 0.0;
   vec eelec(nat);
-  Broyden mix(ndim, opt.max_iter, opt.damping);
   vec packed(ndim);
   bool converged = false;
   int iter = 0;
   double res_norm = 0, de_last = 0;
+  // one SCF per mixer of the plan, each from the start (xtb.rs:590-646): auto = Broyden, then DIIS
+  const int plan[2] = {opt.mixer == MIX_DIIS ? MIX_DIIS : MIX_BROYDEN, MIX_DIIS};
+  const int nplan = opt.mixer == MIX_AUTO ? 2 : 1;
+  int total_iters = 0;
+  for (int ia = 0; ia < nplan && !converged; ia++) {
+  const bool last_attempt = ia + 1 == nplan;
+  VecMixer mix(plan[ia], ndim, opt.max_iter, opt.damping);
+  st = start;
+  elast = 0.0;  // run_scf's first iteration compares with energy 0 (scf/driver.rs:191)
+  std::fill(eelec.begin(), eelec.end(), 0.0);
+  int stop = 0;  // 0 iteration limit or converged, 1 mixer failed, 2 Fermi level not found
+  int iters = 0;
   for (iter = 1; iter <= opt.max_iter; iter++) {
     if (iter > 1) {
-      if (!mix.next()) throw Error(2, "Broyden mixing failed");
-      unpack(mix.q_in, st);
+      if (!mix.next()) {
+        stop = 1;
+        break;
+      }
+      unpack(mix.q_in(), st);
     }
+    iters = iter;
     // potentials (xtb.rs:316-340)
     pot.vsh.assign(nsh, 0.0);
     pot.vat.assign(nat, 0.0);
@@ -1756,6 +2087,7 @@ Result singlepoint(int nat, const int* z, const double* xyz, double charge, cons
     if (opt.dispersion) d4_energy_potential(d4, st.qat, nullptr, &pot.vat);
     for (int mu = 0; mu < n; mu++) vao[mu] = pot.vsh[m.ao2sh[mu]] + pot.vat[m.ao2at[mu]];
     // Fock (scf/fock.rs:22-104)
+#pragma omp parallel for schedule(static)
     for (int nu = 0; nu < n; nu++) {
       int at = m.ao2at[nu];
       for (int mu = 0; mu < n; mu++) {
@@ -1765,12 +2097,15 @@ Result singlepoint(int nat, const int* z, const double* xyz, double charge, cons
         Tmp(mu, nu) = b;
       }
     }
+    bool finite = true;
+#pragma omp parallel for schedule(static) reduction(&& : finite)
     for (int nu = 0; nu < n; nu++)
       for (int mu = 0; mu < n; mu++) {
         F(mu, nu) = I.H0(mu, nu) - 0.5 * (Tmp(mu, nu) + Tmp(nu, mu));
-        if (!std::isfinite(F(mu, nu))) throw Error(2, "non-finite Fock matrix");
+        finite = finite && std::isfinite(F(mu, nu));
       }
-    pack(st, mix.q_in);
+    if (!finite) throw Error(2, "non-finite Fock matrix");
+    pack(st, mix.q_in());
     if (iter == 1) {
       dbg_copy(debug_out.vsh, pot.vsh.data(), nsh);
       dbg_copy(debug_out.vat, pot.vat.data(), nat);
@@ -1779,32 +2114,17 @@ Result singlepoint(int nat, const int* z, const double* xyz, double charge, cons
       dbg_copy(debug_out.F, F.a.data(), (size_t)n * n);
     }
     // A = Linv F Linv^T, eig, C = Linv^T U
-    Mat X(n, n);
-    for (int j = 0; j < n; j++)
-      for (int i = 0; i < n; i++) {
-        double s = 0;
-        for (int k = 0; k <= i; k++) s += Li(i, k) * F(k, j);
-        X(i, j) = s;
-      }
+    mm(false, false, n, n, n, 1.0, Li.a.data(), n, F.a.data(), n, 0.0, Tmp.a.data(), n);  // Linv F
     vec A((size_t)n * n);
-    for (int j = 0; j < n; j++)
-      for (int i = 0; i < n; i++) {
-        double s = 0;
-        for (int k = 0; k <= j; k++) s += X(i, k) * Li(j, k);
-        A[(size_t)j * n + i] = s;
-      }
+    mm(false, true, n, n, n, 1.0, Tmp.a.data(), n, Li.a.data(), n, 0.0, A.data(), n);  // (Linv F) Linv^T
+#pragma omp parallel for schedule(static)
     for (int j = 0; j < n; j++)
       for (int i = 0; i < j; i++) {
         double v = 0.5 * (A[(size_t)j * n + i] + A[(size_t)i * n + j]);
         A[(size_t)j * n + i] = A[(size_t)i * n + j] = v;
       }
     eigh(n, A, emo);
-    for (int j = 0; j < n; j++)
-      for (int i = 0; i < n; i++) {
-        double s = 0;
-        for (int k = i; k < n; k++) s += Li(k, i) * A[(size_t)j * n + k];
-        C(i, j) = s;
-      }
+    mm(true, false, n, n, n, 1.0, Li.a.data(), n, A.data(), n, 0.0, C.a.data(), n);  // Linv^T U
     // Fermi filling (xtb.rs:997-1031)
     if (kt > 0.0) {
       efermi = fill_tblite(nalpha, kt, emo, occa);
@@ -1818,24 +2138,22 @@ Result singlepoint(int nat, const int* z, const double* xyz, double charge, cons
       pa += occa[i];
       pb += occb[i];
     }
-    if (std::fabs(pa - nalpha) > 1e-6 || std::fabs(pb - nbeta) > 1e-6) throw Error(2, "Fermi level not found");
+    if (std::fabs(pa - nalpha) > 1e-6 || std::fabs(pb - nbeta) > 1e-6) {
+      stop = 2;  // ends this attempt; the last attempt reports it (xtb.rs:616-621)
+      break;
+    }
     ts = entropy_tblite(occa, kt) + entropy_tblite(occb, kt);
     for (int i = 0; i < n; i++) focc[i] = occa[i] + occb[i];
     // density P = C f C^T (xtb.rs:1033-1038)
-    for (int j = 0; j < n; j++)
-      for (int i = j; i < n; i++) {
-        double s = 0;
-        for (int k = 0; k < n; k++)
-          if (focc[k] > 1e-16) s += C(i, k) * focc[k] * C(j, k);
-        P(i, j) = s;
-        P(j, i) = s;
-      }
+    weighted_outer(C, focc, nullptr, Tmp, P);
     // Mulliken (scf/mulliken.rs:19-59) and the band energy per atom
     out.qsh = m.refocc;
     out.dp.assign(3 * nat, 0.0);
     out.qp.assign(6 * nat, 0.0);
     vec eband(nat, 0.0);
     double band = 0.0;
+    vec cols((size_t)11 * n);
+#pragma omp parallel for schedule(static)
     for (int nu = 0; nu < n; nu++) {
       double r[11] = {0};
       for (int mu = 0; mu < n; mu++) {
@@ -1845,6 +2163,10 @@ Result singlepoint(int nat, const int* z, const double* xyz, double charge, cons
         for (int k = 0; k < 6; k++) r[4 + k] += p * I.Q[k](mu, nu);
         r[10] += p * I.H0(mu, nu);
       }
+      for (int k = 0; k < 11; k++) cols[(size_t)11 * nu + k] = r[k];
+    }
+    for (int nu = 0; nu < n; nu++) {
+      const double* r = &cols[(size_t)11 * nu];
       int iat = m.ao2at[nu];
       out.qsh[m.ao2sh[nu]] -= r[0];
       for (int k = 0; k < 3; k++) out.dp[3 * iat + k] -= r[1 + k];
@@ -1885,7 +2207,7 @@ Result singlepoint(int nat, const int* z, const double* xyz, double charge, cons
     for (int i = 0; i < nat; i++) energy += eelec[i];
     if (!std::isfinite(energy)) throw Error(2, "non-finite energy");
     res_norm = 0.0;
-    for (double x : mix.dq) res_norm += x * x;
+    for (double x : mix.dq()) res_norm += x * x;
     res_norm = std::sqrt(res_norm / ndim);
     de_last = energy - elast;
     elast = energy;
@@ -1903,7 +2225,27 @@ Result singlepoint(int nat, const int* z, const double* xyz, double charge, cons
       break;
     }
   }
-  if (!converged) throw Error(3, "SCF not converged");
+  res.attempt_iters[ia] = iters;
+  res.attempt_conv[ia] = converged ? 1 : 0;
+  res.nattempts = ia + 1;
+  total_iters += iters;
+  iter = iters;
+  res.mixer_used = opt.mixer == MIX_AUTO && res.nattempts > 1 ? 2 : plan[res.nattempts - 1];
+  res.iterations = total_iters;
+  if (report) {
+    report->iterations = res.iterations;
+    report->mixer_used = res.mixer_used;
+    report->nattempts = res.nattempts;
+    for (int k = 0; k < 2; k++) {
+      report->attempt_iters[k] = res.attempt_iters[k];
+      report->attempt_conv[k] = res.attempt_conv[k];
+    }
+  }
+  if (stop == 1 && last_attempt)
+    throw Error(2, plan[ia] == MIX_BROYDEN ? "Broyden mixing failed" : "DIIS mixing failed");
+  if (stop == 2 && last_attempt) throw Error(2, "Fermi level not found");
+  }
+  if (!converged) throw Error(3, "SCF not converged in " + std::to_string(total_iters) + " iterations");
   res.iterations = iter;
   res.converged = 1;
   res.residual = res_norm;
@@ -1920,22 +2262,32 @@ Result singlepoint(int nat, const int* z, const double* xyz, double charge, cons
   res.emo = emo;
   res.focc = focc;
   res.efermi = efermi;
+  // molecular dipole about the origin (xtb.rs:746-751) and the HOMO-LUMO gap above the alpha HOMO
+  // of the one set of orbitals (properties.rs:204-221)
+  for (int i = 0; i < nat; i++)
+    for (int k = 0; k < 3; k++) res.dipole[k] += out.qat[i] * m.xyz[3 * i + k] + out.dp[3 * i + k];
+  {
+    const double nel = std::max(nalpha, nbeta), fl = std::floor(nel);
+    const double kk = nel - fl > 0.5 ? fl + 1.0 : fl;
+    res.gap = (kk >= 1.0 && (int)kk < n) ? emo[(int)kk] - emo[(int)kk - 1] : std::nan("");
+  }
   if (opt.want_grad) {
     // W = C diag(f eps) C^T (xtb.rs:688-696)
-    for (int j = 0; j < n; j++)
-      for (int i = j; i < n; i++) {
-        double s = 0;
-        for (int k = 0; k < n; k++)
-          if (focc[k] > 1e-16) s += C(i, k) * focc[k] * emo[k] * C(j, k);
-        W(i, j) = s;
-        W(j, i) = s;
-      }
+    weighted_outer(C, focc, &emo, Tmp, W);
     // add_scc_gradient (xtb.rs:346-387), output moments, last input potentials
     es2_gradient(m, cou, out.qsh, g);
     aes_gradient(m, cou, out, g, dedcn);
     if (opt.dispersion) d4_gradient(d4, out.qat, g);
     // H0/S/multipole gradient (hamiltonian.rs:137-223)
+    // atom pairs by the first atom over the threads, each with its own sums, added in thread order
+    const int nt = std::max(1, omp_get_max_threads());
+    vector<vec> tg(nt, vec(3 * nat, 0.0)), tc(nt, vec(nat, 0.0));
+#pragma omp parallel
+    {
     MpGrad mg;
+    vec& g = tg[omp_get_thread_num()];
+    vec& dedcn = tc[omp_get_thread_num()];
+#pragma omp for schedule(static, 1)
     for (int iat = 0; iat < nat; iat++) {
       for (int jat = 0; jat < iat; jat++) {
         double vec3[3];
@@ -1996,6 +2348,11 @@ Result singlepoint(int nat, const int* z, const double* xyz, double charge, cons
         for (int iao = 0; iao < m.nao_sh[s]; iao++) d += dsedcn[s] * P(ii + iao, ii + iao);
         dedcn[iat] += d;
       }
+    }
+    }
+    for (int t = 0; t < nt; t++) {
+      for (int a = 0; a < 3 * nat; a++) g[a] += tg[t][a];
+      for (int a = 0; a < nat; a++) dedcn[a] += tc[t][a];
     }
     // CN chain (terms/chain.rs:288, base/src/ncoord.rs:486-519)
     gfn2_cn_gradient(m, dedcn, g);
@@ -2069,6 +2426,127 @@ int oracle_engrad_guess(int nat, const int* z, const double* xyz, double charge,
     last_error = e.what();
     return 2;
   }
+}
+
+// The single point with the mixer plan named (0 broyden, 1 diis, 2 auto = the axtb conventions'
+// default) and the SCF report: info[0..5] = iterations of the result (of all attempts on
+// ScfNotConverged), mixer used (0 broyden, 1 diis, 2 diis-fallback), attempts, attempt 1
+// iterations, attempt 1 converged, attempt 2 iterations; info[6] = attempt 2 converged.
+// scal[0..4] = efermi, dipole x y z, HOMO-LUMO gap.
+int oracle_singlepoint(int nat, const int* z, const double* xyz, double charge, double acc, double etemp,
+                       int max_iter, int guess, int want_grad, int dispersion, int mixer, const double* gq,
+                       const double* gdp, const double* gqp, double* energy, double* terms, double* grad, double* qat,
+                       double* qsh, double* dp, double* qp, double* emo, int* info, double* scal) {
+  Options o;
+  o.gq = gq;
+  o.gdp = gdp;
+  o.gqp = gqp;
+  o.acc = acc;
+  o.etemp = etemp;
+  o.max_iter = max_iter;
+  o.guess = guess;
+  o.want_grad = want_grad;
+  o.dispersion = dispersion;
+  o.mixer = mixer;
+  Result r;
+  try {
+    r = singlepoint(nat, z, xyz, charge, o, &r);
+  } catch (Error& e) {
+    last_error = e.what();
+    if (info) {
+      info[0] = r.iterations;
+      info[1] = r.mixer_used;
+      info[2] = r.nattempts;
+      info[3] = r.attempt_iters[0];
+      info[4] = r.attempt_conv[0];
+      info[5] = r.attempt_iters[1];
+      info[6] = r.attempt_conv[1];
+    }
+    return e.code;
+  } catch (std::exception& e) {
+    last_error = e.what();
+    return 2;
+  }
+  if (energy) *energy = r.energy;
+  if (terms) std::memcpy(terms, r.terms, sizeof r.terms);
+  if (grad && want_grad) std::memcpy(grad, r.grad.data(), sizeof(double) * 3 * nat);
+  if (qat) std::memcpy(qat, r.qat.data(), sizeof(double) * nat);
+  if (qsh) std::memcpy(qsh, r.qsh.data(), sizeof(double) * r.nsh);
+  if (dp) std::memcpy(dp, r.dp.data(), sizeof(double) * 3 * nat);
+  if (qp) std::memcpy(qp, r.qp.data(), sizeof(double) * 6 * nat);
+  if (emo) std::memcpy(emo, r.emo.data(), sizeof(double) * r.nao);
+  if (info) {
+    info[0] = r.iterations;
+    info[1] = r.mixer_used;
+    info[2] = r.nattempts;
+    info[3] = r.attempt_iters[0];
+    info[4] = r.attempt_conv[0];
+    info[5] = r.attempt_iters[1];
+    info[6] = r.attempt_conv[1];
+  }
+  if (scal) {
+    scal[0] = r.efermi;
+    for (int k = 0; k < 3; k++) scal[1 + k] = r.dipole[k];
+    scal[4] = r.gap;
+  }
+  return 0;
+}
+
+// Many molecules at once, one molecule per OpenMP thread (inner loops serial): the CPU baseline
+// of a batch, as axtb runs many calculations (one thread each). Molecule i has nats[i] atoms at
+// offset off[i] of z / xyz (bohr); energies[i] and status[i] are written; gradients (3 N_i at
+// 3 off[i]) if grads is not null.
+int oracle_engrad_many(int nmol, const int* nats, const long long* off, const int* z, const double* xyz,
+                       double acc, double etemp, int want_grad, double* energies, double* grads, int* status) {
+  const int outer = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(outer)
+  for (int i = 0; i < nmol; i++) {
+    omp_set_num_threads(1);
+    Options o;
+    o.acc = acc;
+    o.etemp = etemp;
+    o.want_grad = want_grad;
+    try {
+      Result r = singlepoint(nats[i], z + off[i], xyz + 3 * off[i], 0.0, o);
+      energies[i] = r.energy;
+      if (grads && want_grad) std::memcpy(grads + 3 * off[i], r.grad.data(), sizeof(double) * 3 * nats[i]);
+      status[i] = 0;
+    } catch (Error& e) {
+      status[i] = e.code;
+    } catch (std::exception&) {
+      status[i] = 2;
+    }
+  }
+  return 0;
+}
+
+// The DIIS mixer alone on the linear fixed point q = diag(eig) q + b (the reference's own test,
+// scf/mixing.rs:523-546): iterates until the RMS residual is below 1e-12 or max_steps steps;
+// returns the steps taken (-1: the mixer failed), the final input in x.
+int oracle_diis_linear(int n, const double* eig, const double* b, double damp, int max_steps, double* x) {
+  Diis m(n, damp);
+  vec out(n);
+  for (int it = 0; it <= max_steps; it++) {
+    for (int k = 0; k < n; k++) out[k] = eig[k] * m.q_in[k] + b[k];
+    for (int k = 0; k < n; k++) m.dq[k] = out[k] - m.q_in[k];
+    if (std::sqrt(dot_seq(m.dq, m.dq) / n) < 1e-12) {
+      std::memcpy(x, m.q_in.data(), sizeof(double) * n);
+      return it;
+    }
+    if (it == max_steps) break;
+    if (!m.next()) return -1;
+  }
+  std::memcpy(x, m.q_in.data(), sizeof(double) * n);
+  return max_steps + 1;
+}
+
+// Pulay coefficients of m residuals of length n (row i = residual i, oldest first) into c[m]
+// (scf/mixing.rs:467-513).
+void oracle_diis_coefficients(int m, int n, const double* r, double* c) {
+  std::deque<vec> h;
+  for (int i = 0; i < m; i++) h.push_back(vec(r + (size_t)i * n, r + (size_t)(i + 1) * n));
+  vec cc = diis_coefficients(h);
+  std::memcpy(c, cc.data(), sizeof(double) * m);
 }
 
 int oracle_engrad(int nat, const int* z, const double* xyz, double charge, double acc, double etemp, int max_iter,

@@ -42,10 +42,11 @@ def test_energy_acc1(name):
 
 
 @pytest.mark.slow
-@pytest.mark.parametrize("name", ["C50H102"])
+@pytest.mark.parametrize("name", ["C50H102", "(H2O)100", "C100H202", "(H2O)200"])
 def test_energy_acc1_large(name):
     g = GOLD["acc1"][name]
-    z, x = structure("c50h102")
+    files = {"C50H102": "c50h102", "(H2O)100": "water100", "(H2O)200": "water200"}
+    z, x = structures.alkane_bench(100) if name == "C100H202" else structure(files[name])
     r = oracle.engrad(z, x, acc=1.0, gradient=False)
     assert r["iterations"] == g["iterations"]
     assert abs(r["energy"] - g["energy"]) < 1e-8
@@ -207,3 +208,66 @@ def test_warm_start_golden_iterations():
     q = oracle.engrad(z, x, start={"qsh": f["qsh"], "dipoles": 0 * f["dipoles"], "quadrupoles": 0 * f["quadrupoles"]})
     assert q["iterations"] > w["iterations"]
     assert abs(q["energy"] - f["energy"]) < 1e-9
+
+
+def test_mixer_plan_stretched_water():
+    """The reference's golden case for the mixer plan (crates/axtb/tests/mixer.rs:56-92): water with
+    both O-H bonds at 3 A, acc 1e-4. Broyden oscillates for all 250 iterations; DIIS from the start
+    converges to -4.6080231096 Eh; auto is DIIS's result, bit for bit, reported as diis-fallback.
+    With max_iter 5 both attempts fail and the error counts the iterations of both."""
+    z, x = structure("scan-H2O-3.0")
+    rep = {}
+    with pytest.raises(oracle.OracleError) as e:
+        oracle.engrad(z, x, acc=1e-4, mixer="broyden", report=rep)
+    assert e.value.code == 3 and rep["attempts"] == [("broyden", 250, False)]
+    d = oracle.engrad(z, x, acc=1e-4, mixer="diis")
+    a = oracle.engrad(z, x, acc=1e-4)
+    assert a["energy"] == d["energy"] and np.array_equal(a["gradient"], d["gradient"])
+    assert a["iterations"] == d["iterations"]
+    assert abs(a["energy"] - -4.6080231096) < 1e-8
+    assert d["mixer_used"] == "diis" and a["mixer_used"] == "diis-fallback"
+    assert a["attempts"] == [("broyden", 250, False), ("diis", d["iterations"], True)]
+    rep = {}
+    with pytest.raises(oracle.OracleError) as e:
+        oracle.engrad(z, x, acc=1e-4, max_iter=5, gradient=False, report=rep)
+    assert e.value.code == 3 and rep["iterations"] == 10
+
+
+def test_mixer_auto_is_broyden_where_broyden_converges():
+    """mixer.rs:37-53: auto equals Broyden bit for bit where Broyden converges, and DIIS alone lands
+    on the same solution."""
+    z, x = structure("caffeine")
+    b = oracle.engrad(z, x, acc=1e-6, mixer="broyden")
+    a = oracle.engrad(z, x, acc=1e-6)
+    assert a["energy"] == b["energy"] and np.array_equal(a["gradient"], b["gradient"])
+    assert a["mixer_used"] == "broyden" and a["attempts"] == [("broyden", b["iterations"], True)]
+    d = oracle.engrad(z, x, acc=1e-6, mixer="diis")
+    assert abs(d["energy"] - b["energy"]) < 1e-10 and d["mixer_used"] == "diis"
+
+
+def test_diis_linear_fixed_point_and_coefficients():
+    """scf/mixing.rs:523-577: on q = A q + b DIIS converges in about as many steps as A has distinct
+    eigenvalues, also where plain damped mixing diverges (eigenvalue -4); the Pulay coefficients
+    sum to one and cancel the residuals; a residual parallel to the newest one gets no weight."""
+    eig = np.array([0.9, 0.5, -0.5, -0.95, -4.0, 0.2])
+    b = np.array([1.0, -2.0, 0.5, 3.0, -1.0, 0.25])
+    steps, xq = oracle.diis_linear(eig, b, 0.4, max_steps=2 * len(eig))
+    assert 0 <= steps <= 2 * len(eig)
+    assert np.abs(xq - b / (1.0 - eig)).max() < 1e-10
+    c = oracle.diis_coefficients([[1.0, 0.0], [-3.0, 0.0]])
+    assert abs(c.sum() - 1.0) < 1e-14 and abs(c[0] * 1.0 + c[1] * -3.0) < 1e-14
+    c = oracle.diis_coefficients([[2.0, 2.0], [1.0, 1.0], [1.0, 1.0]])
+    assert c[1] == 0.0 and abs(c.sum() - 1.0) < 1e-14
+
+
+def test_engrad_many_matches_single():
+    """The batch entry (one molecule per thread) gives each molecule's single-point result."""
+    mols = structures.batch(6, seed=3)
+    e, g, st = oracle.engrad_many(mols, acc=1e-2)
+    assert (st == 0).all()
+    off = 0
+    for i, (z, x) in enumerate(mols):
+        r = oracle.engrad(z, x, acc=1e-2)
+        assert abs(e[i] - r["energy"]) < 1e-10
+        assert np.abs(g[off:off + len(z)] - r["gradient"]).max() < 1e-10
+        off += len(z)
