@@ -15,7 +15,8 @@ OK, ERR_INPUT, ERR_NUMERICAL, ERR_SCF, ERR_ELEMENT, ERR_DEVICE = range(6)
 ERROR_KINDS = {1: "input", 2: "numerical", 3: "scf_not_converged", 4: "unsupported_element", 5: "device"}
 TERMS = ("repulsion", "dispersion", "electronic", "isotropic_es", "third_order", "anisotropic", "entropy", "band")
 EXPORTS = ("axtbhip_version", "axtbhip_context_create", "axtbhip_context_destroy", "axtbhip_last_error",
-           "axtbhip_basis_size", "axtbhip_gfn2_engrad", "axtbhip_gfn2_engrad_batch", "axtbhip_gfn2_engrad_batch_guess",
+           "axtbhip_basis_size", "axtbhip_gfn2_engrad", "axtbhip_gfn2_engrad_guess", "axtbhip_gfn2_engrad_batch",
+           "axtbhip_gfn2_engrad_batch_guess", "axtbhip_batch_adjoint",
            "axtbhip_batch_prepare", "axtbhip_batch_prepare_guess", "axtbhip_batch_update",
            "axtbhip_batch_run", "axtbhip_batch_gradient", "axtbhip_batch_results", "axtbhip_profile_enable",
            "axtbhip_profile_report", "axtbhip_synchronize")
@@ -42,7 +43,11 @@ class Guess(ctypes.Structure):
 class Opts(ctypes.Structure):
     _fields_ = [("accuracy", ctypes.c_double), ("etemp", ctypes.c_double), ("max_iter", ctypes.c_int),
                 ("damping", ctypes.c_double), ("guess", ctypes.c_int), ("want_gradient", ctypes.c_int),
-                ("precision", ctypes.c_int), ("dispersion", ctypes.c_int)]
+                ("precision", ctypes.c_int), ("dispersion", ctypes.c_int), ("mixer", ctypes.c_int)]
+
+
+MIXERS = {"auto": 0, "broyden": 1, "diis": 2}
+MIXER_USED = {0: "broyden", 1: "diis", 2: "diis-fallback"}
 
 
 class Result(ctypes.Structure):
@@ -50,7 +55,10 @@ class Result(ctypes.Structure):
                 ("shell_charges", ctypes.c_void_p), ("dipoles", ctypes.c_void_p), ("quadrupoles", ctypes.c_void_p),
                 ("orbital_energies", ctypes.c_void_p), ("terms", ctypes.c_double * 8), ("efermi", ctypes.c_double),
                 ("residual", ctypes.c_double), ("energy_change", ctypes.c_double), ("iterations", ctypes.c_int),
-                ("status", ctypes.c_int), ("nsh", ctypes.c_int), ("nao", ctypes.c_int)]
+                ("status", ctypes.c_int), ("nsh", ctypes.c_int), ("nao", ctypes.c_int),
+                ("dipole", ctypes.c_double * 3), ("gap", ctypes.c_double), ("mixer_used", ctypes.c_int),
+                ("nattempts", ctypes.c_int), ("attempt_iterations", ctypes.c_int * 2),
+                ("attempt_converged", ctypes.c_int * 2), ("element", ctypes.c_int)]
 
 
 _lib = None
@@ -70,6 +78,9 @@ def lib():
         L.axtbhip_basis_size.argtypes = [ctypes.POINTER(Mol), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         L.axtbhip_gfn2_engrad.argtypes = [ctypes.c_void_p, ctypes.POINTER(Mol), ctypes.POINTER(Opts),
                                           ctypes.POINTER(Result)]
+        L.axtbhip_gfn2_engrad_guess.argtypes = [ctypes.c_void_p, ctypes.POINTER(Mol), ctypes.POINTER(Opts),
+                                                ctypes.POINTER(Guess), ctypes.POINTER(Result)]
+        L.axtbhip_batch_adjoint.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.axtbhip_gfn2_engrad_batch.argtypes = [ctypes.c_void_p, ctypes.POINTER(Mol), ctypes.c_int,
                                                 ctypes.POINTER(Opts), ctypes.POINTER(Result)]
         L.axtbhip_gfn2_engrad_batch_guess.argtypes = [ctypes.c_void_p, ctypes.POINTER(Mol), ctypes.c_int,
@@ -91,9 +102,9 @@ def lib():
 
 
 def make_opts(accuracy=1.0, etemp=300.0, max_iter=250, damping=0.4, guess="eeq", gradient=True, precision="fp64",
-              dispersion=True):
+              dispersion=True, mixer="auto"):
     return Opts(float(accuracy), float(etemp), int(max_iter), float(damping), 0 if guess == "eeq" else 1,
-                int(bool(gradient)), 0 if precision == "fp64" else 1, int(bool(dispersion)))
+                int(bool(gradient)), 0 if precision == "fp64" else 1, int(bool(dispersion)), MIXERS[mixer])
 
 
 class Context:
@@ -184,6 +195,19 @@ class Context:
         if rc:
             raise AxtbError(rc, self.last_error())
 
+    def adjoint(self, dE):
+        """dE[i] * dE_i/dR for the prepared molecules, packed in input order ([sum nat_i, 3])."""
+        mols = self._prep[0]
+        dE = np.ascontiguousarray(dE, dtype=np.float64)
+        if dE.shape != (len(mols),):
+            raise AxtbError(ERR_INPUT, "one dE per prepared molecule")
+        g = np.zeros((sum(m.nat for m in mols), 3))
+        rc = lib().axtbhip_batch_adjoint(self._h, dE.ctypes.data, g.ctypes.data)
+        if rc:
+            raise AxtbError(rc, self.last_error())
+        self._grad_ready = True
+        return g
+
     def gradient(self):
         """The analytic gradient of the last run() (which may have run without gradient)."""
         rc = lib().axtbhip_batch_gradient(self._h)
@@ -239,15 +263,23 @@ class Context:
 
     def _unmarshal(self, rc, res, outs, opts, raise_on_error, with_gradient=None):
         n = len(outs)
-        if rc == ERR_DEVICE or (raise_on_error and rc):
+        # a nonzero return with no molecule's own status is an error of the call itself
+        if rc == ERR_DEVICE or (raise_on_error and rc) or (rc and not any(res[i].status for i in range(n))):
             raise AxtbError(rc, self.last_error())
         results = []
         for i in range(n):
             r = res[i]
             o = outs[i]
+            kinds = ("broyden", "diis")
+            first = 1 if (r.mixer_used == 1 and r.nattempts == 1) else 0
             d = {"status": r.status, "energy": r.energy, "iterations": r.iterations, "efermi": r.efermi,
                  "residual": r.residual, "energy_change": r.energy_change,
-                 "terms": dict(zip(TERMS, list(r.terms))), "nsh": r.nsh, "nao": r.nao}
+                 "terms": dict(zip(TERMS, list(r.terms))), "nsh": r.nsh, "nao": r.nao,
+                 "dipole": np.array(list(r.dipole)), "gap": r.gap,
+                 "mixer_used": MIXER_USED.get(r.mixer_used) if r.nattempts else None,
+                 "attempts": [(kinds[first if a == 0 else 1], r.attempt_iterations[a], bool(r.attempt_converged[a]))
+                              for a in range(r.nattempts)],
+                 "element": r.element or None}
             if r.status == 0:
                 d.update(o)
                 d["shell_charges"] = o["shell_charges"][:r.nsh]
@@ -257,9 +289,14 @@ class Context:
             results.append(d)
         return results
 
-    def engrad(self, numbers, positions, charge=0.0, uhf=None, **kw):
+    def engrad(self, numbers, positions, charge=0.0, uhf=None, start=None, **kw):
+        """One molecule (axtbhip_gfn2_engrad_guess); start: None or a given SCF start (a dict with
+        "shell_charges", "dipoles", "quadrupoles")."""
         opts = make_opts(**kw)
-        r = self.engrad_batch([(numbers, positions, charge, uhf)], opts)[0]
+        mols, res, keep, outs = self._marshal([(numbers, positions, charge, uhf)])
+        gs = self._starts(mols, [start], keep) if start is not None else None
+        rc = lib().axtbhip_gfn2_engrad_guess(self._h, mols, ctypes.byref(opts), gs, res)
+        r = self._unmarshal(rc, res, outs, opts, False)[0]
         if r["status"]:
             raise AxtbError(r["status"], self.last_error())
         return r

@@ -4,9 +4,9 @@
 [nmol] of conformers of one molecule (positions [nmol, nat, 3] in bohr). The forward pass runs
 the engine's batched SCC without the gradient and leaves the converged state (P, the orbitals,
 the potentials) resident on the GPU; backward runs the adjoint step on that state
-(``axtbhip_batch_gradient``: W and P contracted with dS/dR and dH0/dR recomputed per shell-pair
+(``axtbhip_batch_adjoint``: W and P contracted with dS/dR and dH0/dR recomputed per shell-pair
 block, plus the classical and CN-chain terms, as ``XtbModel::gradient_functional``,
-crates/axtb/src/xtb.rs:392-421) and returns grad_output * dE/dR. Nothing of size 3 N^2 is stored.
+crates/axtb/src/xtb.rs:392-421) weighted by grad_output, grad_output[i] * dE_i/dR. Nothing of size 3 N^2 is stored.
 If the context ran another batch between forward and backward, backward recomputes the state.
 """
 import itertools
@@ -58,11 +58,10 @@ class GFN2Energy(torch.autograd.Function):
             c.prepare(ctx.mols, _lib.make_opts(accuracy=accuracy, gradient=False, precision=precision))
             c.run()
             slot[1] = ctx.serial
-        c.gradient()
-        res = c.results()
-        _check(res, c)
-        g = torch.tensor(np.stack([r["gradient"] for r in res]), dtype=torch.float64, device=grad_e.device)
-        return grad_e[:, None, None].to(g.dtype) * g, None, None, None, None, None
+        # the vector-Jacobian product on the device state (axtbhip_batch_adjoint): grad_e[i] dE_i/dR
+        dE = grad_e.detach().to("cpu", torch.float64).numpy()
+        g = c.adjoint(dE).reshape(len(ctx.mols), -1, 3)
+        return torch.tensor(g, dtype=grad_e.dtype, device=grad_e.device), None, None, None, None, None
 
 
 def energies(positions, numbers, charge=0.0, accuracy=1e-2, device=0, precision="fp64"):

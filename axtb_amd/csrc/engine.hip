@@ -37,6 +37,9 @@ struct Fail {
   std::string msg;
 };
 
+// the vector mixer of one SCF attempt (MixerKind, scf/mixing.rs:83-118)
+enum { MIX_BROYDEN = 0, MIX_DIIS = 1 };
+
 // STO-nG contraction with tblite's primitive normalisation (basis.rs:20-81, math.rs:108-111)
 void sto_ng(int ng, int n, int l, double zeta, double* alpha, double* coeff) {
   int ityp = (l == 0 ? n : 4 + n) - 1;
@@ -229,6 +232,15 @@ struct MolOut {
   double terms[8] = {0};
   int iterations = 0, nsh = 0, nao = 0;
   std::vector<double> grad, qat, qsh, dp, qp, emo;
+  double dipole[3] = {0, 0, 0}, gap = 0;
+  // ScfReport: mixer of the result (0 broyden, 1 diis, 2 diis-fallback), attempts
+  int mixer_used = 0, nattempts = 0, att_iter[2] = {0, 0}, att_conv[2] = {0, 0};
+  int element = 0;  // the unsupported element
+};
+
+// the SCF attempts of one molecule (MixerPlan, scf/mixing.rs:120-170)
+struct Attempts {
+  int n = 0, kind[2] = {0, 0}, iter[2] = {0, 0}, conv[2] = {0, 0};
 };
 
 struct Engine {
@@ -252,6 +264,33 @@ struct Engine {
       fst, eeqA, eeqV, eeqZ, eeqw, eeqr, eeqy, qstart, gw, dgw, atmP, e3, g3, dcn3, erep, grep;
   DBuf<double> qin, qout, dq, qlast, dqlast, hdf, hu, hs, qsh, qat, dp, qp, qsh_o, qat_o, dp_o, qp_o, vsh, vat, vdp,
       vqp, gq, cols, eband, eel, eterm, mst, grad, dedcn, dedcn4;
+  // Broyden history (slot-major, grown with the iterations: ensure_hist), DIIS ring and state
+  DBuf<double> hom, ham, gbeta, dhq, dhr, dbas, dbq, dbr, ddst;
+  DBuf<int> dist, dmask;
+  int hist_slots = 0;
+  // df and u of `slots` history slots (slot * tot_vec + vec0), their omega (slot * nmol + m) and
+  // overlaps (packed triangle * nmol + m); a grown buffer keeps the slots already written
+  void ensure_hist(int slots) {
+    if (slots <= hist_slots) return;
+    const int ns = std::min(cap, std::max(slots, std::max(16, 2 * hist_slots)));
+    const size_t NM = mols.size();
+    auto grow = [&](DBuf<double>& d, size_t keep, size_t need) {
+      if (need <= d.n) return;
+      double* p = nullptr;
+      HIP_OK(hipMalloc(&p, need * sizeof(double)));
+      if (keep && d.p) HIP_OK(hipMemcpyAsync(p, d.p, keep * sizeof(double), hipMemcpyDeviceToDevice, st));
+      HIP_OK(hipStreamSynchronize(st));
+      if (d.p) HIP_OK(hipFree(d.p));
+      d.p = p;
+      d.n = need;
+    };
+    auto tri = [](size_t k) { return k * (k + 1) / 2; };
+    grow(hdf, (size_t)hist_slots * tot_vec, (size_t)ns * tot_vec);
+    grow(hu, (size_t)hist_slots * tot_vec, (size_t)ns * tot_vec);
+    grow(hom, (size_t)hist_slots * NM, (size_t)ns * NM);
+    grow(ham, tri(hist_slots) * NM, tri(ns) * NM);
+    hist_slots = ns;
+  }
   DBuf<int> active, status, kocc, itc, jstat, eeqfail;
   DBuf<long long> eq0;
   DBuf<GemmJob> gj;
@@ -280,6 +319,8 @@ struct Engine {
     if (const char* v = std::getenv("AXTB_HIP_GEMM_SMALL")) gemm_small = std::atoi(v) != 0;
     if (const char* v = std::getenv("AXTB_HIP_EXACT_LEVELS")) exact_levels = std::atoi(v) != 0;
     if (const char* v = std::getenv("AXTB_HIP_DENSE_ACCEPT")) dense_accept = std::min(1e-3, std::max(1e-12, std::atof(v)));
+    // a lower limit makes Jacobi solves end unconverged (the error path's test)
+    if (const char* v = std::getenv("AXTB_HIP_JACOBI_SWEEPS")) jacobi_sweeps = std::min(60, std::max(1, std::atoi(v)));
   }
   bool dense(int n) const { return n > dense_min; }
 
@@ -405,8 +446,12 @@ struct Engine {
   // run on a second stream beside the mode 2 launch, both joined back into the engine stream.
   // fp32: the rotations in FP32 (A and V in LDS; Z and w returned in FP64 for the refinement).
   // tol_scale multiplies the rotation thresholds (the mixed path's FP32 solves, refined afterwards)
-  void eigh(std::vector<EigJob>& jobs, bool fp32 = false, int max_sweeps = 60, float tol_scale = 1.0f) {
+  // mol_of_job: the molecule of each job; a job that does not converge within max_sweeps then
+  // marks its molecule with status bit 16 (an eigensolver failure, Error::Numerical)
+  void eigh(std::vector<EigJob>& jobs, bool fp32 = false, int max_sweeps = -1, float tol_scale = 1.0f,
+            const std::vector<int>* mol_of_job = nullptr) {
     if (jobs.empty()) return;
+    if (max_sweeps < 0) max_sweeps = jacobi_sweeps;
     if (!max_lds) {
       HIP_OK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
       if (max_lds <= 0) max_lds = 64 * 1024;
@@ -422,10 +467,14 @@ struct Engine {
       gidx[mode].push_back((int)q);
     }
     std::vector<EigJob> groups[3];
-    for (int g = 0; g < 3; g++) {
+    std::vector<int> jm;  // the molecule of each job in launch order (the order of jstat)
+    for (int g = 0; g < 3; g++)
       std::stable_sort(gidx[g].begin(), gidx[g].end(), [&](int a, int b) { return jobs[a].n > jobs[b].n; });
-      for (int q : gidx[g]) groups[g].push_back(jobs[q]);
-    }
+    for (int g = 2; g >= 0; g--)
+      for (int q : gidx[g]) {
+        groups[g].push_back(jobs[q]);
+        if (mol_of_job) jm.push_back((*mol_of_job)[q]);
+      }
     jstat.ensure(jobs.size());
     const bool side = !groups[2].empty() && (!groups[1].empty() || !groups[0].empty());
     if (side) {
@@ -488,8 +537,15 @@ struct Engine {
       HIP_OK(hipEventRecord(ev_join, st2));
       HIP_OK(hipStreamWaitEvent(st, ev_join, 0));
     }
+    if (mol_of_job) {
+      jmol.upload(jm, st, &stage);
+      LAUNCH(k_jacobi_flag, dim3(blocks((long long)jm.size(), 256)), dim3(256), 0, (const int*)jstat.p,
+             (const int*)jmol.p, (int)jm.size(), status.p);
+    }
   }
   DBuf<EigJob> ejb[3];
+  DBuf<int> jmol;
+  int jacobi_sweeps = 60;  // sweeps of a Jacobi solve before it counts as not converged
 
   // ---------------------------------------------------------------- dense eigensolver (large n)
   // Eigenvalues of the symmetric n x n matrix A into w (ascending). A is overwritten with the
@@ -592,7 +648,7 @@ struct Engine {
     for (int l = 0; l < nleaf; l++)
       jobs[l] = EigJob{lm[l], dleaf.p + (size_t)l * 1024, dleafV.p + (size_t)l * 1024, dleafw.p + (size_t)l * 32,
                        dleafZ.p + (size_t)l * 1024};
-    eigh(jobs);
+    eigh(jobs, false, 60);
     dq_Q2.ensure((size_t)n * n);
     fill(Q, (long long)n * n);
     fill(dq_Q2.p, (long long)n * n);
@@ -725,11 +781,17 @@ struct Engine {
   // mismatch or an invalid geometry or start fails the call and leaves the batch as it was.
   void update(const std::vector<MolIn>& in) {
     HIP_OK(hipSetDevice(device));
+    if (!resident) throw Fail{AXTB_ERR_INPUT, "no prepared batch: batch_prepare first"};
+    if (in.size() != prep_out.size()) throw Fail{AXTB_ERR_INPUT, "the molecule count differs from the prepared batch"};
     for (int m = 0; m < (int)mols.size(); m++) {
       const Mol& M = mols[m];
+      if (map_use[m] >= (int)in.size()) throw Fail{AXTB_ERR_INPUT, "the molecule count differs from the prepared batch"};
       const MolIn& x = in[map_use[m]];
       const std::string id = "molecule " + std::to_string(map_use[m]) + ": ";
       if (x.nat != M.nat) throw Fail{AXTB_ERR_INPUT, id + "the atom count differs from the prepared batch"};
+      // the molecule (numbers, charge, unpaired electrons) is fixed at prepare, as a calculator's
+      if (x.charge != M.charge || x.uhf != huhf[m])
+        throw Fail{AXTB_ERR_INPUT, id + "the charge or unpaired electrons differ from the prepared batch"};
       for (int a = 0; a < M.nat; a++) {
         if (x.z[a] != hz[M.at0 + a]) throw Fail{AXTB_ERR_INPUT, id + "the elements differ from the prepared batch"};
         for (int k = 0; k < 3; k++)
@@ -767,12 +829,14 @@ struct Engine {
   // validate and upload a batch (per-molecule input errors are reported in `out`)
   void prepare(const std::vector<MolIn>& in, const axtb_opts& o, std::vector<MolOut>& out) {
     HIP_OK(hipSetDevice(device));
-    out.assign(in.size(), MolOut());
     // validation (base/src/molecule.rs:35-64, xtb.rs:447,1159-1212)
     if (!(o.accuracy > 0.0 && std::isfinite(o.accuracy))) throw Fail{AXTB_ERR_INPUT, "accuracy must be positive"};
     if (!(o.etemp >= 0.0 && std::isfinite(o.etemp))) throw Fail{AXTB_ERR_INPUT, "electronic temperature must be >= 0 K"};
     if (!(o.damping > 0.0 && o.damping <= 1.0)) throw Fail{AXTB_ERR_INPUT, "damping must be in (0, 1]"};
     if (o.max_iter < 1) throw Fail{AXTB_ERR_INPUT, "max_iter must be >= 1"};
+    if (o.mixer < AXTB_MIXER_AUTO || o.mixer > AXTB_MIXER_DIIS) throw Fail{AXTB_ERR_INPUT, "unknown SCF mixer"};
+    resident = false;
+    out.assign(in.size(), MolOut());
     std::vector<int> use;  // input index of each batch molecule
     for (size_t i = 0; i < in.size(); i++) {
       const MolIn& m = in[i];
@@ -787,6 +851,7 @@ struct Engine {
         if (!sp_element(m.z[a])) {
           r.status = AXTB_ERR_UNSUPPORTED_ELEMENT;
           r.msg = "element " + std::to_string(m.z[a]) + " is not supported by the GPU engine";
+          r.element = m.z[a];
           ok = false;
         }
         for (int k = 0; k < 3 && ok; k++)
@@ -840,11 +905,50 @@ struct Engine {
 
   // the prepared batch on the device: setup, SCC and gradient (inputs resident in HBM)
   bool scf_done = false, grad_done = false;
+  // the resident batch of batch_prepare (valid until a one-shot run re-prepares the engine) and
+  // the per-input results of its validation
+  bool resident = false;
+  std::vector<MolOut> prep_out;
+  std::vector<Attempts> att;
+  // iterations and outcome of attempt k of the molecules in mask, from the iteration records
+  void record_attempt(int k, int kind, const std::vector<int>& mask) {
+    const int NM = nmol();
+    std::vector<double> hm(8 * (size_t)NM);
+    HIP_OK(hipMemcpyAsync(hm.data(), mst.p, sizeof(double) * 8 * NM, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    for (int m = 0; m < NM; m++) {
+      if (!mask[m]) continue;
+      att[m].n = k + 1;
+      att[m].kind[k] = kind;
+      att[m].iter[k] = (int)hm[8 * m + 5];
+      att[m].conv[k] = hm[8 * m + 6] == 1.0;
+    }
+  }
   void compute() {
     scf_done = grad_done = false;
     if (mols.empty()) return;
     setup(opts_);
-    scf(opts_);
+    // the mixer plan (xtb.rs:590-646): one attempt per mixer, each from the start; auto runs DIIS
+    // for the molecules Broyden did not converge (not for model errors: a non-finite energy or an
+    // eigensolver failure ends the calculation, as the reference's `?` does)
+    const int NM = nmol();
+    std::vector<int> all(NM, 1);
+    att.assign(NM, Attempts());
+    const int first = opts_.mixer == AXTB_MIXER_DIIS ? MIX_DIIS : MIX_BROYDEN;
+    scf(opts_, first, &all);
+    record_attempt(0, first, all);
+    if (opts_.mixer == AXTB_MIXER_AUTO) {
+      std::vector<int> hst(NM), again(NM, 0);
+      HIP_OK(hipMemcpyAsync(hst.data(), status.p, sizeof(int) * NM, hipMemcpyDeviceToHost, st));
+      HIP_OK(hipStreamSynchronize(st));
+      bool any = false;
+      for (int m = 0; m < NM; m++)
+        if (!att[m].conv[0] && (hst[m] & ~(2 | 8)) == 0) any = again[m] = 1;
+      if (any) {
+        scf(opts_, MIX_DIIS, &again);
+        record_attempt(1, MIX_DIIS, again);
+      }
+    }
     scf_done = true;
     if (opts_.want_gradient) {
       gradient();
@@ -863,10 +967,19 @@ struct Engine {
     HIP_OK(hipStreamSynchronize(st));
   }
 
+  // a one-shot calculation (the engine's resident batch is replaced and no longer usable)
   void run(const std::vector<MolIn>& in, const axtb_opts& o, std::vector<MolOut>& out) {
     prepare(in, o, out);
     compute();
     if (!mols.empty()) collect(o, out);
+  }
+  // the resident batch: prepared once, then computed by batch_run / updated by batch_update
+  void prepare_resident(const std::vector<MolIn>& in, const axtb_opts& o) {
+    std::vector<MolOut> out;
+    prepare(in, o, out);
+    scf_done = grad_done = false;
+    prep_out.swap(out);
+    resident = true;
   }
   axtb_opts opts_{};
 
@@ -875,6 +988,7 @@ struct Engine {
   std::vector<double> hxyz, hrcov, hen, hrad, hra, hrz, hdk, hqk, har, hav, hchi, heta, hkcn, herad, hreeq, hd4rcov,
       hd4zeff, hd4gi, hd4r4r2, hd4refcn, hd4refq, hsal, hsco, hse0, hskcn, hsshp, hsref, hshard, hsgam3, hsmin, hszeta;
   std::vector<int> map_use;  // batch molecule -> input index
+  std::vector<int> huhf;     // batch molecule -> the unpaired electrons it was prepared with
   // given starts of the batch molecules (flag per molecule; moments in the batch layout)
   std::vector<int> hgflag;
   std::vector<double> hgq, hgdp, hgqp;
@@ -894,11 +1008,12 @@ struct Engine {
       v->clear();
     mols.clear();
     map_use.clear();
+    huhf.clear();
     hgflag.clear();
     hgq.clear();
     hgdp.clear();
     hgqp.clear();
-    cap = std::max(1, std::min(o.max_iter, 64));
+    cap = std::max(1, o.max_iter);  // the reference's Broyden memory: max_iter (mixing.rs:104-109)
     std::map<int, int> species;
     tot_mat = tot_pair = tot_shp = tot_eq = tot_hist = 0;
     tot_vec = 0;
@@ -1039,9 +1154,9 @@ struct Engine {
       tot_pair += (long long)m.nat * m.nat;
       tot_shp += (long long)nsh * nsh;
       tot_eq += (long long)(m.nat + 1) * (m.nat + 1);
-      tot_hist += (long long)cap * M.ndim;
       mols.push_back(M);
       map_use.push_back(ii);
+      huhf.push_back(m.uhf);
       hgflag.push_back(m.gq != nullptr);
       for (int k = 0; k < nsh; k++) hgq.push_back(m.gq ? m.gq[k] : 0.0);
       for (int k = 0; k < 3 * m.nat; k++) hgdp.push_back(m.gq && m.gdp ? m.gdp[k] : 0.0);
@@ -1183,9 +1298,6 @@ struct Engine {
     for (auto* d : {&eeqA, &eeqV, &eeqZ}) d->ensure(tot_eq);
     for (auto* d : {&eeqw, &eeqr, &eeqy}) d->ensure(NA + NM);
     for (auto* d : {&qin, &qout, &dq, &qlast, &dqlast}) d->ensure(tot_vec);
-    hdf.ensure(tot_hist);
-    hu.ensure(tot_hist);
-    hs.ensure((size_t)NM * (cap + cap * cap));
     fst.ensure(2 * (size_t)NM);
     mst.ensure(8 * (size_t)NM);
     for (auto* d : {&active, &status, &kocc, &itc, &eeqfail}) d->ensure(NM);
@@ -1656,23 +1768,36 @@ struct Engine {
   std::vector<int> hactive;
   int iters_ = 0;
 
-  void scf(const axtb_opts& o) {
+  // One SCF attempt of the molecules in `mask` (all when null) with the vector mixer `mixer`
+  // (MIX_BROYDEN or MIX_DIIS), each from its start (xtb.rs:590-646); the other molecules keep
+  // their state.
+  void scf(const axtb_opts& o, int mixer = MIX_BROYDEN, const std::vector<int>* mask = nullptr) {
     Batch b = batch();
     const int NA = nat(), NS = nshell(), NO = naos(), NM = nmol();
     int naomax = 0;
     for (auto& M : mols) naomax = std::max(naomax, M.nao);
     const long long stride = tot_mat;
+    std::vector<int> in_mask = mask ? *mask : std::vector<int>(NM, 1);
+    dmask.upload(in_mask, st, &stage);
+    if (mixer == MIX_DIIS) {
+      for (auto* d : {&dhq, &dhr, &dbas}) d->ensure((size_t)DIIS_SIZE * tot_vec);
+      for (auto* d : {&dbq, &dbr}) d->ensure(tot_vec);
+      ddst.ensure(2 * (size_t)NM);
+      dist.ensure(4 * (size_t)NM);
+    }
+    LAUNCH(k_attempt_start, dim3(blocks(NM, 256)), dim3(256), 0, (const int*)dmask.p, NM, active.p, itc.p, status.p,
+           mst.p, mixer == MIX_DIIS ? ddst.p : (double*)nullptr, dist.p, o.damping);
+    hist_slots = 0;
     // start moments -> q_in (dipoles and quadrupoles zero); a given start replaces them
     fill(dp.p, 3LL * NA);
     fill(qp.p, 6LL * NA);
     if (any_guess)
       LAUNCH(k_start_guess, dim3(blocks(NA, 128)), dim3(128), 0, b, (const int*)g_flag.p, (const double*)g_q.p,
              (const double*)g_dp.p, (const double*)g_qp.p, qstart.p, dp.p, qp.p);
-    LAUNCH(k_pack, dim3(blocks(NA, 128)), dim3(128), 0, b, (const int*)nullptr, qstart.p, dp.p, qp.p,
+    LAUNCH(k_pack, dim3(blocks(NA, 128)), dim3(128), 0, b, (const int*)active.p, qstart.p, dp.p, qp.p,
                        qin.p, (const double*)nullptr, (double*)nullptr);
     double econv = 1e-6 * o.accuracy, pconv = 2e-5 * o.accuracy;
-    size_t bshm = (16 + cap + (size_t)cap * cap) * sizeof(double) + cap * sizeof(int);
-    hactive.assign(NM, 1);
+    hactive = in_mask;
     std::vector<GemmJob> g1(NM), g2(NM), g3v(NM), gP(NM);
     std::vector<EigJob> ejobs(NM);
     std::vector<int> isdense(NM);
@@ -1700,7 +1825,9 @@ struct Engine {
     // the refinement) or by refining the previous iteration's vectors
     const bool mixed = o.precision == 1;
     std::vector<int> dfull(NM, 1), drefined(NM, 0);
-    dense_pfp.assign(NM, 0);
+    dense_pfp.resize(NM, 0);
+    for (int m = 0; m < NM; m++)
+      if (in_mask[m]) dense_pfp[m] = 0;
     HIP_OK(hipMemcpyAsync(Bm.p, X.p, sizeof(double) * tot_mat, hipMemcpyDeviceToDevice, st));
     int it = 0;
     for (it = 1; it <= o.max_iter; it++) {
@@ -1718,9 +1845,23 @@ struct Engine {
           }
         }
       if (a1.empty()) break;
-      if (it > 1)
-        LAUNCH(k_broyden, dim3(NM), dim3(256), bshm, b, active.p, cap, o.damping, qin.p, dq.p,
-                           qlast.p, dqlast.p, hdf.p, hu.p, hs.p, itc.p, status.p);
+      if (it > 1 && mixer == MIX_DIIS) {
+        LAUNCH(k_diis, dim3(NM), dim3(256), 0, b, (const int*)active.p, qin.p, (const double*)dq.p, dhq.p, dhr.p,
+               dbq.p, dbr.p, dbas.p, (long long)tot_vec, ddst.p, dist.p, status.p);
+      } else if (it > 1) {
+        // history slots the kernel reads and writes in this iteration, and the order of its system
+        const int ld = std::max(1, std::min(cap, it - 2));
+        ensure_hist(ld);
+        const size_t lds_small = (16 + 2 * (size_t)ld + (size_t)ld * ld) * sizeof(double);
+        const bool in_lds = lds_small <= 64 * 1024;
+        if (!in_lds) gbeta.ensure((size_t)NM * ld * ld);
+        const size_t bshm = in_lds ? lds_small : (16 + 2 * (size_t)ld) * sizeof(double);
+        if (bshm > 48 * 1024)
+          HIP_OK(hipFuncSetAttribute((const void*)k_broyden, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bshm));
+        LAUNCH(k_broyden, dim3(NM), dim3(256), bshm, b, active.p, cap, ld, o.damping, qin.p, dq.p, qlast.p, dqlast.p,
+               hdf.p, hu.p, (long long)tot_vec, hom.p, ham.p, NM, in_lds ? (double*)nullptr : gbeta.p, itc.p,
+               status.p);
+      }
       LAUNCH(k_unpack, dim3(blocks(NA, 128)), dim3(128), 0, b, active.p, qin.p, qsh.p, qat.p, dp.p,
                          qp.p);
       // potentials of the input moments (xtb.rs:316-340)
@@ -1785,18 +1926,23 @@ struct Engine {
           if (ok_split[qs[t]]) in_t[qs[t]] = ts[t];
         std::sort(qf.begin(), qf.end());
         for (int q : qf) ef.push_back(ea[q]);
-        eigh(ef, true);
+        eigh(ef, true, 60);
         std::vector<EigJob> redo;
-        for (int f : refine(ef, tf)) redo.push_back(ef[f]);
+        std::vector<int> redo_mol;
+        for (int f : refine(ef, tf)) {
+          redo.push_back(ef[f]);
+          redo_mol.push_back(eam[qf[f]]);
+        }
         for (size_t t = 0; t < qf.size(); t++) in_t[qf[t]] = tf[t];
-        eigh(redo);  // not converged by the refinement: FP64 Jacobi (vectors in Z; in_t is 0 for them)
+        // not converged by the refinement: FP64 Jacobi (vectors in Z; in_t is 0 for them)
+        eigh(redo, false, -1, 1.0f, &redo_mol);
         // C = B U from wherever U ended (a3 and ea list the same molecules in the same order)
         for (size_t q = 0; q < ea.size(); q++) {
           if (in_t[q]) a3[q].B = T.p + (ea[q].Z - Z.p);
           dense_pfp[eam[q]] = ok_split[q];
         }
       } else {
-        eigh(ea);
+        eigh(ea, false, -1, 1.0f, &eam);
       }
       for (int m = 0; m < NM; m++) {
         if (!hactive[m] || !isdense[m]) continue;
@@ -1886,7 +2032,7 @@ struct Engine {
     // the blocks is below the acceptance bound, so the blocks' eigenvalues are A's to second order
     // in it), each by an eigenvalue-only FP64 dense solve
     bool anypfp = false;
-    for (int m = 0; m < NM; m++) anypfp |= dense_pfp[m] != 0;
+    for (int m = 0; m < NM; m++) anypfp |= dense_pfp[m] != 0 && in_mask[m];
     if (!mixed || !anypfp) return;
     std::vector<int> hk(NM);
     HIP_OK(hipMemcpyAsync(hk.data(), kocc.p, sizeof(int) * NM, hipMemcpyDeviceToHost, st));
@@ -1897,8 +2043,9 @@ struct Engine {
     {
       std::vector<EigJob> bj;
       std::vector<BlockCopy> bc;
+      std::vector<int> bmol;
       for (int m = 0; m < NM; m++) {
-        if (!dense_pfp[m] || dense(mols[m].nao)) continue;
+        if (!dense_pfp[m] || !in_mask[m] || dense(mols[m].nao)) continue;
         const int n = mols[m].nao, k = hk[m];
         const long long o0 = mols[m].mat0, a0 = mols[m].ao0;
         long long off = 0;
@@ -1907,6 +2054,7 @@ struct Engine {
           if (nb <= 0) continue;
           bc.push_back(BlockCopy{F.p + o0 + s0 + (long long)s0 * n, T.p + o0 + off, nb, n});
           bj.push_back(EigJob{nb, T.p + o0 + off, nullptr, w.p + a0 + s0, nullptr});  // eigenvalues only
+          bmol.push_back(m);
           off += (long long)nb * nb;
         }
       }
@@ -1914,11 +2062,11 @@ struct Engine {
         dbcopy.upload(bc, st, &stage);
         LAUNCH(k_copy_blocks, dim3(bc.size()), dim3(256), 0, (const BlockCopy*)dbcopy.p);
       }
-      eigh(bj);
+      eigh(bj, false, -1, 1.0f, &bmol);
     }
     if (exact_levels) {
       for (int m = 0; m < NM; m++) {
-        if (!dense_pfp[m] || !dense(mols[m].nao)) continue;
+        if (!dense_pfp[m] || !in_mask[m] || !dense(mols[m].nao)) continue;
         const int n = mols[m].nao, k = hk[m];
         const long long o0 = mols[m].mat0, a0 = mols[m].ao0;
         for (int blk = 0; blk < 2; blk++) {
@@ -1933,7 +2081,7 @@ struct Engine {
     // the Fermi level and -TS of the refined molecules from their exact levels (the occupations
     // stay 0 or 2), into the iteration record
     std::vector<int> hp(NM, 0);
-    for (int m = 0; m < NM; m++) hp[m] = dense_pfp[m] && (exact_levels || !dense(mols[m].nao)) ? 1 : 0;
+    for (int m = 0; m < NM; m++) hp[m] = in_mask[m] && dense_pfp[m] && (exact_levels || !dense(mols[m].nao)) ? 1 : 0;
     dmlist.upload(hp, st, &stage);
     LAUNCH(k_fermi, dim3(NM), dim3(256), 0, b, (const int*)dmlist.p, w.p, focc.p, fwork.p, fst.p, kocc.p, status.p);
     HIP_OK(hipMemcpy2DAsync(mst.p + 3, 8 * sizeof(double), fst.p, 2 * sizeof(double), 2 * sizeof(double), NM,
@@ -2074,11 +2222,23 @@ struct Engine {
     std::vector<int> hst(NM), hef(NM);
     HIP_OK(hipMemcpyAsync(hst.data(), status.p, sizeof(int) * NM, hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
+    if (out.size() < mols.size() && !mols.empty()) throw Fail{AXTB_ERR_INPUT, "result list shorter than the batch"};
     for (int m = 0; m < NM; m++) {
       const Mol& M = mols[m];
+      if (map_use[m] >= (int)out.size()) throw Fail{AXTB_ERR_INPUT, "result list shorter than the batch"};
       MolOut& r = out[map_use[m]];
       double* s = &m8[8 * m];
       r.iterations = (int)s[5];
+      // the SCF report (MixerPlan::used, scf/mixing.rs:150-157): on failure the iterations of all
+      // attempts (xtb.rs:640-645)
+      const Attempts& A = m < (int)att.size() ? att[m] : Attempts();
+      r.nattempts = A.n;
+      for (int k = 0; k < 2; k++) {
+        r.att_iter[k] = A.iter[k];
+        r.att_conv[k] = A.conv[k];
+      }
+      r.mixer_used = (o.mixer == AXTB_MIXER_AUTO && A.n > 1) ? 2 : (A.n ? A.kind[A.n - 1] : 0);
+      if (s[6] != 1.0) r.iterations = A.iter[0] + (A.n > 1 ? A.iter[1] : 0);
       r.residual = s[2];
       r.energy_change = s[1];
       r.efermi = s[3];
@@ -2088,9 +2248,12 @@ struct Engine {
       } else if (hst[m] & 4) {
         r.status = AXTB_ERR_NUMERICAL;
         r.msg = "non-finite energy";
+      } else if (hst[m] & 16) {
+        r.status = AXTB_ERR_NUMERICAL;
+        r.msg = "the eigensolver did not converge";
       } else if (hst[m] & 8) {
         r.status = AXTB_ERR_NUMERICAL;
-        r.msg = "Broyden mixing failed";
+        r.msg = (A.n && A.kind[A.n - 1] == MIX_DIIS) ? "DIIS mixing failed" : "Broyden mixing failed";
       } else if (s[6] != 1.0) {
         r.status = AXTB_ERR_SCF_NOT_CONVERGED;
         r.msg = "SCF not converged in " + std::to_string(r.iterations) + " iterations";
@@ -2119,6 +2282,16 @@ struct Engine {
       r.qp.assign(q.begin() + 6 * M.at0, q.begin() + 6 * (M.at0 + M.nat));
       r.emo.assign(e.begin() + M.ao0, e.begin() + M.ao0 + M.nao);
       std::sort(r.emo.begin(), r.emo.end());
+      // molecular dipole about the origin (xtb.rs:746-751), gap above the alpha HOMO (properties.rs:204-221)
+      for (int k = 0; k < 3; k++) {
+        r.dipole[k] = 0.0;
+        for (int a = 0; a < M.nat; a++) r.dipole[k] += qa[M.at0 + a] * hxyz[3 * (M.at0 + a) + k] + d[3 * (M.at0 + a) + k];
+      }
+      {
+        const double nel = std::max(M.nalpha, M.nbeta), fl = std::floor(nel);
+        const double kk = nel - fl > 0.5 ? fl + 1.0 : fl;
+        r.gap = (kk >= 1.0 && (int)kk < M.nao) ? r.emo[(int)kk] - r.emo[(int)kk - 1] : std::nan("");
+      }
       if (grad_done && r.status == 0) r.grad.assign(g.begin() + 3 * M.at0, g.begin() + 3 * (M.at0 + M.nat));
     }
   }
@@ -2132,7 +2305,6 @@ using axtb::Engine;
 struct axtbhip_context {
   std::unique_ptr<Engine> eng;
   std::string err;
-  std::vector<axtb::MolOut> prepared;  // per-molecule input errors of the prepared batch
 };
 
 static void fill_result(const axtb::MolOut& r, const axtb_mol& m, axtb_result* out) {
@@ -2154,6 +2326,15 @@ static void fill_result(const axtb::MolOut& r, const axtb_mol& m, axtb_result* o
   put(out->dipoles, r.dp);
   put(out->quadrupoles, r.qp);
   put(out->orbital_energies, r.emo);
+  for (int k = 0; k < 3; k++) out->dipole[k] = r.dipole[k];
+  out->gap = r.gap;
+  out->mixer_used = r.mixer_used;
+  out->nattempts = r.nattempts;
+  for (int k = 0; k < 2; k++) {
+    out->attempt_iterations[k] = r.att_iter[k];
+    out->attempt_converged[k] = r.att_conv[k];
+  }
+  out->element = r.element;
   (void)m;
 }
 
@@ -2254,6 +2435,11 @@ int axtbhip_gfn2_engrad(axtbhip_context* ctx, const axtb_mol* mol, const axtb_op
   return axtbhip_gfn2_engrad_batch(ctx, mol, 1, opts, out);
 }
 
+int axtbhip_gfn2_engrad_guess(axtbhip_context* ctx, const axtb_mol* mol, const axtb_opts* opts, const axtb_guess* guess,
+                              axtb_result* out) {
+  return axtbhip_gfn2_engrad_batch_guess(ctx, mol, 1, guess, opts, out);
+}
+
 int axtbhip_batch_prepare_guess(axtbhip_context* ctx, const axtb_mol* mols, int nmol, const axtb_guess* guesses,
                                 const axtb_opts* opts) {
   if (!ctx || !mols || !opts || nmol < 0) return AXTB_ERR_INPUT;
@@ -2267,7 +2453,7 @@ int axtbhip_batch_prepare_guess(axtbhip_context* ctx, const axtb_mol* mols, int 
     }
   }
   try {
-    ctx->eng->prepare(in, *opts, ctx->prepared);
+    ctx->eng->prepare_resident(in, *opts);
     ctx->err.clear();
     return 0;
   } catch (axtb::Fail& f) {
@@ -2284,7 +2470,11 @@ int axtbhip_batch_prepare(axtbhip_context* ctx, const axtb_mol* mols, int nmol, 
 }
 
 int axtbhip_batch_update(axtbhip_context* ctx, const axtb_mol* mols, int nmol, const axtb_guess* guesses) {
-  if (!ctx || !mols || nmol != (int)ctx->prepared.size()) return AXTB_ERR_INPUT;
+  if (!ctx || !mols || nmol < 0) return AXTB_ERR_INPUT;
+  if (!ctx->eng->resident || nmol != (int)ctx->eng->prep_out.size()) {
+    ctx->err = "no prepared batch of this size: batch_prepare first";
+    return AXTB_ERR_INPUT;
+  }
   std::vector<axtb::MolIn> in(nmol);
   for (int i = 0; i < nmol; i++) {
     in[i] = axtb::MolIn{mols[i].nat, mols[i].numbers, mols[i].positions, mols[i].charge, mols[i].uhf};
@@ -2309,6 +2499,10 @@ int axtbhip_batch_update(axtbhip_context* ctx, const axtb_mol* mols, int nmol, c
 
 int axtbhip_batch_run(axtbhip_context* ctx) {
   if (!ctx) return AXTB_ERR_INPUT;
+  if (!ctx->eng->resident) {
+    ctx->err = "no prepared batch: batch_prepare first";
+    return AXTB_ERR_INPUT;
+  }
   try {
     ctx->eng->compute();
     return 0;
@@ -2323,6 +2517,10 @@ int axtbhip_batch_run(axtbhip_context* ctx) {
 
 int axtbhip_batch_gradient(axtbhip_context* ctx) {
   if (!ctx) return AXTB_ERR_INPUT;
+  if (!ctx->eng->resident) {
+    ctx->err = "no prepared batch: batch_prepare first";
+    return AXTB_ERR_INPUT;
+  }
   try {
     ctx->eng->gradient_of_state();
     return 0;
@@ -2336,10 +2534,19 @@ int axtbhip_batch_gradient(axtbhip_context* ctx) {
 }
 
 int axtbhip_batch_results(axtbhip_context* ctx, const axtb_mol* mols, int nmol, axtb_result* outs) {
-  if (!ctx || nmol != (int)ctx->prepared.size()) return AXTB_ERR_INPUT;
+  if (!ctx || !outs || nmol < 0) return AXTB_ERR_INPUT;
+  Engine& E = *ctx->eng;
+  if (!E.resident || nmol != (int)E.prep_out.size()) {
+    ctx->err = "no prepared batch of this size: batch_prepare first";
+    return AXTB_ERR_INPUT;
+  }
+  if (!E.scf_done && !E.mols.empty()) {
+    ctx->err = "the prepared batch has no results: batch_run first";
+    return AXTB_ERR_INPUT;
+  }
   try {
-    std::vector<axtb::MolOut> res = ctx->prepared;
-    if (!ctx->eng->mols.empty()) ctx->eng->collect(ctx->eng->opts_, res);
+    std::vector<axtb::MolOut> res = E.prep_out;
+    if (!E.mols.empty()) E.collect(E.opts_, res);
     int worst = 0;
     for (int i = 0; i < nmol; i++) {
       fill_result(res[i], mols[i], &outs[i]);
@@ -2349,6 +2556,37 @@ int axtbhip_batch_results(axtbhip_context* ctx, const axtb_mol* mols, int nmol, 
       }
     }
     return worst;
+  } catch (std::exception& e) {
+    ctx->err = e.what();
+    return AXTB_ERR_DEVICE;
+  }
+}
+
+int axtbhip_batch_adjoint(axtbhip_context* ctx, const double* dE, double* grad) {
+  if (!ctx || !dE || !grad) return AXTB_ERR_INPUT;
+  Engine& E = *ctx->eng;
+  if (!E.resident) {
+    ctx->err = "no prepared batch: batch_prepare first";
+    return AXTB_ERR_INPUT;
+  }
+  try {
+    E.gradient_of_state();
+    const int nin = (int)E.prep_out.size();
+    // input offsets of the atoms; inputs that failed at prepare hold zeros
+    std::vector<long long> off(nin + 1, 0);
+    std::vector<int> nat_in(nin, 0);
+    for (int m = 0; m < E.nmol(); m++) nat_in[E.map_use[m]] = E.mols[m].nat;
+    for (int i = 0; i < nin; i++) off[i + 1] = off[i] + nat_in[i];
+    std::vector<axtb::MolOut> res = E.prep_out;
+    if (!E.mols.empty()) E.collect(E.opts_, res);
+    std::memset(grad, 0, sizeof(double) * 3 * off[nin]);
+    for (int i = 0; i < nin; i++)
+      if (res[i].status == 0 && (int)res[i].grad.size() == 3 * nat_in[i])
+        for (int k = 0; k < 3 * nat_in[i]; k++) grad[3 * off[i] + k] = dE[i] * res[i].grad[k];
+    return 0;
+  } catch (axtb::Fail& f) {
+    ctx->err = f.msg;
+    return f.code;
   } catch (std::exception& e) {
     ctx->err = e.what();
     return AXTB_ERR_DEVICE;

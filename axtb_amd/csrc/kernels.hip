@@ -1153,12 +1153,21 @@ __global__ void k_converge(Batch b, int* active, const double* eel, const double
 
 // ------------------------------------------------------------------ Broyden mixing
 // Modified Broyden mixing of one molecule per workgroup (scf/mixing.rs:251-338, omega0 0.01,
-// minw 1, maxw 1e5, wfac 0.01), history of `cap` slots (the reference's `memory`), the small
-// system solved by Gaussian elimination with the last-maximum pivot (linalg/lu.rs:81-107).
-// State per molecule: q_in, dq, qlast, dqlast (vectors at vec0), df/u history (cap*ndim at hist0),
-// omega[cap], a[cap*cap] at hs0 = m*(cap+cap*cap), iteration count it[m].
-__global__ void k_broyden(Batch b, const int* active, int cap, double damp, double* qin, const double* dq,
-                          double* qlast, double* dqlast, double* df, double* u, double* hs, int* itc, int* status) {
+// minw 1, maxw 1e5, wfac 0.01) with the reference's full history (`memory` = max_iter slots,
+// reused cyclically beyond it). The history is laid out slot-major so that it grows with the
+// iterations without moving: df, u at slot * tot_vec + vec0, omega at slot * nmol + m, the
+// symmetric overlaps a(i, j) packed as (max (max + 1) / 2 + min) * nmol + m. The small system
+// (ld x ld, ld >= the history length) is solved by Gaussian elimination with the last-maximum
+// pivot (linalg/lu.rs:81-107), rows eliminated in parallel; it lives in LDS, or in `gbeta`
+// (ld * ld per molecule) when it does not fit.
+__device__ __forceinline__ long long tri_index(int i, int j) {
+  const int hi = i > j ? i : j, lo = i > j ? j : i;
+  return (long long)hi * (hi + 1) / 2 + lo;
+}
+
+__global__ void k_broyden(Batch b, const int* active, int cap, int ld, double damp, double* qin, const double* dq,
+                          double* qlast, double* dqlast, double* df, double* u, long long tot_vec, double* omega,
+                          double* amat, int nmol, double* gbeta, int* itc, int* status) {
   extern __shared__ double sm[];
   int m = blockIdx.x;
   if (!active[m]) return;
@@ -1168,13 +1177,11 @@ __global__ void k_broyden(Batch b, const int* active, int cap, double damp, doub
   const double* d = dq + M.vec0;
   double* ql = qlast + M.vec0;
   double* dl = dqlast + M.vec0;
-  double* omega = hs + (long long)m * (cap + cap * cap);
-  double* amat = omega + cap;
-  double* red = sm;                 // 16
-  double* cvec = sm + 16;           // cap
-  double* beta = cvec + cap;        // cap*cap
-  int* hist = (int*)(beta + cap * cap);
-  __shared__ int it_sh, ok_sh;
+  double* red = sm;                       // 16
+  double* cvec = sm + 16;                 // ld
+  int* hist = (int*)(cvec + ld);          // ld ints (ld doubles reserved)
+  double* beta = gbeta ? gbeta + (long long)m * ld * ld : cvec + 2 * ld;
+  __shared__ int it_sh, ok_sh, piv_sh;
   if (threadIdx.x == 0) it_sh = ++itc[m];
   __syncthreads();
   const int iter = it_sh;
@@ -1189,92 +1196,106 @@ __global__ void k_broyden(Batch b, const int* active, int cap, double damp, doub
   const int itn = iter - 1;
   const int it1 = (itn - 1) % cap;
   const int nhist = min(cap, itn);
+  if (nhist > ld) {  // the host sizes ld for the iteration; never taken
+    if (threadIdx.x == 0) atomicOr(&status[m], 8);
+    return;
+  }
   const int first = itn + 1 > cap ? itn + 1 - cap : 1;
-  const int th = threadIdx.x;
-  if (th < nhist) hist[th] = (first + th - 1) % cap;
-  double* dfi = df + M.hist0 + (long long)it1 * n;
-  double* ui = u + M.hist0 + (long long)it1 * n;
+  const int th = threadIdx.x, bd = blockDim.x;
+  for (int ih = th; ih < nhist; ih += bd) hist[ih] = (first + ih - 1) % cap;
+  double* dfi = df + (long long)it1 * tot_vec + M.vec0;
+  double* ui = u + (long long)it1 * tot_vec + M.vec0;
   double nn = 0.0;
-  for (int k = threadIdx.x; k < n; k += blockDim.x) nn += d[k] * d[k];
+  for (int k = th; k < n; k += bd) nn += d[k] * d[k];
   nn = block_sum(nn, red);
   double norm = sqrt(nn);
   double om = norm > 0.01 / 100000.0 ? 0.01 / norm : 100000.0;
   if (om < 1.0) om = 1.0;
   double ff = 0.0;
-  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+  for (int k = th; k < n; k += bd) {
     double x = d[k] - dl[k];
     dfi[k] = x;
     ff += x * x;
   }
   ff = block_sum(ff, red);
   double inv = 1.0 / fmax(sqrt(ff), 2.220446049250313e-16);
-  for (int k = threadIdx.x; k < n; k += blockDim.x) dfi[k] *= inv;
-  if (threadIdx.x == 0) omega[it1] = om;
+  for (int k = th; k < n; k += bd) dfi[k] *= inv;
+  if (th == 0) omega[(long long)it1 * nmol + m] = om;
   __syncthreads();
   // a(i, it1) for the history slots and c
   for (int ih = 0; ih < nhist; ih++) {
     int i = hist[ih];
-    const double* dfh = df + M.hist0 + (long long)i * n;
+    const double* dfh = df + (long long)i * tot_vec + M.vec0;
     double s1 = 0.0, s2 = 0.0;
-    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    for (int k = th; k < n; k += bd) {
       s1 += dfh[k] * d[k];
       s2 += dfh[k] * dfi[k];
     }
     s1 = block_sum(s1, red);
     s2 = block_sum(s2, red);
-    if (threadIdx.x == 0) {
-      cvec[ih] = omega[i] * s1;
-      amat[i * cap + it1] = s2;
-      amat[it1 * cap + i] = s2;
+    if (th == 0) {
+      cvec[ih] = omega[(long long)i * nmol + m] * s1;
+      amat[tri_index(i, it1) * nmol + m] = s2;
     }
-    __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    for (int ih = 0; ih < nhist; ih++) {
-      for (int jh = 0; jh < nhist; jh++)
-        beta[ih * cap + jh] = omega[hist[ih]] * omega[hist[jh]] * amat[hist[ih] * cap + hist[jh]];
-      beta[ih * cap + ih] += 0.01 * 0.01;
-    }
-    // Gaussian elimination, pivot on the last maximum of |beta(i, col)|
-    int okv = 1;
-    for (int col = 0; col < nhist && okv; col++) {
+  __syncthreads();
+  for (int x = th; x < nhist * nhist; x += bd) {
+    const int ih = x / nhist, jh = x - ih * nhist;
+    const int i = hist[ih], j = hist[jh];
+    double v = omega[(long long)i * nmol + m] * omega[(long long)j * nmol + m] * amat[tri_index(i, j) * nmol + m];
+    if (ih == jh) v += 0.01 * 0.01;
+    beta[ih * ld + jh] = v;
+  }
+  if (th == 0) ok_sh = 1;
+  __syncthreads();
+  // Gaussian elimination, pivot on the last maximum of |beta(i, col)|; each row's operations are
+  // those of the sequential elimination
+  for (int col = 0; col < nhist; col++) {
+    if (th == 0) {
       int piv = col;
       for (int i = col; i < nhist; i++)
-        if (fabs(beta[i * cap + col]) >= fabs(beta[piv * cap + col])) piv = i;
-      if (beta[piv * cap + col] == 0.0) {
-        okv = 0;
-        break;
+        if (fabs(beta[i * ld + col]) >= fabs(beta[piv * ld + col])) piv = i;
+      piv_sh = piv;
+      if (beta[piv * ld + col] == 0.0) ok_sh = 0;
+    }
+    __syncthreads();
+    if (!ok_sh) break;
+    const int piv = piv_sh;
+    if (piv != col) {
+      for (int k = th; k < nhist; k += bd) {
+        double t = beta[col * ld + k];
+        beta[col * ld + k] = beta[piv * ld + k];
+        beta[piv * ld + k] = t;
       }
-      if (piv != col) {
-        for (int k = 0; k < nhist; k++) {
-          double t = beta[col * cap + k];
-          beta[col * cap + k] = beta[piv * cap + k];
-          beta[piv * cap + k] = t;
-        }
+      if (th == 0) {
         double t = cvec[col];
         cvec[col] = cvec[piv];
         cvec[piv] = t;
       }
-      for (int row = col + 1; row < nhist; row++) {
-        double f = beta[row * cap + col] / beta[col * cap + col];
-        if (f != 0.0) {
-          for (int k = col; k < nhist; k++) beta[row * cap + k] -= f * beta[col * cap + k];
-          cvec[row] -= f * cvec[col];
-        }
+    }
+    __syncthreads();
+    for (int row = col + 1 + th; row < nhist; row += bd) {
+      double f = beta[row * ld + col] / beta[col * ld + col];
+      if (f != 0.0) {
+        for (int k = col; k < nhist; k++) beta[row * ld + k] -= f * beta[col * ld + k];
+        cvec[row] -= f * cvec[col];
       }
     }
-    if (okv)
+    __syncthreads();
+  }
+  if (th == 0) {
+    if (ok_sh)
       for (int col = nhist - 1; col >= 0; col--) {
         double s = cvec[col];
-        for (int k = col + 1; k < nhist; k++) s -= beta[col * cap + k] * cvec[k];
-        cvec[col] = s / beta[col * cap + col];
+        for (int k = col + 1; k < nhist; k++) s -= beta[col * ld + k] * cvec[k];
+        cvec[col] = s / beta[col * ld + col];
       }
-    ok_sh = okv;
-    if (!okv) atomicOr(&status[m], 8);
+    else
+      atomicOr(&status[m], 8);
   }
   __syncthreads();
   if (!ok_sh) return;
-  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+  for (int k = th; k < n; k += bd) {
     double uk = damp * dfi[k] + inv * (q[k] - ql[k]);
     ui[k] = uk;
     dl[k] = d[k];
@@ -1282,11 +1303,190 @@ __global__ void k_broyden(Batch b, const int* active, int cap, double damp, doub
     double x = q[k] + damp * d[k];
     for (int ih = 0; ih < nhist; ih++) {
       int i = hist[ih];
-      double uv = (i == it1) ? uk : u[M.hist0 + (long long)i * n + k];
-      x -= omega[i] * cvec[ih] * uv;
+      double uv = (i == it1) ? uk : u[(long long)i * tot_vec + M.vec0 + k];
+      x -= omega[(long long)i * nmol + m] * cvec[ih] * uv;
     }
     q[k] = x;
   }
+}
+
+// ------------------------------------------------------------------ DIIS mixing
+// Pulay (DIIS) mixing with restarts, one molecule per workgroup (scf/mixing.rs:340-513): the last
+// DIIS_SIZE inputs and residuals in a ring (slot-major, slot * tot_vec + vec0); the coefficients
+// minimise |sum c_i r_i| with sum c_i = 1, from the differences to the newest residual
+// orthogonalised newest first by modified Gram-Schmidt (a difference whose new part is below
+// DIIS_DEPENDENCE of its norm ends the basis); a residual above DIIS_RESTART_GROWTH times the best
+// one (and above DIIS_RESTART_FLOOR), or DIIS_STALL iterations without a better one, restarts from
+// the best input with half the damping. State per molecule: dst[2 m] damping, dst[2 m + 1] best
+// residual norm; ist[4 m ..] has-best, iterations since the best, history length, ring head.
+constexpr int DIIS_SIZE = 30;
+constexpr double DIIS_DEPENDENCE = 1e-2;
+constexpr int DIIS_STALL = 12;
+constexpr double DIIS_RESTART_GROWTH = 1e3;
+constexpr double DIIS_RESTART_FLOOR = 0.1;
+
+__global__ void k_diis(Batch b, const int* active, double* qin, const double* dq, double* hq, double* hr, double* bq,
+                       double* br, double* bas, long long tot_vec, double* dst, int* ist, int* status) {
+  __shared__ double red[16];
+  __shared__ double rcol[DIIS_SIZE][DIIS_SIZE];  // rcol[p][l]: column p of R
+  __shared__ double rhs[DIIS_SIZE], cf[DIIS_SIZE];
+  __shared__ int kept[DIIS_SIZE];
+  const int m = blockIdx.x;
+  if (!active[m]) return;
+  const Mol M = b.mol[m];
+  const int n = M.ndim, th = threadIdx.x, bd = blockDim.x;
+  double* q = qin + M.vec0;
+  const double* d = dq + M.vec0;
+  auto vslot = [&](double* base, int s) { return base + (long long)s * tot_vec + M.vec0; };
+  double nn = 0.0;
+  for (int k = th; k < n; k += bd) nn += d[k] * d[k];
+  const double norm = sqrt(block_sum(nn, red));
+  if (!isfinite(norm)) {
+    if (th == 0) atomicOr(&status[m], 8);
+    return;
+  }
+  double damp = dst[2 * m], best = dst[2 * m + 1];
+  int has = ist[4 * m], since = ist[4 * m + 1], cnt = ist[4 * m + 2], head = ist[4 * m + 3];
+  double* qb = bq + M.vec0;
+  double* rb = br + M.vec0;
+  if (has) {
+    const bool diverging = norm > DIIS_RESTART_GROWTH * best && norm > DIIS_RESTART_FLOOR;
+    if (diverging || since >= DIIS_STALL) {
+      damp *= 0.5;
+      for (int k = th; k < n; k += bd) q[k] = qb[k] + damp * rb[k];
+      if (th == 0) {
+        dst[2 * m] = damp;
+        ist[4 * m + 1] = 0;
+        ist[4 * m + 2] = 0;
+      }
+      return;
+    }
+  }
+  if (!has || norm < best) {
+    best = norm;
+    has = 1;
+    since = 0;
+    for (int k = th; k < n; k += bd) {
+      qb[k] = q[k];
+      rb[k] = d[k];
+    }
+  } else {
+    since++;
+  }
+  if (cnt == DIIS_SIZE) {
+    head = (head + 1) % DIIS_SIZE;
+    cnt--;
+  }
+  {
+    const int s = (head + cnt) % DIIS_SIZE;
+    double* hqs = vslot(hq, s);
+    double* hrs = vslot(hr, s);
+    for (int k = th; k < n; k += bd) {
+      hqs[k] = q[k];
+      hrs[k] = d[k];
+    }
+    cnt++;
+  }
+  // every thread owns the vector entries k = th + j bd: the element-wise passes need no barrier
+  const double* last = vslot(hr, (head + cnt - 1) % DIIS_SIZE);
+  int p = 0;
+  for (int jj = cnt - 2; jj >= 0; jj--) {
+    const double* rj = vslot(hr, (head + jj) % DIIS_SIZE);
+    double* dv = vslot(bas, p);
+    double s = 0.0;
+    for (int k = th; k < n; k += bd) {
+      const double x = last[k] - rj[k];
+      dv[k] = x;
+      s += x * x;
+    }
+    const double norm0 = sqrt(block_sum(s, red));
+    if (isnan(norm0) || norm0 <= 0.0) break;
+    for (int l = 0; l < p; l++) {
+      const double* bl = vslot(bas, l);
+      double h = 0.0;
+      for (int k = th; k < n; k += bd) h += bl[k] * dv[k];
+      h = block_sum(h, red);
+      if (th == 0) rcol[p][l] = h;
+      for (int k = th; k < n; k += bd) dv[k] -= h * bl[k];
+    }
+    double r2 = 0.0;
+    for (int k = th; k < n; k += bd) r2 += dv[k] * dv[k];
+    const double nr = sqrt(block_sum(r2, red));
+    if (nr < DIIS_DEPENDENCE * norm0) break;
+    if (th == 0) {
+      rcol[p][p] = nr;
+      kept[p] = jj;
+    }
+    for (int k = th; k < n; k += bd) dv[k] /= nr;
+    p++;
+  }
+  for (int l = 0; l < p; l++) {
+    const double* bl = vslot(bas, l);
+    double h = 0.0;
+    for (int k = th; k < n; k += bd) h += bl[k] * last[k];
+    h = block_sum(h, red);
+    if (th == 0) rhs[l] = h;
+  }
+  __syncthreads();
+  if (th == 0) {
+    double g[DIIS_SIZE];
+    for (int l = p - 1; l >= 0; l--) {
+      double v = rhs[l];
+      for (int k = l + 1; k < p; k++) v -= rcol[k][l] * g[k];
+      g[l] = v / rcol[l][l];
+    }
+    for (int i = 0; i < cnt; i++) cf[i] = 0.0;
+    cf[cnt - 1] = 1.0;
+    for (int l = 0; l < p; l++) {
+      cf[kept[l]] += g[l];
+      cf[cnt - 1] -= g[l];
+    }
+  }
+  __syncthreads();
+  for (int k = th; k < n; k += bd) {
+    double x = 0.0;
+    for (int i = 0; i < cnt; i++) {
+      const double c = cf[i];
+      if (c != 0.0) {
+        const int s = (head + i) % DIIS_SIZE;
+        x += c * (vslot(hq, s)[k] + damp * vslot(hr, s)[k]);
+      }
+    }
+    q[k] = x;
+  }
+  if (th == 0) {
+    dst[2 * m] = damp;
+    dst[2 * m + 1] = best;
+    ist[4 * m] = has;
+    ist[4 * m + 1] = since;
+    ist[4 * m + 2] = cnt;
+    ist[4 * m + 3] = head;
+  }
+}
+
+// The start of an SCF attempt for the molecules of `mask` (their iteration record, status and
+// mixer state cleared; DIIS at the given damping).
+__global__ void k_attempt_start(const int* mask, int nmol, int* active, int* itc, int* status, double* mst,
+                                double* dst, int* ist, double damp) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nmol) return;
+  active[m] = mask[m];
+  if (!mask[m]) return;
+  itc[m] = 0;
+  status[m] = 0;
+  for (int k = 0; k < 8; k++) mst[8 * m + k] = 0.0;
+  if (dst) {
+    dst[2 * m] = damp;
+    dst[2 * m + 1] = 0.0;
+    for (int k = 0; k < 4; k++) ist[4 * m + k] = 0;
+  }
+}
+
+// A Jacobi solve that did not converge within its sweeps (stat < 0) marks its molecule with
+// status bit 16 (Error::Numerical of the eigensolver).
+__global__ void k_jacobi_flag(const int* stat, const int* mol_of_job, int njobs, int* status) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < njobs && stat[q] < 0) atomicOr(&status[mol_of_job[q]], 16);
 }
 
 // ------------------------------------------------------------------ gradient

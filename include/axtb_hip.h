@@ -52,7 +52,15 @@ typedef struct axtb_opts {
   int want_gradient; /* 1: analytic gradient */
   int precision;     /* 0: FP64 throughout */
   int dispersion;    /* 1: D4 (default) */
+  /* SCF mixer plan (MixerKind, crates/axtb/src/scf/mixing.rs:83-118): 0 auto (default of the axtb
+   * conventions, conventions.rs:260: Broyden, then DIIS from the start if Broyden does not
+   * converge), 1 Broyden alone, 2 DIIS alone */
+  int mixer;
 } axtb_opts;
+
+#define AXTB_MIXER_AUTO 0
+#define AXTB_MIXER_BROYDEN 1
+#define AXTB_MIXER_DIIS 2
 
 /* Results: the SinglePoint fields of the path (crates/base/src/method/results.rs:39-230). */
 typedef struct axtb_result {
@@ -67,6 +75,16 @@ typedef struct axtb_result {
   double terms[8];
   double efermi, residual, energy_change;
   int iterations, status, nsh, nao;
+  /* SinglePoint.dipole: molecular dipole about the origin, e bohr (crates/axtb/src/xtb.rs:746-751) */
+  double dipole[3];
+  /* HOMO-LUMO gap above the alpha HOMO in Eh (properties.rs:204-240); NaN without a LUMO */
+  double gap;
+  /* ScfReport (crates/base/src/method/results.rs:156-176): the mixer of the result (0 broyden,
+   * 1 diis, 2 diis-fallback), the attempts of the plan and each one's iterations and outcome.
+   * iterations above is the converged attempt's; on AXTB_ERR_SCF_NOT_CONVERGED, all attempts'. */
+  int mixer_used, nattempts, attempt_iterations[2], attempt_converged[2];
+  /* AXTB_ERR_UNSUPPORTED_ELEMENT: the element (Error::UnsupportedElement{z}, error.rs:12), else 0 */
+  int element;
 } axtb_result;
 
 typedef struct axtbhip_context axtbhip_context;
@@ -101,6 +119,13 @@ typedef struct axtb_guess {
 int axtbhip_gfn2_engrad_batch_guess(axtbhip_context* ctx, const axtb_mol* mols, int nmol, const axtb_guess* guesses,
                                     const axtb_opts* opts, axtb_result* outs);
 
+/* axtbhip_gfn2_engrad with a given SCF start (guess may be NULL: the start opts->guess names):
+ * Method::singlepoint on a calculator holding a wavefunction (set_wavefunction,
+ * crates/base/src/method/mod.rs:71-101; crates/axtb/src/calculator.rs:185-206), including its
+ * retry from the fresh start when the given start does not converge. */
+int axtbhip_gfn2_engrad_guess(axtbhip_context* ctx, const axtb_mol* mol, const axtb_opts* opts,
+                              const axtb_guess* guess, axtb_result* out);
+
 /* A batch kept on the device between calls, for repeated or timed runs: prepare validates and
  * uploads the molecules (host arrays may be released afterwards), run computes on the resident
  * inputs, results downloads them (mols gives the array sizes). No counterpart in the reference:
@@ -128,6 +153,11 @@ int axtbhip_batch_results(axtbhip_context* ctx, const axtb_mol* mols, int nmol, 
  * (crates/axtb/src/xtb.rs:392-421) applied to the state of the last single point. Returns
  * AXTB_ERR_INPUT before any batch_run. */
 int axtbhip_batch_gradient(axtbhip_context* ctx);
+/* The vector-Jacobian product of the resident batch's energies (the backward of an energy
+ * function, XtbModel::gradient_functional, crates/axtb/src/xtb.rs:392-421, scaled): grad receives
+ * dE[i] * dE_i/dR for every prepared molecule i, packed in input order ([sum of nat_i][3], bohr);
+ * molecules that failed get zeros. Runs batch_gradient first if needed. */
+int axtbhip_batch_adjoint(axtbhip_context* ctx, const double* dE, double* grad);
 
 /* Per-kernel timing with HIP events on the context's stream (the counterpart of the reference's
  * phase timers, crates/base/src/timing.rs:11-54): report lines "kernel launches total_ms work",

@@ -272,21 +272,3 @@ def test_setup_and_first_iteration():
     gi = H.iter1(z, x)
     for k in ("vsh", "vat", "vdp", "vqp", "F", "w", "P", "qsh"):
         assert np.abs(gi[k] - o[k]).max() < 1e-10, k
-
-
-@pytest.mark.parametrize("name", ["c100h202", "water200"])
-def test_mixed_dense_refinement(ctx, name):
-    """Dense-solver molecules in mixed precision: full FP64 solves first, then the previous
-    iteration's vectors refined for the occupied/empty split, W = P F P / 2. Against the FP64
-    GPU path (itself pinned to the oracle): |dE| <= 1e-6 Eh, max |dg| <= 1e-5 Eh/bohr."""
-    from axtb_amd import _lib, structures as S
-    z, x = S.config(name)
-    f = ctx.engrad_batch([(z, x)], _lib.make_opts(accuracy=1e-2))[0]
-    m = ctx.engrad_batch([(z, x)], _lib.make_opts(accuracy=1e-2, precision="mixed"))[0]
-    assert f["status"] == 0 and m["status"] == 0
-    d = [abs(m["energy"] - f["energy"]), np.abs(m["gradient"] - f["gradient"]).max(),
-         np.abs(m["charges"] - f["charges"]).max(),
-         np.abs(m["orbital_energies"] - f["orbital_energies"]).max()]
-    print(f"{name} mixed vs fp64: |dE| {d[0]:.2e} |dg| {d[1]:.2e} |dq| {d[2]:.2e} |deps| {d[3]:.2e} "
-          f"it {m['iterations']}/{f['iterations']}", flush=True)
-    assert d[0] <= 1e-6 and d[1] <= 1e-5
