@@ -518,19 +518,13 @@ struct Engine {
       }
       HIP_OK(hipGetLastError());
       if (prof.on) {
-        // sweep statistics of the launch (profiling only: synchronises)
-        std::vector<int> hs(js.size());
-        HIP_OK(hipMemcpyAsync(hs.data(), stat, sizeof(int) * js.size(), hipMemcpyDeviceToHost, s));
-        HIP_OK(hipStreamSynchronize(s));
-        double sum = 0.0;
-        long bad = 0;
-        for (int v : hs) {
-          sum += std::abs(v);
-          bad += v < 0;
+        // sweep statistics of the launch, summed on the device (read at the profile report)
+        if (!sweep_acc.p) {
+          sweep_acc.ensure(18);
+          HIP_OK(hipMemset(sweep_acc.p, 0, 18 * sizeof(double)));
         }
-        const std::string tag = std::to_string(mode) + (fp32 ? "_f32" : "");
-        prof.count("sweeps_jacobi" + tag, (long)hs.size(), sum);
-        if (bad) prof.count("unconverged_jacobi" + tag, bad, (double)bad);
+        hipLaunchKernelGGL(k_sweep_acc, dim3(1), dim3(256), 0, s, (const int*)stat, (int)js.size(),
+                           sweep_acc.p + 3 * (mode * 2 + (fp32 ? 1 : 0)));
       }
     }
     if (side) {
@@ -545,6 +539,19 @@ struct Engine {
   }
   DBuf<EigJob> ejb[3];
   DBuf<int> jmol;
+  DBuf<double> sweep_acc;  // per Jacobi mode and precision: sweeps, solves, unconverged solves
+  // the device-side sweep statistics into the profile's counters
+  void drain_sweeps() {
+    if (!sweep_acc.p) return;
+    double h[18];
+    HIP_OK(hipMemcpy(h, sweep_acc.p, sizeof h, hipMemcpyDeviceToHost));
+    for (int t = 0; t < 6; t++) {
+      const std::string tag = std::to_string(t / 2) + (t % 2 ? "_f32" : "");
+      if (h[3 * t + 1] > 0) prof.count("sweeps_jacobi" + tag, (long)h[3 * t + 1], h[3 * t]);
+      if (h[3 * t + 2] > 0) prof.count("unconverged_jacobi" + tag, (long)h[3 * t + 2], h[3 * t + 2]);
+    }
+    HIP_OK(hipMemset(sweep_acc.p, 0, sizeof h));
+  }
   int jacobi_sweeps = 60;  // sweeps of a Jacobi solve before it counts as not converged
 
   // ---------------------------------------------------------------- dense eigensolver (large n)
@@ -2609,6 +2616,7 @@ int axtbhip_profile_report(axtbhip_context* ctx, char* buf, size_t len) {
   if (!ctx || !buf || !len) return 0;
   try {
     std::string out;
+    ctx->eng->drain_sweeps();
     for (auto& kv : ctx->eng->prof.drain()) {
       char line[256];
       snprintf(line, sizeof line, "%s %ld %.6f %.6e\n", kv.first.c_str(), kv.second.n, kv.second.ms, kv.second.work);

@@ -189,3 +189,8 @@ def diis_coefficients(residuals):
     c = np.zeros(r.shape[0])
     lib().oracle_diis_coefficients(r.shape[0], r.shape[1], _p(r), _p(c))
     return c
+
+
+def set_threads(n):
+    """OpenMP threads of the following oracle calls."""
+    lib().oracle_set_threads(int(n))

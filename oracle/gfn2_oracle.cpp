@@ -2549,6 +2549,9 @@ void oracle_diis_coefficients(int m, int n, const double* r, double* c) {
   std::memcpy(c, cc.data(), sizeof(double) * m);
 }
 
+// OpenMP threads of the following calls (the batch entry: molecules at once; one molecule: its loops).
+void oracle_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+
 int oracle_engrad(int nat, const int* z, const double* xyz, double charge, double acc, double etemp, int max_iter,
                   int guess, int want_grad, int dispersion, double* energy, double* terms, double* grad, double* qat,
                   double* qsh, double* dp, double* qp, double* emo, int* iterations) {
