@@ -16,7 +16,7 @@ ERROR_KINDS = {1: "input", 2: "numerical", 3: "scf_not_converged", 4: "unsupport
 TERMS = ("repulsion", "dispersion", "electronic", "isotropic_es", "third_order", "anisotropic", "entropy", "band")
 EXPORTS = ("axtbhip_version", "axtbhip_context_create", "axtbhip_context_destroy", "axtbhip_last_error",
            "axtbhip_basis_size", "axtbhip_gfn2_engrad", "axtbhip_gfn2_engrad_guess", "axtbhip_gfn2_engrad_batch",
-           "axtbhip_gfn2_engrad_batch_guess", "axtbhip_batch_adjoint",
+           "axtbhip_gfn2_engrad_batch_guess", "axtbhip_batch_adjoint", "axtbhip_gfn2_engrad_noalloc",
            "axtbhip_batch_prepare", "axtbhip_batch_prepare_guess", "axtbhip_batch_update",
            "axtbhip_batch_run", "axtbhip_batch_gradient", "axtbhip_batch_results", "axtbhip_profile_enable",
            "axtbhip_profile_report", "axtbhip_synchronize")
@@ -80,6 +80,8 @@ def lib():
                                           ctypes.POINTER(Result)]
         L.axtbhip_gfn2_engrad_guess.argtypes = [ctypes.c_void_p, ctypes.POINTER(Mol), ctypes.POINTER(Opts),
                                                 ctypes.POINTER(Guess), ctypes.POINTER(Result)]
+        L.axtbhip_gfn2_engrad_noalloc.argtypes = [ctypes.c_void_p, ctypes.POINTER(Mol), ctypes.POINTER(Opts),
+                                                  ctypes.POINTER(Result)]
         L.axtbhip_batch_adjoint.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.axtbhip_gfn2_engrad_batch.argtypes = [ctypes.c_void_p, ctypes.POINTER(Mol), ctypes.c_int,
                                                 ctypes.POINTER(Opts), ctypes.POINTER(Result)]
@@ -300,6 +302,14 @@ class Context:
         if r["status"]:
             raise AxtbError(r["status"], self.last_error())
         return r
+
+
+def engrad_noalloc(ctx, numbers, positions, charge=0.0, uhf=None, **kw):
+    """One molecule through axtbhip_gfn2_engrad_noalloc (no device allocation during the call)."""
+    opts = make_opts(**kw)
+    mols, res, keep, outs = ctx._marshal([(numbers, positions, charge, uhf)])
+    rc = lib().axtbhip_gfn2_engrad_noalloc(ctx._h, mols, ctypes.byref(opts), res)
+    return ctx._unmarshal(rc, res, outs, opts, False)[0]
 
 
 def basis_size(numbers):

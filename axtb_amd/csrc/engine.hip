@@ -15,6 +15,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -71,10 +72,15 @@ bool sp_element(int z) {
 // Pinned host staging for the small per-launch uploads (job lists): a copy from pageable memory
 // makes the host wait for the stream to drain, one from pinned memory does not. A ring; when it
 // wraps, the device is synchronised so that no copy still reads the part being reused.
+// the arena of the no-allocation call running on this host thread (null: buffers use hipMalloc)
+struct Arena;
+thread_local Arena* t_arena = nullptr;
+
 struct Staging {
   char* base = nullptr;
   size_t cap = 0, off = 0;
   const void* put(const void* src, size_t bytes) {
+    if (t_arena) return nullptr;  // no host allocation in a no-allocation call: pageable copies
     const size_t need = (bytes + 255) & ~(size_t)255;
     if (!base) {
       cap = (size_t)64 << 20;
@@ -101,16 +107,82 @@ struct Staging {
   }
 };
 
+// Device memory handed out from a fixed region instead of hipMalloc (the no-allocation entry
+// point): a bump allocator over the region; reset() makes every buffer it served forget its memory,
+// so that the next call carves them again from the start.
+struct DBufBase {
+  virtual void forget() = 0;
+  virtual ~DBufBase() {}
+};
+struct Arena {
+  char* base = nullptr;
+  size_t cap = 0, off = 0, high = 0;
+  std::vector<DBufBase*> users;
+  void* carve(size_t bytes, DBufBase* u) {
+    const size_t need = (std::max<size_t>(bytes, 1) + 255) & ~(size_t)255;
+    if (off + need > cap)
+      throw Fail{AXTB_ERR_INPUT, "the calculation needs more device memory than the no-allocation scratch holds (" +
+                                     std::to_string(cap >> 20) + " MiB)"};
+    void* p = base + off;
+    off += need;
+    high = std::max(high, off);
+    users.push_back(u);
+    return p;
+  }
+  void drop(DBufBase* u) { users.erase(std::remove(users.begin(), users.end(), u), users.end()); }
+  void reset() {
+    for (DBufBase* u : users) u->forget();
+    users.clear();
+    off = 0;
+  }
+};
+
 template <class T>
-struct DBuf {
+struct DBuf : DBufBase {
   T* p = nullptr;
   size_t n = 0;
+  Arena* arena = nullptr;  // the region p was carved from (null: hipMalloc)
+  void forget() override {
+    p = nullptr;
+    n = 0;
+    arena = nullptr;
+  }
   void ensure(size_t m) {
     if (m <= n) return;
+    if (t_arena) {
+      if (p && !arena) throw DeviceError("a heap buffer in the no-allocation engine");
+      p = (T*)t_arena->carve(std::max<size_t>(m, 1) * sizeof(T), this);
+      arena = t_arena;
+      n = std::max<size_t>(m, 1);
+      return;
+    }
+    if (p && arena) throw DeviceError("an arena buffer outside a no-allocation call");
     if (p) HIP_OK(hipFree(p));
     p = nullptr;
     HIP_OK(hipMalloc(&p, std::max<size_t>(m, 1) * sizeof(T)));
     n = std::max<size_t>(m, 1);
+  }
+  // grow to `m` elements keeping the first `keep` (copied on stream s)
+  void grow_keep(size_t keep, size_t m, hipStream_t s) {
+    if (m <= n) return;
+    T* old = p;
+    const bool old_heap = p && !arena;
+    p = nullptr;
+    n = 0;
+    arena = nullptr;
+    if (t_arena) {
+      if (old_heap) throw DeviceError("a heap buffer in the no-allocation engine");
+      p = (T*)t_arena->carve(m * sizeof(T), this);
+      arena = t_arena;
+    } else {
+      HIP_OK(hipMalloc(&p, m * sizeof(T)));
+    }
+    n = m;
+    if (keep && old) HIP_OK(hipMemcpyAsync(p, old, keep * sizeof(T), hipMemcpyDeviceToDevice, s));
+    if (old_heap) {
+      HIP_OK(hipStreamSynchronize(s));
+      HIP_OK(hipFree(old));
+    }
   }
   void upload(const std::vector<T>& v, hipStream_t s, Staging* stage = nullptr) {
     ensure(v.size());
@@ -119,7 +191,11 @@ struct DBuf {
     HIP_OK(hipMemcpyAsync(p, src ? src : (const void*)v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
   }
   ~DBuf() {
-    if (p) (void)hipFree(p);
+    if (arena) {
+      arena->drop(this);
+    } else if (p) {
+      (void)hipFree(p);
+    }
   }
 };
 
@@ -274,16 +350,7 @@ struct Engine {
     if (slots <= hist_slots) return;
     const int ns = std::min(cap, std::max(slots, std::max(16, 2 * hist_slots)));
     const size_t NM = mols.size();
-    auto grow = [&](DBuf<double>& d, size_t keep, size_t need) {
-      if (need <= d.n) return;
-      double* p = nullptr;
-      HIP_OK(hipMalloc(&p, need * sizeof(double)));
-      if (keep && d.p) HIP_OK(hipMemcpyAsync(p, d.p, keep * sizeof(double), hipMemcpyDeviceToDevice, st));
-      HIP_OK(hipStreamSynchronize(st));
-      if (d.p) HIP_OK(hipFree(d.p));
-      d.p = p;
-      d.n = need;
-    };
+    auto grow = [&](DBuf<double>& d, size_t keep, size_t need) { d.grow_keep(keep, need, st); };
     auto tri = [](size_t k) { return k * (k + 1) / 2; };
     grow(hdf, (size_t)hist_slots * tot_vec, (size_t)ns * tot_vec);
     grow(hu, (size_t)hist_slots * tot_vec, (size_t)ns * tot_vec);
@@ -2309,9 +2376,39 @@ struct Engine {
 // ================================================================================ C ABI
 using axtb::Engine;
 
+// Device memory of the no-allocation entry point (axtbhip_gfn2_engrad_noalloc): one array per GPU,
+// held until the process ends. Host code never uses the name itself (on the host it is a stand-in,
+// not a device address): each device's address comes from hipGetSymbolAddress at context creation.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
+constexpr int NOALLOC_MAX_NAO = 256;  // the largest basis the no-allocation entry point takes
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[64] = {nullptr};  // per device: the address of `scratch`
+
+// the device address of `scratch` on `device` (that device current), looked up once
+void* scratch_address(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (device < 0 || device >= 64) throw axtb::DeviceError("device index out of range");
+  if (!g_scratch_ptr[device]) {
+    HIP_OK(hipSetDevice(device));
+    void* ptr = nullptr;
+    const hipError_t rc = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (rc != hipSuccess || !ptr)
+      throw axtb::DeviceError(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(rc));
+    g_scratch_ptr[device] = ptr;
+  }
+  return g_scratch_ptr[device];
+}
+}  // namespace
+
 struct axtbhip_context {
   std::unique_ptr<Engine> eng;
   std::string err;
+  // the no-allocation path: its own engine (streams made here), its buffers carved from scratch
+  std::unique_ptr<Engine> eng_na;
+  axtb::Arena arena;
 };
 
 static void fill_result(const axtb::MolOut& r, const axtb_mol& m, axtb_result* out) {
@@ -2351,9 +2448,12 @@ const char* axtbhip_version(void) { return "axtb-hip 0.1.0 (GFN2-xTB, gfx950)"; 
 
 axtbhip_context* axtbhip_context_create(int device, char* err, size_t errlen) {
   try {
-    auto* c = new axtbhip_context;
+    std::unique_ptr<axtbhip_context> c(new axtbhip_context);
     c->eng.reset(new Engine(device));
-    return c;
+    c->eng_na.reset(new Engine(device));
+    c->arena.base = (char*)scratch_address(device);
+    c->arena.cap = NOALLOC_SCRATCH_BYTES;
+    return c.release();
   } catch (std::exception& e) {
     if (err && errlen) snprintf(err, errlen, "%s", e.what());
     return nullptr;
@@ -2445,6 +2545,45 @@ int axtbhip_gfn2_engrad(axtbhip_context* ctx, const axtb_mol* mol, const axtb_op
 int axtbhip_gfn2_engrad_guess(axtbhip_context* ctx, const axtb_mol* mol, const axtb_opts* opts, const axtb_guess* guess,
                               axtb_result* out) {
   return axtbhip_gfn2_engrad_batch_guess(ctx, mol, 1, guess, opts, out);
+}
+
+int axtbhip_gfn2_engrad_noalloc(axtbhip_context* ctx, const axtb_mol* mol, const axtb_opts* opts, axtb_result* out) {
+  if (!ctx || !mol || !opts || !out || mol->nat < 0 || (mol->nat > 0 && (!mol->numbers || !mol->positions)))
+    return AXTB_ERR_INPUT;
+  int nsh = 0, nao = 0;
+  if (axtbhip_basis_size(mol, &nsh, &nao) == 0 && nao > NOALLOC_MAX_NAO) {
+    ctx->err = "the no-allocation entry point takes at most " + std::to_string(NOALLOC_MAX_NAO) +
+               " basis functions (this molecule has " + std::to_string(nao) + ")";
+    out->status = AXTB_ERR_INPUT;
+    return AXTB_ERR_INPUT;
+  }
+  std::vector<axtb::MolIn> in{axtb::MolIn{mol->nat, mol->numbers, mol->positions, mol->charge, mol->uhf}};
+  std::vector<axtb::MolOut> res;
+  // every buffer of this call is carved from the device's scratch; nothing is allocated on the
+  // device or pinned on the host
+  struct Scope {
+    axtb::Arena& a;
+    explicit Scope(axtb::Arena& x) : a(x) {
+      a.reset();
+      axtb::t_arena = &a;
+    }
+    ~Scope() { axtb::t_arena = nullptr; }
+  } scope(ctx->arena);
+  try {
+    ctx->eng_na->prof.on = false;
+    ctx->eng_na->run(in, *opts, res);
+  } catch (axtb::Fail& f) {
+    ctx->err = f.msg;
+    out->status = f.code;
+    return f.code;
+  } catch (std::exception& e) {
+    ctx->err = e.what();
+    out->status = AXTB_ERR_DEVICE;
+    return AXTB_ERR_DEVICE;
+  }
+  fill_result(res[0], *mol, out);
+  ctx->err = res[0].status ? res[0].msg : std::string();
+  return res[0].status;
 }
 
 int axtbhip_batch_prepare_guess(axtbhip_context* ctx, const axtb_mol* mols, int nmol, const axtb_guess* guesses,

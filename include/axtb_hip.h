@@ -126,6 +126,15 @@ int axtbhip_gfn2_engrad_batch_guess(axtbhip_context* ctx, const axtb_mol* mols, 
 int axtbhip_gfn2_engrad_guess(axtbhip_context* ctx, const axtb_mol* mol, const axtb_opts* opts,
                               const axtb_guess* guess, axtb_result* out);
 
+/* axtbhip_gfn2_engrad that allocates nothing: no counterpart in the reference. For callers that
+ * cannot allocate device or pinned host memory during the call (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid it). Every device buffer of the calculation is carved
+ * from one static 64 MiB array per GPU (held until the process ends); molecules of at most 256
+ * basis functions (AXTB_ERR_INPUT beyond, and if a calculation would need more of the array).
+ * All calls on one GPU share that array and the last call's contents stay in it: calls that
+ * overlap on one GPU (from several host threads or contexts) race. */
+int axtbhip_gfn2_engrad_noalloc(axtbhip_context* ctx, const axtb_mol* mol, const axtb_opts* opts, axtb_result* out);
+
 /* A batch kept on the device between calls, for repeated or timed runs: prepare validates and
  * uploads the molecules (host arrays may be released afterwards), run computes on the resident
  * inputs, results downloads them (mols gives the array sizes). No counterpart in the reference:
