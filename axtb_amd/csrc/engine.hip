@@ -53,7 +53,10 @@ void sto_ng(int ng, int n, int l, double zeta, double* alpha, double* coeff) {
       case 3: a = STO_P_ALPHA3[ityp][i]; c = STO_P_COEFF3[ityp][i]; break;
       case 4: a = STO_P_ALPHA4[ityp][i]; c = STO_P_COEFF4[ityp][i]; break;
       case 5: a = STO_P_ALPHA5[ityp][i]; c = STO_P_COEFF5[ityp][i]; break;
-      default: a = STO_P_ALPHA6[ityp][i]; c = STO_P_COEFF6[ityp][i]; break;
+      default:  // six primitives; n = 6 has its own 6s / 6p expansions (basis.rs:56-61)
+        a = n == 6 ? (l == 0 ? STO_P_ALPHA_6S[i] : STO_P_ALPHA_6P[i]) : STO_P_ALPHA6[ityp][i];
+        c = n == 6 ? (l == 0 ? STO_P_COEFF_6S[i] : STO_P_COEFF_6P[i]) : STO_P_COEFF6[ityp][i];
+        break;
     }
     alpha[i] = a * z2;
     double top = 2.0 / PI;

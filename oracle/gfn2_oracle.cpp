@@ -66,7 +66,7 @@ double primitive_norm_tblite(int l, double alpha) {
 
 // basis.rs:20-81 (STO-NG of Stewart, GFN2 tables)
 Cgto slater_to_gauss(int ng, int n, int l, double zeta) {
-  if (n > 5 || n <= l) throw Error(1, "STO-NG: unsupported n,l");
+  if ((n > 5 || n <= l) && !(n == 6 && ng == 6 && l <= 1)) throw Error(1, "STO-NG: unsupported n,l");
   int ityp;
   switch (l) {
     case 0: ityp = n; break;
@@ -87,7 +87,10 @@ Cgto slater_to_gauss(int ng, int n, int l, double zeta) {
       case 3: a = STO_P_ALPHA3[ityp][i]; k = STO_P_COEFF3[ityp][i]; break;
       case 4: a = STO_P_ALPHA4[ityp][i]; k = STO_P_COEFF4[ityp][i]; break;
       case 5: a = STO_P_ALPHA5[ityp][i]; k = STO_P_COEFF5[ityp][i]; break;
-      case 6: a = STO_P_ALPHA6[ityp][i]; k = STO_P_COEFF6[ityp][i]; break;
+      case 6:  // n = 6: the dedicated 6s / 6p expansions (basis.rs:56-61)
+        a = n == 6 ? (l == 0 ? STO_P_ALPHA_6S[i] : STO_P_ALPHA_6P[i]) : STO_P_ALPHA6[ityp][i];
+        k = n == 6 ? (l == 0 ? STO_P_COEFF_6S[i] : STO_P_COEFF_6P[i]) : STO_P_COEFF6[ityp][i];
+        break;
       default: throw Error(1, "STO-NG: bad primitive count");
     }
     c.alpha[i] = a * z2;

@@ -272,3 +272,24 @@ def test_setup_and_first_iteration():
     gi = H.iter1(z, x)
     for k in ("vsh", "vat", "vdp", "vqp", "F", "w", "P", "qsh"):
         assert np.abs(gi[k] - o[k]).max() < 1e-10, k
+
+
+def test_element_cases_on_gpu(ctx):
+    """Every element the engine takes, in axtb's reference-matrix molecules (energies and
+    iteration counts of tests/reference_matrix/results.json, accuracy 1e-6), as one batch; and
+    taxol (113 atoms) against the oracle."""
+    from axtb_amd import _lib
+    cases = json.load(open(os.path.join(GOLDEN, "elements.json")))
+    ids = sorted(cases)
+    mols = [(np.array(cases[i]["numbers"], dtype=np.int32), np.array(cases[i]["positions"]),
+             float(cases[i]["charge"])) for i in ids]
+    res = ctx.engrad_batch(mols, _lib.make_opts(accuracy=1e-6))
+    worst = 0.0
+    for i, r in zip(ids, res):
+        assert r["status"] == 0, (i, r["status"])
+        worst = max(worst, abs(r["energy"] - cases[i]["energy"]))
+        assert abs(r["energy"] - cases[i]["energy"]) <= 1e-8, i
+        assert r["iterations"] == cases[i]["iterations"], i
+    print(f"{len(ids)} element cases: max |dE| vs axtb {worst:.2e}", flush=True)
+    z, x = structure("taxol")
+    check(ctx, z, x, 1e-2)

@@ -271,3 +271,23 @@ def test_engrad_many_matches_single():
         assert abs(e[i] - r["energy"]) < 1e-10
         assert np.abs(g[off:off + len(z)] - r["gradient"]).max() < 1e-10
         off += len(z)
+
+
+ELEMENTS = json.load(open(os.path.join(GOLDEN, "elements.json")))
+
+
+@pytest.mark.parametrize("cid", sorted(ELEMENTS))
+def test_element_cases(cid):
+    """Every element the GPU engine takes (s and p shells only), in axtb's reference-matrix
+    molecules (tests/reference_matrix/results.json, accuracy 1e-6): energy, iterations, charges,
+    dipole. Pins the parameter tables of these elements, which the engine shares with the oracle."""
+    c = ELEMENTS[cid]
+    z = np.array(c["numbers"], dtype=np.int32)
+    x = np.array(c["positions"])
+    r = oracle.engrad(z, x, charge=float(c["charge"]), acc=c["accuracy"], gradient=False)
+    assert abs(r["energy"] - c["energy"]) < 1e-8, (r["energy"], c["energy"])
+    assert r["iterations"] == c["iterations"]
+    if c["charges"] is not None:
+        assert np.abs(r["qat"] - np.array(c["charges"])).max() < 1e-7
+    if c["dipole"] is not None:
+        assert np.abs(r["dipole"] - np.array(c["dipole"])).max() < 1e-6
