@@ -37,6 +37,19 @@ def gemm(A, B, ta=False, tb=False, dk=None):
     return C
 
 
+def gemm_upd32(U, B):
+    """U + U (B - I) with the product in FP32 on the device."""
+    Uf = np.asfortranarray(U, dtype=np.float64)
+    Bf = np.asfortranarray(B, dtype=np.float64)
+    m, n = Uf.shape
+    C = np.zeros((m, n), order="F")
+    L = lib()
+    L.axtbhip_test_gemm_upd32.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                          ctypes.c_int, ctypes.c_void_p, ctypes.c_int]
+    _rc(L.axtbhip_test_gemm_upd32(m, n, _p(Uf), m, _p(Bf), n, _p(C), m))
+    return C
+
+
 def eigh(A):
     """Eigenpairs of symmetric matrices A [nb, n, n] (or one [n, n]) by the device Jacobi solver."""
     A = np.asarray(A, dtype=np.float64)

@@ -387,6 +387,7 @@ struct Engine {
     if (const char* v = std::getenv("AXTB_HIP_SPLIT_FROM")) split_from = std::max(2, std::atoi(v));
     if (const char* v = std::getenv("AXTB_HIP_DENSE_REFINE")) dense_refine = std::atoi(v) != 0;
     if (const char* v = std::getenv("AXTB_HIP_GEMM_SMALL")) gemm_small = std::atoi(v) != 0;
+    if (const char* v = std::getenv("AXTB_HIP_UPD32")) upd32 = std::atoi(v) != 0;
     if (const char* v = std::getenv("AXTB_HIP_EXACT_LEVELS")) exact_levels = std::atoi(v) != 0;
     if (const char* v = std::getenv("AXTB_HIP_DENSE_ACCEPT")) dense_accept = std::min(1e-3, std::max(1e-12, std::atof(v)));
     // a lower limit makes Jacobi solves end unconverged (the error path's test)
@@ -413,6 +414,29 @@ struct Engine {
     jobs.erase(std::remove_if(jobs.begin(), jobs.end(), [](const GemmJob& j) { return j.m <= 0 || j.n <= 0; }),
                jobs.end());
     if (jobs.empty()) return;
+    const bool any32 = std::any_of(jobs.begin(), jobs.end(), [](const GemmJob& j) { return j.f32upd != 0; });
+    if (any32 && !std::all_of(jobs.begin(), jobs.end(), [](const GemmJob& j) { return j.f32upd != 0; })) {
+      std::vector<GemmJob> j32, j64;
+      for (auto& j : jobs) (j.f32upd ? j32 : j64).push_back(j);
+      gemm(j32);
+      gemm(j64);
+      return;
+    }
+    if (any32) {  // refinement updates C = A + A (B - I), the product in FP32
+      int mm = 0, nn = 0;
+      double flops = 0.0;
+      for (auto& j : jobs) {
+        if (!j.f32upd || j.ta || j.tb || j.k != j.n || j.m < 1 || j.A2 || j.dk)
+          throw DeviceError("gemm: an FP32 update is C = A + A (B - I) with square B");
+        mm = std::max(mm, j.m);
+        nn = std::max(nn, j.n);
+        flops += 2.0 * j.m * j.n * (double)j.k;
+      }
+      gj.upload(jobs, st, &stage);
+      LAUNCHW(flops, k_gemm128_upd32, dim3(blocks(nn, BT), blocks(mm, BT), (unsigned)jobs.size()), dim3(256), 0, gj.p);
+      HIP_OK(hipGetLastError());
+      return;
+    }
     int mm = 0, nn = 0;
     bool same = true, two = false;
     for (auto& j : jobs) {
@@ -1709,6 +1733,7 @@ struct Engine {
           gsg.push_back(GemmJob{n, n, n, 1, 0, U, U, Vw.p + o0, nullptr, n, n, n, 1.0, 0.0});         // G
           gsg[gsg.size() - 2].lower = gsg.back().lower = sym_lower(n, false);  // both symmetric
           gu.push_back(GemmJob{n, n, n, 0, 0, U, Wm.p + o0, out, nullptr, n, n, n, 1.0, 0.0});        // U'
+          gu.back().f32upd = (upd32 && n > 128) ? 1 : 0;  // small products stay on the FP64 small-matrix kernels
           oj.push_back(EigJob{n, F.p + o0, Vw.p + o0, j.w, Wm.p + o0});
         }
         oa_jobs.upload(oj, st, &stage);
@@ -1776,6 +1801,7 @@ struct Engine {
       LAUNCH(k_oa_big_e, dim3(nb), dim3(256), 0, oj, (const double*)oab_part.p, nb, oab_delta.p, step == 0 ? 1 : 0,
              oab_emax.p, cls, dense_ovmix);
       std::vector<GemmJob> gu{GemmJob{n, n, n, 0, 0, U, Wm.p + o0, out, nullptr, n, n, n, 1.0, 0.0}};
+      gu[0].f32upd = upd32 ? 1 : 0;
       gemm(gu);
       step++;
       std::vector<double> h(nb);
@@ -1819,6 +1845,8 @@ struct Engine {
   // quotients of the refined vectors
   bool dense_refine = true;
   bool gemm_small = true;      // products with m, n <= 128 on the one-workgroup-per-job kernel (AXTB_HIP_GEMM_SMALL)
+  // mixed: the refinement updates U (I + E) as U + U E with U E on the FP32 matrix cores (AXTB_HIP_UPD32)
+  bool upd32 = true;
   bool in_refine = false;      // the refinement's batches always take the 128-tile kernel (measured faster)
   int oa_steps = 2;  // refinement steps per SCC iteration (AXTB_HIP_OA_STEPS)
   bool jacobi_wide = true;  // 1024-thread Jacobi workgroups when one fills a CU (AXTB_HIP_JACOBI_WIDE)
@@ -2792,6 +2820,25 @@ int axtbhip_test_gemm(int m, int n, int k, int ta, int tb, const double* A, int 
     std::vector<GemmJob> j{GemmJob{m, n, k, ta, tb, a.p, b.p, c.p, dk ? d.p : nullptr, lda, ldb, ldc, 1.0, 0.0}};
     e.gemm(j);
     HIP_OK(hipMemcpyAsync(C, c.p, nc * sizeof(double), hipMemcpyDeviceToHost, e.st));
+    HIP_OK(hipStreamSynchronize(e.st));
+    return 0;
+  } catch (std::exception&) {
+    return AXTB_ERR_DEVICE;
+  }
+}
+
+int axtbhip_test_gemm_upd32(int m, int n, const double* U, int ldu, const double* B, int ldb, double* C, int ldc) {
+  try {
+    using namespace axtb;
+    Engine e(0);
+    DBuf<double> u, b, c;
+    u.upload(std::vector<double>(U, U + (size_t)ldu * n), e.st);
+    b.upload(std::vector<double>(B, B + (size_t)ldb * n), e.st);
+    c.ensure((size_t)ldc * n);
+    std::vector<GemmJob> j{GemmJob{m, n, n, 0, 0, u.p, b.p, c.p, nullptr, ldu, ldb, ldc, 1.0, 0.0}};
+    j[0].f32upd = 1;
+    e.gemm(j);
+    HIP_OK(hipMemcpyAsync(C, c.p, (size_t)ldc * n * sizeof(double), hipMemcpyDeviceToHost, e.st));
     HIP_OK(hipStreamSynchronize(e.st));
     return 0;
   } catch (std::exception&) {

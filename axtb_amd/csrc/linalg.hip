@@ -22,6 +22,9 @@ struct GemmJob {
   // 128-tile kernel only: compute only the tiles that hold elements of the lower triangle (row >=
   // column) of C
   int lower = 0;
+  // the refinement update C = A + A (B - I) with the product on the FP32 matrix cores (A, B square
+  // n x n, no transposes; k_gemm128_upd32)
+  int f32upd = 0;
 };
 
 typedef double d4v __attribute__((ext_vector_type(4)));
@@ -282,6 +285,91 @@ __global__ __launch_bounds__(256, 2) void k_gemm128(const GemmJob* jobs) {
           const double v = J.alpha * acc[i][j][r];
           *c = J.beta == 0.0 ? v : v + J.beta * *c;
         }
+      }
+}
+
+// The update of a refinement step, C = U + U E with E = B - I (B holds I + E): the product U E,
+// whose entries are the size of the correction, runs in FP32 (operands rounded to FP32 when they
+// are staged, FP32 accumulation on the matrix cores, v_mfma_f32_16x16x4f32); the sum with U is
+// FP64. The rounding of the product is |E| 2^-24 sqrt(n) relative to U and is itself corrected by
+// the next refinement step, whose residuals (U^T A U, U^T U) stay FP64. 128 x 128 tiles, 4
+// wavefronts of 64 x 64, double-buffered LDS as in k_gemm128.
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256, 2) void k_gemm128_upd32(const GemmJob* jobs) {
+  const GemmJob J = jobs[blockIdx.z];
+  const int row0 = blockIdx.y * BT, col0 = blockIdx.x * BT;
+  if (row0 >= J.m || col0 >= J.n) return;
+  __shared__ float As[2][BK][BP];
+  __shared__ float Bs[2][BK][BP];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wr = (w & 1) * 64, wc = (w >> 1) * 64;
+  double ra[8], rb[8];
+  unsigned ma = 0u, mb = 0u;
+  // element q of this thread's share of a k-step: A (no transpose) row r = tid & 127, k = (tid >> 7) * 8 + q;
+  // B (no transpose) k = tid & 15, column c = (tid >> 4) + 16 q
+  auto load = [&](int k0) {
+    ma = mb = 0u;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int r = tid & 127, k = (tid >> 7) * 8 + q;
+      const int gr = row0 + r, gk = k0 + k;
+      ra[q] = gload(J.A + min(gr, J.m - 1) + (long long)min(gk, J.k - 1) * J.lda);
+      ma |= (gr < J.m && gk < J.k) ? 1u << q : 0u;
+      const int kb = tid & 15, c = (tid >> 4) + 16 * q;
+      const int gc = col0 + c, gkb = k0 + kb;
+      rb[q] = gload(J.B + min(gkb, J.k - 1) + (long long)min(gc, J.n - 1) * J.ldb) - (gkb == gc ? 1.0 : 0.0);
+      mb |= (gc < J.n && gkb < J.k) ? 1u << q : 0u;
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      As[buf][(tid >> 7) * 8 + q][tid & 127] = (ma >> q) & 1u ? (float)ra[q] : 0.0f;
+      Bs[buf][tid & 15][(tid >> 4) + 16 * q] = (mb >> q) & 1u ? (float)rb[q] : 0.0f;
+    }
+  };
+  f4v acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[i][j] = (f4v){0.0f, 0.0f, 0.0f, 0.0f};
+  const int nk = (J.k + BK - 1) / BK;
+  if (nk > 0) {
+    load(0);
+    store(0);
+  }
+  __syncthreads();
+  for (int kt = 0; kt < nk; kt++) {
+    const int buf = kt & 1;
+    if (kt + 1 < nk) load((kt + 1) * BK);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      const int kl = kk + (lane >> 4), li = lane & 15;
+      float a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        a[i] = As[buf][kl][wr + 16 * i + li];
+        b[i] = Bs[buf][kl][wc + 16 * i + li];
+      }
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+    if (kt + 1 < nk) store(buf ^ 1);
+    __syncthreads();
+  }
+  // lane l, element r of a 16 x 16 FP32 block holds row 4 (l / 16) + r, column l % 16
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int gr = row0 + wr + 16 * i + 4 * (lane >> 4) + r;
+        const int gc = col0 + wc + 16 * j + (lane & 15);
+        if (gr < J.m && gc < J.n) J.C[gr + (long long)gc * J.ldc] = J.A[gr + (long long)gc * J.lda] + (double)acc[i][j][r];
       }
 }
 

@@ -215,6 +215,23 @@ def test_gemm():
         assert np.abs(C - R).max() < 1e-12 * k
 
 
+def test_gemm_upd32():
+    """The refinement update C = U + U E (B = I + E) with U E on the FP32 matrix cores: within the
+    rounding bound of an FP32 product (operands rounded to FP32, FP32 accumulation)."""
+    from axtb_amd import _testhooks as H
+    rng = np.random.default_rng(3)
+    for m, n in [(5, 5), (64, 64), (120, 120), (130, 130), (300, 300), (200, 257)]:
+        U = rng.standard_normal((m, n))
+        E = 1e-3 * rng.standard_normal((n, n))
+        C = H.gemm_upd32(U, np.eye(n) + E)
+        R = U + U @ E
+        # the FP32 product's rounding bound |fl(UE) - UE| <= (n + 2) 2^-24 |U| |E|, plus the FP64 sum
+        bound = (n + 2) * 2.0 ** -24 * (np.abs(U) @ np.abs(E)) + 2.0 ** -52 * np.abs(R)
+        ratio = (np.abs(C - R) / bound).max()
+        print(f"upd32 {m}x{n}: max error / rounding bound {ratio:.2e}", flush=True)
+        assert ratio <= 1.0
+
+
 def test_eigh():
     from axtb_amd import _testhooks as H
     rng = np.random.default_rng(2)
