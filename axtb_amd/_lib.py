@@ -17,6 +17,7 @@ TERMS = ("repulsion", "dispersion", "electronic", "isotropic_es", "third_order",
 EXPORTS = ("axtbhip_version", "axtbhip_context_create", "axtbhip_context_destroy", "axtbhip_last_error",
            "axtbhip_basis_size", "axtbhip_gfn2_engrad", "axtbhip_gfn2_engrad_guess", "axtbhip_gfn2_engrad_batch",
            "axtbhip_gfn2_engrad_batch_guess", "axtbhip_batch_adjoint", "axtbhip_gfn2_engrad_noalloc",
+           "axtbhip_gfn2_hessian",
            "axtbhip_batch_prepare", "axtbhip_batch_prepare_guess", "axtbhip_batch_update",
            "axtbhip_batch_run", "axtbhip_batch_gradient", "axtbhip_batch_results", "axtbhip_profile_enable",
            "axtbhip_profile_report", "axtbhip_synchronize")
@@ -48,6 +49,12 @@ class Opts(ctypes.Structure):
 
 MIXERS = {"auto": 0, "broyden": 1, "diis": 2}
 MIXER_USED = {0: "broyden", 1: "diis", 2: "diis-fallback"}
+
+
+class HessianOut(ctypes.Structure):
+    _fields_ = [("energy", ctypes.c_double), ("gradient", ctypes.c_void_p), ("hessian", ctypes.c_void_p),
+                ("dipole_derivatives", ctypes.c_void_p), ("max_asymmetry", ctypes.c_double),
+                ("displacements", ctypes.c_int), ("status", ctypes.c_int)]
 
 
 class Result(ctypes.Structure):
@@ -82,6 +89,9 @@ def lib():
                                                 ctypes.POINTER(Guess), ctypes.POINTER(Result)]
         L.axtbhip_gfn2_engrad_noalloc.argtypes = [ctypes.c_void_p, ctypes.POINTER(Mol), ctypes.POINTER(Opts),
                                                   ctypes.POINTER(Result)]
+        L.axtbhip_gfn2_hessian.argtypes = [ctypes.c_void_p, ctypes.POINTER(Mol), ctypes.POINTER(Opts),
+                                           ctypes.POINTER(Guess), ctypes.c_double, ctypes.c_double,
+                                           ctypes.POINTER(HessianOut)]
         L.axtbhip_batch_adjoint.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.axtbhip_gfn2_engrad_batch.argtypes = [ctypes.c_void_p, ctypes.POINTER(Mol), ctypes.c_int,
                                                 ctypes.POINTER(Opts), ctypes.POINTER(Result)]
@@ -290,6 +300,24 @@ class Context:
                     d["gradient"] = None
             results.append(d)
         return results
+
+    def hessian(self, numbers, positions, charge=0.0, uhf=None, start=None, step=0.005, hessian_accuracy=0.01, **kw):
+        """Semi-numerical Hessian (axtbhip_gfn2_hessian): energy, gradient [N,3], hessian [3N,3N],
+        dipole_derivatives [3N,3], max_asymmetry."""
+        opts = make_opts(**kw)
+        mols, res, keep, outs = self._marshal([(numbers, positions, charge, uhf)])
+        gs = self._starts(mols, [start], keep) if start is not None else None
+        nat = mols[0].nat
+        g = np.zeros((nat, 3))
+        H = np.zeros((3 * nat, 3 * nat))
+        dd = np.zeros((3 * nat, 3))
+        out = HessianOut(0.0, g.ctypes.data, H.ctypes.data, dd.ctypes.data, 0.0, 0, 0)
+        rc = lib().axtbhip_gfn2_hessian(self._h, mols, ctypes.byref(opts), gs, float(step), float(hessian_accuracy),
+                                        ctypes.byref(out))
+        if rc:
+            raise AxtbError(rc, self.last_error())
+        return {"energy": out.energy, "gradient": g, "hessian": H, "dipole_derivatives": dd,
+                "max_asymmetry": out.max_asymmetry, "displacements": out.displacements}
 
     def engrad(self, numbers, positions, charge=0.0, uhf=None, start=None, **kw):
         """One molecule (axtbhip_gfn2_engrad_guess); start: None or a given SCF start (a dict with

@@ -2442,6 +2442,25 @@ struct axtbhip_context {
   axtb::Arena arena;
 };
 
+// A batch of single points; a molecule that started from given moments and did not converge runs
+// once more from the fresh start (RetryLadder::warm_then_fresh, crates/base/src/method/retry.rs:150-158).
+static void run_warm_then_fresh(Engine& eng, const std::vector<axtb::MolIn>& in, const axtb_opts& o,
+                                std::vector<axtb::MolOut>& res) {
+  eng.run(in, o, res);
+  std::vector<int> again;
+  for (size_t i = 0; i < in.size(); i++)
+    if (in[i].gq && res[i].status == AXTB_ERR_SCF_NOT_CONVERGED) again.push_back((int)i);
+  if (again.empty()) return;
+  std::vector<axtb::MolIn> in2;
+  for (int i : again) {
+    in2.push_back(in[i]);
+    in2.back().gq = in2.back().gdp = in2.back().gqp = nullptr;
+  }
+  std::vector<axtb::MolOut> res2;
+  eng.run(in2, o, res2);
+  for (size_t k = 0; k < again.size(); k++) res[again[k]] = res2[k];
+}
+
 static void fill_result(const axtb::MolOut& r, const axtb_mol& m, axtb_result* out) {
   out->status = r.status;
   out->energy = r.energy;
@@ -2528,21 +2547,7 @@ int axtbhip_gfn2_engrad_batch_guess(axtbhip_context* ctx, const axtb_mol* mols, 
   }
   std::vector<axtb::MolOut> res;
   try {
-    ctx->eng->run(in, *opts, res);
-    // the warm start that did not converge: once more from the fresh start
-    std::vector<int> again;
-    for (int i = 0; i < nmol; i++)
-      if (in[i].gq && res[i].status == AXTB_ERR_SCF_NOT_CONVERGED) again.push_back(i);
-    if (!again.empty()) {
-      std::vector<axtb::MolIn> in2;
-      for (int i : again) {
-        in2.push_back(in[i]);
-        in2.back().gq = in2.back().gdp = in2.back().gqp = nullptr;
-      }
-      std::vector<axtb::MolOut> res2;
-      ctx->eng->run(in2, *opts, res2);
-      for (size_t k = 0; k < again.size(); k++) res[again[k]] = res2[k];
-    }
+    run_warm_then_fresh(*ctx->eng, in, *opts, res);
   } catch (axtb::Fail& f) {
     ctx->err = f.msg;
     for (int i = 0; i < nmol; i++) outs[i].status = f.code;
@@ -2576,6 +2581,93 @@ int axtbhip_gfn2_engrad(axtbhip_context* ctx, const axtb_mol* mol, const axtb_op
 int axtbhip_gfn2_engrad_guess(axtbhip_context* ctx, const axtb_mol* mol, const axtb_opts* opts, const axtb_guess* guess,
                               axtb_result* out) {
   return axtbhip_gfn2_engrad_batch_guess(ctx, mol, 1, guess, opts, out);
+}
+
+int axtbhip_gfn2_hessian(axtbhip_context* ctx, const axtb_mol* mol, const axtb_opts* opts, const axtb_guess* guess,
+                         double step, double accuracy, axtb_hessian* out) {
+  if (!ctx || !mol || !opts || !out || mol->nat < 1 || !mol->numbers || !mol->positions) return AXTB_ERR_INPUT;
+  out->status = AXTB_ERR_INPUT;
+  if (!(step > 0.0 && std::isfinite(step))) {
+    ctx->err = "the Hessian step must be a positive number";
+    return AXTB_ERR_INPUT;
+  }
+  if (!(accuracy > 0.0 && std::isfinite(accuracy))) {
+    ctx->err = "the Hessian's SCF accuracy must be a positive number";
+    return AXTB_ERR_INPUT;
+  }
+  const int nat = mol->nat, n3 = 3 * nat;
+  axtb_opts o = *opts;
+  o.accuracy = std::min(accuracy, opts->accuracy);  // never looser than the calculator's
+  o.want_gradient = 1;
+  try {
+    // the reference geometry, from the given start
+    axtb::MolIn m0{nat, mol->numbers, mol->positions, mol->charge, mol->uhf};
+    if (guess && guess->shell_charges) {
+      m0.gq = guess->shell_charges;
+      m0.gdp = guess->dipoles;
+      m0.gqp = guess->quadrupoles;
+    }
+    std::vector<axtb::MolOut> r0;
+    run_warm_then_fresh(*ctx->eng, {m0}, o, r0);
+    if (r0[0].status) {
+      ctx->err = r0[0].msg;
+      out->status = r0[0].status;
+      return r0[0].status;
+    }
+    // every coordinate forward by the step and back by twice the step (fd.rs:142-150, ForwardBack),
+    // all 6 N displaced molecules in one batch, each started from the reference's moments
+    std::vector<double> xyz((size_t)2 * n3 * n3);
+    std::vector<axtb::MolIn> in(2 * n3);
+    for (int y = 0; y < n3; y++)
+      for (int sgn = 0; sgn < 2; sgn++) {
+        double* x = &xyz[(size_t)(2 * y + sgn) * n3];
+        std::copy(mol->positions, mol->positions + n3, x);
+        x[y] = mol->positions[y] + step;
+        if (sgn) x[y] = x[y] - 2.0 * step;
+        in[2 * y + sgn] = axtb::MolIn{nat, mol->numbers, x, mol->charge, mol->uhf, r0[0].qsh.data(), r0[0].dp.data(),
+                                      r0[0].qp.data()};
+      }
+    std::vector<axtb::MolOut> rd;
+    run_warm_then_fresh(*ctx->eng, in, o, rd);
+    for (int q = 0; q < 2 * n3; q++)
+      if (rd[q].status) {  // the first failed displacement in coordinate order (fd.rs:170)
+        ctx->err = "displacement " + std::to_string(q) + ": " + rd[q].msg;
+        out->status = rd[q].status;
+        return rd[q].status;
+      }
+    // column y = (g(x + h e_y) - g(x - h e_y)) / 2h, written as row y; then the symmetric part
+    std::vector<double> H((size_t)n3 * n3);
+    for (int y = 0; y < n3; y++) {
+      const axtb::MolOut &p = rd[2 * y], &m = rd[2 * y + 1];
+      for (int r = 0; r < n3; r++) H[(size_t)y * n3 + r] = (p.grad[r] - m.grad[r]) / (2.0 * step);
+      if (out->dipole_derivatives)
+        for (int k = 0; k < 3; k++) out->dipole_derivatives[3 * y + k] = (p.dipole[k] - m.dipole[k]) / (2.0 * step);
+    }
+    double asym = 0.0;  // fd.rs:128-140
+    for (int i = 0; i < n3; i++)
+      for (int j = 0; j < i; j++) {
+        const double a = H[(size_t)i * n3 + j], b = H[(size_t)j * n3 + i];
+        asym = std::max(asym, std::fabs(a - b));
+        const double v = 0.5 * (a + b);
+        H[(size_t)i * n3 + j] = H[(size_t)j * n3 + i] = v;
+      }
+    out->energy = r0[0].energy;
+    if (out->gradient) std::memcpy(out->gradient, r0[0].grad.data(), sizeof(double) * n3);
+    if (out->hessian) std::memcpy(out->hessian, H.data(), sizeof(double) * H.size());
+    out->max_asymmetry = asym;
+    out->displacements = 2 * n3;
+    out->status = 0;
+    ctx->err.clear();
+    return 0;
+  } catch (axtb::Fail& f) {
+    ctx->err = f.msg;
+    out->status = f.code;
+    return f.code;
+  } catch (std::exception& e) {
+    ctx->err = e.what();
+    out->status = AXTB_ERR_DEVICE;
+    return AXTB_ERR_DEVICE;
+  }
 }
 
 int axtbhip_gfn2_engrad_noalloc(axtbhip_context* ctx, const axtb_mol* mol, const axtb_opts* opts, axtb_result* out) {

@@ -126,6 +126,30 @@ int axtbhip_gfn2_engrad_batch_guess(axtbhip_context* ctx, const axtb_mol* mols, 
 int axtbhip_gfn2_engrad_guess(axtbhip_context* ctx, const axtb_mol* mol, const axtb_opts* opts,
                               const axtb_guess* guess, axtb_result* out);
 
+/* A semi-numerical Hessian (HessianMethod::Numerical of axtb's GFN2 calculator,
+ * crates/axtb/src/calculator.rs:266-296, over crates/base/src/fd.rs:163-209): the reference
+ * geometry's energy and gradient (from `guess` if given), then every Cartesian coordinate moved
+ * forward by `step` bohr and back by twice the step, the 6 N displaced gradients computed as one
+ * GPU batch, each SCF started from the reference's moments (a displacement that does not converge
+ * from them runs once more from the fresh start) at SCF accuracy min(accuracy, opts->accuracy).
+ * Column y of the Hessian is the central difference of the gradients, the matrix is symmetrized
+ * (fd.rs:128-140). The reference's defaults: step 0.005 bohr, accuracy 0.01
+ * (calculator.rs:81-86). A non-positive or non-finite step or accuracy is AXTB_ERR_INPUT; a failed
+ * displacement returns its status (the first in coordinate order). */
+#define AXTB_HESSIAN_STEP 0.005
+#define AXTB_HESSIAN_ACCURACY 0.01
+typedef struct axtb_hessian {
+  double energy;               /* at the reference geometry */
+  double* gradient;            /* [nat][3] or NULL */
+  double* hessian;             /* [3 nat][3 nat] row-major, Eh/bohr^2, or NULL */
+  double* dipole_derivatives;  /* d mu / d R_x, [3 nat][3] (e), or NULL */
+  double max_asymmetry;        /* largest |H_ij - H_ji| before the symmetrization */
+  int displacements;           /* displaced single points computed (6 nat) */
+  int status;
+} axtb_hessian;
+int axtbhip_gfn2_hessian(axtbhip_context* ctx, const axtb_mol* mol, const axtb_opts* opts, const axtb_guess* guess,
+                         double step, double accuracy, axtb_hessian* out);
+
 /* axtbhip_gfn2_engrad that allocates nothing: no counterpart in the reference. For callers that
  * cannot allocate device or pinned host memory during the call (code under HIP graph capture,
  * tight serving loops, harnesses that forbid it). Every device buffer of the calculation is carved
