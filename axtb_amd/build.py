@@ -1,7 +1,8 @@
 """Build of the HIP engine: ``libaxtb_hip.so`` in this package directory, for gfx950.
 
-One translation unit (csrc/engine.hip includes the kernels) compiled by hipcc. Packed FP32
-instructions are disabled for all device code (``-packed-fp32-ops``).
+One translation unit (csrc/engine.hip includes the kernels) compiled by hipcc and linked with
+rocBLAS (large plain FP64 products). Packed FP32 instructions are disabled for all device code
+(``-packed-fp32-ops``).
 """
 import os
 import subprocess
@@ -16,7 +17,7 @@ FLAGS = ["-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC", "-shared",
 def build_engine(verbose=False):
     out = os.path.join(HERE, "libaxtb_hip.so")
     tmp = out + ".tmp"
-    cmd = [HIPCC, *FLAGS, os.path.join(CSRC, "engine.hip"), "-o", tmp]
+    cmd = [HIPCC, *FLAGS, os.path.join(CSRC, "engine.hip"), "-o", tmp, "-lrocblas"]
     r = subprocess.run(cmd, cwd=CSRC, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed:\n" + r.stdout + r.stderr)

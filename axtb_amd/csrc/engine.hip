@@ -19,6 +19,8 @@
 #include <string>
 #include <vector>
 
+#include <rocblas/rocblas.h>
+
 #include "../../include/axtb_hip.h"
 #include "gfn2_tables.h"
 #include "kernels.hip"
@@ -388,6 +390,8 @@ struct Engine {
     if (const char* v = std::getenv("AXTB_HIP_DENSE_REFINE")) dense_refine = std::atoi(v) != 0;
     if (const char* v = std::getenv("AXTB_HIP_GEMM_SMALL")) gemm_small = std::atoi(v) != 0;
     if (const char* v = std::getenv("AXTB_HIP_UPD32")) upd32 = std::atoi(v) != 0;
+    if (const char* v = std::getenv("AXTB_HIP_ROCBLAS")) use_rocblas = std::atoi(v) != 0;
+    if (const char* v = std::getenv("AXTB_HIP_ROCBLAS_MIN")) rocblas_min = std::max(1, std::atoi(v));
     if (const char* v = std::getenv("AXTB_HIP_EXACT_LEVELS")) exact_levels = std::atoi(v) != 0;
     if (const char* v = std::getenv("AXTB_HIP_DENSE_ACCEPT")) dense_accept = std::min(1e-3, std::max(1e-12, std::atof(v)));
     // a lower limit makes Jacobi solves end unconverged (the error path's test)
@@ -404,6 +408,7 @@ struct Engine {
     dense_vectors(A, n, dtau1.p, Q, n, Z);
   }
   ~Engine() {
+    if (blas) (void)rocblas_destroy_handle(blas);
     if (st2) (void)hipStreamDestroy(st2);
     if (st) (void)hipStreamDestroy(st);
     if (ev_fork) (void)hipEventDestroy(ev_fork);
@@ -421,6 +426,26 @@ struct Engine {
       gemm(j32);
       gemm(j64);
       return;
+    }
+    // one large plain product (no inner scaling, no second term, the full C): rocBLAS's dgemm, which
+    // runs at a higher fraction of the FP64 matrix peak than the 128-tile kernel at these sizes (not in
+    // a no-allocation call: the library keeps its own workspace)
+    if (!any32 && use_rocblas && !t_arena && jobs.size() == 1) {
+      const GemmJob& J = jobs[0];
+      if (!J.dk && !J.A2 && !J.lower && J.m >= rocblas_min && J.n >= rocblas_min && J.k >= rocblas_min) {
+        if (!blas) {
+          if (rocblas_create_handle(&blas) != rocblas_status_success) throw DeviceError("rocblas_create_handle failed");
+        }
+        if (rocblas_set_stream(blas, st) != rocblas_status_success) throw DeviceError("rocblas_set_stream failed");
+        const double al = J.alpha, be = J.beta;
+        ProfScope ps_(&prof, st, "rocblas_dgemm", 2.0 * J.m * J.n * (double)J.k);
+        const rocblas_status rs =
+            rocblas_dgemm(blas, J.ta ? rocblas_operation_transpose : rocblas_operation_none,
+                          J.tb ? rocblas_operation_transpose : rocblas_operation_none, J.m, J.n, J.k, &al, J.A,
+                          (rocblas_int)J.lda, J.B, (rocblas_int)J.ldb, &be, J.C, (rocblas_int)J.ldc);
+        if (rs != rocblas_status_success) throw DeviceError("rocblas_dgemm failed: " + std::to_string((int)rs));
+        return;
+      }
     }
     if (any32) {  // refinement updates C = A + A (B - I), the product in FP32
       int mm = 0, nn = 0;
@@ -1466,7 +1491,15 @@ struct Engine {
                        dgw.p, atmP.p);
     if (o.dispersion) {
       atmGD.ensure(2 * (size_t)tot_pair);
-      LAUNCH(k_atm_pairsum, dim3(blocks(NA, 4)), dim3(256), 0, b, (const double*)atmP.p, atmGD.p);
+      std::vector<int4> tl;  // the 16 x 16 tiles (I <= J) of every molecule's atom pairs
+      for (int m = 0; m < NM; m++) {
+        const int nt = (mols[m].nat + ATM_T - 1) / ATM_T;
+        for (int I = 0; I < nt; I++)
+          for (int J = I; J < nt; J++) tl.push_back(make_int4(m, I, J, 0));
+      }
+      atm_tiles.upload(tl, st, &stage);
+      LAUNCH(k_atm_pairsum_tiled, dim3((unsigned)tl.size()), dim3(256), 0, b, (const int4*)atm_tiles.p,
+             (const double*)atmP.p, atmGD.p);
       LAUNCH(k_atm, dim3(blocks(NA, 4)), dim3(256), 0, b, (const double*)atmP.p, (const double*)atmGD.p, e3.p, g3.p,
              dcn3.p);
     } else {
@@ -1827,6 +1860,7 @@ struct Engine {
       }
   }
   DBuf<double> atmGD;  // ATM pair sums (2 x the atom-pair buffer)
+  DBuf<int4> atm_tiles;
   DBuf<int> dcls, dnfrac;
   std::vector<int> dense_pfp;  // per molecule: its last eigenproblem was refined (W = P F P / 2)
   int dense_steps = 4;         // mixed, dense: refinement steps at most per SCC iteration
@@ -1847,6 +1881,10 @@ struct Engine {
   bool gemm_small = true;      // products with m, n <= 128 on the one-workgroup-per-job kernel (AXTB_HIP_GEMM_SMALL)
   // mixed: the refinement updates U (I + E) as U + U E with U E on the FP32 matrix cores (AXTB_HIP_UPD32)
   bool upd32 = true;
+  // large plain products on rocBLAS (AXTB_HIP_ROCBLAS, smallest m, n and k: AXTB_HIP_ROCBLAS_MIN)
+  bool use_rocblas = true;
+  int rocblas_min = 512;
+  rocblas_handle blas = nullptr;
   bool in_refine = false;      // the refinement's batches always take the 128-tile kernel (measured faster)
   int oa_steps = 2;  // refinement steps per SCC iteration (AXTB_HIP_OA_STEPS)
   bool jacobi_wide = true;  // 1024-thread Jacobi workgroups when one fills a CU (AXTB_HIP_JACOBI_WIDE)

@@ -567,54 +567,71 @@ __global__ void k_atm_pairs(Batch b, const double* gw, const double* dgw, double
 }
 
 // ATM three-body energy and gradient by pair sums (disp.rs:607-720, axtb's d4_atm_pair_sums), in
-// two passes. k_atm_pairsum: the pair's (g, d) summed over third atoms k for every unordered atom
-// pair (a, b), a < b, written for both orders: for every k,
-// the damped angular term and its derivative along r_ab (the triple sum of the pair is symmetric
-// in a and b, so each pair is summed once). GD[o] = cab * sum_k (derivative part),
-// GD[tot + o] = cab * sum_k (energy part), cab = -5 sqrt|C6_ab|. One wavefront per atom a.
-__global__ void k_atm_pairsum(Batch b, const double* P, double* GD) {
-  int a = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
-  int lane = threadIdx.x & 63;
-  if (a >= b.nat) return;
-  int m = b.a.mol[a];
-  const Mol M = b.mol[m];
-  long long tot = b.mol[b.nmol - 1].pair0 + (long long)b.mol[b.nmol - 1].nat * b.mol[b.nmol - 1].nat;
+// two passes. k_atm_pairsum_tiled: the pair's (g, d) summed over third atoms k for every unordered
+// atom pair (a, b), a < b, written for both orders: for every k, the damped angular term and its
+// derivative along r_ab (the triple sum of the pair is symmetric in a and b, so each pair is summed
+// once). GD[o] = cab * sum_k (derivative part), GD[tot + o] = cab * sum_k (energy part),
+// cab = -5 sqrt|C6_ab|.
+// Over 16 x 16 tiles of atom pairs (tile (I, J), J >= I, of molecule m: int4 {m, I, J, 0}), the
+// third atoms k in LDS-staged chunks of 32: the rows of the tile's 16 + 16 atoms are read once per
+// tile instead of once per pair. One pair (a, b), a < b, per thread.
+constexpr int ATM_T = 16, ATM_K = 32;
+__global__ __launch_bounds__(256) void k_atm_pairsum_tiled(Batch b, const int4* tiles, const double* P, double* GD) {
+  __shared__ double sa[4][ATM_T][ATM_K + 1], sb[4][ATM_T][ATM_K + 1];  // W, T, IR, R2 rows of the tile's atoms
+  const int4 tl = tiles[blockIdx.x];
+  const Mol M = b.mol[tl.x];
+  const long long tot = b.mol[b.nmol - 1].pair0 + (long long)b.mol[b.nmol - 1].nat * b.mol[b.nmol - 1].nat;
   const double *R2 = P, *IR = P + tot, *T = P + 2 * tot, *W = P + 3 * tot, *C6 = P + 4 * tot, *SW = P + 6 * tot;
-  int la = a - M.at0;
-  long long rowa = M.pair0 + (long long)la * M.nat;
-  const double alp3 = 16.0 / 3.0;
-  for (int lb = la + 1; lb < M.nat; lb++) {
-    double swab = SW[rowa + lb];
-    if (!(swab > 0.0)) continue;
-    long long rowb = M.pair0 + (long long)lb * M.nat;
-    double r2 = R2[rowa + lb], ir = IR[rowa + lb], t6 = 6.0 * T[rowa + lb];
-    double r2_2 = r2 * r2;
-    double gs = 0.0, ds = 0.0;
-    for (int lk = lane; lk < M.nat; lk += 64) {
-      double c = W[rowa + lk] * W[rowb + lk];
-      double cralp = t6 * T[rowa + lk] * T[rowb + lk];
-      double fdmp = 1.0 / (1.0 + cralp);
-      double ir1 = ir * IR[rowa + lk] * IR[rowb + lk];
-      double rrr2 = ir1 * ir1, rrr3 = rrr2 * ir1;
-      double angf = 0.375 * rrr2 * rrr3;
-      double r2ak = R2[rowa + lk], r2bk = R2[rowb + lk];
-      double s1 = r2ak + r2bk, d1 = (r2bk - r2ak) * (r2bk - r2ak);
-      double ang = angf * ((r2_2 - d1) * (s1 - r2)) + rrr3;
-      double cf = c * fdmp;
-      ds += ang * cf;
-      double p = r2 * (r2 * (r2 + s1) + 2.0 * (s1 * s1) + d1) - 5.0 * (d1 * s1);
-      gs += cf * (angf * p - alp3 * (cralp * fdmp) * ang);
+  const int tid = threadIdx.x, ta = tid >> 4, tb = tid & 15;
+  const int la = tl.y * ATM_T + ta, lb = tl.z * ATM_T + tb;
+  const bool valid = la < M.nat && lb < M.nat && la < lb;
+  const long long rowa = M.pair0 + (long long)la * M.nat, rowb = M.pair0 + (long long)lb * M.nat;
+  double swab = 0.0, r2 = 0.0, ir = 0.0, t6 = 0.0;
+  if (valid) {
+    swab = SW[rowa + lb];
+    r2 = R2[rowa + lb];
+    ir = IR[rowa + lb];
+    t6 = 6.0 * T[rowa + lb];
+  }
+  const bool act = valid && swab > 0.0;
+  const double r2_2 = r2 * r2, alp3 = 16.0 / 3.0;
+  const double* src[4] = {W, T, IR, R2};
+  double gs = 0.0, ds = 0.0;
+  for (int k0 = 0; k0 < M.nat; k0 += ATM_K) {
+    __syncthreads();
+    // 4 arrays x 16 rows x 32 k for each side: 8 values per thread and side
+    for (int e = tid; e < 4 * ATM_T * ATM_K; e += 256) {
+      const int arr = e / (ATM_T * ATM_K), r = (e / ATM_K) % ATM_T, k = e % ATM_K;
+      const int gk = k0 + k, ra = tl.y * ATM_T + r, rb = tl.z * ATM_T + r;
+      sa[arr][r][k] = (gk < M.nat && ra < M.nat) ? src[arr][M.pair0 + (long long)ra * M.nat + gk] : 0.0;
+      sb[arr][r][k] = (gk < M.nat && rb < M.nat) ? src[arr][M.pair0 + (long long)rb * M.nat + gk] : 0.0;
     }
-    gs = wave_sum(gs);
-    ds = wave_sum(ds);
-    if (lane == 0) {
-      double cab = -(5.0 * sqrt(fabs(C6[rowa + lb])));
-      GD[rowa + lb] = gs * cab;
-      GD[rowb + la] = gs * cab;
-      GD[tot + rowa + lb] = ds * cab;
-      GD[tot + rowb + la] = ds * cab;
+    __syncthreads();
+    if (!act) continue;
+    const int kn = min(ATM_K, M.nat - k0);
+    for (int k = 0; k < kn; k++) {
+      const double c = sa[0][ta][k] * sb[0][tb][k];
+      const double cralp = t6 * sa[1][ta][k] * sb[1][tb][k];
+      const double fdmp = 1.0 / (1.0 + cralp);
+      const double ir1 = ir * sa[2][ta][k] * sb[2][tb][k];
+      const double rrr2 = ir1 * ir1, rrr3 = rrr2 * ir1;
+      const double angf = 0.375 * rrr2 * rrr3;
+      const double r2ak = sa[3][ta][k], r2bk = sb[3][tb][k];
+      const double s1 = r2ak + r2bk, d1 = (r2bk - r2ak) * (r2bk - r2ak);
+      const double ang = angf * ((r2_2 - d1) * (s1 - r2)) + rrr3;
+      const double cf = c * fdmp;
+      ds += ang * cf;
+      const double pp = r2 * (r2 * (r2 + s1) + 2.0 * (s1 * s1) + d1) - 5.0 * (d1 * s1);
+      gs += cf * (angf * pp - alp3 * (cralp * fdmp) * ang);
     }
   }
+  if (!valid) return;
+  // (k_atm reads only the pairs with sw > 0)
+  const double cab = act ? -(5.0 * sqrt(fabs(C6[rowa + lb]))) : 0.0;
+  GD[rowa + lb] = gs * cab;
+  GD[rowb + la] = gs * cab;
+  GD[tot + rowa + lb] = ds * cab;
+  GD[tot + rowb + la] = ds * cab;
 }
 
 // k_atm: each atom a keeps its share of its pairs with sw > 0: E_a -= sw d / 6 per pair, the pair

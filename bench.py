@@ -42,6 +42,7 @@ PEAK_FP64_MFMA = 78.6e12   # MI355X dense FP64 matrix (flop/s)
 PEAK_FP32_MFMA = 157.3e12  # MI355X dense FP32 matrix (flop/s)
 PEAK_HBM = 8.0e12          # MI355X HBM3E (byte/s)
 STREAMING = {"k_fock", "k_mulliken_cols", "k_integrals", "k_trd_symv"}
+FP32_KERNELS = ("_f32", "_upd32")
 VALENCE = {1: 1, 6: 4, 7: 5, 8: 6}  # GFN2 reference occupations of the workloads' elements
 METRIC = "E+grad/s: (H2O)1000 & 4096-mol batch, mixed vs FP64 GPU at |ΔE|≤1e-6 Eh"
 
@@ -131,7 +132,7 @@ def max_diffs(a, b):
 
 def kernel_roofline(prof, wl):
     """The dominant kernel (largest total time) of a profile and its roofline."""
-    prof = {k: v for k, v in prof.items() if k.startswith("k_")}
+    prof = {k: v for k, v in prof.items() if k.startswith(("k_", "rocblas_"))}
     if not prof:
         return None
     total = sum(v[1] for v in prof.values())
@@ -141,7 +142,7 @@ def kernel_roofline(prof, wl):
     if any(name.startswith(s) for s in STREAMING):
         bound, peak, unit, achieved = "hbm", PEAK_HBM / 1e9, "GB/s", per_launch / avg_s / 1e9
     else:
-        pk = PEAK_FP32_MFMA if name.endswith("_f32") else PEAK_FP64_MFMA
+        pk = PEAK_FP32_MFMA if name.endswith(FP32_KERNELS) else PEAK_FP64_MFMA
         bound, peak, unit, achieved = "mfma", pk / 1e12, "TFLOP/s", per_launch / avg_s / 1e12
     return {"bound": bound, "kernel": name, "achieved": achieved, "peak": peak, "unit": unit,
             "frac": achieved / peak, "traffic": measured_traffic(name, wl), "avg_launch_ms": avg_s * 1e3,
@@ -150,7 +151,7 @@ def kernel_roofline(prof, wl):
 
 
 def counters(prof):
-    c = {k: v for k, v in prof.items() if not k.startswith("k_")}
+    c = {k: v for k, v in prof.items() if not k.startswith(("k_", "rocblas_"))}
     return {"mean_jacobi_sweeps": {k: round(v[2] / v[0], 3) for k, v in c.items() if k.startswith("sweeps_") and v[0]},
             "refinement": {k: (round(v[2] / v[0], 3) if k.startswith("refine_steps") else v[0])
                            for k, v in c.items() if k.startswith("refine") and v[0]}}
