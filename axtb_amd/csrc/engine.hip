@@ -471,7 +471,9 @@ struct Engine {
       two |= j.k2 > 0;
     }
     double flops = 0.0;
-    for (auto& j : jobs) flops += 2.0 * j.m * j.n * ((double)j.k + j.k2);
+    // the products' own flops: a lower-triangle (symmetric) output counts its m (m + 1) / 2 elements
+    for (auto& j : jobs)
+      flops += (j.lower && j.m == j.n ? (double)j.m * (j.m + 1) : 2.0 * j.m * j.n) * ((double)j.k + j.k2);
     // one product with few output tiles and a long inner dimension: split K over workgroups,
     // partials reduced in a fixed order
     if (jobs.size() == 1 && !two) {
