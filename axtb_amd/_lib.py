@@ -113,10 +113,15 @@ def lib():
     return _lib
 
 
+# axtb_opts.precision: FP64 with a full solve per SCC iteration, mixed FP32/FP64, and the mixed
+# path's refinement with every step in FP64
+PRECISIONS = {"fp64": 0, "mixed": 1, "fp64-refine": 2}
+
+
 def make_opts(accuracy=1.0, etemp=300.0, max_iter=250, damping=0.4, guess="eeq", gradient=True, precision="fp64",
               dispersion=True, mixer="auto"):
     return Opts(float(accuracy), float(etemp), int(max_iter), float(damping), 0 if guess == "eeq" else 1,
-                int(bool(gradient)), 0 if precision == "fp64" else 1, int(bool(dispersion)), MIXERS[mixer])
+                int(bool(gradient)), PRECISIONS[precision], int(bool(dispersion)), MIXERS[mixer])
 
 
 class Context:

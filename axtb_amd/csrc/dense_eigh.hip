@@ -83,6 +83,17 @@ __global__ __launch_bounds__(256) void k_trd_col(double* A, int n, int i0, int j
       a_i = A[i * ln + r];
     }
   }
+  // row i's coefficients V(i, c), W(i, c) (thread c) and wave 0's share of row i's symv partials,
+  // issued with the loads above
+  double vic = 0.0, wic = 0.0, pin = 0.0;
+  if (!finalize_only) {
+    if ((int)threadIdx.x < jj) {
+      vic = A[(i0 + threadIdx.x) * ln + i];
+      wic = W[threadIdx.x * ln + i];
+    }
+    if (wv == 0)
+      for (int ch = lane; ch < nch; ch += 64) pin += part[ch * ln + i];
+  }
   double t = 0.0, alpha = 0.0, scal = 1.0;
   if (j > 0) {
     const int ndb = (mp + TRD_DR - 1) / TRD_DR;
@@ -112,15 +123,12 @@ __global__ __launch_bounds__(256) void k_trd_col(double* A, int n, int i0, int j
     // coefficients of row i: V(i, c), W(i, c); V(i, jj) is the unit first element
     const int c = threadIdx.x;
     if (c < jj) {
-      vi[c] = A[(i0 + c) * ln + i];
-      wi[c] = W[c * ln + i];
+      vi[c] = vic;
+      wi[c] = wic;
     }
     if (j > 0 && wv == 0) {
-      double p = 0.0;
-      for (int ch = lane; ch < nch; ch += 64) p += part[ch * ln + i];
-      double sw = lane < jj ? A[(i0 + lane) * ln + i] * y1[lane] + W[lane * ln + i] * y2[lane] : 0.0;
-      p = wave_sum(p);
-      sw = wave_sum(sw);
+      const double p = wave_sum(pin);
+      const double sw = wave_sum(lane < jj ? vic * y1[lane] + wic * y2[lane] : 0.0);
       if (lane == 0) {
         vi[jj] = 1.0;
         wi[jj] = t * (p - sw) + alpha;
@@ -198,11 +206,17 @@ __global__ __launch_bounds__(256) void k_trd_symv(const double* A, int n, int i0
     const int cb = q - rb * (rb + 1) / 2;
     const int R0 = rb * TRD_CH, C0 = cb * TRD_CH;
     const double* a = A + (long long)(i + 1 + C0) * ln + (i + 1 + R0);
-#pragma unroll 4
-    for (int e = tid; e < TRD_CH * TRD_CH; e += 256) {
-      const int c = e / TRD_CH, r = e % TRD_CH;
-      tile[c][r] = (R0 + r < m && C0 + c < m) ? a[c * ln + r] : 0.0;
+    // all 16 loads of this thread in flight at once (lane = row, columns wv + 4 k)
+    constexpr int KT = TRD_CH * TRD_CH / 256;
+    double ld[KT];
+    const bool rin = R0 + lane < m;
+#pragma unroll
+    for (int k = 0; k < KT; k++) {
+      const int c = wv + 4 * k;
+      ld[k] = (rin && C0 + c < m) ? a[c * ln + lane] : 0.0;
     }
+#pragma unroll
+    for (int k = 0; k < KT; k++) tile[wv + 4 * k][lane] = ld[k];
     if (tid < TRD_CH) {
       vr[tid] = R0 + tid < m ? vec(R0 + tid) : 0.0;
     } else if (tid < 2 * TRD_CH) {
@@ -243,7 +257,7 @@ __global__ __launch_bounds__(256) void k_trd_symv(const double* A, int n, int i0
     if (c < 2 * j) {
       const double* y = c < j ? W + c * ln : A + (i0 + c - j) * ln;
       double s = 0.0;
-#pragma unroll 4
+#pragma unroll
       for (int q = lane; q < TRD_DR; q += 64) {
         const int rl = rb * TRD_DR + q;
         if (rl < m) s += y[i + 1 + rl] * vec(rl);

@@ -963,6 +963,8 @@ struct Engine {
     if (!(o.damping > 0.0 && o.damping <= 1.0)) throw Fail{AXTB_ERR_INPUT, "damping must be in (0, 1]"};
     if (o.max_iter < 1) throw Fail{AXTB_ERR_INPUT, "max_iter must be >= 1"};
     if (o.mixer < AXTB_MIXER_AUTO || o.mixer > AXTB_MIXER_DIIS) throw Fail{AXTB_ERR_INPUT, "unknown SCF mixer"};
+    if (o.precision < AXTB_PREC_FP64 || o.precision > AXTB_PREC_FP64_REFINE)
+      throw Fail{AXTB_ERR_INPUT, "unknown precision"};
     resident = false;
     out.assign(in.size(), MolOut());
     std::vector<int> use;  // input index of each batch molecule
@@ -1768,7 +1770,7 @@ struct Engine {
           gsg.push_back(GemmJob{n, n, n, 1, 0, U, U, Vw.p + o0, nullptr, n, n, n, 1.0, 0.0});         // G
           gsg[gsg.size() - 2].lower = gsg.back().lower = sym_lower(n, false);  // both symmetric
           gu.push_back(GemmJob{n, n, n, 0, 0, U, Wm.p + o0, out, nullptr, n, n, n, 1.0, 0.0});        // U'
-          gu.back().f32upd = (upd32 && n > 128) ? 1 : 0;  // small products stay on the FP64 small-matrix kernels
+          gu.back().f32upd = (upd32 && prec == AXTB_PREC_MIXED && n > 128) ? 1 : 0;  // small products stay FP64
           oj.push_back(EigJob{n, F.p + o0, Vw.p + o0, j.w, Wm.p + o0});
         }
         oa_jobs.upload(oj, st, &stage);
@@ -1836,7 +1838,7 @@ struct Engine {
       LAUNCH(k_oa_big_e, dim3(nb), dim3(256), 0, oj, (const double*)oab_part.p, nb, oab_delta.p, step == 0 ? 1 : 0,
              oab_emax.p, cls, dense_ovmix);
       std::vector<GemmJob> gu{GemmJob{n, n, n, 0, 0, U, Wm.p + o0, out, nullptr, n, n, n, 1.0, 0.0}};
-      gu[0].f32upd = upd32 ? 1 : 0;
+      gu[0].f32upd = (upd32 && prec == AXTB_PREC_MIXED) ? 1 : 0;
       gemm(gu);
       step++;
       std::vector<double> h(nb);
@@ -1968,7 +1970,11 @@ struct Engine {
     std::vector<int> hkocc(NM);
     // dense molecules in mixed mode: the next eigenproblem by a full solve (all n vectors kept for
     // the refinement) or by refining the previous iteration's vectors
-    const bool mixed = o.precision == 1;
+    // mixed and FP64-refine: the refinement of the previous orbitals (split for batched molecules,
+    // dense refinement); mixed alone adds the FP32 Jacobi and the FP32 refinement update
+    prec = o.precision;
+    const bool mixed = o.precision != AXTB_PREC_FP64;
+    const bool fp32 = o.precision == AXTB_PREC_MIXED;
     std::vector<int> dfull(NM, 1), drefined(NM, 0);
     dense_pfp.resize(NM, 0);
     for (int m = 0; m < NM; m++)
@@ -2032,7 +2038,7 @@ struct Engine {
       for (auto& j : a2) j.lower = sym_lower(j.n, dense(j.n));  // A = B^T (F B) is symmetric
       gemm(a2);
       mirror_dense(a2);
-      if (o.precision == 1) {
+      if (mixed) {
         // mixed: Jacobi in FP32, then one FP64 Ogita-Aishima step U' = U (I + E) from
         // S = U^T A U and G = U^T U (U in Z, A in Am; S -> F, G -> Vw, I + E -> Wm, U' -> T)
         // Molecules whose previous occupations were all 0 or 2: the previous orbitals (A nearly
@@ -2071,14 +2077,19 @@ struct Engine {
           if (ok_split[qs[t]]) in_t[qs[t]] = ts[t];
         std::sort(qf.begin(), qf.end());
         for (int q : qf) ef.push_back(ea[q]);
-        eigh(ef, true, 60);
         std::vector<EigJob> redo;
         std::vector<int> redo_mol;
-        for (int f : refine(ef, tf)) {
-          redo.push_back(ef[f]);
-          redo_mol.push_back(eam[qf[f]]);
+        if (fp32) {
+          eigh(ef, true, 60);
+          for (int f : refine(ef, tf)) {
+            redo.push_back(ef[f]);
+            redo_mol.push_back(eam[qf[f]]);
+          }
+          for (size_t t = 0; t < qf.size(); t++) in_t[qf[t]] = tf[t];
+        } else {  // FP64-refine: the full solves by the FP64 Jacobi
+          redo = ef;
+          for (int q : qf) redo_mol.push_back(eam[q]);
         }
-        for (size_t t = 0; t < qf.size(); t++) in_t[qf[t]] = tf[t];
         // not converged by the refinement: FP64 Jacobi (vectors in Z; in_t is 0 for them)
         eigh(redo, false, -1, 1.0f, &redo_mol);
         // C = B U from wherever U ended (a3 and ea list the same molecules in the same order)
@@ -2234,6 +2245,7 @@ struct Engine {
     HIP_OK(hipStreamSynchronize(st));
   }
   bool exact_levels = true;  // mixed, dense: exact orbital energies at the end (AXTB_HIP_EXACT_LEVELS)
+  int prec = AXTB_PREC_FP64;  // precision of the running SCF
 
   // ------------------------------------------------------------------------------ gradient
   std::vector<int> hok;

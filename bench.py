@@ -10,7 +10,9 @@ Workloads (default: both):
   one energy + gradient.
 Settings: axtb's default conventions, SCC energy convergence 1e-8 Eh (accuracy 1e-2), 300 K, D4,
 EEQ start. Inputs are resident on the device when the timed region starts (batch_prepare once,
-batch_run per step). Each workload runs in plain FP64 and in mixed precision on the same inputs:
+batch_run per step). Each workload runs in plain FP64, in mixed precision and in FP64-refine (the
+mixed path's algorithm, the previous orbitals refined, with every step in FP64, so that mixed /
+FP64-refine is the gain of the precision alone) on the same inputs:
 `value` is the mixed E+grad/s of the batch; `config` holds every rate, the mixed/FP64 ratios and
 the differences of each precision from the FP64 CPU oracle (the batch: its first molecules,
 computed after timing; (H2O)1000: the oracle's committed result, tests/golden/oracle_water1000.npz)
@@ -174,14 +176,15 @@ def run_workload(wl, args, acc):
         rng = np.random.default_rng(1)
         mols = [(m[0], np.asarray(m[1]) + rng.normal(scale=0.02, size=np.shape(m[1]))) + tuple(m[2:]) for m in mols]
     runs = {}
-    for prec in (["fp64", "mixed"] if args.precision == "both" else [args.precision]):
+    precs = {"all": ["fp64", "mixed", "fp64-refine"], "both": ["fp64", "mixed"]}.get(args.precision, [args.precision])
+    for prec in precs:
         runs[prec] = measure(mols, prec, steps, warmup, args.profile_steps, acc, starts)
     out = {"molecules": len(mols), "steps": steps, "warmup": warmup,
            "warm_start": bool(args.warm)}
     for prec, (dt, prof, res) in runs.items():
         rate = len(mols) * steps / dt
         wf, wbytes = path_work(mols, res)
-        peak = PEAK_FP64_MFMA if prec == "fp64" else PEAK_FP32_MFMA
+        peak = PEAK_FP32_MFMA if prec == "mixed" else PEAK_FP64_MFMA
         t_roof = wf / peak + wbytes / PEAK_HBM
         ok = [r for r in res if r["status"] == 0]
         out[prec] = {"value": rate, "ms_per_step": dt / steps * 1e3,
@@ -215,11 +218,16 @@ def run_workload(wl, args, acc):
         out["oracle"] = f"C++ FP64 oracle on the first {nchk} molecules" if nchk else None
     for p in runs:
         out[p]["vs_oracle_max_abs"] = check[p]
-    if len(runs) == 2:
+    if "mixed" in runs and "fp64" in runs:
         d = max_diffs(runs["mixed"][2], runs["fp64"][2])
         out["mixed_speedup_vs_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
         out["mixed_vs_fp64_max_abs_all_molecules"] = d
         out["mixed_within_1e-6_Eh_of_oracle"] = (check["mixed"]["dE"] is not None and check["mixed"]["dE"] <= 1e-6)
+    if "mixed" in runs and "fp64-refine" in runs:
+        # the same algorithm in both: the ratio of the precisions alone
+        out["mixed_speedup_vs_fp64_refine"] = out["mixed"]["value"] / out["fp64-refine"]["value"]
+        out["fp64_refine_vs_fp64_max_abs_all_molecules"] = (max_diffs(runs["fp64-refine"][2], runs["fp64"][2])
+                                                            if "fp64" in runs else None)
     return out, mols
 
 
@@ -273,7 +281,9 @@ def main():
     ap.add_argument("--workload", default="default",
                     help="default (batch4096 and water1000), or one of batch4096 / water1000 / water200 / "
                          "c100h202 / water20 / c30h62 / conformers2048 / batchN")
-    ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed"])
+    ap.add_argument("--precision", default="all", choices=["all", "both", "fp64", "mixed", "fp64-refine"],
+                    help="all: plain FP64, mixed and FP64-refine (the mixed algorithm with every step in FP64); "
+                         "both: plain FP64 and mixed")
     ap.add_argument("--warm", action="store_true",
                     help="a trajectory step: every molecule moved by 0.02 bohr (seeded) and started from the "
                          "converged moments of its previous geometry (computed untimed)")
@@ -307,11 +317,12 @@ def main():
             mols0 = mols
     main_wl = names[0]
     main = results[main_wl]
-    main_prec = "mixed" if "mixed" in main else "fp64"
+    main_prec = next(p for p in ("mixed", "fp64", "fp64-refine") if p in main)
     dom = main[main_prec]["dominant_kernel"] or {}
     roof = {k: dom.get(k) for k in ("bound", "kernel", "achieved", "peak", "unit", "frac", "traffic",
                                      "avg_launch_ms", "launches", "work_per_launch", "kernel_share_of_gpu_time")}
-    roof["whole_path"] = {wl: {p: r[p]["whole_path_roofline"]["frac"] for p in ("fp64", "mixed") if p in r}
+    roof["whole_path"] = {wl: {p: r[p]["whole_path_roofline"]["frac"] for p in ("fp64", "mixed", "fp64-refine")
+                               if p in r}
                           for wl, r in results.items()}
     line = {
         "metric": METRIC,
@@ -324,7 +335,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if main_prec == "fp64" else "f32+f64",
+        "dtype": "f32+f64" if main_prec == "mixed" else "f64",
         "data": "synthetic (seeded molecules of the benchmark configuration)",
         "config": {"workload": main_wl + (" trajectory step (0.02 bohr, warm start)" if args.warm else ""),
                    "value_is": f"{main_wl} {main_prec}", "accuracy": acc, "etemp_K": 300.0, **results},

@@ -43,6 +43,10 @@ typedef struct axtb_mol {
 
 /* Options of one calculation: axtb::Options / ScfOptions (crates/axtb/src/lib.rs:92-153,
  * crates/axtb/src/xtb.rs:65-99). */
+#define AXTB_PREC_FP64 0
+#define AXTB_PREC_MIXED 1
+#define AXTB_PREC_FP64_REFINE 2
+
 typedef struct axtb_opts {
   double accuracy;   /* SCF accuracy: energy threshold 1e-6*acc, residual 2e-5*acc (default 1) */
   double etemp;      /* electronic temperature in K (default 300) */
@@ -50,7 +54,11 @@ typedef struct axtb_opts {
   double damping;    /* Broyden damping (default 0.4) */
   int guess;         /* 0: EEQ charges (default), 1: neutral atoms */
   int want_gradient; /* 1: analytic gradient */
-  int precision;     /* 0: FP64 throughout */
+  /* AXTB_PREC_FP64 (0): FP64 throughout, a full eigensolve every SCC iteration; AXTB_PREC_MIXED (1):
+   * FP32 Jacobi and FP32 refinement updates, FP64 Ogita-Aishima refinement, later iterations refine
+   * the previous orbitals; AXTB_PREC_FP64_REFINE (2): the refinement of mode 1 with every step in
+   * FP64 (no counterpart in the reference, which solves in full every iteration) */
+  int precision;
   int dispersion;    /* 1: D4 (default) */
   /* SCF mixer plan (MixerKind, crates/axtb/src/scf/mixing.rs:83-118): 0 auto (default of the axtb
    * conventions, conventions.rs:260: Broyden, then DIIS from the start if Broyden does not

@@ -113,6 +113,33 @@ def test_mixed_precision_batch(ctx):
     print("mixed: max |dE| %.2e |dg| %.2e |dq| %.2e |deps| %.2e |defermi| %.2e" % tuple(worst), flush=True)
 
 
+def test_fp64_refine_batch(ctx):
+    """FP64-refine (the mixed path's refinement of the previous orbitals with every step in FP64):
+    the plain FP64 bars against the oracle, |dE| <= 1e-8 Eh and max |dg| <= 1e-7 Eh/bohr, charges
+    and orbital energies to 1e-6; and no FP32 kernel runs (the profile lists no FP32 Jacobi and no
+    FP32 refinement update)."""
+    import oracle
+    from axtb_amd import _lib, structures as S
+    mols = S.batch(16, seed=11) + [S.config("water20"), S.config("c30h62")]
+    ctx.profile(True)
+    try:
+        rs = ctx.engrad_batch(mols, _lib.make_opts(accuracy=1e-2, precision="fp64-refine"))
+        rep = ctx.profile_report()
+    finally:
+        ctx.profile(False)
+    assert not [k for k in rep if k.endswith("_f32") or "upd32" in k], sorted(rep)
+    assert any(k.startswith("refine_steps") for k in rep), sorted(rep)  # the refinement ran
+    worst = [0.0] * 4
+    for (z, x), r in zip(mols, rs):
+        assert r["status"] == 0
+        o = oracle.engrad(z, x, acc=1e-2)
+        d = [abs(r["energy"] - o["energy"]), np.abs(r["gradient"] - o["gradient"]).max(),
+             np.abs(r["charges"] - o["qat"]).max(), np.abs(np.sort(r["orbital_energies"]) - np.sort(o["emo"])).max()]
+        worst = [max(a, b) for a, b in zip(worst, d)]
+        assert d[0] <= 1e-8 and d[1] <= 1e-7 and d[2] <= 1e-6 and d[3] <= 1e-6, d
+    print("fp64-refine: max |dE| %.2e |dg| %.2e |dq| %.2e |deps| %.2e" % tuple(worst), flush=True)
+
+
 def test_mixed_split_edge_cases(ctx):
     """Mixed precision where the occupied / empty split refinement does not apply or works at its
     edges: zero temperature (integral occupations by index), an open-shell cation (a singly
