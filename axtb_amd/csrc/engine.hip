@@ -429,7 +429,9 @@ struct Engine {
     }
     // one large plain product (no inner scaling, no second term, the full C): rocBLAS's dgemm, which
     // runs at a higher fraction of the FP64 matrix peak than the 128-tile kernel at these sizes (not in
-    // a no-allocation call: the library keeps its own workspace)
+    // a no-allocation call: the library keeps its own workspace). Symmetric products stay on the
+    // lower tiles of the 128-tile kernel: the library's full product took longer (6.6 against 6 ms at
+    // n = 6000).
     if (!any32 && use_rocblas && !t_arena && jobs.size() == 1) {
       const GemmJob& J = jobs[0];
       if (!J.dk && !J.A2 && !J.lower && J.m >= rocblas_min && J.n >= rocblas_min && J.k >= rocblas_min) {
@@ -446,6 +448,33 @@ struct Engine {
         if (rs != rocblas_status_success) throw DeviceError("rocblas_dgemm failed: " + std::to_string((int)rs));
         return;
       }
+    }
+    if (any32 && use_rocblas && !t_arena && jobs.size() == 1 && jobs[0].m >= rocblas_min &&
+        jobs[0].n >= rocblas_min && !jobs[0].ta && !jobs[0].tb && jobs[0].k == jobs[0].n && !jobs[0].A2 &&
+        !jobs[0].dk) {
+      // one large update: U and E = B - I rounded to FP32, U E by rocBLAS's sgemm (FP32 accumulation,
+      // a higher fraction of the FP32 matrix peak than the 128-tile kernel), C = U + U E in FP64
+      const GemmJob& J = jobs[0];
+      const long long mk = (long long)J.m * J.k, kn = (long long)J.k * J.n, mn = (long long)J.m * J.n;
+      f32buf.ensure((size_t)(mk + kn + mn));
+      float *Uf = f32buf.p, *Ef = f32buf.p + mk, *Pf = f32buf.p + mk + kn;
+      LAUNCH(k_to_f32, dim3(blocks(mk, 256)), dim3(256), 0, J.A, (long long)J.lda, J.m, J.k, 0, Uf);
+      LAUNCH(k_to_f32, dim3(blocks(kn, 256)), dim3(256), 0, J.B, (long long)J.ldb, J.k, J.n, 1, Ef);
+      if (!blas) {
+        if (rocblas_create_handle(&blas) != rocblas_status_success) throw DeviceError("rocblas_create_handle failed");
+      }
+      if (rocblas_set_stream(blas, st) != rocblas_status_success) throw DeviceError("rocblas_set_stream failed");
+      const float one = 1.0f, zero = 0.0f;
+      {
+        ProfScope ps_(&prof, st, "rocblas_sgemm_upd32", 2.0 * J.m * J.n * (double)J.k);
+        const rocblas_status rs = rocblas_sgemm(blas, rocblas_operation_none, rocblas_operation_none, J.m, J.n, J.k,
+                                                &one, Uf, J.m, Ef, J.k, &zero, Pf, J.m);
+        if (rs != rocblas_status_success) throw DeviceError("rocblas_sgemm failed: " + std::to_string((int)rs));
+      }
+      LAUNCH(k_add_f32, dim3(blocks(mn, 256)), dim3(256), 0, J.A, (long long)J.lda, (const float*)Pf, J.m, J.n, J.C,
+             (long long)J.ldc);
+      HIP_OK(hipGetLastError());
+      return;
     }
     if (any32) {  // refinement updates C = A + A (B - I), the product in FP32
       int mm = 0, nn = 0;
@@ -546,6 +575,7 @@ struct Engine {
     HIP_OK(hipGetLastError());
   }
   DBuf<double> gpart;
+  DBuf<float> f32buf;  // FP32 operands and product of a library FP32 update
   void launch128(int t, dim3 grid, double flops) {
     if (t == 0)
       LAUNCHW(flops, k_gemm128_nn, grid, dim3(256), 0, gj.p);

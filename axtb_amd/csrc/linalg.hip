@@ -373,6 +373,23 @@ __global__ __launch_bounds__(256, 2) void k_gemm128_upd32(const GemmJob* jobs) {
       }
 }
 
+// The operands of the FP32 update by a library product: X (m x n, leading dimension ld) rounded to
+// FP32 into Y (leading dimension m), minus the identity when eye is set (B = I + E -> E).
+__global__ void k_to_f32(const double* X, long long ld, int m, int n, int eye, float* Y) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)m * n) return;
+  const int r = (int)(e % m), c = (int)(e / m);
+  Y[e] = (float)(X[r + c * ld] - (eye && r == c ? 1.0 : 0.0));
+}
+
+// C = U + P: the FP32 product P (m x n, leading dimension m) added to U in FP64.
+__global__ void k_add_f32(const double* U, long long ldu, const float* P, int m, int n, double* C, long long ldc) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)m * n) return;
+  const int r = (int)(e % m), c = (int)(e / m);
+  C[r + c * ldc] = U[r + c * ldu] + (double)P[e];
+}
+
 // C = sum over s of the split-K partials P_s (m x n, ld m, back to back) + beta C, the partials
 // in a fixed order.
 __global__ void k_gemm_reduce(const double* P, int splits, int m, int n, double* C, long long ldc, double beta) {

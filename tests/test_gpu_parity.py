@@ -247,7 +247,8 @@ def test_gemm_upd32():
     rounding bound of an FP32 product (operands rounded to FP32, FP32 accumulation)."""
     from axtb_amd import _testhooks as H
     rng = np.random.default_rng(3)
-    for m, n in [(5, 5), (64, 64), (120, 120), (130, 130), (300, 300), (200, 257)]:
+    # sizes of at least 512 take the library FP32 product (rocBLAS sgemm)
+    for m, n in [(5, 5), (64, 64), (120, 120), (130, 130), (300, 300), (200, 257), (600, 600), (530, 700)]:
         U = rng.standard_normal((m, n))
         E = 1e-3 * rng.standard_normal((n, n))
         C = H.gemm_upd32(U, np.eye(n) + E)
