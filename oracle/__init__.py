@@ -48,7 +48,11 @@ def sizes(numbers):
     return nsh.value, nao.value
 
 
-TERMS = ("repulsion", "dispersion", "electronic", "isotropic_es", "third_order", "anisotropic", "entropy", "band")
+TERMS = ("repulsion", "dispersion", "electronic", "isotropic_es", "third_order", "anisotropic", "entropy", "band",
+         "solvation")
+SOLV_MODELS = {None: 0, "alpb": 1, "gbsa": 2}
+SOLV_STATES = {"gsolv": 0, "bar1M": 1, "bar1m": 1, "reference": 2}
+SOLV_CONVENTIONS = {"xtb": 0, "xtb671": 1, "corrected": 2}
 
 
 MIXERS = {"broyden": 0, "diis": 1, "auto": 2}
@@ -56,18 +60,19 @@ MIXER_USED = {0: "broyden", 1: "diis", 2: "diis-fallback"}
 
 
 def engrad(numbers, positions, charge=0.0, acc=1.0, etemp=300.0, max_iter=250, guess="eeq", gradient=True,
-           dispersion=True, start=None, mixer="auto", report=None):
+           dispersion=True, start=None, mixer="auto", report=None, solvation=None, solvent=None, state="gsolv"):
     """GFN2-xTB energy (+ gradient) of one molecule; positions in bohr [N,3]. ``start``: a given
     SCF start {"qsh", "dipoles", "quadrupoles"} (the moments of a previous result), else ``guess``.
     ``mixer``: the SCF mixer plan (auto = Broyden, then DIIS from the start: the axtb conventions).
     ``report``: a dict that receives the SCF report (iterations, mixer_used, attempts) also when
-    the calculation fails."""
+    the calculation fails. ``solvation``: None, "alpb" or "gbsa" with ``solvent`` (a name xtb
+    accepts) and the reference ``state`` (gsolv, bar1M, reference)."""
     z = np.ascontiguousarray(numbers, dtype=np.int32)
     x = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
     nat = len(z)
     nsh, nao = sizes(z)
     e = ctypes.c_double()
-    terms = np.zeros(8)
+    terms = np.zeros(len(TERMS))
     g = np.zeros((nat, 3))
     qat = np.zeros(nat)
     qsh = np.zeros(nsh)
@@ -84,7 +89,8 @@ def engrad(numbers, positions, charge=0.0, acc=1.0, etemp=300.0, max_iter=250, g
     rc = lib().oracle_singlepoint(nat, _p(z), _p(x), ctypes.c_double(charge), ctypes.c_double(acc),
                                   ctypes.c_double(etemp), max_iter, 0 if guess == "eeq" else 1, int(gradient),
                                   int(dispersion), MIXERS[mixer], _p(gs[0]), _p(gs[1]), _p(gs[2]), ctypes.byref(e),
-                                  _p(terms), _p(g), _p(qat), _p(qsh), _p(dp), _p(qp), _p(emo), _p(info), _p(scal))
+                                  _p(terms), _p(g), _p(qat), _p(qsh), _p(dp), _p(qp), _p(emo), _p(info), _p(scal),
+                                  SOLV_MODELS[solvation], (solvent or "").encode(), SOLV_STATES[state])
     rep = {"iterations": int(info[0]), "mixer_used": MIXER_USED.get(int(info[1])),
            "attempts": [(("broyden", "diis")[0 if (mixer != "diis" and a == 0) else 1], int(info[3 + 2 * a]),
                          bool(info[4 + 2 * a])) for a in range(int(info[2]))]}
@@ -94,6 +100,24 @@ def engrad(numbers, positions, charge=0.0, acc=1.0, etemp=300.0, max_iter=250, g
     return {"energy": e.value, "terms": dict(zip(TERMS, terms)), "gradient": g if gradient else None,
             "qat": qat, "qsh": qsh, "dipoles": dp, "quadrupoles": qp, "emo": emo, "efermi": scal[0],
             "dipole": scal[1:4].copy(), "gap": scal[4], **rep}
+
+
+def solvation_parts(numbers, positions, charges, model, solvent, state="gsolv", conventions="corrected", nang=230,
+                    gradient=False):
+    """The ALPB/GBSA model alone at fixed charges (positions in bohr): {gsolv, gelec, gsasa, ghb,
+    gshift} as xtb prints them, Born radii, surface areas, and the gradient of gsolv at fixed
+    charges if asked."""
+    z = np.ascontiguousarray(numbers, dtype=np.int32)
+    x = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
+    q = np.ascontiguousarray(charges, dtype=np.float64)
+    n = len(z)
+    out, brad, sasa = np.zeros(5), np.zeros(n), np.zeros(n)
+    g = np.zeros((n, 3)) if gradient else None
+    _check(lib().oracle_solv_parts(n, _p(z), _p(x), _p(q), SOLV_MODELS[model], solvent.encode(), SOLV_STATES[state],
+                                   SOLV_CONVENTIONS[conventions], nang, _p(out), _p(brad), _p(sasa), _p(g)))
+    r = dict(zip(("gsolv", "gelec", "gsasa", "ghb", "gshift"), out))
+    r.update(born_radii=brad, sasa=sasa, gradient=g)
+    return r
 
 
 def engrad_many(mols, acc=1.0, etemp=300.0, gradient=True):

@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstring>
 #include <deque>
+#include <limits>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -26,6 +27,7 @@
 #include <omp.h>
 
 #include "../axtb_amd/csrc/gfn2_tables.h"
+#include "../axtb_amd/csrc/solvation.hpp"
 
 namespace {
 
@@ -1853,6 +1855,437 @@ struct VecMixer {
   bool next() { return kind == MIX_DIIS ? di.next() : br.next(); }
 };
 
+// ------------------------------------------------------------------ implicit solvation (crates/solvation)
+// The model's setup (parameters per atom, grid, shift) is axtb_solv::build_model; below, the Born
+// radii (born.rs), the smoothed surface (sasa.rs), the Born matrix and kernel gradients
+// (kernel.rs) and the energy, potential and gradient of lib.rs:585-730, in the reference's loop
+// orders.
+struct SolvCache {
+  vec brad, dbrad, sasa, hbw, dhbdw, B;  // B row-major nat x nat
+  double adet = 0.0, gsasa = 0.0;
+  vector<vector<int>> nb;       // SASA neighbours of each atom, ascending
+  vector<vector<double>> nbg;   // d sasa_i / d R_j, 3 per neighbour
+  vec selfg;                    // d sasa_i / d R_i, 3 per atom
+};
+
+struct BornC {
+  double g = 0.0, dg = 0.0;
+};
+
+// sphere integrals of the GBOBC psi sum (born.rs:20-52); `overlap` sets r1
+inline BornC born_nonoverlap(double r, double rho, double r1) {
+  const double ap = r + rho, am = r - rho, ab = ap * am, rhab = rho / ab;
+  const double lnab = 0.5 * std::log(am / ap) * r1;
+  return BornC{rhab + lnab, -2.0 * rhab / ab + (rhab - lnab) * r1 * r1};
+}
+inline BornC born_overlap(double r, double rho, double rvdw, double& r1) {
+  r1 = 1.0 / r;
+  const double rr1 = r1, r12 = 0.5 * rr1, r24 = r12 * r12, ap = r + rho, am = r - rho;
+  const double rh1 = 1.0 / rvdw, rhr1 = 1.0 / ap, aprh1 = ap * rh1, lnab = std::log(aprh1);
+  const double g = rh1 - rhr1 + r12 * (0.5 * am * (rhr1 - rh1 * aprh1) - lnab);
+  const double dg = rhr1 * rhr1 * (1.0 - 0.25 * am * rr1 * (1.0 + aprh1 * aprh1)) + rho * r24 * (rhr1 - rh1 * aprh1) +
+                    r12 * (rr1 * lnab - rhr1);
+  return BornC{g, dg * rr1};
+}
+// the pair (ii > jj): contributions to psi(ii) and psi(jj) in xtb's four overlap cases, with the
+// r1 carried across pairs (born.rs:54-100)
+inline void born_pair(double r, double rhoi, double rhoj, double rvdwi, double rvdwj, double& r1, BornC& ci, bool& hi,
+                      BornC& cj, bool& hj) {
+  const bool ovij = !(r >= rvdwi + rhoj), ovji = !(r >= rhoi + rvdwj);
+  hi = hj = false;
+  if (!ovij && !ovji) {
+    r1 = 1.0 / r;
+    if (std::fabs(rhoi - rhoj) < 1.0e-8) {
+      ci = cj = born_nonoverlap(r, rhoj, r1);
+    } else {
+      ci = born_nonoverlap(r, rhoj, r1);
+      cj = born_nonoverlap(r, rhoi, r1);
+    }
+    hi = hj = true;
+  } else if (!ovij && ovji) {
+    r1 = 1.0 / r;
+    ci = born_nonoverlap(r, rhoj, r1);
+    hi = true;
+    if (r + rhoi > rvdwj) {
+      cj = born_overlap(r, rhoi, rvdwj, r1);
+      hj = true;
+    }
+  } else if (ovij && !ovji) {
+    if (r + rhoj > rvdwi) {
+      ci = born_overlap(r, rhoj, rvdwi, r1);
+      hi = true;
+    }
+    cj = born_nonoverlap(r, rhoi, r1);  // whatever r1 holds (xtb 6.7.1)
+    hj = true;
+  } else {
+    if (r + rhoj > rvdwi) {
+      ci = born_overlap(r, rhoj, rvdwi, r1);
+      hi = true;
+    }
+    if (r + rhoi > rvdwj) {
+      cj = born_overlap(r, rhoi, rvdwj, r1);
+      hj = true;
+    }
+  }
+}
+
+inline void solv_pair_vec(const vec& x, int i, int j, double d[3], double& r2, double& r) {
+  for (int k = 0; k < 3; k++) d[k] = x[3 * i + k] - x[3 * j + k];
+  r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  r = std::sqrt(r2);
+}
+
+// Born radii and dbrad/dpsi (born.rs:112-160)
+void solv_born_radii(const axtb_solv::SolvModel& S, const vec& x, SolvCache& c) {
+  using namespace axtb_solv;
+  const int nat = (int)S.vdwr.size();
+  const double lrcut2 = S.lrcut * S.lrcut;
+  vec psi(nat, 0.0);
+  double r1 = 0.0;
+  for (int ii = 0; ii < nat; ii++)
+    for (int jj = 0; jj < ii; jj++) {
+      double d[3], dr2, r;
+      solv_pair_vec(x, ii, jj, d, dr2, r);
+      if (!(dr2 < lrcut2)) continue;
+      if (S.fresh_r1) r1 = 1.0 / r;
+      BornC ci, cj;
+      bool hi, hj;
+      born_pair(r, S.rho[ii], S.rho[jj], S.vdwr[ii], S.vdwr[jj], r1, ci, hi, cj, hj);
+      if (hi) psi[ii] += ci.g;
+      if (hj) psi[jj] += cj.g;
+    }
+  c.brad.assign(nat, 0.0);
+  c.dbrad.assign(nat, 0.0);
+  for (int i = 0; i < nat; i++) {
+    double br = psi[i];
+    const double s1 = 1.0 / S.svdw[i], v1 = 1.0 / S.vdwr[i], s2 = 0.5 * S.svdw[i];
+    br *= s2;
+    double arg2 = br * (OBC_GAMMA * br - OBC_BETA);
+    const double arg = br * (OBC_ALPHA + arg2);
+    arg2 = 2.0 * arg2 + OBC_ALPHA + OBC_GAMMA * br * br;
+    const double th = std::tanh(arg), ch = std::cosh(arg);
+    br = 1.0 / (s1 - v1 * th);
+    br *= S.born_scale;
+    double dpsi = ch * (s1 - v1 * th);
+    dpsi = s2 * v1 * arg2 / (dpsi * dpsi);
+    dpsi *= S.born_scale;
+    c.brad[i] = br;
+    c.dbrad[i] = dpsi;
+  }
+}
+
+// sum_i f_i d psi_i / d R (born.rs:162-207)
+void solv_psi_gradient(const axtb_solv::SolvModel& S, const vec& x, const vec& f, vec& g) {
+  const int nat = (int)S.vdwr.size();
+  const double lrcut2 = S.lrcut * S.lrcut;
+  double r1 = 0.0;
+  for (int ii = 0; ii < nat; ii++) {
+    double gi[3] = {0, 0, 0};
+    for (int jj = 0; jj < ii; jj++) {
+      double d[3], dr2, r;
+      solv_pair_vec(x, ii, jj, d, dr2, r);
+      if (!(dr2 < lrcut2)) continue;
+      if (S.fresh_r1) r1 = 1.0 / r;
+      BornC ci, cj;
+      bool hi, hj;
+      born_pair(r, S.rho[ii], S.rho[jj], S.vdwr[ii], S.vdwr[jj], r1, ci, hi, cj, hj);
+      double s = 0.0;
+      if (hi) s += f[ii] * ci.dg;
+      if (hj) s += f[jj] * cj.dg;
+      if (s != 0.0)
+        for (int k = 0; k < 3; k++) {
+          gi[k] += s * d[k];
+          g[3 * jj + k] -= s * d[k];
+        }
+    }
+    for (int k = 0; k < 3; k++) g[3 * ii + k] += gi[k];
+  }
+}
+
+// smoothed SASA per atom and its derivatives (sasa.rs:34-80 neighbour lists, 110-225 numsa)
+void solv_sasa(const axtb_solv::SolvModel& S, const vec& x, SolvCache& c) {
+  const int nat = (int)S.vdwr.size(), np = (int)S.weight.size();
+  const double srcut2 = S.srcut * S.srcut;
+  c.nb.assign(nat, {});
+  for (int i = 0; i < nat; i++)
+    for (int j = 0; j < nat; j++) {
+      if (j == i) continue;
+      double d[3], d2, r;
+      solv_pair_vec(x, i, j, d, d2, r);
+      const double rmax = S.vdwsa[i] + S.vdwsa[j] + S.w + 1.0e-6;
+      if (d2 < srcut2 && d2 < rmax * rmax) c.nb[i].push_back(j);
+    }
+  c.sasa.assign(nat, 0.0);
+  c.selfg.assign(3 * (size_t)nat, 0.0);
+  c.nbg.assign(nat, {});
+  for (int iat = 0; iat < nat; iat++) {
+    const vector<int>& nb = c.nb[iat];
+    const int nno = (int)nb.size();
+    vec grads(3 * (size_t)nno, 0.0), tj2v(nno);
+    double gself[3] = {0, 0, 0}, sasai = 0.0;
+    const double rsas = S.vdwsa[iat], wr = S.wrp[iat];
+    vector<int> grdi(nno);
+    vec grds(3 * (size_t)nno);
+    for (int ip = 0; ip < np; ip++) {
+      double xp[3];
+      for (int k = 0; k < 3; k++) xp[k] = x[3 * iat + k] + rsas * S.grid[3 * ip + k];
+      bool buried = false;
+      for (int sl = 0; sl < nno; sl++) {
+        const int ia = nb[sl];
+        double t2 = 0.0;
+        for (int k = 0; k < 3; k++) t2 += (xp[k] - x[3 * ia + k]) * (xp[k] - x[3 * ia + k]);
+        tj2v[sl] = t2;
+        buried |= t2 <= S.trj2a[ia];
+      }
+      if (buried) continue;
+      int nni = 0;
+      double sasap = 1.0;
+      for (int sl = 0; sl < nno; sl++) {
+        const int ia = nb[sl];
+        const double tj2 = tj2v[sl];
+        if (tj2 < S.trj2b[ia]) {
+          double tj[3];
+          for (int k = 0; k < 3; k++) tj[k] = xp[k] - x[3 * ia + k];
+          const double sqtj = std::sqrt(tj2), uj = sqtj - S.vdwsa[ia], ah3uj2 = S.ah3 * uj * uj;
+          double dsasaij = S.ah1 + 3.0 * ah3uj2;
+          double sasaij = S.ah0 + (S.ah1 + ah3uj2) * uj;
+          if (1.0 < sasaij) sasaij = 1.0;
+          if (sasaij < 0.0) sasaij = 0.0;
+          if (sasaij < std::numeric_limits<double>::min()) {
+            sasap = 0.0;
+            break;
+          }
+          sasap *= sasaij;
+          dsasaij /= sasaij * sqtj;
+          grdi[nni] = sl;
+          for (int k = 0; k < 3; k++) grds[3 * nni + k] = dsasaij * tj[k];
+          nni++;
+        }
+      }
+      if (sasap > axtb_solv::TOLSESP) {
+        const double wsa = S.weight[ip] * wr * sasap;
+        sasai += wsa;
+        for (int jj = 0; jj < nni; jj++) {
+          const int sl = grdi[jj];
+          for (int k = 0; k < 3; k++) {
+            const double drjj = wsa * grds[3 * jj + k];
+            gself[k] += drjj;
+            grads[3 * sl + k] -= drjj;
+          }
+        }
+      }
+    }
+    c.sasa[iat] = sasai;
+    for (int k = 0; k < 3; k++) c.selfg[3 * iat + k] = gself[k];
+    c.nbg[iat] = grads;
+  }
+}
+
+// grad += sum_i w_i d sasa_i / d R (sasa.rs:245-268)
+void solv_sasa_contract(const SolvCache& c, const vec& w, vec& g) {
+  const int nat = (int)c.sasa.size();
+  for (int i = 0; i < nat; i++) {
+    const double ci = w[i];
+    if (ci == 0.0) continue;
+    for (int k = 0; k < 3; k++) g[3 * i + k] += c.selfg[3 * i + k] * ci;
+    for (size_t p = 0; p < c.nb[i].size(); p++) {
+      const int j = c.nb[i][p];
+      for (int k = 0; k < 3; k++) g[3 * j + k] += c.nbg[i][3 * p + k] * ci;
+    }
+  }
+}
+
+// inertia of the atoms weighted by rad^3 (kernel.rs:160-192) and the ALPB shape descriptor
+// aDet (kernel.rs:205-209)
+void solv_inertia(const vec& x, const vec& rad, double I[3][3], double center[3], double& tot_rad3) {
+  const int nat = (int)rad.size();
+  tot_rad3 = 0.0;
+  for (int k = 0; k < 3; k++) center[k] = 0.0;
+  for (int i = 0; i < nat; i++) {
+    const double r2 = rad[i] * rad[i], r3 = r2 * rad[i];
+    tot_rad3 += r3;
+    for (int k = 0; k < 3; k++) center[k] += x[3 * i + k] * r3;
+  }
+  for (int k = 0; k < 3; k++) center[k] /= tot_rad3;
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) I[a][b] = 0.0;
+  for (int i = 0; i < nat; i++) {
+    const double r2 = rad[i] * rad[i], r3 = r2 * rad[i];
+    const double v[3] = {x[3 * i] - center[0], x[3 * i + 1] - center[1], x[3 * i + 2] - center[2]};
+    const double vv = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    const double diag = r3 * (vv + 0.4 * r2);
+    for (int k = 0; k < 3; k++) I[k][k] += diag;
+    for (int col = 0; col < 3; col++)
+      for (int row = 0; row < 3; row++) I[row][col] -= r3 * v[col] * v[row];
+  }
+}
+inline double solv_det3(const double a[3][3]) {
+  return a[0][0] * a[1][1] * a[2][2] - a[0][0] * a[1][2] * a[2][1] - a[0][1] * a[1][0] * a[2][2] +
+         a[0][1] * a[1][2] * a[2][0] + a[0][2] * a[1][0] * a[2][1] - a[0][2] * a[1][1] * a[2][0];
+}
+double solv_adet(const vec& x, const vec& rad) {
+  double I[3][3], c[3], t;
+  solv_inertia(x, rad, I, c, t);
+  return std::sqrt(std::pow(solv_det3(I), 1.0 / 3.0) / (0.4 * t));
+}
+// gradient of keps alpbet / (2 aDet) (sum q)^2 (kernel.rs:211-238)
+void solv_adet_deriv(const vec& x, const vec& rad, double keps_alpbet, const vec& q, vec& g) {
+  double qt = 0.0;
+  for (double v : q) qt += v;
+  double i[3][3], center[3], t;
+  solv_inertia(x, rad, i, center, t);
+  const double adet = std::sqrt(std::pow(solv_det3(i), 1.0 / 3.0) / (0.4 * t));
+  const double t2 = t * t, a2 = adet * adet;
+  const double fa = 250.0 / (48.0 * (t2 * t) * ((a2 * a2) * adet));
+  const double fb = -0.5 * keps_alpbet * (qt * qt) / a2;
+  auto f = [&](double v) { return v * fa * fb; };
+  const double xy = i[0][1] * i[2][2] - i[0][2] * i[1][2], xz = i[0][2] * i[1][1] - i[0][1] * i[2][1];
+  const double yz = i[0][0] * i[1][2] - i[0][1] * i[0][2];
+  const double xx = i[0][0] * (i[1][1] + i[2][2]) - i[0][1] * i[0][1] - i[0][2] * i[0][2];
+  const double yy = i[1][1] * (i[0][0] + i[2][2]) - i[0][1] * i[0][1] - i[1][2] * i[1][2];
+  const double zz = i[2][2] * (i[0][0] + i[1][1]) - i[0][2] * i[0][2] - i[1][2] * i[1][2];
+  const double d[3][3] = {{f(xx), f(xy), f(xz)}, {f(xy), f(yy), f(yz)}, {f(xz), f(yz), f(zz)}};
+  for (size_t a = 0; a < rad.size(); a++) {
+    const double r3 = rad[a] * rad[a] * rad[a];
+    const double v[3] = {x[3 * a] - center[0], x[3 * a + 1] - center[1], x[3 * a + 2] - center[2]};
+    for (int k = 0; k < 3; k++) g[3 * a + k] += r3 * (d[k][0] * v[0] + d[k][1] * v[1] + d[k][2] * v[2]);
+  }
+}
+
+// geometry-dependent intermediates (lib.rs:555-613)
+void solv_update(const axtb_solv::SolvModel& S, const vec& x, SolvCache& c) {
+  using namespace axtb_solv;
+  const int nat = (int)S.vdwr.size();
+  solv_born_radii(S, x, c);
+  solv_sasa(S, x, c);
+  c.gsasa = 0.0;
+  for (int i = 0; i < nat; i++) c.gsasa += c.sasa[i] * S.gamsasa[i];
+  c.hbw.assign(nat, 0.0);
+  c.dhbdw.assign(nat, 0.0);
+  if (S.lhb)
+    for (int i = 0; i < nat; i++) {
+      const double smaxd = 1.0 / (S.vdwsa[i] * S.vdwsa[i]);
+      c.hbw[i] = S.hbmag[i] * c.sasa[i] * smaxd;
+      c.dhbdw[i] = S.hbmag[i] * smaxd;
+    }
+  c.adet = S.alpbet > 0.0 ? solv_adet(x, S.vdwr) : 0.0;
+  // Born matrix: P16 (ALPB) or Still (GBSA) kernel, self terms keps / a_i (kernel.rs:28-70)
+  c.B.assign((size_t)nat * nat, 0.0);
+  const double z16 = ZETA_P16 / 16.0;
+  for (int i = 0; i < nat; i++)
+    for (int j = 0; j < i; j++) {
+      double d[3], r2, r;
+      solv_pair_vec(x, i, j, d, r2, r);
+      double v;
+      if (S.alpb) {
+        const double ab = std::sqrt(c.brad[i] * c.brad[j]);
+        double arg = ab / (ab + z16 * r);
+        arg *= arg;
+        arg *= arg;
+        arg *= arg;
+        arg *= arg;
+        v = S.keps * (1.0 / (r + ab * arg));
+      } else {
+        const double rr2 = r * r, aa = c.brad[i] * c.brad[j], dd = 0.25 * rr2 / aa;
+        v = S.keps * (1.0 / std::sqrt(rr2 + aa * std::exp(-dd)));
+      }
+      c.B[(size_t)i * nat + j] = c.B[(size_t)j * nat + i] = v;
+    }
+  for (int i = 0; i < nat; i++) c.B[(size_t)i * nat + i] = S.keps * (1.0 / c.brad[i]);
+  if (S.lhb)
+    for (int i = 0; i < nat; i++) c.B[(size_t)i * nat + i] += 2.0 * c.hbw[i];
+  if (S.alpbet > 0.0) {
+    const double shift = S.keps * S.alpbet / c.adet;
+    for (double& b : c.B) b += shift;
+  }
+}
+
+// (B q)_i in the reference's dot order (lib.rs:760-779: four interleaved partial sums)
+inline double solv_row_dot(const double* row, const double* q, int n) {
+  double acc[4] = {0, 0, 0, 0};
+  int k = 0;
+  for (; k + 4 <= n; k += 4)
+    for (int t = 0; t < 4; t++) acc[t] += row[k + t] * q[k + t];
+  double s = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+  for (; k < n; k++) s += row[k] * q[k];
+  return s;
+}
+
+// explicit gradient of 1/2 q^T B q at fixed charges (lib.rs:688-709, kernel.rs:75-157)
+void solv_gradient(const axtb_solv::SolvModel& S, const vec& x, const SolvCache& c, const vec& q, vec& g) {
+  using namespace axtb_solv;
+  const int nat = (int)S.vdwr.size();
+  vec dedbr(nat, 0.0);
+  const double keps = S.keps;
+  for (int iat = 0; iat < nat; iat++) {
+    double gi[3] = {0, 0, 0}, di = 0.0;
+    for (int jat = 0; jat < iat; jat++) {
+      double v[3], r2, r;
+      solv_pair_vec(x, iat, jat, v, r2, r);
+      const double qq = q[iat] * q[jat];
+      if (S.alpb) {
+        const double ab = std::sqrt(c.brad[iat] * c.brad[jat]);
+        const double arg1 = ab / (ab + ZETA_P16 / 16.0 * r);
+        double arg16 = arg1 * arg1;
+        arg16 *= arg16;
+        arg16 *= arg16;
+        arg16 *= arg16;
+        const double fgb = r + ab * arg16, dfgb = 1.0 / fgb, dfgb2 = dfgb * dfgb;
+        const double ap = (1.0 - ZETA_P16 * arg1 * arg16) * dfgb2;
+        for (int k = 0; k < 3; k++) {
+          const double dg = ap * v[k] * keps / r * qq;
+          gi[k] -= dg;
+          g[3 * jat + k] += dg;
+        }
+        const double bp = -0.5 * (r * ZETA_P16 / ab * arg1 + 1.0) / ab * arg16 * dfgb2;
+        di += c.brad[jat] * bp * keps * qq;
+        dedbr[jat] += c.brad[iat] * bp * keps * qq;
+      } else {
+        const double rr2 = r * r, aa = c.brad[iat] * c.brad[jat], dd = 0.25 * rr2 / aa, expd = std::exp(-dd);
+        const double fgb2 = rr2 + aa * expd, dfgb2 = 1.0 / fgb2, dfgb = std::sqrt(dfgb2);
+        const double dfgb3 = dfgb2 * dfgb * keps, ap = (1.0 - 0.25 * expd) * dfgb3;
+        for (int k = 0; k < 3; k++) {
+          const double dr = ap * v[k];
+          gi[k] -= dr * qq;
+          g[3 * jat + k] += dr * qq;
+        }
+        const double bp = -0.5 * expd * (1.0 + dd) * dfgb3;
+        di += c.brad[jat] * bp * qq;
+        dedbr[jat] += c.brad[iat] * bp * qq;
+      }
+    }
+    for (int k = 0; k < 3; k++) g[3 * iat + k] += gi[k];
+    dedbr[iat] += di;
+  }
+  for (int i = 0; i < nat; i++) {
+    const double bp = 1.0 / c.brad[i], qq = q[i] * bp;
+    dedbr[i] += -0.5 * keps * qq * bp * q[i];
+  }
+  vec work(nat);
+  for (int i = 0; i < nat; i++) work[i] = dedbr[i] * c.dbrad[i];
+  solv_psi_gradient(S, x, work, g);
+  if (S.lhb) {
+    for (int i = 0; i < nat; i++) work[i] = c.dhbdw[i] * q[i] * q[i];
+    solv_sasa_contract(c, work, g);
+  }
+  if (S.alpbet > 0.0) solv_adet_deriv(x, S.vdwr, S.keps * S.alpbet, q, g);
+}
+
+// energy parts as xtb prints them: gsolv, gelec, gsasa, ghb, gshift (lib.rs:711-735)
+void solv_parts(const axtb_solv::SolvModel& S, const SolvCache& c, const vec& q, double out[5]) {
+  const int nat = (int)q.size();
+  double ghb = 0.0, epol = 0.0;
+  if (S.lhb)
+    for (int i = 0; i < nat; i++) ghb += c.hbw[i] * (q[i] * q[i]);
+  for (int i = 0; i < nat; i++) epol += q[i] * (0.5 * solv_row_dot(&c.B[(size_t)i * nat], q.data(), nat));
+  const double gelec = epol - ghb;
+  out[0] = gelec + c.gsasa + ghb + S.gshift;
+  out[1] = gelec;
+  out[2] = c.gsasa;
+  out[3] = ghb;
+  out[4] = S.gshift;
+}
+
 // ------------------------------------------------------------------ the calculation (xtb.rs:437-861)
 struct Options {
   double acc = 1.0, etemp = 300.0, damping = 0.4;
@@ -1861,11 +2294,15 @@ struct Options {
   // a given start (ScfGuess, xtb.rs:204-208): shell charges [nsh] (null: none), dipoles [3N],
   // quadrupoles [6N] (null: zero)
   const double *gq = nullptr, *gdp = nullptr, *gqp = nullptr;
+  // implicit solvation (SolvationSpec, solvation.rs:14-22): model 0 none / 1 ALPB / 2 GBSA, the
+  // solvent's name, reference state; the axtb conventions' corrected model (conventions.rs:256)
+  int solv_model = 0, solv_state = 0;
+  const char* solvent = nullptr;
 };
 
 struct Result {
   double energy = 0;
-  double terms[8] = {0};  // repulsion dispersion electronic isotropic_es third_order anisotropic entropy band
+  double terms[9] = {0};  // repulsion dispersion electronic isotropic_es third_order anisotropic entropy band solvation
   vec grad, qat, qsh, dp, qp, emo, focc;
   int iterations = 0, nao = 0, nsh = 0, converged = 0;
   double efermi = 0, residual = 0, energy_change = 0;
@@ -1952,6 +2389,24 @@ Result singlepoint(int nat, const int* z, const double* xyz, double charge, cons
     e3sum += e3b[i];
   }
   for (int i = 0; i < nat; i++) energies[i] += erep[i];
+  // implicit solvation: the surface and shift energies are charge independent (lib.rs:615-636)
+  axtb_solv::SolvModel solv;
+  SolvCache sc;
+  const bool solvated = opt.solv_model != 0;
+  double esolv_fixed = 0.0;
+  if (solvated) {
+    std::string err;
+    if (!axtb_solv::build_model(nat, z, opt.solv_model, opt.solvent, opt.solv_state, axtb_solv::CONV_CORRECTED,
+                                axtb_solv::DEFAULT_NANG, axtb_solv::DEFAULT_TEMPERATURE, solv, err))
+      throw Error(1, "solvation: " + err);
+    solv_update(solv, m.xyz, sc);
+    const double shift = solv.gshift / nat;
+    for (int i = 0; i < nat; i++) {
+      const double e = sc.sasa[i] * solv.gamsasa[i] + shift;
+      energies[i] += e;
+      esolv_fixed += e;
+    }
+  }
 
   Ints I = build_integrals(m, se);
   {
@@ -2088,6 +2543,8 @@ Result singlepoint(int nat, const int* z, const double* xyz, double charge, cons
     for (int a = 0; a < nsh; a++) pot.vsh[a] += st.qsh[a] * st.qsh[a] * cou.gamma3[a];
     aes_potentials(m, cou, st, pot);
     if (opt.dispersion) d4_energy_potential(d4, st.qat, nullptr, &pot.vat);
+    if (solvated)  // B q of the input charges (lib.rs:653-661)
+      for (int i = 0; i < nat; i++) pot.vat[i] += solv_row_dot(&sc.B[(size_t)i * nat], st.qat.data(), nat);
     for (int mu = 0; mu < n; mu++) vao[mu] = pot.vsh[m.ao2sh[mu]] + pot.vat[m.ao2at[mu]];
     // Fock (scf/fock.rs:22-104)
 #pragma omp parallel for schedule(static)
@@ -2199,12 +2656,19 @@ Result singlepoint(int nat, const int* z, const double* xyz, double charge, cons
     for (int a = 0; a < nsh; a++) e3[m.sh2at[a]] += std::pow(out.qsh[a], 3) * cou.gamma3[a] / 3.0;
     aes_energies(m, cou, out, eaes);
     if (opt.dispersion) d4_energy_potential(d4, out.qat, &ed, nullptr);
+    vec esq(nat, 0.0);
+    double ssolv = 0.0;
+    if (solvated)  // 1/2 q_i (B q)_i of the output charges (lib.rs:640-650)
+      for (int i = 0; i < nat; i++) {
+        esq[i] = out.qat[i] * (0.5 * solv_row_dot(&sc.B[(size_t)i * nat], out.qat.data(), nat));
+        ssolv += esq[i];
+      }
     double se3 = 0, saes = 0, sd = 0;
     for (int i = 0; i < nat; i++) {
       se3 += e3[i];
       saes += eaes[i];
       sd += ed[i];
-      eelec[i] += e3[i] + eaes[i] + ed[i] + ts / nat + eband[i];
+      eelec[i] += e3[i] + eaes[i] + ed[i] + esq[i] + ts / nat + eband[i];
     }
     double energy = 0.0;
     for (int i = 0; i < nat; i++) energy += eelec[i];
@@ -2222,6 +2686,7 @@ Result singlepoint(int nat, const int* z, const double* xyz, double charge, cons
     res.terms[5] = saes;
     res.terms[6] = ts;
     res.terms[7] = band;
+    res.terms[8] = esolv_fixed + ssolv;
     // convergence (scf/convergence.rs:99-121)
     if (std::fabs(de_last) < 1e-6 * opt.acc && res_norm < 2e-5 * opt.acc) {
       converged = true;
@@ -2281,6 +2746,11 @@ Result singlepoint(int nat, const int* z, const double* xyz, double charge, cons
     es2_gradient(m, cou, out.qsh, g);
     aes_gradient(m, cou, out, g, dedcn);
     if (opt.dispersion) d4_gradient(d4, out.qat, g);
+    if (solvated) {
+      // the charge-dependent part at the output charges, and the surface term (lib.rs:615-636,688-709)
+      solv_gradient(solv, m.xyz, sc, out.qat, g);
+      solv_sasa_contract(sc, solv.gamsasa, g);
+    }
     // H0/S/multipole gradient (hamiltonian.rs:137-223)
     // atom pairs by the first atom over the threads, each with its own sums, added in thread order
     const int nt = std::max(1, omp_get_max_threads());
@@ -2436,11 +2906,17 @@ int oracle_engrad_guess(int nat, const int* z, const double* xyz, double charge,
 // ScfNotConverged), mixer used (0 broyden, 1 diis, 2 diis-fallback), attempts, attempt 1
 // iterations, attempt 1 converged, attempt 2 iterations; info[6] = attempt 2 converged.
 // scal[0..4] = efermi, dipole x y z, HOMO-LUMO gap.
+// Implicit solvation: solv_model 0 none, 1 ALPB, 2 GBSA; the solvent's name; solv_state 0 gsolv,
+// 1 bar1M, 2 reference. terms[9]: ... band, solvation.
 int oracle_singlepoint(int nat, const int* z, const double* xyz, double charge, double acc, double etemp,
                        int max_iter, int guess, int want_grad, int dispersion, int mixer, const double* gq,
                        const double* gdp, const double* gqp, double* energy, double* terms, double* grad, double* qat,
-                       double* qsh, double* dp, double* qp, double* emo, int* info, double* scal) {
+                       double* qsh, double* dp, double* qp, double* emo, int* info, double* scal, int solv_model,
+                       const char* solvent, int solv_state) {
   Options o;
+  o.solv_model = solv_model;
+  o.solvent = solvent;
+  o.solv_state = solv_state;
   o.gq = gq;
   o.gdp = gdp;
   o.gqp = gqp;
@@ -2493,6 +2969,35 @@ int oracle_singlepoint(int nat, const int* z, const double* xyz, double charge, 
     scal[4] = r.gap;
   }
   return 0;
+}
+
+// The solvation model alone at given charges (positions in bohr): out[5] = gsolv, gelec, gsasa, ghb,
+// gshift as xtb prints them; Born radii and surface areas (bohr^2 / 4 pi) per atom if not null; the
+// gradient of gsolv at fixed charges (3N) if not null. conv: 0 xtb 6.8.0, 1 xtb 6.7.1, 2 corrected.
+int oracle_solv_parts(int nat, const int* z, const double* xyz, const double* q, int model, const char* solvent,
+                      int state, int conv, int nang, double* out, double* brad, double* sasa, double* grad) {
+  try {
+    axtb_solv::SolvModel S;
+    std::string err;
+    if (!axtb_solv::build_model(nat, z, model, solvent, state, conv, nang, axtb_solv::DEFAULT_TEMPERATURE, S, err))
+      throw Error(1, err);
+    vec x(xyz, xyz + 3 * nat), qq(q, q + nat);
+    SolvCache c;
+    solv_update(S, x, c);
+    solv_parts(S, c, qq, out);
+    if (brad) std::memcpy(brad, c.brad.data(), sizeof(double) * nat);
+    if (sasa) std::memcpy(sasa, c.sasa.data(), sizeof(double) * nat);
+    if (grad) {
+      vec g(3 * nat, 0.0);
+      solv_gradient(S, x, c, qq, g);
+      solv_sasa_contract(c, S.gamsasa, g);
+      std::memcpy(grad, g.data(), sizeof(double) * 3 * nat);
+    }
+    return 0;
+  } catch (Error& e) {
+    last_error = e.what();
+    return e.code;
+  }
 }
 
 // Many molecules at once, one molecule per OpenMP thread (inner loops serial): the CPU baseline
