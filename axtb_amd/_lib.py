@@ -13,7 +13,8 @@ LIBPATH = os.path.join(HERE, "libaxtb_hip.so")
 
 OK, ERR_INPUT, ERR_NUMERICAL, ERR_SCF, ERR_ELEMENT, ERR_DEVICE = range(6)
 ERROR_KINDS = {1: "input", 2: "numerical", 3: "scf_not_converged", 4: "unsupported_element", 5: "device"}
-TERMS = ("repulsion", "dispersion", "electronic", "isotropic_es", "third_order", "anisotropic", "entropy", "band")
+TERMS = ("repulsion", "dispersion", "electronic", "isotropic_es", "third_order", "anisotropic", "entropy", "band",
+         "solvation")
 EXPORTS = ("axtbhip_version", "axtbhip_context_create", "axtbhip_context_destroy", "axtbhip_last_error",
            "axtbhip_basis_size", "axtbhip_gfn2_engrad", "axtbhip_gfn2_engrad_guess", "axtbhip_gfn2_engrad_batch",
            "axtbhip_gfn2_engrad_batch_guess", "axtbhip_batch_adjoint", "axtbhip_gfn2_engrad_noalloc",
@@ -44,7 +45,8 @@ class Guess(ctypes.Structure):
 class Opts(ctypes.Structure):
     _fields_ = [("accuracy", ctypes.c_double), ("etemp", ctypes.c_double), ("max_iter", ctypes.c_int),
                 ("damping", ctypes.c_double), ("guess", ctypes.c_int), ("want_gradient", ctypes.c_int),
-                ("precision", ctypes.c_int), ("dispersion", ctypes.c_int), ("mixer", ctypes.c_int)]
+                ("precision", ctypes.c_int), ("dispersion", ctypes.c_int), ("mixer", ctypes.c_int),
+                ("solvation", ctypes.c_int), ("solvent", ctypes.c_char_p), ("solvation_state", ctypes.c_int)]
 
 
 MIXERS = {"auto": 0, "broyden": 1, "diis": 2}
@@ -60,7 +62,7 @@ class HessianOut(ctypes.Structure):
 class Result(ctypes.Structure):
     _fields_ = [("energy", ctypes.c_double), ("gradient", ctypes.c_void_p), ("charges", ctypes.c_void_p),
                 ("shell_charges", ctypes.c_void_p), ("dipoles", ctypes.c_void_p), ("quadrupoles", ctypes.c_void_p),
-                ("orbital_energies", ctypes.c_void_p), ("terms", ctypes.c_double * 8), ("efermi", ctypes.c_double),
+                ("orbital_energies", ctypes.c_void_p), ("terms", ctypes.c_double * 9), ("efermi", ctypes.c_double),
                 ("residual", ctypes.c_double), ("energy_change", ctypes.c_double), ("iterations", ctypes.c_int),
                 ("status", ctypes.c_int), ("nsh", ctypes.c_int), ("nao", ctypes.c_int),
                 ("dipole", ctypes.c_double * 3), ("gap", ctypes.c_double), ("mixer_used", ctypes.c_int),
@@ -118,10 +120,18 @@ def lib():
 PRECISIONS = {"fp64": 0, "mixed": 1, "fp64-refine": 2}
 
 
+# implicit solvation (axtb_opts.solvation, .solvation_state)
+SOLVATION = {None: 0, "alpb": 1, "gbsa": 2}
+SOLVATION_STATES = {"gsolv": 0, "bar1M": 1, "bar1m": 1, "reference": 2}
+
+
 def make_opts(accuracy=1.0, etemp=300.0, max_iter=250, damping=0.4, guess="eeq", gradient=True, precision="fp64",
-              dispersion=True, mixer="auto"):
+              dispersion=True, mixer="auto", solvation=None, solvent=None, state="gsolv"):
+    """The options of a calculation; ``solvation`` None, "alpb" or "gbsa" with ``solvent`` (a name
+    xtb accepts) and the reference ``state`` (gsolv, bar1M, reference)."""
     return Opts(float(accuracy), float(etemp), int(max_iter), float(damping), 0 if guess == "eeq" else 1,
-                int(bool(gradient)), PRECISIONS[precision], int(bool(dispersion)), MIXERS[mixer])
+                int(bool(gradient)), PRECISIONS[precision], int(bool(dispersion)), MIXERS[mixer],
+                SOLVATION[solvation], (solvent or "").encode(), SOLVATION_STATES[state])
 
 
 class Context:

@@ -26,6 +26,7 @@
 #include "kernels.hip"
 #include "linalg.hip"
 #include "dense_eigh.hip"
+#include "solvation.hip"
 
 namespace axtb {
 
@@ -310,7 +311,7 @@ struct MolOut {
   int status = 0;
   std::string msg;
   double energy = 0, efermi = 0, residual = 0, energy_change = 0;
-  double terms[8] = {0};
+  double terms[9] = {0};
   int iterations = 0, nsh = 0, nao = 0;
   std::vector<double> grad, qat, qsh, dp, qp, emo;
   double dipole[3] = {0, 0, 0}, gap = 0;
@@ -995,6 +996,15 @@ struct Engine {
     if (o.mixer < AXTB_MIXER_AUTO || o.mixer > AXTB_MIXER_DIIS) throw Fail{AXTB_ERR_INPUT, "unknown SCF mixer"};
     if (o.precision < AXTB_PREC_FP64 || o.precision > AXTB_PREC_FP64_REFINE)
       throw Fail{AXTB_ERR_INPUT, "unknown precision"};
+    if (o.solvation < AXTB_SOLV_NONE || o.solvation > AXTB_SOLV_GBSA) throw Fail{AXTB_ERR_INPUT, "unknown solvation model"};
+    if (o.solvation != AXTB_SOLV_NONE) {
+      if (o.solvation_state < AXTB_SOLV_GSOLV || o.solvation_state > AXTB_SOLV_REFERENCE)
+        throw Fail{AXTB_ERR_INPUT, "unknown solvation reference state"};
+      if (axtb_solv::find_set(o.solvent, o.solvation) < 0)
+        throw Fail{AXTB_ERR_INPUT, std::string("solvation: solvent '") + (o.solvent ? o.solvent : "") +
+                                       "' is not parametrized for GFN2-xTB/" +
+                                       (o.solvation == AXTB_SOLV_ALPB ? "ALPB" : "GBSA")};
+    }
     resident = false;
     out.assign(in.size(), MolOut());
     std::vector<int> use;  // input index of each batch molecule
@@ -1060,7 +1070,10 @@ struct Engine {
       std::stable_sort(use.begin(), use.end(), [&](int a, int b) { return nao[a] > nao[b]; });
     }
     build(in, use, o, out);
+    // the options outlive the call: the solvent's name is kept here
+    solvent_name = o.solvent ? o.solvent : "";
     opts_ = o;
+    opts_.solvent = solvent_name.c_str();
   }
 
   // the prepared batch on the device: setup, SCC and gradient (inputs resident in HBM)
@@ -1142,6 +1155,7 @@ struct Engine {
     resident = true;
   }
   axtb_opts opts_{};
+  std::string solvent_name;
 
   // ------------------------------------------------------------------------------ tables
   std::vector<int> hz, hsp, hmol, hish, hnsh, hnref, hngw, hsat, hsl, hsnp, hsiao, haosh, haoat;
@@ -1390,6 +1404,47 @@ struct Engine {
       g_qp.upload(hgqp, st);
     }
     intcut_ = std::max(25.0 - 10.0 * std::log10(o.accuracy), 20.0);  // basis.rs:135-137
+    // implicit solvation: per-atom parameters of the model of every molecule (one solvent for the
+    // batch; the axtb conventions' corrected model, xtb's default grid and temperature)
+    solv_model = o.solvation;
+    if (solv_model != AXTB_SOLV_NONE) {
+      std::vector<double> hv[9];
+      for (const Mol& M : mols) {
+        axtb_solv::SolvModel sm;
+        std::string err;
+        if (!axtb_solv::build_model(M.nat, &hz[M.at0], o.solvation, o.solvent, o.solvation_state,
+                                    axtb_solv::CONV_CORRECTED, axtb_solv::DEFAULT_NANG, axtb_solv::DEFAULT_TEMPERATURE,
+                                    sm, err))
+          throw Fail{AXTB_ERR_INPUT, "solvation: " + err};
+        const std::vector<double>* src[9] = {&sm.vdwr, &sm.rho, &sm.svdw, &sm.vdwsa, &sm.trj2a,
+                                             &sm.trj2b, &sm.wrp, &sm.gamsasa, &sm.hbmag};
+        for (int k = 0; k < 9; k++) hv[k].insert(hv[k].end(), src[k]->begin(), src[k]->end());
+        sm.vdwr.clear();  // the scalars and the grid are the same for every molecule
+        solv0 = sm;
+      }
+      DBuf<double>* dst[9] = {&sv_vdwr, &sv_rho, &sv_svdw, &sv_vdwsa, &sv_trj2a, &sv_trj2b, &sv_wrp, &sv_gam, &sv_hbmag};
+      for (int k = 0; k < 9; k++) dst[k]->upload(hv[k], st);
+      std::vector<double> hg;
+      for (size_t p = 0; p < solv0.weight.size(); p++) {
+        for (int k = 0; k < 3; k++) hg.push_back(solv0.grid[3 * p + k]);
+        hg.push_back(solv0.weight[p]);
+      }
+      sv_grid.upload(hg, st);
+    }
+  }
+  // implicit solvation (ALPB / GBSA): the model (AXTB_SOLV_*), the scalars and grid of the batch's
+  // solvent, per-atom parameters (build), geometry-dependent intermediates (setup)
+  int solv_model = AXTB_SOLV_NONE;
+  axtb_solv::SolvModel solv0;
+  DBuf<double> sv_vdwr, sv_rho, sv_svdw, sv_vdwsa, sv_trj2a, sv_trj2b, sv_wrp, sv_gam, sv_hbmag, sv_grid;
+  DBuf<double> sv_brad, sv_dbrad, sv_sasa, sv_wsa, sv_efix, sv_adet, sv_B, sv_work;
+  DBuf<int> sv_fail;
+  SolvDev solv_dev() const {
+    const axtb_solv::SolvModel& S = solv0;
+    return SolvDev{sv_vdwr.p, sv_rho.p, sv_svdw.p, sv_vdwsa.p, sv_trj2a.p, sv_trj2b.p, sv_wrp.p, sv_gam.p,
+                   sv_hbmag.p, sv_grid.p, (int)S.weight.size(), S.alpb ? 1 : 0, S.keps, S.alpbet, S.born_scale,
+                   S.lrcut * S.lrcut, S.w, S.ah0, S.ah1, S.ah3, S.gshift, sv_brad.p, sv_dbrad.p, sv_sasa.p,
+                   sv_wsa.p, sv_efix.p, sv_adet.p, sv_B.p, sv_work.p};
   }
   int nsp_ = 0;
   double intcut_ = 25.0;
@@ -1515,6 +1570,19 @@ struct Engine {
     }
     // repulsion
     LAUNCH(k_repulsion, dim3(blocks(NA, 4)), dim3(256), 0, b, (const int*)nullptr, erep.p, grep.p);
+    // implicit solvation: Born radii, surfaces, aDet, Born matrices (solvation lib.rs:555-613)
+    if (solv_model != AXTB_SOLV_NONE) {
+      for (auto* d : {&sv_brad, &sv_dbrad, &sv_sasa, &sv_efix, &sv_work}) d->ensure(NA);
+      sv_wsa.ensure((size_t)NA * solv0.weight.size());
+      sv_adet.ensure(NM);
+      sv_B.ensure(tot_pair);
+      sv_fail.upload(zeros, st);
+      const SolvDev S = solv_dev();
+      LAUNCH(k_solv_born, dim3(blocks(NA, 4)), dim3(256), 0, b, S);
+      LAUNCH(k_solv_sasa, dim3(NA), dim3(64), 0, b, S, sv_fail.p);
+      if (S.alpb) LAUNCH(k_solv_adet, dim3(NM), dim3(256), 0, b, S);
+      LAUNCH(k_solv_bmat, dim3(blocks((long long)natmax * natmax, 256), NM), dim3(256), 0, b, S);
+    }
     // D4 three-body at q = 0 (weights with dgw/dCN)
     fill(fwork.p, 0);
     DBuf<double>& zq = eeqy;  // zero charges
@@ -1965,6 +2033,8 @@ struct Engine {
     LAUNCH(k_attempt_start, dim3(blocks(NM, 256)), dim3(256), 0, (const int*)dmask.p, NM, active.p, itc.p, status.p,
            mst.p, mixer == MIX_DIIS ? ddst.p : (double*)nullptr, dist.p, o.damping);
     hist_slots = 0;
+    if (solv_model != AXTB_SOLV_NONE)
+      LAUNCH(k_solv_flag, dim3(blocks(NM, 256)), dim3(256), 0, (const int*)sv_fail.p, NM, active.p, status.p);
     // start moments -> q_in (dipoles and quadrupoles zero); a given start replaces them
     fill(dp.p, 3LL * NA);
     fill(qp.p, 6LL * NA);
@@ -2052,6 +2122,8 @@ struct Engine {
       if (!o.dispersion) fill(gw.p, MAXREF * (long long)NA);
       LAUNCH(k_atom_pot, dim3(blocks(NA, 4)), dim3(256), 0, b, active.p, mrad.p, qat.p, dp.p, qp.p,
                          gw.p, dgw.p, vat.p, vdp.p, vqp.p);
+      if (solv_model != AXTB_SOLV_NONE)  // the solvation potential B q of the input charges
+        LAUNCH(k_solv_pot, dim3(blocks(NA, 4)), dim3(256), 0, b, active.p, solv_dev(), (const double*)qat.p, vat.p);
       // HBM bytes of the streaming kernels: Fock reads the lower triangles of S, H0, D, Q and writes
       // F (11 n (n + 1) / 2 + n^2 doubles); Mulliken reads P, S, D, Q, H0 (12 n^2 doubles)
       double sbytes = 0.0, fbytes = 0.0;
@@ -2201,6 +2273,9 @@ struct Engine {
       if (!o.dispersion) fill(gw.p, MAXREF * (long long)NA);
       LAUNCH(k_energy_atoms, dim3(blocks(NA, 4)), dim3(256), 0, b, active.p, mrad.p, gq.p, qsh_o.p,
                          qat_o.p, dp_o.p, qp_o.p, gw.p, dgw.p, eband.p, fst.p, eel.p, eterm.p);
+      if (solv_model != AXTB_SOLV_NONE)  // q (B q) / 2 of the output charges
+        LAUNCH(k_solv_energy, dim3(blocks(NA, 4)), dim3(256), 0, b, active.p, solv_dev(), (const double*)qat_o.p,
+               eel.p, eterm.p);
       LAUNCH(k_converge, dim3(NM), dim3(256), 0, b, active.p, eel.p, dq.p, fst.p, econv, pconv, mst.p,
                          status.p);
       HIP_OK(hipGetLastError());
@@ -2389,6 +2464,16 @@ struct Engine {
            dcn3.p, (const double*)atmP.p, (long long)tot_pair, grad.p, dedcn.p, dedcn4.p);
     LAUNCH(k_cn_chain, dim3(blocks(NA, 4)), dim3(256), 0, b, (const int*)okd.p, dedcn.p, dedcn4.p,
                        grad.p);
+    if (solv_model != AXTB_SOLV_NONE) {
+      // solvation at the output charges: kernel and Born-radius chain, surface (with the H-bond
+      // term), ALPB shape (lib.rs:688-709, 627-636)
+      const SolvDev S = solv_dev();
+      LAUNCH(k_solv_grad_pairs, dim3(blocks(NA, 4)), dim3(256), 0, b, (const int*)okd.p, S, (const double*)qat_o.p,
+             grad.p);
+      LAUNCH(k_solv_grad_psi, dim3(blocks(NA, 4)), dim3(256), 0, b, (const int*)okd.p, S, grad.p);
+      LAUNCH(k_solv_grad_sasa, dim3(NA), dim3(64), 0, b, (const int*)okd.p, S, (const double*)qat_o.p, grad.p);
+      if (S.alpb) LAUNCH(k_solv_grad_adet, dim3(NM), dim3(256), 0, b, (const int*)okd.p, S, (const double*)qat_o.p, grad.p);
+    }
     HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(st));
   }
@@ -2405,6 +2490,7 @@ struct Engine {
     auto er = get(erep, NA), e3h = get(e3, NA), eelh = get(eel, NA), et = get(eterm, 8 * (size_t)NA);
     auto qa = get(qat_o, NA), qs = get(qsh_o, NS), d = get(dp_o, 3 * (size_t)NA), q = get(qp_o, 6 * (size_t)NA);
     auto e = get(w, NO);
+    auto ef = solv_model != AXTB_SOLV_NONE ? get(sv_efix, NA) : std::vector<double>(NA, 0.0);
     auto g = grad_done ? get(grad, 3 * (size_t)NA) : std::vector<double>();
     std::vector<int> hst(NM), hef(NM);
     HIP_OK(hipMemcpyAsync(hst.data(), status.p, sizeof(int) * NM, hipMemcpyDeviceToHost, st));
@@ -2429,7 +2515,10 @@ struct Engine {
       r.residual = s[2];
       r.energy_change = s[1];
       r.efermi = s[3];
-      if (hst[m] & 2) {
+      if (hst[m] & 32) {
+        r.status = AXTB_ERR_NUMERICAL;
+        r.msg = "solvation: more atoms than the surface kernel takes within the surface cutoff of one atom";
+      } else if (hst[m] & 2) {
         r.status = AXTB_ERR_NUMERICAL;
         r.msg = "Fermi level not found";
       } else if (hst[m] & 4) {
@@ -2445,13 +2534,15 @@ struct Engine {
         r.status = AXTB_ERR_SCF_NOT_CONVERGED;
         r.msg = "SCF not converged in " + std::to_string(r.iterations) + " iterations";
       }
-      double etot = 0.0, erp = 0.0, ed3 = 0.0, eelt = 0.0, t[5] = {0, 0, 0, 0, 0};
+      double etot = 0.0, erp = 0.0, ed3 = 0.0, eelt = 0.0, esf = 0.0, t[6] = {0, 0, 0, 0, 0, 0};
+      const int nt = solv_model != AXTB_SOLV_NONE ? 6 : 5;
       for (int a = M.at0; a < M.at0 + M.nat; a++) {
-        etot += er[a] + e3h[a] + eelh[a];
+        etot += er[a] + e3h[a] + ef[a] + eelh[a];
         erp += er[a];
         ed3 += e3h[a];
+        esf += ef[a];
         eelt += eelh[a];
-        for (int k = 0; k < 5; k++) t[k] += et[8 * a + k];
+        for (int k = 0; k < nt; k++) t[k] += et[8 * a + k];
       }
       r.energy = etot;
       // terms: repulsion dispersion electronic isotropic_es third_order anisotropic entropy band
@@ -2463,6 +2554,9 @@ struct Engine {
       r.terms[5] = t[2];
       r.terms[6] = s[4];
       r.terms[7] = t[4];
+      // the solvation free energy: its charge-independent part (also in the total beside the
+      // repulsion) and its charge-dependent part (also in the electronic energy), xtb.rs:1094-1119
+      r.terms[8] = esf + t[5];
       r.qat.assign(qa.begin() + M.at0, qa.begin() + M.at0 + M.nat);
       r.qsh.assign(qs.begin() + M.sh0, qs.begin() + M.sh0 + M.nsh);
       r.dp.assign(d.begin() + 3 * M.at0, d.begin() + 3 * (M.at0 + M.nat));
@@ -2552,7 +2646,7 @@ static void fill_result(const axtb::MolOut& r, const axtb_mol& m, axtb_result* o
   out->efermi = r.efermi;
   out->residual = r.residual;
   out->energy_change = r.energy_change;
-  for (int k = 0; k < 8; k++) out->terms[k] = r.terms[k];
+  for (int k = 0; k < 9; k++) out->terms[k] = r.terms[k];
   auto put = [](double* dst, const std::vector<double>& v) {
     if (dst && !v.empty()) std::memcpy(dst, v.data(), v.size() * sizeof(double));
   };

@@ -64,7 +64,22 @@ typedef struct axtb_opts {
    * conventions, conventions.rs:260: Broyden, then DIIS from the start if Broyden does not
    * converge), 1 Broyden alone, 2 DIIS alone */
   int mixer;
+  /* implicit solvation (SolvationSpec, crates/axtb/src/solvation.rs:14-22; the axtb conventions'
+   * corrected ALPB/GBSA model, conventions.rs:256): AXTB_SOLV_NONE, AXTB_SOLV_ALPB or
+   * AXTB_SOLV_GBSA; the solvent's name as xtb accepts it (any case, xtb's aliases; copied by the
+   * call); the reference state AXTB_SOLV_GSOLV, AXTB_SOLV_BAR1M or AXTB_SOLV_REFERENCE. An unknown
+   * solvent or one without parameters for the model is an AXTB_ERR_INPUT of the call. */
+  int solvation;
+  const char* solvent;
+  int solvation_state;
 } axtb_opts;
+
+#define AXTB_SOLV_NONE 0
+#define AXTB_SOLV_ALPB 1
+#define AXTB_SOLV_GBSA 2
+#define AXTB_SOLV_GSOLV 0
+#define AXTB_SOLV_BAR1M 1
+#define AXTB_SOLV_REFERENCE 2
 
 #define AXTB_MIXER_AUTO 0
 #define AXTB_MIXER_BROYDEN 1
@@ -79,8 +94,10 @@ typedef struct axtb_result {
   double* dipoles;          /* atomic dipoles [nat][3] or NULL */
   double* quadrupoles;      /* atomic traceless quadrupoles [nat][6] (xx xy yy xz yz zz) or NULL */
   double* orbital_energies; /* [nao] or NULL */
-  /* repulsion, dispersion, electronic, isotropic_es, third_order, anisotropic, entropy, band */
-  double terms[8];
+  /* repulsion, dispersion, electronic, isotropic_es, third_order, anisotropic, entropy, band,
+   * solvation (EnergyTerms, crates/axtb/src/xtb.rs:109-125: the solvation free energy; its
+   * charge-dependent part is also part of electronic) */
+  double terms[9];
   double efermi, residual, energy_change;
   int iterations, status, nsh, nao;
   /* SinglePoint.dipole: molecular dipole about the origin, e bohr (crates/axtb/src/xtb.rs:746-751) */

@@ -1,0 +1,115 @@
+"""ALPB / GBSA implicit solvation on the GPU, through the C ABI.
+
+- Every solvation case of axtb's reference matrix that the engine takes (H/C/N/O molecules; 25
+  ALPB and 14 GBSA solvents, three reference states, aliases: tests/golden/solvation.json), one
+  batch per solvent and state, against axtb's own results at accuracy 1e-6: energy and the
+  solvation term to 1e-8 Eh, charges to 1e-7, dipole to 1e-6, the same iteration count.
+- Energies and analytic gradients against the CPU oracle on the same inputs (the plain bars
+  |dE| <= 1e-8 Eh, |dg| <= 1e-7 Eh/bohr; mixed precision 1e-6 / 1e-5): caffeine, (H2O)20 and
+  C100H202 (the dense solver) in ALPB water, a batch of the benchmark generator in GBSA methanol.
+- A batch repeated gives the same bits; an unknown solvent is an input error of the call.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, structure
+
+pytestmark = pytest.mark.gpu
+
+SOLV = json.load(open(os.path.join(GOLDEN, "solvation.json")))
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from axtb_amd import _lib
+    return _lib.Context(0)
+
+
+def test_reference_cases(ctx):
+    from axtb_amd import _lib
+    groups = {}
+    for cid, c in sorted(SOLV["cases"].items()):
+        groups.setdefault((c["model"], c["solvent"], c["state"]), []).append(cid)
+    worst = np.zeros(4)
+    n = 0
+    for (model, solvent, state), ids in sorted(groups.items()):
+        mols = []
+        for cid in ids:
+            c = SOLV["cases"][cid]
+            m = SOLV["molecules"][c["molecule"]]
+            mols.append((np.array(m["numbers"]), np.array(m["positions"]), float(c["charge"]),
+                         -1 if c["uhf"] is None else int(c["uhf"])))
+        opts = _lib.make_opts(accuracy=1e-6, gradient=False, solvation=model, solvent=solvent, state=state)
+        rs = ctx.engrad_batch(mols, opts)
+        for cid, r in zip(ids, rs):
+            c = SOLV["cases"][cid]
+            assert r["status"] == 0, (cid, ctx.last_error())
+            d = [abs(r["energy"] - c["energy"]), abs(r["terms"]["solvation"] - c["terms"]["solvation"]),
+                 np.abs(r["charges"] - np.array(c["charges"])).max(), np.abs(r["dipole"] - np.array(c["dipole"])).max()]
+            worst = np.maximum(worst, d)
+            assert d[0] <= 1e-8 and d[1] <= 1e-8 and d[2] <= 1e-7 and d[3] <= 1e-6, (cid, d)
+            assert r["iterations"] == c["iterations"], (cid, r["iterations"], c["iterations"])
+            n += 1
+    print(f"{n} solvated cases vs axtb: max |dE| %.2e |dsolv| %.2e |dq| %.2e |ddip| %.2e" % tuple(worst), flush=True)
+    assert n == len(SOLV["cases"])
+
+
+def _check(ctx, mols, model, solvent, precision="fp64", state="gsolv"):
+    import oracle
+    from axtb_amd import _lib
+    bars = {"fp64": (1e-8, 1e-7), "mixed": (1e-6, 1e-5)}[precision]
+    rs = ctx.engrad_batch(mols, _lib.make_opts(accuracy=1e-2, precision=precision, solvation=model, solvent=solvent,
+                                               state=state))
+    worst = [0.0, 0.0, 0.0]
+    for (z, x), r in zip(mols, rs):
+        assert r["status"] == 0, ctx.last_error()
+        o = oracle.engrad(z, x, acc=1e-2, solvation=model, solvent=solvent, state=state)
+        d = [abs(r["energy"] - o["energy"]), np.abs(r["gradient"] - o["gradient"]).max(),
+             abs(r["terms"]["solvation"] - o["terms"]["solvation"])]
+        worst = [max(a, b) for a, b in zip(worst, d)]
+        assert d[0] <= bars[0] and d[1] <= bars[1] and d[2] <= bars[0], d
+        if precision == "fp64":
+            assert r["iterations"] == o["iterations"]
+    print(f"{model} {solvent} {precision} {len(mols)} molecules: max |dE| %.2e |dg| %.2e |dsolv| %.2e" % tuple(worst),
+          flush=True)
+    return rs
+
+
+def test_against_oracle_small(ctx):
+    from axtb_amd import structures as S
+    z, x = structure("caffeine")
+    _check(ctx, [(z, x), S.config("water20")], "alpb", "water")
+    _check(ctx, [(z, x)], "gbsa", "acetonitrile", state="reference")
+
+
+def test_against_oracle_dense(ctx):
+    """C100H202 (n = 602): the dense eigensolver with the solvation potential."""
+    from axtb_amd import structures as S
+    _check(ctx, [S.config("c100h202")], "alpb", "hexane", state="bar1M")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch(ctx, precision):
+    from axtb_amd import _lib, structures as S
+    mols = S.batch(16, seed=3)
+    rs = _check(ctx, mols, "gbsa", "methanol", precision=precision)
+    again = ctx.engrad_batch(mols, _lib.make_opts(accuracy=1e-2, precision=precision, solvation="gbsa",
+                                                  solvent="methanol"))
+    for a, b in zip(rs, again):
+        assert a["energy"] == b["energy"] and np.array_equal(a["gradient"], b["gradient"])
+
+
+def test_unknown_solvent(ctx):
+    from axtb_amd import _lib
+    z, x = structure("water")
+    with pytest.raises(_lib.AxtbError) as e:
+        ctx.engrad_batch([(z, x)], _lib.make_opts(solvation="alpb", solvent="unobtainium"), raise_on_error=True)
+    assert e.value.code == 1
+    with pytest.raises(_lib.AxtbError):  # aniline has ALPB parameters only
+        ctx.engrad_batch([(z, x)], _lib.make_opts(solvation="gbsa", solvent="aniline"), raise_on_error=True)
+    # the gas phase is unchanged by a solvation call before it
+    r = ctx.engrad_batch([(z, x)], _lib.make_opts(accuracy=1e-6))[0]
+    assert r["terms"]["solvation"] == 0.0
