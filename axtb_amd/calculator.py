@@ -34,6 +34,11 @@ class Options:
     # the SCF start of the next single point (ScfSettings::restart, crates/base/src/method/settings.rs:5-41):
     # "previous" (the default, None) the previous geometry's converged moments, "fresh" the initial guess
     restart: Optional[str] = None
+    # implicit solvation (SolvationSpec, crates/axtb/src/solvation.rs:14-22): None, "alpb" or "gbsa",
+    # the solvent's name (xtb's names and aliases) and the reference state (gsolv, bar1M, reference)
+    solvation: Optional[str] = None
+    solvent: Optional[str] = None
+    solvation_state: str = "gsolv"
 
 
 @dataclass
@@ -94,7 +99,8 @@ class Calculator:
         o = self.options
         opts = _lib.make_opts(accuracy=o.accuracy, etemp=o.etemp, max_iter=o.max_iter, damping=o.damping,
                               guess=o.guess, gradient=(want == Want.GRADIENT), precision=o.precision,
-                              dispersion=o.dispersion)
+                              dispersion=o.dispersion, solvation=o.solvation, solvent=o.solvent,
+                              state=o.solvation_state)
         start = self._take_start()
         r = self.ctx.engrad_batch([(self.numbers, x, self.charge, self.uhf)], opts, starts=[start])[0]
         if r["status"]:

@@ -120,7 +120,8 @@ class Session:
             self._ctx = _lib.Context(self.options.device)
         o = self.options
         opts = _lib.make_opts(accuracy=accuracy, etemp=o.etemp, max_iter=o.max_iter, damping=o.damping,
-                              guess=o.guess, gradient=gradient, precision=o.precision, dispersion=o.dispersion)
+                              guess=o.guess, gradient=gradient, precision=o.precision, dispersion=o.dispersion,
+                              solvation=o.solvation, solvent=o.solvent, state=o.solvation_state)
         mols = [(self.numbers, np.asarray(x, dtype=np.float64).reshape(-1, 3), self.charge, self.uhf)
                 for x in positions]
         res = self._ctx.engrad_batch(mols, opts, starts=list(starts))
