@@ -111,6 +111,7 @@ def lib():
         L.axtbhip_profile_enable.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.axtbhip_profile_report.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]
         L.axtbhip_synchronize.argtypes = [ctypes.c_void_p]
+        L.axtbhip_tuning_set.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -137,11 +138,22 @@ def make_opts(accuracy=1.0, etemp=300.0, max_iter=250, damping=0.4, guess="eeq",
 class Context:
     """A device context (one HIP stream and its buffers) of the engine."""
 
-    def __init__(self, device=0):
+    def __init__(self, device=0, tuning="default"):
+        """tuning: "default" applies the committed autotuner cache (axtb_amd/tuning_gfx950.json) if
+        present, a path applies that cache, None keeps the built-in launch rules."""
         err = ctypes.create_string_buffer(512)
         self._h = lib().axtbhip_context_create(int(device), err, 512)
         if not self._h:
             raise AxtbError(ERR_DEVICE, err.value.decode())
+        if tuning is not None:
+            from . import autotune
+            path = autotune.DEFAULT if tuning == "default" else tuning
+            if tuning != "default" or os.path.exists(path):
+                autotune.apply(self, path)
+
+    def tune(self, knob, variant, bucket, value):
+        """One launch-shape entry of the tuning table (axtbhip_tuning_set); 0 or an error code."""
+        return lib().axtbhip_tuning_set(self._h, knob.encode(), int(variant), int(bucket), int(value))
 
     def close(self):
         if self._h:

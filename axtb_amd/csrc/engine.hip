@@ -537,7 +537,7 @@ struct Engine {
       }
     }
     // products with m, n <= 128: one workgroup per job, 16 x 16 granularity
-    if (gemm_small && same && !two && mm <= 128 && nn <= 128) {
+    if (gemm_small && same && !two && mm <= 128 && nn <= 128 && tune_gemm_small[tune_bucket(std::max(mm, nn))] != 2) {
       gj.upload(jobs, st, &stage);
       const int t = jobs[0].ta * 2 + jobs[0].tb;
       const dim3 grid((unsigned)jobs.size());
@@ -613,8 +613,10 @@ struct Engine {
     for (size_t q = 0; q < jobs.size(); q++) {
       const EigJob& j = jobs[q];
       if (j.n > 200) throw DeviceError("Jacobi eigensolver called for n > 200");
-      // eigenvalues only (Z null): no V, so A alone in LDS (mode 1) when it fits
+      // eigenvalues only (Z null): no V, so A alone in LDS (mode 1) when it fits; the tuning table
+      // may keep V in HBM (mode 1) where both fit
       int mode = (j.Z && jacobi_lds(j.n, 2, es) <= (size_t)max_lds) ? 2 : (jacobi_lds(j.n, 1, es) <= (size_t)max_lds ? 1 : 0);
+      if (mode == 2 && tune_jac_mode[fp32 ? 1 : 0][tune_bucket(j.n)] == 1) mode = 1;
       if (fp32 && mode == 0) throw DeviceError("FP32 Jacobi: matrix does not fit in LDS");
       gidx[mode].push_back((int)q);
     }
@@ -663,7 +665,9 @@ struct Engine {
         HIP_OK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
         // 16 wavefronts per matrix while they fit (32 per CU: one workgroup per CU, or two of the FP32
         // kernels, which use at most 64 registers), else 8
-        const unsigned nthr = (mode != 0 && jacobi_wide && (pc == 1 || (pc == 2 && fp32))) ? 1024u : 512u;
+        unsigned nthr = (mode != 0 && jacobi_wide && (pc == 1 || (pc == 2 && fp32))) ? 1024u : 512u;
+        const int tt = tune_jac_threads[fp32 ? 1 : 0][mode][tune_bucket(js[b0].n)];
+        if (tt && jacobi_threads_ok(js[b0].n, tt)) nthr = (unsigned)tt;
         hipLaunchKernelGGL(kern, dim3((unsigned)(b1 - b0)), dim3(nthr), shm, s, buf.p + b0, max_sweeps, stat + b0,
                            tol_scale);
         b0 = b1;
@@ -691,6 +695,19 @@ struct Engine {
   }
   DBuf<EigJob> ejb[3];
   DBuf<int> jmol;
+  // launch shapes per size bucket (n / 8 for n <= 207), set by the autotuner through
+  // axtbhip_tuning_set; 0 keeps the built-in rule
+  static constexpr int TUNE_NB = 26;
+  int tune_jac_threads[2][3][TUNE_NB] = {};  // [FP32][mode][bucket]: 256, 512 or 1024 threads
+  int tune_jac_mode[2][TUNE_NB] = {};         // [FP32][bucket]: 1 keeps V in HBM where mode 2 fits
+  int tune_gemm_small[TUNE_NB] = {};          // [bucket of max(m, n)]: 1 one-workgroup kernel, 2 64-tile kernel
+  static int tune_bucket(int n) { return std::min(TUNE_NB - 1, std::max(0, n / 8)); }
+  // a Jacobi workgroup of t threads takes every 2 x 2 block of an n x n matrix (at most 10 per
+  // thread) and one rotation pair per thread
+  static bool jacobi_threads_ok(int n, int t) {
+    const int np = (n + (n & 1)) / 2;
+    return (t == 256 || t == 512 || t == 1024) && (long long)t * 10 >= (long long)np * (np + 1) / 2 && t > np;
+  }
   DBuf<double> sweep_acc;  // per Jacobi mode and precision: sweeps, solves, unconverged solves
   // the device-side sweep statistics into the profile's counters
   void drain_sweeps() {
@@ -3036,6 +3053,27 @@ int axtbhip_batch_adjoint(axtbhip_context* ctx, const double* dE, double* grad) 
     ctx->err = e.what();
     return AXTB_ERR_DEVICE;
   }
+}
+
+int axtbhip_tuning_set(axtbhip_context* ctx, const char* knob, int variant, int bucket, int value) {
+  if (!ctx || !knob || bucket < 0 || bucket >= axtb::Engine::TUNE_NB) return AXTB_ERR_INPUT;
+  axtb::Engine& E = *ctx->eng;
+  const std::string k(knob);
+  if (k == "jacobi_threads" && variant >= 0 && variant < 6 && (value == 0 || value == 256 || value == 512 ||
+                                                                 value == 1024)) {
+    E.tune_jac_threads[variant / 3][variant % 3][bucket] = value;
+    return 0;
+  }
+  if (k == "jacobi_mode" && (variant == 0 || variant == 1) && value >= 0 && value <= 2) {
+    E.tune_jac_mode[variant][bucket] = value;
+    return 0;
+  }
+  if (k == "gemm_small" && variant == 0 && value >= 0 && value <= 2) {
+    E.tune_gemm_small[bucket] = value;
+    return 0;
+  }
+  ctx->err = "unknown tuning knob or value";
+  return AXTB_ERR_INPUT;
 }
 
 int axtbhip_synchronize(axtbhip_context* ctx) {

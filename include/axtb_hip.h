@@ -223,6 +223,14 @@ int axtbhip_batch_adjoint(axtbhip_context* ctx, const double* dE, double* grad);
 void axtbhip_profile_enable(axtbhip_context* ctx, int on);
 /* hipDeviceSynchronize on the context's device. */
 int axtbhip_synchronize(axtbhip_context* ctx);
+/* Launch shapes per size bucket (n / 8, buckets 0..25, n up to 207) as an autotuner measured them
+ * (axtb_amd/autotune.py; no counterpart in the reference, which has no GPU path): knob
+ * "jacobi_threads" (variant 3 * fp32 + mode, mode 1: A in LDS, 2: A and V in LDS; value 256, 512
+ * or 1024 threads per matrix, applied where the matrix's 2 x 2 blocks fit), "jacobi_mode" (variant
+ * fp32; value 1: keep the eigenvectors in HBM where both fit in LDS) and "gemm_small" (variant 0;
+ * value 1: products up to 128 x 128 on the one-workgroup kernel, 2: on the 64-tile kernel).
+ * Value 0 restores the built-in rule. Returns 0 or AXTB_ERR_INPUT. */
+int axtbhip_tuning_set(axtbhip_context* ctx, const char* knob, int variant, int bucket, int value);
 int axtbhip_profile_report(axtbhip_context* ctx, char* buf, size_t len);
 
 #ifdef __cplusplus

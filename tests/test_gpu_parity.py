@@ -338,3 +338,27 @@ def test_element_cases_on_gpu(ctx):
     print(f"{len(ids)} element cases: max |dE| vs axtb {worst:.2e}", flush=True)
     z, x = structure("taxol")
     check(ctx, z, x, 1e-2)
+
+
+def test_tuning_table(ctx):
+    """Launch shapes from a tuning table (axtbhip_tuning_set) change the kernels' schedules, not the
+    results beyond rounding; invalid entries are refused."""
+    from axtb_amd import _lib, structures as S
+    mols = S.batch(24, seed=5)
+    plain = _lib.Context(0, tuning=None)
+    tuned = _lib.Context(0, tuning=None)
+    for b in range(26):
+        assert tuned.tune("jacobi_mode", 0, b, 1) == 0
+        assert tuned.tune("jacobi_threads", 1, b, 512) == 0
+        assert tuned.tune("gemm_small", 0, b, 2) == 0
+    assert tuned.tune("jacobi_threads", 1, 3, 300) != 0
+    assert tuned.tune("no_such_knob", 0, 3, 1) != 0
+    assert tuned.tune("gemm_small", 0, 26, 1) != 0
+    o = _lib.make_opts(accuracy=1e-2)
+    ra, rb = plain.engrad_batch(mols, o), tuned.engrad_batch(mols, o)
+    for a, c in zip(ra, rb):
+        assert a["status"] == 0 and c["status"] == 0
+        assert abs(a["energy"] - c["energy"]) <= 1e-10
+        assert np.abs(a["gradient"] - c["gradient"]).max() <= 1e-9
+    plain.close()
+    tuned.close()
