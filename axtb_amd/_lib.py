@@ -46,10 +46,13 @@ class Opts(ctypes.Structure):
     _fields_ = [("accuracy", ctypes.c_double), ("etemp", ctypes.c_double), ("max_iter", ctypes.c_int),
                 ("damping", ctypes.c_double), ("guess", ctypes.c_int), ("want_gradient", ctypes.c_int),
                 ("precision", ctypes.c_int), ("dispersion", ctypes.c_int), ("mixer", ctypes.c_int),
-                ("solvation", ctypes.c_int), ("solvent", ctypes.c_char_p), ("solvation_state", ctypes.c_int)]
+                ("solvation", ctypes.c_int), ("solvent", ctypes.c_char_p), ("solvation_state", ctypes.c_int),
+                ("conventions", ctypes.c_int)]
 
 
-MIXERS = {"auto": 0, "broyden": 1, "diis": 2}
+# axtb_opts.mixer: the conventions' default (axtb: Broyden then DIIS; tblite: Broyden), or a plan
+MIXERS = {"default": 0, "broyden": 1, "diis": 2, "auto": 3}
+CONVENTIONS = {"axtb": 0, "xtb": 1, "xtb-6.7.1": 2, "tblite": 3}
 MIXER_USED = {0: "broyden", 1: "diis", 2: "diis-fallback"}
 
 
@@ -127,12 +130,13 @@ SOLVATION_STATES = {"gsolv": 0, "bar1M": 1, "bar1m": 1, "reference": 2}
 
 
 def make_opts(accuracy=1.0, etemp=300.0, max_iter=250, damping=0.4, guess="eeq", gradient=True, precision="fp64",
-              dispersion=True, mixer="auto", solvation=None, solvent=None, state="gsolv"):
+              dispersion=True, mixer="default", solvation=None, solvent=None, state="gsolv", conventions="axtb"):
     """The options of a calculation; ``solvation`` None, "alpb" or "gbsa" with ``solvent`` (a name
-    xtb accepts) and the reference ``state`` (gsolv, bar1M, reference)."""
+    xtb accepts) and the reference ``state`` (gsolv, bar1M, reference); ``conventions`` "axtb" or
+    "tblite"."""
     return Opts(float(accuracy), float(etemp), int(max_iter), float(damping), 0 if guess == "eeq" else 1,
                 int(bool(gradient)), PRECISIONS[precision], int(bool(dispersion)), MIXERS[mixer],
-                SOLVATION[solvation], (solvent or "").encode(), SOLVATION_STATES[state])
+                SOLVATION[solvation], (solvent or "").encode(), SOLVATION_STATES[state], CONVENTIONS[conventions])
 
 
 class Context:

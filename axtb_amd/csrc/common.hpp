@@ -22,6 +22,7 @@ struct Mol {
   int at0, sh0, ao0, vec0;
   long long mat0, pair0, shp0, hist0;
   double nalpha, nbeta, kt, charge;
+  double pair_cut2;              // atom pairs of the integrals within this squared distance (bohr^2)
 };
 
 // Per-atom parameters (SoA over the batch).
@@ -56,6 +57,9 @@ struct Batch {
   const double* c6ref;          // [nsp][nsp][7][7]
   int nsp;
   double intcut;
+  // real-space cutoffs of the conventions: squared distances (bohr^2) of the GFN2 CN, the D4 CN and
+  // the repulsion; radii and switch widths (bohr) of the D4 two-body and three-body terms
+  double cn_cut2, d4cn_cut2, rep_cut2, d4_rc_2b, d4_w_2b, d4_rc_3b, d4_w_3b;
 };
 
 #define HIP_OK(x)                                                                                \

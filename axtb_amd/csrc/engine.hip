@@ -34,6 +34,7 @@ namespace {
 
 const double AATOAU_2022 = 1.0 / (5.29177210544e-11 * 1e10);  // base/src/units.rs:18-20
 const double KB_HARTREE = 3.1668115634564e-6;                   // base/src/units.rs:24
+const double KB_XTB_TBLITE = 3.166808578545117e-06;             // base/src/units.rs:81 (tblite's kB)
 const double PI = 3.14159265358979323846;
 
 struct Fail {
@@ -1010,7 +1011,11 @@ struct Engine {
     if (!(o.etemp >= 0.0 && std::isfinite(o.etemp))) throw Fail{AXTB_ERR_INPUT, "electronic temperature must be >= 0 K"};
     if (!(o.damping > 0.0 && o.damping <= 1.0)) throw Fail{AXTB_ERR_INPUT, "damping must be in (0, 1]"};
     if (o.max_iter < 1) throw Fail{AXTB_ERR_INPUT, "max_iter must be >= 1"};
-    if (o.mixer < AXTB_MIXER_AUTO || o.mixer > AXTB_MIXER_DIIS) throw Fail{AXTB_ERR_INPUT, "unknown SCF mixer"};
+    if (o.mixer < AXTB_MIXER_AUTO || o.mixer > AXTB_MIXER_BROYDEN_DIIS) throw Fail{AXTB_ERR_INPUT, "unknown SCF mixer"};
+    if (o.conventions != AXTB_CONV_AXTB && o.conventions != AXTB_CONV_TBLITE)
+      throw Fail{AXTB_ERR_INPUT, o.conventions == AXTB_CONV_XTB || o.conventions == AXTB_CONV_XTB671
+                                     ? "the GPU engine runs the axtb and tblite conventions, not the xtb programs'"
+                                     : "unknown conventions"};
     if (o.precision < AXTB_PREC_FP64 || o.precision > AXTB_PREC_FP64_REFINE)
       throw Fail{AXTB_ERR_INPUT, "unknown precision"};
     if (o.solvation < AXTB_SOLV_NONE || o.solvation > AXTB_SOLV_GBSA) throw Fail{AXTB_ERR_INPUT, "unknown solvation model"};
@@ -1087,10 +1092,13 @@ struct Engine {
       std::stable_sort(use.begin(), use.end(), [&](int a, int b) { return nao[a] > nao[b]; });
     }
     build(in, use, o, out);
-    // the options outlive the call: the solvent's name is kept here
+    // the options outlive the call: the solvent's name is kept here; the mixer of the conventions'
+    // default resolved (xtb.rs:516: axtb Broyden then DIIS, tblite Broyden; conventions.rs:260,348)
     solvent_name = o.solvent ? o.solvent : "";
     opts_ = o;
     opts_.solvent = solvent_name.c_str();
+    if (o.mixer == AXTB_MIXER_AUTO && o.conventions == AXTB_CONV_TBLITE) opts_.mixer = AXTB_MIXER_BROYDEN;
+    if (o.mixer == AXTB_MIXER_BROYDEN_DIIS) opts_.mixer = AXTB_MIXER_AUTO;
   }
 
   // the prepared batch on the device: setup, SCC and gradient (inputs resident in HBM)
@@ -1205,6 +1213,12 @@ struct Engine {
     hgdp.clear();
     hgqp.clear();
     cap = std::max(1, o.max_iter);  // the reference's Broyden memory: max_iter (mixing.rs:104-109)
+    // the conventions (conventions.rs:244-353): axtb, or tblite 0.8.0's constants, radii, cutoffs and
+    // screening
+    const bool tbl = o.conventions == AXTB_CONV_TBLITE;
+    tblite_ = tbl;
+    auto COVALENT_RAD_D3_MCTC = [](int iz) { return AXTB_MCTC_AATOAU * EL_COVALENT_RAD_2009_AA[iz]; };
+    double amin_mol = INFINITY;  // the molecule's smallest primitive exponent (basis.rs:207)
     std::map<int, int> species;
     tot_mat = tot_pair = tot_shp = tot_eq = tot_hist = 0;
     tot_vec = 0;
@@ -1231,9 +1245,11 @@ struct Engine {
         hish.push_back(sh0 + nsh);
         hnsh.push_back(GFN2_NSHELL[iz]);
         for (int k = 0; k < 3; k++) hxyz.push_back(m.xyz[3 * a + k]);
-        hrcov.push_back(4.0 / 3.0 * (EL_COVALENT_RAD_2009_AA[iz] * AATOAU_2022));
+        // radii in bohr: CODATA 2022 (axtb), mctc-lib's bohr (tblite; param/mod.rs:26-47,
+        // element_data.rs:26,94-102)
+        hrcov.push_back(tbl ? 4.0 / 3.0 * COVALENT_RAD_D3_MCTC(iz) : 4.0 / 3.0 * (EL_COVALENT_RAD_2009_AA[iz] * AATOAU_2022));
         hen.push_back(EL_PAULING_EN[iz]);
-        hrad.push_back(EL_ATOMIC_RAD_AA[iz] * AATOAU_2022);
+        hrad.push_back(tbl ? AXTB_MCTC_AATOAU * EL_ATOMIC_RAD_AA[iz] : EL_ATOMIC_RAD_AA[iz] * AATOAU_2022);
         hra.push_back(GFN2_REP_ALPHA[iz]);
         hrz.push_back(GFN2_REP_ZEFF[iz]);
         hdk.push_back(GFN2_P_DKERNEL[iz]);
@@ -1246,7 +1262,7 @@ struct Engine {
         herad.push_back(EEQ_RAD[iz]);
         hreeq.push_back(4.0 / 3.0 * (AXTB_MCTC_AATOAU * EL_COVALENT_RAD_2009_AA[iz]));
         // D4 (d4/src/model.rs:60-71, 260-340), GFN2 reference charges, axtb CN radii
-        hd4rcov.push_back(4.0 / 3.0 * (EL_COVALENT_RAD_2009_AA[iz] * AATOAU_2022));
+        hd4rcov.push_back(tbl ? 4.0 / 3.0 * COVALENT_RAD_D3_MCTC(iz) : 4.0 / 3.0 * (EL_COVALENT_RAD_2009_AA[iz] * AATOAU_2022));
         hd4zeff.push_back(D4_EFFECTIVE_NUCLEAR_CHARGE[iz]);
         hd4gi.push_back(D4_CHEMICAL_HARDNESS[iz] * 2.0);
         hd4r4r2.push_back(std::sqrt(0.5 * (D4_R4_OVER_R2[iz] * std::sqrt((double)z))));
@@ -1269,6 +1285,7 @@ struct Engine {
           sto_ng(np, GFN2_PRINCIPAL_QUANTUM_NUMBER[iz][s], l, GFN2_SLATER_EXPONENT[iz][s], al, co);
           double amin = 1e300;
           for (int k = 0; k < np; k++) amin = std::min(amin, al[k]);
+          amin_mol = std::min(amin_mol, amin);
           hsat.push_back(at0 + a);
           hsl.push_back(l);
           hsnp.push_back(np);
@@ -1334,7 +1351,18 @@ struct Engine {
       M.ndim = nsh + 9 * m.nat;
       M.nalpha = (nocc + nuhf) / 2.0;
       M.nbeta = (nocc - nuhf) / 2.0;
-      M.kt = o.etemp > 0.0 ? o.etemp * KB_HARTREE : 0.0;
+      M.kt = o.etemp > 0.0 ? o.etemp * (tbl ? KB_XTB_TBLITE : KB_HARTREE) : 0.0;
+      // tblite screens the integrals' atom pairs beyond sqrt(2 intcut(acc) / alpha_min), at most
+      // 40 bohr (basis.rs:139-143,247-249); axtb takes every pair
+      if (tbl) {
+        const double acc = std::min(std::max(o.accuracy, 1.0e-4), 1.0e3);
+        const double ic = std::min(std::max(25.0 - 10.0 * std::log10(acc), 5.0), 25.0);
+        const double rc = std::min(std::sqrt(2.0 * ic / amin_mol), 40.0);
+        M.pair_cut2 = rc * rc;
+      } else {
+        M.pair_cut2 = INFINITY;
+      }
+      amin_mol = INFINITY;
       M.vec0 = tot_vec;
       M.mat0 = tot_mat;
       M.pair0 = tot_pair;
@@ -1420,7 +1448,8 @@ struct Engine {
       g_dp.upload(hgdp, st);
       g_qp.upload(hgqp, st);
     }
-    intcut_ = std::max(25.0 - 10.0 * std::log10(o.accuracy), 20.0);  // basis.rs:135-137
+    // primitive screening: xtb's cutoff at the accuracy (axtb), tblite's at accuracy 1 (basis.rs:135-143)
+    intcut_ = tbl ? 25.0 : std::max(25.0 - 10.0 * std::log10(o.accuracy), 20.0);
     // implicit solvation: per-atom parameters of the model of every molecule (one solvent for the
     // batch; the axtb conventions' corrected model, xtb's default grid and temperature)
     solv_model = o.solvation;
@@ -1465,6 +1494,7 @@ struct Engine {
   }
   int nsp_ = 0;
   double intcut_ = 25.0;
+  bool tblite_ = false;  // tblite 0.8.0's conventions
 
   Batch batch() const {
     Batch b{};
@@ -1483,6 +1513,14 @@ struct Engine {
     b.c6ref = c6ref.p;
     b.nsp = nsp_;
     b.intcut = intcut_;
+    // conventions.rs:255-262 (axtb), 338-346 (tblite); D4 cutoffs disp.rs:38-47
+    b.cn_cut2 = tblite_ ? 25.0 * 25.0 : 40.0 * 40.0;
+    b.rep_cut2 = tblite_ ? 25.0 * 25.0 : 40.0 * 40.0;
+    b.d4cn_cut2 = tblite_ ? 30.0 * 30.0 : 40.0 * 40.0;
+    b.d4_rc_2b = tblite_ ? 50.0 : 72.0;
+    b.d4_w_2b = tblite_ ? 0.05 : 1.0;
+    b.d4_rc_3b = tblite_ ? 25.0 : 40.0;
+    b.d4_w_3b = tblite_ ? 0.05 : 1.0;
     return b;
   }
 
@@ -1635,7 +1673,7 @@ struct Engine {
     auto fits = [&](int n) { return ((size_t)n * n + n) * sizeof(double) <= (size_t)max_lds; };
     auto fits_packed = [&](int n) { return ((size_t)n * (n + 1) / 2 + n) * sizeof(double) <= (size_t)max_lds; };
     fill(qstart.p, NS);
-    if (o.guess == 0) {
+    if (o.guess == 0 && !tblite_) {  // tblite starts from zero charges (conventions.rs:335)
       LAUNCH(k_eeq_system, dim3(blocks((long long)(natmax + 1) * (natmax + 1), 256), NM), dim3(256), 0,
                          b, eq0.p, cneeq.p, eeqA.p, eeqr.p);
       std::vector<EigJob> jobs;

@@ -39,8 +39,9 @@ __device__ __forceinline__ double d4_en_factor(double ei, double ej) {  // d4/sr
 }
 
 // ------------------------------------------------------------------ setup: per-atom quantities
-// One wavefront per atom: the GFN2 CN (40 bohr), the EEQ CN (multicharge, 25 bohr, capped at 8)
-// and the D4 CN (40 bohr, EN-weighted); then shell levels and AES radii of the atom.
+// One wavefront per atom: the GFN2 CN (40 bohr, tblite's conventions 25), the EEQ CN (multicharge,
+// 25 bohr, capped at 8) and the D4 CN (40 bohr, tblite 30; EN-weighted); then shell levels and AES
+// radii of the atom.
 __global__ void k_atom_setup(Batch b, const int* active, double* cn, double* cn_eeq, double* cn_d4, double* se,
                              double* dsedcn, double* mrad, double* dmrdcn) {
   int a = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
@@ -58,10 +59,8 @@ __global__ void k_atom_setup(Batch b, const int* active, double* cn, double* cn_
     double r2 = dot3(v, v);
     if (r2 < 1e-12) continue;
     double r = sqrt(r2);
-    if (r2 <= 1600.0) {
-      c1 += dexp_count(r, b.a.rcov_cn[a] + b.a.rcov_cn[j]);
-      c3 += d4_en_factor(b.a.en[a], b.a.en[j]) * erf_count(r, b.a.d4_rcov[a] + b.a.d4_rcov[j]);
-    }
+    if (r2 <= b.cn_cut2) c1 += dexp_count(r, b.a.rcov_cn[a] + b.a.rcov_cn[j]);
+    if (r2 <= b.d4cn_cut2) c3 += d4_en_factor(b.a.en[a], b.a.en[j]) * erf_count(r, b.a.d4_rcov[a] + b.a.d4_rcov[j]);
     if (r2 <= 625.0) c2 += erf_count(r, b.a.rcov_eeq[a] + b.a.rcov_eeq[j]);
   }
   c1 = wave_sum(c1);
@@ -302,7 +301,7 @@ __global__ __launch_bounds__(64) void k_int_keys(Batch b, int* cnt, int2* list, 
         const double vec[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
         const double r2 = dot3(vec, vec);
         const double a1 = b.s.minalpha[si], a2 = b.s.minalpha[sj];
-        keep = !(a1 * a2 * r2 / (a1 + a2) > b.intcut * (1.0 + 1e-10));
+        keep = !(a1 * a2 * r2 / (a1 + a2) > b.intcut * (1.0 + 1e-10)) && !(r2 > b.mol[m].pair_cut2);
       }
       if (keep) key = 2 * b.s.l[si] + b.s.l[sj];
     }
@@ -355,7 +354,7 @@ template __global__ void k_integrals_key<1, 1>(Batch, const int2*, const int*, c
                                                double*, long long);
 
 // ------------------------------------------------------------------ repulsion (repulsion.rs:44-73)
-// One wavefront per atom: E_A (half of each pair) and dE/dR_A, cutoff 40 bohr.
+// One wavefront per atom: E_A (half of each pair) and dE/dR_A, cutoff 40 bohr (tblite 25).
 __global__ void k_repulsion(Batch b, const int* active, double* e, double* g) {
   int a = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
   int lane = threadIdx.x & 63;
@@ -370,7 +369,7 @@ __global__ void k_repulsion(Batch b, const int* active, double* e, double* g) {
     const double* xj = &b.a.xyz[3 * j];
     double v[3] = {xa[0] - xj[0], xa[1] - xj[1], xa[2] - xj[2]};
     double r2 = dot3(v, v);
-    if (r2 > 1600.0 || r2 < 1e-12) continue;
+    if (r2 > b.rep_cut2 || r2 < 1e-12) continue;
     double r1 = sqrt(r2);
     double kexp = (b.a.z[a] > 2 || b.a.z[j] > 2) ? 1.5 : 1.0;
     double alpha = sqrt(b.a.rep_alpha[j] * b.a.rep_alpha[a]);
@@ -504,12 +503,13 @@ __device__ __forceinline__ void smooth_cut(double r, double cut, double width, d
   }
 }
 
-// rational-damped pair energy factor with the smooth 72/1 bohr cutoff (disp.rs:240-272)
+// rational-damped pair energy factor with the smooth two-body cutoff (axtb 72 bohr switched over
+// the last 1 bohr, tblite 50 / 0.05; disp.rs:240-272)
 __device__ __forceinline__ bool d4_pair_e(const Batch& b, int i, int j, const double* v, double r2, double& e,
                                           double& gdr) {
-  if (r2 > 72.0 * 72.0 || r2 < 2.220446049250313e-16) return false;
+  if (r2 > b.d4_rc_2b * b.d4_rc_2b || r2 < 2.220446049250313e-16) return false;
   double r = sqrt(r2), sw, dsw;
-  smooth_cut(r, 72.0, 1.0, sw, dsw);
+  smooth_cut(r, b.d4_rc_2b, b.d4_w_2b, sw, dsw);
   if (sw <= 0.0) return false;
   double rr = 3.0 * b.a.d4_r4r2[i] * b.a.d4_r4r2[j];
   double r0 = 0.52 * sqrt(rr) + 5.0;
@@ -539,9 +539,9 @@ __global__ void k_atm_pairs(Batch b, const double* gw, const double* dgw, double
     const double* xk = &b.a.xyz[3 * k];
     double v[3] = {xk[0] - xa[0], xk[1] - xa[1], xk[2] - xa[2]};
     double r2 = dot3(v, v);
-    if (r2 <= 1600.0 && r2 >= 2.220446049250313e-16) {
+    if (r2 <= b.d4_rc_3b * b.d4_rc_3b && r2 >= 2.220446049250313e-16) {
       double r = sqrt(r2), sw, dsw;
-      smooth_cut(r, 40.0, 1.0, sw, dsw);
+      smooth_cut(r, b.d4_rc_3b, b.d4_w_3b, sw, dsw);
       double rr = 3.0 * b.a.d4_r4r2[a] * b.a.d4_r4r2[k];
       double r0 = 0.52 * sqrt(rr) + 5.0;
       double c6, da, dk;
@@ -1706,7 +1706,8 @@ __global__ __launch_bounds__(64) void k_h0_keys(Batch b, const int* okm, int* cn
       double amin_i = 1e300, amin_j = 1e300;
       for (int s = b.a.ish[iat]; s < b.a.ish[iat] + b.a.nsh[iat]; s++) amin_i = fmin(amin_i, b.s.minalpha[s]);
       for (int s = b.a.ish[jat]; s < b.a.ish[jat] + b.a.nsh[jat]; s++) amin_j = fmin(amin_j, b.s.minalpha[s]);
-      if (!(amin_i * amin_j * r2 / (amin_i + amin_j) > b.intcut * (1.0 + 1e-10)) && r2 >= 2.220446049250313e-16)
+      if (!(amin_i * amin_j * r2 / (amin_i + amin_j) > b.intcut * (1.0 + 1e-10)) && r2 >= 2.220446049250313e-16 &&
+          !(r2 > M.pair_cut2))  // the conventions' atom-pair cutoff (ao.rs:96-103)
         for (int si = b.a.ish[iat]; si < b.a.ish[iat] + b.a.nsh[iat]; si++)
           for (int sj = b.a.ish[jat]; sj < b.a.ish[jat] + b.a.nsh[jat]; sj++) {
             const double a1 = b.s.minalpha[si], a2 = b.s.minalpha[sj];
@@ -1792,12 +1793,13 @@ __global__ void k_cn_chain(Batch b, const int* okm, const double* dedcn, const d
     const double* xj = &b.a.xyz[3 * j];
     double v[3] = {xa[0] - xj[0], xa[1] - xj[1], xa[2] - xj[2]};
     double r2 = dot3(v, v);
-    if (r2 > 1600.0 || r2 < 1e-12) continue;
+    if (r2 < 1e-12 || (r2 > b.cn_cut2 && r2 > b.d4cn_cut2)) continue;
     double r = sqrt(r2);
-    double t = dexp_dcount(r, b.a.rcov_cn[a] + b.a.rcov_cn[j]) * (dedcn[a] + dedcn[j]) / r;
+    double t = r2 <= b.cn_cut2 ? dexp_dcount(r, b.a.rcov_cn[a] + b.a.rcov_cn[j]) * (dedcn[a] + dedcn[j]) / r : 0.0;
     double s4 = dedcn_d4[a] + dedcn_d4[j];
-    double t4 = s4 != 0.0 ? d4_en_factor(b.a.en[a], b.a.en[j]) * erf_dcount(r, b.a.d4_rcov[a] + b.a.d4_rcov[j]) * s4 / r
-                          : 0.0;
+    double t4 = s4 != 0.0 && r2 <= b.d4cn_cut2
+                    ? d4_en_factor(b.a.en[a], b.a.en[j]) * erf_dcount(r, b.a.d4_rcov[a] + b.a.d4_rcov[j]) * s4 / r
+                    : 0.0;
     for (int k = 0; k < 3; k++) ga[k] += (t + t4) * v[k];
   }
   for (int k = 0; k < 3; k++) ga[k] = wave_sum(ga[k]);

@@ -60,9 +60,10 @@ typedef struct axtb_opts {
    * FP64 (no counterpart in the reference, which solves in full every iteration) */
   int precision;
   int dispersion;    /* 1: D4 (default) */
-  /* SCF mixer plan (MixerKind, crates/axtb/src/scf/mixing.rs:83-118): 0 auto (default of the axtb
-   * conventions, conventions.rs:260: Broyden, then DIIS from the start if Broyden does not
-   * converge), 1 Broyden alone, 2 DIIS alone */
+  /* SCF mixer plan (MixerKind, crates/axtb/src/scf/mixing.rs:83-118): 0 the conventions' default
+   * (xtb.rs:516; axtb, conventions.rs:260: Broyden, then DIIS from the start if Broyden does not
+   * converge; tblite: Broyden), 1 Broyden alone, 2 DIIS alone, 3 Broyden then DIIS under any
+   * conventions */
   int mixer;
   /* implicit solvation (SolvationSpec, crates/axtb/src/solvation.rs:14-22; the axtb conventions'
    * corrected ALPB/GBSA model, conventions.rs:256): AXTB_SOLV_NONE, AXTB_SOLV_ALPB or
@@ -72,7 +73,19 @@ typedef struct axtb_opts {
   int solvation;
   const char* solvent;
   int solvation_state;
+  /* numerical conventions (Conventions, crates/axtb/src/conventions.rs:73-84, 237-353):
+   * AXTB_CONV_AXTB (default) or AXTB_CONV_TBLITE (tblite 0.8.0: its kB, mctc-lib's bohr in the radius
+   * tables, 25 bohr CN and repulsion cutoffs, D4 cutoffs 30/50/25 bohr with 0.05 bohr switches,
+   * primitive screening at accuracy 1 and atom pairs screened by the basis extent, zero-charge
+   * start, Broyden mixing); the xtb programs' conventions are not run by the GPU engine
+   * (AXTB_ERR_INPUT). */
+  int conventions;
 } axtb_opts;
+
+#define AXTB_CONV_AXTB 0
+#define AXTB_CONV_XTB 1
+#define AXTB_CONV_XTB671 2
+#define AXTB_CONV_TBLITE 3
 
 #define AXTB_SOLV_NONE 0
 #define AXTB_SOLV_ALPB 1
@@ -84,6 +97,7 @@ typedef struct axtb_opts {
 #define AXTB_MIXER_AUTO 0
 #define AXTB_MIXER_BROYDEN 1
 #define AXTB_MIXER_DIIS 2
+#define AXTB_MIXER_BROYDEN_DIIS 3
 
 /* Results: the SinglePoint fields of the path (crates/base/src/method/results.rs:39-230). */
 typedef struct axtb_result {

@@ -55,18 +55,22 @@ SOLV_STATES = {"gsolv": 0, "bar1M": 1, "bar1m": 1, "reference": 2}
 SOLV_CONVENTIONS = {"xtb": 0, "xtb671": 1, "corrected": 2}
 
 
-MIXERS = {"broyden": 0, "diis": 1, "auto": 2}
+MIXERS = {"broyden": 0, "diis": 1, "auto": 2, "default": -1}  # default: the conventions' plan
+CONVENTIONS = {"axtb": 0, "tblite": 3}
 MIXER_USED = {0: "broyden", 1: "diis", 2: "diis-fallback"}
 
 
 def engrad(numbers, positions, charge=0.0, acc=1.0, etemp=300.0, max_iter=250, guess="eeq", gradient=True,
-           dispersion=True, start=None, mixer="auto", report=None, solvation=None, solvent=None, state="gsolv"):
+           dispersion=True, start=None, mixer="default", report=None, solvation=None, solvent=None, state="gsolv",
+           conventions="axtb", uhf=None):
     """GFN2-xTB energy (+ gradient) of one molecule; positions in bohr [N,3]. ``start``: a given
     SCF start {"qsh", "dipoles", "quadrupoles"} (the moments of a previous result), else ``guess``.
     ``mixer``: the SCF mixer plan (auto = Broyden, then DIIS from the start: the axtb conventions).
     ``report``: a dict that receives the SCF report (iterations, mixer_used, attempts) also when
     the calculation fails. ``solvation``: None, "alpb" or "gbsa" with ``solvent`` (a name xtb
-    accepts) and the reference ``state`` (gsolv, bar1M, reference)."""
+    accepts) and the reference ``state`` (gsolv, bar1M, reference). ``conventions``: "axtb" or
+    "tblite" (conventions.rs:244-353); "default" mixer: the conventions' (auto for axtb, Broyden
+    for tblite)."""
     z = np.ascontiguousarray(numbers, dtype=np.int32)
     x = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
     nat = len(z)
@@ -90,7 +94,8 @@ def engrad(numbers, positions, charge=0.0, acc=1.0, etemp=300.0, max_iter=250, g
                                   ctypes.c_double(etemp), max_iter, 0 if guess == "eeq" else 1, int(gradient),
                                   int(dispersion), MIXERS[mixer], _p(gs[0]), _p(gs[1]), _p(gs[2]), ctypes.byref(e),
                                   _p(terms), _p(g), _p(qat), _p(qsh), _p(dp), _p(qp), _p(emo), _p(info), _p(scal),
-                                  SOLV_MODELS[solvation], (solvent or "").encode(), SOLV_STATES[state])
+                                  SOLV_MODELS[solvation], (solvent or "").encode(), SOLV_STATES[state],
+                                  CONVENTIONS[conventions], -1 if uhf is None else int(uhf))
     rep = {"iterations": int(info[0]), "mixer_used": MIXER_USED.get(int(info[1])),
            "attempts": [(("broyden", "diis")[0 if (mixer != "diis" and a == 0) else 1], int(info[3 + 2 * a]),
                          bool(info[4 + 2 * a])) for a in range(int(info[2]))]}

@@ -38,6 +38,7 @@ typedef vector<double> vec;
 const double AUTOAA = 5.29177210544e-11 * 1e10;
 const double AATOAU = 1.0 / AUTOAA;
 const double KB_HARTREE = 3.1668115634564e-6;
+const double KB_XTB_TBLITE = 3.166808578545117e-06;  // base/src/units.rs:81
 const double PI = 3.14159265358979323846;
 
 struct Error : std::runtime_error {
@@ -122,6 +123,11 @@ struct Model {
   vector<int> ish_at, nsh_at, iao_at, nao_at, sh2at, iao_sh, nao_sh, ao2at, ao2sh;
   vec min_alpha_sh, min_alpha_at;
   double intcut = 0;
+  // conventions (conventions.rs:244-353): tblite 0.8.0's (mctc-lib radii, 25 bohr CN and repulsion
+  // cutoffs, atom-pair screening of the integrals) or axtb's
+  bool tblite = false;
+  vec rcov;                      // GFN2 CN radius per atom
+  double pair_cut2 = INFINITY, cn_cut2 = 1600.0, rep_cut2 = 1600.0;
   // H0 (gfn2.rs:61-99)
   vec se0, kcn, shpoly, refocc;  // per shell
   vec rad;                       // per atom
@@ -151,8 +157,9 @@ double hscale(const Model& m, int zi, int ish, int zj, int jsh) {
   return zij * km;
 }
 
-Model build_model(int nat, const int* z, const double* xyz, double charge, double acc) {
+Model build_model(int nat, const int* z, const double* xyz, double charge, double acc, bool tblite = false) {
   Model m;
+  m.tblite = tblite;
   m.nat = nat;
   m.z.assign(z, z + nat);
   m.xyz.assign(xyz, xyz + 3 * nat);
@@ -205,12 +212,30 @@ Model build_model(int nat, const int* z, const double* xyz, double charge, doubl
     m.nao_at.push_back(iao - m.iao_at[i]);
     m.min_alpha_at.push_back(amin);
     m.shells.push_back(sh);
-    m.rad.push_back(atomic_rad_codata2022(z[i]));
+    // radii: CODATA 2022 bohr (axtb) or mctc-lib's (tblite: element_data.rs:26,94-102)
+    m.rad.push_back(tblite ? AXTB_MCTC_AATOAU * EL_ATOMIC_RAD_AA[z[i] - 1] : atomic_rad_codata2022(z[i]));
+    m.rcov.push_back(tblite ? covalent_rad_d3_mctc(z[i]) : covalent_rad_d3_codata2022(z[i]));
   }
   m.nsh = ish;
   m.nao = iao;
-  // basis.rs:135-137 (xtb's cutoff, axtb conventions: primitive_cutoff = integral_cutoff_xtb)
-  m.intcut = std::max(25.0 - 10.0 * std::log10(acc), 20.0);
+  if (tblite) {
+    // tblite: primitive screening at accuracy 1, atom pairs within sqrt(2 intcut(acc) / alpha_min)
+    // (at most 40 bohr), CN and repulsion within 25 bohr (basis.rs:139-143,207,247-249;
+    // conventions.rs:338-346)
+    auto integral_cutoff = [](double a) {
+      a = std::min(std::max(a, 1.0e-4), 1.0e3);
+      return std::min(std::max(25.0 - 10.0 * std::log10(a), 5.0), 25.0);
+    };
+    m.intcut = integral_cutoff(1.0);
+    double amin = INFINITY;
+    for (double a : m.min_alpha_sh) amin = std::min(amin, a);
+    const double rc = std::min(std::sqrt(2.0 * integral_cutoff(acc) / amin), 40.0);
+    m.pair_cut2 = rc * rc;
+    m.cn_cut2 = m.rep_cut2 = 25.0 * 25.0;
+  } else {
+    // basis.rs:135-137 (xtb's cutoff, axtb conventions: primitive_cutoff = integral_cutoff_xtb)
+    m.intcut = std::max(25.0 - 10.0 * std::log10(acc), 20.0);
+  }
   return m;
 }
 
@@ -233,10 +258,11 @@ inline double erf_dcount(double k, double r, double rc) {
   return -(k * std::exp(-(e * e))) / (SQRTPI * rc);
 }
 
-// GFN2 CN: axtb/src/ncoord.rs:29-41 over base/src/ncoord.rs:471-484 (cutoff 40, r2 < 1e-12 skipped)
+// GFN2 CN: axtb/src/ncoord.rs:29-41 over base/src/ncoord.rs:471-484 (cutoff 40, tblite 25; r2 < 1e-12
+// skipped)
 vec gfn2_cn(const Model& m) {
   vec cn(m.nat, 0.0);
-  double c2 = 40.0 * 40.0;
+  double c2 = m.cn_cut2;
   for (int i = 0; i < m.nat; i++)
     for (int j = 0; j < i; j++) {
       const double* a = &m.xyz[3 * i];
@@ -244,7 +270,7 @@ vec gfn2_cn(const Model& m) {
       double v[3] = {a[0] - b[0], a[1] - b[1], a[2] - b[2]};
       double r2 = dot3(v, v);
       if (r2 > c2 || r2 < 1e-12) continue;
-      double rc = covalent_rad_d3_codata2022(m.z[i]) + covalent_rad_d3_codata2022(m.z[j]);
+      double rc = m.rcov[i] + m.rcov[j];
       double c = dexp_count(std::sqrt(r2), rc);
       cn[i] += c;
       cn[j] += c;
@@ -254,7 +280,7 @@ vec gfn2_cn(const Model& m) {
 
 // base/src/ncoord.rs:486-519 (GradientOrder::Scaled)
 void gfn2_cn_gradient(const Model& m, const vec& dedcn, vec& g) {
-  double c2 = 40.0 * 40.0;
+  double c2 = m.cn_cut2;
   for (int i = 0; i < m.nat; i++)
     for (int j = 0; j < i; j++) {
       double s = dedcn[i] + dedcn[j];
@@ -264,7 +290,7 @@ void gfn2_cn_gradient(const Model& m, const vec& dedcn, vec& g) {
       double r2 = dot3(v, v);
       if (r2 > c2 || r2 < 1e-12) continue;
       double r = std::sqrt(r2);
-      double rc = covalent_rad_d3_codata2022(m.z[i]) + covalent_rad_d3_codata2022(m.z[j]);
+      double rc = m.rcov[i] + m.rcov[j];
       double t = dexp_dcount(r, rc) * s / r;
       for (int k = 0; k < 3; k++) {
         g[3 * i + k] += t * v[k];
@@ -713,6 +739,7 @@ Ints build_integrals(const Model& m, const vec& se) {
       for (int k = 0; k < 3; k++) vec[k] = m.xyz[3 * iat + k] - m.xyz[3 * jat + k];
       double r2 = dot3(vec, vec);
       bool onsite = iat == jat;
+      if (r2 > m.pair_cut2) continue;  // the conventions' atom-pair cutoff (ao.rs:96-103)
       if (!onsite && pair_screened(m.min_alpha_at[iat], m.min_alpha_at[jat], r2, m.intcut)) continue;
       double rr = onsite ? 0.0 : std::sqrt(std::sqrt(r2) / (m.rad[jat] + m.rad[iat]));
       for (int ish = 0; ish < m.nsh_at[iat]; ish++) {
@@ -967,7 +994,7 @@ void es2_gradient(const Model& m, const Coulomb& c, const vec& qsh, vec& g) {
 
 // ------------------------------------------------------------------ repulsion (repulsion.rs:44-73)
 double repulsion(const Model& m, vec& e, vec* g) {
-  double c2 = 40.0 * 40.0;
+  double c2 = m.rep_cut2;
   for (int i = 0; i < m.nat; i++)
     for (int j = 0; j < i; j++) {
       double v[3];
@@ -2291,6 +2318,9 @@ struct Options {
   double acc = 1.0, etemp = 300.0, damping = 0.4;
   int max_iter = 250, guess = 0 /*0 EEQ, 1 SAD*/, want_grad = 1, dispersion = 1;
   int mixer = MIX_AUTO;  // the axtb conventions' plan (conventions.rs:260)
+  // conventions: 0 axtb, 3 tblite 0.8.0 (conventions.rs:73-84)
+  int conventions = 0;
+  int uhf = -1;  // unpaired electrons; -1: the lowest spin of the electron count
   // a given start (ScfGuess, xtb.rs:204-208): shell charges [nsh] (null: none), dipoles [3N],
   // quadrupoles [6N] (null: zero)
   const double *gq = nullptr, *gdp = nullptr, *gqp = nullptr;
@@ -2352,7 +2382,9 @@ Result singlepoint(int nat, const int* z, const double* xyz, double charge, cons
   if (!(opt.etemp >= 0.0 && std::isfinite(opt.etemp))) throw Error(1, "etemp must be >= 0");
   if (!(opt.damping > 0.0 && opt.damping <= 1.0)) throw Error(1, "damping must be in (0, 1]");
   if (opt.max_iter <= 0) throw Error(1, "max_iter must be >= 1");
-  Model m = build_model(nat, z, xyz, charge, opt.acc);
+  const bool tbl = opt.conventions == 3;
+  if (opt.conventions != 0 && !tbl) throw Error(1, "the oracle runs the axtb and tblite conventions");
+  Model m = build_model(nat, z, xyz, charge, opt.acc, tbl);
   int nsh = m.nsh, n = m.nao;
   // electrons (xtb.rs:1178-1212), closed shell / lowest spin
   double nref = 0.0;
@@ -2360,12 +2392,16 @@ Result singlepoint(int nat, const int* z, const double* xyz, double charge, cons
   if (std::fmod(charge, 1.0) != 0.0) throw Error(1, "the charge must be an integer");
   double nocc = nref - charge;
   long nel_int = std::lround(nocc);
-  double nuhf = (double)(((nel_int % 2) + 2) % 2);
+  // unpaired electrons: given (xtb.rs:1178-1212, checked against the electron count's parity),
+  // else the lowest spin
+  if (opt.uhf >= 0 && (opt.uhf % 2) != (int)(((nel_int % 2) + 2) % 2))
+    throw Error(1, "electron count and unpaired electrons are not compatible");
+  double nuhf = opt.uhf >= 0 ? (double)opt.uhf : (double)(((nel_int % 2) + 2) % 2);
   if (nocc < 0.0) throw Error(1, "negative electron count");
   if (nuhf > nocc) throw Error(1, "too many unpaired electrons");
   double nalpha = (nocc + nuhf) / 2.0, nbeta = (nocc - nuhf) / 2.0;
   if (nalpha > n + 1e-9) throw Error(1, "too many electrons for the basis");
-  double kt = opt.etemp * KB_HARTREE;
+  double kt = opt.etemp * (tbl ? KB_XTB_TBLITE : KB_HARTREE);
 
   vec g(3 * nat, 0.0), dedcn(nat, 0.0);
   vec energies(nat, 0.0), erep(nat, 0.0);
@@ -2378,6 +2414,13 @@ Result singlepoint(int nat, const int* z, const double* xyz, double charge, cons
   }
   Coulomb cou = build_coulomb(m, cn);
   D4Options d4o;
+  if (tbl) {  // mctc-lib's radii and D4's cutoffs (RealspaceCutoff::TBLITE_GFN2, disp.rs:47)
+    d4o.rcov_mode = 0;
+    d4o.cn_cut = 30.0;
+    d4o.disp2 = 50.0;
+    d4o.disp3 = 25.0;
+    d4o.width2 = d4o.width3 = 0.05;
+  }
   D4 d4 = build_d4(nat, z, xyz, d4o);
   Result res;
   res.terms[0] = repulsion(m, erep, opt.want_grad ? &g : nullptr);
@@ -2484,7 +2527,7 @@ Result singlepoint(int nat, const int* z, const double* xyz, double charge, cons
     for (int s = 0; s < nsh; s++) start.qsh[s] = opt.gq[s];
     if (opt.gdp) start.dp.assign(opt.gdp, opt.gdp + 3 * nat);
     if (opt.gqp) start.qp.assign(opt.gqp, opt.gqp + 6 * nat);
-  } else if (opt.guess == 0) {
+  } else if (opt.guess == 0 && !tbl) {  // tblite starts from zero charges (conventions.rs:335)
     try {
       vec q = eeq_charges(nat, z, xyz, charge);
       for (int iat = 0; iat < nat; iat++) {
@@ -2510,8 +2553,10 @@ Result singlepoint(int nat, const int* z, const double* xyz, double charge, cons
   int iter = 0;
   double res_norm = 0, de_last = 0;
   // one SCF per mixer of the plan, each from the start (xtb.rs:590-646): auto = Broyden, then DIIS
-  const int plan[2] = {opt.mixer == MIX_DIIS ? MIX_DIIS : MIX_BROYDEN, MIX_DIIS};
-  const int nplan = opt.mixer == MIX_AUTO ? 2 : 1;
+  // the conventions' default mixer (xtb.rs:516): axtb Broyden then DIIS, tblite Broyden
+  const int mixer = opt.mixer >= 0 ? opt.mixer : (tbl ? MIX_BROYDEN : MIX_AUTO);
+  const int plan[2] = {mixer == MIX_DIIS ? MIX_DIIS : MIX_BROYDEN, MIX_DIIS};
+  const int nplan = mixer == MIX_AUTO ? 2 : 1;
   int total_iters = 0;
   for (int ia = 0; ia < nplan && !converged; ia++) {
   const bool last_attempt = ia + 1 == nplan;
@@ -2698,7 +2743,7 @@ Result singlepoint(int nat, const int* z, const double* xyz, double charge, cons
   res.nattempts = ia + 1;
   total_iters += iters;
   iter = iters;
-  res.mixer_used = opt.mixer == MIX_AUTO && res.nattempts > 1 ? 2 : plan[res.nattempts - 1];
+  res.mixer_used = mixer == MIX_AUTO && res.nattempts > 1 ? 2 : plan[res.nattempts - 1];
   res.iterations = total_iters;
   if (report) {
     report->iterations = res.iterations;
@@ -2766,6 +2811,7 @@ Result singlepoint(int nat, const int* z, const double* xyz, double charge, cons
         double vec3[3];
         for (int k = 0; k < 3; k++) vec3[k] = m.xyz[3 * iat + k] - m.xyz[3 * jat + k];
         double r2 = dot3(vec3, vec3);
+        if (r2 > m.pair_cut2) continue;
         if (pair_screened(m.min_alpha_at[iat], m.min_alpha_at[jat], r2, m.intcut)) continue;
         if (r2 < 2.220446049250313e-16) continue;
         double rr = std::sqrt(std::sqrt(r2) / (m.rad[jat] + m.rad[iat]));
@@ -2912,8 +2958,10 @@ int oracle_singlepoint(int nat, const int* z, const double* xyz, double charge, 
                        int max_iter, int guess, int want_grad, int dispersion, int mixer, const double* gq,
                        const double* gdp, const double* gqp, double* energy, double* terms, double* grad, double* qat,
                        double* qsh, double* dp, double* qp, double* emo, int* info, double* scal, int solv_model,
-                       const char* solvent, int solv_state) {
+                       const char* solvent, int solv_state, int conventions, int uhf) {
   Options o;
+  o.conventions = conventions;
+  o.uhf = uhf;
   o.solv_model = solv_model;
   o.solvent = solvent;
   o.solv_state = solv_state;

@@ -1,0 +1,93 @@
+"""The oracle in tblite 0.8.0's conventions (crates/axtb/src/conventions.rs:325-353): kB, radii in
+mctc-lib's bohr, 25 bohr CN and repulsion cutoffs, D4 cutoffs 30/50/25 bohr with 0.05 bohr
+switches, primitive screening at accuracy 1 with atom pairs screened by the basis extent,
+zero-charge start, Broyden mixing.
+
+Pinned by axtb's own results in these conventions (program "axtb[tblite]" of the reference
+matrix, tests/golden/tblite.json from make_tblite_fixtures.py): 613 cases of s/p-element molecules,
+gas phase and ALPB/GBSA, charges, spins, electronic temperatures and accuracies; taxol (30 bohr
+across) exercises the 25-bohr cutoffs and the atom-pair screening. Iteration counts are compared
+except for SCFs whose path is chaotic (degenerate levels near 0 K, where axtb's own conventions
+differ from the reference's counts as well); gh106 (51 bohr, accuracy 1e-4, 400 Broyden
+iterations) has several SCF solutions that the mixers reach in turn and is left out.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+oracle = pytest.importorskip("oracle")
+
+TBL = json.load(open(os.path.join(GOLDEN, "tblite.json")))
+CHAOTIC = {"eta-O2-singlet-T0.1", "eta-O2-singlet-T0.11", "eta-O2-singlet-T1"}
+MULTIPLE_SOLUTIONS = {"struct-gh106"}
+# Twisted ethylene (two degenerate pi levels at the Fermi level) stopped at a loose threshold: the
+# energy is stationary and agrees to 1e-13 Eh, the charges are first order in the unconverged
+# remainder; tblite 0.8.0 itself and axtb[tblite] differ by 1.8e-6 (acc 1) and 2.5e-7 (acc 0.01) there.
+LOOSE_DEGENERATE = {"eta-ethylene-twisted-T1000-acc1": (3e-6, 6e-6), "eta-ethylene-twisted-T1000-acc0.01": (5e-7, 1e-6)}
+
+
+def run(c, **kw):
+    m = TBL["molecules"][c["molecule"]]
+    o = c["options"]
+    model = "alpb" if "alpb" in o else ("gbsa" if "gbsa" in o else None)
+    return oracle.engrad(np.array(m["numbers"]), np.array(m["positions"]), charge=float(c["charge"]),
+                         uhf=c["uhf"], acc=o.get("accuracy", 1.0), etemp=o.get("etemp", 300.0),
+                         max_iter=o.get("max_iter", 250), dispersion=o.get("dispersion", True), solvation=model,
+                         solvent=o.get(model) if model else None, state=o.get("solvation_state", "gsolv"),
+                         conventions="tblite", **kw)
+
+
+def test_reference_cases():
+    worst = np.zeros(3)
+    n = 0
+    for cid, c in sorted(TBL["cases"].items()):
+        if cid in MULTIPLE_SOLUTIONS:
+            continue
+        r = run(c, gradient=False)
+        d = [abs(r["energy"] - c["energy"]),
+             np.abs(r["qat"] - np.array(c["charges"])).max() if c["charges"] is not None else 0.0,
+             np.abs(r["dipole"] - np.array(c["dipole"])).max() if c["dipole"] is not None else 0.0]
+        worst = np.maximum(worst, d)
+        bq, bd = LOOSE_DEGENERATE.get(cid, (1e-7, 1e-6))
+        assert d[0] <= 1e-8 and d[1] <= bq and d[2] <= bd, (cid, d)
+        if cid not in CHAOTIC:
+            assert r["iterations"] == c["iterations"], (cid, r["iterations"], c["iterations"])
+        n += 1
+    print(f"{n} cases in tblite conventions vs axtb[tblite]: max |dE| %.2e |dq| %.2e |ddip| %.2e" % tuple(worst))
+
+
+def test_conventions_differ_on_taxol():
+    """The two conventions give different energies on taxol (the cutoffs, radii and kB differ),
+    each equal to the reference's result in its conventions."""
+    c = TBL["cases"]["struct-taxol"]
+    m = TBL["molecules"][c["molecule"]]
+    z, x = np.array(m["numbers"]), np.array(m["positions"])
+    a = oracle.engrad(z, x, acc=1e-6, gradient=False)
+    t = oracle.engrad(z, x, acc=1e-6, gradient=False, conventions="tblite")
+    assert abs(t["energy"] - c["energy"]) <= 1e-8
+    assert abs(a["energy"] - t["energy"]) > 1e-7
+    assert t["mixer_used"] == "broyden"
+
+
+def test_gradient_finite_differences():
+    """Central differences (h = 1e-4 bohr) of the energy in tblite conventions on all-trans C20H42
+    (47 bohr long: beyond the 25-bohr CN, repulsion and three-body cutoffs and the atom-pair
+    screening), against the analytic gradient, the reference's 2e-7 bar."""
+    from axtb_amd import structures as S
+    z, x = S.alkane(20, rng=np.random.default_rng(4), noise=0.02)
+    r = oracle.engrad(z, x, acc=1e-8, conventions="tblite")
+    rng = np.random.default_rng(2)
+    worst = 0.0
+    h = 1e-4
+    for _ in range(4):
+        d = rng.standard_normal(x.shape)
+        d /= np.linalg.norm(d)
+        ep = oracle.engrad(z, x + h * d, acc=1e-8, gradient=False, conventions="tblite")["energy"]
+        em = oracle.engrad(z, x - h * d, acc=1e-8, gradient=False, conventions="tblite")["energy"]
+        worst = max(worst, abs((ep - em) / (2 * h) - np.sum(r["gradient"] * d)))
+    print("tblite conventions, C20H42: FD error %.2e" % worst)
+    assert worst <= 2e-7
