@@ -1,0 +1,116 @@
+"""The GPU engine in tblite 0.8.0's conventions (crates/axtb/src/conventions.rs:325-353), through
+the C ABI.
+
+- Every case of axtb's reference matrix in these conventions (program "axtb[tblite]",
+  tests/golden/tblite.json: s/p-element molecules in the gas phase and in ALPB/GBSA, charges,
+  unpaired electrons, electronic temperatures 0-5000 K and accuracies 1e-6-1), one batch per set of
+  options, against axtb's own results: energy to 1e-8 Eh, charges to 1e-7, dipole to 1e-6, the same
+  iteration count (the exceptions are the oracle's, tests/test_oracle_tblite.py, and two symmetric
+  molecules whose count the reference's round-off sets, SYMMETRY_ROUNDOFF).
+- Energies and analytic gradients against the oracle in the same conventions on molecules longer
+  than the 25-bohr cutoffs and the atom-pair screening radius (C20H42, taxol), in FP64 and mixed
+  precision, and in a batch of the benchmark generator.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, structure
+
+pytestmark = pytest.mark.gpu
+
+TBL = json.load(open(os.path.join(GOLDEN, "tblite.json")))
+CHAOTIC = {"eta-O2-singlet-T0.1", "eta-O2-singlet-T0.11", "eta-O2-singlet-T1"}
+MULTIPLE_SOLUTIONS = {"struct-gh106"}
+LOOSE_DEGENERATE = {"eta-ethylene-twisted-T1000-acc1": (3e-6, 6e-6), "eta-ethylene-twisted-T1000-acc0.01": (5e-7, 1e-6)}
+# Iteration counts set by round-off: in stretched N2 (inversion symmetry) and open-shell NO (axial
+# symmetry, a half-filled degenerate pi* pair) the exact SCC keeps the symmetry, but the reference's
+# round-off seeds a symmetry-breaking mode (N2: charge transfer between the atoms; NO: xx - yy of
+# the quadrupoles) that Broyden amplifies to 1e-9 of residual near convergence, and removes one
+# iteration later. The GPU's arithmetic is symmetric between the equivalent atoms and directions,
+# so the mode is never seeded: the oracle with N2's symmetry imposed on the output moments
+# converges in 8 iterations, as the GPU does, against the reference's 9. Energies, charges and
+# dipoles are still checked.
+SYMMETRY_ROUNDOFF = {"eta-N2-stretched-T0-acc0.0001", "eta-N2-stretched-T0-acc1e-06", "eta-N2-stretched-T300-acc0.0001",
+                     "eta-N2-stretched-T300-acc1e-06", "eta-N2-stretched-T1000-acc0.0001", "spin-NO"}
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from axtb_amd import _lib
+    return _lib.Context(0)
+
+
+def _opts_key(o):
+    model = "alpb" if "alpb" in o else ("gbsa" if "gbsa" in o else None)
+    return (o.get("accuracy", 1.0), o.get("etemp", 300.0), o.get("max_iter", 250), bool(o.get("dispersion", True)),
+            model, o.get(model) if model else None, o.get("solvation_state", "gsolv"))
+
+
+def test_reference_cases(ctx):
+    from axtb_amd import _lib
+    groups = {}
+    for cid, c in sorted(TBL["cases"].items()):
+        if cid in MULTIPLE_SOLUTIONS:
+            continue
+        groups.setdefault(_opts_key(c["options"]), []).append(cid)
+    worst = np.zeros(3)
+    n = 0
+    for (acc, etemp, max_iter, disp, model, solvent, state), ids in sorted(groups.items(), key=lambda g: str(g[0])):
+        mols = []
+        for cid in ids:
+            c = TBL["cases"][cid]
+            m = TBL["molecules"][c["molecule"]]
+            mols.append((np.array(m["numbers"]), np.array(m["positions"]), float(c["charge"]),
+                         -1 if c["uhf"] is None else int(c["uhf"])))
+        opts = _lib.make_opts(accuracy=acc, etemp=etemp, max_iter=max_iter, dispersion=disp, gradient=False,
+                              solvation=model, solvent=solvent, state=state, conventions="tblite")
+        rs = ctx.engrad_batch(mols, opts)
+        for cid, r in zip(ids, rs):
+            c = TBL["cases"][cid]
+            assert r["status"] == 0, (cid, ctx.last_error())
+            d = [abs(r["energy"] - c["energy"]),
+                 np.abs(r["charges"] - np.array(c["charges"])).max() if c["charges"] is not None else 0.0,
+                 np.abs(r["dipole"] - np.array(c["dipole"])).max() if c["dipole"] is not None else 0.0]
+            worst = np.maximum(worst, d)
+            bq, bd = LOOSE_DEGENERATE.get(cid, (1e-7, 1e-6))
+            assert d[0] <= 1e-8 and d[1] <= bq and d[2] <= bd, (cid, d)
+            if cid not in CHAOTIC and cid not in SYMMETRY_ROUNDOFF:
+                assert r["iterations"] == c["iterations"], (cid, r["iterations"], c["iterations"])
+            assert r["mixer_used"] == "broyden", cid
+            n += 1
+    print(f"{n} cases in tblite conventions vs axtb[tblite]: max |dE| %.2e |dq| %.2e |ddip| %.2e" % tuple(worst),
+          flush=True)
+    assert n == len(TBL["cases"]) - len(MULTIPLE_SOLUTIONS)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_against_oracle(ctx, precision):
+    """C20H42 (47 bohr long, 0.02 A noise) and taxol (30 bohr across) one at a time, a 24-molecule
+    batch of the benchmark generator together, all in tblite conventions, against the oracle."""
+    import oracle
+    from axtb_amd import _lib, structures as S
+    bars = {"fp64": (1e-8, 1e-7), "mixed": (1e-6, 1e-5)}[precision]
+    z20, x20 = S.alkane(20, rng=np.random.default_rng(4), noise=0.02)
+    mols = [(z20, x20), structure("taxol")]
+    mols += S.batch(24, seed=11)
+    opts = _lib.make_opts(accuracy=1e-2, precision=precision, conventions="tblite")
+    rs = ctx.engrad_batch(mols, opts)
+    worst = [0.0, 0.0, 0.0]
+    for i, ((z, x), r) in enumerate(zip(mols, rs)):
+        assert r["status"] == 0, ctx.last_error()
+        o = oracle.engrad(z, x, acc=1e-2, conventions="tblite")
+        d = [abs(r["energy"] - o["energy"]), np.abs(r["gradient"] - o["gradient"]).max(),
+             np.abs(r["charges"] - o["qat"]).max()]
+        worst = [max(a, b) for a, b in zip(worst, d)]
+        assert d[0] <= bars[0] and d[1] <= bars[1] and d[2] <= 10 * bars[1], (i, d)
+        if precision == "fp64":
+            assert r["iterations"] == o["iterations"], (i, r["iterations"], o["iterations"])
+    print(f"tblite {precision} {len(mols)} molecules vs oracle: max |dE| %.2e |dg| %.2e |dq| %.2e" % tuple(worst),
+          flush=True)
+    # the conventions matter on these molecules: the axtb conventions give other energies
+    ra = ctx.engrad_batch(mols[:2], _lib.make_opts(accuracy=1e-2, precision=precision))
+    for a, t in zip(ra, rs[:2]):
+        assert abs(a["energy"] - t["energy"]) > 1e-7
