@@ -47,7 +47,8 @@ enum { MIX_BROYDEN = 0, MIX_DIIS = 1 };
 
 // STO-nG contraction with tblite's primitive normalisation (basis.rs:20-81, math.rs:108-111)
 void sto_ng(int ng, int n, int l, double zeta, double* alpha, double* coeff) {
-  int ityp = (l == 0 ? n : 4 + n) - 1;
+  int ityp = (l == 0 ? n : (l == 1 ? 4 + n : 7 + n)) - 1;
+  const double DFACT[3] = {1.0, 1.0, 3.0};  // (2l - 1)!!
   double z2 = zeta * zeta;
   for (int i = 0; i < ng; i++) {
     double a, c;
@@ -64,15 +65,16 @@ void sto_ng(int ng, int n, int l, double zeta, double* alpha, double* coeff) {
     }
     alpha[i] = a * z2;
     double top = 2.0 / PI;
-    coeff[i] = c * (std::pow(top * alpha[i], 0.75) * std::pow(std::sqrt(4.0 * alpha[i]), l));
+    coeff[i] = c * (std::pow(top * alpha[i], 0.75) * std::pow(std::sqrt(4.0 * alpha[i]), l) / std::sqrt(DFACT[l]));
   }
 }
 
-// Whether the GFN2 basis of element z has only s and p shells (what the integral kernels take).
-bool sp_element(int z) {
+// Whether the GFN2 basis of element z has shells up to d only (what the integral kernels take:
+// every element of the parametrisation, H-Rn).
+bool supported_element(int z) {
   if (z < 1 || z > AXTB_GFN2_MAX_ELEM) return false;
   for (int s = 0; s < GFN2_NSHELL[z - 1]; s++)
-    if (GFN2_ANG_SHELL[z - 1][s] > 1) return false;
+    if (GFN2_ANG_SHELL[z - 1][s] > 2) return false;
   return true;
 }
 
@@ -1040,7 +1042,11 @@ struct Engine {
       }
       bool ok = true;
       for (int a = 0; a < m.nat && ok; a++) {
-        if (!sp_element(m.z[a])) {
+        if (m.z[a] < 1 || m.z[a] > 118) {  // not an element (base/src/molecule.rs:41-43)
+          r.status = AXTB_ERR_INPUT;
+          r.msg = "invalid atomic number " + std::to_string(m.z[a]);
+          ok = false;
+        } else if (!supported_element(m.z[a])) {  // beyond the parametrisation (gfn2.rs:37-39)
           r.status = AXTB_ERR_UNSUPPORTED_ELEMENT;
           r.msg = "element " + std::to_string(m.z[a]) + " is not supported by the GPU engine";
           r.element = m.z[a];
@@ -1601,27 +1607,32 @@ struct Engine {
         if (m % INT_GROUPS + 1 < INT_GROUPS) go[m % INT_GROUPS + 1] += (long long)mols[m].nsh * mols[m].nsh;
       for (int g = 1; g < INT_GROUPS; g++) go[g] += go[g - 1];
       intgoff.upload(go, st, &stage);
-      intcnt.ensure(4 * INT_GROUPS);
-      intlist.ensure(4 * (size_t)tot_shp);
-      LAUNCH(k_zero_u32, dim3(1), dim3(256), 0, (unsigned*)intcnt.p, 4 * INT_GROUPS);
+      intcnt.ensure(NKEYS * INT_GROUPS);
+      intlist.ensure(NKEYS * (size_t)tot_shp);
+      LAUNCH(k_zero_u32, dim3(blocks(NKEYS * INT_GROUPS, 256)), dim3(256), 0, (unsigned*)intcnt.p, NKEYS * INT_GROUPS);
       LAUNCH(k_int_keys, dim3(blocks((long long)nshmax * nshmax, 64), NM), dim3(64), 0, b, intcnt.p, intlist.p,
              (long long)tot_shp, (const long long*)intgoff.p);
-      std::vector<int> hc(4 * INT_GROUPS);
+      std::vector<int> hc(NKEYS * INT_GROUPS);
       HIP_OK(hipMemcpyAsync(hc.data(), intcnt.p, sizeof(int) * hc.size(), hipMemcpyDeviceToHost, st));
       HIP_OK(hipStreamSynchronize(st));
-      auto ikey = [&](int k, auto kern) {
+      auto ikey = [&](int k, int nch, auto kern) {
         int mx = 0;
         for (int g = 0; g < INT_GROUPS; g++) mx = std::max(mx, hc[k * INT_GROUPS + g]);
         if (mx <= 0) return;
         ProfScope ps_(&prof, st, "k_integrals", k == 0 ? ibytes : 0.0);
-        hipLaunchKernelGGL(kern, dim3(blocks(mx, 64), INT_GROUPS), dim3(64), 0, st, b,
+        hipLaunchKernelGGL(kern, dim3(blocks(mx, 64), INT_GROUPS, nch), dim3(64), 0, st, b,
                            (const int2*)intlist.p + (long long)k * tot_shp, (const int*)intcnt.p + k * INT_GROUPS,
                            (const long long*)intgoff.p, (const double*)se.p, pool.p, (long long)tot_mat);
       };
-      ikey(0, k_integrals_key<0, 0>);
-      ikey(1, k_integrals_key<0, 1>);
-      ikey(2, k_integrals_key<1, 0>);
-      ikey(3, k_integrals_key<1, 1>);
+      ikey(0, 1, k_integrals_key<0, 0>);
+      ikey(1, 1, k_integrals_key<0, 1>);
+      ikey(2, 1, k_integrals_key<0, 2>);
+      ikey(3, 1, k_integrals_key<1, 0>);
+      ikey(4, 1, k_integrals_key<1, 1>);
+      ikey(5, 3, k_integrals_key<1, 2>);
+      ikey(6, 5, k_integrals_key<2, 0>);
+      ikey(7, 5, k_integrals_key<2, 1>);
+      ikey(8, 5, k_integrals_key<2, 2>);
     }
     // repulsion
     LAUNCH(k_repulsion, dim3(blocks(NA, 4)), dim3(256), 0, b, (const int*)nullptr, erep.p, grep.p);
@@ -2491,13 +2502,13 @@ struct Engine {
       for (int g = 1; g < H0_GROUPS; g++) go[g] += go[g - 1];
       h0goff.upload(go, st, &stage);
     }
-    h0cnt.ensure(4 * H0_GROUPS);
-    h0list.ensure(4 * (size_t)tot_pair);
-    LAUNCH(k_zero_u32, dim3(1), dim3(256), 0, (unsigned*)h0cnt.p, 4 * H0_GROUPS);
+    h0cnt.ensure(NKEYS * H0_GROUPS);
+    h0list.ensure(NKEYS * (size_t)tot_pair);
+    LAUNCH(k_zero_u32, dim3(blocks(NKEYS * H0_GROUPS, 256)), dim3(256), 0, (unsigned*)h0cnt.p, NKEYS * H0_GROUPS);
     LAUNCH(k_h0_keys, dim3(blocks((long long)natmax * natmax, 64), NM), dim3(64), 0, b, (const int*)okd.p, h0cnt.p,
            h0list.p, (long long)tot_pair, (const long long*)h0goff.p);
     fill(atmP.p, 5 * (long long)tot_pair);
-    std::vector<int> hcnt(4 * H0_GROUPS);
+    std::vector<int> hcnt(NKEYS * H0_GROUPS);
     HIP_OK(hipMemcpyAsync(hcnt.data(), h0cnt.p, sizeof(int) * hcnt.size(), hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
     auto h0key = [&](int k, auto kern) {
@@ -2512,8 +2523,13 @@ struct Engine {
     };
     h0key(0, k_h0_grad_key<0, 0>);
     h0key(1, k_h0_grad_key<0, 1>);
-    h0key(2, k_h0_grad_key<1, 0>);
-    h0key(3, k_h0_grad_key<1, 1>);
+    h0key(2, k_h0_grad_key<0, 2>);
+    h0key(3, k_h0_grad_key<1, 0>);
+    h0key(4, k_h0_grad_key<1, 1>);
+    h0key(5, k_h0_grad_key<1, 2>);
+    h0key(6, k_h0_grad_key<2, 0>);
+    h0key(7, k_h0_grad_key<2, 1>);
+    h0key(8, k_h0_grad_key<2, 2>);
     LAUNCH(k_pair_grad, dim3(blocks(NA, 4)), dim3(256), 0, b, (const int*)okd.p, pool.p, stride, P.p, Wm.p, se.p,
            dsedcn.p, vsh.p, vat.p, vdp.p, vqp.p, qsh_o.p, qat_o.p, dp_o.p, qp_o.p, mrad.p, dmrdcn.p, gw.p, dgw.p,
            dcn3.p, (const double*)atmP.p, (long long)tot_pair, grad.p, dedcn.p, dedcn4.p);
@@ -2753,7 +2769,8 @@ int axtbhip_basis_size(const axtb_mol* mol, int* nsh, int* nao) {
   int s = 0, a = 0;
   for (int i = 0; i < mol->nat; i++) {
     int z = mol->numbers[i];
-    if (z < 1 || z > AXTB_GFN2_MAX_ELEM) return AXTB_ERR_UNSUPPORTED_ELEMENT;
+    if (z < 1 || z > 118) return AXTB_ERR_INPUT;
+    if (z > AXTB_GFN2_MAX_ELEM) return AXTB_ERR_UNSUPPORTED_ELEMENT;
     for (int k = 0; k < GFN2_NSHELL[z - 1]; k++) {
       s++;
       a += 2 * GFN2_ANG_SHELL[z - 1][k] + 1;

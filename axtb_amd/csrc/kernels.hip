@@ -230,30 +230,40 @@ __global__ void k_eeq_rhs(Batch b, int m, const double* rhs, double* R) {
 __device__ __forceinline__ double hscale(const Batch& b, int sa, int sb) {
   const double KD[3] = {1.85, 2.23, 2.23};
   int la = b.s.l[sa], lb = b.s.l[sb];
-  double km = (KD[la] + KD[lb]) / 2.0;  // s and p shells only: kshell = average of KDIAG
+  // kshell: 2 between a d shell and an s or p shell, else the average of KDIAG (gfn2.rs:24-30)
+  double km = ((la == 2 && lb <= 1) || (lb == 2 && la <= 1)) ? 2.0 : (KD[la] + KD[lb]) / 2.0;
   double dEN = b.a.en[b.s.at[sa]] - b.a.en[b.s.at[sb]];
   double za = b.s.zeta[sa], zb = b.s.zeta[sb];
   double zij = sqrt(2.0 * sqrt(za * zb) / (za + zb));
   return zij * (km * (1.0 + 0.02 * (dEN * dEN)));
 }
 
-// One shell-pair block of the integral pools: element (mu, nu) and, off-site, its transpose with
-// the moments shifted to the bra atom.
-template <int LI, int LJ>
+// Ket rows of a shell-pair type computed by one work item: all of them for s and p pairs, one row
+// when either shell is d (keeps the accumulators of a work item at most 5 blocks x 10 moments).
+template <int LI, int LJ> struct RowSplit {
+  static constexpr bool split = LI == 2 || LJ == 2;
+  static constexpr int NR = split ? 1 : 2 * LI + 1;   // rows per work item
+  static constexpr int NCH = split ? 2 * LI + 1 : 1;  // work items per shell pair
+};
+
+// Ket rows [IM0, IM0 + NR) of one shell-pair block of the integral pools: element (mu, nu) and,
+// off-site, its transpose with the moments shifted to the bra atom.
+template <int LI, int LJ, int IM0, int NR>
 __device__ __forceinline__ void integral_block(const Batch& b, const Mol& M, int si, int sj, const double* vec,
                                                bool onsite, double hij, double* pool, long long stride) {
-  constexpr int NSI = 2 * LI + 1, NSJ = 2 * LJ + 1;
-  double blk[NSI * NSJ][10];
-  shellpair_multipole_t<LI, LJ>(b.s.nprim[si], &b.s.alpha[MAXPRIM * si], &b.s.coeff[MAXPRIM * si], b.s.nprim[sj],
-                                &b.s.alpha[MAXPRIM * sj], &b.s.coeff[MAXPRIM * sj], vec, b.intcut, blk);
+  constexpr int NSJ = 2 * LJ + 1;
+  double blk[NR * NSJ][10];
+  shellpair_multipole_t<LI, LJ, IM0, NR>(b.s.nprim[si], &b.s.alpha[MAXPRIM * si], &b.s.coeff[MAXPRIM * si],
+                                         b.s.nprim[sj], &b.s.alpha[MAXPRIM * sj], &b.s.coeff[MAXPRIM * sj], vec,
+                                         b.intcut, blk);
   const int ii = b.s.iao[si], jj = b.s.iao[sj];
   const long long n = M.nao;
 #pragma unroll
-  for (int im = 0; im < NSI; im++)
+  for (int r = 0; r < NR; r++)
 #pragma unroll
     for (int jm = 0; jm < NSJ; jm++) {
-      const double* o = blk[jm + NSJ * im];
-      long long mu = jj + jm, nu = ii + im;
+      const double* o = blk[jm + NSJ * r];
+      long long mu = jj + jm, nu = ii + IM0 + r;
       long long e1 = M.mat0 + mu + nu * n;
       pool[e1] = o[0];
       pool[stride + e1] = o[0] * hij;
@@ -275,14 +285,31 @@ __device__ __forceinline__ void integral_block(const Batch& b, const Mol& M, int
     }
 }
 
+// integral_block of the row chunk `ch` (split types: row ch; others: all rows)
+template <int LI, int LJ, int IM = 0>
+__device__ __forceinline__ void integral_rows(int ch, const Batch& b, const Mol& M, int si, int sj, const double* vec,
+                                              bool onsite, double hij, double* pool, long long stride) {
+  using RS = RowSplit<LI, LJ>;
+  if constexpr (!RS::split) {
+    integral_block<LI, LJ, 0, RS::NR>(b, M, si, sj, vec, onsite, hij, pool, stride);
+  } else if constexpr (IM < RS::NCH) {
+    if (ch == IM)
+      integral_block<LI, LJ, IM, 1>(b, M, si, sj, vec, onsite, hij, pool, stride);
+    else
+      integral_rows<LI, LJ, IM + 1>(ch, b, M, si, sj, vec, onsite, hij, pool, stride);
+  }
+}
+
 // S, H0, D (3), Q (6) of every shell pair (ish of atom A >= jsh of atom B): each AO element
 // written once (ao.rs:195-291, hamiltonian.rs:77-94). Pools are `stride` elements apart:
 // slot 0 S, 1 H0, 2-4 D, 5-10 Q; element (mu, nu) of molecule m at mat0 + mu + nu nao; elements of
 // screened shell pairs stay as they are (zero). Split by shell-pair type, so that a wavefront runs
-// one type: k_int_keys lists the shell pairs (i, j) of each key 2 l_i + l_j that pass the screening
+// one type: k_int_keys lists the shell pairs (i, j) of each key 3 l_i + l_j that pass the screening
 // (entry (m, i nsh + j); the lists grouped by molecule like k_h0_keys' lists, capacity of a group the
-// sum of its nsh^2), k_integrals_key<LI, LJ> (grid.y = group) computes them.
+// sum of its nsh^2), k_integrals_key<LI, LJ> (grid.y = group, grid.z = row chunk of the types with a
+// d shell) computes them.
 constexpr int INT_GROUPS = 64;
+constexpr int NKEYS = 9;  // shell-pair types (l_i, l_j), l <= 2
 __global__ __launch_bounds__(64) void k_int_keys(Batch b, int* cnt, int2* list, long long cap, const long long* goff) {
   const int m = blockIdx.y;
   const Mol M = b.mol[m];
@@ -303,11 +330,11 @@ __global__ __launch_bounds__(64) void k_int_keys(Batch b, int* cnt, int2* list, 
         const double a1 = b.s.minalpha[si], a2 = b.s.minalpha[sj];
         keep = !(a1 * a2 * r2 / (a1 + a2) > b.intcut * (1.0 + 1e-10)) && !(r2 > b.mol[m].pair_cut2);
       }
-      if (keep) key = 2 * b.s.l[si] + b.s.l[sj];
+      if (keep) key = 3 * b.s.l[si] + b.s.l[sj];
     }
   }
   const int lane = threadIdx.x, g = m % INT_GROUPS;
-  for (int k = 0; k < 4; k++) {
+  for (int k = 0; k < NKEYS; k++) {
     const bool has = key == k;
     const unsigned long long bal = __ballot(has);
     if (!bal) continue;
@@ -342,16 +369,21 @@ __global__ __launch_bounds__(64) void k_integrals_key(Batch b, const int2* list,
     const double sp = (1.0 + b.s.shpoly[si] * rr) * (1.0 + b.s.shpoly[sj] * rr);
     hij = 0.5 * (se[si] + se[sj]) * hscale(b, sj, si) * sp;
   }
-  integral_block<LI, LJ>(b, M, si, sj, vec, onsite, hij, pool, stride);
+  integral_rows<LI, LJ>(blockIdx.z, b, M, si, sj, vec, onsite, hij, pool, stride);
 }
-template __global__ void k_integrals_key<0, 0>(Batch, const int2*, const int*, const long long*, const double*,
-                                               double*, long long);
-template __global__ void k_integrals_key<0, 1>(Batch, const int2*, const int*, const long long*, const double*,
-                                               double*, long long);
-template __global__ void k_integrals_key<1, 0>(Batch, const int2*, const int*, const long long*, const double*,
-                                               double*, long long);
-template __global__ void k_integrals_key<1, 1>(Batch, const int2*, const int*, const long long*, const double*,
-                                               double*, long long);
+#define AXTB_INT_KEY(LI, LJ)                                                                                          \
+  template __global__ void k_integrals_key<LI, LJ>(Batch, const int2*, const int*, const long long*, const double*, \
+                                                   double*, long long);
+AXTB_INT_KEY(0, 0)
+AXTB_INT_KEY(0, 1)
+AXTB_INT_KEY(0, 2)
+AXTB_INT_KEY(1, 0)
+AXTB_INT_KEY(1, 1)
+AXTB_INT_KEY(1, 2)
+AXTB_INT_KEY(2, 0)
+AXTB_INT_KEY(2, 1)
+AXTB_INT_KEY(2, 2)
+#undef AXTB_INT_KEY
 
 // ------------------------------------------------------------------ repulsion (repulsion.rs:44-73)
 // One wavefront per atom: E_A (half of each pair) and dE/dR_A, cutoff 40 bohr (tblite 25).
@@ -1644,50 +1676,77 @@ __global__ __launch_bounds__(256) void k_pair_grad(Batch b, const int* okm, cons
 // screened shell pairs, contracted with P, W, the AO potentials and the multipole potentials
 // (hamiltonian.rs:137-223). Writes hg[x * hstride + pair0 + iat*nat + jat] for x = dg (3),
 // dE/dCN of iat, dE/dCN of jat.
+// Block coefficients of the ket rows [IM0, IM0 + NR) of a shell pair (hamiltonian.rs:169-211) and
+// their contraction with the integral derivatives; every row chunk of the type in turn (all rows
+// at once for s and p pairs). Returns sum_blk p s.
+template <int LI, int LJ, int IM0, int NR>
+__device__ __forceinline__ double h0_grad_block(const Batch& b, const Mol& M, int si, int sj, int iat, int jat,
+                                                const double* v, double hij, const double* dsv, const double* P,
+                                                const double* W, const double* vsh, const double* vat,
+                                                const double* vdp, const double* vqp, double* dg) {
+  constexpr int NSJ = 2 * LJ + 1;
+  double sval[NR * NSJ], hp2[NR * NSJ], pp[NR * NSJ];
+  const int ii = b.s.iao[si], jj = b.s.iao[sj];
+  const double vi = vsh[si] + vat[iat], vj = vsh[sj] + vat[jat];
+#pragma unroll
+  for (int r = 0; r < NR; r++)
+#pragma unroll
+    for (int jm = 0; jm < NSJ; jm++) {
+      const int blk = jm + NSJ * r;
+      long long e = M.mat0 + (jj + jm) + (long long)(ii + IM0 + r) * M.nao;
+      double pij = P[e];
+      pp[blk] = pij;
+      hp2[blk] = 2.0 * pij * hij;
+      sval[blk] = -pij * (vj + vi) + 2.0 * pij * hij - 2.0 * W[e];
+    }
+  return shellpair_grad_t<LI, LJ, IM0, NR>(b.s.nprim[si], &b.s.alpha[MAXPRIM * si], &b.s.coeff[MAXPRIM * si],
+                                           b.s.nprim[sj], &b.s.alpha[MAXPRIM * sj], &b.s.coeff[MAXPRIM * sj], v,
+                                           b.intcut, sval, hp2, pp, dsv, &vdp[3 * iat], &vdp[3 * jat], &vqp[6 * iat],
+                                           &vqp[6 * jat], dg);
+}
+
+template <int LI, int LJ, int IM = 0>
+__device__ __forceinline__ double h0_grad_rows(const Batch& b, const Mol& M, int si, int sj, int iat, int jat,
+                                               const double* v, double hij, const double* dsv, const double* P,
+                                               const double* W, const double* vsh, const double* vat,
+                                               const double* vdp, const double* vqp, double* dg) {
+  using RS = RowSplit<LI, LJ>;
+  if constexpr (!RS::split) {
+    return h0_grad_block<LI, LJ, 0, RS::NR>(b, M, si, sj, iat, jat, v, hij, dsv, P, W, vsh, vat, vdp, vqp, dg);
+  } else if constexpr (IM < RS::NCH) {
+    const double ps = h0_grad_block<LI, LJ, IM, 1>(b, M, si, sj, iat, jat, v, hij, dsv, P, W, vsh, vat, vdp, vqp, dg);
+    return ps + h0_grad_rows<LI, LJ, IM + 1>(b, M, si, sj, iat, jat, v, hij, dsv, P, W, vsh, vat, vdp, vqp, dg);
+  } else {
+    return 0.0;
+  }
+}
+
 template <int LI, int LJ>
 __device__ __forceinline__ void h0_grad_shell(const Batch& b, const Mol& M, int si, int sj, int iat, int jat,
                                               const double* v, double r2, double rr, const double* P,
                                               const double* W, const double* se, const double* dsedcn,
                                               const double* vsh, const double* vat, const double* vdp,
                                               const double* vqp, double* dg, double& dcni, double& dcnj) {
-  constexpr int NSI = 2 * LI + 1, NSJ = 2 * LJ + 1;
   double shpi = 1.0 + b.s.shpoly[si] * rr, shpj = 1.0 + b.s.shpoly[sj] * rr;
   double shp = shpi * shpj;
   double dshp = (shpi * b.s.shpoly[sj] + shpj * b.s.shpoly[si]) * 0.5 * rr / r2;
   double dsv[3] = {dshp / shp * v[0], dshp / shp * v[1], dshp / shp * v[2]};
   double hs = hscale(b, sj, si) * shp;
   double hij = 0.5 * (se[si] + se[sj]) * hs;
-  double sval[NSI * NSJ], hp2[NSI * NSJ], pp[NSI * NSJ];
-  const int ii = b.s.iao[si], jj = b.s.iao[sj];
-  const double vi = vsh[si] + vat[iat], vj = vsh[sj] + vat[jat];
-#pragma unroll
-  for (int im = 0; im < NSI; im++)
-#pragma unroll
-    for (int jm = 0; jm < NSJ; jm++) {
-      const int blk = jm + NSJ * im;
-      long long e = M.mat0 + (jj + jm) + (long long)(ii + im) * M.nao;
-      double pij = P[e];
-      pp[blk] = pij;
-      hp2[blk] = 2.0 * pij * hij;
-      sval[blk] = -pij * (vj + vi) + 2.0 * pij * hij - 2.0 * W[e];
-    }
-  double ps = shellpair_grad_t<LI, LJ>(b.s.nprim[si], &b.s.alpha[MAXPRIM * si], &b.s.coeff[MAXPRIM * si],
-                                       b.s.nprim[sj], &b.s.alpha[MAXPRIM * sj], &b.s.coeff[MAXPRIM * sj], v, b.intcut,
-                                       sval, hp2, pp, dsv, &vdp[3 * iat], &vdp[3 * jat], &vqp[6 * iat], &vqp[6 * jat],
-                                       dg);
+  const double ps = h0_grad_rows<LI, LJ>(b, M, si, sj, iat, jat, v, hij, dsv, P, W, vsh, vat, vdp, vqp, dg);
   dcni += dsedcn[si] * hs * ps;
   dcnj += dsedcn[sj] * hs * ps;
 }
 
 // The H0 / overlap part of the gradient per atom pair li > lj into hg (stride hstride: dE/dR_ij (3),
 // dE/dCN_i, dE/dCN_j), split by shell-pair type so that a wavefront runs one type: k_h0_keys lists,
-// per key 2 l_i + l_j (s and p shells), the atom pairs that hold such a shell pair after the
+// per key 3 l_i + l_j, the atom pairs that hold such a shell pair after the
 // integral screening, in H0_GROUPS lists per key (molecule m in group m % H0_GROUPS, whose list
 // starts at goff[group] within the key's region of cap entries; slots taken per wavefront, counted
 // in cnt[key * H0_GROUPS + group]: many counters, little contention); k_h0_grad_key<LI, LJ>
 // (grid.y = group) then adds that shell pair's terms to hg (zeroed before the first key).
-// Launched in key order, every atom pair accumulates its shell pairs in the order of a loop over
-// the shells of i, then of j (s before p on each atom).
+// Launched in key order, every atom pair accumulates its shell pairs type by type (the same order
+// on every run), within a type in the order of a loop over the shells of i, then of j.
 constexpr int H0_GROUPS = 64;
 __global__ __launch_bounds__(64) void k_h0_keys(Batch b, const int* okm, int* cnt, int2* list, long long cap,
                                                 const long long* goff) {
@@ -1712,12 +1771,12 @@ __global__ __launch_bounds__(64) void k_h0_keys(Batch b, const int* okm, int* cn
           for (int sj = b.a.ish[jat]; sj < b.a.ish[jat] + b.a.nsh[jat]; sj++) {
             const double a1 = b.s.minalpha[si], a2 = b.s.minalpha[sj];
             if (a1 * a2 * r2 / (a1 + a2) > b.intcut * (1.0 + 1e-10)) continue;
-            mask |= 1 << (2 * b.s.l[si] + b.s.l[sj]);
+            mask |= 1 << (3 * b.s.l[si] + b.s.l[sj]);
           }
     }
   }
   const int lane = threadIdx.x;
-  for (int k = 0; k < 4; k++) {
+  for (int k = 0; k < NKEYS; k++) {
     const bool has = (mask >> k) & 1;
     const unsigned long long bal = __ballot(has);
     if (!bal) continue;
@@ -1763,18 +1822,20 @@ __global__ __launch_bounds__(64) void k_h0_grad_key(Batch b, const int2* list, c
   hg[3 * hstride + e] = dcni;
   hg[4 * hstride + e] = dcnj;
 }
-template __global__ void k_h0_grad_key<0, 0>(Batch, const int2*, const int*, const long long*, const double*,
-                                             const double*, const double*, const double*, const double*,
-                                             const double*, const double*, const double*, double*, long long);
-template __global__ void k_h0_grad_key<0, 1>(Batch, const int2*, const int*, const long long*, const double*,
-                                             const double*, const double*, const double*, const double*,
-                                             const double*, const double*, const double*, double*, long long);
-template __global__ void k_h0_grad_key<1, 0>(Batch, const int2*, const int*, const long long*, const double*,
-                                             const double*, const double*, const double*, const double*,
-                                             const double*, const double*, const double*, double*, long long);
-template __global__ void k_h0_grad_key<1, 1>(Batch, const int2*, const int*, const long long*, const double*,
-                                             const double*, const double*, const double*, const double*,
-                                             const double*, const double*, const double*, double*, long long);
+#define AXTB_H0_KEY(LI, LJ)                                                                                       \
+  template __global__ void k_h0_grad_key<LI, LJ>(Batch, const int2*, const int*, const long long*, const double*, \
+                                                 const double*, const double*, const double*, const double*,      \
+                                                 const double*, const double*, const double*, double*, long long);
+AXTB_H0_KEY(0, 0)
+AXTB_H0_KEY(0, 1)
+AXTB_H0_KEY(0, 2)
+AXTB_H0_KEY(1, 0)
+AXTB_H0_KEY(1, 1)
+AXTB_H0_KEY(1, 2)
+AXTB_H0_KEY(2, 0)
+AXTB_H0_KEY(2, 1)
+AXTB_H0_KEY(2, 2)
+#undef AXTB_H0_KEY
 
 // Chain rules of the two coordination numbers (base/src/ncoord.rs:486-519): the GFN2 CN with
 // dE/dCN from the levels and the AES radii, and the D4 CN (Jacobian order, zero weights skipped)

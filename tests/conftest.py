@@ -9,6 +9,17 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 
+# Element and multi-element cases of axtb's reference matrix (tests/golden/elements.json) whose
+# iteration count the reference's round-off sets: the oracle's count changes when the molecule is
+# rotated (ReO4-, tetrahedral: 14 or 15; mb16-04: 60 or 61; mbheavy-113: 149-153), the SCF runs for
+# hundreds of iterations (mbheavy-103: 212, mbheavy-112: 170), or the last step is borderline
+# (Ni(CO)4: residual 2.23e-11 at iteration 17 against the bound 2e-11, converged at 18). Energies,
+# charges and dipoles are still checked.
+ELEMENT_ROUNDOFF_ITERATIONS = {"el-75-Re", "el-28-Ni", "mb16-04", "mbheavy-103", "mbheavy-112", "mbheavy-113"}
+# mbheavy-101: the reference's Broyden converges in 249 of its 250 iterations; a run that is one
+# iteration slower ends ScfNotConverged, so the case does not pin anything.
+ELEMENT_EDGE_OF_MAX_ITER = {"mbheavy-101"}
+
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP engine)")

@@ -46,6 +46,17 @@ def test_version_and_basis_size():
     nsh, nao = ctypes.c_int(), ctypes.c_int()
     assert L.axtbhip_basis_size(ctypes.byref(m), ctypes.byref(nsh), ctypes.byref(nao)) == 0
     assert (nsh.value, nao.value) == (4, 6)
+    # d shells: Fe (3d 4s 4p), S (3s 3p 3d), Au (5d 6s 6p): 3 shells and 9 functions each
+    for zz in (26, 16, 79):
+        z1 = np.array([zz], dtype=np.int32)
+        m1 = _lib.Mol(1, z1.ctypes.data, x.ctypes.data, 0.0, -1)
+        assert L.axtbhip_basis_size(ctypes.byref(m1), ctypes.byref(nsh), ctypes.byref(nao)) == 0
+        assert (nsh.value, nao.value) == (3, 9), zz
+    # Fr lies beyond the GFN2 parametrisation (UnsupportedElement), 119 and 0 are no elements (Input)
+    for zz, rc in ((87, _lib.ERR_ELEMENT), (119, _lib.ERR_INPUT), (0, _lib.ERR_INPUT)):
+        z1 = np.array([zz], dtype=np.int32)
+        m1 = _lib.Mol(1, z1.ctypes.data, x.ctypes.data, 0.0, -1)
+        assert L.axtbhip_basis_size(ctypes.byref(m1), ctypes.byref(nsh), ctypes.byref(nao)) == rc, zz
 
 
 def test_workloads():

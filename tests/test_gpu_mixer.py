@@ -75,9 +75,9 @@ def test_report_fields(ctx):
     o = oracle.engrad(z, x, acc=1e-6)
     assert np.abs(r["dipole"] - o["dipole"]).max() <= 1e-7
     assert abs(r["gap"] - o["gap"]) <= 1e-7
-    bad = ctx.engrad_batch([(np.array([26, 1], dtype=np.int32), np.array([[0.0, 0, 0], [0, 0, 3.0]]))],
+    bad = ctx.engrad_batch([(np.array([88, 1], dtype=np.int32), np.array([[0.0, 0, 0], [0, 0, 3.0]]))],
                            _lib.make_opts())[0]
-    assert bad["status"] == _lib.ERR_ELEMENT and bad["element"] == 26
+    assert bad["status"] == _lib.ERR_ELEMENT and bad["element"] == 88
 
 
 def test_single_entry_with_start(ctx):
