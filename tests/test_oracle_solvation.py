@@ -5,7 +5,8 @@
   0.6e-12 Eh, crates/solvation/tests/xtb_reference.rs:11-13), in xtb 6.7.1's conventions
   (its Angstrom factor, its reference-state density and its Born-radius quirk).
 - The solvated SCF in the axtb conventions (the corrected model) against axtb's results for every
-  reference-matrix solvation case of H/C/N/O molecules (tests/golden/solvation.json, written by
+  reference-matrix solvation case (H/C/N/O molecules and the Mg2+ and Fe3+ aqua ions;
+  tests/golden/solvation.json, written by
   make_solvation_fixtures.py): energy, the solvation term, charges, dipole, iteration count.
 - The gradient of the solvated energy by central differences (the reference's 2e-7 bar,
   crates/axtb/tests/consistency.rs:26-58), and the model's fixed-charge gradient by central
@@ -47,7 +48,7 @@ def parse_ref(path):
 
 def test_reference_files_present():
     assert len(REFS) == 74
-    assert len(SOLV["cases"]) == 475
+    assert len(SOLV["cases"]) == 483
 
 
 @pytest.mark.parametrize("path", REFS, ids=[os.path.basename(p)[:-4] for p in REFS])
@@ -60,15 +61,16 @@ def test_model_against_xtb(path):
 
 
 def test_scf_cases():
-    """All 475 cases in one test (about 2 s)."""
+    """All 483 cases in one test (about 2 s)."""
     worst = np.zeros(4)
     for cid, c in sorted(SOLV["cases"].items()):
         m = SOLV["molecules"][c["molecule"]]
-        r = oracle.engrad(np.array(m["numbers"]), np.array(m["positions"]), charge=float(c["charge"]),
+        r = oracle.engrad(np.array(m["numbers"]), np.array(m["positions"]), charge=float(c["charge"]), uhf=c["uhf"],
                           acc=c["accuracy"], gradient=False, solvation=c["model"], solvent=c["solvent"],
                           state=c["state"])
         d = [abs(r["energy"] - c["energy"]), abs(r["terms"]["solvation"] - c["terms"]["solvation"]),
-             np.abs(r["qat"] - np.array(c["charges"])).max(), np.abs(r["dipole"] - np.array(c["dipole"])).max()]
+             np.abs(r["qat"] - np.array(c["charges"])).max() if c["charges"] is not None else 0.0,
+             np.abs(r["dipole"] - np.array(c["dipole"])).max()]
         worst = np.maximum(worst, d)
         assert d[0] <= 1e-8 and d[1] <= 1e-8 and d[2] <= 1e-7 and d[3] <= 1e-6, (cid, d)
         assert r["iterations"] == c["iterations"], cid

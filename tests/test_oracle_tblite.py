@@ -4,7 +4,7 @@ switches, primitive screening at accuracy 1 with atom pairs screened by the basi
 zero-charge start, Broyden mixing.
 
 Pinned by axtb's own results in these conventions (program "axtb[tblite]" of the reference
-matrix, tests/golden/tblite.json from make_tblite_fixtures.py): 613 cases of s/p-element molecules,
+matrix, tests/golden/tblite.json from make_tblite_fixtures.py): 861 cases of molecules of H-Rn,
 gas phase and ALPB/GBSA, charges, spins, electronic temperatures and accuracies; taxol (30 bohr
 across) exercises the 25-bohr cutoffs and the atom-pair screening. Iteration counts are compared
 except for SCFs whose path is chaotic (degenerate levels near 0 K, where axtb's own conventions
@@ -23,7 +23,22 @@ oracle = pytest.importorskip("oracle")
 
 TBL = json.load(open(os.path.join(GOLDEN, "tblite.json")))
 CHAOTIC = {"eta-O2-singlet-T0.1", "eta-O2-singlet-T0.11", "eta-O2-singlet-T1"}
-MULTIPLE_SOLUTIONS = {"struct-gh106"}
+# Several SCF solutions that the mixers reach depending on round-off: gh106 (51 bohr, accuracy
+# 1e-4, 400 Broyden iterations) and Cu5 at 0 K (aufbau occupations of degenerate d levels; the
+# oracle lands 3e-4 Eh from the reference or on its solution when the molecule is rotated).
+MULTIPLE_SOLUTIONS = {"struct-gh106", "eta-Cu5-T0-acc0.0001", "eta-Cu5-T0-acc0.01", "eta-Cu5-T0-acc1",
+                      "eta-Cu5-T0-acc1e-06"}
+# Open-shell transition-metal clusters, dimers and metallocenes with near-degenerate d levels at the
+# Fermi level: the SCF path is set by round-off (the oracle's iteration count changes when the
+# molecule is rotated: Fe4 20-23, Cr2 46-51, Mn3 28-30, Fe2 19-20, ferrocenium 22-23, Fe(CN)6(3-)
+# 19-20; the Fe4 quintet at 1000 K stops one iteration either side of a residual 8 times the bound), and with it,
+# at accuracies 1 and 0.01, the charges where the loose SCF stops (up to 2.3e-5). The energy is
+# stationary and is checked to 1e-8 Eh; the charges to 1e-7 at accuracies 1e-4 and 1e-6.
+SMALL_GAP_METALS = {"eta-Fe4-T1000-acc0.01", "eta-Fe4-T1000-acc1", "eta-Fe4-T300-acc0.0001", "eta-Fe4-T300-acc0.01",
+                    "eta-Fe4-T300-acc1", "eta-Fe4-T300-acc1e-06", "eta-Fe4-T5000-acc1", "eta-Fe4-T1000-acc0.0001",
+                    "eta-Fe4-T1000-acc1e-06", "eta-Fe4-T5000-acc0.01", "eta-Fe4-T5000-acc0.0001",
+                    "eta-Fe4-T5000-acc1e-06", "spin-Fe4-uhf4", "tm-Cr2-stretched", "tm-Fe2-septet",
+                    "tm-Mn2-undectet", "tm-Mn3", "tm-ferrocenium", "tm-Fe4-quintet-T1000", "chg-3-FeCN6_3-"}
 # Twisted ethylene (two degenerate pi levels at the Fermi level) stopped at a loose threshold: the
 # energy is stationary and agrees to 1e-13 Eh, the charges are first order in the unconverged
 # remainder; tblite 0.8.0 itself and axtb[tblite] differ by 1.8e-6 (acc 1) and 2.5e-7 (acc 0.01) there.
@@ -53,8 +68,10 @@ def test_reference_cases():
              np.abs(r["dipole"] - np.array(c["dipole"])).max() if c["dipole"] is not None else 0.0]
         worst = np.maximum(worst, d)
         bq, bd = LOOSE_DEGENERATE.get(cid, (1e-7, 1e-6))
+        if cid in SMALL_GAP_METALS and c["options"].get("accuracy", 1.0) > 1e-3:
+            bq, bd = 1e-4, 1e-3
         assert d[0] <= 1e-8 and d[1] <= bq and d[2] <= bd, (cid, d)
-        if cid not in CHAOTIC:
+        if cid not in CHAOTIC and cid not in SMALL_GAP_METALS:
             assert r["iterations"] == c["iterations"], (cid, r["iterations"], c["iterations"])
         n += 1
     print(f"{n} cases in tblite conventions vs axtb[tblite]: max |dE| %.2e |dq| %.2e |ddip| %.2e" % tuple(worst))
