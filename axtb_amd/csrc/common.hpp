@@ -11,6 +11,8 @@
 namespace axtb {
 
 constexpr int MAXPRIM = 6;
+// status of a Jacobi job skipped by its run flag (neither converged nor failed)
+constexpr int JACOBI_SKIPPED = -2147483647 - 1;
 constexpr int MAXREF = 7;
 constexpr int NFREQ = 23;
 

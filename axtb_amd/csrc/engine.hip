@@ -419,10 +419,24 @@ struct Engine {
     if (ev_join) (void)hipEventDestroy(ev_join);
   }
 
-  void gemm(std::vector<GemmJob>& jobs) {
-    jobs.erase(std::remove_if(jobs.begin(), jobs.end(), [](const GemmJob& j) { return j.m <= 0 || j.n <= 0; }),
-               jobs.end());
+  // dj: the same jobs already on the device (uploaded once by the caller for repeated launches);
+  // they must all be non-empty and of one kind (FP32 updates or not), and are never sent to the
+  // library or split over K (those paths cannot honour run flags)
+  void gemm(std::vector<GemmJob>& jobs, const GemmJob* dj = nullptr) {
+    if (dj) {
+      for (const GemmJob& j : jobs)
+        if (j.m <= 0 || j.n <= 0 || j.f32upd != jobs[0].f32upd) throw DeviceError("gemm: device job list not uniform");
+    } else {
+      jobs.erase(std::remove_if(jobs.begin(), jobs.end(), [](const GemmJob& j) { return j.m <= 0 || j.n <= 0; }),
+                 jobs.end());
+    }
     if (jobs.empty()) return;
+    const bool flagged = dj || std::any_of(jobs.begin(), jobs.end(), [](const GemmJob& j) { return j.run != nullptr; });
+    auto jobs_dev = [&]() -> const GemmJob* {
+      if (dj) return dj;
+      gj.upload(jobs, st, &stage);
+      return gj.p;
+    };
     const bool any32 = std::any_of(jobs.begin(), jobs.end(), [](const GemmJob& j) { return j.f32upd != 0; });
     if (any32 && !std::all_of(jobs.begin(), jobs.end(), [](const GemmJob& j) { return j.f32upd != 0; })) {
       std::vector<GemmJob> j32, j64;
@@ -436,7 +450,7 @@ struct Engine {
     // a no-allocation call: the library keeps its own workspace). Symmetric products stay on the
     // lower tiles of the 128-tile kernel: the library's full product took longer (6.6 against 6 ms at
     // n = 6000).
-    if (!any32 && use_rocblas && !t_arena && jobs.size() == 1) {
+    if (!any32 && use_rocblas && !t_arena && jobs.size() == 1 && !flagged) {
       const GemmJob& J = jobs[0];
       if (!J.dk && !J.A2 && !J.lower && J.m >= rocblas_min && J.n >= rocblas_min && J.k >= rocblas_min) {
         if (!blas) {
@@ -453,7 +467,7 @@ struct Engine {
         return;
       }
     }
-    if (any32 && use_rocblas && !t_arena && jobs.size() == 1 && jobs[0].m >= rocblas_min &&
+    if (any32 && use_rocblas && !t_arena && jobs.size() == 1 && !flagged && jobs[0].m >= rocblas_min &&
         jobs[0].n >= rocblas_min && !jobs[0].ta && !jobs[0].tb && jobs[0].k == jobs[0].n && !jobs[0].A2 &&
         !jobs[0].dk) {
       // one large update: U and E = B - I rounded to FP32, U E by rocBLAS's sgemm (FP32 accumulation,
@@ -490,8 +504,8 @@ struct Engine {
         nn = std::max(nn, j.n);
         flops += 2.0 * j.m * j.n * (double)j.k;
       }
-      gj.upload(jobs, st, &stage);
-      LAUNCHW(flops, k_gemm128_upd32, dim3(blocks(nn, BT), blocks(mm, BT), (unsigned)jobs.size()), dim3(256), 0, gj.p);
+      const GemmJob* jp = jobs_dev();
+      LAUNCHW(flops, k_gemm128_upd32, dim3(blocks(nn, BT), blocks(mm, BT), (unsigned)jobs.size()), dim3(256), 0, jp);
       HIP_OK(hipGetLastError());
       return;
     }
@@ -509,7 +523,7 @@ struct Engine {
       flops += (j.lower && j.m == j.n ? (double)j.m * (j.m + 1) : 2.0 * j.m * j.n) * ((double)j.k + j.k2);
     // one product with few output tiles and a long inner dimension: split K over workgroups,
     // partials reduced in a fixed order
-    if (jobs.size() == 1 && !two) {
+    if (jobs.size() == 1 && !two && !flagged) {
       const GemmJob& J = jobs[0];
       const long long tiles = (long long)blocks(J.m, BT) * blocks(J.n, BT);
       const int splits = (int)std::min<long long>(std::min<long long>(256 / std::max(1LL, tiles), J.k / 256), 64);
@@ -532,7 +546,7 @@ struct Engine {
         }
         gj.upload(parts, st, &stage);
         dim3 grid(blocks(J.n, BT), blocks(J.m, BT), (unsigned)parts.size());
-        launch128(J.ta * 2 + J.tb, grid, flops);
+        launch128(J.ta * 2 + J.tb, grid, flops, gj.p);
         LAUNCH(k_gemm_reduce, dim3(blocks((long long)J.m * J.n, 256)), dim3(256), 0, (const double*)gpart.p,
                (int)parts.size(), J.m, J.n, J.C, J.ldc, J.beta);
         HIP_OK(hipGetLastError());
@@ -541,17 +555,17 @@ struct Engine {
     }
     // products with m, n <= 128: one workgroup per job, 16 x 16 granularity
     if (gemm_small && same && !two && mm <= 128 && nn <= 128 && tune_gemm_small[tune_bucket(std::max(mm, nn))] != 2) {
-      gj.upload(jobs, st, &stage);
+      const GemmJob* jp = jobs_dev();
       const int t = jobs[0].ta * 2 + jobs[0].tb;
       const dim3 grid((unsigned)jobs.size());
       if (t == 0)
-        LAUNCHW(flops, k_gemm_small_nn, grid, dim3(512), 0, gj.p);
+        LAUNCHW(flops, k_gemm_small_nn, grid, dim3(512), 0, jp);
       else if (t == 1)
-        LAUNCHW(flops, k_gemm_small_nt, grid, dim3(512), 0, gj.p);
+        LAUNCHW(flops, k_gemm_small_nt, grid, dim3(512), 0, jp);
       else if (t == 2)
-        LAUNCHW(flops, k_gemm_small_tn, grid, dim3(512), 0, gj.p);
+        LAUNCHW(flops, k_gemm_small_tn, grid, dim3(512), 0, jp);
       else
-        LAUNCHW(flops, k_gemm_small_tt, grid, dim3(512), 0, gj.p);
+        LAUNCHW(flops, k_gemm_small_tt, grid, dim3(512), 0, jp);
       HIP_OK(hipGetLastError());
       return;
     }
@@ -560,35 +574,35 @@ struct Engine {
     if (two || (same && (long long)mm * nn >= 256LL * 256 && jobs.size() <= 8) ||
         (same && in_refine && std::max(mm, nn) > 64)) {
       if (!same) throw DeviceError("gemm: mixed transposes in one launch");
-      gj.upload(jobs, st, &stage);
+      const GemmJob* jp = jobs_dev();
       dim3 grid(blocks(nn, BT), blocks(mm, BT), (unsigned)jobs.size());
       if (two) {
         if (jobs[0].ta != 0 || jobs[0].tb != 1) throw DeviceError("gemm: two-term products are A B^T + A2 B2^T");
         for (auto& j : jobs)
           if (j.dk) throw DeviceError("gemm: two-term products take no inner scaling");
-        LAUNCHW(flops, k_gemm128_nt2, grid, dim3(256), 0, gj.p);
+        LAUNCHW(flops, k_gemm128_nt2, grid, dim3(256), 0, jp);
       } else {
-        launch128(jobs[0].ta * 2 + jobs[0].tb, grid, flops);
+        launch128(jobs[0].ta * 2 + jobs[0].tb, grid, flops, jp);
       }
       HIP_OK(hipGetLastError());
       return;
     }
-    gj.upload(jobs, st, &stage);
+    const GemmJob* jp = jobs_dev();
     dim3 grid(blocks(nn, GT), blocks(mm, GT), (unsigned)jobs.size());
-    LAUNCHW(flops, k_gemm, grid, dim3(256), 0, gj.p);
+    LAUNCHW(flops, k_gemm, grid, dim3(256), 0, jp);
     HIP_OK(hipGetLastError());
   }
   DBuf<double> gpart;
   DBuf<float> f32buf;  // FP32 operands and product of a library FP32 update
-  void launch128(int t, dim3 grid, double flops) {
+  void launch128(int t, dim3 grid, double flops, const GemmJob* jp) {
     if (t == 0)
-      LAUNCHW(flops, k_gemm128_nn, grid, dim3(256), 0, gj.p);
+      LAUNCHW(flops, k_gemm128_nn, grid, dim3(256), 0, jp);
     else if (t == 1)
-      LAUNCHW(flops, k_gemm128_nt, grid, dim3(256), 0, gj.p);
+      LAUNCHW(flops, k_gemm128_nt, grid, dim3(256), 0, jp);
     else if (t == 2)
-      LAUNCHW(flops, k_gemm128_tn, grid, dim3(256), 0, gj.p);
+      LAUNCHW(flops, k_gemm128_tn, grid, dim3(256), 0, jp);
     else
-      LAUNCHW(flops, k_gemm128_tt, grid, dim3(256), 0, gj.p);
+      LAUNCHW(flops, k_gemm128_tt, grid, dim3(256), 0, jp);
   }
 
   // eigenpairs of every job's symmetric matrix (Jacobi, one workgroup per matrix)
@@ -714,6 +728,16 @@ struct Engine {
   DBuf<double> sweep_acc;  // per Jacobi mode and precision: sweeps, solves, unconverged solves
   // the device-side sweep statistics into the profile's counters
   void drain_sweeps() {
+    if (rd_cnt.p) {  // the device-driven refinements: [full, split] x [jobs, steps, failures]
+      unsigned long long c[6];
+      HIP_OK(hipMemcpy(c, rd_cnt.p, sizeof c, hipMemcpyDeviceToHost));
+      for (int k = 0; k < 2; k++) {
+        if (c[3 * k]) prof.count(k ? "refine_steps_split" : "refine_steps", (long)c[3 * k], (double)c[3 * k + 1]);
+        if (c[3 * k + 2])
+          prof.count(k ? "refine_split_fallbacks" : "refine_fallbacks", (long)c[3 * k + 2], (double)c[3 * k + 2]);
+      }
+      HIP_OK(hipMemset(rd_cnt.p, 0, sizeof c));
+    }
     if (!sweep_acc.p) return;
     double h[18];
     HIP_OK(hipMemcpy(h, sweep_acc.p, sizeof h, hipMemcpyDeviceToHost));
@@ -1882,93 +1906,118 @@ struct Engine {
   DBuf<int> mlu, mev, cfail;
   DBuf<EigJob> cjobs;
   DBuf<double> s12_, mtau, oa_emax, oa_delta;
-  DBuf<int> oa_slot;
-  DBuf<EigJob> oa_jobs;
-  // FP64 refinement steps of the eigenvectors U (job.Z) of job.A; the eigenvalues go to job.w.
-  // Steps come in pairs (Z -> T -> Z, so the refined vectors end in Z): every matrix gets
-  // oa_steps (even) steps, a matrix whose last step still corrected by more than 1e-7 (max |e_ij|;
-  // the error left is about its square) two more; one still above that is solved again by the
-  // FP64 Jacobi (returned list). Scratch: C (A U), F (S), Vw (G), Wm (I + E).
-  // Steps alternate U between Z and T; round 0 is oa_steps steps (rounded up to a pair), round 1 a
-  // pair, each accepted at max |e_ij| <= 1e-7. in_t[q] tells whether job q's vectors ended in T (odd
-  // number of steps) or in Z. Returns the positions in `ea` of the jobs not accepted.
-  std::vector<int> refine(const std::vector<EigJob>& ea, std::vector<char>& in_t, const int* cls = nullptr) {
-    // cls: the split of the orbitals into occupied / empty (k_oa_step), refined from U = I: the
-    // first step is k_oa_step alone (S = A, G = I, I + E is U), the later rounds are single steps,
-    // and a job not converged within split_steps steps is returned for a full solve
-    std::vector<EigJob> cur = ea;
-    std::vector<int> redo;
-    in_t.assign(ea.size(), 0);
-    if (ea.empty()) return redo;
+  // FP64 refinement steps of the eigenvectors U (job.Z) of job.A for batched molecules; the
+  // eigenvalues go to job.w. Scratch: C (A U), F (S), Vw (G), Wm (I + E). Steps alternate U between
+  // Z and T. Without cls: round 0 is oa_steps steps (rounded up to a pair), round 1 a pair, each
+  // accepted at max |e_ij| <= 1e-7 (the error left is about its square). With cls (the split of
+  // the orbitals into occupied / empty, k_oa_step), refined from U = I: the first step is k_oa_step
+  // alone (S = A, G = I, I + E is U), the later rounds are single steps, accepted at split_accept,
+  // for at most split_steps steps.
+  // No host read and no job list per iteration: refine_prepare uploads the step lists of every
+  // batched molecule of the SCF once (both buffer parities), each job running while rd_run[m] != 0;
+  // refine_run launches every job of every step (the others skip by flag), and k_refine_round stops
+  // the jobs at the end of each round (accepted: in_t = 1 when the vectors ended in T, 0 in Z; after
+  // the last round: fail = 1, for a solve by the FP64 Jacobi).
+  DBuf<GemmJob> rd_g;
+  DBuf<EigJob> rd_e;
+  DBuf<int> rd_mol, rd_run;
+  DBuf<unsigned long long> rd_cnt;  // [full, split] x [jobs refined, steps, failures]
+  std::vector<GemmJob> rd_L[2][4];  // per parity: A U, S and G, FP32 updates, FP64 updates
+  size_t rd_goff[2][4] = {};
+  int rd_ne = 0;
+  void refine_prepare(const std::vector<EigJob>& ea, const std::vector<int>& mol) {
+    const int NE = (int)ea.size();
+    rd_ne = NE;
+    for (auto& par : rd_L)
+      for (auto& l : par) l.clear();
+    if (!NE) return;
     for (const EigJob& j : ea)
       if (j.n > 200) throw DeviceError("refine: batched refinement called for n > 200");
-    std::vector<int> idx(ea.size());
-    for (size_t q = 0; q < ea.size(); q++) idx[q] = (int)q;
-    oa_delta.ensure(ea.size());
-    long steps_total = 0;
+    oa_delta.ensure(NE);
+    oa_emax.ensure(NE);
+    rd_run.ensure(nmol());
+    if (!rd_cnt.p) {
+      rd_cnt.ensure(6);
+      HIP_OK(hipMemsetAsync(rd_cnt.p, 0, 6 * sizeof(unsigned long long), st));
+    }
+    std::vector<EigJob> O[2], Oid;
+    for (int q = 0; q < NE; q++) {
+      const EigJob& j = ea[q];
+      const long long o0 = j.Z - Z.p;
+      const int n = j.n;
+      const int* rp = rd_run.p + mol[q];
+      for (int par = 0; par < 2; par++) {
+        double* U = (par == 0 ? Z.p : T.p) + o0;
+        double* out = (par == 0 ? T.p : Z.p) + o0;
+        GemmJob au{n, n, n, 0, 0, j.A, U, C.p + o0, nullptr, n, n, n, 1.0, 0.0};
+        GemmJob gs{n, n, n, 1, 0, U, C.p + o0, F.p + o0, nullptr, n, n, n, 1.0, 0.0};
+        GemmJob gg{n, n, n, 1, 0, U, U, Vw.p + o0, nullptr, n, n, n, 1.0, 0.0};
+        GemmJob gu{n, n, n, 0, 0, U, Wm.p + o0, out, nullptr, n, n, n, 1.0, 0.0};
+        gs.lower = gg.lower = sym_lower(n, false);
+        gu.f32upd = (upd32 && prec == AXTB_PREC_MIXED && n > 128) ? 1 : 0;  // small products stay FP64
+        au.run = gs.run = gg.run = gu.run = rp;
+        rd_L[par][0].push_back(au);
+        rd_L[par][1].push_back(gs);
+        rd_L[par][1].push_back(gg);
+        rd_L[par][gu.f32upd ? 2 : 3].push_back(gu);
+        EigJob e{n, F.p + o0, Vw.p + o0, j.w, Wm.p + o0};
+        e.run = rp;
+        O[par].push_back(e);
+      }
+      EigJob e{n, j.A, nullptr, j.w, T.p + o0};
+      e.run = rp;
+      Oid.push_back(e);
+    }
+    std::vector<GemmJob> G;
+    for (int par = 0; par < 2; par++)
+      for (int k = 0; k < 4; k++) {
+        rd_goff[par][k] = G.size();
+        G.insert(G.end(), rd_L[par][k].begin(), rd_L[par][k].end());
+      }
+    std::vector<EigJob> E;
+    E.insert(E.end(), O[0].begin(), O[0].end());
+    E.insert(E.end(), O[1].begin(), O[1].end());
+    E.insert(E.end(), Oid.begin(), Oid.end());
+    rd_g.upload(G, st, &stage);
+    rd_e.upload(E, st, &stage);
+    rd_mol.upload(mol, st, &stage);
+  }
+  // One refinement over the prepared lists: the jobs of the molecules with src[m] != 0 (their
+  // in_t and fail cleared by the caller); cls as above.
+  void refine_run(const int* src, int* in_t, int* fail, const int* cls) {
+    const int NE = rd_ne;
+    if (!NE) return;
+    LAUNCH(k_set_run, dim3(blocks(NE, 256)), dim3(256), 0, NE, (const int*)rd_mol.p, src, rd_run.p);
+    const int kind = cls ? 1 : 0;
     in_refine = true;
-    int gstep = 0;  // steps done by every job still in `cur`
-    // the split refinement: rounds of 2 (U = I and one product step), 1 and 1 steps
+    int gstep = 0;
+    // the rounds: split 2, 1, 1, ... steps; full 2 and 2
     const int nrounds = cls ? std::max(2, split_steps - 1) : 2;
-    for (int round = 0; round < nrounds && !cur.empty(); round++) {
+    for (int round = 0; round < nrounds; round++) {
       const int nsteps = round == 0 ? std::max(2, oa_steps + (oa_steps & 1)) : (cls ? 1 : 2);
-      // the split refinement converges linearly at the rate (coupling inside the blocks) / gap, so
-      // it is accepted at a tighter bound
       const double accept = cls ? split_accept : 1e-7;
+      int nst = 0;
       for (int step = 0; step < nsteps; step++, gstep++) {
         const bool ident = cls && gstep == 0;
-        std::vector<GemmJob> gau, gsg, gu;
-        std::vector<EigJob> oj;
-        for (size_t q = 0; q < cur.size(); q++) {
-          const EigJob& j = cur[q];
-          const long long o0 = j.Z - Z.p;
-          const int n = j.n;
-          double* U = (gstep % 2 == 0 ? Z.p : T.p) + o0;
-          double* out = (gstep % 2 == 0 ? T.p : Z.p) + o0;
-          if (ident) {  // S = A, G = I: I + E goes straight to the output
-            oj.push_back(EigJob{n, j.A, nullptr, j.w, out});
-            continue;
-          }
-          gau.push_back(GemmJob{n, n, n, 0, 0, j.A, U, C.p + o0, nullptr, n, n, n, 1.0, 0.0});        // A U
-          gsg.push_back(GemmJob{n, n, n, 1, 0, U, C.p + o0, F.p + o0, nullptr, n, n, n, 1.0, 0.0});   // S
-          gsg.push_back(GemmJob{n, n, n, 1, 0, U, U, Vw.p + o0, nullptr, n, n, n, 1.0, 0.0});         // G
-          gsg[gsg.size() - 2].lower = gsg.back().lower = sym_lower(n, false);  // both symmetric
-          gu.push_back(GemmJob{n, n, n, 0, 0, U, Wm.p + o0, out, nullptr, n, n, n, 1.0, 0.0});        // U'
-          gu.back().f32upd = (upd32 && prec == AXTB_PREC_MIXED && n > 128) ? 1 : 0;  // small products stay FP64
-          oj.push_back(EigJob{n, F.p + o0, Vw.p + o0, j.w, Wm.p + o0});
+        const int par = gstep % 2;
+        if (ident) {
+          LAUNCH(k_oa_step, dim3(NE), dim3(256), 0, (const EigJob*)rd_e.p + 2 * NE, oa_emax.p, oa_delta.p,
+                 (const int*)nullptr, 1, 1, cls, (const double*)w.p, oa_ovmix);
+          continue;
         }
-        oa_jobs.upload(oj, st, &stage);
-        oa_emax.ensure(oj.size());
-        oa_slot.upload(idx, st, &stage);  // the kept cluster threshold of a job lives at its index in `ea`
-        gemm(gau);
-        gemm(gsg);
-        LAUNCH(k_oa_step, dim3(oj.size()), dim3(256), 0, (const EigJob*)oa_jobs.p, oa_emax.p, oa_delta.p,
-               (const int*)oa_slot.p, gstep == 0 ? 1 : 0, ident ? 1 : 0, cls, (const double*)w.p, oa_ovmix);
-        gemm(gu);
-        if (!ident) steps_total += (long)cur.size();
+        gemm(rd_L[par][0], rd_g.p + rd_goff[par][0]);
+        gemm(rd_L[par][1], rd_g.p + rd_goff[par][1]);
+        LAUNCH(k_oa_step, dim3(NE), dim3(256), 0, (const EigJob*)rd_e.p + par * NE, oa_emax.p, oa_delta.p,
+               (const int*)nullptr, gstep == 0 ? 1 : 0, 0, cls, (const double*)w.p, oa_ovmix);
+        if (!rd_L[par][2].empty()) gemm(rd_L[par][2], rd_g.p + rd_goff[par][2]);
+        if (!rd_L[par][3].empty()) gemm(rd_L[par][3], rd_g.p + rd_goff[par][3]);
+        nst++;
       }
-      std::vector<double> h(cur.size());
-      HIP_OK(hipMemcpyAsync(h.data(), oa_emax.p, sizeof(double) * cur.size(), hipMemcpyDeviceToHost, st));
-      HIP_OK(hipStreamSynchronize(st));
-      std::vector<EigJob> next;
-      std::vector<int> nidx;
-      for (size_t q = 0; q < cur.size(); q++) {
-        if (h[q] <= accept) {
-          in_t[idx[q]] = gstep % 2;
-        } else {
-          next.push_back(cur[q]);
-          nidx.push_back(idx[q]);
-        }
-      }
-      cur.swap(next);
-      idx.swap(nidx);
+      LAUNCH(k_refine_round, dim3(blocks(NE, 256)), dim3(256), 0, NE, (const int*)rd_mol.p, rd_run.p,
+             (const double*)oa_emax.p, accept, gstep % 2, round == nrounds - 1 ? 1 : 0, nst, round == 0 ? 1 : 0, in_t,
+             fail, rd_cnt.p + 3 * kind);
     }
     in_refine = false;
-    const char* nsteps = cls ? "refine_steps_split" : "refine_steps";
-    const char* nfall = cls ? "refine_split_fallbacks" : "refine_fallbacks";
-    prof.count(nsteps, (long)ea.size(), (double)steps_total);
-    if (!idx.empty()) prof.count(nfall, (long)idx.size(), (double)idx.size());
-    return idx;
   }
 
   // The same refinement for one large matrix (the dense-solver molecules): the GEMMs on 128-tile
@@ -2077,6 +2126,10 @@ struct Engine {
 
   // ------------------------------------------------------------------------------ SCC loop
   std::vector<int> hactive;
+  std::vector<int> hsplit;          // batched molecule -> may refine for the split in the next iteration
+  DBuf<int> splitok, rflags, rg_mol, dpfp;
+  DBuf<GemmJob> sjobs;              // the per-iteration products of the batched molecules
+  DBuf<EigJob> rg_jobs;
   int iters_ = 0;
 
   // One SCF attempt of the molecules in `mask` (all when null) with the vector mixer `mixer`
@@ -2111,6 +2164,7 @@ struct Engine {
                        qin.p, (const double*)nullptr, (double*)nullptr);
     double econv = 1e-6 * o.accuracy, pconv = 2e-5 * o.accuracy;
     hactive = in_mask;
+    hsplit.assign(NM, 0);
     std::vector<GemmJob> g1(NM), g2(NM), g3v(NM), gP(NM);
     std::vector<EigJob> ejobs(NM);
     std::vector<int> isdense(NM);
@@ -2146,22 +2200,58 @@ struct Engine {
     for (int m = 0; m < NM; m++)
       if (in_mask[m]) dense_pfp[m] = 0;
     HIP_OK(hipMemcpyAsync(Bm.p, X.p, sizeof(double) * tot_mat, hipMemcpyDeviceToDevice, st));
+    // the products of every iteration for the batched molecules of this SCF (T = F B, A = B^T T,
+    // C = B U, P = C f C^T), uploaded once; each runs while its molecule is active
+    std::vector<GemmJob> sl[4];
+    for (int m = 0; m < NM; m++) {
+      if (!in_mask[m] || isdense[m]) continue;
+      GemmJob j[4] = {g1[m], g2[m], g3v[m], gP[m]};
+      j[1].lower = j[3].lower = sym_lower(mols[m].nao, false);
+      for (int k = 0; k < 4; k++) {
+        j[k].run = active.p + m;
+        sl[k].push_back(j[k]);
+      }
+    }
+    size_t sloff[4];
+    {
+      std::vector<GemmJob> all;
+      for (int k = 0; k < 4; k++) {
+        sloff[k] = all.size();
+        all.insert(all.end(), sl[k].begin(), sl[k].end());
+      }
+      sjobs.upload(all, st, &stage);
+    }
+    auto gemm_static = [&](int k) {
+      if (!sl[k].empty()) gemm(sl[k], sjobs.p + sloff[k]);
+    };
+    if (mixed) {  // the refinement's step lists of every batched molecule of this SCF, uploaded once
+      std::vector<EigJob> ball;
+      std::vector<int> ballm;
+      for (int m = 0; m < NM; m++)
+        if (in_mask[m] && !isdense[m]) {
+          ball.push_back(ejobs[m]);
+          ballm.push_back(m);
+        }
+      refine_prepare(ball, ballm);
+    }
     int it = 0;
     for (it = 1; it <= o.max_iter; it++) {
-      std::vector<GemmJob> a1, a2, a3;
+      std::vector<GemmJob> a1, a2;  // the dense molecules' products (the batched ones: gemm_static)
       std::vector<EigJob> ea;
       std::vector<int> eam;  // molecule of each entry of ea
+      bool any_active = false;
       for (int m = 0; m < NM; m++)
         if (hactive[m]) {
-          a1.push_back(g1[m]);
-          a2.push_back(g2[m]);
-          if (!isdense[m]) {
-            a3.push_back(g3v[m]);
+          any_active = true;
+          if (isdense[m]) {
+            a1.push_back(g1[m]);
+            a2.push_back(g2[m]);
+          } else {
             ea.push_back(ejobs[m]);
             eam.push_back(m);
           }
         }
-      if (a1.empty()) break;
+      if (!any_active) break;
       if (it > 1 && mixer == MIX_DIIS) {
         LAUNCH(k_diis, dim3(NM), dim3(256), 0, b, (const int*)active.p, qin.p, (const double*)dq.p, dhq.p, dhr.p,
                dbq.p, dbr.p, dbas.p, (long long)tot_vec, ddst.p, dist.p, status.p);
@@ -2202,68 +2292,60 @@ struct Engine {
       LAUNCHW(fbytes, k_fock, dim3(n_mat_tiles), dim3(256), 0, b, active.p, (const int2*)mat_tiles.p,
                          pool.p, stride, vsh.p, vat.p, vdp.p, vqp.p, F.p);
       // generalized eigenproblem in the Loewdin basis
+      gemm_static(0);
       gemm(a1);
-      for (auto& j : a2) j.lower = sym_lower(j.n, dense(j.n));  // A = B^T (F B) is symmetric
+      gemm_static(1);  // A = B^T (F B) is symmetric: lower blocks
+      for (auto& j : a2) j.lower = sym_lower(j.n, dense(j.n));
       gemm(a2);
       mirror_dense(a2);
       if (mixed) {
-        // mixed: Jacobi in FP32, then one FP64 Ogita-Aishima step U' = U (I + E) from
-        // S = U^T A U and G = U^T U (U in Z, A in Am; S -> F, G -> Vw, I + E -> Wm, U' -> T)
-        // Molecules whose previous occupations were all 0 or 2: the previous orbitals (A nearly
-        // block diagonal) refined for the occupied / empty split alone, from U = I, no Jacobi; P
-        // needs no more, W is then P F P / 2 and the orbital energies come from the blocks at the
-        // end. The others, and those not converged that way: the FP32 Jacobi and the full
-        // refinement.
-        std::vector<int> qs, qf;  // positions in ea (and a3: the same molecules in the same order)
-        if (it >= split_from && split_refine && !ea.empty()) {
-          std::vector<int> ml;
-          for (int m = 0; m < NM; m++)
-            if (hactive[m] && !isdense[m]) ml.push_back(m);
-          dmlist.upload(ml, st, &stage);
-          ocls.ensure(NO);
-          onfrac.ensure(NM);
-          LAUNCH(k_occ_class_batch, dim3(ml.size()), dim3(256), 0, b, (const int*)dmlist.p, (const double*)focc.p,
-                 ocls.p, onfrac.p);
-          std::vector<int> hnf(NM);
-          std::vector<double> hm8(8 * (size_t)NM);
-          HIP_OK(hipMemcpyAsync(hnf.data(), onfrac.p, sizeof(int) * NM, hipMemcpyDeviceToHost, st));
-          HIP_OK(hipMemcpyAsync(hm8.data(), mst.p, sizeof(double) * 8 * NM, hipMemcpyDeviceToHost, st));
-          HIP_OK(hipStreamSynchronize(st));
-          for (size_t q = 0; q < ml.size(); q++)
-            (hnf[ml[q]] == 0 && hm8[8 * ml[q] + 2] < split_residual ? qs : qf).push_back((int)q);
-        } else {
-          for (size_t q = 0; q < ea.size(); q++) qf.push_back((int)q);
-        }
-        std::vector<char> in_t(ea.size(), 0), ts, tf;
-        std::vector<EigJob> es, ef;
-        for (int q : qs) es.push_back(ea[q]);
-        for (int f : refine(es, ts, ocls.p)) qf.push_back(qs[f]);
-        std::vector<char> ok_split(ea.size(), 0);
-        for (size_t t = 0; t < qs.size(); t++) ok_split[qs[t]] = 1;
-        for (int q : qf) ok_split[q] = 0;
-        for (size_t t = 0; t < qs.size(); t++)
-          if (ok_split[qs[t]]) in_t[qs[t]] = ts[t];
-        std::sort(qf.begin(), qf.end());
-        for (int q : qf) ef.push_back(ea[q]);
-        std::vector<EigJob> redo;
-        std::vector<int> redo_mol;
-        if (fp32) {
-          eigh(ef, true, 60);
-          for (int f : refine(ef, tf)) {
-            redo.push_back(ef[f]);
-            redo_mol.push_back(eam[qf[f]]);
-          }
-          for (size_t t = 0; t < qf.size(); t++) in_t[qf[t]] = tf[t];
-        } else {  // FP64-refine: the full solves by the FP64 Jacobi
-          redo = ef;
-          for (int q : qf) redo_mol.push_back(eam[q]);
-        }
-        // not converged by the refinement: FP64 Jacobi (vectors in Z; in_t is 0 for them)
-        eigh(redo, false, -1, 1.0f, &redo_mol);
-        // C = B U from wherever U ended (a3 and ea list the same molecules in the same order)
+        // mixed: the FP32 Jacobi, then FP64 Ogita-Aishima steps U' = U (I + E) from S = U^T A U and
+        // G = U^T U (U in Z, A in Am; S -> F, G -> Vw, I + E -> Wm, U' -> T or Z).
+        // Molecules whose previous occupations were all 0 or 2 and whose last residual is below the
+        // gate (decided on the device at the end of the previous iteration and read with the active
+        // mask): the previous orbitals (A nearly block diagonal) refined for the occupied / empty
+        // split alone, from U = I, no Jacobi; P needs no more, W is then P F P / 2 and the orbital
+        // energies come from the blocks at the end. The others: the FP32 Jacobi and the full
+        // refinement (FP64-refine: the FP64 Jacobi). A refinement that does not converge: the FP64
+        // Jacobi. No host read in between: per-molecule flags (run, fail, in_t, split) set on the
+        // device and launches over fixed job lists.
+        const bool use_split = it >= split_from && split_refine;
+        std::vector<EigJob> ef;
+        int nsplit = 0;
+        std::vector<int> hflags(4 * (size_t)NM, 0);  // split | full (FP32 path) | fail | in_t
         for (size_t q = 0; q < ea.size(); q++) {
-          if (in_t[q]) a3[q].B = T.p + (ea[q].Z - Z.p);
-          dense_pfp[eam[q]] = ok_split[q];
+          const int m = eam[q];
+          const bool sp = use_split && hsplit[m];
+          if (sp)
+            nsplit++;
+          else
+            ef.push_back(ea[q]);
+          hflags[m] = sp ? 1 : 0;
+          hflags[(size_t)NM + m] = (!sp && fp32) ? 1 : 0;
+          hflags[2 * (size_t)NM + m] = (!sp && !fp32) ? 1 : 0;
+        }
+        if (!ea.empty()) {
+          rflags.upload(hflags, st, &stage);
+          const int* f_split = rflags.p;
+          const int* f_full = rflags.p + NM;
+          int* f_fail = rflags.p + 2 * (size_t)NM;
+          int* f_int = rflags.p + 3 * (size_t)NM;
+          if (nsplit) refine_run(f_split, f_int, f_fail, ocls.p);
+          if (fp32 && !ef.empty()) {
+            eigh(ef, true, 60);
+            refine_run(f_full, f_int, f_fail, nullptr);
+          }
+          // not converged by a refinement, and FP64-refine's full solves: the FP64 Jacobi (vectors in Z)
+          std::vector<EigJob> redo = ea;
+          for (size_t q = 0; q < redo.size(); q++) redo[q].run = f_fail + eam[q];
+          eigh(redo, false, -1, 1.0f, &eam);
+          // C = B U reads Z: the refined vectors that ended in T are copied there
+          dpfp.ensure(NM);
+          rg_jobs.upload(ea, st, &stage);
+          rg_mol.upload(eam, st, &stage);
+          LAUNCH(k_refine_gather, dim3((unsigned)ea.size()), dim3(256), 0, (const EigJob*)rg_jobs.p,
+                 (const int*)rg_mol.p, (long long)(T.p - Z.p), (const int*)f_int, (const int*)f_fail, f_split,
+                 dpfp.p);
         }
       } else {
         eigh(ea, false, -1, 1.0f, &eam);
@@ -2293,7 +2375,7 @@ struct Engine {
         dfull[m] = it < dense_full ? 1 : 0;
         dense_pfp[m] = drefined[m];
       }
-      gemm(a3);
+      gemm_static(2);  // C = B U
       HIP_OK(hipMemcpyAsync(Bm.p, C.p, sizeof(double) * tot_mat, hipMemcpyDeviceToDevice, st));
       LAUNCH(k_fermi, dim3(NM), dim3(256), 0, b, active.p, w.p, focc.p, fwork.p, fst.p, kocc.p,
                          status.p);
@@ -2304,9 +2386,9 @@ struct Engine {
       }
       std::vector<GemmJob> a4, a5;
       for (int m = 0; m < NM; m++) {
-        if (!hactive[m]) continue;
+        if (!hactive[m] || !isdense[m]) continue;
         GemmJob p = gP[m];
-        if (isdense[m]) {
+        {
           const int n = mols[m].nao, k = hkocc[m];
           GemmJob c = g3v[m];
           if (!drefined[m]) {
@@ -2323,8 +2405,9 @@ struct Engine {
         a4.push_back(p);
       }
       gemm(a5);
+      gemm_static(3);  // P = C diag(f) C^T
       for (auto& j : a4) j.lower = sym_lower(j.n, dense(j.n));
-      gemm(a4);  // P = C diag(f) C^T
+      gemm(a4);
       mirror_dense(a4);
       LAUNCHW(sbytes, k_mulliken_cols, dim3(blocks(NO, 4)), dim3(256), 0, b, active.p, pool.p, stride, P.p,
                          cols.p);
@@ -2344,6 +2427,26 @@ struct Engine {
                eel.p, eterm.p);
       LAUNCH(k_converge, dim3(NM), dim3(256), 0, b, active.p, eel.p, dq.p, fst.p, econv, pconv, mst.p,
                          status.p);
+      // mixed: which batched molecules may refine for the occupied / empty split in the next
+      // iteration (occupation classes of this iteration's orbitals, this iteration's residual),
+      // read with the active mask: the one host read of an iteration
+      const bool next_split = mixed && split_refine && it + 1 >= split_from;
+      if (next_split) {
+        std::vector<int> ml;
+        for (int m = 0; m < NM; m++)
+          if (hactive[m] && !isdense[m]) ml.push_back(m);
+        if (!ml.empty()) {
+          dmlist.upload(ml, st, &stage);
+          ocls.ensure(NO);
+          onfrac.ensure(NM);
+          splitok.ensure(NM);
+          LAUNCH(k_occ_class_batch, dim3(ml.size()), dim3(256), 0, b, (const int*)dmlist.p, (const double*)focc.p,
+                 ocls.p, onfrac.p);
+          LAUNCH(k_split_ok, dim3(blocks((long long)ml.size(), 256)), dim3(256), 0, (const int*)dmlist.p,
+                 (int)ml.size(), (const int*)onfrac.p, (const double*)mst.p, split_residual, splitok.p);
+          HIP_OK(hipMemcpyAsync(hsplit.data(), splitok.p, sizeof(int) * NM, hipMemcpyDeviceToHost, st));
+        }
+      }
       HIP_OK(hipGetLastError());
       HIP_OK(hipMemcpyAsync(hactive.data(), active.p, sizeof(int) * NM, hipMemcpyDeviceToHost, st));
       HIP_OK(hipStreamSynchronize(st));
@@ -2352,6 +2455,13 @@ struct Engine {
       if (!any) break;
     }
     iters_ = it;
+    if (mixed && dpfp.p) {  // batched molecules: whether their last iteration was a converged split refinement
+      std::vector<int> hp(NM);
+      HIP_OK(hipMemcpyAsync(hp.data(), dpfp.p, sizeof(int) * NM, hipMemcpyDeviceToHost, st));
+      HIP_OK(hipStreamSynchronize(st));
+      for (int m = 0; m < NM; m++)
+        if (in_mask[m] && !isdense[m]) dense_pfp[m] = hp[m];
+    }
     // refined dense molecules: the orbital energies of the last iteration's matrix (the refinement
     // leaves Rayleigh quotients inside the occupied and empty blocks; P, W and the energy do not
     // use them) as the eigenvalues of the occupied and the empty block of S = U^T A U from the last

@@ -1535,15 +1535,19 @@ __global__ void k_attempt_start(const int* mask, int nmol, int* active, int* itc
 __global__ void k_sweep_acc(const int* stat, int n, double* acc) {
   __shared__ double red[16];
   double a = 0.0, c = 0.0;
+  double nj = 0.0;
   for (int q = threadIdx.x; q < n; q += blockDim.x) {
+    if (stat[q] == JACOBI_SKIPPED) continue;
     a += abs(stat[q]);
     c += stat[q] < 0;
+    nj += 1.0;
   }
   a = block_sum(a, red);
   c = block_sum(c, red);
+  nj = block_sum(nj, red);
   if (threadIdx.x == 0) {
     acc[0] += a;
-    acc[1] += n;
+    acc[1] += nj;
     acc[2] += c;
   }
 }
@@ -1552,7 +1556,7 @@ __global__ void k_sweep_acc(const int* stat, int n, double* acc) {
 // status bit 16 (Error::Numerical of the eigensolver).
 __global__ void k_jacobi_flag(const int* stat, const int* mol_of_job, int njobs, int* status) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q < njobs && stat[q] < 0) atomicOr(&status[mol_of_job[q]], 16);
+  if (q < njobs && stat[q] < 0 && stat[q] != JACOBI_SKIPPED) atomicOr(&status[mol_of_job[q]], 16);
 }
 
 // ------------------------------------------------------------------ gradient

@@ -25,7 +25,12 @@ struct GemmJob {
   // the refinement update C = A + A (B - I) with the product on the FP32 matrix cores (A, B square
   // n x n, no transposes; k_gemm128_upd32)
   int f32upd = 0;
+  // optional device flag: the job runs only while *run != 0 (set and cleared by the device, so that
+  // a launch over a fixed job list needs no host decision)
+  const int* run = nullptr;
 };
+
+__device__ __forceinline__ bool job_skipped(const int* run) { return run && *run == 0; }
 
 typedef double d4v __attribute__((ext_vector_type(4)));
 
@@ -43,7 +48,7 @@ constexpr int GK = 16;   // k step
 __global__ __launch_bounds__(256) void k_gemm(const GemmJob* jobs) {
   const GemmJob J = jobs[blockIdx.z];
   const int row0 = blockIdx.y * GT, col0 = blockIdx.x * GT;
-  if (row0 >= J.m || col0 >= J.n) return;
+  if (row0 >= J.m || col0 >= J.n || job_skipped(J.run)) return;
   __shared__ double As[2][GK][GT + 1];
   __shared__ double Bs[2][GK][GT + 1];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -163,7 +168,7 @@ template <int TA, int TB, bool TWO>
 __global__ __launch_bounds__(256, 2) void k_gemm128(const GemmJob* jobs) {
   const GemmJob J = jobs[blockIdx.z];
   const int row0 = blockIdx.y * BT, col0 = blockIdx.x * BT;
-  if (row0 >= J.m || col0 >= J.n) return;
+  if (row0 >= J.m || col0 >= J.n || job_skipped(J.run)) return;
   if (J.lower && col0 > row0 + BT - 1) return;
   __shared__ double As[2][BK][BP];
   __shared__ double Bs[2][BK][BP];
@@ -299,7 +304,7 @@ typedef float f4v __attribute__((ext_vector_type(4)));
 __global__ __launch_bounds__(256, 2) void k_gemm128_upd32(const GemmJob* jobs) {
   const GemmJob J = jobs[blockIdx.z];
   const int row0 = blockIdx.y * BT, col0 = blockIdx.x * BT;
-  if (row0 >= J.m || col0 >= J.n) return;
+  if (row0 >= J.m || col0 >= J.n || job_skipped(J.run)) return;
   __shared__ float As[2][BK][BP];
   __shared__ float Bs[2][BK][BP];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -411,6 +416,7 @@ constexpr int SP = 129;  // padded LDS row of the small-matrix kernel (odd, as B
 template <int TA, int TB>
 __global__ __launch_bounds__(512, 4) void k_gemm_small(const GemmJob* jobs) {
   const GemmJob J = jobs[blockIdx.x];
+  if (job_skipped(J.run)) return;
   __shared__ double As[2][BK][SP];
   __shared__ double Bs[2][BK][SP];
   // the wavefront index as a scalar: the block lists and the block branches are then uniform
@@ -555,7 +561,9 @@ struct EigJob {
   double* V;      // n x n work (ld n)
   double* w;      // eigenvalues, ascending
   double* Z;      // eigenvectors (ld n), columns in the order of w; null: eigenvalues only (V unused)
+  const int* run = nullptr;  // optional device flag: the job runs only while *run != 0
 };
+
 
 // Pair k of round r of the round-robin tournament over nn (even) indices: index 0 stays, the
 // others rotate (r < nn - 1, so one conditional subtraction replaces the modulo).
@@ -609,6 +617,10 @@ __global__ __launch_bounds__(1024) void k_jacobi(const EigJob* jobs, int max_swe
   __shared__ T anorm;
   const EigJob J = jobs[blockIdx.x];
   const int n = J.n, tid = threadIdx.x, nt = blockDim.x;
+  if (job_skipped(J.run)) {
+    if (tid == 0) status[blockIdx.x] = JACOBI_SKIPPED;
+    return;
+  }
   if (n <= 0) return;
   const int nn = n + (n & 1), np = nn / 2;
   const int lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
@@ -852,7 +864,9 @@ __global__ __launch_bounds__(256) void k_oa_step(const EigJob* jobs, double* ema
   __shared__ double red[16];
   __shared__ double lam[200];  // batched eigenproblems have n <= 200
   __shared__ signed char cl[200];
-  const EigJob J = jobs[blockIdx.x];
+  const int jq = blockIdx.x;
+  const EigJob J = jobs[jq];
+  if (job_skipped(J.run)) return;
   const int n = J.n, tid = threadIdx.x, nt = blockDim.x;
   const double* S = J.A;
   const double* G = J.V;
@@ -897,10 +911,11 @@ __global__ __launch_bounds__(256) void k_oa_step(const EigJob* jobs, double* ema
     // cluster is not rotated later from a smaller delta; pairs whose first-order mixing would
     // exceed 0.05 are treated as clusters too
     delta = 2.0 * (sqrt(so) + am * sqrt(ro));
+    const int sl = slot ? slot[jq] : jq;
     if (first) {
-      if (tid == 0) delta_keep[slot[blockIdx.x]] = delta;
+      if (tid == 0) delta_keep[sl] = delta;
     } else {
-      delta = fmax(delta, delta_keep[slot[blockIdx.x]]);
+      delta = fmax(delta, delta_keep[sl]);
     }
   }
   // emax: the largest correction (pairs left unrotated as clusters are not counted: a coupling
@@ -946,8 +961,64 @@ __global__ __launch_bounds__(256) void k_oa_step(const EigJob* jobs, double* ema
   if (tid == 0) {
     double m = 0.0;
     for (int w = 0; w < (int)(nt >> 6); w++) m = fmax(m, red[w]);
-    emax[blockIdx.x] = m;
+    emax[jq] = m;
   }
+}
+
+// Device control of the batched refinement (no host reads between its rounds). Flags are per
+// molecule: run[m] while molecule m refines, in_t[m] = 1 when its refined vectors ended in the
+// second buffer (T) instead of Z, fail[m] = 1 when it did not converge in its rounds.
+// End of a round: a running job whose largest correction is within `accept` stops (its vectors are
+// in the buffer of `parity`); after the last round the others stop as failures. cnt (nullable)
+// accumulates [jobs refined, steps, failures].
+__global__ void k_refine_round(int nj, const int* mol, int* run, const double* emax, double accept, int parity,
+                               int last, int nst, int first_round, int* in_t, int* fail, unsigned long long* cnt) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nj) return;
+  const int m = mol[q];
+  if (!run[m]) return;
+  if (cnt) {
+    if (first_round) atomicAdd(&cnt[0], 1ull);
+    atomicAdd(&cnt[1], (unsigned long long)nst);
+  }
+  if (emax[q] <= accept) {
+    run[m] = 0;
+    in_t[m] = parity;
+  } else if (last) {
+    run[m] = 0;
+    fail[m] = 1;
+    if (cnt) atomicAdd(&cnt[2], 1ull);
+  }
+}
+
+// run[m] = src[m] for the molecules of the job list (the next refinement's participants)
+__global__ void k_set_run(int nj, const int* mol, const int* src, int* run) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < nj) run[mol[q]] = src[mol[q]];
+}
+
+// The refined vectors of the jobs whose result is in T (in_t and not failed) copied to Z, so that
+// the products after the eigenproblem read Z for every molecule; pfp[m] (nullable) records whether
+// molecule m ended its iteration with a converged split refinement (split[m] and not failed).
+__global__ void k_refine_gather(const EigJob* jobs, const int* mol, long long t_minus_z, const int* in_t,
+                                const int* fail, const int* split, int* pfp) {
+  const EigJob J = jobs[blockIdx.x];
+  const int m = mol[blockIdx.x];
+  if (pfp && threadIdx.x == 0) pfp[m] = (split[m] && !fail[m]) ? 1 : 0;
+  if (!in_t[m] || fail[m]) return;
+  const long long nn = (long long)J.n * J.n;
+  const double* src = J.Z + t_minus_z;
+  for (long long e = threadIdx.x; e < nn; e += blockDim.x) J.Z[e] = src[e];
+}
+
+// Whether molecule m may refine its previous orbitals for the occupied / empty split in the next
+// SCC iteration: no fractional occupation and the last residual (mst[8 m + 2]) below `gate`;
+// ok[m] = 0 for molecules not in mlist.
+__global__ void k_split_ok(const int* mlist, int nm, const int* nfrac, const double* mst, double gate, int* ok) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nm) return;
+  const int m = mlist[q];
+  ok[m] = (nfrac[m] == 0 && mst[8 * m + 2] < gate) ? 1 : 0;
 }
 
 // The Ogita-Aishima step of k_oa_step for one large matrix, over many workgroups:
