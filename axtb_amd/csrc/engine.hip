@@ -1816,6 +1816,7 @@ struct Engine {
           int n = mols[m].nao;
           gjobs.push_back(GemmJob{n, n, n, 0, 1, Z.p + o0, Z.p + o0, X.p + o0, s12_.p + mols[m].ao0, n, n, n, 1.0, 0.0});
         }
+        largest_first(gjobs);
         gemm(gjobs);
       }
     }
@@ -1971,6 +1972,7 @@ struct Engine {
     std::vector<GemmJob> G;
     for (int par = 0; par < 2; par++)
       for (int k = 0; k < 4; k++) {
+        largest_first(rd_L[par][k]);
         rd_goff[par][k] = G.size();
         G.insert(G.end(), rd_L[par][k].begin(), rd_L[par][k].end());
       }
@@ -2069,6 +2071,13 @@ struct Engine {
   // symmetric products: small ones (n <= 128) on the lower 16 x 16 blocks of k_gemm_small, which
   // writes the mirror itself; dense-solver molecules' on their lower 128-tiles, mirrored here
   static int sym_lower(int n, bool dense_mol) { return (dense_mol || n <= 128) ? 1 : 0; }
+  // a list of per-molecule products in launch order, largest first (one workgroup per product on
+  // the small-matrix kernel: the long workgroups start first and the short ones fill the end)
+  static void largest_first(std::vector<GemmJob>& v) {
+    std::stable_sort(v.begin(), v.end(), [](const GemmJob& a, const GemmJob& b) {
+      return (long long)a.m * a.n * (a.k + a.k2) > (long long)b.m * b.n * (b.k + b.k2);
+    });
+  }
   void mirror_dense(const std::vector<GemmJob>& jobs) {
     for (const GemmJob& j : jobs)
       if (j.lower && j.m > 128) {
@@ -2216,6 +2225,7 @@ struct Engine {
     {
       std::vector<GemmJob> all;
       for (int k = 0; k < 4; k++) {
+        largest_first(sl[k]);
         sloff[k] = all.size();
         all.insert(all.end(), sl[k].begin(), sl[k].end());
       }
@@ -2530,6 +2540,7 @@ struct Engine {
 
   // ------------------------------------------------------------------------------ gradient
   std::vector<int> hok;
+  DBuf<int> gokd, gpfd;  // the gradient's molecule masks (converged without errors; W = P F P / 2)
 
   void gradient() {
     Batch b = batch();
@@ -2569,36 +2580,30 @@ struct Engine {
       gW.push_back(GemmJob{n, n, k, 0, 1, C.p + M.mat0, C.p + M.mat0, Wm.p + M.mat0, fwork.p + M.ao0, n, n, n, 1.0, 0.0});
     }
     if (anypfp) {
-      DBuf<int> pfd;
-      pfd.upload(hpfp, st);
+      DBuf<int>& pfd = gpfd;
+      pfd.upload(hpfp, st, &stage);
       LAUNCH(k_fock, dim3(n_mat_tiles), dim3(256), 0, b, (const int*)pfd.p, (const int2*)mat_tiles.p, pool.p,
              stride, vsh.p, vat.p, vdp.p, vqp.p, F.p);
+      largest_first(gF);
+      largest_first(gW2);
       gemm(gF);
       for (auto& j : gW2) j.lower = 1;  // W = P (F P) / 2 is symmetric
       gemm(gW2);
       mirror_dense(gW2);
       HIP_OK(hipStreamSynchronize(st));
     }
-    DBuf<int> okd;
-    okd.upload(hok, st);
+    DBuf<int>& okd = gokd;
+    okd.upload(hok, st, &stage);
     // focc * eps into fwork (k_fermi's work is no longer needed)
-    std::vector<double> hf(naos()), hw(naos());
-    HIP_OK(hipMemcpyAsync(hf.data(), focc.p, sizeof(double) * naos(), hipMemcpyDeviceToHost, st));
-    HIP_OK(hipMemcpyAsync(hw.data(), w.p, sizeof(double) * naos(), hipMemcpyDeviceToHost, st));
-    HIP_OK(hipStreamSynchronize(st));
-    for (int i = 0; i < naos(); i++) hf[i] *= hw[i];
-    HIP_OK(hipMemcpyAsync(fwork.p, hf.data(), sizeof(double) * naos(), hipMemcpyHostToDevice, st));
+    LAUNCH(k_pairwise, dim3(std::min<long long>(blocks(naos(), 256), 4096)), dim3(256), 0, (const double*)focc.p,
+           (const double*)w.p, (long long)naos(), 1, fwork.p);
     for (auto& j : gW) j.lower = sym_lower(j.n, dense(j.n));
+    largest_first(gW);
     gemm(gW);
     mirror_dense(gW);
     // gradient = repulsion + three-body (setup) + pair terms + CN chains
-    HIP_OK(hipMemcpyAsync(grad.p, grep.p, sizeof(double) * 3 * NA, hipMemcpyDeviceToDevice, st));
-    std::vector<double> hg3(3 * NA), hgr(3 * NA);
-    HIP_OK(hipMemcpyAsync(hg3.data(), g3.p, sizeof(double) * 3 * NA, hipMemcpyDeviceToHost, st));
-    HIP_OK(hipMemcpyAsync(hgr.data(), grep.p, sizeof(double) * 3 * NA, hipMemcpyDeviceToHost, st));
-    HIP_OK(hipStreamSynchronize(st));
-    for (int i = 0; i < 3 * NA; i++) hgr[i] += hg3[i];
-    HIP_OK(hipMemcpyAsync(grad.p, hgr.data(), sizeof(double) * 3 * NA, hipMemcpyHostToDevice, st));
+    LAUNCH(k_pairwise, dim3(std::min<long long>(blocks(3LL * NA, 256), 4096)), dim3(256), 0, (const double*)grep.p,
+           (const double*)g3.p, 3LL * NA, 0, grad.p);
     LAUNCH(k_d4_weights, dim3(blocks(NA, 128)), dim3(128), 0, b, (const int*)okd.p, cnd4.p, qat_o.p,
                        gw.p, dgw.p, 1);
     int natmax = 0;

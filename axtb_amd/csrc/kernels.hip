@@ -16,6 +16,13 @@ __global__ void k_fill(double* p, long long n, double v) {
   for (; i < n; i += st) p[i] = v;
 }
 
+// out[i] = a[i] * b[i] (mul) or a[i] + b[i]
+__global__ void k_pairwise(const double* a, const double* b, long long n, int mul, double* out) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long st = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += st) out[i] = mul ? a[i] * b[i] : a[i] + b[i];
+}
+
 // ------------------------------------------------------------------ counting functions
 __device__ __forceinline__ double exp_count(double k, double r, double r0) { return 1.0 / (1.0 + exp(-k * (r0 / r - 1.0))); }
 __device__ __forceinline__ double exp_dcount(double k, double r, double r0) {

@@ -407,39 +407,30 @@ __global__ void k_gemm_reduce(const double* P, int splits, int m, int n, double*
   *out = beta == 0.0 ? s : s + beta * *out;
 }
 
-// Batched products of small matrices (m, n <= 128): one workgroup of 8 wavefronts per job, the
-// output as ceil(m/16) x ceil(n/16) MFMA blocks dealt round-robin to the wavefronts (so no
-// wavefront works on padding beyond the next multiple of 16), K in steps of 16 through
-// double-buffered LDS with the next step's loads in registers (clamped addresses, masks applied at
-// the store, as in k_gemm128). Few registers per thread: two workgroups share a CU.
+// Batched products of small matrices (m, n <= 128): one workgroup of 8 wavefronts per job. The
+// 8 x 8 grid of 16 x 16 MFMA output blocks is split 4 x 2 over the wavefronts: wavefront
+// (wr, wc) owns block rows wr and wr + 4 and block columns wc, wc + 2, wc + 4, wc + 6 (blocks past
+// the matrix edge skipped), so each k sub-step reads 2 A and 4 B fragments from LDS for up to 8
+// MFMAs. K in steps of 16 through double-buffered LDS with the next step's loads in registers
+// (clamped addresses, masks applied at the store, as in k_gemm128). Few registers per thread: two
+// workgroups share a CU.
 constexpr int SP = 129;  // padded LDS row of the small-matrix kernel (odd, as BP)
 template <int TA, int TB>
-__global__ __launch_bounds__(512, 4) void k_gemm_small(const GemmJob* jobs) {
+__global__ __launch_bounds__(512) void k_gemm_small(const GemmJob* jobs) {
   const GemmJob J = jobs[blockIdx.x];
   if (job_skipped(J.run)) return;
   __shared__ double As[2][BK][SP];
   __shared__ double Bs[2][BK][SP];
-  // the wavefront index as a scalar: the block lists and the block branches are then uniform
+  // the wavefront index as a scalar: the block branches are then uniform
   const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wr = w & 3, wc = w >> 2;
   const int mb = (J.m + 15) >> 4, nbk = (J.n + 15) >> 4;
   // lower (square C): only the blocks on or below the diagonal, the others written as their mirror
   const bool low = J.lower && J.m == J.n && J.beta == 0.0;
-  const int nblk = low ? mb * (mb + 1) / 2 : mb * nbk;
-  int bi_q[8], bj_q[8];
-#pragma unroll
-  for (int q = 0; q < 8; q++) {
-    const int bq = w + 8 * q;
-    if (low) {
-      int bi = (int)((sqrt(8.0 * bq + 1.0) - 1.0) * 0.5);
-      while ((bi + 1) * (bi + 2) / 2 <= bq) bi++;
-      while (bi * (bi + 1) / 2 > bq) bi--;
-      bi_q[q] = bi;
-      bj_q[q] = bq - bi * (bi + 1) / 2;
-    } else {
-      bi_q[q] = bq % mb;
-      bj_q[q] = bq / mb;
-    }
-  }
+  auto live = [&](int i, int j) {
+    const int bi = wr + 4 * i, bj = wc + 2 * j;
+    return bi < mb && bj < nbk && (!low || bi >= bj);
+  };
   // element q (of 4) of this thread's share of a k-step: (row / col within 128, k within 16)
   auto a_rk = [&](int q, int& r, int& k) {
     if (TA == 0) {
@@ -497,9 +488,11 @@ __global__ __launch_bounds__(512, 4) void k_gemm_small(const GemmJob* jobs) {
       Bs[buf][k][c] = (mbk >> q) & 1u ? rb[q] : 0.0;
     }
   };
-  d4v acc[8];
+  d4v acc[2][4];
 #pragma unroll
-  for (int q = 0; q < 8; q++) acc[q] = (d4v){0.0, 0.0, 0.0, 0.0};
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[i][j] = (d4v){0.0, 0.0, 0.0, 0.0};
   const int nk = (J.k + BK - 1) / BK;
   if (nk > 0) {
     load(0);
@@ -512,35 +505,39 @@ __global__ __launch_bounds__(512, 4) void k_gemm_small(const GemmJob* jobs) {
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       const int kl = kk + (lane >> 4), li = lane & 15;
+      // rows 16 (wr + 4 i) + li and columns 16 (wc + 2 j) + li stay below SP: no bounds test
+      double a[2], b[4];
 #pragma unroll
-      for (int q = 0; q < 8; q++) {
-        const int bq = w + 8 * q;
-        if (bq < nblk) {
-          const double a = As[buf][kl][16 * bi_q[q] + li], b = Bs[buf][kl][16 * bj_q[q] + li];
-          acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[q], 0, 0, 0);
-        }
-      }
+      for (int i = 0; i < 2; i++) a[i] = As[buf][kl][16 * (wr + 4 * i) + li];
+#pragma unroll
+      for (int j = 0; j < 4; j++) b[j] = Bs[buf][kl][16 * (wc + 2 * j) + li];
+#pragma unroll
+      for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+          if (live(i, j)) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
     }
     if (kt + 1 < nk) store(buf ^ 1);
     __syncthreads();
   }
 #pragma unroll
-  for (int q = 0; q < 8; q++) {
-    const int bq = w + 8 * q;
-    if (bq >= nblk) continue;
-    const int bi = bi_q[q], bj = bj_q[q];
+  for (int i = 0; i < 2; i++)
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int gr = 16 * bi + (lane >> 4) + 4 * r, gc = 16 * bj + (lane & 15);
-      if (gr < J.m && gc < J.n) {
-        double* c = &J.C[gr + (long long)gc * J.ldc];
-        const double v = J.alpha * acc[q][r];
-        const double o = J.beta == 0.0 ? v : v + J.beta * *c;
-        *c = o;
-        if (low && bi != bj) J.C[gc + (long long)gr * J.ldc] = o;  // the mirror above the diagonal
+    for (int j = 0; j < 4; j++) {
+      if (!live(i, j)) continue;
+      const int bi = wr + 4 * i, bj = wc + 2 * j;
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int gr = 16 * bi + (lane >> 4) + 4 * r, gc = 16 * bj + (lane & 15);
+        if (gr < J.m && gc < J.n) {
+          double* c = &J.C[gr + (long long)gc * J.ldc];
+          const double v = J.alpha * acc[i][j][r];
+          const double o = J.beta == 0.0 ? v : v + J.beta * *c;
+          *c = o;
+          if (low && bi != bj) J.C[gc + (long long)gr * J.ldc] = o;  // the mirror above the diagonal
+        }
       }
     }
-  }
 }
 
 static constexpr auto k_gemm_small_nn = k_gemm_small<0, 0>;
