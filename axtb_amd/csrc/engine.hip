@@ -334,6 +334,7 @@ struct Engine {
   hipStream_t st = nullptr;
   hipStream_t st2 = nullptr;  // side stream (default priority) for overlapped eigensolver launches
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipEvent_t ev_act[2] = {nullptr, nullptr};  // the active mask of an SCC iteration of either parity copied
   Prof prof;
   std::string last_error;
   // host mirrors of the batch tables
@@ -367,7 +368,7 @@ struct Engine {
     grow(ham, tri(hist_slots) * NM, tri(ns) * NM);
     hist_slots = ns;
   }
-  DBuf<int> active, status, kocc, itc, jstat, eeqfail;
+  DBuf<int> active, status, kocc, itc, eeqfail;
   DBuf<long long> eq0;
   DBuf<GemmJob> gj;
   DBuf<EigJob> ej;
@@ -382,6 +383,7 @@ struct Engine {
     HIP_OK(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));
     HIP_OK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
     HIP_OK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+    for (auto& e : ev_act) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     // the batched Jacobi kernel handles n <= 200
     if (const char* v = std::getenv("AXTB_HIP_DENSE_MIN")) dense_min = std::min(200, std::max(1, std::atoi(v)));
     if (const char* v = std::getenv("AXTB_HIP_OA_STEPS")) oa_steps = std::min(4, std::max(1, std::atoi(v)));
@@ -417,6 +419,9 @@ struct Engine {
     if (st) (void)hipStreamDestroy(st);
     if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (ev_join) (void)hipEventDestroy(ev_join);
+    for (auto& e : ev_act)
+      if (e) (void)hipEventDestroy(e);
+    if (pin_act) (void)hipHostFree(pin_act);
   }
 
   // dj: the same jobs already on the device (uploaded once by the caller for repeated launches);
@@ -610,16 +615,34 @@ struct Engine {
   // tridiagonal form plus 2 n^2 k for the k needed eigenvectors (k = n here). Matrices that fit
   // the LDS of a workgroup run from LDS, the others from global memory (separate launches).
   int max_lds = 0;
+  // A batched Jacobi solve as a plan: the jobs in launch order and the launches, uploaded once.
   // Launches: matrices that fit in LDS with their vectors (mode 2), with A only (mode 1), neither
   // (mode 0). Within a launch the largest matrices go first (shorter tail); the mode 1/0 launches
   // run on a second stream beside the mode 2 launch, both joined back into the engine stream.
   // fp32: the rotations in FP32 (A and V in LDS; Z and w returned in FP64 for the refinement).
   // tol_scale multiplies the rotation thresholds (the mixed path's FP32 solves, refined afterwards)
   // mol_of_job: the molecule of each job; a job that does not converge within max_sweeps then
-  // marks its molecule with status bit 16 (an eigensolver failure, Error::Numerical)
-  void eigh(std::vector<EigJob>& jobs, bool fp32 = false, int max_sweeps = -1, float tol_scale = 1.0f,
-            const std::vector<int>* mol_of_job = nullptr) {
-    if (jobs.empty()) return;
+  // marks its molecule with status bit 16 (an eigensolver failure, Error::Numerical). A plan over
+  // jobs with run flags is launched as it is in every SCC iteration (the flags pick the jobs).
+  struct EighPlan {
+    struct Launch {
+      int mode;
+      size_t b0, b1, shm;
+      unsigned nthr;
+    };
+    std::vector<Launch> launches;
+    size_t moff[3] = {0, 0, 0}, mcnt[3] = {0, 0, 0};
+    size_t njobs = 0;
+    bool fp32 = false, flag = false, side = false;
+    int max_sweeps = 60;
+    float tol_scale = 1.0f;
+    DBuf<EigJob> jobs;  // launch order: mode 2, then 1, then 0
+    DBuf<int> stat, mol;
+  };
+  EighPlan eadhoc;  // the plan of eigh()
+
+  void eigh_plan(EighPlan& P, const std::vector<EigJob>& jobs, bool fp32, int max_sweeps, float tol_scale,
+                 const std::vector<int>* mol_of_job) {
     if (max_sweeps < 0) max_sweeps = jacobi_sweeps;
     if (!max_lds) {
       HIP_OK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
@@ -637,81 +660,100 @@ struct Engine {
       if (fp32 && mode == 0) throw DeviceError("FP32 Jacobi: matrix does not fit in LDS");
       gidx[mode].push_back((int)q);
     }
-    std::vector<EigJob> groups[3];
-    std::vector<int> jm;  // the molecule of each job in launch order (the order of jstat)
+    std::vector<EigJob> order;
+    std::vector<int> jm;  // the molecule of each job in launch order (the order of stat)
     for (int g = 0; g < 3; g++)
       std::stable_sort(gidx[g].begin(), gidx[g].end(), [&](int a, int b) { return jobs[a].n > jobs[b].n; });
-    for (int g = 2; g >= 0; g--)
-      for (int q : gidx[g]) {
-        groups[g].push_back(jobs[q]);
+    P.launches.clear();
+    for (int mode = 2; mode >= 0; mode--) {
+      P.moff[mode] = order.size();
+      P.mcnt[mode] = gidx[mode].size();
+      for (int q : gidx[mode]) {
+        order.push_back(jobs[q]);
         if (mol_of_job) jm.push_back((*mol_of_job)[q]);
       }
-    jstat.ensure(jobs.size());
-    const bool side = !groups[2].empty() && (!groups[1].empty() || !groups[0].empty());
-    if (side) {
-      HIP_OK(hipEventRecord(ev_fork, st));
-      HIP_OK(hipStreamWaitEvent(st2, ev_fork, 0));
-    }
-    size_t off = 0;
-    for (int mode = 2; mode >= 0; mode--) {
-      auto& js = groups[mode];
-      if (js.empty()) continue;
-      hipStream_t s = (side && mode < 2) ? st2 : st;
-      DBuf<EigJob>& buf = ejb[mode];
-      buf.upload(js, s, &stage);
-      int* stat = jstat.p + off;
-      off += js.size();
-      const char* nm = fp32 ? (mode == 2 ? "k_jacobi2_f32" : "k_jacobi1_f32")
-                            : (mode == 2 ? "k_jacobi2" : (mode == 1 ? "k_jacobi1" : "k_jacobi0"));
       // the jobs (largest first) in launches of one LDS size class each: a launch reserves the LDS
       // of its largest matrix in every workgroup, so small matrices in a launch of their own fit 2-4
       // workgroups per CU
       auto per_cu = [&](int n) {
         return std::max<size_t>(1, std::min<size_t>(4, (size_t)max_lds / std::max<size_t>(jacobi_lds(n, mode, es), 1)));
       };
-      size_t b0 = 0;
-      while (b0 < js.size()) {
-        const size_t shm = jacobi_lds(js[b0].n, mode, es);
-        const size_t pc = mode == 0 ? 1 : per_cu(js[b0].n);
+      const size_t e0 = P.moff[mode], e1 = order.size();
+      size_t b0 = e0;
+      while (b0 < e1) {
+        const size_t shm = jacobi_lds(order[b0].n, mode, es);
+        const size_t pc = mode == 0 ? 1 : per_cu(order[b0].n);
         size_t b1 = b0 + 1;
-        while (b1 < js.size() && (mode == 0 || per_cu(js[b1].n) <= pc)) b1++;
-        double flops = 0.0;  // dense eigendecompositions with all vectors
-        for (size_t q = b0; q < b1; q++) flops += 4.0 / 3.0 * js[q].n * (double)js[q].n * js[q].n + 2.0 * js[q].n * (double)js[q].n * js[q].n;
-        ProfScope ps_(&prof, s, nm, flops);
-        const JacobiKernel kern = jacobi_kernel(mode, fp32);
-        HIP_OK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+        while (b1 < e1 && (mode == 0 || per_cu(order[b1].n) <= pc)) b1++;
         // 16 wavefronts per matrix while they fit (32 per CU: one workgroup per CU, or two of the FP32
         // kernels, which use at most 64 registers), else 8
         unsigned nthr = (mode != 0 && jacobi_wide && (pc == 1 || (pc == 2 && fp32))) ? 1024u : 512u;
-        const int tt = tune_jac_threads[fp32 ? 1 : 0][mode][tune_bucket(js[b0].n)];
-        if (tt && jacobi_threads_ok(js[b0].n, tt)) nthr = (unsigned)tt;
-        hipLaunchKernelGGL(kern, dim3((unsigned)(b1 - b0)), dim3(nthr), shm, s, buf.p + b0, max_sweeps, stat + b0,
-                           tol_scale);
+        const int tt = tune_jac_threads[fp32 ? 1 : 0][mode][tune_bucket(order[b0].n)];
+        if (tt && jacobi_threads_ok(order[b0].n, tt)) nthr = (unsigned)tt;
+        P.launches.push_back(EighPlan::Launch{mode, b0, b1, shm, nthr});
         b0 = b1;
+      }
+    }
+    P.njobs = order.size();
+    P.fp32 = fp32;
+    P.flag = mol_of_job != nullptr;
+    P.max_sweeps = max_sweeps;
+    P.tol_scale = tol_scale;
+    P.side = P.mcnt[2] && (P.mcnt[1] || P.mcnt[0]);
+    P.jobs.upload(order, st, &stage);
+    P.stat.ensure(order.size());
+    if (P.flag) P.mol.upload(jm, st, &stage);
+  }
+
+  void eigh_launch(EighPlan& P) {
+    if (!P.njobs) return;
+    if (P.side) {
+      HIP_OK(hipEventRecord(ev_fork, st));
+      HIP_OK(hipStreamWaitEvent(st2, ev_fork, 0));
+    }
+    for (int mode = 2; mode >= 0; mode--) {
+      if (!P.mcnt[mode]) continue;
+      hipStream_t s = (P.side && mode < 2) ? st2 : st;
+      const char* nm = P.fp32 ? (mode == 2 ? "k_jacobi2_f32" : "k_jacobi1_f32")
+                              : (mode == 2 ? "k_jacobi2" : (mode == 1 ? "k_jacobi1" : "k_jacobi0"));
+      const JacobiKernel kern = jacobi_kernel(mode, P.fp32);
+      for (const auto& L : P.launches) {
+        if (L.mode != mode) continue;
+        // the work of the solves that ran is summed on the device (k_sweep_acc) when profiling
+        ProfScope ps_(&prof, s, nm, 0.0);
+        HIP_OK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.shm));
+        hipLaunchKernelGGL(kern, dim3((unsigned)(L.b1 - L.b0)), dim3(L.nthr), L.shm, s, P.jobs.p + L.b0, P.max_sweeps,
+                           P.stat.p + L.b0, P.tol_scale);
       }
       HIP_OK(hipGetLastError());
       if (prof.on) {
-        // sweep statistics of the launch, summed on the device (read at the profile report)
+        // sweep statistics and the algorithmic work of the solves that ran, summed on the device
+        // (read at the profile report)
         if (!sweep_acc.p) {
-          sweep_acc.ensure(18);
-          HIP_OK(hipMemset(sweep_acc.p, 0, 18 * sizeof(double)));
+          sweep_acc.ensure(24);
+          HIP_OK(hipMemset(sweep_acc.p, 0, 24 * sizeof(double)));
         }
-        hipLaunchKernelGGL(k_sweep_acc, dim3(1), dim3(256), 0, s, (const int*)stat, (int)js.size(),
-                           sweep_acc.p + 3 * (mode * 2 + (fp32 ? 1 : 0)));
+        const int t = mode * 2 + (P.fp32 ? 1 : 0);
+        hipLaunchKernelGGL(k_sweep_acc, dim3(1), dim3(256), 0, s, (const int*)P.stat.p + P.moff[mode],
+                           (const EigJob*)P.jobs.p + P.moff[mode], (int)P.mcnt[mode], sweep_acc.p + 3 * t,
+                           sweep_acc.p + 18 + t);
       }
     }
-    if (side) {
+    if (P.side) {
       HIP_OK(hipEventRecord(ev_join, st2));
       HIP_OK(hipStreamWaitEvent(st, ev_join, 0));
     }
-    if (mol_of_job) {
-      jmol.upload(jm, st, &stage);
-      LAUNCH(k_jacobi_flag, dim3(blocks((long long)jm.size(), 256)), dim3(256), 0, (const int*)jstat.p,
-             (const int*)jmol.p, (int)jm.size(), status.p);
-    }
+    if (P.flag)
+      LAUNCH(k_jacobi_flag, dim3(blocks((long long)P.njobs, 256)), dim3(256), 0, (const int*)P.stat.p,
+             (const int*)P.mol.p, (int)P.njobs, status.p);
   }
-  DBuf<EigJob> ejb[3];
-  DBuf<int> jmol;
+
+  void eigh(std::vector<EigJob>& jobs, bool fp32 = false, int max_sweeps = -1, float tol_scale = 1.0f,
+            const std::vector<int>* mol_of_job = nullptr) {
+    if (jobs.empty()) return;
+    eigh_plan(eadhoc, jobs, fp32, max_sweeps, tol_scale, mol_of_job);
+    eigh_launch(eadhoc);
+  }
   // launch shapes per size bucket (n / 8 for n <= 207), set by the autotuner through
   // axtbhip_tuning_set; 0 keeps the built-in rule
   static constexpr int TUNE_NB = 26;
@@ -725,7 +767,7 @@ struct Engine {
     const int np = (n + (n & 1)) / 2;
     return (t == 256 || t == 512 || t == 1024) && (long long)t * 10 >= (long long)np * (np + 1) / 2 && t > np;
   }
-  DBuf<double> sweep_acc;  // per Jacobi mode and precision: sweeps, solves, unconverged solves
+  DBuf<double> sweep_acc;  // per Jacobi mode and precision: sweeps, solves, unconverged solves; then the work
   // the device-side sweep statistics into the profile's counters
   void drain_sweeps() {
     if (rd_cnt.p) {  // the device-driven refinements: [full, split] x [jobs, steps, failures]
@@ -739,12 +781,13 @@ struct Engine {
       HIP_OK(hipMemset(rd_cnt.p, 0, sizeof c));
     }
     if (!sweep_acc.p) return;
-    double h[18];
+    double h[24];
     HIP_OK(hipMemcpy(h, sweep_acc.p, sizeof h, hipMemcpyDeviceToHost));
     for (int t = 0; t < 6; t++) {
       const std::string tag = std::to_string(t / 2) + (t % 2 ? "_f32" : "");
       if (h[3 * t + 1] > 0) prof.count("sweeps_jacobi" + tag, (long)h[3 * t + 1], h[3 * t]);
       if (h[3 * t + 2] > 0) prof.count("unconverged_jacobi" + tag, (long)h[3 * t + 2], h[3 * t + 2]);
+      if (h[18 + t] > 0) prof.count("k_jacobi" + tag, 0, h[18 + t]);  // work only: launches come from the events
     }
     HIP_OK(hipMemset(sweep_acc.p, 0, sizeof h));
   }
@@ -1217,6 +1260,7 @@ struct Engine {
   std::vector<double> hxyz, hrcov, hen, hrad, hra, hrz, hdk, hqk, har, hav, hchi, heta, hkcn, herad, hreeq, hd4rcov,
       hd4zeff, hd4gi, hd4r4r2, hd4refcn, hd4refq, hsal, hsco, hse0, hskcn, hsshp, hsref, hshard, hsgam3, hsmin, hszeta;
   std::vector<int> map_use;  // batch molecule -> input index
+  std::vector<int> by_size;  // batch molecules by basis size, largest first
   std::vector<int> huhf;     // batch molecule -> the unpaired electrons it was prepared with
   // given starts of the batch molecules (flag per molecule; moments in the batch layout)
   std::vector<int> hgflag;
@@ -1416,6 +1460,12 @@ struct Engine {
       out[ii].nsh = nsh;
       out[ii].nao = nao;
     }
+    // the molecules by basis size, largest first: the order of the per-molecule job lists (one
+    // workgroup per job: the long workgroups start first and the short ones fill the end)
+    batch_gen++;
+    by_size.resize(mols.size());
+    for (size_t m = 0; m < mols.size(); m++) by_size[m] = (int)m;
+    std::stable_sort(by_size.begin(), by_size.end(), [&](int a, int b) { return mols[a].nao > mols[b].nao; });
     // reference C6 of every species pair (d4/src/model.rs:197-258)
     int nsp = (int)species.size();
     std::vector<double> hc6((size_t)nsp * nsp * 49, 0.0);
@@ -1816,7 +1866,6 @@ struct Engine {
           int n = mols[m].nao;
           gjobs.push_back(GemmJob{n, n, n, 0, 1, Z.p + o0, Z.p + o0, X.p + o0, s12_.p + mols[m].ao0, n, n, n, 1.0, 0.0});
         }
-        largest_first(gjobs);
         gemm(gjobs);
       }
     }
@@ -1972,7 +2021,6 @@ struct Engine {
     std::vector<GemmJob> G;
     for (int par = 0; par < 2; par++)
       for (int k = 0; k < 4; k++) {
-        largest_first(rd_L[par][k]);
         rd_goff[par][k] = G.size();
         G.insert(G.end(), rd_L[par][k].begin(), rd_L[par][k].end());
       }
@@ -2071,13 +2119,6 @@ struct Engine {
   // symmetric products: small ones (n <= 128) on the lower 16 x 16 blocks of k_gemm_small, which
   // writes the mirror itself; dense-solver molecules' on their lower 128-tiles, mirrored here
   static int sym_lower(int n, bool dense_mol) { return (dense_mol || n <= 128) ? 1 : 0; }
-  // a list of per-molecule products in launch order, largest first (one workgroup per product on
-  // the small-matrix kernel: the long workgroups start first and the short ones fill the end)
-  static void largest_first(std::vector<GemmJob>& v) {
-    std::stable_sort(v.begin(), v.end(), [](const GemmJob& a, const GemmJob& b) {
-      return (long long)a.m * a.n * (a.k + a.k2) > (long long)b.m * b.n * (b.k + b.k2);
-    });
-  }
   void mirror_dense(const std::vector<GemmJob>& jobs) {
     for (const GemmJob& j : jobs)
       if (j.lower && j.m > 128) {
@@ -2136,6 +2177,14 @@ struct Engine {
   // ------------------------------------------------------------------------------ SCC loop
   std::vector<int> hactive;
   std::vector<int> hsplit;          // batched molecule -> may refine for the split in the next iteration
+  EighPlan ep64, ep32;              // the fixed eigensolver lists of an SCF without dense molecules
+  std::vector<GemmJob> sl_[4];      // the fixed product lists (host copies of sjobs)
+  size_t sloff_[4] = {0, 0, 0, 0};
+  DBuf<int> dlist;                  // the batched molecules of the fixed lists
+  std::vector<long long> list_sig;  // what the fixed lists on the device were built for
+  long long batch_gen = 0;          // counts the batches built
+  int* pin_act = nullptr;           // pinned: the active mask after iterations of either parity
+  size_t pin_cap = 0;
   DBuf<int> splitok, rflags, rg_mol, dpfp;
   DBuf<GemmJob> sjobs;              // the per-iteration products of the batched molecules
   DBuf<EigJob> rg_jobs;
@@ -2209,23 +2258,39 @@ struct Engine {
     for (int m = 0; m < NM; m++)
       if (in_mask[m]) dense_pfp[m] = 0;
     HIP_OK(hipMemcpyAsync(Bm.p, X.p, sizeof(double) * tot_mat, hipMemcpyDeviceToDevice, st));
+    // The fixed job lists of this SCF (products, refinement steps, eigensolves) hold pointers into
+    // the engine's buffers and per-molecule flags only: an SCF of the same batch, mask and
+    // precision over the same buffers reuses the lists already on the device.
+    rflags.ensure(4 * (size_t)NM);
+    std::vector<long long> sig;
+    bool cached = false;
+    if (!t_arena) {
+      sig = {batch_gen, NM, (long long)o.precision};
+      for (const void* q : {(const void*)F.p, (const void*)T.p, (const void*)Am.p, (const void*)Bm.p, (const void*)C.p,
+                            (const void*)P.p, (const void*)Z.p, (const void*)Vw.p, (const void*)Wm.p, (const void*)w.p,
+                            (const void*)focc.p, (const void*)X.p, (const void*)active.p, (const void*)rflags.p})
+        sig.push_back((long long)(intptr_t)q);
+      sig.insert(sig.end(), in_mask.begin(), in_mask.end());
+      cached = sig == list_sig;
+    }
+    list_sig.clear();  // set again once the lists below are on the device
+    std::vector<GemmJob>(&sl)[4] = sl_;
+    size_t(&sloff)[4] = sloff_;
     // the products of every iteration for the batched molecules of this SCF (T = F B, A = B^T T,
     // C = B U, P = C f C^T), uploaded once; each runs while its molecule is active
-    std::vector<GemmJob> sl[4];
-    for (int m = 0; m < NM; m++) {
-      if (!in_mask[m] || isdense[m]) continue;
-      GemmJob j[4] = {g1[m], g2[m], g3v[m], gP[m]};
-      j[1].lower = j[3].lower = sym_lower(mols[m].nao, false);
-      for (int k = 0; k < 4; k++) {
-        j[k].run = active.p + m;
-        sl[k].push_back(j[k]);
+    if (!cached) {
+      for (auto& l : sl) l.clear();
+      for (int m : by_size) {
+        if (!in_mask[m] || isdense[m]) continue;
+        GemmJob j[4] = {g1[m], g2[m], g3v[m], gP[m]};
+        j[1].lower = j[3].lower = sym_lower(mols[m].nao, false);
+        for (int k = 0; k < 4; k++) {
+          j[k].run = active.p + m;
+          sl[k].push_back(j[k]);
+        }
       }
-    }
-    size_t sloff[4];
-    {
       std::vector<GemmJob> all;
       for (int k = 0; k < 4; k++) {
-        largest_first(sl[k]);
         sloff[k] = all.size();
         all.insert(all.end(), sl[k].begin(), sl[k].end());
       }
@@ -2234,33 +2299,90 @@ struct Engine {
     auto gemm_static = [&](int k) {
       if (!sl[k].empty()) gemm(sl[k], sjobs.p + sloff[k]);
     };
-    if (mixed) {  // the refinement's step lists of every batched molecule of this SCF, uploaded once
+    if (mixed && !cached) {  // the refinement's step lists of every batched molecule of this SCF, uploaded once
       std::vector<EigJob> ball;
       std::vector<int> ballm;
-      for (int m = 0; m < NM; m++)
+      for (int m : by_size)
         if (in_mask[m] && !isdense[m]) {
           ball.push_back(ejobs[m]);
           ballm.push_back(m);
         }
       refine_prepare(ball, ballm);
     }
+    // An SCF without dense-solver molecules runs every launch of an iteration over fixed job lists
+    // with device flags (the eigensolves too), so that the host enqueues an iteration while the
+    // previous one runs and reads the active mask one iteration late: a batch that has converged
+    // runs one more iteration of launches that all skip. With profiling on, the mask is read at the
+    // end of each iteration (the per-launch work counts then see the exact set).
+    bool anyd = false, anym = false;
+    for (int m = 0; m < NM; m++) {
+      anyd |= in_mask[m] && isdense[m];
+      anym |= in_mask[m] != 0;
+    }
+    const bool devloop = !anyd && !t_arena;
+    std::vector<EigJob> dall;
+    std::vector<int> dallm;
+    if (devloop) {
+      for (int m : by_size)
+        if (in_mask[m]) {
+          dall.push_back(ejobs[m]);
+          dallm.push_back(m);
+        }
+      const int ND = (int)dall.size();
+      splitok.ensure(NM);
+      dpfp.ensure(NM);
+      LAUNCH(k_zero_u32, dim3(blocks(4LL * NM, 256)), dim3(256), 0, (unsigned*)rflags.p, 4 * NM);
+      LAUNCH(k_zero_u32, dim3(blocks(NM, 256)), dim3(256), 0, (unsigned*)splitok.p, NM);
+      if (!cached) {
+        dlist.upload(dallm, st, &stage);
+        std::vector<EigJob> j64 = dall;
+        for (int q = 0; q < ND; q++) j64[q].run = (mixed ? rflags.p + 2 * (size_t)NM : active.p) + dallm[q];
+        eigh_plan(ep64, j64, false, -1, 1.0f, &dallm);
+        if (fp32) {
+          std::vector<EigJob> j32 = dall;
+          for (int q = 0; q < ND; q++) j32[q].run = rflags.p + NM + dallm[q];
+          eigh_plan(ep32, j32, true, 60, 1.0f, nullptr);
+        }
+        if (mixed) {
+          rg_jobs.upload(dall, st, &stage);
+          rg_mol.upload(dallm, st, &stage);
+        }
+      }
+      if (pin_cap < 2 * (size_t)NM) {
+        if (pin_act) HIP_OK(hipHostFree(pin_act));
+        pin_act = nullptr;
+        HIP_OK(hipHostMalloc((void**)&pin_act, 2 * sizeof(int) * NM, hipHostMallocDefault));
+        pin_cap = 2 * (size_t)NM;
+      }
+    }
+    list_sig = sig;
     int it = 0;
     for (it = 1; it <= o.max_iter; it++) {
       std::vector<GemmJob> a1, a2;  // the dense molecules' products (the batched ones: gemm_static)
       std::vector<EigJob> ea;
       std::vector<int> eam;  // molecule of each entry of ea
       bool any_active = false;
-      for (int m = 0; m < NM; m++)
-        if (hactive[m]) {
-          any_active = true;
-          if (isdense[m]) {
-            a1.push_back(g1[m]);
-            a2.push_back(g2[m]);
-          } else {
-            ea.push_back(ejobs[m]);
-            eam.push_back(m);
-          }
+      if (devloop) {
+        // the mask after iteration it - 2 (its slot is it's parity); iteration it - 1 is queued
+        if (!prof.on && it >= 3) {
+          HIP_OK(hipEventSynchronize(ev_act[it & 1]));
+          std::memcpy(hactive.data(), pin_act + (size_t)(it & 1) * NM, sizeof(int) * NM);
         }
+        for (int m = 0; m < NM; m++) any_active |= hactive[m] != 0;
+        if (it == 1) any_active = anym;
+      } else {
+        for (int m = 0; m < NM; m++)
+          if (hactive[m]) {
+            any_active = true;
+            if (isdense[m]) {
+              a1.push_back(g1[m]);
+              a2.push_back(g2[m]);
+            } else {
+              ea.push_back(ejobs[m]);
+              eam.push_back(m);
+            }
+          }
+      }
       if (!any_active) break;
       if (it > 1 && mixer == MIX_DIIS) {
         LAUNCH(k_diis, dim3(NM), dim3(256), 0, b, (const int*)active.p, qin.p, (const double*)dq.p, dhq.p, dhr.p,
@@ -2320,6 +2442,23 @@ struct Engine {
         // Jacobi. No host read in between: per-molecule flags (run, fail, in_t, split) set on the
         // device and launches over fixed job lists.
         const bool use_split = it >= split_from && split_refine;
+        if (devloop) {
+          const int ND = (int)dall.size();
+          const int* f_split = rflags.p;
+          const int* f_full = rflags.p + NM;
+          int* f_fail = rflags.p + 2 * (size_t)NM;
+          int* f_int = rflags.p + 3 * (size_t)NM;
+          LAUNCH(k_iter_flags, dim3(blocks(ND, 256)), dim3(256), 0, ND, (const int*)dlist.p, (const int*)active.p,
+                 (const int*)splitok.p, use_split ? 1 : 0, fp32 ? 1 : 0, NM, rflags.p);
+          if (use_split) refine_run(f_split, f_int, f_fail, ocls.p);
+          if (fp32) {
+            eigh_launch(ep32);
+            refine_run(f_full, f_int, f_fail, nullptr);
+          }
+          eigh_launch(ep64);
+          LAUNCH(k_refine_gather, dim3((unsigned)ND), dim3(256), 0, (const EigJob*)rg_jobs.p, (const int*)rg_mol.p,
+                 (long long)(T.p - Z.p), (const int*)f_int, (const int*)f_fail, f_split, dpfp.p, (const int*)active.p);
+        }
         std::vector<EigJob> ef;
         int nsplit = 0;
         std::vector<int> hflags(4 * (size_t)NM, 0);  // split | full (FP32 path) | fail | in_t
@@ -2334,7 +2473,7 @@ struct Engine {
           hflags[(size_t)NM + m] = (!sp && fp32) ? 1 : 0;
           hflags[2 * (size_t)NM + m] = (!sp && !fp32) ? 1 : 0;
         }
-        if (!ea.empty()) {
+        if (!devloop && !ea.empty()) {
           rflags.upload(hflags, st, &stage);
           const int* f_split = rflags.p;
           const int* f_full = rflags.p + NM;
@@ -2355,8 +2494,10 @@ struct Engine {
           rg_mol.upload(eam, st, &stage);
           LAUNCH(k_refine_gather, dim3((unsigned)ea.size()), dim3(256), 0, (const EigJob*)rg_jobs.p,
                  (const int*)rg_mol.p, (long long)(T.p - Z.p), (const int*)f_int, (const int*)f_fail, f_split,
-                 dpfp.p);
+                 dpfp.p, (const int*)nullptr);
         }
+      } else if (devloop) {
+        eigh_launch(ep64);
       } else {
         eigh(ea, false, -1, 1.0f, &eam);
       }
@@ -2441,6 +2582,28 @@ struct Engine {
       // iteration (occupation classes of this iteration's orbitals, this iteration's residual),
       // read with the active mask: the one host read of an iteration
       const bool next_split = mixed && split_refine && it + 1 >= split_from;
+      if (next_split && devloop) {
+        const int ND = (int)dall.size();
+        ocls.ensure(NO);
+        onfrac.ensure(NM);
+        LAUNCH(k_occ_class_batch, dim3((unsigned)ND), dim3(256), 0, b, (const int*)dlist.p, (const double*)focc.p,
+               ocls.p, onfrac.p);
+        LAUNCH(k_split_ok, dim3(blocks((long long)ND, 256)), dim3(256), 0, (const int*)dlist.p, ND,
+               (const int*)onfrac.p, (const double*)mst.p, split_residual, splitok.p);
+      }
+      if (devloop) {
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpyAsync(pin_act + (size_t)(it & 1) * NM, active.p, sizeof(int) * NM, hipMemcpyDeviceToHost, st));
+        HIP_OK(hipEventRecord(ev_act[it & 1], st));
+        if (prof.on) {
+          HIP_OK(hipEventSynchronize(ev_act[it & 1]));
+          std::memcpy(hactive.data(), pin_act + (size_t)(it & 1) * NM, sizeof(int) * NM);
+          bool any = false;
+          for (int m = 0; m < NM; m++) any |= hactive[m] != 0;
+          if (!any) break;
+        }
+        continue;
+      }
       if (next_split) {
         std::vector<int> ml;
         for (int m = 0; m < NM; m++)
@@ -2560,7 +2723,7 @@ struct Engine {
     std::vector<int> hk(NM);
     HIP_OK(hipMemcpyAsync(hk.data(), kocc.p, sizeof(int) * NM, hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
-    for (int m = 0; m < NM; m++) {
+    for (int m : by_size) {  // the product lists largest first
       hok[m] = (hm[8 * m + 6] == 1.0 && hst[m] == 0) ? 1 : 0;
       if (!hok[m]) continue;
       const Mol& M = mols[m];
@@ -2584,8 +2747,6 @@ struct Engine {
       pfd.upload(hpfp, st, &stage);
       LAUNCH(k_fock, dim3(n_mat_tiles), dim3(256), 0, b, (const int*)pfd.p, (const int2*)mat_tiles.p, pool.p,
              stride, vsh.p, vat.p, vdp.p, vqp.p, F.p);
-      largest_first(gF);
-      largest_first(gW2);
       gemm(gF);
       for (auto& j : gW2) j.lower = 1;  // W = P (F P) / 2 is symmetric
       gemm(gW2);
@@ -2598,7 +2759,6 @@ struct Engine {
     LAUNCH(k_pairwise, dim3(std::min<long long>(blocks(naos(), 256), 4096)), dim3(256), 0, (const double*)focc.p,
            (const double*)w.p, (long long)naos(), 1, fwork.p);
     for (auto& j : gW) j.lower = sym_lower(j.n, dense(j.n));
-    largest_first(gW);
     gemm(gW);
     mirror_dense(gW);
     // gradient = repulsion + three-body (setup) + pair terms + CN chains
@@ -3338,7 +3498,7 @@ int axtbhip_test_eigh(int n, int nb, const double* A, double* w, double* Z, int*
     e.eigh(jobs);
     HIP_OK(hipMemcpyAsync(w, ww.p, (size_t)n * nb * sizeof(double), hipMemcpyDeviceToHost, e.st));
     HIP_OK(hipMemcpyAsync(Z, z.p, nn * nb * sizeof(double), hipMemcpyDeviceToHost, e.st));
-    if (sweeps) HIP_OK(hipMemcpyAsync(sweeps, e.jstat.p, nb * sizeof(int), hipMemcpyDeviceToHost, e.st));
+    if (sweeps) HIP_OK(hipMemcpyAsync(sweeps, e.eadhoc.stat.p, nb * sizeof(int), hipMemcpyDeviceToHost, e.st));
     HIP_OK(hipStreamSynchronize(e.st));
     return 0;
   } catch (std::exception&) {

@@ -1566,27 +1566,6 @@ __global__ void k_attempt_start(const int* mask, int nmol, int* active, int* itc
   }
 }
 
-// Sweep statistics of a Jacobi launch: acc[0] += sweeps, acc[1] += solves, acc[2] += unconverged.
-__global__ void k_sweep_acc(const int* stat, int n, double* acc) {
-  __shared__ double red[16];
-  double a = 0.0, c = 0.0;
-  double nj = 0.0;
-  for (int q = threadIdx.x; q < n; q += blockDim.x) {
-    if (stat[q] == JACOBI_SKIPPED) continue;
-    a += abs(stat[q]);
-    c += stat[q] < 0;
-    nj += 1.0;
-  }
-  a = block_sum(a, red);
-  c = block_sum(c, red);
-  nj = block_sum(nj, red);
-  if (threadIdx.x == 0) {
-    acc[0] += a;
-    acc[1] += nj;
-    acc[2] += c;
-  }
-}
-
 // A Jacobi solve that did not converge within its sweeps (stat < 0) marks its molecule with
 // status bit 16 (Error::Numerical of the eigensolver).
 __global__ void k_jacobi_flag(const int* stat, const int* mol_of_job, int njobs, int* status) {

@@ -989,6 +989,32 @@ __global__ void k_refine_round(int nj, const int* mol, int* run, const double* e
 }
 
 // run[m] = src[m] for the molecules of the job list (the next refinement's participants)
+// Statistics of a Jacobi launch over the solves that ran: acc[0] += sweeps, acc[1] += solves,
+// acc[2] += unconverged, *work += their algorithmic flops (4/3 n^3 for the reduction and 2 n^3 for
+// the vectors of each n x n matrix).
+__global__ void k_sweep_acc(const int* stat, const EigJob* jobs, int n, double* acc, double* work) {
+  __shared__ double red[16];
+  double a = 0.0, c = 0.0, nj = 0.0, wk = 0.0;
+  for (int q = threadIdx.x; q < n; q += blockDim.x) {
+    if (stat[q] == JACOBI_SKIPPED) continue;
+    a += abs(stat[q]);
+    c += stat[q] < 0;
+    nj += 1.0;
+    const double m = jobs[q].n;
+    wk += (4.0 / 3.0 + 2.0) * m * m * m;
+  }
+  a = block_sum(a, red);
+  c = block_sum(c, red);
+  nj = block_sum(nj, red);
+  wk = block_sum(wk, red);
+  if (threadIdx.x == 0) {
+    acc[0] += a;
+    acc[1] += nj;
+    acc[2] += c;
+    *work += wk;
+  }
+}
+
 __global__ void k_set_run(int nj, const int* mol, const int* src, int* run) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q < nj) run[mol[q]] = src[mol[q]];
@@ -997,15 +1023,33 @@ __global__ void k_set_run(int nj, const int* mol, const int* src, int* run) {
 // The refined vectors of the jobs whose result is in T (in_t and not failed) copied to Z, so that
 // the products after the eigenproblem read Z for every molecule; pfp[m] (nullable) records whether
 // molecule m ended its iteration with a converged split refinement (split[m] and not failed).
+// With `active` (nullable) the molecules that are not active are left alone.
 __global__ void k_refine_gather(const EigJob* jobs, const int* mol, long long t_minus_z, const int* in_t,
-                                const int* fail, const int* split, int* pfp) {
+                                const int* fail, const int* split, int* pfp, const int* active) {
   const EigJob J = jobs[blockIdx.x];
   const int m = mol[blockIdx.x];
+  if (active && !active[m]) return;
   if (pfp && threadIdx.x == 0) pfp[m] = (split[m] && !fail[m]) ? 1 : 0;
   if (!in_t[m] || fail[m]) return;
   const long long nn = (long long)J.n * J.n;
   const double* src = J.Z + t_minus_z;
   for (long long e = threadIdx.x; e < nn; e += blockDim.x) J.Z[e] = src[e];
+}
+
+// The refinement flags of an SCC iteration (flags = [split | full | fail | in_t], nm each) for the
+// molecules mol[0..nj): split = active and allowed to refine for the occupied / empty split
+// (ok[m], when use_split); full = active, not split, FP32 path; fail = active, not split, FP64 path
+// (the FP64 Jacobi); in_t = 0.
+__global__ void k_iter_flags(int nj, const int* mol, const int* active, const int* ok, int use_split, int fp32, int nm,
+                             int* flags) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nj) return;
+  const int m = mol[q];
+  const int a = active[m] != 0, sp = a && use_split && ok[m] != 0;
+  flags[m] = sp;
+  flags[nm + m] = (a && !sp && fp32) ? 1 : 0;
+  flags[2 * nm + m] = (a && !sp && !fp32) ? 1 : 0;
+  flags[3 * nm + m] = 0;
 }
 
 // Whether molecule m may refine its previous orbitals for the occupied / empty split in the next
