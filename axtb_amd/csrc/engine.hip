@@ -427,23 +427,59 @@ struct Engine {
   // dj: the same jobs already on the device (uploaded once by the caller for repeated launches);
   // they must all be non-empty and of one kind (FP32 updates or not), and are never sent to the
   // library or split over K (those paths cannot honour run flags)
+  // what a launch needs to know of a job list, in one pass over it (kept for the fixed device lists,
+  // which are launched in every SCC iteration)
+  struct GemmMeta {
+    int mm = 0, nn = 0;
+    bool same = true, two = false, any32 = false, all32 = true, anyrun = false, uniform = true;
+    double flops = 0.0;  // a lower-triangle (symmetric) output counts its m (m + 1) / 2 elements
+  };
+  static GemmMeta gemm_meta(const std::vector<GemmJob>& jobs) {
+    GemmMeta M;
+    for (const GemmJob& j : jobs) {
+      M.mm = std::max(M.mm, j.m);
+      M.nn = std::max(M.nn, j.n);
+      M.same &= j.ta == jobs[0].ta && j.tb == jobs[0].tb;
+      M.two |= j.k2 > 0;
+      M.any32 |= j.f32upd != 0;
+      M.all32 &= j.f32upd != 0;
+      M.anyrun |= j.run != nullptr;
+      M.uniform &= j.m > 0 && j.n > 0 && j.f32upd == jobs[0].f32upd;
+      M.flops += (j.lower && j.m == j.n ? (double)j.m * (j.m + 1) : 2.0 * j.m * j.n) * ((double)j.k + j.k2);
+    }
+    return M;
+  }
+  std::vector<std::pair<std::pair<const GemmJob*, size_t>, GemmMeta>> gmeta;  // of the fixed device lists
+
   void gemm(std::vector<GemmJob>& jobs, const GemmJob* dj = nullptr) {
+    GemmMeta M;
     if (dj) {
-      for (const GemmJob& j : jobs)
-        if (j.m <= 0 || j.n <= 0 || j.f32upd != jobs[0].f32upd) throw DeviceError("gemm: device job list not uniform");
+      bool found = false;
+      for (const auto& e : gmeta)
+        if (e.first.first == dj && e.first.second == jobs.size()) {
+          M = e.second;
+          found = true;
+          break;
+        }
+      if (!found) {
+        M = gemm_meta(jobs);
+        if (!M.uniform) throw DeviceError("gemm: device job list not uniform");
+        gmeta.push_back({{dj, jobs.size()}, M});
+      }
     } else {
       jobs.erase(std::remove_if(jobs.begin(), jobs.end(), [](const GemmJob& j) { return j.m <= 0 || j.n <= 0; }),
                  jobs.end());
+      M = gemm_meta(jobs);
     }
     if (jobs.empty()) return;
-    const bool flagged = dj || std::any_of(jobs.begin(), jobs.end(), [](const GemmJob& j) { return j.run != nullptr; });
+    const bool flagged = dj || M.anyrun;
     auto jobs_dev = [&]() -> const GemmJob* {
       if (dj) return dj;
       gj.upload(jobs, st, &stage);
       return gj.p;
     };
-    const bool any32 = std::any_of(jobs.begin(), jobs.end(), [](const GemmJob& j) { return j.f32upd != 0; });
-    if (any32 && !std::all_of(jobs.begin(), jobs.end(), [](const GemmJob& j) { return j.f32upd != 0; })) {
+    const bool any32 = M.any32;
+    if (any32 && !M.all32) {
       std::vector<GemmJob> j32, j64;
       for (auto& j : jobs) (j.f32upd ? j32 : j64).push_back(j);
       gemm(j32);
@@ -514,18 +550,9 @@ struct Engine {
       HIP_OK(hipGetLastError());
       return;
     }
-    int mm = 0, nn = 0;
-    bool same = true, two = false;
-    for (auto& j : jobs) {
-      mm = std::max(mm, j.m);
-      nn = std::max(nn, j.n);
-      same &= j.ta == jobs[0].ta && j.tb == jobs[0].tb;
-      two |= j.k2 > 0;
-    }
-    double flops = 0.0;
-    // the products' own flops: a lower-triangle (symmetric) output counts its m (m + 1) / 2 elements
-    for (auto& j : jobs)
-      flops += (j.lower && j.m == j.n ? (double)j.m * (j.m + 1) : 2.0 * j.m * j.n) * ((double)j.k + j.k2);
+    const int mm = M.mm, nn = M.nn;
+    const bool same = M.same, two = M.two;
+    const double flops = M.flops;
     // one product with few output tiles and a long inner dimension: split K over workgroups,
     // partials reduced in a fixed order
     if (jobs.size() == 1 && !two && !flagged) {
@@ -2028,6 +2055,7 @@ struct Engine {
     E.insert(E.end(), O[0].begin(), O[0].end());
     E.insert(E.end(), O[1].begin(), O[1].end());
     E.insert(E.end(), Oid.begin(), Oid.end());
+    gmeta.clear();
     rd_g.upload(G, st, &stage);
     rd_e.upload(E, st, &stage);
     rd_mol.upload(mol, st, &stage);
@@ -2294,6 +2322,7 @@ struct Engine {
         sloff[k] = all.size();
         all.insert(all.end(), sl[k].begin(), sl[k].end());
       }
+      gmeta.clear();
       sjobs.upload(all, st, &stage);
     }
     auto gemm_static = [&](int k) {
