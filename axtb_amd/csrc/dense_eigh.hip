@@ -37,6 +37,24 @@ __device__ __forceinline__ void trd_reflector(double ax, double tot, double& tau
   scal = 1.0 / (ax - beta);
 }
 
+// s = sum of p[k * stride] over k = k0, k0 + step, ... < n, in that order, with the loads issued 8 at
+// a time (a plain loop waits for each load before it issues the next).
+__device__ __forceinline__ double strided_sum(const double* p, int k0, int n, int step, long long stride) {
+  double s = 0.0;
+  for (int k = k0; k < n; k += 8 * step) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int kk = k + u * step;
+      v[u] = kk < n ? p[(long long)kk * stride] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+      if (k + u * step < n) s += v[u];
+  }
+  return s;
+}
+
 // Sum of the column kernel's squares partials of column i (fixed order; the same in every lane
 // and every wavefront).
 __device__ __forceinline__ double trd_sumsq(const double* sq, int n, int i, int lane) {
@@ -71,7 +89,7 @@ __global__ __launch_bounds__(256) void k_trd_col(double* A, int n, int i0, int j
   constexpr int KR = (TRD_NB + 3) / 4;
   double ps = 0.0, vr[KR], wr_[KR], a_ip = 0.0, a_i = 0.0;
   if (r < n) {
-    for (int ch = wv; ch < nch; ch += 4) ps += part[ch * ln + r];
+    ps = strided_sum(part + r, wv, nch, 4, ln);
 #pragma unroll
     for (int k = 0; k < KR; k++) {
       const int c = wv + 4 * k;
@@ -91,8 +109,7 @@ __global__ __launch_bounds__(256) void k_trd_col(double* A, int n, int i0, int j
       vic = A[(i0 + threadIdx.x) * ln + i];
       wic = W[threadIdx.x * ln + i];
     }
-    if (wv == 0)
-      for (int ch = lane; ch < nch; ch += 64) pin += part[ch * ln + i];
+    if (wv == 0) pin = strided_sum(part + i, lane, nch, 64, ln);
   }
   double t = 0.0, alpha = 0.0, scal = 1.0;
   if (j > 0) {
@@ -106,15 +123,13 @@ __global__ __launch_bounds__(256) void k_trd_col(double* A, int n, int i0, int j
       d[ip] = A[ip * ln + ip];
     }
     for (int c = threadIdx.x; c < 2 * jj; c += 256) {
-      double s = 0.0;
-      for (int q = 0; q < ndb; q++) s += dpart[q * 2 * TRD_NB + c];
+      const double s = strided_sum(dpart + c, 0, ndb, 1, 2 * TRD_NB);
       if (c < jj)
         y1[c] = s;
       else
         y2[c - jj] = s;
     }
-    double va = 0.0;
-    for (int k = threadIdx.x; k < ntile; k += 256) va += vpart[k];
+    double va = strided_sum(vpart, threadIdx.x, ntile, 256, 1);
     va = block_sum(va, rs);  // its barriers also publish y1, y2
     const double yy = wave_sum(lane < jj ? y1[lane] * y2[lane] : 0.0);
     alpha = -0.5 * t * t * (va - 2.0 * yy);
@@ -256,11 +271,20 @@ __global__ __launch_bounds__(256) void k_trd_symv(const double* A, int n, int i0
     const int c = 4 * (d / ndb) + wv;
     if (c < 2 * j) {
       const double* y = c < j ? W + c * ln : A + (i0 + c - j) * ln;
+      // all 2 x 16 loads of the lane first, then the products in row order
+      constexpr int NQ = TRD_DR / 64;
+      double yv[NQ], xv[NQ];
+#pragma unroll
+      for (int u = 0; u < NQ; u++) {
+        const int rl = rb * TRD_DR + lane + 64 * u;
+        yv[u] = rl < m ? y[i + 1 + rl] : 0.0;
+        xv[u] = rl < m ? x[rl] : 0.0;
+      }
       double s = 0.0;
 #pragma unroll
-      for (int q = lane; q < TRD_DR; q += 64) {
-        const int rl = rb * TRD_DR + q;
-        if (rl < m) s += y[i + 1 + rl] * vec(rl);
+      for (int u = 0; u < NQ; u++) {
+        const int rl = rb * TRD_DR + lane + 64 * u;
+        if (rl < m) s += yv[u] * (rl == 0 ? 1.0 : xv[u] * scal);
       }
       s = wave_sum(s);
       if (lane == 0) dpart[rb * 2 * TRD_NB + c] = s;
