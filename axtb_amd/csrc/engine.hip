@@ -333,7 +333,9 @@ struct Engine {
   int device = 0;
   hipStream_t st = nullptr;
   hipStream_t st2 = nullptr;  // side stream (default priority) for overlapped eigensolver launches
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipStream_t st3 = nullptr;  // a second one (default priority) for the batched Jacobi's size classes
+  bool jacobi_streams = true;  // AXTB_HIP_JACOBI_STREAMS
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_join3 = nullptr;
   hipEvent_t ev_act[2] = {nullptr, nullptr};  // the active mask of an SCC iteration of either parity copied
   Prof prof;
   std::string last_error;
@@ -381,8 +383,10 @@ struct Engine {
     HIP_OK(hipSetDevice(dev));
     HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     HIP_OK(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));
+    HIP_OK(hipStreamCreateWithFlags(&st3, hipStreamNonBlocking));
     HIP_OK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
     HIP_OK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+    HIP_OK(hipEventCreateWithFlags(&ev_join3, hipEventDisableTiming));
     for (auto& e : ev_act) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     // the batched Jacobi kernel handles n <= 200
     if (const char* v = std::getenv("AXTB_HIP_DENSE_MIN")) dense_min = std::min(200, std::max(1, std::atoi(v)));
@@ -390,6 +394,7 @@ struct Engine {
     if (const char* v = std::getenv("AXTB_HIP_DENSE_FULL")) dense_full = std::max(1, std::atoi(v));
     if (const char* v = std::getenv("AXTB_HIP_SPLIT_REFINE")) split_refine = std::atoi(v) != 0;
     if (const char* v = std::getenv("AXTB_HIP_JACOBI_WIDE")) jacobi_wide = std::atoi(v) != 0;
+    if (const char* v = std::getenv("AXTB_HIP_JACOBI_STREAMS")) jacobi_streams = std::atoi(v) != 0;
     if (const char* v = std::getenv("AXTB_HIP_SPLIT_RESIDUAL")) split_residual = std::atof(v);
     if (const char* v = std::getenv("AXTB_HIP_SPLIT_STEPS")) split_steps = std::max(3, std::atoi(v));
     if (const char* v = std::getenv("AXTB_HIP_SPLIT_FROM")) split_from = std::max(2, std::atoi(v));
@@ -416,9 +421,11 @@ struct Engine {
   ~Engine() {
     if (blas) (void)rocblas_destroy_handle(blas);
     if (st2) (void)hipStreamDestroy(st2);
+    if (st3) (void)hipStreamDestroy(st3);
     if (st) (void)hipStreamDestroy(st);
     if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (ev_join) (void)hipEventDestroy(ev_join);
+    if (ev_join3) (void)hipEventDestroy(ev_join3);
     for (auto& e : ev_act)
       if (e) (void)hipEventDestroy(e);
     if (pin_act) (void)hipHostFree(pin_act);
@@ -732,11 +739,18 @@ struct Engine {
     if (P.flag) P.mol.upload(jm, st, &stage);
   }
 
+  // Streams: the mode 1/0 launches on st2 beside the mode 2 ones; with jacobi_streams the mode 2
+  // size classes after the first (largest) run on st3, so that they fill the CUs the first launch
+  // leaves idle in its last round of workgroups.
   void eigh_launch(EighPlan& P) {
     if (!P.njobs) return;
-    if (P.side) {
+    int n2 = 0;
+    for (const auto& L : P.launches) n2 += L.mode == 2;
+    const bool split2 = jacobi_streams && n2 >= 2;
+    if (P.side || split2) {
       HIP_OK(hipEventRecord(ev_fork, st));
-      HIP_OK(hipStreamWaitEvent(st2, ev_fork, 0));
+      if (P.side) HIP_OK(hipStreamWaitEvent(st2, ev_fork, 0));
+      if (split2) HIP_OK(hipStreamWaitEvent(st3, ev_fork, 0));
     }
     for (int mode = 2; mode >= 0; mode--) {
       if (!P.mcnt[mode]) continue;
@@ -744,13 +758,19 @@ struct Engine {
       const char* nm = P.fp32 ? (mode == 2 ? "k_jacobi2_f32" : "k_jacobi1_f32")
                               : (mode == 2 ? "k_jacobi2" : (mode == 1 ? "k_jacobi1" : "k_jacobi0"));
       const JacobiKernel kern = jacobi_kernel(mode, P.fp32);
+      int nl = 0;
       for (const auto& L : P.launches) {
         if (L.mode != mode) continue;
+        hipStream_t sl = (mode == 2 && split2 && nl++ > 0) ? st3 : s;
         // the work of the solves that ran is summed on the device (k_sweep_acc) when profiling
-        ProfScope ps_(&prof, s, nm, 0.0);
+        ProfScope ps_(&prof, sl, nm, 0.0);
         HIP_OK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.shm));
-        hipLaunchKernelGGL(kern, dim3((unsigned)(L.b1 - L.b0)), dim3(L.nthr), L.shm, s, P.jobs.p + L.b0, P.max_sweeps,
+        hipLaunchKernelGGL(kern, dim3((unsigned)(L.b1 - L.b0)), dim3(L.nthr), L.shm, sl, P.jobs.p + L.b0, P.max_sweeps,
                            P.stat.p + L.b0, P.tol_scale);
+      }
+      if (mode == 2 && split2) {
+        HIP_OK(hipEventRecord(ev_join3, st3));
+        HIP_OK(hipStreamWaitEvent(st, ev_join3, 0));
       }
       HIP_OK(hipGetLastError());
       if (prof.on) {
