@@ -334,8 +334,9 @@ struct Engine {
   hipStream_t st = nullptr;
   hipStream_t st2 = nullptr;  // side stream (default priority) for overlapped eigensolver launches
   hipStream_t st3 = nullptr;  // a second one (default priority) for the batched Jacobi's size classes
+  hipStream_t st4 = nullptr;  // a third one (default priority): a batched Jacobi beside other work
   bool jacobi_streams = true;  // AXTB_HIP_JACOBI_STREAMS
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_join3 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_join3 = nullptr, ev_fork4 = nullptr, ev_join4 = nullptr;
   hipEvent_t ev_act[2] = {nullptr, nullptr};  // the active mask of an SCC iteration of either parity copied
   Prof prof;
   std::string last_error;
@@ -384,9 +385,12 @@ struct Engine {
     HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     HIP_OK(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));
     HIP_OK(hipStreamCreateWithFlags(&st3, hipStreamNonBlocking));
+    HIP_OK(hipStreamCreateWithFlags(&st4, hipStreamNonBlocking));
     HIP_OK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
     HIP_OK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
     HIP_OK(hipEventCreateWithFlags(&ev_join3, hipEventDisableTiming));
+    HIP_OK(hipEventCreateWithFlags(&ev_fork4, hipEventDisableTiming));
+    HIP_OK(hipEventCreateWithFlags(&ev_join4, hipEventDisableTiming));
     for (auto& e : ev_act) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     // the batched Jacobi kernel handles n <= 200
     if (const char* v = std::getenv("AXTB_HIP_DENSE_MIN")) dense_min = std::min(200, std::max(1, std::atoi(v)));
@@ -422,10 +426,13 @@ struct Engine {
     if (blas) (void)rocblas_destroy_handle(blas);
     if (st2) (void)hipStreamDestroy(st2);
     if (st3) (void)hipStreamDestroy(st3);
+    if (st4) (void)hipStreamDestroy(st4);
     if (st) (void)hipStreamDestroy(st);
     if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (ev_join) (void)hipEventDestroy(ev_join);
     if (ev_join3) (void)hipEventDestroy(ev_join3);
+    if (ev_fork4) (void)hipEventDestroy(ev_fork4);
+    if (ev_join4) (void)hipEventDestroy(ev_join4);
     for (auto& e : ev_act)
       if (e) (void)hipEventDestroy(e);
     if (pin_act) (void)hipHostFree(pin_act);
@@ -742,19 +749,21 @@ struct Engine {
   // Streams: the mode 1/0 launches on st2 beside the mode 2 ones; with jacobi_streams the mode 2
   // size classes after the first (largest) run on st3, so that they fill the CUs the first launch
   // leaves idle in its last round of workgroups.
-  void eigh_launch(EighPlan& P) {
+  // base: the stream the solve is ordered on (the engine stream when null).
+  void eigh_launch(EighPlan& P, hipStream_t base = nullptr) {
     if (!P.njobs) return;
+    if (!base) base = st;
     int n2 = 0;
     for (const auto& L : P.launches) n2 += L.mode == 2;
     const bool split2 = jacobi_streams && n2 >= 2;
     if (P.side || split2) {
-      HIP_OK(hipEventRecord(ev_fork, st));
+      HIP_OK(hipEventRecord(ev_fork, base));
       if (P.side) HIP_OK(hipStreamWaitEvent(st2, ev_fork, 0));
       if (split2) HIP_OK(hipStreamWaitEvent(st3, ev_fork, 0));
     }
     for (int mode = 2; mode >= 0; mode--) {
       if (!P.mcnt[mode]) continue;
-      hipStream_t s = (P.side && mode < 2) ? st2 : st;
+      hipStream_t s = (P.side && mode < 2) ? st2 : base;
       const char* nm = P.fp32 ? (mode == 2 ? "k_jacobi2_f32" : "k_jacobi1_f32")
                               : (mode == 2 ? "k_jacobi2" : (mode == 1 ? "k_jacobi1" : "k_jacobi0"));
       const JacobiKernel kern = jacobi_kernel(mode, P.fp32);
@@ -770,7 +779,7 @@ struct Engine {
       }
       if (mode == 2 && split2) {
         HIP_OK(hipEventRecord(ev_join3, st3));
-        HIP_OK(hipStreamWaitEvent(st, ev_join3, 0));
+        HIP_OK(hipStreamWaitEvent(base, ev_join3, 0));
       }
       HIP_OK(hipGetLastError());
       if (prof.on) {
@@ -788,11 +797,11 @@ struct Engine {
     }
     if (P.side) {
       HIP_OK(hipEventRecord(ev_join, st2));
-      HIP_OK(hipStreamWaitEvent(st, ev_join, 0));
+      HIP_OK(hipStreamWaitEvent(base, ev_join, 0));
     }
     if (P.flag)
-      LAUNCH(k_jacobi_flag, dim3(blocks((long long)P.njobs, 256)), dim3(256), 0, (const int*)P.stat.p,
-             (const int*)P.mol.p, (int)P.njobs, status.p);
+      LAUNCHS(base, 0.0, k_jacobi_flag, dim3(blocks((long long)P.njobs, 256)), dim3(256), 0, (const int*)P.stat.p,
+              (const int*)P.mol.p, (int)P.njobs, status.p);
   }
 
   void eigh(std::vector<EigJob>& jobs, bool fp32 = false, int max_sweeps = -1, float tol_scale = 1.0f,
@@ -2499,9 +2508,21 @@ struct Engine {
           int* f_int = rflags.p + 3 * (size_t)NM;
           LAUNCH(k_iter_flags, dim3(blocks(ND, 256)), dim3(256), 0, ND, (const int*)dlist.p, (const int*)active.p,
                  (const int*)splitok.p, use_split ? 1 : 0, fp32 ? 1 : 0, NM, rflags.p);
+          // the FP32 Jacobi of the molecules on the full path on st4 beside the split refinement of
+          // the others (their buffers are disjoint), joined before the full refinement
+          const bool beside = fp32 && use_split && jacobi_streams;
+          if (beside) {
+            HIP_OK(hipEventRecord(ev_fork4, st));
+            HIP_OK(hipStreamWaitEvent(st4, ev_fork4, 0));
+            eigh_launch(ep32, st4);
+            HIP_OK(hipEventRecord(ev_join4, st4));
+          }
           if (use_split) refine_run(f_split, f_int, f_fail, ocls.p);
           if (fp32) {
-            eigh_launch(ep32);
+            if (beside)
+              HIP_OK(hipStreamWaitEvent(st, ev_join4, 0));
+            else
+              eigh_launch(ep32);
             refine_run(f_full, f_int, f_fail, nullptr);
           }
           eigh_launch(ep64);
