@@ -268,13 +268,15 @@ class Context:
         lib().axtbhip_profile_enable(self._h, int(on))
 
     def profile_report(self):
-        """{kernel: (launches, total_ms, work)} of the launches since the last report."""
+        """{kernel: (launches, total_ms, work)} of the launches since the last report; the ms
+        covered by the union of each kernel's launch intervals go to `self.last_spans`."""
         buf = ctypes.create_string_buffer(1 << 16)
         lib().axtbhip_profile_report(self._h, buf, 1 << 16)
-        out = {}
+        out, self.last_spans = {}, {}
         for line in buf.value.decode().splitlines():
-            name, cnt, ms, work = line.rsplit(" ", 3)
+            name, cnt, ms, work, span = line.rsplit(" ", 4)
             out[name] = (int(cnt), float(ms), float(work))
+            self.last_spans[name] = float(span)
         return out
 
     def synchronize(self):

@@ -236,6 +236,7 @@ struct Prof {
   struct Sum {
     long n = 0;
     double ms = 0, work = 0;
+    double span = 0;  // ms covered by the union of the launches' intervals (launches on several streams overlap)
   };
   // host-side counters reported like kernels (launches = items, work = total), 0 ms
   std::map<std::string, Sum> counters;
@@ -248,14 +249,34 @@ struct Prof {
   std::map<std::string, Sum> drain() {
     std::map<std::string, Sum> m = counters;
     counters.clear();
+    std::map<std::string, std::vector<std::pair<double, double>>> iv;
     for (auto& r : recs) {
       HIP_OK(hipEventSynchronize(r.b));
-      float ms = 0.f;
+      float ms = 0.f, t0 = 0.f;
       HIP_OK(hipEventElapsedTime(&ms, r.a, r.b));
+      HIP_OK(hipEventElapsedTime(&t0, recs[0].a, r.a));  // start relative to the first launch
       auto& x = m[r.name];
       x.n++;
       x.ms += ms;
       x.work += r.work;
+      iv[r.name].push_back({t0, t0 + ms});
+    }
+    for (auto& kv : iv) {
+      auto& v = kv.second;
+      std::sort(v.begin(), v.end());
+      double span = 0.0, a = v[0].first, b = v[0].second;
+      for (size_t k = 1; k < v.size(); k++) {
+        if (v[k].first > b) {
+          span += b - a;
+          a = v[k].first;
+          b = v[k].second;
+        } else {
+          b = std::max(b, v[k].second);
+        }
+      }
+      m[kv.first].span = span + (b - a);
+    }
+    for (auto& r : recs) {
       pool.push_back(r.a);
       pool.push_back(r.b);
     }
@@ -3495,7 +3516,8 @@ int axtbhip_profile_report(axtbhip_context* ctx, char* buf, size_t len) {
     ctx->eng->drain_sweeps();
     for (auto& kv : ctx->eng->prof.drain()) {
       char line[256];
-      snprintf(line, sizeof line, "%s %ld %.6f %.6e\n", kv.first.c_str(), kv.second.n, kv.second.ms, kv.second.work);
+      snprintf(line, sizeof line, "%s %ld %.6f %.6e %.6f\n", kv.first.c_str(), kv.second.n, kv.second.ms,
+               kv.second.work, kv.second.span);
       out += line;
     }
     snprintf(buf, len, "%s", out.c_str());

@@ -87,7 +87,7 @@ def path_work(mols, results):
 def measure(mols, precision, steps, warmup, profile_steps, acc, starts=None):
     """Prepare once, `warmup` untimed and `steps` timed passes (profiling off), then
     `profile_steps` passes with kernel events; returns (wall time of the timed passes, kernel
-    profile, results)."""
+    profile, results, the ms covered by the union of each kernel's launch intervals)."""
     from axtb_amd import _lib
     opts = _lib.make_opts(accuracy=acc, etemp=300.0, precision=precision, gradient=True)
     ctx = _lib.Context(0)
@@ -102,7 +102,7 @@ def measure(mols, precision, steps, warmup, profile_steps, acc, starts=None):
         print(f"[bench] {precision} step {i + 1}/{steps} done", file=sys.stderr, flush=True)
     ctx.synchronize()
     dt = time.perf_counter() - t0
-    prof = {}
+    prof, spans = {}, {}
     if profile_steps:
         ctx.profile(True)
         ctx.profile_report()
@@ -110,11 +110,12 @@ def measure(mols, precision, steps, warmup, profile_steps, acc, starts=None):
             ctx.run()
         ctx.synchronize()
         prof = ctx.profile_report()
+        spans = ctx.last_spans
         ctx.profile(False)
         print(f"[bench] {precision} profiled pass done", file=sys.stderr, flush=True)
     res = ctx.results()
     ctx.close()
-    return dt, prof, res
+    return dt, prof, res, spans
 
 
 def max_diffs(a, b):
@@ -132,8 +133,10 @@ def max_diffs(a, b):
     return dict(zip(("dE", "dg", "dq", "deps"), d if seen else [None] * 4))
 
 
-def kernel_roofline(prof, wl):
-    """The dominant kernel (largest total time) of a profile and its roofline."""
+def kernel_roofline(prof, wl, spans=None):
+    """The dominant kernel (largest total time) of a profile and its roofline. `frac` divides the
+    work of a launch by the launches' mean duration; where launches of the kernel overlap on two
+    streams, `frac_span` divides the total work by the time their union covers instead."""
     prof = {k: v for k, v in prof.items() if k.startswith(("k_", "rocblas_"))}
     if not prof:
         return None
@@ -146,8 +149,11 @@ def kernel_roofline(prof, wl):
     else:
         pk = PEAK_FP32_MFMA if name.endswith(FP32_KERNELS) else PEAK_FP64_MFMA
         bound, peak, unit, achieved = "mfma", pk / 1e12, "TFLOP/s", per_launch / avg_s / 1e12
+    span_ms = (spans or {}).get(name)
+    frac_span = work / (span_ms / 1e3) / (peak * (1e9 if bound == "hbm" else 1e12)) if span_ms else None
     return {"bound": bound, "kernel": name, "achieved": achieved, "peak": peak, "unit": unit,
-            "frac": achieved / peak, "traffic": measured_traffic(name, wl), "avg_launch_ms": avg_s * 1e3,
+            "frac": achieved / peak, "span_ms": span_ms, "frac_span": frac_span,
+            "traffic": measured_traffic(name, wl), "avg_launch_ms": avg_s * 1e3,
             "launches": nl, "work_per_launch": per_launch, "kernel_share_of_gpu_time": ms / total if total else None,
             "kernels_ms": {k: round(v[1], 3) for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])[:10]}}
 
@@ -181,7 +187,7 @@ def run_workload(wl, args, acc):
         runs[prec] = measure(mols, prec, steps, warmup, args.profile_steps, acc, starts)
     out = {"molecules": len(mols), "steps": steps, "warmup": warmup,
            "warm_start": bool(args.warm)}
-    for prec, (dt, prof, res) in runs.items():
+    for prec, (dt, prof, res, spans) in runs.items():
         rate = len(mols) * steps / dt
         wf, wbytes = path_work(mols, res)
         peak = PEAK_FP32_MFMA if prec == "mixed" else PEAK_FP64_MFMA
@@ -192,7 +198,7 @@ def run_workload(wl, args, acc):
                      "failed": len(res) - len(ok),
                      "whole_path_roofline": {"flop": wf, "bytes": wbytes, "roofline_ms": t_roof * 1e3,
                                              "frac": t_roof / (dt / steps), "peak_matrix_TFLOPs": peak / 1e12},
-                     "dominant_kernel": kernel_roofline(prof, wl), **counters(prof)}
+                     "dominant_kernel": kernel_roofline(prof, wl, spans), **counters(prof)}
     # parity against the FP64 CPU oracle (not timed)
     check = {}
     if big:
@@ -320,7 +326,8 @@ def main():
     main_prec = next(p for p in ("mixed", "fp64", "fp64-refine") if p in main)
     dom = main[main_prec]["dominant_kernel"] or {}
     roof = {k: dom.get(k) for k in ("bound", "kernel", "achieved", "peak", "unit", "frac", "traffic",
-                                     "avg_launch_ms", "launches", "work_per_launch", "kernel_share_of_gpu_time")}
+                                     "avg_launch_ms", "launches", "work_per_launch", "kernel_share_of_gpu_time",
+                                     "span_ms", "frac_span")}
     roof["whole_path"] = {wl: {p: r[p]["whole_path_roofline"]["frac"] for p in ("fp64", "mixed", "fp64-refine")
                                if p in r}
                           for wl, r in results.items()}

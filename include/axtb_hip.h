@@ -232,8 +232,10 @@ int axtbhip_batch_gradient(axtbhip_context* ctx);
 int axtbhip_batch_adjoint(axtbhip_context* ctx, const double* dE, double* grad);
 
 /* Per-kernel timing with HIP events on the context's stream (the counterpart of the reference's
- * phase timers, crates/base/src/timing.rs:11-54): report lines "kernel launches total_ms work",
- * work = the launches' algorithmic flops (GEMM, eigensolver) or HBM bytes (streaming kernels). */
+ * phase timers, crates/base/src/timing.rs:11-54): report lines "kernel launches total_ms work
+ * span_ms", work = the launches' algorithmic flops (GEMM, eigensolver) or HBM bytes (streaming
+ * kernels), span_ms = the time covered by the union of the launches' intervals (launches on two
+ * streams overlap, so span_ms can be below total_ms). */
 void axtbhip_profile_enable(axtbhip_context* ctx, int on);
 /* hipDeviceSynchronize on the context's device. */
 int axtbhip_synchronize(axtbhip_context* ctx);
