@@ -259,7 +259,7 @@ struct Prof {
       x.n++;
       x.ms += ms;
       x.work += r.work;
-      iv[r.name].push_back({t0, t0 + ms});
+      iv[r.name].push_back({(double)t0, (double)t0 + (double)ms});  // an interval's length is its ms exactly
     }
     for (auto& kv : iv) {
       auto& v = kv.second;

@@ -129,6 +129,10 @@ def test_fp64_refine_batch(ctx):
         ctx.profile(False)
     assert not [k for k in rep if k.endswith("_f32") or "upd32" in k], sorted(rep)
     assert any(k.startswith("refine_steps") for k in rep), sorted(rep)  # the refinement ran
+    # the union of a kernel's launch intervals lies within their summed durations
+    for k, (n, ms, _work) in rep.items():
+        if k.startswith("k_") and n:
+            assert 0.0 < ctx.last_spans[k] <= ms * (1 + 1e-6) + 1e-6, (k, ctx.last_spans[k], ms)
     worst = [0.0] * 4
     for (z, x), r in zip(mols, rs):
         assert r["status"] == 0
