@@ -957,10 +957,10 @@ struct Engine {
       cur = next;
     }
     DBuf<double>& dt = dkd;  // torn diagonal (dkd is free until the merges)
-    dt.upload(hd, st);
+    dt.upload(hd, st, &stage);
     const int nleaf = (int)ls.size();
-    dls.upload(ls, st);
-    dlm.upload(lm, st);
+    dls.upload(ls, st, &stage);
+    dlm.upload(lm, st, &stage);
     dleaf.ensure((size_t)nleaf * 32 * 32);
     dleafV.ensure((size_t)nleaf * 32 * 32);
     dleafZ.ensure((size_t)nleaf * 32 * 32);
@@ -981,7 +981,7 @@ struct Engine {
     for (int lv = (int)levels.size() - 1; lv >= 0; lv--) {
       auto& nodes = levels[lv];
       const int nn = (int)nodes.size();
-      dnodes.upload(nodes, st);
+      dnodes.upload(nodes, st, &stage);
       dnk.ensure(nn);
       dnr.ensure(nn);
       drho.ensure(nn);
@@ -1005,8 +1005,8 @@ struct Engine {
         ut += (long long)hk[i] * hk[i];
         gt += (long long)nodes[i].m * hk[i];
       }
-      duoff.upload(uo, st);
-      dgoff.upload(go, st);
+      duoff.upload(uo, st, &stage);
+      dgoff.upload(go, st, &stage);
       dU.ensure(std::max<long long>(ut, 1));
       dG.ensure(std::max<long long>(gt, 1));
       dQG.ensure(std::max<long long>(gt, 1));
@@ -1137,13 +1137,13 @@ struct Engine {
       for (int k = 0; k < 6 * M.nat; k++) hgqp[6 * M.at0 + k] = x.gq && x.gqp ? x.gqp[k] : 0.0;
     }
     if (mols.empty()) return;
-    a_xyz.upload(hxyz, st);
+    a_xyz.upload(hxyz, st, &stage);
     any_guess = std::find(hgflag.begin(), hgflag.end(), 1) != hgflag.end();
     if (any_guess) {
-      g_flag.upload(hgflag, st);
-      g_q.upload(hgq, st);
-      g_dp.upload(hgdp, st);
-      g_qp.upload(hgqp, st);
+      g_flag.upload(hgflag, st, &stage);
+      g_q.upload(hgq, st, &stage);
+      g_dp.upload(hgdp, st, &stage);
+      g_qp.upload(hgqp, st, &stage);
     }
     HIP_OK(hipStreamSynchronize(st));
     scf_done = grad_done = false;
@@ -1582,28 +1582,28 @@ struct Engine {
           }
       }
     nsp_ = nsp;
-    c6ref.upload(hc6, st);
+    c6ref.upload(hc6, st, &stage);
     if (mols.empty()) return;
-    d_mol.upload(mols, st);
-    i_z.upload(hz, st); i_sp.upload(hsp, st); i_mol.upload(hmol, st); i_ish.upload(hish, st); i_nsh.upload(hnsh, st);
-    i_d4nref.upload(hnref, st); i_d4ngw.upload(hngw, st);
-    s_at.upload(hsat, st); s_l.upload(hsl, st); s_np.upload(hsnp, st); s_iao.upload(hsiao, st);
-    ao_sh.upload(haosh, st); ao_at.upload(haoat, st);
-    a_xyz.upload(hxyz, st); a_rcov.upload(hrcov, st); a_en.upload(hen, st); a_rad.upload(hrad, st);
-    a_ralpha.upload(hra, st); a_rzeff.upload(hrz, st); a_dk.upload(hdk, st); a_qk.upload(hqk, st);
-    a_arad.upload(har, st); a_avcn.upload(hav, st); a_chi.upload(hchi, st); a_eta.upload(heta, st);
-    a_kcn.upload(hkcn, st); a_erad.upload(herad, st); a_reeq.upload(hreeq, st); a_d4rcov.upload(hd4rcov, st);
-    a_d4zeff.upload(hd4zeff, st); a_d4gi.upload(hd4gi, st); a_d4r4r2.upload(hd4r4r2, st);
-    a_d4refcn.upload(hd4refcn, st); a_d4refq.upload(hd4refq, st);
-    s_alpha.upload(hsal, st); s_coeff.upload(hsco, st); s_se0.upload(hse0, st); s_kcn.upload(hskcn, st);
-    s_shpoly.upload(hsshp, st); s_refocc.upload(hsref, st); s_hard.upload(hshard, st); s_gam3.upload(hsgam3, st);
-    s_minal.upload(hsmin, st); s_zeta.upload(hszeta, st);
+    d_mol.upload(mols, st, &stage);
+    i_z.upload(hz, st, &stage); i_sp.upload(hsp, st, &stage); i_mol.upload(hmol, st, &stage); i_ish.upload(hish, st, &stage); i_nsh.upload(hnsh, st, &stage);
+    i_d4nref.upload(hnref, st, &stage); i_d4ngw.upload(hngw, st, &stage);
+    s_at.upload(hsat, st, &stage); s_l.upload(hsl, st, &stage); s_np.upload(hsnp, st, &stage); s_iao.upload(hsiao, st, &stage);
+    ao_sh.upload(haosh, st, &stage); ao_at.upload(haoat, st, &stage);
+    a_xyz.upload(hxyz, st, &stage); a_rcov.upload(hrcov, st, &stage); a_en.upload(hen, st, &stage); a_rad.upload(hrad, st, &stage);
+    a_ralpha.upload(hra, st, &stage); a_rzeff.upload(hrz, st, &stage); a_dk.upload(hdk, st, &stage); a_qk.upload(hqk, st, &stage);
+    a_arad.upload(har, st, &stage); a_avcn.upload(hav, st, &stage); a_chi.upload(hchi, st, &stage); a_eta.upload(heta, st, &stage);
+    a_kcn.upload(hkcn, st, &stage); a_erad.upload(herad, st, &stage); a_reeq.upload(hreeq, st, &stage); a_d4rcov.upload(hd4rcov, st, &stage);
+    a_d4zeff.upload(hd4zeff, st, &stage); a_d4gi.upload(hd4gi, st, &stage); a_d4r4r2.upload(hd4r4r2, st, &stage);
+    a_d4refcn.upload(hd4refcn, st, &stage); a_d4refq.upload(hd4refq, st, &stage);
+    s_alpha.upload(hsal, st, &stage); s_coeff.upload(hsco, st, &stage); s_se0.upload(hse0, st, &stage); s_kcn.upload(hskcn, st, &stage);
+    s_shpoly.upload(hsshp, st, &stage); s_refocc.upload(hsref, st, &stage); s_hard.upload(hshard, st, &stage); s_gam3.upload(hsgam3, st, &stage);
+    s_minal.upload(hsmin, st, &stage); s_zeta.upload(hszeta, st, &stage);
     any_guess = std::find(hgflag.begin(), hgflag.end(), 1) != hgflag.end();
     if (any_guess) {
-      g_flag.upload(hgflag, st);
-      g_q.upload(hgq, st);
-      g_dp.upload(hgdp, st);
-      g_qp.upload(hgqp, st);
+      g_flag.upload(hgflag, st, &stage);
+      g_q.upload(hgq, st, &stage);
+      g_dp.upload(hgdp, st, &stage);
+      g_qp.upload(hgqp, st, &stage);
     }
     // primitive screening: xtb's cutoff at the accuracy (axtb), tblite's at accuracy 1 (basis.rs:135-143)
     intcut_ = tbl ? 25.0 : std::max(25.0 - 10.0 * std::log10(o.accuracy), 20.0);
@@ -1632,7 +1632,7 @@ struct Engine {
         for (int k = 0; k < 3; k++) hg.push_back(solv0.grid[3 * p + k]);
         hg.push_back(solv0.weight[p]);
       }
-      sv_grid.upload(hg, st);
+      sv_grid.upload(hg, st, &stage);
     }
   }
   // implicit solvation (ALPB / GBSA): the model (AXTB_SOLV_*), the scalars and grid of the batch's
@@ -1717,7 +1717,7 @@ struct Engine {
         for (int I = 0; I < nt; I++)
           for (int J = 0; J <= I; J++) tl.push_back(make_int2(m, (I << 16) | J));
       }
-      mat_tiles.upload(tl, st);
+      mat_tiles.upload(tl, st, &stage);
       n_mat_tiles = (int)tl.size();
     }
     for (auto* d : {&X, &Bm, &F, &T, &Am, &Vw, &Z, &C, &P, &Wm}) d->ensure(tot_mat);
@@ -1735,12 +1735,12 @@ struct Engine {
       heq0[m] = off;
       off += (long long)(mols[m].nat + 1) * (mols[m].nat + 1);
     }
-    eq0.upload(heq0, st);
+    eq0.upload(heq0, st, &stage);
     std::vector<int> ones(NM, 1), zeros(NM, 0);
-    active.upload(ones, st);
-    status.upload(zeros, st);
-    itc.upload(zeros, st);
-    eeqfail.upload(zeros, st);
+    active.upload(ones, st, &stage);
+    status.upload(zeros, st, &stage);
+    itc.upload(zeros, st, &stage);
+    eeqfail.upload(zeros, st, &stage);
     fill(mst.p, 8LL * NM);
 
     // coordination numbers, levels, AES radii
@@ -1793,7 +1793,7 @@ struct Engine {
       sv_wsa.ensure((size_t)NA * solv0.weight.size());
       sv_adet.ensure(NM);
       sv_B.ensure(tot_pair);
-      sv_fail.upload(zeros, st);
+      sv_fail.upload(zeros, st, &stage);
       const SolvDev S = solv_dev();
       LAUNCH(k_solv_born, dim3(blocks(NA, 4)), dim3(256), 0, b, S);
       LAUNCH(k_solv_sasa, dim3(NA), dim3(64), 0, b, S, sv_fail.p);
@@ -1857,7 +1857,7 @@ struct Engine {
           jobs.push_back(j);
       }
       if (!lu.empty()) {
-        mlu.upload(lu, st);
+        mlu.upload(lu, st, &stage);
         const size_t shm = ((size_t)nlu * nlu + nlu) * sizeof(double);
         HIP_OK(hipFuncSetAttribute((const void*)k_eeq_lu, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
         LAUNCH(k_eeq_lu, dim3(lu.size()), dim3(512), shm, b, (const int*)mlu.p, eq0.p, (const double*)eeqA.p,
@@ -1865,7 +1865,7 @@ struct Engine {
       }
       if (!ev.empty()) {
         eigh(jobs);
-        mev.upload(ev, st);
+        mev.upload(ev, st, &stage);
         LAUNCH(k_eeq_solve, dim3(ev.size()), dim3(256), 0, b, (const int*)mev.p, eq0.p, eeqZ.p, eeqw.p, eeqr.p,
                eeqy.p, qstart.p, eeqfail.p);
       }
@@ -1900,7 +1900,7 @@ struct Engine {
         jobs.push_back(EigJob{n, Am.p + o0, Vw.p + o0, w.p + mols[m].ao0, Z.p + o0});
       }
       if (!cj.empty()) {
-        cjobs.upload(cj, st);
+        cjobs.upload(cj, st, &stage);
         cfail.ensure(cj.size());
         const size_t shm = ((size_t)nch * nch + nch) * sizeof(double);
         HIP_OK(hipFuncSetAttribute((const void*)k_chol_basis, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
@@ -1912,7 +1912,7 @@ struct Engine {
           if (f) throw Fail{AXTB_ERR_NUMERICAL, "overlap matrix is not positive definite"};
       }
       if (!cjp.empty()) {
-        cjobs.upload(cjp, st);
+        cjobs.upload(cjp, st, &stage);
         cfail.ensure(cjp.size());
         const size_t shm = ((size_t)nchp * (nchp + 1) / 2 + nchp) * sizeof(double);
         HIP_OK(hipFuncSetAttribute((const void*)k_chol_basis_packed, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1936,7 +1936,7 @@ struct Engine {
             if (!(sv > 0.0)) throw Fail{AXTB_ERR_NUMERICAL, "overlap matrix is not positive definite"};
             hs12[mols[m].ao0 + i] = 1.0 / std::sqrt(sv);
           }
-        s12_.upload(hs12, st);
+        s12_.upload(hs12, st, &stage);
         std::vector<GemmJob> gjobs;
         for (int m : lw) {
           long long o0 = mols[m].mat0;
