@@ -3480,6 +3480,7 @@ int axtbhip_tuning_set(axtbhip_context* ctx, const char* knob, int variant, int 
   if (!ctx || !knob || bucket < 0 || bucket >= axtb::Engine::TUNE_NB) return AXTB_ERR_INPUT;
   axtb::Engine& E = *ctx->eng;
   const std::string k(knob);
+  E.list_sig.clear();  // the fixed eigensolver plans on the device hold launch shapes: build them again
   if (k == "jacobi_threads" && variant >= 0 && variant < 6 && (value == 0 || value == 256 || value == 512 ||
                                                                  value == 1024)) {
     E.tune_jac_threads[variant / 3][variant % 3][bucket] = value;
