@@ -204,6 +204,39 @@ def test_repeat_bitwise(ctx):
         assert np.array_equal(r1["gradient"], r2["gradient"])
 
 
+def test_resident_runs_bitwise(ctx):
+    """A prepared batch run repeatedly (the job lists kept on the device after the first run), with
+    profiling off (the active mask read one iteration late, a converged batch runs one more
+    iteration whose launches all skip) and on (the mask read at the end of every iteration), in FP64
+    and mixed precision: every run gives the same bits, and the first run after a new prepare of a
+    different batch on the same context gives that batch's own bits."""
+    from axtb_amd import _lib, structures as S
+    mols = S.batch(24, seed=5)
+    other = S.batch(12, seed=6)
+    for prec in ("fp64", "mixed"):
+        opts = _lib.make_opts(accuracy=1e-2, precision=prec)
+        ref = ctx.engrad_batch(mols, opts)
+        ref_other = ctx.engrad_batch(other, opts)
+        ctx.prepare(mols, opts)
+        runs = []
+        for prof in (False, True, False):
+            ctx.profile(prof)
+            try:
+                ctx.run()
+                runs.append(ctx.results())
+            finally:
+                ctx.profile(False)
+        for rs in runs:
+            for r, q in zip(rs, ref):
+                assert r["status"] == 0 and q["status"] == 0
+                assert r["energy"] == q["energy"] and r["iterations"] == q["iterations"], prec
+                assert np.array_equal(r["gradient"], q["gradient"]), prec
+        ctx.prepare(other, opts)
+        ctx.run()
+        for r, q in zip(ctx.results(), ref_other):
+            assert r["energy"] == q["energy"] and np.array_equal(r["gradient"], q["gradient"]), prec
+
+
 def test_errors(ctx):
     from axtb_amd import _lib, structures as S
     z, x = structure("water")
