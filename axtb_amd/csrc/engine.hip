@@ -2238,7 +2238,7 @@ struct Engine {
   bool split_refine = true;
   double oa_ovmix = 0.05;  // largest first-order occupied / empty mixing a split step corrects
   double split_accept = 1e-9;  // max |e_ij| of the last split step
-  int split_steps = 8;  // steps of the split refinement before a full solve (AXTB_HIP_SPLIT_STEPS)
+  int split_steps = 6;  // steps of the split refinement before a full solve (AXTB_HIP_SPLIT_STEPS)
   int split_from = 3;   // first SCC iteration that may use it (AXTB_HIP_SPLIT_FROM)
   // the split refinement starts once the previous iteration's residual is below this: the coupling
   // inside the blocks (the change of F since the last full solve) sets its convergence rate
