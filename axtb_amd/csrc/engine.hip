@@ -2212,7 +2212,7 @@ struct Engine {
   // mixed, dense: a refinement is accepted once its step's largest correction max |e_ij| (over the
   // occupied-empty pairs and the orthogonality terms) is <= dense_accept; the error it leaves is of
   // second order in that correction (AXTB_HIP_DENSE_ACCEPT)
-  double dense_accept = 1e-4;
+  double dense_accept = 1e-3;
   // mixed, dense: largest first-order mixing of an occupied-empty pair the refinement takes on; a
   // larger one ends it (a full solve follows)
   double dense_ovmix = 0.05;
