@@ -192,16 +192,17 @@ __global__ __launch_bounds__(256) void k_trd_col(double* A, int n, int i0, int j
 }
 
 // Partials of p = A22 v (A22 = A[i+1:, i+1:], v the reflector of column i, scaled on the fly)
-// from the lower triangle only: the workgroups < nt = nb (nb + 1) / 2 take the 64 x 64 tiles
-// (rb, cb), rb >= cb, of A22 (nb = ceil(m / 64)); tile (rb, cb) writes the row sums of its rows as
+// from the lower triangle only: the first ntw workgroups take the nt = nb (nb + 1) / 2 64 x 64 tiles
+// (rb, cb), rb >= cb, of A22 (nb = ceil(m / 64)), tpw consecutive tiles each, the next tile's loads
+// issued before the current one is summed; tile (rb, cb) writes the row sums of its rows as
 // partial cb (part[cb][rows of rb]) and, below the diagonal, its column sums A^T v as partial rb of
 // the rows of block cb (part[rb][rows of cb]), so that every row r gets exactly one partial q for
 // each q < nb; vpart[t] = v^T (both sums) of tile t. Diagonal tiles read the upper half from its
-// mirror in the lower half. The workgroups after nt form W^T v and V^T v partials, one column per
+// mirror in the lower half. The workgroups after ntw form W^T v and V^T v partials, one column per
 // wavefront over 1024 rows.
 __global__ __launch_bounds__(256) void k_trd_symv(const double* A, int n, int i0, int j, const double* W,
                                                   const double* sq, const double* axv, double* part, double* dpart,
-                                                  double* vpart) {
+                                                  double* vpart, int tpw) {
   __shared__ double tile[TRD_CH][TRD_CH + 1];  // tile[c][r] = A22(R0 + r, C0 + c)
   __shared__ double vr[TRD_CH], vc[TRD_CH];
   __shared__ double red[2][4][TRD_CH];
@@ -209,64 +210,84 @@ __global__ __launch_bounds__(256) void k_trd_symv(const double* A, int n, int i0
   const long long ln = n;
   const double* x = A + i * ln + i + 1;
   const int nb = (m + TRD_CH - 1) / TRD_CH, nt = nb * (nb + 1) / 2, ndb = (m + TRD_DR - 1) / TRD_DR;
+  const int ntw = (nt + tpw - 1) / tpw;  // tile workgroups, tpw consecutive tiles each
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   double t, beta, scal;
   trd_reflector(axv[i], trd_sumsq(sq, n, i, lane), t, beta, scal);
   auto vec = [&](int k) { return k == 0 ? 1.0 : x[k] * scal; };
-  if ((int)blockIdx.x < nt) {
-    const int q = blockIdx.x;
-    int rb = (int)((sqrt(8.0 * q + 1.0) - 1.0) * 0.5);
-    while ((rb + 1) * (rb + 2) / 2 <= q) rb++;
-    while (rb * (rb + 1) / 2 > q) rb--;
-    const int cb = q - rb * (rb + 1) / 2;
-    const int R0 = rb * TRD_CH, C0 = cb * TRD_CH;
-    const double* a = A + (long long)(i + 1 + C0) * ln + (i + 1 + R0);
-    // all 16 loads of this thread in flight at once (lane = row, columns wv + 4 k)
+  if ((int)blockIdx.x < ntw) {
     constexpr int KT = TRD_CH * TRD_CH / 256;
-    double ld[KT];
-    const bool rin = R0 + lane < m;
+    auto tile_of = [](int q, int& rb, int& cb) {
+      rb = (int)((sqrt(8.0 * q + 1.0) - 1.0) * 0.5);
+      while ((rb + 1) * (rb + 2) / 2 <= q) rb++;
+      while (rb * (rb + 1) / 2 > q) rb--;
+      cb = q - rb * (rb + 1) / 2;
+    };
+    // all 16 loads of this thread for tile q in flight at once (lane = row, columns wv + 4 k)
+    auto load = [&](int q, double (&ld)[KT]) {
+      int rb, cb;
+      tile_of(q, rb, cb);
+      const int R0 = rb * TRD_CH, C0 = cb * TRD_CH;
+      const double* a = A + (long long)(i + 1 + C0) * ln + (i + 1 + R0);
+      const bool rin = R0 + lane < m;
 #pragma unroll
-    for (int k = 0; k < KT; k++) {
-      const int c = wv + 4 * k;
-      ld[k] = (rin && C0 + c < m) ? a[c * ln + lane] : 0.0;
-    }
-#pragma unroll
-    for (int k = 0; k < KT; k++) tile[wv + 4 * k][lane] = ld[k];
-    if (tid < TRD_CH) {
-      vr[tid] = R0 + tid < m ? vec(R0 + tid) : 0.0;
-    } else if (tid < 2 * TRD_CH) {
-      const int c = tid - TRD_CH;
-      vc[c] = C0 + c < m ? vec(C0 + c) : 0.0;
-    }
-    __syncthreads();
-    const bool diag = rb == cb;
-    // row sums: lane = row, wavefront = quarter of the columns
-    double s = 0.0, u = 0.0;
-    const int k0 = wv * (TRD_CH / 4);
-#pragma unroll 8
-    for (int c = k0; c < k0 + TRD_CH / 4; c++) s += (diag && c > lane ? tile[lane][c] : tile[c][lane]) * vc[c];
-    // column sums (below the diagonal): lane = column, wavefront = quarter of the rows
-    if (!diag) {
-#pragma unroll 8
-      for (int r = k0; r < k0 + TRD_CH / 4; r++) u += tile[lane][r] * vr[r];
-    }
-    red[0][wv][lane] = s;
-    red[1][wv][lane] = u;
-    __syncthreads();
-    if (wv == 0) {
-      const double rs = red[0][0][lane] + red[0][1][lane] + red[0][2][lane] + red[0][3][lane];
-      if (R0 + lane < m) part[cb * ln + (i + 1 + R0 + lane)] = rs;
-      double pv = rs * vr[lane];
-      if (!diag) {
-        const double cs = red[1][0][lane] + red[1][1][lane] + red[1][2][lane] + red[1][3][lane];
-        if (C0 + lane < m) part[rb * ln + (i + 1 + C0 + lane)] = cs;
-        pv += cs * vc[lane];
+      for (int k = 0; k < KT; k++) {
+        const int c = wv + 4 * k;
+        ld[k] = (rin && C0 + c < m) ? a[c * ln + lane] : 0.0;
       }
-      pv = wave_sum(pv);
-      if (lane == 0) vpart[q] = pv;
+    };
+    const int qa = blockIdx.x * tpw, qe = min(qa + tpw, nt);
+    double ld[KT], nx[KT];
+    load(qa, ld);
+    for (int q = qa; q < qe; q++) {
+      // the next tile's loads are in flight while this one is summed
+      if (q + 1 < qe) load(q + 1, nx);
+      if (q > qa) __syncthreads();  // the previous tile's reads of tile, vr, vc are done
+      int rb, cb;
+      tile_of(q, rb, cb);
+      const int R0 = rb * TRD_CH, C0 = cb * TRD_CH;
+#pragma unroll
+      for (int k = 0; k < KT; k++) tile[wv + 4 * k][lane] = ld[k];
+      if (tid < TRD_CH) {
+        vr[tid] = R0 + tid < m ? vec(R0 + tid) : 0.0;
+      } else if (tid < 2 * TRD_CH) {
+        const int c = tid - TRD_CH;
+        vc[c] = C0 + c < m ? vec(C0 + c) : 0.0;
+      }
+      __syncthreads();
+      const bool diag = rb == cb;
+      // row sums: lane = row, wavefront = quarter of the columns
+      double s = 0.0, u = 0.0;
+      const int k0 = wv * (TRD_CH / 4);
+#pragma unroll 8
+      for (int c = k0; c < k0 + TRD_CH / 4; c++) s += (diag && c > lane ? tile[lane][c] : tile[c][lane]) * vc[c];
+      // column sums (below the diagonal): lane = column, wavefront = quarter of the rows
+      if (!diag) {
+#pragma unroll 8
+        for (int r = k0; r < k0 + TRD_CH / 4; r++) u += tile[lane][r] * vr[r];
+      }
+      red[0][wv][lane] = s;
+      red[1][wv][lane] = u;
+      __syncthreads();
+      if (wv == 0) {
+        const double rs = red[0][0][lane] + red[0][1][lane] + red[0][2][lane] + red[0][3][lane];
+        if (R0 + lane < m) part[cb * ln + (i + 1 + R0 + lane)] = rs;
+        double pv = rs * vr[lane];
+        if (!diag) {
+          const double cs = red[1][0][lane] + red[1][1][lane] + red[1][2][lane] + red[1][3][lane];
+          if (C0 + lane < m) part[rb * ln + (i + 1 + C0 + lane)] = cs;
+          pv += cs * vc[lane];
+        }
+        pv = wave_sum(pv);
+        if (lane == 0) vpart[q] = pv;
+      }
+      if (q + 1 < qe) {
+#pragma unroll
+        for (int k = 0; k < KT; k++) ld[k] = nx[k];
+      }
     }
   } else {
-    const int d = blockIdx.x - nt;
+    const int d = blockIdx.x - ntw;
     const int rb = d % ndb;
     const int c = 4 * (d / ndb) + wv;
     if (c < 2 * j) {

@@ -400,6 +400,9 @@ struct Engine {
   // matrices of order above dense_min are diagonalised by the dense solver (Householder, divide
   // and conquer), smaller ones by the batched Jacobi solver; AXTB_HIP_DENSE_MIN overrides
   int dense_min = 200;
+  // the tridiagonalisation's symv takes two tiles per workgroup from this many tiles of the
+  // trailing matrix on (AXTB_HIP_TRD_PAIR_MIN)
+  int trd_pair_min = 2048;
 
   explicit Engine(int dev) : device(dev) {
     HIP_OK(hipSetDevice(dev));
@@ -415,6 +418,7 @@ struct Engine {
     for (auto& e : ev_act) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     // the batched Jacobi kernel handles n <= 200
     if (const char* v = std::getenv("AXTB_HIP_DENSE_MIN")) dense_min = std::min(200, std::max(1, std::atoi(v)));
+    if (const char* v = std::getenv("AXTB_HIP_TRD_PAIR_MIN")) trd_pair_min = std::max(1, std::atoi(v));
     if (const char* v = std::getenv("AXTB_HIP_OA_STEPS")) oa_steps = std::min(4, std::max(1, std::atoi(v)));
     if (const char* v = std::getenv("AXTB_HIP_DENSE_FULL")) dense_full = std::max(1, std::atoi(v));
     if (const char* v = std::getenv("AXTB_HIP_SPLIT_REFINE")) split_refine = std::atoi(v) != 0;
@@ -908,10 +912,11 @@ struct Engine {
         if (j == b) break;
         const int m = n - i - 1;
         const double bytes = 4.0 * m * (double)(m + 1);  // the lower triangle of A22
-        const int nbt = blocks(m, TRD_CH);
-        LAUNCHW(bytes, k_trd_symv, dim3(nbt * (nbt + 1) / 2 + blocks(m, TRD_DR) * blocks(2 * j, 4)), dim3(256), 0,
+        const int nbt = blocks(m, TRD_CH), nt = nbt * (nbt + 1) / 2;
+        const int tpw = nt >= trd_pair_min ? 2 : 1;  // tiles per workgroup
+        LAUNCHW(bytes, k_trd_symv, dim3(blocks(nt, tpw) + blocks(m, TRD_DR) * blocks(2 * j, 4)), dim3(256), 0,
                 (const double*)A, n, i0, j, (const double*)dtrW.p, (const double*)sq_cur, (const double*)dtrAx.p,
-                dpart.p, dtrDots.p, dtrVp.p);
+                dpart.p, dtrDots.p, dtrVp.p, tpw);
       }
       const int r0 = i0 + b, mt = n - r0;
       double* A22 = A + (long long)r0 * n + r0;
